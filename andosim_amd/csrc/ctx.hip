@@ -1,0 +1,1022 @@
+// Host driver of the gfx950 cloth step and the C ABI declared in
+// include/andosim_hip.h.  One context = one cloth (mesh + state + constraints)
+// resident in HBM, one HIP stream.  Integrator::step (integrator.cpp:70-150)
+// runs as a sequence of kernel launches; the host only reads back the scalars
+// that steer control flow (gradient max-norm, line-search mask, PCG status,
+// contact totals).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/andosim_hip.h"
+#include "kernels.hip"
+#include "model.h"
+#include "pcg.hip"
+
+namespace ando {
+
+static thread_local std::string g_last_error;
+
+struct Fail : std::runtime_error {
+  int code;
+  Fail(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      throw Fail(ANDO_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void ensure(size_t count) {
+    if (count <= n && p) return;
+    release();
+    size_t c = std::max<size_t>(count, 1);
+    HIPCHK(hipMalloc(&p, c * sizeof(T)));
+    n = c;
+  }
+  void upload(const T* src, size_t count, hipStream_t s) {
+    ensure(count);
+    if (count) HIPCHK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+};
+
+static inline int nblk(long long work, int bs = kBlock) { return (int)((work + bs - 1) / bs); }
+
+struct BodyCache {
+  std::vector<float> local;
+  std::vector<int32_t> tris;
+  DBuf<float> d_local, d_world, d_gbox, d_pose;
+  DBuf<int32_t> d_tris;
+};
+
+}  // namespace ando
+
+using namespace ando;
+
+struct ando_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int n = 0, m = 0, nb = 0;
+  ando::Material mat;
+  float mu = 0.0f;
+  bool have_mesh = false, have_state = false;
+  // Host copies of topology.
+  std::vector<int32_t> tris, row_ptr, cols, diag, block_row;
+  // Device: mesh and topology.
+  DBuf<float> mverts, dm, area, ec, fe, egrad;
+  DBuf<int32_t> d_tris, d_row_ptr, d_cols, d_diag, d_block_row, blk_cptr, blk_contrib, v2f_ptr, v2f, face_blk;
+  // Device: state.
+  DBuf<float> x, v, masses, xold, xhat, g, d;
+  // Device: system.
+  DBuf<float> Hconst, Helast3, H;
+  bool const_valid = false;
+  float const_dt = 0.0f;
+  // Strain scratch.
+  DBuf<int32_t> sflags;
+  DBuf<float> sgrad, shess;
+  // Pins and walls.
+  int num_pins = 0;
+  std::vector<int32_t> pin_vertex;
+  std::vector<float> pin_target;
+  DBuf<int32_t> pin_ptr;
+  DBuf<float> pin_t;
+  int num_walls = 0;
+  std::vector<float> wall_n, wall_off, wall_gap;
+  DBuf<float> d_wall_n, d_wall_off, kw;
+  bool kw_valid = false;
+  float kw_wall_gap = -1.0f, kw_min_gap = -1.0f;
+  // Rigid contacts.
+  std::vector<BodyCache> bodies;
+  DBuf<int32_t> rc_count, rc_ptr, scan_sums, scan_total;
+  DBuf<ando::RigidContact> rc;
+  int rc_total = 0;
+  // PCG.
+  DBuf<float> r, z, pbuf0, pbuf1, Ap, Pinv;
+  DBuf<double> part_rz, part_pAp, rz_hist;
+  DBuf<float> part_max;
+  DBuf<ando::PcgStatus> pcg_status;
+  // Small readbacks.
+  DBuf<uint32_t> scalars;  // [0] gradient max bits, [1] line-search mask, [2] strain active count
+  uint32_t* host_pinned = nullptr;
+  int pcg_chunk = 8;
+  ando_stats stats{};
+  int fp64_near = 1;
+  int kernel_timing = 0;
+  std::vector<hipEvent_t> events;  // pool: start/stop pairs for timed launches
+  ~ando_ctx() {
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+void set_error(const std::string& s) { ando::g_last_error = s; }
+
+template <class F>
+int guarded(F&& fn) {
+  try {
+    fn();
+    return ANDO_OK;
+  } catch (const Fail& f) {
+    set_error(f.what());
+    return f.code;
+  } catch (const std::invalid_argument& e) {
+    set_error(e.what());
+    return ANDO_EINVAL;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return ANDO_EDEVICE;
+  }
+}
+
+void need(bool cond, int code, const char* msg) {
+  if (!cond) throw Fail(code, msg);
+}
+
+void sync(ando_ctx* c) { HIPCHK(hipStreamSynchronize(c->stream)); }
+
+// Block pattern, gather lists and per-face block slots.
+void build_topology(ando_ctx* c, const std::vector<int32_t>& edges) {
+  const int n = c->n, m = c->m;
+  std::vector<std::vector<int32_t>> nbr(n);
+  for (int i = 0; i < n; ++i) nbr[i].push_back(i);
+  for (size_t e = 0; e + 1 < edges.size(); e += 2) {
+    nbr[edges[e]].push_back(edges[e + 1]);
+    nbr[edges[e + 1]].push_back(edges[e]);
+  }
+  c->row_ptr.assign(n + 1, 0);
+  c->cols.clear();
+  c->diag.assign(n, -1);
+  for (int i = 0; i < n; ++i) {
+    auto& l = nbr[i];
+    std::sort(l.begin(), l.end());
+    l.erase(std::unique(l.begin(), l.end()), l.end());
+    for (int j : l) {
+      if (j == i) c->diag[i] = (int32_t)c->cols.size();
+      c->cols.push_back(j);
+    }
+    c->row_ptr[i + 1] = (int32_t)c->cols.size();
+  }
+  c->nb = (int)c->cols.size();
+  c->block_row.resize(c->nb);
+  for (int i = 0; i < n; ++i)
+    for (int b = c->row_ptr[i]; b < c->row_ptr[i + 1]; ++b) c->block_row[b] = i;
+  auto blk = [&](int i, int j) {
+    auto b0 = c->cols.begin() + c->row_ptr[i], b1 = c->cols.begin() + c->row_ptr[i + 1];
+    auto it = std::lower_bound(b0, b1, j);
+    return (int32_t)(it - c->cols.begin());
+  };
+  std::vector<int32_t> face_blk(9 * (size_t)m), cnt(c->nb + 1, 0), vcnt(n + 1, 0);
+  for (int f = 0; f < m; ++f)
+    for (int a = 0; a < 3; ++a) {
+      ++vcnt[c->tris[3 * f + a] + 1];
+      for (int b = 0; b < 3; ++b) {
+        int32_t bi = blk(c->tris[3 * f + a], c->tris[3 * f + b]);
+        face_blk[9 * (size_t)f + 3 * a + b] = bi;
+        ++cnt[bi + 1];
+      }
+    }
+  for (int b = 0; b < c->nb; ++b) cnt[b + 1] += cnt[b];
+  for (int i = 0; i < n; ++i) vcnt[i + 1] += vcnt[i];
+  std::vector<int32_t> contrib(cnt[c->nb]), v2f(vcnt[n]);
+  std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1), vfill(vcnt.begin(), vcnt.end() - 1);
+  for (int f = 0; f < m; ++f)
+    for (int a = 0; a < 3; ++a) {
+      v2f[vfill[c->tris[3 * f + a]]++] = 3 * f + a;
+      for (int b = 0; b < 3; ++b) contrib[fill[face_blk[9 * (size_t)f + 3 * a + b]]++] = 9 * f + 3 * a + b;
+    }
+  hipStream_t s = c->stream;
+  c->d_row_ptr.upload(c->row_ptr, s);
+  c->d_cols.upload(c->cols, s);
+  c->d_diag.upload(c->diag, s);
+  c->d_block_row.upload(c->block_row, s);
+  c->blk_cptr.upload(cnt, s);
+  c->blk_contrib.upload(contrib, s);
+  c->v2f_ptr.upload(vcnt, s);
+  c->v2f.upload(v2f, s);
+  c->face_blk.upload(face_blk, s);
+  sync(c);
+}
+
+void alloc_state(ando_ctx* c) {
+  size_t n3 = 3 * (size_t)c->n;
+  c->x.ensure(n3); c->v.ensure(n3); c->masses.ensure(c->n); c->xold.ensure(n3); c->xhat.ensure(n3);
+  c->g.ensure(n3); c->d.ensure(n3);
+  c->Hconst.ensure(9 * (size_t)c->nb); c->Helast3.ensure(3 * (size_t)c->nb); c->H.ensure(9 * (size_t)c->nb);
+  c->sflags.ensure(c->m); c->sgrad.ensure(18 * (size_t)c->m); c->shess.ensure(20 * (size_t)c->m);
+  c->r.ensure(n3); c->z.ensure(n3); c->pbuf0.ensure(n3); c->pbuf1.ensure(n3); c->Ap.ensure(n3);
+  c->Pinv.ensure(9 * (size_t)c->n);
+  c->part_rz.ensure(nblk(c->n) + 1); c->part_max.ensure(nblk(c->n) + 1); c->part_pAp.ensure(nblk(3LL * c->n) + 1);
+  c->rc_count.ensure(c->n + 1); c->rc_ptr.ensure(c->n + 1);
+  c->scan_sums.ensure((c->n + 1023) / 1024 + 1); c->scan_total.ensure(1);
+}
+
+StepParams step_params(ando_ctx* c, const ando_params& p) {
+  StepParams P;
+  P.dt = p.dt;
+  P.gap_max = p.contact_gap_max;
+  P.normal_eps = p.contact_normal_epsilon;
+  P.barrier_tol = p.barrier_tolerance;
+  P.min_gap = p.min_gap;
+  P.num_walls = c->num_walls;
+  P.strain = p.enable_strain_limiting ? 1 : 0;
+  P.fp64_near = c->fp64_near;
+  return P;
+}
+
+void ensure_const(ando_ctx* c, const ando_params& p) {
+  hipStream_t s = c->stream;
+  if (!c->const_valid || c->const_dt != p.dt) {
+    float dtinv2 = 1.0f / (p.dt * p.dt);
+    hipLaunchKernelGGL(k_fold_const, dim3(c->n), dim3(64), 0, s, c->n, c->d_row_ptr.p, c->d_cols.p, c->blk_cptr.p,
+                       c->blk_contrib.p, c->ec.p, c->masses.p, dtinv2, c->Hconst.p, c->Helast3.p);
+    HIPCHK(hipGetLastError());
+    c->const_valid = true;
+    c->const_dt = p.dt;
+    c->kw_valid = false;
+  }
+  if (c->num_walls && (!c->kw_valid || c->kw_wall_gap != p.wall_gap || c->kw_min_gap != p.min_gap)) {
+    c->kw.ensure((size_t)c->num_walls * c->n);
+    hipLaunchKernelGGL(k_wall_stiffness, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, c->num_walls, c->d_diag.p,
+                       c->Hconst.p, c->masses.p, c->d_wall_n.p, p.wall_gap, p.min_gap, c->kw.p);
+    HIPCHK(hipGetLastError());
+    c->kw_valid = true;
+    c->kw_wall_gap = p.wall_gap;
+    c->kw_min_gap = p.min_gap;
+  }
+}
+
+void upload_bodies(ando_ctx* c, const ando_rigid_body* bodies, int nbod) {
+  need(nbod >= 0 && nbod <= kMaxBodies, ANDO_EINVAL, "too many rigid bodies (limit 16)");
+  c->bodies.resize(nbod);
+  hipStream_t s = c->stream;
+  for (int b = 0; b < nbod; ++b) {
+    const ando_rigid_body& B = bodies[b];
+    BodyCache& C = c->bodies[b];
+    need(B.num_vertices >= 0 && B.num_triangles >= 0, ANDO_EINVAL, "negative rigid body size");
+    size_t nv3 = 3 * (size_t)B.num_vertices, nt3 = 3 * (size_t)B.num_triangles;
+    for (size_t t = 0; t < nt3; ++t)
+      need(B.triangles[t] >= 0 && B.triangles[t] < B.num_vertices, ANDO_EINVAL, "rigid triangle index out of range");
+    bool same = C.local.size() == nv3 && C.tris.size() == nt3 &&
+                (nv3 == 0 || std::memcmp(C.local.data(), B.local_vertices, nv3 * sizeof(float)) == 0) &&
+                (nt3 == 0 || std::memcmp(C.tris.data(), B.triangles, nt3 * sizeof(int32_t)) == 0);
+    if (!same) {
+      C.local.assign(B.local_vertices, B.local_vertices + nv3);
+      C.tris.assign(B.triangles, B.triangles + nt3);
+      C.d_local.upload(C.local, s);
+      C.d_tris.upload(C.tris, s);
+      C.d_world.ensure(nv3);
+      C.d_gbox.ensure(6 * (size_t)((B.num_triangles + kGroup - 1) / kGroup + 1));
+    }
+    float pose[12];
+    std::memcpy(pose, B.position, 3 * sizeof(float));
+    std::memcpy(pose + 3, B.rotation, 9 * sizeof(float));
+    C.d_pose.upload(pose, 12, s);
+    if (B.num_vertices)
+      hipLaunchKernelGGL(k_rigid_world, dim3(nblk(B.num_vertices)), dim3(kBlock), 0, s, B.num_vertices,
+                         C.d_local.p, C.d_pose.p, C.d_world.p);
+    int ng = (B.num_triangles + kGroup - 1) / kGroup;
+    if (ng)
+      hipLaunchKernelGGL(k_rigid_groups, dim3(nblk(ng)), dim3(kBlock), 0, s, B.num_triangles, C.d_tris.p,
+                         C.d_world.p, C.d_gbox.p);
+    HIPCHK(hipGetLastError());
+  }
+}
+
+DetectBodies detect_bodies(ando_ctx* c, const ando_rigid_body* bodies) {
+  DetectBodies D{};
+  D.count = (int)c->bodies.size();
+  for (int b = 0; b < D.count; ++b) {
+    D.b[b].nv = bodies[b].num_vertices;
+    D.b[b].nt = bodies[b].num_triangles;
+    D.b[b].world = c->bodies[b].d_world.p;
+    D.b[b].tris = c->bodies[b].d_tris.p;
+    D.b[b].gbox = c->bodies[b].d_gbox.p;
+  }
+  return D;
+}
+
+// Rigid contacts at the current positions, ordered (vertex, body, triangle).
+void detect_rigid(ando_ctx* c, const ando_rigid_body* bodies, int nbod, float dt) {
+  c->rc_total = 0;
+  if (nbod == 0) return;
+  hipStream_t s = c->stream;
+  DetectBodies D = detect_bodies(c, bodies);
+  hipLaunchKernelGGL(k_rigid_detect, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, 0, c->x.p, D, c->rc_ptr.p,
+                     c->rc_count.p, c->rc.p, c->masses.p, c->d_diag.p, c->Helast3.p, dt);
+  int nbs = (c->n + 1023) / 1024;
+  hipLaunchKernelGGL(k_scan_local, dim3(nbs), dim3(kBlock), 0, s, c->n, c->rc_count.p, c->rc_ptr.p,
+                     c->scan_sums.p);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, s, nbs, c->scan_sums.p, c->scan_total.p);
+  hipLaunchKernelGGL(k_scan_add, dim3(nblk(c->n + 1)), dim3(kBlock), 0, s, c->n, c->rc_ptr.p, c->scan_sums.p,
+                     c->scan_total.p);
+  HIPCHK(hipGetLastError());
+  int32_t total = 0;
+  HIPCHK(hipMemcpyAsync(&total, c->scan_total.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  sync(c);
+  c->rc_total = total;
+  c->rc.ensure((size_t)total + 1);
+  if (total)
+    hipLaunchKernelGGL(k_rigid_detect, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, 1, c->x.p, D, c->rc_ptr.p,
+                       c->rc_count.p, c->rc.p, c->masses.p, c->d_diag.p, c->Helast3.p, dt);
+  HIPCHK(hipGetLastError());
+}
+
+// PCG on the matrix in c->H (or a caller's buffers) with b = -g; result in c->d.
+struct PcgOut {
+  int iters;
+  bool converged;
+};
+
+PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int32_t* cols, const int32_t* diag,
+               const float* rhs, int negate, float* xsol, float tol, int maxit) {
+  hipStream_t s = c->stream;
+  int nwg_v = nblk(n), nwg_r = nblk(3LL * n);
+  c->rz_hist.ensure((size_t)maxit + 2);
+  c->pcg_status.ensure(1);
+  hipLaunchKernelGGL(k_pcg_init, dim3(nwg_v), dim3(kBlock), 0, s, n, A, diag, rhs, xsol, c->r.p, c->z.p,
+                     c->pbuf0.p, c->Pinv.p, c->part_rz.p, c->part_max.p, negate);
+  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, s, nwg_v, c->part_rz.p, c->part_max.p, c->rz_hist.p,
+                     c->pcg_status.p, tol);
+  HIPCHK(hipGetLastError());
+  PcgStatus st{};
+  int it = 0;
+  int chunk = std::max(1, c->pcg_chunk);
+  const bool timing = c->kernel_timing != 0;
+  size_t ev_used = 0;
+  auto ev = [&](size_t k) -> hipEvent_t {
+    while (c->events.size() <= k) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      c->events.push_back(e);
+    }
+    return c->events[k];
+  };
+  for (;;) {
+    HIPCHK(hipMemcpyAsync(&st, c->pcg_status.p, sizeof(PcgStatus), hipMemcpyDeviceToHost, s));
+    sync(c);
+    if (st.done) break;
+    if (it >= maxit) {
+      hipLaunchKernelGGL(k_pcg_final, dim3(1), dim3(kBlock), 0, s, it, c->part_max.p, nwg_v, c->pcg_status.p, tol);
+      HIPCHK(hipMemcpyAsync(&st, c->pcg_status.p, sizeof(PcgStatus), hipMemcpyDeviceToHost, s));
+      sync(c);
+      break;
+    }
+    int end = std::min(maxit, it + chunk);
+    for (; it < end; ++it) {
+      float* pold = (it == 0) ? c->pbuf0.p : ((it - 1) & 1 ? c->pbuf1.p : c->pbuf0.p);
+      float* pcur = (it & 1) ? c->pbuf1.p : c->pbuf0.p;
+      if (timing) HIPCHK(hipEventRecord(ev(ev_used), s));
+      hipLaunchKernelGGL(k_pcg_spmv, dim3(nwg_r), dim3(kBlock), 0, s, n, it, A, rp, cols, c->z.p, pold, pcur,
+                         c->Ap.p, c->part_rz.p, c->part_max.p, nwg_v, c->rz_hist.p, c->part_pAp.p, c->pcg_status.p,
+                         tol);
+      if (timing) HIPCHK(hipEventRecord(ev(ev_used + 1), s));
+      hipLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(kBlock), 0, s, n, it, pcur, c->Ap.p, c->Pinv.p, xsol,
+                         c->r.p, c->z.p, c->part_pAp.p, nwg_r, c->rz_hist.p, c->part_rz.p, c->part_max.p,
+                         c->pcg_status.p);
+      if (timing) {
+        HIPCHK(hipEventRecord(ev(ev_used + 2), s));
+        ev_used += 3;
+      }
+    }
+    HIPCHK(hipGetLastError());
+    chunk = std::min(chunk * 2, 64);
+  }
+  if (timing) {
+    // Launch k of the solve ran in full iff k < iterations.
+    for (size_t k = 0; 3 * k + 2 < ev_used + 0 && (int)k < st.iters; ++k) {
+      float a = 0.0f, b = 0.0f;
+      HIPCHK(hipEventElapsedTime(&a, c->events[3 * k], c->events[3 * k + 1]));
+      HIPCHK(hipEventElapsedTime(&b, c->events[3 * k + 1], c->events[3 * k + 2]));
+      c->stats.timed_launches[0]++;
+      c->stats.timed_launches[1]++;
+      c->stats.timed_ms[0] += a;
+      c->stats.timed_ms[1] += b;
+    }
+  }
+  return PcgOut{st.iters, !st.failed};
+}
+
+float read_gmax(ando_ctx* c) {
+  uint32_t bits = 0;
+  HIPCHK(hipMemcpyAsync(&bits, c->scalars.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  float f;
+  std::memcpy(&f, &bits, sizeof(float));
+  return f;
+}
+
+// Gradient at the current x (compute_gradient), result in c->g; returns max |g|.
+void launch_gradient(ando_ctx* c, const ando_params& p, bool with_rigid) {
+  hipStream_t s = c->stream;
+  StepParams P = step_params(c, p);
+  HIPCHK(hipMemsetAsync(c->scalars.p, 0, 3 * sizeof(uint32_t), s));
+  if (P.strain) {
+    float tau = p.strain_tau > 1.0f ? p.strain_tau * 0.01f : p.strain_tau;
+    float eps = p.strain_epsilon > 0.0f ? (p.strain_epsilon > 1.0f ? p.strain_epsilon * 0.01f : p.strain_epsilon)
+                                        : tau;
+    if (eps <= 0.0f) {
+      P.strain = 0;
+    } else {
+      StrainParams sp;
+      sp.tau = tau;
+      sp.eps = eps;
+      sp.min_gap = std::max(p.min_gap, 1e-8f);
+      sp.svd_eps = std::max(p.strain_svd_epsilon, 1e-8f);
+      sp.thick = c->mat.thickness;
+      sp.rho = c->mat.density;
+      sp.fp64_near = c->fp64_near;
+      hipLaunchKernelGGL(k_strain_face, dim3(nblk(c->m)), dim3(kBlock), 0, s, c->m, sp, c->d_tris.p, c->x.p,
+                         c->dm.p, c->area.p, c->face_blk.p, c->Helast3.p, c->sflags.p, c->sgrad.p, c->shess.p,
+                         c->scalars.p + 2);
+    }
+  }
+  const int32_t* rcp = (with_rigid && c->rc_total > 0) ? c->rc_ptr.p : nullptr;
+  const int32_t* pinp = c->num_pins ? c->pin_ptr.p : nullptr;
+  hipLaunchKernelGGL(k_gradient, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, P, c->x.p, c->xhat.p, c->masses.p,
+                     c->egrad.p, c->v2f_ptr.p, c->v2f.p, c->sflags.p, c->sgrad.p, rcp, c->rc.p, pinp, c->pin_t.p,
+                     c->d_diag.p, c->Hconst.p, c->d_wall_n.p, c->d_wall_off.p, c->kw.p, c->g.p, c->scalars.p);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_assemble(ando_ctx* c, const ando_params& p, bool with_rigid) {
+  StepParams P = step_params(c, p);
+  if (P.strain) {
+    float tau = p.strain_tau > 1.0f ? p.strain_tau * 0.01f : p.strain_tau;
+    float eps = p.strain_epsilon > 0.0f ? (p.strain_epsilon > 1.0f ? p.strain_epsilon * 0.01f : p.strain_epsilon)
+                                        : tau;
+    if (eps <= 0.0f) P.strain = 0;
+  }
+  const int32_t* rcp = (with_rigid && c->rc_total > 0) ? c->rc_ptr.p : nullptr;
+  const int32_t* pinp = c->num_pins ? c->pin_ptr.p : nullptr;
+  hipLaunchKernelGGL(k_assemble, dim3(nblk(c->nb)), dim3(kBlock), 0, c->stream, c->n, P, c->d_block_row.p,
+                     c->d_cols.p, c->blk_cptr.p, c->blk_contrib.p, c->Hconst.p, c->sflags.p, c->shess.p, c->x.p,
+                     c->masses.p, rcp, c->rc.p, pinp, c->pin_t.p, c->d_wall_n.p, c->d_wall_off.p, c->kw.p, c->H.p,
+                     c->nb);
+  HIPCHK(hipGetLastError());
+}
+
+// LineSearch::search over the faithful contact set: the first active wall.
+float line_search(ando_ctx* c) {
+  hipStream_t s = c->stream;
+  if (c->num_walls == 0) return 1.0f;
+  const float* wn = c->wall_n.data();
+  f3 w = mk(wn[0], wn[1], wn[2]);
+  double sq = (double)(w.x * w.x + (w.y * w.y + w.z * w.z));
+  if (!(sq > 0.5)) return 1.0f;
+  uint32_t init = 0xFFFFFu;
+  HIPCHK(hipMemcpyAsync(c->scalars.p + 1, &init, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_ls_mask, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, w, c->wall_off[0], 1, c->x.p, c->d.p,
+                     c->scalars.p + 1);
+  HIPCHK(hipGetLastError());
+  uint32_t mask = 0;
+  HIPCHK(hipMemcpyAsync(&mask, c->scalars.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  sync(c);
+  for (int t = 0; t < 20; ++t)
+    if (mask & (1u << t)) return std::ldexp(1.0f, -t);
+  return 0.0f;
+}
+
+// Integrator::inner_newton_step (integrator.cpp:152-236).
+float inner_newton(ando_ctx* c, const ando_params& p) {
+  int maxit = p.max_newton_steps;
+  if (p.enable_friction) maxit = std::max(maxit, p.friction_min_newton_steps);
+  hipStream_t s = c->stream;
+  for (int it = 0; it < maxit; ++it) {
+    c->stats.newton_iterations++;
+    launch_gradient(c, p, true);
+    float gmax = read_gmax(c);
+    if (gmax < p.pcg_tol) return 1.0f;
+    launch_assemble(c, p, true);
+    PcgOut po = run_pcg(c, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol,
+                        p.pcg_max_iters);
+    if (c->stats.pcg_solves < ANDO_MAX_RECORDED_SOLVES) c->stats.pcg_per_solve[c->stats.pcg_solves] = po.iters;
+    c->stats.pcg_solves++;
+    c->stats.pcg_iterations += po.iters;
+    if (!po.converged) c->stats.pcg_failures++;
+    float alpha = line_search(c);
+    c->stats.last_alpha = alpha;
+    if ((double)alpha < 1e-8) {
+      c->stats.line_search_failures++;
+      return 0.0f;
+    }
+    float sc = (float)((double)alpha * 1.25);
+    hipLaunchKernelGGL(k_axpy, dim3(nblk(3LL * c->n)), dim3(kBlock), 0, s, 3 * c->n, sc, c->d.p, c->x.p);
+    HIPCHK(hipGetLastError());
+    if ((double)alpha > 0.99) return 1.0f;
+  }
+  return 0.5f;
+}
+
+// RigidBody::integrate (rigid_body.cpp:151-176) with the accumulated force and
+// torque; rotation update by the axis-angle rotation matrix.
+void rigid_integrate(ando_rigid_body& B, const double* force, const double* torque, float dt) {
+  if (B.mass <= 0.0f) return;
+  float F[3] = {(float)force[0], (float)force[1], (float)force[2]};
+  float T[3] = {(float)torque[0], (float)torque[1], (float)torque[2]};
+  auto mul33 = [](const float* A, const float* Bm, float* O) {
+    for (int r = 0; r < 3; ++r)
+      for (int cc = 0; cc < 3; ++cc)
+        O[3 * r + cc] = A[3 * r] * Bm[cc] + (A[3 * r + 1] * Bm[3 + cc] + A[3 * r + 2] * Bm[6 + cc]);
+  };
+  float acc[3];
+  for (int k = 0; k < 3; ++k) acc[k] = F[k] / B.mass;
+  for (int k = 0; k < 3; ++k) B.linear_velocity[k] = B.linear_velocity[k] + dt * acc[k];
+  float RI[9], Rt[9], Iw[9];
+  mul33(B.rotation, B.inertia_body_inv, RI);
+  for (int r = 0; r < 3; ++r)
+    for (int cc = 0; cc < 3; ++cc) Rt[3 * r + cc] = B.rotation[3 * cc + r];
+  mul33(RI, Rt, Iw);
+  for (int k = 0; k < 3; ++k) {
+    float aa = Iw[3 * k] * T[0] + (Iw[3 * k + 1] * T[1] + Iw[3 * k + 2] * T[2]);
+    B.angular_velocity[k] = B.angular_velocity[k] + dt * aa;
+  }
+  for (int k = 0; k < 3; ++k) B.position[k] = B.position[k] + dt * B.linear_velocity[k];
+  const float* w = B.angular_velocity;
+  float om = std::sqrt(w[0] * w[0] + (w[1] * w[1] + w[2] * w[2]));
+  if ((double)om > 1e-8) {
+    float th = om * dt;
+    float ax[3] = {w[0] / om, w[1] / om, w[2] / om};
+    float sn = std::sin(th), cs = std::cos(th);
+    float sa[3], ca[3];
+    for (int k = 0; k < 3; ++k) { sa[k] = sn * ax[k]; ca[k] = (1.0f - cs) * ax[k]; }
+    float Rm[9];
+    float tmp = ca[0] * ax[1];
+    Rm[1] = tmp - sa[2]; Rm[3] = tmp + sa[2];
+    tmp = ca[0] * ax[2];
+    Rm[2] = tmp + sa[1]; Rm[6] = tmp - sa[1];
+    tmp = ca[1] * ax[2];
+    Rm[5] = tmp - sa[0]; Rm[7] = tmp + sa[0];
+    for (int k = 0; k < 3; ++k) Rm[4 * k] = ca[k] * ax[k] + cs;
+    float Rn[9];
+    mul33(Rm, B.rotation, Rn);
+    std::memcpy(B.rotation, Rn, sizeof(Rn));
+  }
+}
+
+// Integrator::apply_rigid_coupling (integrator.cpp:676-754): contacts at the
+// final positions push the bodies with -dV/dg n at the witness point.
+void rigid_coupling(ando_ctx* c, const ando_params& p, ando_rigid_body* bodies, int nbod) {
+  upload_bodies(c, bodies, nbod);
+  detect_rigid(c, bodies, nbod, p.dt);
+  std::vector<RigidContact> hc(c->rc_total);
+  if (c->rc_total)
+    HIPCHK(hipMemcpyAsync(hc.data(), c->rc.p, sizeof(RigidContact) * c->rc_total, hipMemcpyDeviceToHost,
+                          c->stream));
+  sync(c);
+  std::vector<float> F(3 * nbod, 0.0f), T(3 * nbod, 0.0f);
+  for (const auto& rcn : hc) {
+    if (rcn.body < 0 || rcn.body >= nbod) continue;
+    float nv[3] = {rcn.nx, rcn.ny, rcn.nz};
+    float nl = std::sqrt(nv[0] * nv[0] + (nv[1] * nv[1] + nv[2] * nv[2]));
+    if ((double)nl < 1e-8) continue;
+    for (float& q : nv) q = q / nl;
+    float dV = 0.0f;
+    float gmax = p.contact_gap_max, g = rcn.gap;
+    if (!(gmax <= 0.0f || g >= gmax)) {
+      float dd = gmax - g;
+      float inv = 1.0f / gmax;
+      dV = -1.5f * rcn.k * inv * dd * dd;
+    }
+    if ((double)std::fabs(dV) < 1e-12) continue;
+    const ando_rigid_body& B = bodies[rcn.body];
+    if (B.mass <= 0.0f) continue;
+    float f[3] = {-(dV * nv[0]), -(dV * nv[1]), -(dV * nv[2])};
+    float rr[3] = {rcn.qx - B.position[0], rcn.qy - B.position[1], rcn.qz - B.position[2]};
+    float* Fb = &F[3 * rcn.body];
+    float* Tb = &T[3 * rcn.body];
+    for (int k = 0; k < 3; ++k) Fb[k] = Fb[k] + f[k];
+    Tb[0] = Tb[0] + (rr[1] * f[2] - rr[2] * f[1]);
+    Tb[1] = Tb[1] + (rr[2] * f[0] - rr[0] * f[2]);
+    Tb[2] = Tb[2] + (rr[0] * f[1] - rr[1] * f[0]);
+  }
+  for (int b = 0; b < nbod; ++b) {
+    double fd[3] = {F[3 * b], F[3 * b + 1], F[3 * b + 2]}, td[3] = {T[3 * b], T[3 * b + 1], T[3 * b + 2]};
+    rigid_integrate(bodies[b], fd, td, p.dt);
+  }
+}
+
+void check_params(const ando_params* p) {
+  need(p != nullptr, ANDO_EINVAL, "params is null");
+  need(std::isfinite(p->dt), ANDO_EINVAL, "dt must be finite");
+  need(p->pcg_max_iters >= 0 && p->pcg_max_iters <= 1000000, ANDO_EINVAL, "pcg_max_iters out of range");
+  need(p->max_newton_steps >= 0 && p->max_newton_steps <= 100000, ANDO_EINVAL, "max_newton_steps out of range");
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ando_abi_version(void) { return 1; }
+const char* ando_last_error(void) { return ando::g_last_error.c_str(); }
+
+int ando_create(int32_t device, ando_ctx** out) {
+  return guarded([&] {
+    need(out != nullptr, ANDO_EINVAL, "out is null");
+    int count = 0;
+    HIPCHK(hipGetDeviceCount(&count));
+    need(device >= 0 && device < count, ANDO_EINVAL, "device index out of range");
+    HIPCHK(hipSetDevice(device));
+    auto* c = new ando_ctx;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      throw Fail(ANDO_EDEVICE, "hipStreamCreate failed");
+    }
+    c->scalars.ensure(4);
+    *out = c;
+  });
+}
+
+int ando_destroy(ando_ctx* c) {
+  if (!c) return ANDO_OK;
+  return guarded([&] {
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+  });
+}
+
+int ando_set_mesh(ando_ctx* c, const float* vertices, int32_t n, const int32_t* triangles, int32_t m,
+                  const ando_material* material) {
+  return guarded([&] {
+    need(c && vertices && material && (m == 0 || triangles), ANDO_EINVAL, "null argument");
+    need(n > 0 && m >= 0, ANDO_EINVAL, "mesh needs at least one vertex");
+    HIPCHK(hipSetDevice(c->device));
+    c->n = n;
+    c->m = m;
+    c->mat.youngs_modulus = material->youngs_modulus;
+    c->mat.poisson_ratio = material->poisson_ratio;
+    c->mat.density = material->density;
+    c->mat.thickness = material->thickness;
+    c->mat.bending_stiffness = material->bending_stiffness;
+    c->mu = ando::shear_modulus(c->mat);
+    c->tris.assign(triangles, triangles + 3 * (size_t)m);
+    RestState rs = compute_rest_state(vertices, n, c->tris.data(), m);
+    build_topology(c, rs.edges);
+    alloc_state(c);
+    hipStream_t s = c->stream;
+    c->mverts.upload(vertices, 3 * (size_t)n, s);
+    c->d_tris.upload(c->tris, s);
+    c->dm.upload(rs.dm_inv, s);
+    c->area.upload(rs.area, s);
+    c->ec.ensure(9 * (size_t)std::max(m, 1));
+    c->fe.ensure(9 * (size_t)std::max(m, 1));
+    c->egrad.ensure(3 * (size_t)n);
+    if (m)
+      hipLaunchKernelGGL(k_elastic_face, dim3(nblk(m)), dim3(kBlock), 0, s, m, c->d_tris.p, c->mverts.p, c->dm.p,
+                         c->area.p, c->mat.thickness, c->mu, c->ec.p, c->fe.p);
+    hipLaunchKernelGGL(k_egrad_gather, dim3(nblk(n)), dim3(kBlock), 0, s, n, c->v2f_ptr.p, c->v2f.p, c->fe.p,
+                       c->egrad.p);
+    HIPCHK(hipGetLastError());
+    sync(c);
+    c->have_mesh = true;
+    c->have_state = false;
+    c->const_valid = false;
+    c->kw_valid = false;
+    c->num_pins = 0;
+    c->num_walls = 0;
+    c->pin_vertex.clear();
+    c->pin_target.clear();
+    c->wall_n.clear();
+    c->wall_off.clear();
+    c->wall_gap.clear();
+  });
+}
+
+int ando_set_mesh_vertices(ando_ctx* c, const float* vertices, int32_t n) {
+  return guarded([&] {
+    need(c && vertices, ANDO_EINVAL, "null argument");
+    need(c->have_mesh && n == c->n, ANDO_EINVAL, "vertex count does not match the mesh");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    c->mverts.upload(vertices, 3 * (size_t)n, s);
+    // The rest state (Dm_inv, areas) is not recomputed by the reference's
+    // vertex setter either; only the elastic gradient reads mesh.vertices.
+    if (c->m)
+      hipLaunchKernelGGL(k_elastic_face, dim3(nblk(c->m)), dim3(kBlock), 0, s, c->m, c->d_tris.p, c->mverts.p,
+                         c->dm.p, c->area.p, c->mat.thickness, c->mu, c->ec.p, c->fe.p);
+    hipLaunchKernelGGL(k_egrad_gather, dim3(nblk(n)), dim3(kBlock), 0, s, n, c->v2f_ptr.p, c->v2f.p, c->fe.p,
+                       c->egrad.p);
+    HIPCHK(hipGetLastError());
+    sync(c);
+  });
+}
+
+int ando_lumped_masses(const float* vertices, int32_t n, const int32_t* triangles, int32_t m,
+                       const ando_material* material, float* masses_out) {
+  return guarded([&] {
+    need(vertices && material && masses_out && (m == 0 || triangles), ANDO_EINVAL, "null argument");
+    need(n > 0 && m >= 0, ANDO_EINVAL, "Mesh must be initialised before creating a state");
+    ando::Material mat;
+    mat.youngs_modulus = material->youngs_modulus;
+    mat.poisson_ratio = material->poisson_ratio;
+    mat.density = material->density;
+    mat.thickness = material->thickness;
+    mat.bending_stiffness = material->bending_stiffness;
+    RestState rs = compute_rest_state(vertices, n, triangles, m);
+    std::vector<float> ms = lumped_masses(n, triangles, m, rs.area, mat);
+    std::memcpy(masses_out, ms.data(), sizeof(float) * n);
+  });
+}
+
+int ando_set_state(ando_ctx* c, const float* positions, const float* velocities, const float* masses, int32_t n) {
+  return guarded([&] {
+    need(c && c->have_mesh, ANDO_ESTATE, "mesh must be set before the state");
+    need(n == c->n, ANDO_EINVAL, "vertex count does not match the mesh");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    if (positions) c->x.upload(positions, 3 * (size_t)n, s);
+    if (velocities) c->v.upload(velocities, 3 * (size_t)n, s);
+    if (masses) {
+      c->masses.upload(masses, n, s);
+      c->const_valid = false;
+      c->kw_valid = false;
+    }
+    sync(c);
+    if (positions && velocities && masses) c->have_state = true;
+  });
+}
+
+int ando_get_state(ando_ctx* c, float* positions, float* velocities, int32_t n) {
+  return guarded([&] {
+    need(c && c->have_state, ANDO_ESTATE, "state has not been set");
+    need(n == c->n, ANDO_EINVAL, "vertex count does not match the state");
+    HIPCHK(hipSetDevice(c->device));
+    if (positions)
+      HIPCHK(hipMemcpyAsync(positions, c->x.p, 3 * sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    if (velocities)
+      HIPCHK(hipMemcpyAsync(velocities, c->v.p, 3 * sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+  });
+}
+
+int ando_set_pins(ando_ctx* c, const int32_t* vertices, const float* targets, int32_t count) {
+  return guarded([&] {
+    need(c && c->have_mesh, ANDO_ESTATE, "mesh must be set before pins");
+    need(count >= 0 && (count == 0 || (vertices && targets)), ANDO_EINVAL, "bad pin arrays");
+    for (int k = 0; k < count; ++k)
+      need(vertices[k] >= 0 && vertices[k] < c->n, ANDO_EINVAL, "pin vertex index out of range");
+    HIPCHK(hipSetDevice(c->device));
+    c->num_pins = count;
+    c->pin_vertex.assign(vertices, vertices + count);
+    c->pin_target.assign(targets, targets + 3 * (size_t)count);
+    // Per-vertex lists in pin order.
+    std::vector<int32_t> ptr(c->n + 1, 0);
+    for (int k = 0; k < count; ++k) ++ptr[vertices[k] + 1];
+    for (int i = 0; i < c->n; ++i) ptr[i + 1] += ptr[i];
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    std::vector<float> tgt(3 * (size_t)std::max(count, 1));
+    for (int k = 0; k < count; ++k) {
+      int slot = fill[vertices[k]]++;
+      for (int q = 0; q < 3; ++q) tgt[3 * slot + q] = targets[3 * k + q];
+    }
+    c->pin_ptr.upload(ptr, c->stream);
+    c->pin_t.upload(tgt, c->stream);
+    sync(c);
+  });
+}
+
+int ando_set_walls(ando_ctx* c, const float* normals, const float* offsets, const float* gaps, int32_t count) {
+  return guarded([&] {
+    need(c && c->have_mesh, ANDO_ESTATE, "mesh must be set before walls");
+    need(count >= 0 && (count == 0 || (normals && offsets && gaps)), ANDO_EINVAL, "bad wall arrays");
+    HIPCHK(hipSetDevice(c->device));
+    c->num_walls = count;
+    c->wall_n.assign(normals, normals + 3 * (size_t)count);
+    c->wall_off.assign(offsets, offsets + count);
+    c->wall_gap.assign(gaps, gaps + count);
+    if (count) {
+      c->d_wall_n.upload(c->wall_n, c->stream);
+      c->d_wall_off.upload(c->wall_off, c->stream);
+    }
+    c->kw_valid = false;
+    sync(c);
+  });
+}
+
+int ando_set_mode(ando_ctx* c, int32_t flags) {
+  return guarded([&] {
+    need(c != nullptr, ANDO_EINVAL, "null context");
+    need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING)) == 0, ANDO_EINVAL, "unknown mode flag");
+    c->fp64_near = (flags & ANDO_MODE_FP64_NEAR_CONTACT) ? 1 : 0;
+    c->kernel_timing = (flags & ANDO_MODE_KERNEL_TIMING) ? 1 : 0;
+  });
+}
+
+int ando_apply_gravity(ando_ctx* c, const float* g3, float dt) {
+  return guarded([&] {
+    need(c && c->have_state, ANDO_ESTATE, "State has not been initialised");
+    need(g3 != nullptr, ANDO_EINVAL, "gravity is null");
+    HIPCHK(hipSetDevice(c->device));
+    hipLaunchKernelGGL(k_gravity, dim3(nblk(c->n)), dim3(kBlock), 0, c->stream, c->n, g3[0], g3[1], g3[2], dt,
+                       c->x.p, c->v.p);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int ando_step(ando_ctx* c, const ando_params* params, ando_rigid_body* bodies, int32_t num_bodies) {
+  return guarded([&] {
+    need(c && c->have_state, ANDO_ESTATE, "state has not been set");
+    check_params(params);
+    need(num_bodies == 0 || bodies != nullptr, ANDO_EINVAL, "bodies is null");
+    HIPCHK(hipSetDevice(c->device));
+    auto t0 = std::chrono::steady_clock::now();
+    const ando_params& p = *params;
+    hipStream_t s = c->stream;
+    c->stats = ando_stats{};
+    ensure_const(c, p);
+    hipLaunchKernelGGL(k_predict, dim3(nblk(3LL * c->n)), dim3(kBlock), 0, s, 3 * c->n, p.dt, c->x.p, c->v.p,
+                       c->xold.p, c->xhat.p);
+    HIPCHK(hipGetLastError());
+    upload_bodies(c, bodies, num_bodies);
+    detect_rigid(c, bodies, num_bodies, p.dt);
+    c->stats.contacts = c->rc_total;
+    c->stats.rigid_contacts = c->rc_total;
+    float beta = 0.0f;
+    int bi = 0;
+    while (beta < p.beta_max && bi < 20) {
+      float a = inner_newton(c, p);
+      beta = (float)((double)beta + (1.0 - (double)beta) * (double)a);
+      ++bi;
+      if ((double)a < 1e-6) break;
+    }
+    if ((double)beta > 1e-6) inner_newton(c, p);
+    c->stats.beta = beta;
+    if ((double)beta > 1e-6) {
+      float bdt = beta * p.dt;
+      float damp = p.velocity_damping;
+      int do_damp = (double)damp > 0.0 ? 1 : 0;
+      float scale = 1.0f - std::min(std::max(damp, 0.0f), 1.0f);
+      hipLaunchKernelGGL(k_velocity, dim3(nblk(3LL * c->n)), dim3(kBlock), 0, s, 3 * c->n, bdt, scale, do_damp,
+                         c->x.p, c->xold.p, c->v.p);
+      HIPCHK(hipGetLastError());
+      need(!((double)p.contact_restitution > 0.0), ANDO_EINVAL,
+           "contact_restitution > 0 is not supported by the GPU path yet");
+    }
+    uint32_t sact = 0;
+    HIPCHK(hipMemcpyAsync(&sact, c->scalars.p + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    sync(c);
+    c->stats.strain_active = (int32_t)sact;
+    if (num_bodies > 0) rigid_coupling(c, p, bodies, num_bodies);
+    sync(c);
+    c->stats.step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  });
+}
+
+int ando_get_stats(ando_ctx* c, ando_stats* out) {
+  return guarded([&] {
+    need(c && out, ANDO_EINVAL, "null argument");
+    *out = c->stats;
+  });
+}
+
+int ando_compute_contacts(ando_ctx* c, const ando_rigid_body* bodies, int32_t num_bodies, ando_contact* out,
+                          int32_t capacity, int32_t* count) {
+  return guarded([&] {
+    need(c && c->have_state && count, ANDO_ESTATE, "state has not been set");
+    need(num_bodies == 0 || bodies != nullptr, ANDO_EINVAL, "bodies is null");
+    HIPCHK(hipSetDevice(c->device));
+    upload_bodies(c, bodies, num_bodies);
+    float dt = c->const_valid ? c->const_dt : 0.0f;
+    if (!c->const_valid) {
+      // Stiffness is not part of the contact record; Helast3 may be unset.
+      HIPCHK(hipMemsetAsync(c->Helast3.p, 0, 3 * sizeof(float) * (size_t)c->nb, c->stream));
+    }
+    detect_rigid(c, bodies, num_bodies, dt);
+    *count = c->rc_total;
+    if (!out || capacity <= 0) return;
+    std::vector<RigidContact> hc(c->rc_total);
+    std::vector<int32_t> ptr(c->n + 1);
+    if (c->rc_total) {
+      HIPCHK(hipMemcpyAsync(hc.data(), c->rc.p, sizeof(RigidContact) * c->rc_total, hipMemcpyDeviceToHost,
+                            c->stream));
+      HIPCHK(hipMemcpyAsync(ptr.data(), c->rc_ptr.p, sizeof(int32_t) * (c->n + 1), hipMemcpyDeviceToHost,
+                            c->stream));
+    }
+    std::vector<float> xh(3 * (size_t)c->n);
+    HIPCHK(hipMemcpyAsync(xh.data(), c->x.p, 3 * sizeof(float) * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    int w = 0;
+    for (int i = 0; i < c->n && w < capacity; ++i)
+      for (int e = ptr[i]; e < ptr[i + 1] && w < capacity; ++e, ++w) {
+        const RigidContact& r = hc[e];
+        ando_contact& o = out[w];
+        std::memset(&o, 0, sizeof(o));
+        o.type = 3;
+        o.idx[0] = i;
+        const int32_t* t = bodies[r.body].triangles + 3 * r.tri;
+        o.idx[1] = t[0]; o.idx[2] = t[1]; o.idx[3] = t[2];
+        o.rigid_body_index = r.body;
+        o.vertex_count = 1;
+        o.gap = r.gap;
+        o.normal[0] = r.nx; o.normal[1] = r.ny; o.normal[2] = r.nz;
+        o.witness_p[0] = xh[3 * i]; o.witness_p[1] = xh[3 * i + 1]; o.witness_p[2] = xh[3 * i + 2];
+        o.witness_q[0] = r.qx; o.witness_q[1] = r.qy; o.witness_q[2] = r.qz;
+        o.weights[0] = 1.0f;
+      }
+  });
+}
+
+int ando_get_pattern(ando_ctx* c, int32_t* row_ptr, int32_t* cols, int32_t* num_blocks) {
+  return guarded([&] {
+    need(c && c->have_mesh && num_blocks, ANDO_ESTATE, "mesh has not been set");
+    *num_blocks = c->nb;
+    if (row_ptr) std::memcpy(row_ptr, c->row_ptr.data(), sizeof(int32_t) * (c->n + 1));
+    if (cols) std::memcpy(cols, c->cols.data(), sizeof(int32_t) * c->nb);
+  });
+}
+
+int ando_linearize(ando_ctx* c, const ando_params* params, const float* x_target, const ando_rigid_body* bodies,
+                   int32_t num_bodies, float* gradient, float* values) {
+  return guarded([&] {
+    need(c && c->have_state, ANDO_ESTATE, "state has not been set");
+    check_params(params);
+    need(x_target != nullptr, ANDO_EINVAL, "x_target is null");
+    HIPCHK(hipSetDevice(c->device));
+    const ando_params& p = *params;
+    ensure_const(c, p);
+    c->xhat.upload(x_target, 3 * (size_t)c->n, c->stream);
+    upload_bodies(c, bodies, num_bodies);
+    detect_rigid(c, bodies, num_bodies, p.dt);
+    launch_gradient(c, p, true);
+    launch_assemble(c, p, true);
+    if (gradient)
+      HIPCHK(hipMemcpyAsync(gradient, c->g.p, 3 * sizeof(float) * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+    if (values)
+      HIPCHK(hipMemcpyAsync(values, c->H.p, 9 * sizeof(float) * (size_t)c->nb, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+  });
+}
+
+int ando_pcg(ando_ctx* c, int32_t num_rows, const int32_t* row_ptr, const int32_t* cols, const float* values,
+             const float* b, float* x, float tol, int32_t max_iters, int32_t* iterations, int32_t* converged) {
+  return guarded([&] {
+    need(c && row_ptr && cols && values && b && x, ANDO_EINVAL, "null argument");
+    need(num_rows > 0 && max_iters >= 0, ANDO_EINVAL, "bad sizes");
+    HIPCHK(hipSetDevice(c->device));
+    int n = num_rows;
+    int nbk = row_ptr[n];
+    need(nbk >= n, ANDO_EINVAL, "pattern needs a diagonal block per row");
+    std::vector<int32_t> dg(n, -1);
+    for (int i = 0; i < n; ++i) {
+      need(row_ptr[i + 1] >= row_ptr[i], ANDO_EINVAL, "row_ptr not monotone");
+      for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+        need(cols[k] >= 0 && cols[k] < n, ANDO_EINVAL, "column index out of range");
+        if (cols[k] == i) dg[i] = k;
+      }
+      need(dg[i] >= 0, ANDO_EINVAL, "every row needs its diagonal block");
+    }
+    DBuf<int32_t> drp, dci, ddg;
+    DBuf<float> dv, db, dx;
+    hipStream_t s = c->stream;
+    drp.upload(row_ptr, n + 1, s);
+    dci.upload(cols, nbk, s);
+    ddg.upload(dg, s);
+    dv.upload(values, 9 * (size_t)nbk, s);
+    db.upload(b, 3 * (size_t)n, s);
+    dx.ensure(3 * (size_t)n);
+    if (n > c->n || !c->r.p) {
+      size_t n3 = 3 * (size_t)n;
+      c->r.ensure(n3); c->z.ensure(n3); c->pbuf0.ensure(n3); c->pbuf1.ensure(n3); c->Ap.ensure(n3);
+      c->Pinv.ensure(9 * (size_t)n);
+      c->part_rz.ensure(nblk(n) + 1); c->part_max.ensure(nblk(n) + 1); c->part_pAp.ensure(nblk(3LL * n) + 1);
+    }
+    PcgOut po = run_pcg(c, n, dv.p, drp.p, dci.p, ddg.p, db.p, 0, dx.p, tol, max_iters);
+    HIPCHK(hipMemcpyAsync(x, dx.p, 3 * sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, s));
+    sync(c);
+    if (iterations) *iterations = po.iters;
+    if (converged) *converged = po.converged ? 1 : 0;
+  });
+}
+
+}  // extern "C"

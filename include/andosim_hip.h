@@ -1,0 +1,184 @@
+/*
+ * andosim_hip.h — C ABI of the MI355X (gfx950) per-step cloth solve.
+ *
+ * This is the boundary beneath the Python module `ando_barrier_core`: every
+ * entry point takes plain pointers and sizes, owns its device memory inside an
+ * opaque context, and returns 0 on success or a negative ANDO_E* code with a
+ * message readable through ando_last_error().  Solver non-convergence is not
+ * an error (the reference only prints it); it is reported in ando_stats.
+ *
+ * Reference interfaces replaced (paths relative to the AndoSim source tree):
+ *   ando_set_mesh          Mesh::initialize + compute_rest_state       src/core/mesh.cpp:8-75
+ *   ando_set_mesh_vertices Mesh::set_positions / Mesh.vertices setter   src/core/mesh.cpp:122-124,
+ *                                                                       src/py/bindings.cpp:166-192
+ *   ando_lumped_masses     State::compute_lumped_masses                 src/core/state.cpp:19-33
+ *   ando_set_state         State::initialize / State.set_velocities    src/core/state.cpp:7-17,
+ *                                                                       src/py/bindings.cpp:268-275
+ *   ando_get_state         State.get_positions / get_velocities        src/py/bindings.cpp:248-267
+ *   ando_set_pins          Constraints::add_pin                         src/core/constraints.cpp:5-11
+ *   ando_set_walls         Constraints::add_wall                        src/core/constraints.cpp:13-20
+ *   ando_apply_gravity     State.apply_gravity                          src/py/bindings.cpp:284-300
+ *   ando_step              Integrator::step                             src/core/integrator.cpp:70-150,
+ *                                                                       src/py/bindings.cpp:469-494
+ *   ando_compute_contacts  Integrator::compute_contacts                 src/core/integrator.cpp:238-244,
+ *                                                                       src/py/bindings.cpp:495-509
+ *   ando_get_stats         (new, additive) solver statistics; the reference exposes none
+ *                          (src/core/pcg_solver.h:26-27 returns bool only)
+ *   ando_linearize         one Newton linearisation: Integrator::compute_gradient +
+ *                          assemble_system_matrix                      src/core/integrator.cpp:246-544
+ *   ando_pcg               PCGSolver::solve on a caller-supplied BSR   src/core/pcg_solver.cpp:8-146
+ *   ando_get_pattern       block sparsity of the system matrix (the CSR pattern that
+ *                          setFromTriplets builds at integrator.cpp:539)
+ */
+#ifndef ANDOSIM_HIP_H
+#define ANDOSIM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ANDO_OK 0
+#define ANDO_EINVAL -1   /* bad argument (the reference raises ValueError) */
+#define ANDO_ESTATE -2   /* call order violated (the reference raises RuntimeError) */
+#define ANDO_EDEVICE -3  /* HIP runtime failure */
+
+typedef struct ando_ctx ando_ctx;
+
+/* Material (src/core/types.h:51-57). */
+typedef struct {
+  float youngs_modulus;
+  float poisson_ratio;
+  float density;
+  float thickness;
+  float bending_stiffness;
+} ando_material;
+
+/* SimParams (src/core/types.h:60-100), every field including the ones the
+ * Python binding does not expose. */
+typedef struct {
+  float dt;
+  float beta_max;
+  int32_t min_newton_steps;
+  int32_t max_newton_steps;
+  int32_t friction_min_newton_steps;
+  float pcg_tol;
+  int32_t pcg_max_iters;
+  float contact_gap_max;
+  float wall_gap;
+  int32_t enable_ccd;
+  float contact_normal_epsilon;
+  float barrier_tolerance;
+  int32_t enable_friction;
+  float friction_mu;
+  float friction_epsilon;
+  float friction_tangent_threshold;
+  float velocity_damping;
+  float contact_restitution;
+  int32_t enable_strain_limiting;
+  float strain_limit;
+  float strain_tau;
+  float strain_epsilon;
+  float strain_svd_epsilon;
+  float hessian_epsilon;
+  float min_gap;
+} ando_params;
+
+/* Rigid body in/out of a step (src/core/rigid_body.h:16-86).  The library
+ * reads the pose and local geometry, and writes back pose and velocities after
+ * Integrator::apply_rigid_coupling (integrator.cpp:676-754). */
+typedef struct {
+  int32_t num_vertices;
+  int32_t num_triangles;
+  const float* local_vertices; /* (num_vertices, 3), body frame */
+  const int32_t* triangles;    /* (num_triangles, 3) */
+  float mass;
+  float inertia_body_inv[9];   /* row-major */
+  float position[3];
+  float rotation[9];           /* row-major */
+  float linear_velocity[3];
+  float angular_velocity[3];
+} ando_rigid_body;
+
+/* ContactPair (src/core/collision.h:78-103). */
+typedef struct {
+  int32_t type; /* 0 point-triangle, 1 edge-edge, 2 wall, 3 rigid point-triangle */
+  int32_t idx[4];
+  int32_t rigid_body_index;
+  int32_t vertex_count;
+  float gap;
+  float normal[3];
+  float witness_p[3];
+  float witness_q[3];
+  float weights[4];
+} ando_contact;
+
+#define ANDO_MAX_RECORDED_SOLVES 64
+
+/* Solver statistics of the last ando_step (new, additive API). */
+typedef struct {
+  int32_t newton_iterations;
+  int32_t pcg_iterations;           /* summed over every PCG solve of the step */
+  int32_t pcg_solves;
+  int32_t pcg_failures;             /* cap reached or |p.Ap| < 1e-16 */
+  int32_t line_search_failures;     /* alpha = 0 */
+  int32_t contacts;                 /* contacts detected at the start of the step */
+  int32_t rigid_contacts;
+  int32_t strain_active;            /* active strain constraints at the last linearisation */
+  float beta;
+  float last_alpha;
+  int32_t pcg_per_solve[ANDO_MAX_RECORDED_SOLVES];
+  double step_ms;                   /* wall time of the step, host clock */
+  int32_t timed_launches[2];        /* ANDO_MODE_KERNEL_TIMING: launches that ran (not skipped) */
+  double timed_ms[2];               /* summed event durations of those launches */
+} ando_stats;
+
+int32_t ando_abi_version(void);
+const char* ando_last_error(void);
+
+int ando_create(int32_t device, ando_ctx** out);
+int ando_destroy(ando_ctx* ctx);
+
+int ando_set_mesh(ando_ctx* ctx, const float* vertices, int32_t num_vertices, const int32_t* triangles,
+                  int32_t num_triangles, const ando_material* material);
+int ando_set_mesh_vertices(ando_ctx* ctx, const float* vertices, int32_t num_vertices);
+/* Host helper: State::compute_lumped_masses (src/core/state.cpp:19-33) for a
+ * mesh, from its rest areas (mesh.cpp:18-75).  Needs no context and no GPU. */
+int ando_lumped_masses(const float* vertices, int32_t num_vertices, const int32_t* triangles,
+                       int32_t num_triangles, const ando_material* material, float* masses_out);
+int ando_set_state(ando_ctx* ctx, const float* positions, const float* velocities, const float* masses,
+                   int32_t num_vertices);
+int ando_get_state(ando_ctx* ctx, float* positions, float* velocities, int32_t num_vertices);
+int ando_set_pins(ando_ctx* ctx, const int32_t* vertices, const float* targets, int32_t count);
+int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, const float* gaps, int32_t count);
+
+/* Precision selection (new, additive).  ANDO_MODE_FP64_NEAR_CONTACT (default
+ * on) evaluates barrier derivatives in FP64 for gaps below 1e-2 * gap_max;
+ * with flags = 0 every barrier term rounds exactly as the reference's FP32
+ * build. */
+#define ANDO_MODE_FP64_NEAR_CONTACT 1
+/* ANDO_MODE_KERNEL_TIMING brackets every launch of the two PCG kernels with
+ * HIP events on the context's stream and accumulates their durations into
+ * ando_stats.timed_ms (index 0: SpMV kernel, 1: update kernel). */
+#define ANDO_MODE_KERNEL_TIMING 2
+int ando_set_mode(ando_ctx* ctx, int32_t flags);
+
+int ando_apply_gravity(ando_ctx* ctx, const float* gravity3, float dt);
+int ando_step(ando_ctx* ctx, const ando_params* params, ando_rigid_body* bodies, int32_t num_bodies);
+int ando_get_stats(ando_ctx* ctx, ando_stats* out);
+
+int ando_compute_contacts(ando_ctx* ctx, const ando_rigid_body* bodies, int32_t num_bodies, ando_contact* out,
+                          int32_t capacity, int32_t* count);
+
+int ando_get_pattern(ando_ctx* ctx, int32_t* row_ptr, int32_t* cols, int32_t* num_blocks);
+int ando_linearize(ando_ctx* ctx, const ando_params* params, const float* x_target, const ando_rigid_body* bodies,
+                   int32_t num_bodies, float* gradient, float* values);
+int ando_pcg(ando_ctx* ctx, int32_t num_rows, const int32_t* row_ptr, const int32_t* cols, const float* values,
+             const float* b, float* x, float tol, int32_t max_iters, int32_t* iterations, int32_t* converged);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

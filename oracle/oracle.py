@@ -1,0 +1,244 @@
+"""ctypes wrapper over the CPU oracle (oracle/liborc.so) — TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, and only as the checker or the CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborc.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+PARAM_ORDER = ["dt", "beta_max", "min_newton_steps", "max_newton_steps", "pcg_tol", "pcg_max_iters",
+               "contact_gap_max", "wall_gap", "enable_ccd", "enable_friction", "friction_mu", "friction_epsilon",
+               "velocity_damping", "contact_restitution", "enable_strain_limiting", "strain_limit", "strain_tau"]
+MATERIAL_ORDER = ["youngs_modulus", "poisson_ratio", "density", "thickness", "bending_stiffness"]
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", HERE, "liborc.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_create.restype = C.c_void_p
+        L.orc_create.argtypes = [C.c_int]
+        L.orc_destroy.argtypes = [C.c_void_p]
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_set_mesh.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip, _dp]
+        L.orc_init_state.argtypes = [C.c_void_p]
+        L.orc_set_params.argtypes = [C.c_void_p, _dp]
+        L.orc_add_pin.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_add_wall.argtypes = [C.c_void_p, _dp, C.c_double, C.c_double]
+        L.orc_add_rigid.argtypes = [C.c_void_p, C.c_int, _dp, C.c_int, _ip, C.c_double]
+        L.orc_apply_gravity.argtypes = [C.c_void_p, _dp, C.c_double]
+        L.orc_step.argtypes = [C.c_void_p]
+        L.orc_num_vertices.argtypes = [C.c_void_p]
+        L.orc_num_blocks.argtypes = [C.c_void_p]
+        L.orc_get.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_set.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_stats.argtypes = [C.c_void_p, _dp]
+        L.orc_pcg_per_solve.argtypes = [C.c_void_p, _ip, C.c_int]
+        L.orc_rigid_state.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_set_rigid_state.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_pattern.argtypes = [C.c_void_p, _ip, _ip]
+        L.orc_linearize.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.orc_pcg.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, C.POINTER(C.c_int)]
+        L.orc_contacts.argtypes = [C.c_void_p, _dp, C.c_int]
+        L.orc_barrier.restype = C.c_double
+        L.orc_barrier.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
+        L.orc_narrow_pt.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_narrow_ee.argtypes = [C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_enforce_spd.argtypes = [C.c_int, _dp]
+        L.orc_pin_stiffness.restype = C.c_double
+        L.orc_pin_stiffness.argtypes = [C.c_int, C.c_double, _dp, _dp, C.c_double]
+        L.orc_svd32.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+        L.orc_line_search.restype = C.c_double
+        L.orc_line_search.argtypes = [C.c_void_p, _dp, C.c_double]
+        L.orc_set_contacts_pt.argtypes = [C.c_void_p, C.c_int, _ip]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _chk(rc):
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+
+
+class OracleSim:
+    """One oracle simulation: mesh + state + constraints + params (+ rigid bodies)."""
+
+    def __init__(self, scene=None, precision: int = 32):
+        self.L = lib()
+        self.h = self.L.orc_create(precision)
+        self.prec = precision
+        if scene is not None:
+            self.load(scene)
+
+    def __del__(self):
+        try:
+            self.L.orc_destroy(self.h)
+        except Exception:
+            pass
+
+    def load(self, sc):
+        L = self.L
+        mat = _d([sc.material[k] for k in MATERIAL_ORDER])
+        _chk(L.orc_set_mesh(self.h, len(sc.verts), _d(sc.verts).ravel(), len(sc.tris),
+                            np.ascontiguousarray(sc.tris, dtype=np.int32).ravel(), mat))
+        _chk(L.orc_init_state(self.h))
+        self.set_params(sc.params)
+        for vi, t in sc.pins:
+            _chk(L.orc_add_pin(self.h, int(vi), _d(t)))
+        for nrm, off, gap in sc.walls:
+            _chk(L.orc_add_wall(self.h, _d(nrm), float(off), float(gap)))
+        for r in sc.rigids:
+            _chk(L.orc_add_rigid(self.h, len(r.verts), _d(r.verts).ravel(), len(r.tris),
+                                 np.ascontiguousarray(r.tris, dtype=np.int32).ravel(), float(r.density)))
+        self.n = L.orc_num_vertices(self.h)
+        self.gravity = np.asarray(sc.gravity, dtype=np.float64)
+        self.dt = float(sc.params["dt"])
+
+    def set_params(self, p: dict):
+        _chk(self.L.orc_set_params(self.h, _d([float(p[k]) for k in PARAM_ORDER])))
+
+    def apply_gravity(self, g=None, dt=None):
+        _chk(self.L.orc_apply_gravity(self.h, _d(self.gravity if g is None else g), self.dt if dt is None else dt))
+
+    def step(self):
+        _chk(self.L.orc_step(self.h))
+
+    def substep(self):
+        """One add-on substep: apply_gravity then Integrator.step (operators.py:900-905)."""
+        self.apply_gravity()
+        self.step()
+
+    def get(self, which: str) -> np.ndarray:
+        sel = {"x": 0, "v": 1, "m": 2, "mesh": 3, "egrad": 4}[which]
+        out = np.zeros(self.n if sel == 2 else 3 * self.n)
+        _chk(self.L.orc_get(self.h, sel, out))
+        return out if sel == 2 else out.reshape(-1, 3)
+
+    def set(self, which: str, arr):
+        _chk(self.L.orc_set(self.h, {"x": 0, "v": 1}[which], _d(arr).ravel()))
+
+    def stats(self) -> dict:
+        out = np.zeros(11)
+        self.L.orc_stats(self.h, out)
+        keys = ["newton_iters", "pcg_iters", "pcg_solves", "pcg_failures", "line_search_failures", "contacts",
+                "rigid_contacts", "strain_active", "beta", "asym_count", "spd_general_calls"]
+        d = dict(zip(keys, out.tolist()))
+        buf = np.zeros(4096, dtype=np.int32)
+        cnt = self.L.orc_pcg_per_solve(self.h, buf, len(buf))
+        d["pcg_per_solve"] = buf[:min(cnt, len(buf))].tolist()
+        return d
+
+    def rigid_state(self, rb: int) -> dict:
+        out = np.zeros(19)
+        _chk(self.L.orc_rigid_state(self.h, rb, out))
+        return dict(position=out[0:3], linear_velocity=out[3:6], angular_velocity=out[6:9],
+                    rotation=out[9:18].reshape(3, 3), mass=out[18])
+
+    def set_rigid_pose(self, rb: int, rs: dict):
+        st = np.concatenate([np.asarray(rs["position"], np.float64), np.asarray(rs["linear_velocity"], np.float64),
+                             np.asarray(rs["angular_velocity"], np.float64),
+                             np.asarray(rs["rotation"], np.float64).ravel()])
+        _chk(self.L.orc_set_rigid_state(self.h, rb, _d(st)))
+
+    def pattern(self):
+        nb = self.L.orc_num_blocks(self.h)
+        rp = np.zeros(self.n + 1, dtype=np.int32)
+        ci = np.zeros(nb, dtype=np.int32)
+        self.L.orc_pattern(self.h, rp, ci)
+        return rp, ci
+
+    def linearize(self, xhat):
+        nb = self.L.orc_num_blocks(self.h)
+        g = np.zeros(3 * self.n)
+        H = np.zeros(9 * nb)
+        _chk(self.L.orc_linearize(self.h, _d(xhat).ravel(), g, H))
+        return g, H.reshape(nb, 3, 3)
+
+    def contacts(self) -> np.ndarray:
+        cap = 1 << 16
+        buf = np.zeros(17 * cap)
+        cnt = self.L.orc_contacts(self.h, buf, cap)
+        if cnt > cap:
+            buf = np.zeros(17 * cnt)
+            cnt = self.L.orc_contacts(self.h, buf, cnt)
+        return buf[:17 * cnt].reshape(cnt, 17)
+
+    def line_search(self, d, ext=1.25) -> float:
+        return self.L.orc_line_search(self.h, _d(d).ravel(), ext)
+
+    def set_pt_contacts(self, idx4):
+        idx = np.ascontiguousarray(np.asarray(idx4, dtype=np.int32).reshape(-1, 4))
+        self.L.orc_set_contacts_pt(self.h, len(idx), idx.ravel())
+
+
+def pcg(rp, ci, vals, b, x0=None, tol=1e-6, maxit=1000, precision=32):
+    """Block-Jacobi PCG on a BSR matrix; returns (x, iterations, converged)."""
+    n = len(rp) - 1
+    x = np.zeros(3 * n) if x0 is None else _d(x0).ravel().copy()
+    it = C.c_int(0)
+    _chk(lib().orc_pcg(precision, n, np.ascontiguousarray(rp, dtype=np.int32),
+                       np.ascontiguousarray(ci, dtype=np.int32), _d(vals).ravel(), _d(b).ravel(), x, tol, maxit,
+                       C.byref(it)))
+    iters = it.value
+    return x, (iters if iters >= 0 else -iters - 1), iters >= 0
+
+
+def barrier(which: str, g, gmax, k, precision=32) -> float:
+    return lib().orc_barrier(precision, {"energy": 0, "gradient": 1, "hessian": 2}[which], g, gmax, k)
+
+
+def narrow_pt(p, a, b, c, precision=32):
+    out = np.zeros(7)
+    lib().orc_narrow_pt(precision, _d(p), _d(a), _d(b), _d(c), out)
+    return out[0], out[1:4], out[4:7]
+
+
+def narrow_ee(p0, p1, q0, q1, precision=32):
+    out = np.zeros(10)
+    lib().orc_narrow_ee(precision, _d(p0), _d(p1), _d(q0), _d(q1), out)
+    return out[0], out[1:4], out[4:7], out[7:10]
+
+
+def enforce_spd(H, precision=32):
+    h = _d(H).ravel().copy()
+    lib().orc_enforce_spd(precision, h)
+    return h.reshape(3, 3)
+
+
+def pin_stiffness(mass, offset, H, min_gap, precision=32):
+    return lib().orc_pin_stiffness(precision, mass, _d(offset), _d(H).ravel(), min_gap)
+
+
+def svd32(F, precision=32):
+    U = np.zeros(6)
+    s = np.zeros(2)
+    V = np.zeros(4)
+    ok = lib().orc_svd32(precision, _d(F).ravel(), U, s, V)
+    return U.reshape(3, 2), s, V.reshape(2, 2), bool(ok)
