@@ -426,6 +426,7 @@ void sync_to_device(const Mesh& mesh, State& state, const Constraints* cons) {
   if (state.host_dirty) {
     abi_check(ando_set_state(s.ctx, state.positions.data(), state.velocities.data(), state.masses.data(),
                              (int32_t)state.num_vertices()));
+    abi_check(ando_set_previous_positions(s.ctx, state.positions_prev.data(), (int32_t)state.num_vertices()));
     state.host_dirty = false;
   }
   if (cons && (s.constraints != cons || s.constraints_version != cons->version)) {

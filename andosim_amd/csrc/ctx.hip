@@ -94,7 +94,9 @@ struct ando_ctx {
   DBuf<float> mverts, dm, area, ec, fe, egrad;
   DBuf<int32_t> d_tris, d_row_ptr, d_cols, d_diag, d_block_row, blk_cptr, blk_contrib, v2f_ptr, v2f, face_blk;
   // Device: state.
-  DBuf<float> x, v, masses, xold, xhat, g, d;
+  DBuf<float> x, v, masses, xold, xhat, g, d, xprev;
+  bool have_prev = false;
+  DBuf<BodyState> body_state;
   // Device: system.
   DBuf<float> Hconst, Helast3, H;
   bool const_valid = false;
@@ -229,7 +231,8 @@ void build_topology(ando_ctx* c, const std::vector<int32_t>& edges) {
 void alloc_state(ando_ctx* c) {
   size_t n3 = 3 * (size_t)c->n;
   c->x.ensure(n3); c->v.ensure(n3); c->masses.ensure(c->n); c->xold.ensure(n3); c->xhat.ensure(n3);
-  c->g.ensure(n3); c->d.ensure(n3);
+  c->g.ensure(n3); c->d.ensure(n3); c->xprev.ensure(n3);
+  c->have_prev = false;
   c->Hconst.ensure(9 * (size_t)c->nb); c->Helast3.ensure(3 * (size_t)c->nb); c->H.ensure(9 * (size_t)c->nb);
   c->sflags.ensure(c->m); c->sgrad.ensure(18 * (size_t)c->m); c->shess.ensure(20 * (size_t)c->m);
   c->r.ensure(n3); c->z.ensure(n3); c->pbuf0.ensure(n3); c->pbuf1.ensure(n3); c->Ap.ensure(n3);
@@ -249,6 +252,10 @@ StepParams step_params(ando_ctx* c, const ando_params& p) {
   P.num_walls = c->num_walls;
   P.strain = p.enable_strain_limiting ? 1 : 0;
   P.fp64_near = c->fp64_near;
+  P.friction = (p.enable_friction && (double)p.friction_mu > 0.0) ? 1 : 0;
+  P.mu = p.friction_mu;
+  P.friction_eps = p.friction_epsilon;
+  P.friction_tangent = p.friction_tangent_threshold;
   return P;
 }
 
@@ -463,7 +470,8 @@ void launch_gradient(ando_ctx* c, const ando_params& p, bool with_rigid) {
   const int32_t* pinp = c->num_pins ? c->pin_ptr.p : nullptr;
   hipLaunchKernelGGL(k_gradient, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, P, c->x.p, c->xhat.p, c->masses.p,
                      c->egrad.p, c->v2f_ptr.p, c->v2f.p, c->sflags.p, c->sgrad.p, rcp, c->rc.p, pinp, c->pin_t.p,
-                     c->d_diag.p, c->Hconst.p, c->d_wall_n.p, c->d_wall_off.p, c->kw.p, c->g.p, c->scalars.p);
+                     c->d_diag.p, c->Hconst.p, c->d_wall_n.p, c->d_wall_off.p, c->kw.p, c->xprev.p, c->g.p,
+                     c->scalars.p);
   HIPCHK(hipGetLastError());
 }
 
@@ -479,7 +487,7 @@ void launch_assemble(ando_ctx* c, const ando_params& p, bool with_rigid) {
   const int32_t* pinp = c->num_pins ? c->pin_ptr.p : nullptr;
   hipLaunchKernelGGL(k_assemble, dim3(nblk(c->nb)), dim3(kBlock), 0, c->stream, c->n, P, c->d_block_row.p,
                      c->d_cols.p, c->blk_cptr.p, c->blk_contrib.p, c->Hconst.p, c->sflags.p, c->shess.p, c->x.p,
-                     c->masses.p, rcp, c->rc.p, pinp, c->pin_t.p, c->d_wall_n.p, c->d_wall_off.p, c->kw.p, c->H.p,
+                     c->masses.p, rcp, c->rc.p, pinp, c->pin_t.p, c->d_wall_n.p, c->d_wall_off.p, c->kw.p, c->xprev.p, c->H.p,
                      c->nb);
   HIPCHK(hipGetLastError());
 }
@@ -624,6 +632,45 @@ void rigid_coupling(ando_ctx* c, const ando_params& p, ando_rigid_body* bodies, 
   }
 }
 
+// Integrator::apply_contact_restitution (integrator.cpp:565-674).
+void restitution(ando_ctx* c, const ando_params& p, ando_rigid_body* bodies, int nbod) {
+  float e = std::min(std::max(p.contact_restitution, 0.0f), 1.0f);
+  if (e <= 0.0f) return;
+  hipStream_t s = c->stream;
+  if (nbod > 0) {
+    upload_bodies(c, bodies, nbod);
+    detect_rigid(c, bodies, nbod, p.dt);
+    std::vector<BodyState> bs(nbod);
+    for (int b = 0; b < nbod; ++b) {
+      const ando_rigid_body& B = bodies[b];
+      std::memcpy(bs[b].pos, B.position, sizeof(bs[b].pos));
+      std::memcpy(bs[b].rot, B.rotation, sizeof(bs[b].rot));
+      std::memcpy(bs[b].lin, B.linear_velocity, sizeof(bs[b].lin));
+      std::memcpy(bs[b].ang, B.angular_velocity, sizeof(bs[b].ang));
+      std::memcpy(bs[b].Ibinv, B.inertia_body_inv, sizeof(bs[b].Ibinv));
+      bs[b].mass = B.mass;
+    }
+    c->body_state.upload(bs, s);
+    if (c->rc_total)
+      hipLaunchKernelGGL(k_restitution_rigid, dim3(1), dim3(64), 0, s, c->rc_total, c->rc.p, c->v.p, c->masses.p,
+                         c->body_state.p, nbod, e, std::max(p.contact_gap_max, 1e-5f));
+    HIPCHK(hipMemcpyAsync(bs.data(), c->body_state.p, sizeof(BodyState) * nbod, hipMemcpyDeviceToHost, s));
+    sync(c);
+    for (int b = 0; b < nbod; ++b) {
+      std::memcpy(bodies[b].linear_velocity, bs[b].lin, sizeof(bs[b].lin));
+      std::memcpy(bodies[b].angular_velocity, bs[b].ang, sizeof(bs[b].ang));
+    }
+  }
+  if (c->num_walls) {
+    DBuf<float> gaps;
+    gaps.upload(c->wall_gap, s);
+    hipLaunchKernelGGL(k_restitution_walls, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, c->num_walls,
+                       c->d_wall_n.p, c->d_wall_off.p, gaps.p, p.contact_gap_max, e, c->x.p, c->v.p);
+    sync(c);
+  }
+  HIPCHK(hipGetLastError());
+}
+
 void check_params(const ando_params* p) {
   need(p != nullptr, ANDO_EINVAL, "params is null");
   need(std::isfinite(p->dt), ANDO_EINVAL, "dt must be finite");
@@ -762,8 +809,23 @@ int ando_set_state(ando_ctx* c, const float* positions, const float* velocities,
       c->const_valid = false;
       c->kw_valid = false;
     }
+    if (positions && velocities && masses && !c->have_prev) {
+      HIPCHK(hipMemcpyAsync(c->xprev.p, c->x.p, 3 * sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, s));
+      c->have_prev = true;
+    }
     sync(c);
     if (positions && velocities && masses) c->have_state = true;
+  });
+}
+
+int ando_set_previous_positions(ando_ctx* c, const float* prev, int32_t n) {
+  return guarded([&] {
+    need(c && c->have_mesh && prev, ANDO_ESTATE, "mesh must be set before the state");
+    need(n == c->n, ANDO_EINVAL, "vertex count does not match the mesh");
+    HIPCHK(hipSetDevice(c->device));
+    c->xprev.upload(prev, 3 * (size_t)n, c->stream);
+    c->have_prev = true;
+    sync(c);
   });
 }
 
@@ -880,8 +942,7 @@ int ando_step(ando_ctx* c, const ando_params* params, ando_rigid_body* bodies, i
       hipLaunchKernelGGL(k_velocity, dim3(nblk(3LL * c->n)), dim3(kBlock), 0, s, 3 * c->n, bdt, scale, do_damp,
                          c->x.p, c->xold.p, c->v.p);
       HIPCHK(hipGetLastError());
-      need(!((double)p.contact_restitution > 0.0), ANDO_EINVAL,
-           "contact_restitution > 0 is not supported by the GPU path yet");
+      if ((double)p.contact_restitution > 0.0) restitution(c, p, bodies, num_bodies);
     }
     uint32_t sact = 0;
     HIPCHK(hipMemcpyAsync(&sact, c->scalars.p + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));

@@ -18,6 +18,17 @@ struct RigidContact {
   float k;
   int32_t body;
   int32_t tri;
+  int32_t vertex;
+};
+
+// Rigid body dynamic state on the device (rigid_body.h:72-85).
+struct BodyState {
+  float pos[3];
+  float rot[9];
+  float lin[3];
+  float ang[3];
+  float Ibinv[9];
+  float mass;
 };
 
 struct DetectBody {
@@ -40,7 +51,28 @@ struct StepParams {
   int32_t num_walls;
   int32_t strain;
   int32_t fp64_near;
+  int32_t friction;        // enable_friction && friction_mu > 0
+  float mu;
+  float friction_eps;
+  float friction_tangent;
 };
+
+// Friction of one contact on its first vertex (integrator.cpp:28-66,
+// friction.cpp:6-90): tangential slip since the state's initial positions,
+// stiffness mu |k gap| / max(eps, |slip|) capped at 1e8.
+__device__ inline bool friction_data(const RigidContact& c, f3 disp, const StepParams& P, float& kf, f3& tan) {
+  f3 n = mk(c.nx, c.ny, c.nz);
+  float nc = dot(disp, n);
+  tan = disp - nc * n;
+  float tn = nrm(tan);
+  if (!(tn > P.friction_tangent)) return false;
+  float nf = c.k * fabsf(c.gap);
+  float den = fmaxf(P.friction_eps, fabsf(tn));
+  if (den <= 0.0f) den = P.friction_eps;
+  float s = P.mu * fabsf(nf) / den;
+  kf = fminf(s, 1e8f);
+  return true;
+}
 
 struct StrainParams {
   float tau, eps, min_gap, svd_eps;

@@ -259,6 +259,7 @@ __global__ void k_rigid_detect(int n, int pass, const float* __restrict__ x, Det
             o.k = rigid_contact_k(mass, E, r.normal, dt);
             o.body = b;
             o.tri = t;
+            o.vertex = i;
           }
           ++c;
         }
@@ -464,7 +465,8 @@ __global__ void k_gradient(int n, StepParams P, const float* __restrict__ x, con
                            const int32_t* __restrict__ pin_ptr, const float* __restrict__ pin_t,
                            const int32_t* __restrict__ diag, const float* __restrict__ Hconst,
                            const float* __restrict__ wall_n, const float* __restrict__ wall_off,
-                           const float* __restrict__ kw, float* __restrict__ g, uint32_t* __restrict__ gmax_bits) {
+                           const float* __restrict__ kw, const float* __restrict__ xprev, float* __restrict__ g,
+                           uint32_t* __restrict__ gmax_bits) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   float local = 0.0f;
   if (i < n) {
@@ -512,6 +514,15 @@ __global__ void k_gradient(int n, StepParams P, const float* __restrict__ x, con
       float dV = barrier_grad(gap, P.gap_max, kb, P.fp64_near);
       gi = gi + dV * nn;
     }
+    if (P.friction && rc_ptr) {
+      f3 disp = xi - ld3(xprev, i);
+      for (int e = rc_ptr[i]; e < rc_ptr[i + 1]; ++e) {
+        float kf;
+        f3 tan;
+        if (!friction_data(rc[e], disp, P, kf, tan)) continue;
+        gi = gi + kf * tan;
+      }
+    }
     st3(g, i, gi);
     local = fmaxf(fabsf(gi.x), fmaxf(fabsf(gi.y), fabsf(gi.z)));
     if (local != local) local = __int_as_float(0x7f800000);  // NaN never passes the test
@@ -536,7 +547,7 @@ __global__ void k_assemble(int n, StepParams P, const int32_t* __restrict__ bloc
                            const RigidContact* __restrict__ rc, const int32_t* __restrict__ pin_ptr,
                            const float* __restrict__ pin_t, const float* __restrict__ wall_n,
                            const float* __restrict__ wall_off, const float* __restrict__ kw,
-                           float* __restrict__ H, int nb) {
+                           const float* __restrict__ xprev, float* __restrict__ H, int nb) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   float h[9];
@@ -617,10 +628,118 @@ __global__ void k_assemble(int n, StepParams P, const int32_t* __restrict__ bloc
           }
       }
     }
+    if (P.friction && rc_ptr) {
+      // k_f (I - n n^T) + 1e-8 I, entries above 1e-12 (integrator.cpp:515-536, friction.cpp:54-72).
+      f3 disp = ld3(x, i) - ld3(xprev, i);
+      for (int e = rc_ptr[i]; e < rc_ptr[i + 1]; ++e) {
+        float kf;
+        f3 tan;
+        if (!friction_data(rc[e], disp, P, kf, tan)) continue;
+        f3 nn = mk(rc[e].nx, rc[e].ny, rc[e].nz);
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) {
+            float val = r == c ? kf : 0.0f;
+            val = val - kf * (comp(nn, r) * comp(nn, c));
+            val = val + (r == c ? 1e-8f : 0.0f);
+            if (!((double)fabsf(val) > 1e-12)) continue;
+            h[3 * r + c] = h[3 * r + c] + val;
+          }
+      }
+    }
   }
   float* o = H + 9 * (size_t)b;
 #pragma unroll
   for (int e = 0; e < 9; ++e) o[e] = h[e];
+}
+
+// ---------------------------------------------------------------------------
+// Contact restitution (Integrator::apply_contact_restitution,
+// integrator.cpp:565-674).  Rigid contacts first, one thread in contact order,
+// because every impulse changes the body velocity the next contact sees; then
+// every wall, per vertex.
+// ---------------------------------------------------------------------------
+__device__ inline void mul33(const float* A, const float* B, float* O) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) O[3 * r + c] = A[3 * r] * B[c] + (A[3 * r + 1] * B[3 + c] + A[3 * r + 2] * B[6 + c]);
+}
+__device__ inline f3 mulv33(const float* A, f3 v) {
+  return mk(A[0] * v.x + (A[1] * v.y + A[2] * v.z), A[3] * v.x + (A[4] * v.y + A[5] * v.z),
+            A[6] * v.x + (A[7] * v.y + A[8] * v.z));
+}
+__device__ inline void world_inv_inertia(const BodyState& b, float* O) {
+  float RI[9], Rt[9];
+  mul33(b.rot, b.Ibinv, RI);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) Rt[3 * r + c] = b.rot[3 * c + r];
+  mul33(RI, Rt, O);
+}
+
+__global__ void k_restitution_rigid(int total, const RigidContact* __restrict__ rc, float* __restrict__ v,
+                                    const float* __restrict__ masses, BodyState* __restrict__ bodies, int nbod,
+                                    float e, float gap_limit) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int k = 0; k < total; ++k) {
+    const RigidContact& c = rc[k];
+    if (c.gap > gap_limit) continue;
+    if (c.body < 0 || c.body >= nbod) continue;
+    f3 nn = mk(c.nx, c.ny, c.nz);
+    float nl = nrm(nn);
+    if ((double)nl < 1e-8) continue;
+    nn = nn / nl;
+    BodyState& B = bodies[c.body];
+    f3 q = mk(c.qx, c.qy, c.qz);
+    f3 pos = mk(B.pos[0], B.pos[1], B.pos[2]);
+    f3 vv = ld3(v, c.vertex);
+    f3 bv = mk(B.lin[0], B.lin[1], B.lin[2]) + cross(mk(B.ang[0], B.ang[1], B.ang[2]), q - pos);
+    float vn = dot(vv - bv, nn);
+    if (vn >= 0.0f) continue;
+    float m = masses[c.vertex];
+    float im = m > 0.0f ? 1.0f / m : 0.0f;
+    f3 r = q - pos;
+    f3 cr = cross(r, nn);
+    float Iw[9];
+    world_inv_inertia(B, Iw);
+    float angular = dot(cr, mulv33(Iw, cr));
+    float den = im + angular;
+    if (den <= 0.0f) continue;
+    float j = -(1.0f + e) * vn / den;
+    f3 J = j * nn;
+    vv = vv + mk(J.x * im, J.y * im, J.z * im);
+    st3(v, c.vertex, vv);
+    if (B.mass <= 0.0f) continue;
+    f3 mJ = -J;
+    B.lin[0] = B.lin[0] + mJ.x / B.mass;
+    B.lin[1] = B.lin[1] + mJ.y / B.mass;
+    B.lin[2] = B.lin[2] + mJ.z / B.mass;
+    f3 dw = mulv33(Iw, cross(r, mJ));
+    B.ang[0] = B.ang[0] + dw.x;
+    B.ang[1] = B.ang[1] + dw.y;
+    B.ang[2] = B.ang[2] + dw.z;
+  }
+}
+
+__global__ void k_restitution_walls(int n, int nw, const float* __restrict__ wall_n, const float* __restrict__ wall_off,
+                                    const float* __restrict__ wall_gap, float gap_max, float e,
+                                    const float* __restrict__ x, float* __restrict__ v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  f3 xi = ld3(x, i);
+  f3 vi = ld3(v, i);
+  for (int w = 0; w < nw; ++w) {
+    f3 nn = ld3(wall_n, w);
+    float nl = nrm(nn);
+    if ((double)nl < 1e-8) continue;
+    nn = nn / nl;
+    float sd = dot(nn, xi) - wall_off[w];
+    if (!(sd <= wall_gap[w] + gap_max)) continue;
+    f3 n2 = nn;
+    float l2 = nrm(n2);
+    if ((double)l2 < 1e-8) continue;
+    n2 = n2 / l2;
+    float vn = dot(vi, n2);
+    if (vn < 0.0f) vi = vi - ((1.0f + e) * vn) * n2;
+  }
+  st3(v, i, vi);
 }
 
 // ---------------------------------------------------------------------------

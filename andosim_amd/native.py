@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(HERE, "libandosim_hip.so")
 
 ABI_SYMBOLS = [
     "ando_abi_version", "ando_last_error", "ando_create", "ando_destroy", "ando_set_mesh", "ando_set_mesh_vertices",
-    "ando_lumped_masses",
+    "ando_lumped_masses", "ando_set_previous_positions",
     "ando_set_state", "ando_get_state", "ando_set_pins", "ando_set_walls", "ando_set_mode", "ando_apply_gravity",
     "ando_step",
     "ando_get_stats", "ando_compute_contacts", "ando_get_pattern", "ando_linearize", "ando_pcg",
@@ -87,6 +87,7 @@ def load():
     L.ando_lumped_masses.argtypes = [fp, C.c_int32, ip, C.c_int32, C.POINTER(Material), fp]
     L.ando_set_state.argtypes = [C.c_void_p, fp, fp, fp, C.c_int32]
     L.ando_get_state.argtypes = [C.c_void_p, fp, fp, C.c_int32]
+    L.ando_set_previous_positions.argtypes = [C.c_void_p, fp, C.c_int32]
     L.ando_set_pins.argtypes = [C.c_void_p, ip, fp, C.c_int32]
     L.ando_set_walls.argtypes = [C.c_void_p, fp, fp, fp, C.c_int32]
     L.ando_apply_gravity.argtypes = [C.c_void_p, fp, C.c_float]

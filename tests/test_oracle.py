@@ -1,0 +1,312 @@
+"""Pins the CPU oracle (no GPU).
+
+Each check is chosen so that a plausible restatement mistake (a dropped term,
+a wrong sign, a swapped index) fails it: the reference's own known answers
+(tests/golden/known_answers.json, written by make_known_answers.py),
+closed forms, invariants, and brute force.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from andosim_amd import scenes
+from oracle import oracle as orc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KA = json.load(open(os.path.join(HERE, "golden", "known_answers.json")))
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_barrier_known_answers_and_fd(prec):
+    b = KA["barrier"]
+    g, gmax, k = b["g"], b["gmax"], b["k"]
+    rel = 1e-6 if prec == 32 else 1e-12
+    assert orc.barrier("energy", g, gmax, k, prec) == pytest.approx(b["V"], rel=rel)
+    assert orc.barrier("gradient", g, gmax, k, prec) == pytest.approx(b["dV"], rel=rel)
+    assert orc.barrier("hessian", g, gmax, k, prec) == pytest.approx(b["d2V"], rel=rel)
+    for z in b["domain_zero_at"]:
+        assert orc.barrier("energy", z, gmax, k, prec) == 0.0
+        assert orc.barrier("gradient", z, gmax, k, prec) == 0.0
+    assert orc.barrier("energy", b["positive_at"], gmax, k, prec) > 0.0
+    h = 1e-6
+    fd = (orc.barrier("energy", g + h, gmax, k, 64) - orc.barrier("energy", g - h, gmax, k, 64)) / (2 * h)
+    assert abs(fd - b["dV"]) / abs(b["dV"]) < b["fd_rel_tol"]
+    fd2 = (orc.barrier("gradient", g + h, gmax, k, 64) - orc.barrier("gradient", g - h, gmax, k, 64)) / (2 * h)
+    assert abs(fd2 - b["d2V"]) / abs(b["d2V"]) < b["fd_rel_tol"]
+
+
+def test_pin_stiffness_known_answer():
+    p = KA["pin_stiffness"]
+    H = np.eye(3) * p["H_diag"]
+    kk = orc.pin_stiffness(p["mass"], p["offset"], H, p["min_gap"])
+    assert abs(kk - p["expected"]) < p["abs_tol"]
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_point_triangle_known_answer(prec):
+    p = KA["point_triangle"]
+    d, n, q = orc.narrow_pt(p["p"], p["a"], p["b"], p["c"], prec)
+    assert d == pytest.approx(p["distance"], rel=1e-6)
+    assert n[2] > p["normal_z_min"]
+    assert np.allclose(q, [0.25, 0.25, 0.0], atol=1e-7)
+
+
+def test_point_triangle_regions_brute_force():
+    """Closest point agrees with a dense barycentric brute force in every Voronoi region."""
+    rng = np.random.default_rng(5)
+    a, b, c = np.array([0.0, 0, 0]), np.array([1.0, 0.2, 0]), np.array([0.3, 0.9, 0.1])
+    us = np.linspace(0, 1, 401)
+    U, V = np.meshgrid(us, us)
+    mask = U + V <= 1
+    P = a + U[mask][:, None] * (b - a) + V[mask][:, None] * (c - a)
+    for _ in range(60):
+        p = rng.uniform(-1, 2, 3)
+        d, n, q = orc.narrow_pt(p, a, b, c, 64)
+        dbrute = np.min(np.linalg.norm(P - p, axis=1))
+        assert d <= dbrute + 1e-12 and d >= dbrute - 5e-3
+        assert abs(np.linalg.norm(p - q) - d) < 1e-12
+
+
+def test_edge_edge_brute_force():
+    rng = np.random.default_rng(6)
+    s = np.linspace(0, 1, 801)
+    for _ in range(40):
+        p0, p1, q0, q1 = rng.uniform(-1, 1, (4, 3))
+        d, n, wp, wq = orc.narrow_ee(p0, p1, q0, q1, 64)
+        A = p0 + s[:, None] * (p1 - p0)
+        B = q0 + s[:, None] * (q1 - q0)
+        D = np.linalg.norm(A[:, None, :] - B[None, :, :], axis=2)
+        assert abs(d - D.min()) < 5e-3 and d <= D.min() + 1e-12
+
+
+def test_enforce_spd_diagonal_exact_and_general():
+    # Diagonal input: entries kept, negatives and zeros clamped to 1e-8.
+    h = orc.enforce_spd(np.diag([3.0, -2.0, 0.0]), 64)
+    assert np.array_equal(h, np.diag([3.0, 1e-8, 1e-8]))
+    # General symmetric input: SPD result with clamped spectrum, same eigenvectors.
+    rng = np.random.default_rng(1)
+    Q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    lam = np.array([2.0, -1.0, 0.5])
+    A = Q @ np.diag(lam) @ Q.T
+    out = orc.enforce_spd(A, 64)
+    w = np.linalg.eigvalsh(out)
+    assert np.allclose(np.sort(w), np.sort([2.0, 1e-8, 0.5]), atol=1e-9)
+
+
+def test_svd32_identities_and_derivative():
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        F = rng.normal(size=(3, 2))
+        U, s, V, ok = orc.svd32(F, 64)
+        assert ok and s[0] >= s[1] >= 0
+        assert np.allclose(U @ np.diag(s) @ V.T, F, atol=1e-12)
+        assert np.allclose(U.T @ U, np.eye(2), atol=1e-12)
+        assert np.allclose(s, np.linalg.svd(F, compute_uv=False), atol=1e-12)
+        # d sigma_i / dF = u_i v_i^T (finite differences).
+        E = rng.normal(size=(3, 2))
+        eps = 1e-7
+        _, sp, _, _ = orc.svd32(F + eps * E, 64)
+        _, sm, _, _ = orc.svd32(F - eps * E, 64)
+        for i in range(2):
+            fd = (sp[i] - sm[i]) / (2 * eps)
+            an = np.sum(np.outer(U[:, i], V[:, i]) * E)
+            assert abs(fd - an) < 1e-5
+    _, s, _, _ = orc.svd32(np.zeros((3, 2)), 64)
+    assert np.allclose(s, [1e-6, 1e-6])  # singular-value floor (strain_limiting.cpp:43-47)
+
+
+def test_pcg_known_answer_and_dense():
+    p = KA["pcg9"]
+    n = p["n"]
+    A = np.eye(n) * p["diag"]
+    for r, c, v in p["off"]:
+        A[r, c] = v
+    rp = np.array([0, 1, 2, 3], np.int32)
+    ci = np.array([0, 1, 2], np.int32)
+    vals = np.stack([A[3 * i:3 * i + 3, 3 * i:3 * i + 3] for i in range(3)])
+    b = np.full(n, p["b"])
+    for prec in (32, 64):
+        x, it, ok = orc.pcg(rp, ci, vals, b, tol=p["tol"], maxit=p["max_iters"], precision=prec)
+        assert ok
+        assert np.linalg.norm(b - A @ x) < p["residual_l2_max"]
+        # Block-diagonal matrix: block-Jacobi is the exact inverse, one iteration.
+        assert it == 1
+    # Coupled SPD system against a dense solve.
+    rng = np.random.default_rng(3)
+    nv = 6
+    M = rng.normal(size=(3 * nv, 3 * nv))
+    A = M @ M.T + 3 * np.eye(3 * nv)
+    rp = np.arange(0, nv * nv + 1, nv).astype(np.int32)
+    ci = np.tile(np.arange(nv), nv).astype(np.int32)
+    vals = np.stack([A[3 * i:3 * i + 3, 3 * j:3 * j + 3] for i in range(nv) for j in range(nv)])
+    b = rng.normal(size=3 * nv)
+    x, it, ok = orc.pcg(rp, ci, vals, b, tol=1e-12, maxit=200, precision=64)
+    assert ok and np.allclose(x, np.linalg.solve(A, b), atol=1e-9)
+
+
+def single_face_scene(h=0.05):
+    V = np.array([[0, 0, 0.5], [h, 0, 0.5], [h, h, 0.5]], np.float32)
+    T = np.array([[0, 1, 2]], np.int32)
+    return scenes.Scene("face", V, T, dict(scenes.BASE_MATERIAL), scenes.base_params(enable_strain_limiting=False),
+                        [], [], [])
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_elastic_blocks_closed_form(prec):
+    """K = k/h^2 [[1,-1],[-1,2]] for the face (0,0),(h,0),(h,h); every block is
+    SPD-clamped separately, so H01 and H02 collapse to 1e-8 I and H12 keeps k/h^2."""
+    e = KA["elastic_face"]
+    sc = single_face_scene(e["h"])
+    o = orc.OracleSim(sc, prec)
+    x = o.get("x")
+    g, H = o.linearize(x)  # x_hat = x: no inertia gradient
+    rp, ci = o.pattern()
+    m = o.get("m")
+    dti = 1.0 / sc.params["dt"] ** 2
+    kh2 = e["k_over_h2"]
+    rel = 1e-5 if prec == 32 else 1e-12
+
+    def block(i, j):
+        return H[rp[i] + list(ci[rp[i]:rp[i + 1]]).index(j)]
+
+    assert block(0, 0)[0, 0] == pytest.approx(m[0] * dti + kh2, rel=rel)     # K00+K11+2K01 = k/h^2
+    assert block(1, 1)[0, 0] == pytest.approx(m[1] * dti + kh2, rel=rel)     # K00
+    assert block(2, 2)[1, 1] == pytest.approx(m[2] * dti + 2 * kh2, rel=rel)  # K11
+    assert block(2, 2)[2, 2] == pytest.approx(m[2] * dti + 1e-8, rel=rel)
+    assert np.allclose(block(0, 1), np.eye(3) * 1e-8, rtol=rel, atol=0)    # -K00 clamped
+    assert np.allclose(block(0, 2), np.eye(3) * 1e-8, rtol=rel, atol=0)    # -K11 clamped
+    assert block(1, 2)[0, 0] == pytest.approx(kh2, rel=rel)                # -K01 = k/h^2
+    assert block(1, 2)[2, 2] == pytest.approx(1e-8, rel=rel)
+    # Rest-state elastic gradient vanishes up to rounding (F = I).
+    assert np.max(np.abs(o.get("egrad"))) < (1e-4 if prec == 32 else 1e-12) * kh2 * e["h"]
+
+
+def test_lumped_mass_total():
+    sc = scenes.drop21()
+    o = orc.OracleSim(sc, 64)
+    m = o.get("m")
+    V = sc.verts.astype(np.float64)
+    T = sc.tris
+    area = 0.5 * np.linalg.norm(np.cross(V[T[:, 1]] - V[T[:, 0]], V[T[:, 2]] - V[T[:, 0]]), axis=1).sum()
+    assert np.sum(m) == pytest.approx(sc.material["density"] * sc.material["thickness"] * area, rel=1e-12)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_free_fall_closed_form(prec):
+    """Unconstrained triangle: Newton solves exactly (1 PCG iteration), alpha=1
+    with extension 1.25, then the error-reduction pass pulls back by 0.25 d, so
+    the step keeps (1 - 1/16) of the velocity after gravity."""
+    sc = single_face_scene()
+    o = orc.OracleSim(sc, prec)
+    for _ in range(3):
+        o.apply_gravity()
+        vg = o.get("v")[:, 2].copy()
+        o.step()
+        ratio = o.get("v")[:, 2] / vg
+        tol = 1e-4 if prec == 32 else 1e-7
+        assert np.allclose(ratio, KA["free_fall"]["velocity_ratio"], atol=tol)
+        st = o.stats()
+        assert st["beta"] == 1.0 and st["newton_iters"] == 2
+
+
+def test_pin_and_wall_gradient_signs():
+    # Pins repel: a vertex 1 m from its target inside a 2 m barrier is pushed away.
+    V = np.array([[1.0, 0, 0], [2, 0, 0], [1, 1, 0]], np.float32)
+    sc = scenes.Scene("pin", V, np.array([[0, 1, 2]], np.int32), dict(scenes.BASE_MATERIAL),
+                      scenes.base_params(enable_strain_limiting=False, contact_gap_max=2.0), [(0, np.zeros(3))], [],
+                      [])
+    o = orc.OracleSim(sc, 64)
+    g, _ = o.linearize(o.get("x"))
+    egrad = o.get("egrad")
+    assert (g.reshape(-1, 3) - egrad)[0, 0] < 0.0
+    # Walls repel: gradient along -n for a vertex inside the barrier band.
+    V = np.array([[0.0, 0, 0.5], [1, 0, 0.5], [0, 1, 0.5]], np.float32)
+    sc = scenes.Scene("wall", V, np.array([[0, 1, 2]], np.int32), dict(scenes.BASE_MATERIAL),
+                      scenes.base_params(enable_strain_limiting=False, contact_gap_max=1.0), [],
+                      [(np.array([0, 0, 1.0]), 0.0, 1.0)], [])
+    o = orc.OracleSim(sc, 64)
+    g, _ = o.linearize(o.get("x"))
+    assert np.all((g.reshape(-1, 3) - o.get("egrad"))[:, 2] < 0.0)
+
+
+def test_line_search_known_answers():
+    w = KA["line_search_wall"]
+    V = np.array(w["verts"], np.float32)
+    sc = scenes.Scene("ls", V, np.array([[0, 1, 2]], np.int32), dict(scenes.BASE_MATERIAL),
+                      scenes.base_params(enable_strain_limiting=False), [], [(np.array([0, 0, 1.0]), 0.0, 1e-3)], [])
+    o = orc.OracleSim(sc, 32)
+    d = np.zeros(9)
+    d[2] = w["dir_z0"]
+    a = o.line_search(d, w["ext"])
+    assert w["alpha_open_interval"][0] < a < w["alpha_open_interval"][1]
+    assert a == w["expected_alpha"]
+    # Brute force over alpha = 2^-k: the first feasible trial.
+    x = o.get("x")
+    for k in range(20):
+        al = 2.0 ** -k
+        if np.all((x[:, 2] + al * w["ext"] * d.reshape(-1, 3)[:, 2]) >= 0):
+            break
+    assert a == al
+    p = KA["line_search_pt"]
+    V = np.array(p["verts"], np.float32)
+    sc = scenes.Scene("lspt", V, np.array([p["tri"]], np.int32), dict(scenes.BASE_MATERIAL),
+                      scenes.base_params(enable_strain_limiting=False), [], [], [])
+    o = orc.OracleSim(sc, 32)
+    o.set_pt_contacts([[0, 1, 2, 3]])
+    d = np.zeros(12)
+    d[2] = p["dir_z0"]
+    assert o.line_search(d, p["ext"]) < p["alpha_below"]
+
+
+def test_rigid_candidates_match_brute_force():
+    """Rigid candidates = every (vertex, rigid triangle) with distance < 0.01,
+    checked against an FP64 numpy brute force away from the threshold."""
+    sc = scenes.small_sheet(9, pins=0, wall=False, rigid=True)
+    o = orc.OracleSim(sc, 64)
+    for _ in range(5):
+        o.substep()
+    cs = o.contacts()
+    got = {(int(c[1]), int(c[5]), tuple(int(v) for v in c[2:5])) for c in cs}
+    x = o.get("x")
+    rs = o.rigid_state(0)
+    R = sc.rigids[0]
+    Vl = np.asarray(R.verts, np.float64)
+    cen = Vl.mean(axis=0)
+    W = (Vl - cen) @ rs["rotation"].T + rs["position"]
+    want, borderline = set(), set()
+    for vi in range(len(x)):
+        for t in R.tris:
+            d, _, _ = orc.narrow_pt(x[vi], W[t[0]], W[t[1]], W[t[2]], 64)
+            key = (vi, 0, tuple(int(v) for v in t))
+            if abs(d - 0.01) < 1e-9:
+                borderline.add(key)
+            elif d < 0.01:
+                want.add(key)
+    assert len(want) > 0
+    assert want <= got and got - want <= borderline
+
+
+def test_faithful_self_contact_is_empty():
+    """F1: the reference BVH yields no primitive leaves for meshes with more than
+    one triangle, so even a folded sheet with touching layers has no self pairs."""
+    V, T = scenes.grid(5, 0.1, 0.0)
+    V2 = V.copy()
+    V2[:, 2] += 1e-4
+    VV = np.concatenate([V, V2]).astype(np.float32)
+    TT = np.concatenate([T, T + len(V)]).astype(np.int32)
+    sc = scenes.Scene("fold", VV, TT, dict(scenes.BASE_MATERIAL), scenes.base_params(), [], [], [])
+    o = orc.OracleSim(sc, 32)
+    assert len(o.contacts()) == 0
+
+
+def test_matrix_is_exactly_symmetric_and_spd_calls_diagonal():
+    sc = scenes.drop21()
+    o = orc.OracleSim(sc, 32)
+    before = o.stats()["spd_general_calls"]  # process-wide counter
+    for _ in range(25):
+        o.substep()
+    st = o.stats()
+    assert st["asym_count"] == 0
+    assert st["spd_general_calls"] == before
