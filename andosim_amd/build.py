@@ -49,6 +49,14 @@ def build_lib():
     return LIB
 
 
+def build_copy_probe():
+    """Counter-calibration program (plain copy of known size); measurement only."""
+    out = os.path.join(HERE, "copy_probe")
+    _run([HIPCC, *HIP_FLAGS, "-o", out + ".tmp", os.path.join(CSRC, "copy_probe.hip")])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_module():
     import pybind11
 
@@ -70,6 +78,7 @@ def main(argv=None):
     both = not (a.lib or a.module)
     if a.lib or both:
         build_lib()
+        build_copy_probe()
     if a.module or both:
         build_module()
 

@@ -1,0 +1,151 @@
+"""The drop-in module on the GPU, and the optional step terms (friction,
+restitution, damping) against the oracle.
+
+ando_barrier_core.Integrator.step is driven the way the Blender add-on drives
+it (operators.py:900-905): State.apply_gravity, then Integrator.step.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from andosim_amd import scenes
+from oracle.oracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "andosim_amd"))
+native = pytest.importorskip("andosim_amd.native")
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def sync_state(o, g, nb=0):
+    g.set_state(o.get("x").astype(np.float32), o.get("v").astype(np.float32))
+    for b in range(nb):
+        rs = o.rigid_state(b)
+        s = g.bodies[b]
+        for k in range(3):
+            s.position[k] = float(np.float32(rs["position"][k]))
+            s.linear_velocity[k] = float(np.float32(rs["linear_velocity"][k]))
+            s.angular_velocity[k] = float(np.float32(rs["angular_velocity"][k]))
+        for k in range(9):
+            s.rotation[k] = float(np.float32(rs["rotation"].ravel()[k]))
+
+
+def friction_scene():
+    sc = scenes.small_sheet(9, pins=0, wall=False, rigid=True)
+    sc.params.update(enable_friction=True, friction_mu=0.5)
+    return sc
+
+
+def restitution_scene(rigid):
+    sc = scenes.small_sheet(9, pins=0, wall=not rigid, rigid=rigid, z=0.02 if not rigid else 0.3)
+    sc.params.update(contact_restitution=0.5, velocity_damping=0.1)
+    return sc
+
+
+@pytest.mark.parametrize("advance", [2, 4])
+def test_friction_linearize_bit_exact(advance):
+    sc = friction_scene()
+    o = OracleSim(sc, 32)
+    for _ in range(advance):
+        o.substep()
+    g = native.GpuSim(sc, fp64_near=False)
+    sync_state(o, g, 1)
+    x = o.get("x").astype(np.float32)
+    v = o.get("v").astype(np.float32)
+    xhat = (x + np.float32(sc.params["dt"]) * v).astype(np.float32)
+    go, Ho = o.linearize(xhat)
+    gg, Hg = g.linearize(xhat)
+    assert np.array_equal(gg, go.astype(np.float32).reshape(-1))
+    assert np.array_equal(Hg, Ho.astype(np.float32))
+
+
+@pytest.mark.parametrize("make,nb", [(friction_scene, 1), (lambda: restitution_scene(False), 0),
+                                     (lambda: restitution_scene(True), 1)])
+def test_optional_terms_step_parity(make, nb):
+    sc = make()
+    o = OracleSim(sc, 32)
+    worst = 0.0
+    for _ in range(8):
+        g = native.GpuSim(sc)
+        sync_state(o, g, nb)
+        o.substep()
+        g.substep()
+        worst = max(worst, rel_err(g.get("x"), o.get("x")))
+        if nb:
+            assert np.allclose(g.rigid_state(0)["linear_velocity"], o.rigid_state(0)["linear_velocity"],
+                               rtol=1e-4, atol=1e-12)
+    assert worst < 1e-6, worst
+
+
+def build_module_scene(abc, sc):
+    mesh = abc.Mesh()
+    mesh.initialize(sc.verts, sc.tris, abc.Material(**sc.material))
+    state = abc.State()
+    state.initialize(mesh)
+    cons = abc.Constraints()
+    for vi, t in sc.pins:
+        cons.add_pin(int(vi), t)
+    for n, off, gap in sc.walls:
+        cons.add_wall(n, off, gap)
+    p = abc.SimParams()
+    for k, v in sc.params.items():
+        setattr(p, k, v)
+    bodies = []
+    for r in sc.rigids:
+        b = abc.RigidBody()
+        b.initialize(r.verts, r.tris, r.density)
+        bodies.append(b)
+    return mesh, state, cons, p, bodies
+
+
+@pytest.mark.parametrize("make", [scenes.drop21, lambda: scenes.small_sheet(9, pins=0, wall=False, rigid=True)])
+def test_module_step_matches_oracle(make):
+    abc = pytest.importorskip("ando_barrier_core")
+    sc = make()
+    mesh, state, cons, p, bodies = build_module_scene(abc, sc)
+    o = OracleSim(sc, 32)
+    g = np.asarray(sc.gravity, np.float32)
+    worst = 0.0
+    for _ in range(20):
+        state.apply_gravity(g, p.dt)
+        if bodies:
+            abc.Integrator.step(mesh, state, cons, p, bodies)
+        else:
+            abc.Integrator.step(mesh, state, cons, p)
+        o.substep()
+        worst = max(worst, rel_err(state.get_positions(), o.get("x")))
+    assert worst < 1e-4, worst
+    st = abc.Integrator.last_stats()
+    assert st.pcg_solves >= 1 and st.newton_iterations >= 1
+    if bodies:
+        assert np.allclose(bodies[0].position, o.rigid_state(0)["position"], atol=1e-6)
+    # The per-frame consumers of the add-on still work after GPU steps.
+    contacts = abc.Integrator.compute_contacts(mesh, state, bodies) if bodies else abc.Integrator.compute_contacts(
+        mesh, state)
+    metrics = abc.CollisionValidator.compute_metrics(mesh, state, contacts, p.contact_gap_max, True)
+    assert metrics.num_total_contacts == len(contacts)
+    diag = abc.EnergyTracker.compute(mesh, state, cons, p)
+    assert diag.max_velocity > 0.0
+
+
+def test_module_host_edits_reach_device():
+    """set_velocities after a GPU step is honoured by the next step."""
+    abc = pytest.importorskip("ando_barrier_core")
+    sc = scenes.small_sheet(7, pins=0, wall=False)
+    mesh, state, cons, p, _ = build_module_scene(abc, sc)
+    state.apply_gravity((0.0, 0.0, -9.81), p.dt)
+    abc.Integrator.step(mesh, state, cons, p)
+    state.set_velocities(np.zeros((mesh.num_vertices(), 3), np.float32))
+    x0 = state.get_positions().copy()
+    abc.Integrator.step(mesh, state, cons, p)
+    # No gravity and zero velocity: the cloth stays (up to the strain-free rest gradient).
+    assert np.max(np.abs(state.get_positions() - x0)) < 1e-6
