@@ -10,6 +10,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -73,7 +74,7 @@ static inline int nblk(long long work, int bs = kBlock) { return (int)((work + b
 struct BodyCache {
   std::vector<float> local;
   std::vector<int32_t> tris;
-  DBuf<float> d_local, d_world, d_gbox, d_pose;
+  DBuf<float> d_local, d_world, d_gbox, d_pose, d_bbox;
   DBuf<int32_t> d_tris;
 };
 
@@ -132,6 +133,9 @@ struct ando_ctx {
   ando_stats stats{};
   int fp64_near = 1;
   int kernel_timing = 0;
+  int pcg_persistent = 1;  // one cooperative launch per solve
+  int pcg_wgs = 0;         // requested workgroups (0 = default)
+  int pcg_grid_cap = 0;    // co-resident workgroups (0 = not yet queried, -1 = no cooperative launch)
   std::vector<hipEvent_t> events;  // pool: start/stop pairs for timed launches
   ~ando_ctx() {
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
@@ -314,6 +318,8 @@ void upload_bodies(ando_ctx* c, const ando_rigid_body* bodies, int nbod) {
     if (ng)
       hipLaunchKernelGGL(k_rigid_groups, dim3(nblk(ng)), dim3(kBlock), 0, s, B.num_triangles, C.d_tris.p,
                          C.d_world.p, C.d_gbox.p);
+    C.d_bbox.ensure(6);
+    hipLaunchKernelGGL(k_rigid_bbox, dim3(1), dim3(kBlock), 0, s, ng, C.d_gbox.p, C.d_bbox.p);
     HIPCHK(hipGetLastError());
   }
 }
@@ -327,6 +333,7 @@ DetectBodies detect_bodies(ando_ctx* c, const ando_rigid_body* bodies) {
     D.b[b].world = c->bodies[b].d_world.p;
     D.b[b].tris = c->bodies[b].d_tris.p;
     D.b[b].gbox = c->bodies[b].d_gbox.p;
+    D.b[b].bbox = c->bodies[b].d_bbox.p;
   }
   return D;
 }
@@ -363,9 +370,80 @@ struct PcgOut {
   bool converged;
 };
 
+// Number of co-resident workgroups for the persistent PCG, a multiple of 8 so
+// that the XCD-contiguous row mapping holds.
+int persistent_grid(ando_ctx* c, int n) {
+  if (c->pcg_grid_cap == 0) {
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, c->device));
+    int per_cu = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_pcg_persistent),
+                                                        kBlock, 0));
+    c->pcg_grid_cap = std::max(1, per_cu) * prop.multiProcessorCount;
+    if (!prop.cooperativeLaunch) c->pcg_grid_cap = -1;
+  }
+  if (c->pcg_grid_cap < 0) return 0;
+  int want = c->pcg_wgs > 0 ? c->pcg_wgs : 512;
+  int rows_per_wg_min = 32;
+  want = std::min(want, (n + rows_per_wg_min - 1) / rows_per_wg_min);
+  want = std::min(want, c->pcg_grid_cap);
+  if (want >= 8) want -= want % 8;
+  return std::max(want, 1);
+}
+
+PcgOut run_pcg_persistent(ando_ctx* c, int n, const float* A, const int32_t* rp, const int32_t* cols,
+                          const int32_t* diag, const float* rhs, int negate, float* xsol, float tol, int maxit,
+                          int G) {
+  hipStream_t s = c->stream;
+  c->part_rz.ensure(G + 1);
+  c->part_pAp.ensure(G + 1);
+  c->part_max.ensure(G + 1);
+  c->pcg_status.ensure(1);
+  PcgPersistentArgs a;
+  a.n = n; a.A = A; a.row_ptr = rp; a.cols = cols; a.diag = diag; a.rhs = rhs; a.negate = negate;
+  a.x = xsol; a.r = c->r.p; a.z = c->z.p; a.p0 = c->pbuf0.p; a.p1 = c->pbuf1.p; a.Ap = c->Ap.p; a.Pinv = c->Pinv.p;
+  a.part_a = c->part_rz.p; a.part_b = c->part_pAp.p; a.part_max = c->part_max.p; a.st = c->pcg_status.p;
+  a.tol = tol; a.maxit = maxit;
+  void* args[] = {&a};
+  const bool timing = c->kernel_timing != 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (timing) {
+    while (c->events.size() < 2) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      c->events.push_back(e);
+    }
+    e0 = c->events[0];
+    e1 = c->events[1];
+    HIPCHK(hipEventRecord(e0, s));
+  }
+  HIPCHK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent), dim3(G), dim3(kBlock), args, 0,
+                                    s));
+  if (timing) HIPCHK(hipEventRecord(e1, s));
+  PcgStatus st{};
+  HIPCHK(hipMemcpyAsync(&st, c->pcg_status.p, sizeof(PcgStatus), hipMemcpyDeviceToHost, s));
+  sync(c);
+  if (timing) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    c->stats.timed_launches[0]++;
+    c->stats.timed_ms[0] += ms;
+    c->stats.timed_iterations += st.iters;
+  }
+  return PcgOut{st.iters, !st.failed};
+}
+
 PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int32_t* cols, const int32_t* diag,
                const float* rhs, int negate, float* xsol, float tol, int maxit) {
   hipStream_t s = c->stream;
+  if (c->pcg_persistent) {
+    int G = persistent_grid(c, n);
+    if (G > 0) {
+      c->stats.pcg_kernel = 1;
+      return run_pcg_persistent(c, n, A, rp, cols, diag, rhs, negate, xsol, tol, maxit, G);
+    }
+  }
+  c->stats.pcg_kernel = 0;
   int nwg_v = nblk(n), nwg_r = nblk(3LL * n);
   c->rz_hist.ensure((size_t)maxit + 2);
   c->pcg_status.ensure(1);
@@ -699,6 +777,7 @@ int ando_create(int32_t device, ando_ctx** out) {
       throw Fail(ANDO_EDEVICE, "hipStreamCreate failed");
     }
     c->scalars.ensure(4);
+    if (const char* w = std::getenv("ANDO_PCG_WGS")) c->pcg_wgs = std::atoi(w);
     *out = c;
   });
 }
@@ -889,9 +968,11 @@ int ando_set_walls(ando_ctx* c, const float* normals, const float* offsets, cons
 int ando_set_mode(ando_ctx* c, int32_t flags) {
   return guarded([&] {
     need(c != nullptr, ANDO_EINVAL, "null context");
-    need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING)) == 0, ANDO_EINVAL, "unknown mode flag");
+    need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING | ANDO_MODE_PCG_TWO_KERNEL)) == 0,
+         ANDO_EINVAL, "unknown mode flag");
     c->fp64_near = (flags & ANDO_MODE_FP64_NEAR_CONTACT) ? 1 : 0;
     c->kernel_timing = (flags & ANDO_MODE_KERNEL_TIMING) ? 1 : 0;
+    c->pcg_persistent = (flags & ANDO_MODE_PCG_TWO_KERNEL) ? 0 : 1;
   });
 }
 

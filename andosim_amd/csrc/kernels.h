@@ -36,6 +36,7 @@ struct DetectBody {
   const float* world;    // (nv, 3)
   const int32_t* tris;   // (nt, 3)
   const float* gbox;     // (ceil(nt/kGroup), 6): min xyz, max xyz, grown by 0.01 + 1e-5
+  const float* bbox;     // (6): union of gbox
 };
 struct DetectBodies {
   int32_t count;

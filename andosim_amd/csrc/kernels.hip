@@ -223,50 +223,89 @@ __device__ inline float rigid_contact_k(float mass, f3 E, f3 normal, float dt) {
   return inertial + el;
 }
 
-// Pass 0 counts, pass 1 writes records at ptr[i] in reference order.
-__global__ void k_rigid_detect(int n, int pass, const float* __restrict__ x, DetectBodies bodies,
-                               const int32_t* __restrict__ ptr, int32_t* __restrict__ count,
-                               RigidContact* __restrict__ out, const float* __restrict__ masses,
-                               const int32_t* __restrict__ diag, const float* __restrict__ Helast3, float dt) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  f3 p = ld3(x, i);
+// Whole-body box (union of the grown group boxes), one workgroup.
+__global__ void k_rigid_bbox(int ng, const float* __restrict__ gbox, float* __restrict__ bbox) {
+  __shared__ float lds[4];
+  float mn[3] = {3.4e38f, 3.4e38f, 3.4e38f}, mx[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+  for (int g = threadIdx.x; g < ng; g += blockDim.x)
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = fminf(mn[k], gbox[6 * g + k]);
+      mx[k] = fmaxf(mx[k], gbox[6 * g + 3 + k]);
+    }
+  for (int k = 0; k < 3; ++k) {
+    float a = block_reduce(mn[k], lds, [](float u, float v) { return fminf(u, v); });
+    float b = block_reduce(mx[k], lds, [](float u, float v) { return fmaxf(u, v); });
+    if (threadIdx.x == 0) {
+      bbox[k] = a;
+      bbox[3 + k] = b;
+    }
+  }
+}
+
+// Pass 0 counts, pass 1 writes records at ptr[i] in reference order.  Group
+// boxes are staged through LDS 256 at a time; a workgroup skips a body when
+// none of its 256 vertices lies in the body's grown box.
+constexpr int kGroupTile = 256;
+
+__global__ void __launch_bounds__(kBlock) k_rigid_detect(int n, int pass, const float* __restrict__ x,
+                                                         DetectBodies bodies, const int32_t* __restrict__ ptr,
+                                                         int32_t* __restrict__ count, RigidContact* __restrict__ out,
+                                                         const float* __restrict__ masses,
+                                                         const int32_t* __restrict__ diag,
+                                                         const float* __restrict__ Helast3, float dt) {
+  __shared__ float sbox[6 * kGroupTile];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = i < n;
+  f3 p = live ? ld3(x, i) : mk(0.0f, 0.0f, 0.0f);
   int c = 0;
-  int base = pass ? ptr[i] : 0;
+  int base = (pass && live) ? ptr[i] : 0;
   f3 E = mk(0.0f, 0.0f, 0.0f);
   float mass = 0.0f;
-  if (pass) {
+  if (pass && live) {
     const float* e = Helast3 + 3 * (size_t)diag[i];
     E = mk(e[0], e[1], e[2]);
     mass = masses[i];
   }
   for (int b = 0; b < bodies.count; ++b) {
     const DetectBody& B = bodies.b[b];
-    int ng = (B.nt + kGroup - 1) / kGroup;
-    for (int gi = 0; gi < ng; ++gi) {
-      const float* bx = B.gbox + 6 * gi;
-      if (p.x < bx[0] || p.y < bx[1] || p.z < bx[2] || p.x > bx[3] || p.y > bx[4] || p.z > bx[5]) continue;
-      int t1 = min(B.nt, (gi + 1) * kGroup);
-      for (int t = gi * kGroup; t < t1; ++t) {
-        f3 a = ld3(B.world, B.tris[3 * t]), bb = ld3(B.world, B.tris[3 * t + 1]), cc = ld3(B.world, B.tris[3 * t + 2]);
-        PTResult r = point_triangle(p, a, bb, cc);
-        if (r.dist < 0.01f) {
-          if (pass) {
-            RigidContact& o = out[base + c];
-            o.gap = r.dist;
-            o.nx = r.normal.x; o.ny = r.normal.y; o.nz = r.normal.z;
-            o.qx = r.wq.x; o.qy = r.wq.y; o.qz = r.wq.z;
-            o.k = rigid_contact_k(mass, E, r.normal, dt);
-            o.body = b;
-            o.tri = t;
-            o.vertex = i;
+    const float* bb0 = B.bbox;
+    bool in_body = live && !(p.x < bb0[0] || p.y < bb0[1] || p.z < bb0[2] || p.x > bb0[3] || p.y > bb0[4] ||
+                             p.z > bb0[5]);
+    if (!__syncthreads_or(in_body)) continue;
+    const int ng = (B.nt + kGroup - 1) / kGroup;
+    for (int g0 = 0; g0 < ng; g0 += kGroupTile) {
+      const int gn = min(kGroupTile, ng - g0);
+      for (int k = threadIdx.x; k < 6 * gn; k += blockDim.x) sbox[k] = B.gbox[6 * g0 + k];
+      __syncthreads();
+      if (in_body)
+        for (int gl = 0; gl < gn; ++gl) {
+          const float* bx = sbox + 6 * gl;
+          if (p.x < bx[0] || p.y < bx[1] || p.z < bx[2] || p.x > bx[3] || p.y > bx[4] || p.z > bx[5]) continue;
+          const int gi = g0 + gl;
+          const int t1 = min(B.nt, (gi + 1) * kGroup);
+          for (int t = gi * kGroup; t < t1; ++t) {
+            f3 a = ld3(B.world, B.tris[3 * t]), bq = ld3(B.world, B.tris[3 * t + 1]);
+            f3 cq = ld3(B.world, B.tris[3 * t + 2]);
+            PTResult r = point_triangle(p, a, bq, cq);
+            if (r.dist < 0.01f) {
+              if (pass) {
+                RigidContact& o = out[base + c];
+                o.gap = r.dist;
+                o.nx = r.normal.x; o.ny = r.normal.y; o.nz = r.normal.z;
+                o.qx = r.wq.x; o.qy = r.wq.y; o.qz = r.wq.z;
+                o.k = rigid_contact_k(mass, E, r.normal, dt);
+                o.body = b;
+                o.tri = t;
+                o.vertex = i;
+              }
+              ++c;
+            }
           }
-          ++c;
         }
-      }
+      __syncthreads();
     }
   }
-  if (!pass) count[i] = c;
+  if (!pass && live) count[i] = c;
 }
 
 // ---------------------------------------------------------------------------

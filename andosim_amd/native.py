@@ -63,7 +63,8 @@ class Stats(C.Structure):
                 ("pcg_failures", C.c_int32), ("line_search_failures", C.c_int32), ("contacts", C.c_int32),
                 ("rigid_contacts", C.c_int32), ("strain_active", C.c_int32), ("beta", C.c_float),
                 ("last_alpha", C.c_float), ("pcg_per_solve", C.c_int32 * MAX_RECORDED), ("step_ms", C.c_double),
-                ("timed_launches", C.c_int32 * 2), ("timed_ms", C.c_double * 2)]
+                ("timed_launches", C.c_int32 * 2), ("timed_ms", C.c_double * 2), ("timed_iterations", C.c_int32),
+                ("pcg_kernel", C.c_int32)]
 
 
 _lib = None
@@ -264,8 +265,9 @@ class GpuSim:
         self.gravity = _f(sc.gravity)
         self.dt = float(sc.params["dt"])
 
-    def set_mode(self, fp64_near: bool = True, kernel_timing: bool = False):
-        check(self.L.ando_set_mode(self.h, (1 if fp64_near else 0) | (2 if kernel_timing else 0)))
+    def set_mode(self, fp64_near: bool = True, kernel_timing: bool = False, pcg_two_kernel: bool = False):
+        flags = (1 if fp64_near else 0) | (2 if kernel_timing else 0) | (4 if pcg_two_kernel else 0)
+        check(self.L.ando_set_mode(self.h, flags))
 
     def apply_gravity(self, g=None, dt=None):
         g = self.gravity if g is None else _f(g)

@@ -133,6 +133,8 @@ typedef struct {
   double step_ms;                   /* wall time of the step, host clock */
   int32_t timed_launches[2];        /* ANDO_MODE_KERNEL_TIMING: launches that ran (not skipped) */
   double timed_ms[2];               /* summed event durations of those launches */
+  int32_t timed_iterations;         /* PCG iterations covered by timed persistent launches */
+  int32_t pcg_kernel;               /* 1: persistent single-launch PCG, 0: two kernels per iteration */
 } ando_stats;
 
 int32_t ando_abi_version(void);
@@ -167,6 +169,10 @@ int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, co
  * HIP events on the context's stream and accumulates their durations into
  * ando_stats.timed_ms (index 0: SpMV kernel, 1: update kernel). */
 #define ANDO_MODE_KERNEL_TIMING 2
+/* ANDO_MODE_PCG_TWO_KERNEL runs PCG as two kernels per iteration with a host
+ * check every few iterations instead of the single persistent launch (same
+ * arithmetic apart from the SpMV row-sum association). */
+#define ANDO_MODE_PCG_TWO_KERNEL 4
 int ando_set_mode(ando_ctx* ctx, int32_t flags);
 
 int ando_apply_gravity(ando_ctx* ctx, const float* gravity3, float dt);

