@@ -204,9 +204,13 @@ def test_persistent_and_two_kernel_pcg_agree(name, make, nb):
     x1, it1, ok1 = gs.pcg(rp, ci, H.astype(np.float32), b, tol=1e-6, maxit=1000)
     gs.set_mode(fp64_near=False, pcg_two_kernel=True)
     x2, it2, ok2 = gs.pcg(rp, ci, H.astype(np.float32), b, tol=1e-6, maxit=1000)
-    assert ok1 and ok2
-    assert abs(it1 - it2) <= max(2, it2 // 10)
+    x32, it32, ok32 = oracle_pcg(rp, ci, H, -g_, tol=1e-6, maxit=1000, precision=32)
+    # Both variants reach the same verdict as the FP32 oracle (some of these
+    # states are too stiff for FP32 PCG to reach 1e-6 within the cap).
+    assert ok1 == ok32 and ok2 == ok32
+    if ok32:
+        assert abs(it1 - it2) <= max(2, it2 // 10)
     x64, _, _ = oracle_pcg(rp, ci, H, -g_, tol=1e-12, maxit=5000, precision=64)
-    e32 = rel_err(oracle_pcg(rp, ci, H, -g_, tol=1e-6, maxit=1000, precision=32)[0], x64)
+    e32 = rel_err(x32, x64)
     assert rel_err(x1, x64) <= max(2.0 * e32, 1e-6)
     assert rel_err(x2, x64) <= max(2.0 * e32, 1e-6)
