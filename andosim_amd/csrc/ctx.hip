@@ -4,6 +4,7 @@
 // runs as a sequence of kernel launches; the host only reads back the scalars
 // that steer control flow (gradient max-norm, line-search mask, PCG status,
 // contact totals).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -70,6 +71,9 @@ struct DBuf {
 };
 
 static inline int nblk(long long work, int bs = kBlock) { return (int)((work + bs - 1) / bs); }
+// Workgroups of the block-row SpMV with R rows per lane group
+// (R * kRowsPerPass block rows per workgroup); R = 1 is the largest count.
+static inline int spmv_groups(long long n, int R = 1) { return nblk(n, ando::kRowsPerPass * R); }
 
 struct BodyCache {
   std::vector<float> local;
@@ -81,6 +85,50 @@ struct BodyCache {
 }  // namespace ando
 
 using namespace ando;
+
+// Metadata of the block-parallel SpMV for one block pattern (see
+// k_pcg_spmv_blk): packed column / row offset / row-start per block, the first
+// row of each workgroup run, and for each row the wave holding its remainder.
+struct SpmvMeta {
+  bool ok = false;  // pattern fits the packing (n < 2^23, rows of at most 64 blocks)
+  int groups = 0, nb = 0;
+  DBuf<uint32_t> meta;
+  DBuf<int32_t> run_row0, straddle;
+  DBuf<float> extra;
+};
+
+void build_spmv_meta(SpmvMeta& sm, int n, const std::vector<int32_t>& row_ptr, const std::vector<int32_t>& cols,
+                     hipStream_t s) {
+  const int nb = row_ptr[n];
+  sm.ok = false;
+  if (n <= 0 || nb <= 0 || n >= (1 << ando::kMetaColBits)) return;
+  for (int i = 0; i < n; ++i)
+    if (row_ptr[i + 1] - row_ptr[i] > 64 || row_ptr[i + 1] <= row_ptr[i]) return;
+  const int G = nblk(nb, kBlock), waves = G * (kBlock / 64);
+  std::vector<int32_t> brow(nb);
+  for (int i = 0; i < n; ++i)
+    for (int b = row_ptr[i]; b < row_ptr[i + 1]; ++b) brow[b] = i;
+  std::vector<uint32_t> meta(nb);
+  std::vector<int32_t> row0(G), straddle(n, -1);
+  for (int g = 0; g < G; ++g) row0[g] = brow[(size_t)g * kBlock];
+  for (int b = 0; b < nb; ++b) {
+    const int delta = brow[b] - row0[b / kBlock];
+    if (delta > 255) return;
+    meta[b] = (uint32_t)cols[b] | ((uint32_t)delta << ando::kMetaColBits) |
+              (b == row_ptr[brow[b]] ? 0x80000000u : 0u);
+  }
+  for (int w = 0; w < waves; ++w) {
+    const int b = 64 * w;
+    if (b > 0 && b < nb && brow[b] == brow[b - 1]) straddle[brow[b]] = w;
+  }
+  sm.meta.upload(meta, s);
+  sm.run_row0.upload(row0, s);
+  sm.straddle.upload(straddle, s);
+  sm.extra.ensure(3 * (size_t)waves);
+  sm.groups = G;
+  sm.nb = nb;
+  sm.ok = true;
+}
 
 struct ando_ctx {
   int device = 0;
@@ -133,8 +181,13 @@ struct ando_ctx {
   ando_stats stats{};
   int fp64_near = 1;
   int kernel_timing = 0;
-  int pcg_persistent = 1;  // one cooperative launch per solve
+  int pcg_persistent = 0;  // 1: one cooperative launch per solve, 0: two kernels per iteration
   int pcg_wgs = 0;         // requested workgroups (0 = default)
+  int pcg_barrier = 0;     // persistent grid barrier: 0 cooperative-groups sync, 1 counter barrier
+  int spmv_rowsets = 3;    // block rows per lane group of the row SpMV (1..4)
+  int spmv_blocks = 0;     // 1: block-parallel SpMV where the pattern allows it, 0: row SpMV
+  SpmvMeta spmv_meta;      // for the mesh's block pattern
+  DBuf<uint32_t> pcg_barrier_ctr;
   int pcg_grid_cap = 0;    // co-resident workgroups (0 = not yet queried, -1 = no cooperative launch)
   std::vector<hipEvent_t> events;  // pool: start/stop pairs for timed launches
   ~ando_ctx() {
@@ -229,6 +282,7 @@ void build_topology(ando_ctx* c, const std::vector<int32_t>& edges) {
   c->v2f_ptr.upload(vcnt, s);
   c->v2f.upload(v2f, s);
   c->face_blk.upload(face_blk, s);
+  build_spmv_meta(c->spmv_meta, n, c->row_ptr, c->cols, s);
   sync(c);
 }
 
@@ -241,7 +295,7 @@ void alloc_state(ando_ctx* c) {
   c->sflags.ensure(c->m); c->sgrad.ensure(18 * (size_t)c->m); c->shess.ensure(20 * (size_t)c->m);
   c->r.ensure(n3); c->z.ensure(n3); c->pbuf0.ensure(n3); c->pbuf1.ensure(n3); c->Ap.ensure(n3);
   c->Pinv.ensure(9 * (size_t)c->n);
-  c->part_rz.ensure(nblk(c->n) + 1); c->part_max.ensure(nblk(c->n) + 1); c->part_pAp.ensure(nblk(3LL * c->n) + 1);
+  c->part_rz.ensure(nblk(c->n) + 1); c->part_max.ensure(nblk(c->n) + 1); c->part_pAp.ensure(spmv_groups(c->n) + 1);
   c->rc_count.ensure(c->n + 1); c->rc_ptr.ensure(c->n + 1);
   c->scan_sums.ensure((c->n + 1023) / 1024 + 1); c->scan_total.ensure(1);
 }
@@ -399,7 +453,11 @@ PcgOut run_pcg_persistent(ando_ctx* c, int n, const float* A, const int32_t* rp,
   c->part_pAp.ensure(G + 1);
   c->part_max.ensure(G + 1);
   c->pcg_status.ensure(1);
+  c->pcg_barrier_ctr.ensure(1);
+  HIPCHK(hipMemsetAsync(c->pcg_barrier_ctr.p, 0, sizeof(uint32_t), s));
   PcgPersistentArgs a;
+  a.barrier = c->pcg_barrier_ctr.p;
+  a.barrier_kind = c->pcg_barrier;
   a.n = n; a.A = A; a.row_ptr = rp; a.cols = cols; a.diag = diag; a.rhs = rhs; a.negate = negate;
   a.x = xsol; a.r = c->r.p; a.z = c->z.p; a.p0 = c->pbuf0.p; a.p1 = c->pbuf1.p; a.Ap = c->Ap.p; a.Pinv = c->Pinv.p;
   a.part_a = c->part_rz.p; a.part_b = c->part_pAp.p; a.part_max = c->part_max.p; a.st = c->pcg_status.p;
@@ -434,7 +492,7 @@ PcgOut run_pcg_persistent(ando_ctx* c, int n, const float* A, const int32_t* rp,
 }
 
 PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int32_t* cols, const int32_t* diag,
-               const float* rhs, int negate, float* xsol, float tol, int maxit) {
+               const float* rhs, int negate, float* xsol, float tol, int maxit, SpmvMeta* sm) {
   hipStream_t s = c->stream;
   if (c->pcg_persistent) {
     int G = persistent_grid(c, n);
@@ -443,10 +501,13 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
       return run_pcg_persistent(c, n, A, rp, cols, diag, rhs, negate, xsol, tol, maxit, G);
     }
   }
-  c->stats.pcg_kernel = 0;
-  int nwg_v = nblk(n), nwg_r = nblk(3LL * n);
+  const int R = c->spmv_rowsets;
+  const bool blk = c->spmv_blocks && sm != nullptr && sm->ok;
+  int nwg_v = nblk(n), nwg_r = blk ? sm->groups : spmv_groups(n, R);
+  c->stats.pcg_kernel = blk ? 2 : 0;
   c->rz_hist.ensure((size_t)maxit + 2);
   c->pcg_status.ensure(1);
+  c->part_pAp.ensure((size_t)nwg_r + 1);
   hipLaunchKernelGGL(k_pcg_init, dim3(nwg_v), dim3(kBlock), 0, s, n, A, diag, rhs, xsol, c->r.p, c->z.p,
                      c->pbuf0.p, c->Pinv.p, c->part_rz.p, c->part_max.p, negate);
   hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, s, nwg_v, c->part_rz.p, c->part_max.p, c->rz_hist.p,
@@ -457,6 +518,8 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
   int chunk = std::max(1, c->pcg_chunk);
   const bool timing = c->kernel_timing != 0;
   size_t ev_used = 0;
+  // Timed launches record their start and end with the dispatch itself
+  // (hipExtLaunchKernelGGL), so the events add no packets between kernels.
   auto ev = [&](size_t k) -> hipEvent_t {
     while (c->events.size() <= k) {
       hipEvent_t e;
@@ -479,28 +542,45 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
     for (; it < end; ++it) {
       float* pold = (it == 0) ? c->pbuf0.p : ((it - 1) & 1 ? c->pbuf1.p : c->pbuf0.p);
       float* pcur = (it & 1) ? c->pbuf1.p : c->pbuf0.p;
-      if (timing) HIPCHK(hipEventRecord(ev(ev_used), s));
-      hipLaunchKernelGGL(k_pcg_spmv, dim3(nwg_r), dim3(kBlock), 0, s, n, it, A, rp, cols, c->z.p, pold, pcur,
-                         c->Ap.p, c->part_rz.p, c->part_max.p, nwg_v, c->rz_hist.p, c->part_pAp.p, c->pcg_status.p,
-                         tol);
-      if (timing) HIPCHK(hipEventRecord(ev(ev_used + 1), s));
-      hipLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(kBlock), 0, s, n, it, pcur, c->Ap.p, c->Pinv.p, xsol,
-                         c->r.p, c->z.p, c->part_pAp.p, nwg_r, c->rz_hist.p, c->part_rz.p, c->part_max.p,
-                         c->pcg_status.p);
+      hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
       if (timing) {
-        HIPCHK(hipEventRecord(ev(ev_used + 2), s));
-        ev_used += 3;
+        e0 = ev(ev_used); e1 = ev(ev_used + 1); e2 = ev(ev_used + 2); e3 = ev(ev_used + 3);
+        ev_used += 4;
       }
+      auto spmv = [&](auto kernel) {
+        hipExtLaunchKernelGGL(kernel, dim3(nwg_r), dim3(kBlock), 0, s, e0, e1, 0, n, it, A, rp, cols,
+                              (const float*)c->z.p, (const float*)pold, pcur, c->Ap.p, (const double*)c->part_rz.p,
+                              (const float*)c->part_max.p, nwg_v, c->rz_hist.p, c->part_pAp.p, c->pcg_status.p, tol);
+      };
+      if (blk) {
+        hipExtLaunchKernelGGL(k_pcg_spmv_blk, dim3(nwg_r), dim3(kBlock), 0, s, e0, e1, 0, sm->nb, it, A,
+                              (const uint32_t*)sm->meta.p, (const int32_t*)sm->run_row0.p, (const float*)c->z.p,
+                              (const float*)pold, pcur, c->Ap.p, sm->extra.p, (const double*)c->part_rz.p,
+                              (const float*)c->part_max.p, nwg_v, c->rz_hist.p, c->part_pAp.p, c->pcg_status.p, tol);
+      } else {
+        switch (R) {
+          case 1: spmv(k_pcg_spmv<1>); break;
+          case 2: spmv(k_pcg_spmv<2>); break;
+          case 4: spmv(k_pcg_spmv<4>); break;
+          default: spmv(k_pcg_spmv<3>); break;
+        }
+      }
+      hipExtLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(kBlock), 0, s, e2, e3, 0, n, it, (const float*)pcur,
+                            (const float*)c->Ap.p, (const float*)(blk ? sm->extra.p : nullptr),
+                            (const int32_t*)(blk ? sm->straddle.p : nullptr), (const float*)c->Pinv.p, xsol, c->r.p,
+                            c->z.p,
+                            (const double*)c->part_pAp.p, nwg_r, (const double*)c->rz_hist.p, c->part_rz.p,
+                            c->part_max.p, c->pcg_status.p);
     }
     HIPCHK(hipGetLastError());
     chunk = std::min(chunk * 2, 64);
   }
   if (timing) {
     // Launch k of the solve ran in full iff k < iterations.
-    for (size_t k = 0; 3 * k + 2 < ev_used + 0 && (int)k < st.iters; ++k) {
+    for (size_t k = 0; 4 * k + 3 < ev_used && (int)k < st.iters; ++k) {
       float a = 0.0f, b = 0.0f;
-      HIPCHK(hipEventElapsedTime(&a, c->events[3 * k], c->events[3 * k + 1]));
-      HIPCHK(hipEventElapsedTime(&b, c->events[3 * k + 1], c->events[3 * k + 2]));
+      HIPCHK(hipEventElapsedTime(&a, c->events[4 * k], c->events[4 * k + 1]));
+      HIPCHK(hipEventElapsedTime(&b, c->events[4 * k + 2], c->events[4 * k + 3]));
       c->stats.timed_launches[0]++;
       c->stats.timed_launches[1]++;
       c->stats.timed_ms[0] += a;
@@ -603,7 +683,7 @@ float inner_newton(ando_ctx* c, const ando_params& p) {
     if (gmax < p.pcg_tol) return 1.0f;
     launch_assemble(c, p, true);
     PcgOut po = run_pcg(c, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol,
-                        p.pcg_max_iters);
+                        p.pcg_max_iters, &c->spmv_meta);
     if (c->stats.pcg_solves < ANDO_MAX_RECORDED_SOLVES) c->stats.pcg_per_solve[c->stats.pcg_solves] = po.iters;
     c->stats.pcg_solves++;
     c->stats.pcg_iterations += po.iters;
@@ -778,6 +858,9 @@ int ando_create(int32_t device, ando_ctx** out) {
     }
     c->scalars.ensure(4);
     if (const char* w = std::getenv("ANDO_PCG_WGS")) c->pcg_wgs = std::atoi(w);
+    if (const char* w = std::getenv("ANDO_PCG_BARRIER")) c->pcg_barrier = std::atoi(w) == 1 ? 1 : 0;
+    if (const char* w = std::getenv("ANDO_PCG_ROWSETS")) c->spmv_rowsets = std::min(4, std::max(1, std::atoi(w)));
+    if (const char* w = std::getenv("ANDO_PCG_SPMV")) c->spmv_blocks = std::strcmp(w, "blk") == 0 ? 1 : 0;
     *out = c;
   });
 }
@@ -968,11 +1051,11 @@ int ando_set_walls(ando_ctx* c, const float* normals, const float* offsets, cons
 int ando_set_mode(ando_ctx* c, int32_t flags) {
   return guarded([&] {
     need(c != nullptr, ANDO_EINVAL, "null context");
-    need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING | ANDO_MODE_PCG_TWO_KERNEL)) == 0,
+    need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING | ANDO_MODE_PCG_PERSISTENT)) == 0,
          ANDO_EINVAL, "unknown mode flag");
     c->fp64_near = (flags & ANDO_MODE_FP64_NEAR_CONTACT) ? 1 : 0;
     c->kernel_timing = (flags & ANDO_MODE_KERNEL_TIMING) ? 1 : 0;
-    c->pcg_persistent = (flags & ANDO_MODE_PCG_TWO_KERNEL) ? 0 : 1;
+    c->pcg_persistent = (flags & ANDO_MODE_PCG_PERSISTENT) ? 1 : 0;
   });
 }
 
@@ -1151,9 +1234,11 @@ int ando_pcg(ando_ctx* c, int32_t num_rows, const int32_t* row_ptr, const int32_
       size_t n3 = 3 * (size_t)n;
       c->r.ensure(n3); c->z.ensure(n3); c->pbuf0.ensure(n3); c->pbuf1.ensure(n3); c->Ap.ensure(n3);
       c->Pinv.ensure(9 * (size_t)n);
-      c->part_rz.ensure(nblk(n) + 1); c->part_max.ensure(nblk(n) + 1); c->part_pAp.ensure(nblk(3LL * n) + 1);
+      c->part_rz.ensure(nblk(n) + 1); c->part_max.ensure(nblk(n) + 1); c->part_pAp.ensure(spmv_groups(n) + 1);
     }
-    PcgOut po = run_pcg(c, n, dv.p, drp.p, dci.p, ddg.p, db.p, 0, dx.p, tol, max_iters);
+    SpmvMeta sm;
+    build_spmv_meta(sm, n, std::vector<int32_t>(row_ptr, row_ptr + n + 1), std::vector<int32_t>(cols, cols + nbk), s);
+    PcgOut po = run_pcg(c, n, dv.p, drp.p, dci.p, ddg.p, db.p, 0, dx.p, tol, max_iters, &sm);
     HIPCHK(hipMemcpyAsync(x, dx.p, 3 * sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, s));
     sync(c);
     if (iterations) *iterations = po.iters;

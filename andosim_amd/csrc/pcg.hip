@@ -5,9 +5,9 @@
 // Two kernels per iteration, no host round trip inside a chunk of iterations:
 //   k_pcg_spmv(it):   reduce the previous iteration's r.z and max|r| partials,
 //                     test convergence, form p = z + beta p on the fly for every
-//                     column it touches (double-buffered p), Ap = A p with each
-//                     row summed left to right in column order (the order of
-//                     Eigen's row-major SpMV), and partial p.Ap.
+//                     column it touches (double-buffered p), Ap = A p with 8
+//                     lanes per block row (one block each, summed by a shuffle
+//                     tree), and partial p.Ap.
 //   k_pcg_update(it): reduce p.Ap, alpha = r.z / p.Ap, x += alpha p,
 //                     r -= alpha Ap, z = P^-1 r, partial r.z and max|r|.
 // Every workgroup reduces the partial arrays itself in a fixed order, so all
@@ -19,6 +19,9 @@
 
 namespace ando {
 
+constexpr int kLanesPerRow = 8;
+constexpr int kRowsPerPass = kBlock / kLanesPerRow;  // 32 block rows per workgroup pass
+
 struct PcgStatus {
   int32_t done;
   int32_t iters;
@@ -28,14 +31,38 @@ struct PcgStatus {
   float rel;
 };
 
+// Reductions of per-workgroup partials.  Each thread issues its kUnroll loads
+// of a chunk back to back before summing them (a plain strided loop would wait
+// for every load in turn), then the workgroup combines the thread sums; the
+// order is fixed, so every workgroup obtains the same value.
+constexpr int kUnroll = 12;
+
 __device__ inline double reduce_sum_partials(const double* p, int count, double* lds) {
   double s = 0.0;
-  for (int k = threadIdx.x; k < count; k += blockDim.x) s += p[k];
+  for (int base = 0; base < count; base += kUnroll * (int)blockDim.x) {
+    double v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int k = base + u * (int)blockDim.x + (int)threadIdx.x;
+      v[u] = k < count ? p[k] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) s += v[u];
+  }
   return block_reduce(s, lds, [](double a, double b) { return a + b; });
 }
 __device__ inline float reduce_max_partials(const float* p, int count, float* lds) {
   float s = 0.0f;
-  for (int k = threadIdx.x; k < count; k += blockDim.x) s = fmaxf(s, p[k]);
+  for (int base = 0; base < count; base += kUnroll * (int)blockDim.x) {
+    float v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int k = base + u * (int)blockDim.x + (int)threadIdx.x;
+      v[u] = k < count ? p[k] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) s = fmaxf(s, v[u]);
+  }
   return block_reduce(s, lds, [](float a, float b) { return fmaxf(a, b); });
 }
 
@@ -113,20 +140,60 @@ __global__ void k_pcg_start(int nwg, const double* __restrict__ part_rz, const f
   }
 }
 
-__global__ void k_pcg_spmv(int n, int it, const float* __restrict__ A, const int32_t* __restrict__ row_ptr,
-                           const int32_t* __restrict__ cols, const float* __restrict__ z,
-                           const float* __restrict__ p_old, float* __restrict__ p_new, float* __restrict__ Ap,
-                           const double* __restrict__ part_rz, const float* __restrict__ part_max, int nwg_v,
-                           double* __restrict__ rz_hist, double* __restrict__ part_pAp, PcgStatus* __restrict__ st,
-                           float tol) {
-  if (st->done) return;
+// One block row per group of kLanesPerRow lanes, one 3x3 block per lane per
+// pass, so a row's blocks (contiguous in BSR order) are read as one run; the
+// three row sums are combined by an xor-shuffle tree inside the lane group.
+// Each lane group handles R rows (kRowsPerPass apart), so a workgroup covers
+// R * kRowsPerPass rows; R is chosen so that the whole grid is resident at
+// once (no second round of waves).  Every
+// load that does not depend on beta (the rows' first pass of blocks, and the
+// z, p_old entries they touch) is issued before the reduction of the previous
+// iteration's partials, so the memory latency chains overlap.  Each row holds
+// its diagonal block, so row_ptr[i] is a valid block index for masked lanes.
+template <int R>
+__global__ void __launch_bounds__(kBlock) k_pcg_spmv(int n, int it, const float* __restrict__ A,
+                           const int32_t* __restrict__ row_ptr, const int32_t* __restrict__ cols,
+                           const float* __restrict__ z, const float* __restrict__ p_old, float* __restrict__ p_new,
+                           float* __restrict__ Ap, const double* __restrict__ part_rz,
+                           const float* __restrict__ part_max, int nwg_v, double* __restrict__ rz_hist,
+                           double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol) {
+  constexpr int kRowsPerGroup = kRowsPerPass * R;
   __shared__ double ldsd[4];
   __shared__ float ldsf[4];
+  const int lane = threadIdx.x % kLanesPerRow;
+  int row[R], ri[R], beg[R], end[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    row[r] = blockIdx.x * kRowsPerGroup + r * kRowsPerPass + threadIdx.x / kLanesPerRow;
+    ri[r] = row[r] < n ? row[r] : n - 1;
+    beg[r] = row_ptr[ri[r]];  // in flight while the done flag is read
+    end[r] = row_ptr[ri[r] + 1];
+  }
+  if (st->done) return;
+  bool act[R];
+  int bl[R];
+  float Ab[R][9], po[R][3], zo[R][3], pi[R][3], zi[R][3];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    act[r] = row[r] < n && beg[r] + lane < end[r];
+    bl[r] = act[r] ? beg[r] + lane : beg[r];
+    const int j = cols[bl[r]];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) Ab[r][e] = A[9 * (size_t)bl[r] + e];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      po[r][c] = p_old[3 * j + c];
+      pi[r][c] = p_old[3 * ri[r] + c];
+      zo[r][c] = it > 0 ? z[3 * j + c] : 0.0f;
+      zi[r][c] = it > 0 ? z[3 * ri[r] + c] : 0.0f;
+    }
+  }
   float beta = 0.0f;
   if (it > 0) {
+    const float bmax = st->bmax;
+    const double rz_prev = rz_hist[it - 1];
     double rz = reduce_sum_partials(part_rz, nwg_v, ldsd);
     float rmax = reduce_max_partials(part_max, nwg_v, ldsf);
-    float bmax = st->bmax;
     float rel = ((double)bmax < 1e-16) ? rmax : rmax / bmax;
     if (rel < tol) {
       if (threadIdx.x == 0) {
@@ -136,41 +203,232 @@ __global__ void k_pcg_spmv(int n, int it, const float* __restrict__ A, const int
       }
       return;
     }
-    beta = (float)(rz / rz_hist[it - 1]);
+    beta = (float)(rz / rz_prev);
     if (blockIdx.x == 0 && threadIdx.x == 0) rz_hist[it] = rz;
   }
-  int row = blockIdx.x * blockDim.x + threadIdx.x;
   double pap = 0.0;
-  if (row < 3 * n) {
-    int i = row / 3, rr = row - 3 * i;
-    float tmp = 0.0f;
-    for (int bb = row_ptr[i]; bb < row_ptr[i + 1]; ++bb) {
-      int j = cols[bb];
-      const float* Ab = A + 9 * (size_t)bb + 3 * rr;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        int col = 3 * j + c;
-        float pv = it > 0 ? z[col] + beta * p_old[col] : p_old[col];
-        tmp = tmp + Ab[c] * pv;
+  for (int r = 0; r < R; ++r) {
+    float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+    if (act[r]) {
+      float pv0, pv1, pv2;
+      if (it > 0) {
+        pv0 = zo[r][0] + beta * po[r][0];
+        pv1 = zo[r][1] + beta * po[r][1];
+        pv2 = zo[r][2] + beta * po[r][2];
+      } else {
+        pv0 = po[r][0]; pv1 = po[r][1]; pv2 = po[r][2];
       }
+      s0 = s0 + (Ab[r][0] * pv0 + (Ab[r][1] * pv1 + Ab[r][2] * pv2));
+      s1 = s1 + (Ab[r][3] * pv0 + (Ab[r][4] * pv1 + Ab[r][5] * pv2));
+      s2 = s2 + (Ab[r][6] * pv0 + (Ab[r][7] * pv1 + Ab[r][8] * pv2));
     }
-    float pr = it > 0 ? z[row] + beta * p_old[row] : p_old[row];
-    if (it > 0) p_new[row] = pr;
-    Ap[row] = tmp;
-    pap = (double)pr * (double)tmp;
+    // Rows with more than kLanesPerRow blocks.
+    for (int bb = beg[r] + lane + kLanesPerRow; row[r] < n && bb < end[r]; bb += kLanesPerRow) {
+      const int j = cols[bb];
+      float pv0, pv1, pv2;
+      if (it > 0) {
+        pv0 = z[3 * j] + beta * p_old[3 * j];
+        pv1 = z[3 * j + 1] + beta * p_old[3 * j + 1];
+        pv2 = z[3 * j + 2] + beta * p_old[3 * j + 2];
+      } else {
+        pv0 = p_old[3 * j];
+        pv1 = p_old[3 * j + 1];
+        pv2 = p_old[3 * j + 2];
+      }
+      const float* B = A + 9 * (size_t)bb;
+      s0 = s0 + (B[0] * pv0 + (B[1] * pv1 + B[2] * pv2));
+      s1 = s1 + (B[3] * pv0 + (B[4] * pv1 + B[5] * pv2));
+      s2 = s2 + (B[6] * pv0 + (B[7] * pv1 + B[8] * pv2));
+    }
+    for (int off = kLanesPerRow / 2; off > 0; off >>= 1) {
+      s0 += __shfl_xor(s0, off, kLanesPerRow);
+      s1 += __shfl_xor(s1, off, kLanesPerRow);
+      s2 += __shfl_xor(s2, off, kLanesPerRow);
+    }
+    const int i = row[r];
+    if (i < n && lane == 0) {
+      float q0, q1, q2;
+      if (it > 0) {
+        q0 = zi[r][0] + beta * pi[r][0];
+        q1 = zi[r][1] + beta * pi[r][1];
+        q2 = zi[r][2] + beta * pi[r][2];
+        p_new[3 * i] = q0;
+        p_new[3 * i + 1] = q1;
+        p_new[3 * i + 2] = q2;
+      } else {
+        q0 = pi[r][0]; q1 = pi[r][1]; q2 = pi[r][2];
+      }
+      Ap[3 * i] = s0;
+      Ap[3 * i + 1] = s1;
+      Ap[3 * i + 2] = s2;
+      pap += (double)q0 * s0 + (double)q1 * s1 + (double)q2 * s2;
+    }
   }
   double s = block_reduce(pap, ldsd, [](double a, double b) { return a + b; });
   if (threadIdx.x == 0) part_pAp[blockIdx.x] = s;
 }
 
-__global__ void k_pcg_update(int n, int it, const float* __restrict__ p, const float* __restrict__ Ap,
-                             const float* __restrict__ Pinv, float* __restrict__ x, float* __restrict__ r,
-                             float* __restrict__ z, const double* __restrict__ part_pAp, int nwg_r,
-                             const double* __restrict__ rz_hist, double* __restrict__ part_rz,
-                             float* __restrict__ part_max, PcgStatus* __restrict__ st) {
+// Block-parallel SpMV: one 3x3 block per lane, workgroups over contiguous
+// runs of kBlock blocks, so a wave reads 64 consecutive blocks (2304 bytes)
+// and no lane waits on a row pointer.  Per block, meta packs
+//   bits 0..22  column j,
+//   bits 23..30 row minus the first row of the workgroup's run,
+//   bit  31     the block is the first of its row.
+// A lane's three products are summed per row by a segmented suffix scan
+// inside the wave (rows are runs of lanes).  The lane that heads a segment
+// holds its sum: a row's first segment goes to Ap[row]; a row that continues
+// from the previous wave (at most one per wave, in lane 0; rows have at most
+// 64 blocks) stores its remainder in Ap_extra[wave] and the update kernel adds
+// it.  Workgroups are numbered XCD-contiguously (hardware places workgroup w
+// on XCD w % 8), so neighbouring runs share an L2.
+constexpr int kMetaColBits = 23;
+constexpr uint32_t kMetaColMask = (1u << kMetaColBits) - 1u;
+
+__device__ inline int xcd_logical_group(int w, int G) {
+  const int x = w % 8, q = G / 8, rem = G % 8;
+  return x * q + min(x, rem) + w / 8;
+}
+
+__global__ void __launch_bounds__(kBlock) k_pcg_spmv_blk(int nb, int it, const float* __restrict__ A,
+                           const uint32_t* __restrict__ meta, const int32_t* __restrict__ run_row0,
+                           const float* __restrict__ z, const float* __restrict__ p_old, float* __restrict__ p_new,
+                           float* __restrict__ Ap, float* __restrict__ Ap_extra, const double* __restrict__ part_rz,
+                           const float* __restrict__ part_max, int nwg_v, double* __restrict__ rz_hist,
+                           double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol) {
+  __shared__ double ldsd[4];
+  __shared__ float ldsf[4];
+  const int g = xcd_logical_group(blockIdx.x, gridDim.x);
+  const int b = g * kBlock + threadIdx.x;
+  const bool act = b < nb;
+  const int bl = act ? b : nb - 1;
+  const uint32_t mt = meta[bl];  // in flight while the done flag is read
+  const int row0 = run_row0[g];
+  if (st->done) return;
+  float Ab[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) Ab[e] = A[9 * (size_t)bl + e];
+  const int j = (int)(mt & kMetaColMask);
+  const int row = row0 + (int)((mt >> kMetaColBits) & 0xffu);
+  const bool first = (mt >> 31) != 0u;
+  float po[3], zo[3], pr[3], zr[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    po[c] = p_old[3 * j + c];
+    pr[c] = p_old[3 * row + c];
+    zo[c] = it > 0 ? z[3 * j + c] : 0.0f;
+    zr[c] = it > 0 ? z[3 * row + c] : 0.0f;
+  }
+  float beta = 0.0f;
+  if (it > 0) {
+    const float bmax = st->bmax;
+    const double rz_prev = rz_hist[it - 1];
+    double rz = reduce_sum_partials(part_rz, nwg_v, ldsd);
+    float rmax = reduce_max_partials(part_max, nwg_v, ldsf);
+    float rel = ((double)bmax < 1e-16) ? rmax : rmax / bmax;
+    if (rel < tol) {
+      if (threadIdx.x == 0) {
+        st->rel = rel;
+        st->iters = it;
+        st->done = 1;
+      }
+      return;
+    }
+    beta = (float)(rz / rz_prev);
+    if (g == 0 && threadIdx.x == 0) rz_hist[it] = rz;
+  }
+  float pv0, pv1, pv2;
+  if (it > 0) {
+    pv0 = zo[0] + beta * po[0];
+    pv1 = zo[1] + beta * po[1];
+    pv2 = zo[2] + beta * po[2];
+  } else {
+    pv0 = po[0]; pv1 = po[1]; pv2 = po[2];
+  }
+  float s0 = Ab[0] * pv0 + (Ab[1] * pv1 + Ab[2] * pv2);
+  float s1 = Ab[3] * pv0 + (Ab[4] * pv1 + Ab[5] * pv2);
+  float s2 = Ab[6] * pv0 + (Ab[7] * pv1 + Ab[8] * pv2);
+  // Inactive tail lanes get distinct negative rows, so they join no segment.
+  const int lane = threadIdx.x & 63;
+  const int key = act ? row : -1 - lane;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const float u0 = __shfl_down(s0, off, 64);
+    const float u1 = __shfl_down(s1, off, 64);
+    const float u2 = __shfl_down(s2, off, 64);
+    const int ku = __shfl_down(key, off, 64);
+    if (lane + off < 64 && ku == key) {
+      s0 += u0;
+      s1 += u1;
+      s2 += u2;
+    }
+  }
+  const int kprev = __shfl_up(key, 1, 64);
+  const bool head = act && (lane == 0 || kprev != key);
+  double pap = 0.0;
+  if (head) {
+    float q0, q1, q2;
+    if (it > 0) {
+      q0 = zr[0] + beta * pr[0];
+      q1 = zr[1] + beta * pr[1];
+      q2 = zr[2] + beta * pr[2];
+    } else {
+      q0 = pr[0]; q1 = pr[1]; q2 = pr[2];
+    }
+    if (first) {
+      Ap[3 * row] = s0;
+      Ap[3 * row + 1] = s1;
+      Ap[3 * row + 2] = s2;
+      if (it > 0) {
+        p_new[3 * row] = q0;
+        p_new[3 * row + 1] = q1;
+        p_new[3 * row + 2] = q2;
+      }
+    } else {
+      const int wave = g * (kBlock / 64) + (int)(threadIdx.x >> 6);
+      Ap_extra[3 * wave] = s0;
+      Ap_extra[3 * wave + 1] = s1;
+      Ap_extra[3 * wave + 2] = s2;
+    }
+    pap = (double)q0 * s0 + (double)q1 * s1 + (double)q2 * s2;
+  }
+  double s = block_reduce(pap, ldsd, [](double a, double b2) { return a + b2; });
+  if (threadIdx.x == 0) part_pAp[g] = s;
+}
+
+// alpha = r.z / p.Ap, x += alpha p, r -= alpha Ap, z = P^-1 r; one thread per
+// vertex.  The vertex's operands are loaded before the p.Ap reduction.
+// With the block-parallel SpMV, straddle[i] >= 0 names the wave whose
+// Ap_extra entry holds the rest of row i (straddle == nullptr otherwise).
+__global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const float* __restrict__ p,
+                             const float* __restrict__ Ap, const float* __restrict__ Ap_extra,
+                             const int32_t* __restrict__ straddle, const float* __restrict__ Pinv,
+                             float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
+                             const double* __restrict__ part_pAp, int nwg_r, const double* __restrict__ rz_hist,
+                             double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st) {
   if (st->done) return;
   __shared__ double ldsd[4];
   __shared__ float ldsf[4];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ii = i < n ? i : n - 1;
+  float pv[3], av[3], xv[3], rv0[3], P[9];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    pv[k] = p[3 * ii + k];
+    av[k] = Ap[3 * ii + k];
+    xv[k] = x[3 * ii + k];
+    rv0[k] = r[3 * ii + k];
+  }
+  if (straddle != nullptr) {
+    const int w = straddle[ii];
+    if (w >= 0) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) av[k] = av[k] + Ap_extra[3 * w + k];
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) P[e] = Pinv[9 * (size_t)ii + e];
+  const double rz_it = rz_hist[it];
   double pAp = reduce_sum_partials(part_pAp, nwg_r, ldsd);
   if (fabs(pAp) < 1e-16) {
     if (threadIdx.x == 0) {
@@ -180,20 +438,18 @@ __global__ void k_pcg_update(int n, int it, const float* __restrict__ p, const f
     }
     return;
   }
-  float alpha = (float)(rz_hist[it] / pAp);
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float alpha = (float)(rz_it / pAp);
   double rz = 0.0;
   float rm = 0.0f;
   if (i < n) {
     float rv[3];
     for (int k = 0; k < 3; ++k) {
       int e = 3 * i + k;
-      x[e] = x[e] + alpha * p[e];
-      rv[k] = r[e] - alpha * Ap[e];
+      x[e] = xv[k] + alpha * pv[k];
+      rv[k] = rv0[k] - alpha * av[k];
       r[e] = rv[k];
       rm = fmaxf(rm, fabsf(rv[k]));
     }
-    const float* P = Pinv + 9 * (size_t)i;
     for (int k = 0; k < 3; ++k) {
       float zk = P[3 * k] * rv[0] + (P[3 * k + 1] * rv[1] + P[3 * k + 2] * rv[2]);
       z[3 * i + k] = zk;
@@ -225,6 +481,8 @@ __global__ void k_pcg_update(int n, int it, const float* __restrict__ p, const f
 // partial r.z and max|r| | grid barrier | convergence test and beta.
 // ---------------------------------------------------------------------------
 struct PcgPersistentArgs {
+  unsigned int* barrier;  // monotonically increasing arrival counter, zeroed before the launch
+  int barrier_kind;       // 0: cooperative-groups grid sync, 1: counter barrier
   int n;
   const float* A;
   const int32_t* row_ptr;
@@ -247,8 +505,6 @@ struct PcgPersistentArgs {
   int maxit;
 };
 
-constexpr int kLanesPerRow = 8;
-constexpr int kRowsPerPass = kBlock / kLanesPerRow;  // 32 block rows per workgroup pass
 
 __device__ inline void wg_range(int n, int& lo, int& hi) {
   const int G = gridDim.x, w = blockIdx.x;
@@ -259,11 +515,38 @@ __device__ inline void wg_range(int n, int& lo, int& hi) {
   hi = (int)min((long long)n, lo + per);
 }
 
+// Grid barrier for a co-resident grid (the launch is cooperative, so every
+// workgroup is resident and every wait ends).  After the workgroup barrier
+// (which waits for the workgroup's stores), thread 0 counts the workgroup in
+// with an agent-scope release, spins with relaxed loads until all G workgroups
+// of this generation have arrived, and then takes one agent-scope acquire
+// fence; the counter only grows within a launch.
+__device__ inline void counter_barrier(unsigned int* counter, unsigned int target) {
+  __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have reached L2
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
+      __builtin_amdgcn_s_sleep(1);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
 __global__ void __launch_bounds__(kBlock) k_pcg_persistent(PcgPersistentArgs a) {
   cooperative_groups::grid_group grid = cooperative_groups::this_grid();
   __shared__ double ldsd[4];
   __shared__ float ldsf[4];
   const int G = gridDim.x;
+  unsigned int generation = 0;
+  auto grid_sync = [&]() {
+    if (a.barrier_kind == 1) {
+      ++generation;
+      counter_barrier(a.barrier, generation * (unsigned int)G);
+    } else {
+      grid.sync();
+    }
+  };
   int lo, hi;
   wg_range(a.n, lo, hi);
 
@@ -307,7 +590,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_persistent(PcgPersistentArgs a) 
       a.part_max[blockIdx.x] = m;
     }
   }
-  grid.sync();
+  grid_sync();
   double rz = reduce_sum_partials(a.part_a, G, ldsd);
   const float bmax = reduce_max_partials(a.part_max, G, ldsf);
   {
@@ -380,7 +663,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_persistent(PcgPersistentArgs a) 
       double s = block_reduce(pap_part, ldsd, [](double u, double v) { return u + v; });
       if (threadIdx.x == 0) a.part_b[blockIdx.x] = s;
     }
-    grid.sync();
+    grid_sync();
     double pAp = reduce_sum_partials(a.part_b, G, ldsd);
     if (fabs(pAp) < 1e-16) {
       if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -417,7 +700,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_persistent(PcgPersistentArgs a) 
         a.part_max[blockIdx.x] = m;
       }
     }
-    grid.sync();
+    grid_sync();
     double rz_new = reduce_sum_partials(a.part_a, G, ldsd);
     float rmax = reduce_max_partials(a.part_max, G, ldsf);
     float rel = ((double)bmax < 1e-16) ? rmax : rmax / bmax;

@@ -265,8 +265,8 @@ class GpuSim:
         self.gravity = _f(sc.gravity)
         self.dt = float(sc.params["dt"])
 
-    def set_mode(self, fp64_near: bool = True, kernel_timing: bool = False, pcg_two_kernel: bool = False):
-        flags = (1 if fp64_near else 0) | (2 if kernel_timing else 0) | (4 if pcg_two_kernel else 0)
+    def set_mode(self, fp64_near: bool = True, kernel_timing: bool = False, pcg_persistent: bool = False):
+        flags = (1 if fp64_near else 0) | (2 if kernel_timing else 0) | (4 if pcg_persistent else 0)
         check(self.L.ando_set_mode(self.h, flags))
 
     def apply_gravity(self, g=None, dt=None):

@@ -165,14 +165,15 @@ int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, co
  * with flags = 0 every barrier term rounds exactly as the reference's FP32
  * build. */
 #define ANDO_MODE_FP64_NEAR_CONTACT 1
-/* ANDO_MODE_KERNEL_TIMING brackets every launch of the two PCG kernels with
- * HIP events on the context's stream and accumulates their durations into
- * ando_stats.timed_ms (index 0: SpMV kernel, 1: update kernel). */
+/* ANDO_MODE_KERNEL_TIMING brackets every PCG launch with HIP events on the
+ * context's stream and accumulates their durations into ando_stats.timed_ms
+ * (two-kernel PCG: index 0 SpMV kernel, 1 update kernel; persistent PCG:
+ * index 0 the whole-solve launch). */
 #define ANDO_MODE_KERNEL_TIMING 2
-/* ANDO_MODE_PCG_TWO_KERNEL runs PCG as two kernels per iteration with a host
- * check every few iterations instead of the single persistent launch (same
- * arithmetic apart from the SpMV row-sum association). */
-#define ANDO_MODE_PCG_TWO_KERNEL 4
+/* ANDO_MODE_PCG_PERSISTENT runs each PCG solve as one cooperative launch with
+ * grid barriers instead of the default two kernels per iteration (host check
+ * every few iterations); same arithmetic. */
+#define ANDO_MODE_PCG_PERSISTENT 4
 int ando_set_mode(ando_ctx* ctx, int32_t flags);
 
 int ando_apply_gravity(ando_ctx* ctx, const float* gravity3, float dt);

@@ -200,9 +200,9 @@ def test_persistent_and_two_kernel_pcg_agree(name, make, nb):
     rp, ci = o.pattern()
     b = (-g_).astype(np.float32)
     gs = gpu(sc)
-    gs.set_mode(fp64_near=False, pcg_two_kernel=False)
+    gs.set_mode(fp64_near=False, pcg_persistent=True)
     x1, it1, ok1 = gs.pcg(rp, ci, H.astype(np.float32), b, tol=1e-6, maxit=1000)
-    gs.set_mode(fp64_near=False, pcg_two_kernel=True)
+    gs.set_mode(fp64_near=False, pcg_persistent=False)
     x2, it2, ok2 = gs.pcg(rp, ci, H.astype(np.float32), b, tol=1e-6, maxit=1000)
     x32, it32, ok32 = oracle_pcg(rp, ci, H, -g_, tol=1e-6, maxit=1000, precision=32)
     # Both variants reach the same verdict as the FP32 oracle (some of these
