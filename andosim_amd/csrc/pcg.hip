@@ -140,12 +140,21 @@ __global__ void k_pcg_start(int nwg, const double* __restrict__ part_rz, const f
   }
 }
 
+// Workgroup numbering that gives each XCD a contiguous run of logical
+// workgroups: hardware places workgroup w on XCD w % 8, and XCD x receives
+// logical groups [x*q + min(x, rem), ...) in order, so neighbouring row or
+// block ranges (which read the same z and p entries) share one L2.
+__device__ inline int xcd_logical_group(int w, int G) {
+  const int x = w % 8, q = G / 8, rem = G % 8;
+  return x * q + min(x, rem) + w / 8;
+}
+
 // One block row per group of kLanesPerRow lanes, one 3x3 block per lane per
 // pass, so a row's blocks (contiguous in BSR order) are read as one run; the
 // three row sums are combined by an xor-shuffle tree inside the lane group.
 // Each lane group handles R rows (kRowsPerPass apart), so a workgroup covers
 // R * kRowsPerPass rows; R is chosen so that the whole grid is resident at
-// once (no second round of waves).  Every
+// once (no second round of waves).  Row ranges follow xcd_logical_group.  Every
 // load that does not depend on beta (the rows' first pass of blocks, and the
 // z, p_old entries they touch) is issued before the reduction of the previous
 // iteration's partials, so the memory latency chains overlap.  Each row holds
@@ -161,10 +170,11 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int n, int it, const float*
   __shared__ double ldsd[4];
   __shared__ float ldsf[4];
   const int lane = threadIdx.x % kLanesPerRow;
+  const int g = xcd_logical_group(blockIdx.x, gridDim.x);
   int row[R], ri[R], beg[R], end[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    row[r] = blockIdx.x * kRowsPerGroup + r * kRowsPerPass + threadIdx.x / kLanesPerRow;
+    row[r] = g * kRowsPerGroup + r * kRowsPerPass + threadIdx.x / kLanesPerRow;
     ri[r] = row[r] < n ? row[r] : n - 1;
     beg[r] = row_ptr[ri[r]];  // in flight while the done flag is read
     end[r] = row_ptr[ri[r] + 1];
@@ -204,7 +214,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int n, int it, const float*
       return;
     }
     beta = (float)(rz / rz_prev);
-    if (blockIdx.x == 0 && threadIdx.x == 0) rz_hist[it] = rz;
+    if (g == 0 && threadIdx.x == 0) rz_hist[it] = rz;
   }
   double pap = 0.0;
 #pragma unroll
@@ -266,7 +276,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int n, int it, const float*
     }
   }
   double s = block_reduce(pap, ldsd, [](double a, double b) { return a + b; });
-  if (threadIdx.x == 0) part_pAp[blockIdx.x] = s;
+  if (threadIdx.x == 0) part_pAp[g] = s;
 }
 
 // Block-parallel SpMV: one 3x3 block per lane, workgroups over contiguous
@@ -284,11 +294,6 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int n, int it, const float*
 // on XCD w % 8), so neighbouring runs share an L2.
 constexpr int kMetaColBits = 23;
 constexpr uint32_t kMetaColMask = (1u << kMetaColBits) - 1u;
-
-__device__ inline int xcd_logical_group(int w, int G) {
-  const int x = w % 8, q = G / 8, rem = G % 8;
-  return x * q + min(x, rem) + w / 8;
-}
 
 __global__ void __launch_bounds__(kBlock) k_pcg_spmv_blk(int nb, int it, const float* __restrict__ A,
                            const uint32_t* __restrict__ meta, const int32_t* __restrict__ run_row0,

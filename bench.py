@@ -57,6 +57,24 @@ def init_bytes(n: int) -> int:
     return 40 * n + 12 * n + 36 * n + 4 * 12 * n
 
 
+def measured_traffic(workload: str, n: int, nb: int, kernel_prefix: str):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel
+    calibration), or None when no summary matches this workload."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("vertices") != n or d.get("blocks") != nb:
+        return None
+    for name, rec in d.get("kernels", {}).items():
+        if kernel_prefix in name:
+            return int(round(rec["traffic_bytes"]))
+    return None
+
+
 def finite_state(sim) -> bool:
     x = sim.get("x")
     v = sim.get("v")
@@ -182,7 +200,8 @@ def main(argv=None):
         by = spmv_bytes(n, nb)
         ach = by / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_pcg_spmv",
+                "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(a.workload, n, nb, "k_pcg_spmv"), "kernel": "k_pcg_spmv",
                 "bytes_per_launch": by, "avg_launch_us": round(avg_s * 1e6, 2), "launches": n_launch[0]}
         if n_launch[1]:
             avg_u = t_launch[1] / n_launch[1] * 1e-3
