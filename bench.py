@@ -20,8 +20,10 @@ non-finite state aborts the bench.
 The steps/s value comes from a pass without per-launch events.  A second
 pass over the same window records HIP events around every PCG launch on the
 library's stream; the roofline entry for the dominant kernel (the PCG SpMV)
-comes from it.  The CPU baseline is the oracle, one thread, on a bounded
-sample of the same window.
+comes from it.  The CPU baseline is the oracle on a bounded sample of the same
+window, with its element-wise loops on all host threads OpenMP is given; the
+same oracle on one pinned core (the reference is single-threaded) is printed
+beside it.
 """
 from __future__ import annotations
 
@@ -88,7 +90,7 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=3, help="untimed substeps replayed before timing")
     ap.add_argument("--workload", default="cyl257")
     ap.add_argument("--max-explore", type=int, default=300, help="substeps searched for the valid window")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="per CPU baseline run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip the kernel-event pass")
     ap.add_argument("--pcg", choices=["persistent", "two-kernel"], default="two-kernel")
@@ -210,27 +212,46 @@ def main(argv=None):
             roof["pcg_kernels_share_of_step"] = round((t_launch[0] + t_launch[1]) * 1e-3 / ev_elapsed, 3)
         roof["event_pass_ms_per_step"] = round(1e3 * ev_elapsed / len(ev_recs), 3)
 
-    cpu = None
+    cpu = cpu_single = None
     if not a.no_cpu:
-        from oracle.oracle import OracleSim
+        from oracle import oracle as orc
 
-        o = OracleSim(sc, 32)
-        x0, v0, rig = saved
-        o.set("x", x0.astype(np.float64))
-        o.set("v", v0.astype(np.float64))
-        for b, rs in enumerate(rig):
-            o.set_rigid_pose(b, rs)
-        k = 0
-        c0 = time.perf_counter()
-        while k < len(window):
-            o.substep()
-            k += 1
-            if time.perf_counter() - c0 > a.cpu_seconds:
-                break
-        cs = time.perf_counter() - c0
-        cpu = {"value": round(k / cs, 4), "unit": "steps/s", "cores": 1, "kind": "port",
+        def cpu_run(threads: int):
+            """Oracle steps/s over the first substeps of the timed window, from
+            the same start state, stopping after --cpu-seconds."""
+            orc.set_threads(threads)
+            o = orc.OracleSim(sc, 32)
+            x0, v0, rig = saved
+            o.set("x", x0.astype(np.float64))
+            o.set("v", v0.astype(np.float64))
+            for b, rs in enumerate(rig):
+                o.set_rigid_pose(b, rs)
+            k = 0
+            c0 = time.perf_counter()
+            while k < len(window):
+                o.substep()
+                k += 1
+                if time.perf_counter() - c0 > a.cpu_seconds:
+                    break
+            cs = time.perf_counter() - c0
+            return k, cs
+
+        threads = orc.get_threads()
+        k, cs = cpu_run(threads)
+        cpu = {"value": round(k / cs, 4), "unit": "steps/s", "cores": threads, "kind": "port",
                "sample": f"first {k} substeps of the timed window (steps {window[0]}..{window[0] + k - 1} of "
-                         f"{a.workload}), oracle FP32 scan-free, 1 thread, {cs:.1f} s"}
+                         f"{a.workload}), oracle FP32 scan-free, OpenMP loops on {threads} threads "
+                         f"(results identical to 1 thread), {cs:.1f} s"}
+        # The reference itself is single-threaded: the same oracle on one pinned core.
+        aff = os.sched_getaffinity(0)
+        try:
+            os.sched_setaffinity(0, {min(aff)})
+            k1, cs1 = cpu_run(1)
+        finally:
+            os.sched_setaffinity(0, aff)
+        cpu_single = {"value": round(k1 / cs1, 4), "unit": "steps/s", "cores": 1, "kind": "port",
+                      "sample": f"first {k1} substeps of the timed window, oracle FP32 scan-free, 1 thread "
+                                f"pinned to one core, {cs1:.1f} s"}
 
     out = {
         "metric": METRIC, "value": round(steps_per_s, 3), "unit": "steps/s", "n_gpus": 1, "steps": len(recs),
@@ -244,7 +265,7 @@ def main(argv=None):
                    "first_nonfinite_step": last_ok + 1 if last_ok + 1 < a.max_explore else None},
         "pcg_iters_per_step": round(pcg_iters / len(recs), 2),
         "newton_iters_per_step": round(newton / len(recs), 2),
-        "roofline": roof, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu_single,
     }
     print(json.dumps(out), flush=True)
     return 0

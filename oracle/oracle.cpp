@@ -40,6 +40,8 @@
 #include <unordered_set>
 #include <vector>
 
+#include <omp.h>
+
 namespace orc {
 
 template <class R> struct V3 {
@@ -826,6 +828,8 @@ template <class R> struct Sim {
     const R min_gap = std::max(P.min_gap, R(1e-8));
     const R svd_eps = std::max(P.strain_svd_eps, R(1e-8));
     const R tiny = R(1e-12);
+    int active_count = 0;
+#pragma omp parallel for schedule(static) reduction(+ : active_count)
     for (size_t f = 0; f < tris.size(); ++f) {
       if (area[f] <= tiny) continue;
       const auto& tr = tris[f];
@@ -865,7 +869,7 @@ template <class R> struct Sim {
         S.active[s] = 1;
         S.k[s] = fmass / (gc * gc) + el;
         S.gap[s] = gap;
-        ++st.strain_active;
+        ++active_count;
         R dS[3][2];
         for (int r = 0; r < 3; ++r)
           for (int c = 0; c < 2; ++c)
@@ -899,6 +903,7 @@ template <class R> struct Sim {
         if (!(std::fabs(d2) < tiny)) { S.hess_on[s] = 1; S.d2V[s] = d2; }
       }
     }
+    st.strain_active = active_count;
   }
 
   // Collision::detect_all_collisions (collision.cpp:478-580).  Faithful: the
@@ -936,9 +941,14 @@ template <class R> struct Sim {
         }
       }
     }
+    // Per-vertex lists filled in parallel, then appended in vertex order, so
+    // the list order is the sequential (body, vertex, triangle) order.
+    std::vector<std::vector<Contact<R>>> per(n);
     for (size_t rb = 0; rb < rigids.size(); ++rb) {
       auto W = rigids[rb].world();
+#pragma omp parallel for schedule(dynamic, 64)
       for (int vi = 0; vi < n; ++vi) {
+        per[vi].clear();
         V3<R> p = X(vi);
         for (const auto& t : rigids[rb].tris) {
           Contact<R> cp;
@@ -946,9 +956,10 @@ template <class R> struct Sim {
           cp.rigid = (int)rb;
           narrow_pt(p, W[t[0]], W[t[1]], W[t[2]], cp.gap, cp.normal, cp.wq);
           cp.wp = p;
-          if (cp.gap < R(0.01)) { cp.vcount = 1; cp.w[0] = R(1); out.push_back(cp); }
+          if (cp.gap < R(0.01)) { cp.vcount = 1; cp.w[0] = R(1); per[vi].push_back(cp); }
         }
       }
+      for (int vi = 0; vi < n; ++vi) out.insert(out.end(), per[vi].begin(), per[vi].end());
     }
   }
 
@@ -994,6 +1005,7 @@ template <class R> struct Sim {
     }
     for (const auto& w : walls) {
       if (!w.active) continue;
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < n; ++i) {
         R kb = wall_stiffness(m[i], w.n, block_of(Hconst, i));
         V3<R> nn = normalize_or_default(w.n);
@@ -1171,6 +1183,7 @@ template <class R> struct Sim {
     int N = 3 * nv;
     iters = 0;
     std::vector<M3<R>> Pc(nv);
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nv; ++i) {
       M3<R> B;
       for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) B.m[r][c] = A[9 * dg[i] + 3 * r + c];
@@ -1178,7 +1191,11 @@ template <class R> struct Sim {
       if ((double)std::fabs(det) > 1e-16) Pc[i] = inv3(B);
       else Pc[i] = M3<R>::identity();
     }
+    // Row-, vertex- and element-wise loops run on OpenMP threads (each
+    // iteration writes its own outputs, so any thread count gives identical
+    // results); the dot products stay sequential in the SSE2 order.
     auto spmv = [&](const std::vector<R>& p, std::vector<R>& out) {
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < nv; ++i)
         for (int r = 0; r < 3; ++r) {
           R tmp = R(0);
@@ -1195,6 +1212,7 @@ template <class R> struct Sim {
       return ri / bi;
     };
     auto precond = [&](const std::vector<R>& r, std::vector<R>& z) {
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < nv; ++i) {
         V3<R> rb(r[3 * i], r[3 * i + 1], r[3 * i + 2]);
         V3<R> zb = mul(Pc[i], rb);
@@ -1215,14 +1233,18 @@ template <class R> struct Sim {
       iters = it + 1;
       if ((double)std::fabs(pAp) < 1e-16) return false;
       R alpha = rz / pAp;
-      for (int i = 0; i < N; ++i) xs[i] = xs[i] + alpha * p[i];
-      for (int i = 0; i < N; ++i) r[i] = r[i] - alpha * Ap[i];
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < N; ++i) {
+        xs[i] = xs[i] + alpha * p[i];
+        r[i] = r[i] - alpha * Ap[i];
+      }
       rel = relres(r);
       if (rel < tol) return true;
       precond(r, z);
       R rzn = vdot(r.data(), z.data(), N);
       R beta = rzn / rz;
       rz = rzn;
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < N; ++i) p[i] = z[i] + beta * p[i];
     }
     return false;
@@ -1256,9 +1278,13 @@ template <class R> struct Sim {
       if (!p.active) continue;
       if (norm3(Xof(xn, p.v) - p.t) < gap_min) return false;
     }
-    if ((double)sqn3(wn) > 0.5)
+    if ((double)sqn3(wn) > 0.5) {
+      bool ok = true;
+#pragma omp parallel for schedule(static) reduction(&& : ok)
       for (int i = 0; i < n; ++i)
-        if (dot3(wn, Xof(xn, i)) - woff < gap_min) return false;
+        if (dot3(wn, Xof(xn, i)) - woff < gap_min) ok = false;
+      if (!ok) return false;
+    }
     return true;
   }
   R ccd_pt(const V3<R>& p0, const V3<R>& p1, const V3<R>& a0, const V3<R>& a1, const V3<R>& b0, const V3<R>& b1,
@@ -1518,6 +1544,10 @@ void orc_destroy(void* h) {
 }
 
 // material: E, nu, density, thickness, bending
+// Threads for the oracle's parallel loops (results do not depend on it).
+void orc_set_threads(int k) { omp_set_num_threads(k > 0 ? k : 1); }
+int orc_get_threads() { return omp_get_max_threads(); }
+
 int orc_set_mesh(void* h, int nv, const double* verts, int nt, const int32_t* tris, const double* mat) {
   OSim* s = (OSim*)h;
   return guard([&] {

@@ -74,8 +74,19 @@ def lib():
         L.orc_line_search.restype = C.c_double
         L.orc_line_search.argtypes = [C.c_void_p, _dp, C.c_double]
         L.orc_set_contacts_pt.argtypes = [C.c_void_p, C.c_int, _ip]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_get_threads.restype = C.c_int
         _lib = L
     return _lib
+
+
+def set_threads(k: int) -> None:
+    """Threads for the oracle's element-wise loops; results do not depend on it."""
+    lib().orc_set_threads(int(k))
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())
 
 
 def _d(a):

@@ -310,3 +310,20 @@ def test_matrix_is_exactly_symmetric_and_spd_calls_diagonal():
     st = o.stats()
     assert st["asym_count"] == 0
     assert st["spd_general_calls"] == before
+
+
+def test_threads_do_not_change_results():
+    """The OpenMP loops (SpMV, preconditioner, vector updates, rigid detection,
+    wall terms, strain faces) write disjoint outputs and the dot products stay
+    sequential, so 1 and 4 threads give bit-identical trajectories."""
+    sc = scenes.small_sheet(9, pins=2, wall=True, rigid=True)
+    out = []
+    for k in (1, 4):
+        orc.set_threads(k)
+        o = orc.OracleSim(sc, 32)
+        for _ in range(6):
+            o.substep()
+        out.append((o.get("x").copy(), o.get("v").copy(), o.stats()["pcg_per_solve"]))
+    orc.set_threads(1)
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]
