@@ -21,6 +21,7 @@
 #include "kernels.hip"
 #include "model.h"
 #include "pcg.hip"
+#include "learned.hip"
 
 namespace ando {
 
@@ -130,6 +131,8 @@ void build_spmv_meta(SpmvMeta& sm, int n, const std::vector<int32_t>& row_ptr, c
   sm.ok = true;
 }
 
+#include "learned_host.hip"
+
 struct ando_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -172,7 +175,7 @@ struct ando_ctx {
   // PCG.
   DBuf<float> r, z, pbuf0, pbuf1, Ap, Pinv;
   DBuf<double> part_rz, part_pAp, rz_hist;
-  DBuf<float> part_max;
+  DBuf<float> part_max, part_bmax;
   DBuf<ando::PcgStatus> pcg_status;
   // Small readbacks.
   DBuf<uint32_t> scalars;  // [0] gradient max bits, [1] line-search mask, [2] strain active count
@@ -187,6 +190,9 @@ struct ando_ctx {
   int spmv_rowsets = 3;    // block rows per lane group of the row SpMV (1..4)
   int spmv_blocks = 0;     // 1: block-parallel SpMV where the pattern allows it, 0: row SpMV
   SpmvMeta spmv_meta;      // for the mesh's block pattern
+  int learned = 0;         // learned preconditioner + warm start in the step's PCG
+  nnh::Net nn;
+  nnh::Work nnw;
   DBuf<uint32_t> pcg_barrier_ctr;
   int pcg_grid_cap = 0;    // co-resident workgroups (0 = not yet queried, -1 = no cooperative launch)
   std::vector<hipEvent_t> events;  // pool: start/stop pairs for timed launches
@@ -491,10 +497,13 @@ PcgOut run_pcg_persistent(ando_ctx* c, int n, const float* A, const int32_t* rp,
   return PcgOut{st.iters, !st.failed};
 }
 
+// P_given / x0_given (device, both or neither): a preconditioner of 9 floats
+// per vertex and a starting guess instead of block-Jacobi from x0 = 0.
 PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int32_t* cols, const int32_t* diag,
-               const float* rhs, int negate, float* xsol, float tol, int maxit, SpmvMeta* sm) {
+               const float* rhs, int negate, float* xsol, float tol, int maxit, SpmvMeta* sm,
+               const float* P_given = nullptr, const float* x0_given = nullptr) {
   hipStream_t s = c->stream;
-  if (c->pcg_persistent) {
+  if (c->pcg_persistent && P_given == nullptr) {
     int G = persistent_grid(c, n);
     if (G > 0) {
       c->stats.pcg_kernel = 1;
@@ -508,9 +517,18 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
   c->rz_hist.ensure((size_t)maxit + 2);
   c->pcg_status.ensure(1);
   c->part_pAp.ensure((size_t)nwg_r + 1);
-  hipLaunchKernelGGL(k_pcg_init, dim3(nwg_v), dim3(kBlock), 0, s, n, A, diag, rhs, xsol, c->r.p, c->z.p,
-                     c->pbuf0.p, c->Pinv.p, c->part_rz.p, c->part_max.p, negate);
-  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, s, nwg_v, c->part_rz.p, c->part_max.p, c->rz_hist.p,
+  const float* Pc = c->Pinv.p;
+  if (P_given) {
+    c->part_bmax.ensure((size_t)nwg_v + 1);
+    hipLaunchKernelGGL(k_pcg_init_given, dim3(nwg_v), dim3(kBlock), 0, s, n, A, rp, cols, P_given, rhs, negate,
+                       x0_given, xsol, c->r.p, c->z.p, c->pbuf0.p, c->part_rz.p, c->part_max.p, c->part_bmax.p);
+    Pc = P_given;
+  } else {
+    hipLaunchKernelGGL(k_pcg_init, dim3(nwg_v), dim3(kBlock), 0, s, n, A, diag, rhs, xsol, c->r.p, c->z.p,
+                       c->pbuf0.p, c->Pinv.p, c->part_rz.p, c->part_max.p, negate);
+  }
+  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, s, nwg_v, (const double*)c->part_rz.p,
+                     (const float*)c->part_max.p, (const float*)(P_given ? c->part_bmax.p : nullptr), c->rz_hist.p,
                      c->pcg_status.p, tol);
   HIPCHK(hipGetLastError());
   PcgStatus st{};
@@ -567,7 +585,7 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
       }
       hipExtLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(kBlock), 0, s, e2, e3, 0, n, it, (const float*)pcur,
                             (const float*)c->Ap.p, (const float*)(blk ? sm->extra.p : nullptr),
-                            (const int32_t*)(blk ? sm->straddle.p : nullptr), (const float*)c->Pinv.p, xsol, c->r.p,
+                            (const int32_t*)(blk ? sm->straddle.p : nullptr), Pc, xsol, c->r.p,
                             c->z.p,
                             (const double*)c->part_pAp.p, nwg_r, (const double*)c->rz_hist.p, c->part_rz.p,
                             c->part_max.p, c->pcg_status.p);
@@ -682,8 +700,28 @@ float inner_newton(ando_ctx* c, const ando_params& p) {
     float gmax = read_gmax(c);
     if (gmax < p.pcg_tol) return 1.0f;
     launch_assemble(c, p, true);
-    PcgOut po = run_pcg(c, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol,
-                        p.pcg_max_iters, &c->spmv_meta);
+    PcgOut po;
+    if (c->learned && c->nn.ready) {
+      // Learned preconditioner and warm start; plain block-Jacobi PCG from
+      // x0 = 0 is the fallback when it misses the tolerance.
+      nnh::forward(c->nn, c->nnw, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, false,
+                   c->stream);
+      nnh::warm_start(c->nnw, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->g.p, 1, c->stream);
+      po = run_pcg(c, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol,
+                   p.pcg_max_iters, &c->spmv_meta, c->learned == 2 ? c->nnw.Dinv.p : c->nnw.P.p, c->nnw.y.p);
+      c->stats.learned_solves++;
+      if (!po.converged) {
+        c->stats.learned_fallbacks++;
+        const int first = po.iters;
+        po = run_pcg(c, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol,
+                     p.pcg_max_iters, &c->spmv_meta);
+        c->stats.fallback_iterations += po.iters;
+        po.iters += first;
+      }
+    } else {
+      po = run_pcg(c, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol,
+                   p.pcg_max_iters, &c->spmv_meta);
+    }
     if (c->stats.pcg_solves < ANDO_MAX_RECORDED_SOLVES) c->stats.pcg_per_solve[c->stats.pcg_solves] = po.iters;
     c->stats.pcg_solves++;
     c->stats.pcg_iterations += po.iters;
@@ -1051,11 +1089,15 @@ int ando_set_walls(ando_ctx* c, const float* normals, const float* offsets, cons
 int ando_set_mode(ando_ctx* c, int32_t flags) {
   return guarded([&] {
     need(c != nullptr, ANDO_EINVAL, "null context");
-    need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING | ANDO_MODE_PCG_PERSISTENT)) == 0,
+    need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING | ANDO_MODE_PCG_PERSISTENT |
+                    ANDO_MODE_LEARNED_PRECOND | ANDO_MODE_LEARNED_WARM_START_ONLY)) == 0,
          ANDO_EINVAL, "unknown mode flag");
+    need(!(flags & ANDO_MODE_LEARNED_PRECOND) || c->nn.ready, ANDO_ESTATE,
+         "learned preconditioner mode needs ando_set_precond_weights first");
     c->fp64_near = (flags & ANDO_MODE_FP64_NEAR_CONTACT) ? 1 : 0;
     c->kernel_timing = (flags & ANDO_MODE_KERNEL_TIMING) ? 1 : 0;
     c->pcg_persistent = (flags & ANDO_MODE_PCG_PERSISTENT) ? 1 : 0;
+    c->learned = (flags & ANDO_MODE_LEARNED_PRECOND) ? ((flags & ANDO_MODE_LEARNED_WARM_START_ONLY) ? 2 : 1) : 0;
   });
 }
 
@@ -1240,6 +1282,224 @@ int ando_pcg(ando_ctx* c, int32_t num_rows, const int32_t* row_ptr, const int32_
     build_spmv_meta(sm, n, std::vector<int32_t>(row_ptr, row_ptr + n + 1), std::vector<int32_t>(cols, cols + nbk), s);
     PcgOut po = run_pcg(c, n, dv.p, drp.p, dci.p, ddg.p, db.p, 0, dx.p, tol, max_iters, &sm);
     HIPCHK(hipMemcpyAsync(x, dx.p, 3 * sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, s));
+    sync(c);
+    if (iterations) *iterations = po.iters;
+    if (converged) *converged = po.converged ? 1 : 0;
+  });
+}
+
+int32_t ando_nn_num_weights(void) { return ando::nn::kNumWeights; }
+
+int ando_set_precond_weights(ando_ctx* c, const float* weights, int32_t count) {
+  return guarded([&] {
+    need(c && weights, ANDO_EINVAL, "null argument");
+    need(count == ando::nn::kNumWeights, ANDO_EINVAL, "weight count differs from ando_nn_num_weights()");
+    HIPCHK(hipSetDevice(c->device));
+    nnh::upload(c->nn, weights, c->stream);
+    sync(c);
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Trainer for the learned preconditioner.
+// ---------------------------------------------------------------------------
+struct ando_trainer {
+  ando_ctx* ctx = nullptr;
+  nnh::Net net;
+  nnh::Work wk;
+  struct System {
+    int n = 0, nb = 0;
+    DBuf<int32_t> rp, cols, diag;
+    DBuf<float> A, b;
+    SpmvMeta sm;
+  };
+  std::vector<System> systems;
+  DBuf<double> loss_hist;
+};
+
+namespace {
+
+void ensure_pcg_buffers(ando_ctx* c, int n) {
+  const size_t n3 = 3 * (size_t)n;
+  c->r.ensure(n3); c->z.ensure(n3); c->pbuf0.ensure(n3); c->pbuf1.ensure(n3); c->Ap.ensure(n3);
+  c->Pinv.ensure(9 * (size_t)n);
+  c->part_rz.ensure(nblk(n) + 1); c->part_max.ensure(nblk(n) + 1); c->part_pAp.ensure(spmv_groups(n) + 1);
+}
+
+ando_trainer::System& trainer_system(ando_trainer* tr, int32_t i) {
+  need(tr != nullptr, ANDO_EINVAL, "null trainer");
+  need(i >= 0 && i < (int32_t)tr->systems.size(), ANDO_EINVAL, "system index out of range");
+  need(tr->net.ready, ANDO_ESTATE, "trainer has no weights (ando_trainer_init_weights or _set_weights)");
+  return tr->systems[i];
+}
+
+}  // namespace
+
+extern "C" {
+
+int ando_trainer_create(int32_t device, ando_trainer** out) {
+  return guarded([&] {
+    need(out != nullptr, ANDO_EINVAL, "out is null");
+    ando_ctx* c = nullptr;
+    int rc = ando_create(device, &c);
+    if (rc != ANDO_OK) throw Fail(rc, ando::g_last_error);
+    auto* tr = new ando_trainer;
+    tr->ctx = c;
+    *out = tr;
+  });
+}
+
+int ando_trainer_destroy(ando_trainer* tr) {
+  if (!tr) return ANDO_OK;
+  ando_ctx* c = tr->ctx;
+  (void)hipSetDevice(c->device);
+  delete tr;
+  return ando_destroy(c);
+}
+
+int ando_trainer_add_system(ando_trainer* tr, int32_t num_rows, const int32_t* row_ptr, const int32_t* cols,
+                            const float* values, const float* b, int32_t* index) {
+  return guarded([&] {
+    need(tr && row_ptr && cols && values && b, ANDO_EINVAL, "null argument");
+    need(num_rows > 0, ANDO_EINVAL, "bad sizes");
+    const int n = num_rows, nbk = row_ptr[n];
+    std::vector<int32_t> dg(n, -1);
+    for (int i = 0; i < n; ++i) {
+      need(row_ptr[i + 1] >= row_ptr[i], ANDO_EINVAL, "row_ptr not monotone");
+      for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+        need(cols[k] >= 0 && cols[k] < n, ANDO_EINVAL, "column index out of range");
+        if (cols[k] == i) dg[i] = k;
+      }
+      need(dg[i] >= 0, ANDO_EINVAL, "every row needs its diagonal block");
+    }
+    ando_ctx* c = tr->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    tr->systems.emplace_back();
+    auto& S = tr->systems.back();
+    S.n = n;
+    S.nb = nbk;
+    S.rp.upload(row_ptr, n + 1, s);
+    S.cols.upload(cols, nbk, s);
+    S.diag.upload(dg, s);
+    S.A.upload(values, 9 * (size_t)nbk, s);
+    S.b.upload(b, 3 * (size_t)n, s);
+    build_spmv_meta(S.sm, n, std::vector<int32_t>(row_ptr, row_ptr + n + 1), std::vector<int32_t>(cols, cols + nbk),
+                    s);
+    sync(c);
+    if (index) *index = (int32_t)tr->systems.size() - 1;
+  });
+}
+
+int ando_trainer_init_weights(ando_trainer* tr, uint64_t seed) {
+  return guarded([&] {
+    need(tr != nullptr, ANDO_EINVAL, "null trainer");
+    HIPCHK(hipSetDevice(tr->ctx->device));
+    auto w = nnh::init_weights(seed);
+    nnh::upload(tr->net, w.data(), tr->ctx->stream);
+    sync(tr->ctx);
+  });
+}
+
+int ando_trainer_set_weights(ando_trainer* tr, const float* weights, int32_t count) {
+  return guarded([&] {
+    need(tr && weights, ANDO_EINVAL, "null argument");
+    need(count == ando::nn::kNumWeights, ANDO_EINVAL, "weight count differs from ando_nn_num_weights()");
+    HIPCHK(hipSetDevice(tr->ctx->device));
+    nnh::upload(tr->net, weights, tr->ctx->stream);
+    sync(tr->ctx);
+  });
+}
+
+int ando_trainer_get_weights(ando_trainer* tr, float* weights, int32_t count) {
+  return guarded([&] {
+    need(tr && weights, ANDO_EINVAL, "null argument");
+    need(count == ando::nn::kNumWeights, ANDO_EINVAL, "weight count differs from ando_nn_num_weights()");
+    need(tr->net.ready, ANDO_ESTATE, "trainer has no weights");
+    HIPCHK(hipSetDevice(tr->ctx->device));
+    HIPCHK(hipMemcpyAsync(weights, tr->net.w.p, sizeof(float) * count, hipMemcpyDeviceToHost, tr->ctx->stream));
+    sync(tr->ctx);
+  });
+}
+
+int ando_trainer_forward(ando_trainer* tr, int32_t system, float* P, float* y) {
+  return guarded([&] {
+    auto& S = trainer_system(tr, system);
+    ando_ctx* c = tr->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    nnh::forward(tr->net, tr->wk, S.n, S.A.p, S.rp.p, S.cols.p, S.diag.p, S.b.p, 0, false, c->stream);
+    if (P) HIPCHK(hipMemcpyAsync(P, tr->wk.P.p, sizeof(float) * 9 * S.n, hipMemcpyDeviceToHost, c->stream));
+    if (y) HIPCHK(hipMemcpyAsync(y, tr->wk.y.p, sizeof(float) * 3 * S.n, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+  });
+}
+
+int ando_trainer_loss_grad(ando_trainer* tr, int32_t system, double* loss, float* grad) {
+  return guarded([&] {
+    auto& S = trainer_system(tr, system);
+    ando_ctx* c = tr->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    nnh::forward(tr->net, tr->wk, S.n, S.A.p, S.rp.p, S.cols.p, S.diag.p, S.b.p, 0, true, c->stream);
+    nnh::backward(tr->net, tr->wk, S.n, S.A.p, S.rp.p, S.cols.p, S.b.p, 0, c->stream);
+    double fin[3] = {0, 0, 0};
+    HIPCHK(hipMemcpyAsync(fin, tr->wk.lfin.p, sizeof(fin), hipMemcpyDeviceToHost, c->stream));
+    if (grad)
+      HIPCHK(hipMemcpyAsync(grad, tr->net.grad.p, sizeof(float) * ando::nn::kNumWeights, hipMemcpyDeviceToHost,
+                            c->stream));
+    sync(c);
+    if (loss) *loss = fin[1];
+  });
+}
+
+int ando_trainer_train(ando_trainer* tr, int32_t steps, float lr, uint64_t seed, double* losses) {
+  return guarded([&] {
+    need(tr != nullptr, ANDO_EINVAL, "null trainer");
+    need(steps >= 0 && lr > 0.0f, ANDO_EINVAL, "bad steps or learning rate");
+    need(!tr->systems.empty(), ANDO_ESTATE, "trainer has no systems");
+    need(tr->net.ready, ANDO_ESTATE, "trainer has no weights");
+    ando_ctx* c = tr->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    tr->loss_hist.ensure((size_t)steps + 1);
+    uint64_t st = seed;
+    for (int t = 0; t < steps; ++t) {
+      uint64_t z = (st += 0x9E3779B97F4A7C15ull);
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      auto& S = tr->systems[z % tr->systems.size()];
+      nnh::forward(tr->net, tr->wk, S.n, S.A.p, S.rp.p, S.cols.p, S.diag.p, S.b.p, 0, true, s);
+      nnh::backward(tr->net, tr->wk, S.n, S.A.p, S.rp.p, S.cols.p, S.b.p, 0, s);
+      HIPCHK(hipMemcpyAsync(tr->loss_hist.p + t, tr->wk.lfin.p + 1, sizeof(double), hipMemcpyDeviceToDevice, s));
+      nnh::adam(tr->net, lr, s);
+    }
+    if (losses && steps > 0)
+      HIPCHK(hipMemcpyAsync(losses, tr->loss_hist.p, sizeof(double) * steps, hipMemcpyDeviceToHost, s));
+    sync(c);
+  });
+}
+
+int ando_trainer_solve(ando_trainer* tr, int32_t system, int32_t learned, float tol, int32_t max_iters,
+                       int32_t* iterations, int32_t* converged) {
+  return guarded([&] {
+    auto& S = trainer_system(tr, system);
+    need(max_iters >= 0, ANDO_EINVAL, "bad max_iters");
+    ando_ctx* c = tr->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    ensure_pcg_buffers(c, S.n);
+    DBuf<float> x;
+    x.ensure(3 * (size_t)S.n);
+    PcgOut po;
+    if (learned) {
+      nnh::forward(tr->net, tr->wk, S.n, S.A.p, S.rp.p, S.cols.p, S.diag.p, S.b.p, 0, false, c->stream);
+      nnh::warm_start(tr->wk, S.n, S.A.p, S.rp.p, S.cols.p, S.b.p, 0, c->stream);
+      po = run_pcg(c, S.n, S.A.p, S.rp.p, S.cols.p, S.diag.p, S.b.p, 0, x.p, tol, max_iters, &S.sm,
+                   learned == 2 ? tr->wk.Dinv.p : tr->wk.P.p, tr->wk.y.p);
+    } else {
+      po = run_pcg(c, S.n, S.A.p, S.rp.p, S.cols.p, S.diag.p, S.b.p, 0, x.p, tol, max_iters, &S.sm);
+    }
     sync(c);
     if (iterations) *iterations = po.iters;
     if (converged) *converged = po.converged ? 1 : 0;

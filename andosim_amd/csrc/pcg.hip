@@ -123,15 +123,70 @@ __global__ void k_pcg_init(int n, const float* __restrict__ A, const int32_t* __
   }
 }
 
+// With a starting guess, part_bmax holds the max|b| partials and part_max the
+// max|r0| ones (pcg_solver.cpp:16-21 tests the initial residual); without
+// one (part_bmax == nullptr) r0 = b.
+// Set-up with a given preconditioner P (9 floats per vertex) and starting
+// guess x0: r = b - A x0, x = x0, z = P r, p = z, partial r.z, max|r| and
+// max|b| (the learned preconditioner's warm start).
+__global__ void k_pcg_init_given(int n, const float* __restrict__ A, const int32_t* __restrict__ rp,
+                                 const int32_t* __restrict__ cols, const float* __restrict__ P,
+                                 const float* __restrict__ g, int negate, const float* __restrict__ x0,
+                                 float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
+                                 float* __restrict__ p, double* __restrict__ part_rz, float* __restrict__ part_max,
+                                 float* __restrict__ part_bmax) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double rz = 0.0;
+  float rm = 0.0f, bm = 0.0f;
+  if (i < n) {
+    float ax[3] = {0.0f, 0.0f, 0.0f};
+    for (int b = rp[i]; b < rp[i + 1]; ++b) {
+      const int j = cols[b];
+      const float* B = A + 9 * (size_t)b;
+      for (int k = 0; k < 3; ++k)
+        ax[k] = ax[k] + (B[3 * k] * x0[3 * j] + (B[3 * k + 1] * x0[3 * j + 1] + B[3 * k + 2] * x0[3 * j + 2]));
+    }
+    float rv[3];
+    for (int k = 0; k < 3; ++k) {
+      const float bk = negate ? -g[3 * i + k] : g[3 * i + k];
+      rv[k] = bk - ax[k];
+      r[3 * i + k] = rv[k];
+      x[3 * i + k] = x0[3 * i + k];
+      rm = fmaxf(rm, fabsf(rv[k]));
+      bm = fmaxf(bm, fabsf(bk));
+    }
+    const float* Pi = P + 9 * (size_t)i;
+    for (int k = 0; k < 3; ++k) {
+      float zk = Pi[3 * k] * rv[0] + (Pi[3 * k + 1] * rv[1] + Pi[3 * k + 2] * rv[2]);
+      z[3 * i + k] = zk;
+      p[3 * i + k] = zk;
+      rz += (double)rv[k] * (double)zk;
+    }
+    if (rm != rm) rm = __int_as_float(0x7f800000);
+  }
+  __shared__ double ldsd[4];
+  __shared__ float ldsf[4];
+  double s = block_reduce(rz, ldsd, [](double a, double b) { return a + b; });
+  float mx = block_reduce(rm, ldsf, [](float a, float b) { return fmaxf(a, b); });
+  float mb = block_reduce(bm, ldsf, [](float a, float b) { return fmaxf(a, b); });
+  if (threadIdx.x == 0) {
+    part_rz[blockIdx.x] = s;
+    part_max[blockIdx.x] = mx;
+    part_bmax[blockIdx.x] = mb;
+  }
+}
+
 __global__ void k_pcg_start(int nwg, const double* __restrict__ part_rz, const float* __restrict__ part_max,
-                            double* __restrict__ rz_hist, PcgStatus* __restrict__ st, float tol) {
+                            const float* __restrict__ part_bmax, double* __restrict__ rz_hist,
+                            PcgStatus* __restrict__ st, float tol) {
   __shared__ double ldsd[4];
   __shared__ float ldsf[4];
   double rz = reduce_sum_partials(part_rz, nwg, ldsd);
-  float bmax = reduce_max_partials(part_max, nwg, ldsf);
+  float rmax = reduce_max_partials(part_max, nwg, ldsf);
+  float bmax = part_bmax ? reduce_max_partials(part_bmax, nwg, ldsf) : rmax;
   if (threadIdx.x == 0) {
     rz_hist[0] = rz;
-    float rel = ((double)bmax < 1e-16) ? bmax : bmax / bmax;
+    float rel = ((double)bmax < 1e-16) ? rmax : rmax / bmax;
     st->bmax = bmax;
     st->rel = rel;
     st->failed = 0;

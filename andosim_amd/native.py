@@ -21,6 +21,9 @@ ABI_SYMBOLS = [
     "ando_set_state", "ando_get_state", "ando_set_pins", "ando_set_walls", "ando_set_mode", "ando_apply_gravity",
     "ando_step",
     "ando_get_stats", "ando_compute_contacts", "ando_get_pattern", "ando_linearize", "ando_pcg",
+    "ando_nn_num_weights", "ando_set_precond_weights", "ando_trainer_create", "ando_trainer_destroy",
+    "ando_trainer_add_system", "ando_trainer_init_weights", "ando_trainer_set_weights", "ando_trainer_get_weights",
+    "ando_trainer_forward", "ando_trainer_loss_grad", "ando_trainer_train", "ando_trainer_solve",
 ]
 
 
@@ -64,7 +67,8 @@ class Stats(C.Structure):
                 ("rigid_contacts", C.c_int32), ("strain_active", C.c_int32), ("beta", C.c_float),
                 ("last_alpha", C.c_float), ("pcg_per_solve", C.c_int32 * MAX_RECORDED), ("step_ms", C.c_double),
                 ("timed_launches", C.c_int32 * 2), ("timed_ms", C.c_double * 2), ("timed_iterations", C.c_int32),
-                ("pcg_kernel", C.c_int32)]
+                ("pcg_kernel", C.c_int32), ("learned_solves", C.c_int32), ("learned_fallbacks", C.c_int32),
+                ("fallback_iterations", C.c_int32)]
 
 
 _lib = None
@@ -100,6 +104,19 @@ def load():
     L.ando_get_pattern.argtypes = [C.c_void_p, ip, ip, ip]
     L.ando_linearize.argtypes = [C.c_void_p, C.POINTER(Params), fp, C.POINTER(RigidBody), C.c_int32, fp, fp]
     L.ando_pcg.argtypes = [C.c_void_p, C.c_int32, ip, ip, fp, fp, fp, C.c_float, C.c_int32, ip, ip]
+    dp = C.POINTER(C.c_double)
+    L.ando_nn_num_weights.restype = C.c_int32
+    L.ando_set_precond_weights.argtypes = [C.c_void_p, fp, C.c_int32]
+    L.ando_trainer_create.argtypes = [C.c_int32, C.POINTER(C.c_void_p)]
+    L.ando_trainer_destroy.argtypes = [C.c_void_p]
+    L.ando_trainer_add_system.argtypes = [C.c_void_p, C.c_int32, ip, ip, fp, fp, ip]
+    L.ando_trainer_init_weights.argtypes = [C.c_void_p, C.c_uint64]
+    L.ando_trainer_set_weights.argtypes = [C.c_void_p, fp, C.c_int32]
+    L.ando_trainer_get_weights.argtypes = [C.c_void_p, fp, C.c_int32]
+    L.ando_trainer_forward.argtypes = [C.c_void_p, C.c_int32, fp, fp]
+    L.ando_trainer_loss_grad.argtypes = [C.c_void_p, C.c_int32, dp, fp]
+    L.ando_trainer_train.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.c_uint64, dp]
+    L.ando_trainer_solve.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_int32, ip, ip]
     _lib = L
     return L
 
@@ -265,8 +282,10 @@ class GpuSim:
         self.gravity = _f(sc.gravity)
         self.dt = float(sc.params["dt"])
 
-    def set_mode(self, fp64_near: bool = True, kernel_timing: bool = False, pcg_persistent: bool = False):
-        flags = (1 if fp64_near else 0) | (2 if kernel_timing else 0) | (4 if pcg_persistent else 0)
+    def set_mode(self, fp64_near: bool = True, kernel_timing: bool = False, pcg_persistent: bool = False,
+                 learned: bool = False, warm_start_only: bool = False):
+        flags = ((1 if fp64_near else 0) | (2 if kernel_timing else 0) | (4 if pcg_persistent else 0) |
+                 (8 if learned else 0) | (16 if learned and warm_start_only else 0))
         check(self.L.ando_set_mode(self.h, flags))
 
     def apply_gravity(self, g=None, dt=None):
@@ -328,6 +347,10 @@ class GpuSim:
         check(self.L.ando_compute_contacts(self.h, self.bodies, self.nbodies, buf, cap, C.byref(cnt)))
         return [buf[i] for i in range(min(cnt.value, cap))], cnt.value
 
+    def set_precond_weights(self, w):
+        w = _f(w).ravel()
+        check(self.L.ando_set_precond_weights(self.h, _fp(w), len(w)))
+
     def pcg(self, rp, ci, vals, b, tol=1e-6, maxit=1000):
         n = len(rp) - 1
         x = np.zeros(3 * n, np.float32)
@@ -337,3 +360,71 @@ class GpuSim:
         check(self.L.ando_pcg(self.h, n, _ip(rp_), _ip(ci_), _fp(v_), _fp(b_), _fp(x), tol, maxit, C.byref(it),
                               C.byref(conv)))
         return x, it.value, bool(conv.value)
+
+
+def nn_num_weights() -> int:
+    return int(load().ando_nn_num_weights())
+
+
+class Trainer:
+    """Learned-preconditioner trainer on one device (ando_trainer_*)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        h = C.c_void_p()
+        check(self.L.ando_trainer_create(device, C.byref(h)))
+        self.h = h
+        self.nw = nn_num_weights()
+        self.sizes = []
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and self.L is not None:
+            self.L.ando_trainer_destroy(self.h)
+            self.h = None
+
+    def add_system(self, rp, ci, vals, b) -> int:
+        rp_, ci_, v_, b_ = _i(rp), _i(ci), _f(vals).ravel(), _f(b).ravel()
+        idx = C.c_int32(0)
+        check(self.L.ando_trainer_add_system(self.h, len(rp_) - 1, _ip(rp_), _ip(ci_), _fp(v_), _fp(b_),
+                                             C.byref(idx)))
+        self.sizes.append(len(rp_) - 1)
+        return idx.value
+
+    def init_weights(self, seed: int = 1):
+        check(self.L.ando_trainer_init_weights(self.h, C.c_uint64(seed)))
+
+    def set_weights(self, w):
+        w = _f(w).ravel()
+        check(self.L.ando_trainer_set_weights(self.h, _fp(w), len(w)))
+
+    def get_weights(self) -> np.ndarray:
+        w = np.zeros(self.nw, np.float32)
+        check(self.L.ando_trainer_get_weights(self.h, _fp(w), self.nw))
+        return w
+
+    def forward(self, i: int):
+        n = self.sizes[i]
+        P = np.zeros(9 * n, np.float32)
+        y = np.zeros(3 * n, np.float32)
+        check(self.L.ando_trainer_forward(self.h, i, _fp(P), _fp(y)))
+        return P.reshape(n, 3, 3), y.reshape(n, 3)
+
+    def loss_grad(self, i: int):
+        loss = C.c_double(0.0)
+        g = np.zeros(self.nw, np.float32)
+        check(self.L.ando_trainer_loss_grad(self.h, i, C.byref(loss), _fp(g)))
+        return loss.value, g
+
+    def train(self, steps: int, lr: float = 1e-3, seed: int = 7) -> np.ndarray:
+        out = np.zeros(max(steps, 1), np.float64)
+        check(self.L.ando_trainer_train(self.h, steps, C.c_float(lr), C.c_uint64(seed),
+                                        out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out[:steps]
+
+    def solve(self, i: int, learned, tol: float = 1e-6, maxit: int = 1000):
+        """learned: False/0 plain, True/1 learned preconditioner + warm start, 2 warm start only."""
+        it = C.c_int32(0)
+        conv = C.c_int32(0)
+        check(self.L.ando_trainer_solve(self.h, i, int(learned), C.c_float(tol), maxit, C.byref(it),
+                                        C.byref(conv)))
+        return it.value, bool(conv.value)

@@ -77,6 +77,75 @@ def measured_traffic(workload: str, n: int, nb: int, kernel_prefix: str):
     return None
 
 
+def learned_comparison(sim, sc, window, restore, replay, a, pers):
+    """Learned preconditioner vs plain block-Jacobi PCG on the same window.
+
+    The network is trained here, on the GPU, on Newton systems of the batched
+    workload's sheets (never on this workload).  Two comparisons:
+      * the first Newton system of every window step (taken from the plain
+        run), solved by plain PCG and by the learned preconditioner with its
+        warm start (no fallback);
+      * the window replayed with ANDO_MODE_LEARNED_PRECOND (learned solves,
+        plain fallback when one misses the tolerance), PCG iterations per step
+        and steps/s, with the learned preconditioner and with the warm start
+        only (block-Jacobi preconditioner).
+    """
+    from andosim_amd import learned_precond, native
+
+    t0 = time.perf_counter()
+    systems = learned_precond.training_systems()
+    w, losses = learned_precond.train(systems, steps=a.train_steps)
+    t_train = time.perf_counter() - t0
+    # First Newton systems of the window steps, from the plain run.
+    restore()
+    dt = np.float32(sc.params["dt"])
+    rp, ci = sim.pattern()
+    firsts = []
+    for _ in window:
+        x, v = sim.get("x"), sim.get("v")
+        g, H = sim.linearize((x + dt * v).astype(np.float32))
+        firsts.append((rp, ci, H, (-g).astype(np.float32)))
+        sim.substep()
+    table = learned_precond.compare(w, firsts, tol=float(sc.params["pcg_tol"]),
+                                    maxit=int(sc.params["pcg_max_iters"]))
+    # Whole steps in each learned mode.
+    sim.set_precond_weights(w)
+
+    def steps(warm_only):
+        sim.set_mode(fp64_near=True, kernel_timing=False, pcg_persistent=pers, learned=True,
+                     warm_start_only=warm_only)
+        restore()
+        recs, nonfinite = [], None
+        for k in window:
+            t1 = time.perf_counter()
+            sim.substep()
+            recs.append((time.perf_counter() - t1, sim.stats()))
+            if not finite_state(sim):
+                nonfinite = k
+                break
+        sim.set_mode(fp64_near=True, kernel_timing=False, pcg_persistent=pers, learned=False)
+        el = sum(r[0] for r in recs)
+        return {"pcg_iters_per_step": round(sum(r[1]["pcg_iterations"] for r in recs) / len(recs), 2),
+                "steps_per_s": round(len(recs) / el, 3), "nonfinite_at_step": nonfinite,
+                "solves": sum(r[1]["learned_solves"] for r in recs),
+                "fallbacks": sum(r[1]["learned_fallbacks"] for r in recs),
+                "fallback_iters_per_step": round(sum(r[1]["fallback_iterations"] for r in recs) / len(recs), 2)}
+
+    return {
+        "precond_and_warm_start": steps(False),
+        "warm_start_only": steps(True),
+        "first_newton_system": {"plain_iters": [t[0] for t in table], "plain_converged": [t[1] for t in table],
+                                "learned_iters": [t[2] for t in table], "learned_converged": [t[3] for t in table],
+                                "warm_start_only_iters": [t[4] for t in table],
+                                "warm_start_only_converged": [t[5] for t in table]},
+        "training": {"systems": len(systems), "steps": a.train_steps, "seconds": round(t_train, 1),
+                     "loss_first": round(float(np.mean(losses[:200])), 4),
+                     "loss_last": round(float(np.mean(losses[-200:])), 4),
+                     "data": "batched workload sheets (side U{17..65}, 1-4 pins, wind), substeps 20..100, "
+                             "and 4 sheets draped over a sphere"},
+    }
+
+
 def finite_state(sim) -> bool:
     x = sim.get("x")
     v = sim.get("v")
@@ -94,6 +163,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip the kernel-event pass")
     ap.add_argument("--pcg", choices=["persistent", "two-kernel"], default="two-kernel")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner comparison")
+    ap.add_argument("--train-steps", type=int, default=8000)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU design is described "
@@ -212,6 +283,10 @@ def main(argv=None):
             roof["pcg_kernels_share_of_step"] = round((t_launch[0] + t_launch[1]) * 1e-3 / ev_elapsed, 3)
         roof["event_pass_ms_per_step"] = round(1e3 * ev_elapsed / len(ev_recs), 3)
 
+    learned = None
+    if not a.no_learned:
+        learned = learned_comparison(sim, sc, window, restore, replay, a, pers)
+
     cpu = cpu_single = None
     if not a.no_cpu:
         from oracle import oracle as orc
@@ -265,6 +340,7 @@ def main(argv=None):
                    "first_nonfinite_step": last_ok + 1 if last_ok + 1 < a.max_explore else None},
         "pcg_iters_per_step": round(pcg_iters / len(recs), 2),
         "newton_iters_per_step": round(newton / len(recs), 2),
+        "learned_precond": learned,
         "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu_single,
     }
     print(json.dumps(out), flush=True)

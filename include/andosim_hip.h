@@ -134,7 +134,11 @@ typedef struct {
   int32_t timed_launches[2];        /* ANDO_MODE_KERNEL_TIMING: launches that ran (not skipped) */
   double timed_ms[2];               /* summed event durations of those launches */
   int32_t timed_iterations;         /* PCG iterations covered by timed persistent launches */
-  int32_t pcg_kernel;               /* 1: persistent single-launch PCG, 0: two kernels per iteration */
+  int32_t pcg_kernel;               /* 0: two kernels per iteration (row SpMV), 1: persistent
+                                       single launch, 2: two kernels with the block-parallel SpMV */
+  int32_t learned_solves;           /* solves that used the learned preconditioner */
+  int32_t learned_fallbacks;        /* of those, solves that fell back to plain PCG */
+  int32_t fallback_iterations;      /* plain PCG iterations spent in fallbacks (included in pcg_iterations) */
 } ando_stats;
 
 int32_t ando_abi_version(void);
@@ -174,6 +178,14 @@ int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, co
  * grid barriers instead of the default two kernels per iteration (host check
  * every few iterations); same arithmetic. */
 #define ANDO_MODE_PCG_PERSISTENT 4
+/* ANDO_MODE_LEARNED_PRECOND (needs ando_set_precond_weights) solves each
+ * Newton system with the learned block preconditioner and the warm start it
+ * predicts; a solve that misses the tolerance is redone with plain
+ * block-Jacobi PCG from x0 = 0 (counted in ando_stats.learned_fallbacks). */
+#define ANDO_MODE_LEARNED_PRECOND 8
+/* With ANDO_MODE_LEARNED_PRECOND, ANDO_MODE_LEARNED_WARM_START_ONLY keeps the
+ * block-Jacobi preconditioner and uses the network only for the warm start. */
+#define ANDO_MODE_LEARNED_WARM_START_ONLY 16
 int ando_set_mode(ando_ctx* ctx, int32_t flags);
 
 int ando_apply_gravity(ando_ctx* ctx, const float* gravity3, float dt);
@@ -188,6 +200,42 @@ int ando_linearize(ando_ctx* ctx, const ando_params* params, const float* x_targ
                    int32_t num_bodies, float* gradient, float* values);
 int ando_pcg(ando_ctx* ctx, int32_t num_rows, const int32_t* row_ptr, const int32_t* cols, const float* values,
              const float* b, float* x, float tol, int32_t max_iters, int32_t* iterations, int32_t* converged);
+
+/* ---- Learned preconditioner (new; no counterpart in the reference, whose PCG
+ * is block-Jacobi from x0 = 0: pcg_solver.cpp:8-75, integrator.cpp:187). ----
+ * A mesh message-passing network (4 layers, width 64, BF16 weights, FP32
+ * accumulation) maps each Newton system to per-vertex 3x3 preconditioner
+ * blocks P_i and the warm start x0 = alpha P b (alpha minimises the A-norm
+ * error along P b).  Weights are a flat float array of
+ * ando_nn_num_weights() values. */
+int32_t ando_nn_num_weights(void);
+int ando_set_precond_weights(ando_ctx* ctx, const float* weights, int32_t count);
+
+/* Trainer: holds Newton systems (BSR, b) on one device and trains the network
+ * on them with the library's own forward and backward kernels and Adam. */
+typedef struct ando_trainer ando_trainer;
+int ando_trainer_create(int32_t device, ando_trainer** out);
+int ando_trainer_destroy(ando_trainer* tr);
+/* Adds the system A x = b (same BSR layout as ando_pcg); returns its index. */
+int ando_trainer_add_system(ando_trainer* tr, int32_t num_rows, const int32_t* row_ptr, const int32_t* cols,
+                            const float* values, const float* b, int32_t* index);
+int ando_trainer_init_weights(ando_trainer* tr, uint64_t seed);
+int ando_trainer_set_weights(ando_trainer* tr, const float* weights, int32_t count);
+int ando_trainer_get_weights(ando_trainer* tr, float* weights, int32_t count);
+/* Forward pass on one system: P (9 per row) and y = P b (the warm start is
+ * alpha y with alpha = b^T y / y^T A y). */
+int ando_trainer_forward(ando_trainer* tr, int32_t system, float* P, float* y);
+/* Loss L = -(b^T y)^2 / ((y^T A y)(b^T D^-1 b)) (the A-norm error left by
+ * the warm start, constant terms dropped) and its gradient (nullable). */
+int ando_trainer_loss_grad(ando_trainer* tr, int32_t system, double* loss, float* grad);
+/* steps Adam updates on systems drawn by a SplitMix64 stream from seed;
+ * losses (nullable) receives the loss of each step's system before its update. */
+int ando_trainer_train(ando_trainer* tr, int32_t steps, float lr, uint64_t seed, double* losses);
+/* PCG on one system: learned = 0 plain block-Jacobi from x0 = 0, 1 learned
+ * preconditioner and warm start, 2 block-Jacobi from the learned warm start
+ * (no fallback). */
+int ando_trainer_solve(ando_trainer* tr, int32_t system, int32_t learned, float tol, int32_t max_iters,
+                       int32_t* iterations, int32_t* converged);
 
 #ifdef __cplusplus
 }
