@@ -508,6 +508,16 @@ template <class R> struct Sim {
   std::vector<StrainFace<R>> sf;
   std::vector<R> H;        // system matrix values (last assembly)
   std::vector<Contact<R>> contacts;
+  // Contact mode: 0 faithful (self broad phase empty, F1), 1 corrected (every
+  // vertex/triangle and edge/edge pair tested by brute force with the
+  // reference's intended predicate; PT and EE barriers both enter the energy).
+  int contact_mode = 0;
+  R pt_box = R(1e-4), keep_gap = R(0.01);
+  // Contact Hessian blocks that couple two different vertices.  The reference
+  // folds them into a pattern built from the triplets; here the pattern is the
+  // mesh's, so they are kept beside it and applied in the PCG product.
+  struct Coupling { int i, j; R b[9]; };
+  std::vector<Coupling> couplings;
   bool assembled_symmetric = true;
   long asym_count = 0;
 
@@ -908,9 +918,65 @@ template <class R> struct Sim {
 
   // Collision::detect_all_collisions (collision.cpp:478-580).  Faithful: the
   // self BVH yields no primitive leaves unless the mesh has one triangle (F1).
+  // Corrected self detection (collision.cpp:198-276, 478-534 as intended):
+  // vertex v against every triangle not incident to v whose box overlaps
+  // [p - pt_box, p + pt_box] (closed intervals); edges e1 < e2 sharing no
+  // vertex whose boxes overlap; narrow phase, kept if gap < keep_gap.
+  void detect_self_corrected(std::vector<Contact<R>>& out) const {
+    const R eb = pt_box;
+    for (int vi = 0; vi < n; ++vi) {
+      V3<R> p = X(vi);
+      for (size_t t = 0; t < tris.size(); ++t) {
+        const auto& tr = tris[t];
+        if (tr[0] == vi || tr[1] == vi || tr[2] == vi) continue;
+        V3<R> a = X(tr[0]), b = X(tr[1]), c = X(tr[2]);
+        bool ov = true;
+        for (int k = 0; k < 3; ++k) {
+          R mn = std::min(std::min(a[k], b[k]), c[k]), mx = std::max(std::max(a[k], b[k]), c[k]);
+          ov = ov && (p[k] - eb <= mx && p[k] + eb >= mn);
+        }
+        if (!ov) continue;
+        Contact<R> cp;
+        cp.type = PT; cp.idx[0] = vi; cp.idx[1] = tr[0]; cp.idx[2] = tr[1]; cp.idx[3] = tr[2];
+        narrow_pt(p, a, b, c, cp.gap, cp.normal, cp.wq);
+        cp.wp = p;
+        if (cp.gap < keep_gap) {
+          cp.bary = triangle_barycentric(cp.wq, a, b, c);
+          cp.w[0] = R(1); cp.w[1] = -cp.bary[0]; cp.w[2] = -cp.bary[1]; cp.w[3] = -cp.bary[2];
+          cp.vcount = 4;
+          out.push_back(cp);
+        }
+      }
+    }
+    for (size_t e1 = 0; e1 < edges.size(); ++e1)
+      for (size_t e2 = e1 + 1; e2 < edges.size(); ++e2) {
+        const auto& A = edges[e1];
+        const auto& B = edges[e2];
+        if (A[0] == B[0] || A[0] == B[1] || A[1] == B[0] || A[1] == B[1]) continue;
+        V3<R> p0 = X(A[0]), p1 = X(A[1]), q0 = X(B[0]), q1 = X(B[1]);
+        bool ov = true;
+        for (int k = 0; k < 3; ++k) {
+          R amn = std::min(p0[k], p1[k]), amx = std::max(p0[k], p1[k]);
+          R bmn = std::min(q0[k], q1[k]), bmx = std::max(q0[k], q1[k]);
+          ov = ov && (amn <= bmx && amx >= bmn);
+        }
+        if (!ov) continue;
+        Contact<R> cp;
+        cp.type = EE; cp.idx[0] = A[0]; cp.idx[1] = A[1]; cp.idx[2] = B[0]; cp.idx[3] = B[1];
+        narrow_ee(p0, p1, q0, q1, cp.gap, cp.normal, cp.wp, cp.wq);
+        if (cp.gap < keep_gap) {
+          cp.vcount = 4;
+          cp.w[0] = R(0.5); cp.w[1] = R(0.5); cp.w[2] = R(-0.5); cp.w[3] = R(-0.5);
+          out.push_back(cp);
+        }
+      }
+  }
+
   void detect(std::vector<Contact<R>>& out) {
     out.clear();
-    if (tris.size() == 1) {
+    if (contact_mode == 1) {
+      detect_self_corrected(out);
+    } else if (tris.size() == 1) {
       // Single-leaf BVH: vertex boxes (±1e-4) against the one triangle box,
       // incident vertices skipped (collision.cpp:198-244, 279-387).
       const auto& tr = tris[0];
@@ -979,7 +1045,7 @@ template <class R> struct Sim {
           for (int a = 0; a < 3; ++a)
             for (int k = 0; k < 3; ++k) g[tris[f][a] * 3 + k] += sf[f].gradc[s][a][k];
     for (const auto& c : contacts) {
-      if (c.type != PT && c.type != RIGID_PT) continue;
+      if (c.type != PT && c.type != RIGID_PT && !(c.type == EE && contact_mode == 1)) continue;
       R kb = contact_stiffness(c);
       if (!in_domain(c.gap, P.gap_max)) continue;
       V3<R> nn = normalize_or_default(c.normal);
@@ -1087,8 +1153,9 @@ template <class R> struct Sim {
                 }
             }
         }
+    couplings.clear();
     for (const auto& c : contacts) {
-      if (c.type != PT && c.type != RIGID_PT) continue;
+      if (c.type != PT && c.type != RIGID_PT && !(c.type == EE && contact_mode == 1)) continue;
       R kb = contact_stiffness(c);
       if (!in_domain(c.gap, P.gap_max)) continue;
       V3<R> nn = normalize_or_default(c.normal);
@@ -1104,6 +1171,19 @@ template <class R> struct Sim {
           if (wj == R(0)) continue;
           V3<R> gj = wj * nn;
           if (c.idx[i] < 0 || c.idx[j] < 0) continue;
+          if (c.idx[i] != c.idx[j]) {
+            // Off-diagonal contact block (matrix_assembly.cpp:116-145 entry
+            // rule): kept beside the pattern.
+            Coupling cu{c.idx[i], c.idx[j], {}};
+            bool any = false;
+            for (int k = 0; k < 9; ++k) {
+              R val = d2 * (gi[k / 3] * gj[k % 3]);
+              cu.b[k] = std::fabs(val) < P.barrier_tol ? R(0) : val;
+              any = any || cu.b[k] != R(0);
+            }
+            if (any) couplings.push_back(cu);
+            continue;
+          }
           for (int k = 0; k < 9; ++k) {
             R val = d2 * (gi[k / 3] * gj[k % 3]);
             if (std::fabs(val) < P.barrier_tol) continue;
@@ -1179,7 +1259,7 @@ template <class R> struct Sim {
   // PCGSolver::solve (pcg_solver.cpp:8-146).  Returns converged flag; iters out.
   static bool pcg(int nv, const std::vector<int>& rp_, const std::vector<int>& ci_, const std::vector<int>& dg,
                   const std::vector<R>& A, const std::vector<R>& b, std::vector<R>& xs, R tol, int maxit,
-                  int& iters) {
+                  int& iters, const std::vector<Coupling>* cpl = nullptr) {
     int N = 3 * nv;
     iters = 0;
     std::vector<M3<R>> Pc(nv);
@@ -1205,6 +1285,14 @@ template <class R> struct Sim {
           }
           out[3 * i + r] = tmp;
         }
+      // Contact couplings outside the mesh pattern, in contact order.
+      if (cpl)
+        for (const auto& cu : *cpl)
+          for (int r = 0; r < 3; ++r) {
+            R t = R(0);
+            for (int c = 0; c < 3; ++c) t += cu.b[3 * r + c] * p[3 * cu.j + c];
+            out[3 * cu.i + r] += t;
+          }
     };
     auto relres = [&](const std::vector<R>& r) {
       R ri = vinf(r.data(), N), bi = vinf(b.data(), N);
@@ -1343,7 +1431,7 @@ template <class R> struct Sim {
       std::vector<R> b(3 * n);
       for (int i = 0; i < 3 * n; ++i) b[i] = -g[i];
       int iters = 0;
-      bool ok = pcg(n, rp, ci, diag, H, b, d, P.pcg_tol, P.pcg_max_iters, iters);
+      bool ok = pcg(n, rp, ci, diag, H, b, d, P.pcg_tol, P.pcg_max_iters, iters, &couplings);
       st.pcg_iters += iters;
       st.pcg_solves += 1;
       st.pcg_per_solve.push_back(iters);
@@ -1544,6 +1632,14 @@ void orc_destroy(void* h) {
 }
 
 // material: E, nu, density, thickness, bending
+// Contact mode: 0 faithful, 1 corrected (brute-force self PT/EE with the
+// reference's intended predicate, PT and EE barriers in the energy).
+int orc_set_contact_mode(void* h, int mode, double pt_box, double keep_gap) {
+  OSim* s = (OSim*)h;
+  DISPATCH(s, S.contact_mode = mode; S.pt_box = (decltype(S.pt_box))pt_box; S.keep_gap = (decltype(S.keep_gap))keep_gap);
+  return 0;
+}
+
 // Threads for the oracle's parallel loops (results do not depend on it).
 void orc_set_threads(int k) { omp_set_num_threads(k > 0 ? k : 1); }
 int orc_get_threads() { return omp_get_max_threads(); }

@@ -75,6 +75,7 @@ def lib():
         L.orc_line_search.argtypes = [C.c_void_p, _dp, C.c_double]
         L.orc_set_contacts_pt.argtypes = [C.c_void_p, C.c_int, _ip]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_contact_mode.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
         L.orc_get_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -154,6 +155,11 @@ class OracleSim:
 
     def set(self, which: str, arr):
         _chk(self.L.orc_set(self.h, {"x": 0, "v": 1}[which], _d(arr).ravel()))
+
+    def set_contact_mode(self, corrected: bool, pt_box: float = 1e-4, keep_gap: float = 0.01):
+        """Faithful (self broad phase empty) or corrected (brute-force PT/EE with
+        the intended predicate, both barriers in the energy)."""
+        self.L.orc_set_contact_mode(self.h, 1 if corrected else 0, pt_box, keep_gap)
 
     def stats(self) -> dict:
         out = np.zeros(11)

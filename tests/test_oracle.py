@@ -327,3 +327,81 @@ def test_threads_do_not_change_results():
     orc.set_threads(1)
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     assert out[0][2] == out[1][2]
+
+
+def _two_sheet_scene(gap, s=4, shift=0.0):
+    """Two s x s sheets (0.1 m) stacked `gap` apart in one mesh, no walls/pins."""
+    V1, T1 = scenes.grid(s, 0.1, 0.2)
+    V2, T2 = scenes.grid(s, 0.1, 0.2 + gap)
+    V2[:, 0] += shift
+    V = np.concatenate([V1, V2]).astype(np.float32)
+    T = np.concatenate([T1, T2 + s * s]).astype(np.int32)
+    return scenes.Scene("two", V, T, dict(scenes.BASE_MATERIAL), scenes.base_params(), [], [], [])
+
+
+def test_corrected_self_contacts_known_answer():
+    """Two parallel sheets 5e-5 apart: every vertex of either sheet lies over a
+    triangle of the other within the 1e-4 point box, so each vertex has PT
+    contacts at gap 5e-5 with a +-z normal; faithful mode finds none (F1)."""
+    sc = _two_sheet_scene(5e-5)
+    o = orc.OracleSim(sc, 64)
+    assert len(o.contacts()) == 0
+    o.set_contact_mode(True)
+    cs = o.contacts()
+    pt = cs[cs[:, 0] == 0]
+    assert len(pt) > 0
+    assert set(pt[:, 1].astype(int)) == set(range(32))
+    zgap = float(np.float32(0.2 + 5e-5)) - float(np.float32(0.2))   # float32 sheet heights
+    np.testing.assert_allclose(pt[:, 6], zgap, rtol=1e-9)
+    np.testing.assert_allclose(np.abs(pt[:, 9]), 1.0, rtol=1e-9)
+    lower = pt[:, 1] < 16
+    assert np.all(pt[lower, 9] < 0) and np.all(pt[~lower, 9] > 0)   # normal points toward the vertex
+    # Incident triangles are never candidates.
+    assert not np.any((pt[:, 2:5] == pt[:, 1:2]).any(axis=1))
+    # Weights {1, -b0, -b1, -b2} sum to zero.
+    np.testing.assert_allclose(pt[:, 13:17].sum(axis=1), 0.0, atol=1e-12)
+    # Sheets 2 mm apart: outside the point box, no contacts.
+    o2 = orc.OracleSim(_two_sheet_scene(2e-3), 64)
+    o2.set_contact_mode(True)
+    assert len(o2.contacts()) == 0
+
+
+def test_corrected_edge_edge_known_answer():
+    """Two edges crossing in projection, 1e-3 apart along z, boxes overlapping
+    in z: one EE contact with gap 1e-3, normal -z (from the second edge to the
+    first), weights {1/2, 1/2, -1/2, -1/2}."""
+    V = np.array([[-1, 0, -0.01], [1, 0, 0.01], [0.5, 5, 0.0],       # tri 0 with edge (0,1)
+                  [0, -1, 0.0], [0, 1, 0.0], [-5, 0.5, 9.0]], np.float32)
+    # Shift the second edge so that the closest points are 1e-3 apart in z.
+    V[3, 2] = V[4, 2] = 0.001
+    T = np.array([[0, 1, 2], [3, 4, 5]], np.int32)
+    sc = scenes.Scene("ee", V, T, dict(scenes.BASE_MATERIAL), scenes.base_params(), [], [], [])
+    o = orc.OracleSim(sc, 64)
+    o.set_contact_mode(True, pt_box=1e-6)
+    cs = o.contacts()
+    ee = cs[cs[:, 0] == 1]
+    hit = [c for c in ee if set(c[1:3].astype(int)) == {0, 1} and set(c[3:5].astype(int)) == {3, 4}]
+    assert len(hit) == 1
+    c = hit[0]
+    assert abs(c[6] - 1e-3) < 1e-7   # float32 end points
+    np.testing.assert_allclose(c[7:10], [0, 0, -1], atol=2e-2)   # edge A is tilted by 1e-2
+    np.testing.assert_allclose(c[13:17], [0.5, 0.5, -0.5, -0.5])
+
+
+def test_corrected_step_uses_self_barrier():
+    """Two sheets 5e-5 apart: in corrected mode the PT barrier pushes them
+    apart, the upper sheet stays above the lower one and the run stays finite."""
+    sc = _two_sheet_scene(5e-5)
+    o = orc.OracleSim(sc, 64)
+    o.set_contact_mode(True)
+    o.substep()
+    assert o.stats()["contacts"] > 0
+    for _ in range(2):
+        o.substep()
+    x = o.get("x").reshape(-1, 3)
+    assert np.isfinite(x).all()
+    assert x[16:, 2].min() > x[:16, 2].max()      # upper sheet stays above the lower one
+    # Faithful mode ignores the other sheet: both fall together, gap unchanged.
+    f = orc.OracleSim(sc, 64)
+    f.substep()
+    assert f.stats()["contacts"] == 0
