@@ -22,6 +22,7 @@
 #include "model.h"
 #include "pcg.hip"
 #include "learned.hip"
+#include "lbvh.hip"
 
 namespace ando {
 
@@ -133,6 +134,14 @@ void build_spmv_meta(SpmvMeta& sm, int n, const std::vector<int32_t>& row_ptr, c
 
 #include "learned_host.hip"
 
+// One linear BVH (triangles or edges) and its build scratch.
+struct LbvhTree {
+  int P = 0;
+  DBuf<float> boxes, table, nodes, bpart, bounds;
+  DBuf<uint32_t> keys, keys2;
+  DBuf<int32_t> vals, vals2, counts, offsets, left, right, first, last, scan_sums, scan_total;
+};
+
 struct ando_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -190,6 +199,16 @@ struct ando_ctx {
   int spmv_rowsets = 3;    // block rows per lane group of the row SpMV (1..4)
   int spmv_blocks = 0;     // 1: block-parallel SpMV where the pattern allows it, 0: row SpMV
   SpmvMeta spmv_meta;      // for the mesh's block pattern
+  // Corrected contact mode (self point-triangle and edge-edge through the LBVH).
+  int corrected = 0;
+  float pt_box = 1e-4f, keep_gap = 0.01f;
+  std::vector<int32_t> edges_h;
+  DBuf<int32_t> d_edges;
+  LbvhTree tri_tree, edge_tree;
+  DBuf<ando::SelfContact> sc;
+  DBuf<int32_t> sc_key, sc_count, sc_ptr, sc_sums, sc_total_d;
+  DBuf<uint32_t> sc_err;
+  int sc_pt = 0, sc_total = 0;
   int learned = 0;         // learned preconditioner + warm start in the step's PCG
   nnh::Net nn;
   nnh::Work nnw;
@@ -422,6 +441,131 @@ void detect_rigid(ando_ctx* c, const ando_rigid_body* bodies, int nbod, float dt
     hipLaunchKernelGGL(k_rigid_detect, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, 1, c->x.p, D, c->rc_ptr.p,
                        c->rc_count.p, c->rc.p, c->masses.p, c->d_diag.p, c->Helast3.p, dt);
   HIPCHK(hipGetLastError());
+}
+
+// Exclusive scan of n int32 (out has n + 1 entries); returns the total.
+int32_t exclusive_scan(ando_ctx* c, int n, const int32_t* in, int32_t* out, DBuf<int32_t>& sums,
+                       DBuf<int32_t>& total) {
+  hipStream_t s = c->stream;
+  const int nbs = std::max(1, (n + 1023) / 1024);
+  sums.ensure((size_t)nbs + 1);
+  total.ensure(1);
+  hipLaunchKernelGGL(k_scan_local, dim3(nbs), dim3(kBlock), 0, s, n, in, out, sums.p);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, s, nbs, sums.p, total.p);
+  hipLaunchKernelGGL(k_scan_add, dim3(nblk(n + 1)), dim3(kBlock), 0, s, n, out, (const int32_t*)sums.p,
+                     (const int32_t*)total.p);
+  HIPCHK(hipGetLastError());
+  int32_t t = 0;
+  HIPCHK(hipMemcpyAsync(&t, total.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  sync(c);
+  return t;
+}
+
+// LBVH over P primitives of `kind` vertices each (device index array prims):
+// boxes, Morton codes, stable radix sort, Karras hierarchy, node boxes.
+void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims) {
+  using namespace ando::lbvh;
+  hipStream_t s = c->stream;
+  T.P = P;
+  if (P == 0) return;
+  const int G = nblk(P);
+  T.boxes.ensure(6 * (size_t)P);
+  hipLaunchKernelGGL(k_prim_boxes, dim3(G), dim3(256), 0, s, P, kind, prims, (const float*)c->x.p, T.boxes.p);
+  T.bpart.ensure(6 * (size_t)G);
+  T.bounds.ensure(6);
+  hipLaunchKernelGGL(k_centroid_bounds, dim3(G), dim3(256), 0, s, P, (const float*)T.boxes.p, T.bpart.p);
+  hipLaunchKernelGGL(k_centroid_bounds_final, dim3(1), dim3(64), 0, s, G, (const float*)T.bpart.p, T.bounds.p);
+  T.keys.ensure(P); T.keys2.ensure(P); T.vals.ensure(P); T.vals2.ensure(P);
+  hipLaunchKernelGGL(k_morton, dim3(G), dim3(256), 0, s, P, (const float*)T.boxes.p, (const float*)T.bounds.p,
+                     T.keys.p, T.vals.p);
+  const int nc = 256 * G;
+  T.counts.ensure(nc);
+  T.offsets.ensure((size_t)nc + 1);
+  for (int shift = 0; shift < 32; shift += 8) {
+    hipLaunchKernelGGL(k_radix_count, dim3(G), dim3(256), 0, s, P, (const uint32_t*)T.keys.p, shift, T.counts.p);
+    exclusive_scan(c, nc, T.counts.p, T.offsets.p, T.scan_sums, T.scan_total);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(G), dim3(256), 0, s, P, (const uint32_t*)T.keys.p,
+                       (const int32_t*)T.vals.p, shift, (const int32_t*)T.offsets.p, T.keys2.p, T.vals2.p);
+    std::swap(T.keys, T.keys2);
+    std::swap(T.vals, T.vals2);
+  }
+  int levels = 1;
+  while ((1 << levels) <= P) ++levels;
+  T.table.ensure(6 * (size_t)P * levels);
+  hipLaunchKernelGGL(k_table_level0, dim3(G), dim3(256), 0, s, P, (const float*)T.boxes.p,
+                     (const int32_t*)T.vals.p, T.table.p);
+  for (int k = 1; k < levels; ++k)
+    hipLaunchKernelGGL(k_table_level, dim3(G), dim3(256), 0, s, P, 1 << (k - 1),
+                       (const float*)(T.table.p + (size_t)(k - 1) * 6 * P), T.table.p + (size_t)k * 6 * P);
+  if (P >= 2) {
+    T.left.ensure(P - 1); T.right.ensure(P - 1); T.first.ensure(P - 1); T.last.ensure(P - 1);
+    T.nodes.ensure(6 * (size_t)(P - 1));
+    hipLaunchKernelGGL(k_karras, dim3(nblk(P - 1)), dim3(256), 0, s, P, (const uint32_t*)T.keys.p, T.left.p,
+                       T.right.p, T.first.p, T.last.p);
+    hipLaunchKernelGGL(k_node_boxes, dim3(nblk(P - 1)), dim3(256), 0, s, P, (const int32_t*)T.first.p,
+                       (const int32_t*)T.last.p, (const float*)T.table.p, T.nodes.p);
+  }
+  HIPCHK(hipGetLastError());
+}
+
+// Corrected self detection at the current positions: c->sc holds the PT
+// contacts (vertex, triangle order) followed by the EE contacts (e1, e2).
+void detect_self(ando_ctx* c) {
+  using namespace ando::lbvh;
+  hipStream_t s = c->stream;
+  c->sc_pt = c->sc_total = 0;
+  if (!c->corrected) return;
+  const int n = c->n, m = c->m, E = (int)c->edges_h.size() / 2;
+  lbvh_build(c, c->tri_tree, m, 3, c->d_tris.p);
+  lbvh_build(c, c->edge_tree, E, 2, c->d_edges.p);
+  c->sc_err.ensure(1);
+  HIPCHK(hipMemsetAsync(c->sc_err.p, 0, sizeof(uint32_t), s));
+  const int q = std::max(n, E);
+  c->sc_count.ensure((size_t)q + 1);
+  c->sc_ptr.ensure((size_t)q + 2);
+  const LbvhTree& T = c->tri_tree;
+  const LbvhTree& Q = c->edge_tree;
+  auto pt = [&](int pass, ando::SelfContact* out, int32_t* key) {
+    hipLaunchKernelGGL(k_query_pt, dim3(nblk(n)), dim3(kBlock), 0, s, pass, n, (const float*)c->x.p, c->pt_box,
+                       c->keep_gap, (const int32_t*)c->d_tris.p, T.P, (const int32_t*)T.left.p,
+                       (const int32_t*)T.right.p, (const float*)T.nodes.p, (const float*)T.table.p,
+                       (const int32_t*)T.vals.p, c->sc_count.p, (const int32_t*)c->sc_ptr.p, out, key, c->sc_err.p);
+  };
+  auto ee = [&](int pass, ando::SelfContact* out, int32_t* key) {
+    hipLaunchKernelGGL(k_query_ee, dim3(nblk(E)), dim3(kBlock), 0, s, pass, E, (const float*)c->x.p, c->keep_gap,
+                       (const int32_t*)c->d_edges.p, Q.P, (const int32_t*)Q.left.p, (const int32_t*)Q.right.p,
+                       (const float*)Q.nodes.p, (const float*)Q.table.p, (const int32_t*)Q.vals.p, c->sc_count.p,
+                       (const int32_t*)c->sc_ptr.p, out, key, c->sc_err.p);
+  };
+  int npt = 0, nee = 0;
+  if (m > 0) {
+    pt(0, nullptr, nullptr);
+    npt = exclusive_scan(c, n, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
+  }
+  c->sc.ensure((size_t)npt + 1);
+  c->sc_key.ensure((size_t)npt + 1);
+  if (npt) pt(1, c->sc.p, c->sc_key.p);
+  if (E > 1) {
+    ee(0, nullptr, nullptr);
+    nee = exclusive_scan(c, E, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
+  }
+  if (nee) {
+    // Grow (keeping the PT part) and append the EE part.
+    DBuf<ando::SelfContact> all;
+    all.ensure((size_t)npt + nee);
+    if (npt)
+      HIPCHK(hipMemcpyAsync(all.p, c->sc.p, sizeof(ando::SelfContact) * npt, hipMemcpyDeviceToDevice, s));
+    c->sc = std::move(all);
+    c->sc_key.ensure((size_t)npt + nee);
+    ee(1, c->sc.p + npt, c->sc_key.p + npt);
+  }
+  HIPCHK(hipGetLastError());
+  uint32_t err = 0;
+  HIPCHK(hipMemcpyAsync(&err, c->sc_err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  sync(c);
+  if (err) throw Fail(ANDO_EDEVICE, "BVH traversal stack exceeded its depth");
+  c->sc_pt = npt;
+  c->sc_total = npt + nee;
 }
 
 // PCG on the matrix in c->H (or a caller's buffers) with b = -g; result in c->d.
@@ -930,6 +1074,8 @@ int ando_set_mesh(ando_ctx* c, const float* vertices, int32_t n, const int32_t* 
     c->tris.assign(triangles, triangles + 3 * (size_t)m);
     RestState rs = compute_rest_state(vertices, n, c->tris.data(), m);
     build_topology(c, rs.edges);
+    c->edges_h = rs.edges;
+    c->d_edges.upload(c->edges_h, c->stream);
     alloc_state(c);
     hipStream_t s = c->stream;
     c->mverts.upload(vertices, 3 * (size_t)n, s);
@@ -1090,7 +1236,8 @@ int ando_set_mode(ando_ctx* c, int32_t flags) {
   return guarded([&] {
     need(c != nullptr, ANDO_EINVAL, "null context");
     need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING | ANDO_MODE_PCG_PERSISTENT |
-                    ANDO_MODE_LEARNED_PRECOND | ANDO_MODE_LEARNED_WARM_START_ONLY)) == 0,
+                    ANDO_MODE_LEARNED_PRECOND | ANDO_MODE_LEARNED_WARM_START_ONLY |
+                    ANDO_MODE_CORRECTED_CONTACT)) == 0,
          ANDO_EINVAL, "unknown mode flag");
     need(!(flags & ANDO_MODE_LEARNED_PRECOND) || c->nn.ready, ANDO_ESTATE,
          "learned preconditioner mode needs ando_set_precond_weights first");
@@ -1098,6 +1245,7 @@ int ando_set_mode(ando_ctx* c, int32_t flags) {
     c->kernel_timing = (flags & ANDO_MODE_KERNEL_TIMING) ? 1 : 0;
     c->pcg_persistent = (flags & ANDO_MODE_PCG_PERSISTENT) ? 1 : 0;
     c->learned = (flags & ANDO_MODE_LEARNED_PRECOND) ? ((flags & ANDO_MODE_LEARNED_WARM_START_ONLY) ? 2 : 1) : 0;
+    c->corrected = (flags & ANDO_MODE_CORRECTED_CONTACT) ? 1 : 0;
   });
 }
 
@@ -1180,7 +1328,8 @@ int ando_compute_contacts(ando_ctx* c, const ando_rigid_body* bodies, int32_t nu
       HIPCHK(hipMemsetAsync(c->Helast3.p, 0, 3 * sizeof(float) * (size_t)c->nb, c->stream));
     }
     detect_rigid(c, bodies, num_bodies, dt);
-    *count = c->rc_total;
+    detect_self(c);
+    *count = c->sc_total + c->rc_total;
     if (!out || capacity <= 0) return;
     std::vector<RigidContact> hc(c->rc_total);
     std::vector<int32_t> ptr(c->n + 1);
@@ -1192,8 +1341,27 @@ int ando_compute_contacts(ando_ctx* c, const ando_rigid_body* bodies, int32_t nu
     }
     std::vector<float> xh(3 * (size_t)c->n);
     HIPCHK(hipMemcpyAsync(xh.data(), c->x.p, 3 * sizeof(float) * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+    std::vector<ando::SelfContact> hs(c->sc_total);
+    if (c->sc_total)
+      HIPCHK(hipMemcpyAsync(hs.data(), c->sc.p, sizeof(ando::SelfContact) * c->sc_total, hipMemcpyDeviceToHost,
+                            c->stream));
     sync(c);
     int w = 0;
+    // Self contacts first (detect_all_collisions order), then rigid ones.
+    for (int e = 0; e < c->sc_total && w < capacity; ++e, ++w) {
+      const ando::SelfContact& r = hs[e];
+      ando_contact& o = out[w];
+      std::memset(&o, 0, sizeof(o));
+      o.type = r.type;
+      for (int k = 0; k < 4; ++k) o.idx[k] = r.idx[k];
+      o.rigid_body_index = -1;
+      o.vertex_count = 4;
+      o.gap = r.gap;
+      o.normal[0] = r.nx; o.normal[1] = r.ny; o.normal[2] = r.nz;
+      for (int k = 0; k < 3; ++k) o.witness_p[k] = xh[3 * r.idx[0] + k];
+      o.witness_q[0] = r.qx; o.witness_q[1] = r.qy; o.witness_q[2] = r.qz;
+      for (int k = 0; k < 4; ++k) o.weights[k] = r.w[k];
+    }
     for (int i = 0; i < c->n && w < capacity; ++i)
       for (int e = ptr[i]; e < ptr[i + 1] && w < capacity; ++e, ++w) {
         const RigidContact& r = hc[e];

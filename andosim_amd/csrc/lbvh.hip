@@ -1,0 +1,471 @@
+// Corrected-mode self contact: a linear BVH over triangles and over edges,
+// rebuilt every detection, and the point-triangle / edge-edge queries.
+//
+// The reference's self broad phase (collision.cpp:63-276) never reports a
+// pair for meshes with two or more primitives (its BVH only stores a
+// primitive in size-1 ranges).  This restates the predicate it intends:
+//   PT: vertex v, triangle t not incident to v, box [p - e, p + e] overlaps
+//       t's box (closed intervals, e = 1e-4 in the reference);
+//   EE: edges e1 < e2 sharing no vertex, boxes overlapping;
+// then the narrow phases (collision.cpp:279-452), kept if gap < 0.01.
+//
+// Build: primitive boxes -> centroid bounds -> 30-bit Morton codes -> stable
+// LSD radix sort (8-bit digits, per-workgroup LDS histograms and wave-ballot
+// ranking) -> Karras hierarchy (one thread per internal node) -> node boxes
+// from a sparse table of box unions over the sorted primitives (every node
+// covers a contiguous sorted range, so its box is the union of two
+// power-of-two windows; no atomics, deterministic).  Query: one thread per
+// vertex (or edge), stack traversal, a count pass and a fill pass into an
+// exclusive-scanned list, each query's hits sorted by primitive index so the
+// list order is the brute-force order (vertex, triangle) / (e1, e2).
+#include <stdint.h>
+
+#include "device.h"
+#include "kernels.h"
+
+namespace ando {
+
+// A self contact (ContactPair, collision.h:78-103): type 0 PT (idx = v, t0, t1,
+// t2; weights 1, -b0, -b1, -b2) or 1 EE (idx = e1 ends, e2 ends; weights
+// 1/2, 1/2, -1/2, -1/2).
+struct SelfContact {
+  int type;
+  int idx[4];
+  float gap;
+  float nx, ny, nz;
+  float w[4];
+  float qx, qy, qz;
+};
+
+namespace lbvh {
+
+constexpr int kStack = 64;
+
+struct Box {
+  float lo[3], hi[3];
+};
+
+__device__ inline Box load_box(const float* b, int i) {
+  Box r;
+  for (int k = 0; k < 3; ++k) {
+    r.lo[k] = b[6 * (size_t)i + k];
+    r.hi[k] = b[6 * (size_t)i + 3 + k];
+  }
+  return r;
+}
+__device__ inline void store_box(float* b, int i, const Box& r) {
+  for (int k = 0; k < 3; ++k) {
+    b[6 * (size_t)i + k] = r.lo[k];
+    b[6 * (size_t)i + 3 + k] = r.hi[k];
+  }
+}
+__device__ inline Box unite(const Box& a, const Box& b) {
+  Box r;
+  for (int k = 0; k < 3; ++k) {
+    r.lo[k] = fminf(a.lo[k], b.lo[k]);
+    r.hi[k] = fmaxf(a.hi[k], b.hi[k]);
+  }
+  return r;
+}
+// AABB::overlaps (collision.h:37-41): closed intervals.
+__device__ inline bool overlaps(const Box& a, const Box& b) {
+  return (a.lo[0] <= b.hi[0] && a.hi[0] >= b.lo[0]) && (a.lo[1] <= b.hi[1] && a.hi[1] >= b.lo[1]) &&
+         (a.lo[2] <= b.hi[2] && a.hi[2] >= b.lo[2]);
+}
+
+// Primitive boxes: min / max over the primitive's `kind` vertices (3 for
+// triangles, 2 for edges), as AABB::expand (compute_triangle_aabb /
+// compute_edge_aabb).
+__global__ void k_prim_boxes(int P, int kind, const int32_t* __restrict__ prims, const float* __restrict__ x,
+                             float* __restrict__ boxes) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  Box b;
+  for (int k = 0; k < 3; ++k) {
+    b.lo[k] = x[3 * prims[kind * i] + k];
+    b.hi[k] = b.lo[k];
+  }
+  for (int c = 1; c < kind; ++c)
+    for (int k = 0; k < 3; ++k) {
+      const float v = x[3 * prims[kind * i + c] + k];
+      b.lo[k] = fminf(b.lo[k], v);
+      b.hi[k] = fmaxf(b.hi[k], v);
+    }
+  store_box(boxes, i, b);
+}
+
+// Centroid bounds: per-workgroup partial min / max, then one workgroup.
+__global__ void k_centroid_bounds(int P, const float* __restrict__ boxes, float* __restrict__ part) {
+  __shared__ float s[6][256];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float c[3] = {INFINITY, INFINITY, INFINITY}, d[3] = {-INFINITY, -INFINITY, -INFINITY};
+  if (i < P) {
+    const Box b = load_box(boxes, i);
+    for (int k = 0; k < 3; ++k) c[k] = d[k] = 0.5f * (b.lo[k] + b.hi[k]);
+  }
+  for (int k = 0; k < 3; ++k) {
+    s[k][threadIdx.x] = c[k];
+    s[3 + k][threadIdx.x] = d[k];
+  }
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int k = 0; k < 3; ++k) {
+        s[k][threadIdx.x] = fminf(s[k][threadIdx.x], s[k][threadIdx.x + st]);
+        s[3 + k][threadIdx.x] = fmaxf(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + st]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+__global__ void k_centroid_bounds_final(int G, const float* __restrict__ part, float* __restrict__ out) {
+  if (threadIdx.x >= 6) return;
+  const int k = threadIdx.x;
+  float v = k < 3 ? INFINITY : -INFINITY;
+  for (int g = 0; g < G; ++g) v = k < 3 ? fminf(v, part[6 * g + k]) : fmaxf(v, part[6 * g + k]);
+  out[k] = v;
+}
+
+__device__ inline uint32_t spread10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// 30-bit Morton code of the box centroid in the centroid bounds; the value is
+// the primitive index (the sort is stable, so equal codes keep index order).
+__global__ void k_morton(int P, const float* __restrict__ boxes, const float* __restrict__ bounds,
+                         uint32_t* __restrict__ keys, int32_t* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const Box b = load_box(boxes, i);
+  uint32_t q[3];
+  for (int k = 0; k < 3; ++k) {
+    const float c = 0.5f * (b.lo[k] + b.hi[k]);
+    const float ext = bounds[3 + k] - bounds[k];
+    float u = ext > 0.0f ? (c - bounds[k]) / ext : 0.0f;
+    u = fminf(fmaxf(u, 0.0f), 1.0f);
+    q[k] = (uint32_t)fminf(u * 1024.0f, 1023.0f);
+  }
+  keys[i] = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+  vals[i] = i;
+}
+
+// Radix sort, pass `shift`: digit histogram per workgroup (256 keys), stored
+// digit-major (counts[d * G + g]) so that an exclusive scan gives each
+// (digit, workgroup) its output offset.
+__global__ void __launch_bounds__(256) k_radix_count(int P, const uint32_t* __restrict__ keys, int shift,
+                                                     int32_t* __restrict__ counts) {
+  __shared__ int32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < P) atomicAdd(&h[(keys[i] >> shift) & 0xffu], 1);
+  __syncthreads();
+  counts[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: rank of each key among the workgroup's keys with the same
+// digit that precede it = (same-digit keys in earlier waves) + (same-digit
+// lanes below it in its wave, found with 9 ballots: 8 digit bits + validity).
+__global__ void __launch_bounds__(256) k_radix_scatter(int P, const uint32_t* __restrict__ keys,
+                                                       const int32_t* __restrict__ vals, int shift,
+                                                       const int32_t* __restrict__ offsets,
+                                                       uint32_t* __restrict__ keys_out,
+                                                       int32_t* __restrict__ vals_out) {
+  __shared__ int32_t wc[4][256];
+  for (int k = threadIdx.x; k < 4 * 256; k += 256) wc[k / 256][k % 256] = 0;
+  __syncthreads();
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const bool ok = i < P;
+  const uint32_t key = ok ? keys[i] : 0u;
+  const uint32_t d = ok ? ((key >> shift) & 0xffu) : 0x100u;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t m = ~0ull;
+  for (int bit = 0; bit < 9; ++bit) {
+    const uint64_t bb = __ballot((d >> bit) & 1u);
+    m &= ((d >> bit) & 1u) ? bb : ~bb;
+  }
+  const int rank = __popcll(m & ((1ull << lane) - 1ull));
+  if (ok && rank == 0) wc[wave][d] = __popcll(m);
+  __syncthreads();
+  if (!ok) return;
+  int before = 0;
+  for (int w = 0; w < wave; ++w) before += wc[w][d];
+  const int dst = offsets[(size_t)d * gridDim.x + blockIdx.x] + before + rank;
+  keys_out[dst] = key;
+  vals_out[dst] = vals[i];
+}
+
+// Karras (2012) hierarchy over sorted codes.  Internal node i in [0, P-1);
+// child c >= 0 is internal node c, c < 0 is leaf ~c (sorted position).
+// delta(i, j): common prefix of (code, position) pairs, -1 outside [0, P).
+__device__ inline int delta(const uint32_t* keys, int P, int i, int j) {
+  if (j < 0 || j >= P) return -1;
+  const uint32_t a = keys[i], b = keys[j];
+  if (a != b) return __clz(a ^ b);
+  return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+}
+
+__global__ void k_karras(int P, const uint32_t* __restrict__ keys, int32_t* __restrict__ left,
+                         int32_t* __restrict__ right, int32_t* __restrict__ first, int32_t* __restrict__ last) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P - 1) return;
+  const int dl = delta(keys, P, i, i - 1), dr = delta(keys, P, i, i + 1);
+  const int d = dr > dl ? 1 : -1;
+  const int dmin = d > 0 ? dl : dr;
+  int lmax = 2;
+  while (delta(keys, P, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (delta(keys, P, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = delta(keys, P, i, j);
+  int s = 0;
+  for (int div = 2;; div *= 2) {
+    const int t = (l + div - 1) / div;
+    if (delta(keys, P, i, i + (s + t) * d) > dnode) s += t;
+    if (t <= 1) break;
+  }
+  const int g = i + s * d + (d < 0 ? -1 : 0);
+  const int lo = min(i, j), hi = max(i, j);
+  left[i] = (lo == g) ? ~g : g;
+  right[i] = (hi == g + 1) ? ~(g + 1) : g + 1;
+  first[i] = lo;
+  last[i] = hi;
+}
+
+// Sparse table of box unions: level 0 = boxes in sorted order; level k holds
+// the union of [s, s + 2^k).
+__global__ void k_table_level0(int P, const float* __restrict__ boxes, const int32_t* __restrict__ order,
+                               float* __restrict__ T0) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P) return;
+  store_box(T0, s, load_box(boxes, order[s]));
+}
+__global__ void k_table_level(int P, int half, const float* __restrict__ prev, float* __restrict__ next) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s + 2 * half > P) return;
+  store_box(next, s, unite(load_box(prev, s), load_box(prev, s + half)));
+}
+// Node box = union over its sorted range [first, last] (two overlapping
+// power-of-two windows of the table).
+__global__ void k_node_boxes(int P, const int32_t* __restrict__ first, const int32_t* __restrict__ last,
+                             const float* __restrict__ table, float* __restrict__ nodes) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P - 1) return;
+  const int a = first[i], b = last[i];
+  const int len = b - a + 1;
+  const int k = 31 - __clz((uint32_t)len);
+  const float* Tk = table + (size_t)k * 6 * P;
+  store_box(nodes, i, unite(load_box(Tk, a), load_box(Tk, b - (1 << k) + 1)));
+}
+
+// Triangle barycentrics of p (collision.cpp:11-43): clamped, renormalised.
+__device__ inline f3 triangle_bary(f3 p, f3 a, f3 b, f3 c) {
+  const f3 v0 = b - a, v1 = c - a, v2 = p - a;
+  const float d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1), d20 = dot(v2, v0), d21 = dot(v2, v1);
+  const float denom = d00 * d11 - d01 * d01;
+  float v = 0.0f, w = 0.0f;
+  const float feps = 1.1920928955078125e-07f;
+  if (fabsf(denom) > feps) {
+    v = (d11 * d20 - d01 * d21) / denom;
+    w = (d00 * d21 - d01 * d20) / denom;
+  }
+  v = fminf(fmaxf(v, 0.0f), 1.0f);
+  w = fminf(fmaxf(w, 0.0f), 1.0f);
+  if (v + w > 1.0f) {
+    const float sum = v + w;
+    if (sum > feps) {
+      v /= sum;
+      w /= sum;
+    }
+  }
+  return f3{1.0f - v - w, v, w};
+}
+
+// Segment-segment closest points (collision.cpp:389-452).
+__device__ inline void edge_edge(f3 p0, f3 p1, f3 q0, f3 q1, float& dist, f3& normal, f3& wp, f3& wq) {
+  const f3 d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
+  const float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+  const float eps = 1e-10f;
+  auto cl = [](float v) { return fminf(fmaxf(v, 0.0f), 1.0f); };
+  if (a <= eps && e <= eps) {
+    wp = p0;
+    wq = q0;
+  } else {
+    float s, t;
+    if (a <= eps) {
+      s = 0.0f;
+      t = cl(f / e);
+    } else {
+      const float c = dot(d1, r);
+      if (e <= eps) {
+        t = 0.0f;
+        s = cl(-c / a);
+      } else {
+        const float b = dot(d1, d2);
+        const float denom = a * e - b * b;
+        s = denom != 0.0f ? cl((b * f - c * e) / denom) : 0.0f;
+        t = (b * s + f) / e;
+        if (t < 0.0f) {
+          t = 0.0f;
+          s = cl(-c / a);
+        } else if (t > 1.0f) {
+          t = 1.0f;
+          s = cl((b - c) / a);
+        }
+      }
+    }
+    wp = p0 + s * d1;
+    wq = q0 + t * d2;
+  }
+  const f3 diff = wp - wq;
+  dist = nrm(diff);
+  normal = (double)dist > 1e-10 ? diff / dist : f3{0.0f, 0.0f, 1.0f};
+}
+
+// Stack traversal of one BVH for query box q; visit(sorted position) for every
+// overlapping leaf.  Returns false if the stack would overflow (the caller
+// reports an error; nothing is written out of bounds).
+template <class F>
+__device__ inline bool traverse(int P, const Box& q, const int32_t* left, const int32_t* right, const float* nodes,
+                                const float* leaves, F&& visit) {
+  if (P == 1) {
+    if (overlaps(q, load_box(leaves, 0))) visit(0);
+    return true;
+  }
+  int stack[kStack];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const int node = stack[--sp];
+    if (node < 0) {
+      const int s = ~node;
+      if (overlaps(q, load_box(leaves, s))) visit(s);
+      continue;
+    }
+    if (!overlaps(q, load_box(nodes, node))) continue;
+    if (sp + 2 > kStack) return false;
+    stack[sp++] = right[node];
+    stack[sp++] = left[node];
+  }
+  return true;
+}
+
+// Insertion sort of one query's hits out[base .. base+k) by key (the
+// traversal visits them in Morton order; the lists are short).
+__device__ inline void sort_hits(SelfContact* out, int32_t* key, int base, int k) {
+  for (int a = 1; a < k; ++a) {
+    const SelfContact cur = out[base + a];
+    const int ck = key[base + a];
+    int b = a - 1;
+    while (b >= 0 && key[base + b] > ck) {
+      out[base + b + 1] = out[base + b];
+      key[base + b + 1] = key[base + b];
+      --b;
+    }
+    out[base + b + 1] = cur;
+    key[base + b + 1] = ck;
+  }
+}
+
+// PT query, pass 0 counts hits per vertex, pass 1 writes them at ptr[v] and
+// sorts the vertex's hits by triangle index.
+__global__ void k_query_pt(int pass, int n, const float* __restrict__ x, float pt_box, float keep,
+                           const int32_t* __restrict__ tris, int P, const int32_t* __restrict__ left,
+                           const int32_t* __restrict__ right, const float* __restrict__ nodes,
+                           const float* __restrict__ leaves, const int32_t* __restrict__ order,
+                           int32_t* __restrict__ count, const int32_t* __restrict__ ptr,
+                           SelfContact* __restrict__ out, int32_t* __restrict__ key, uint32_t* __restrict__ err) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n || P == 0) {
+    if (pass == 0 && v < n) count[v] = 0;
+    return;
+  }
+  const f3 p = ld3(x, v);
+  Box q;
+  q.lo[0] = p.x - pt_box; q.lo[1] = p.y - pt_box; q.lo[2] = p.z - pt_box;
+  q.hi[0] = p.x + pt_box; q.hi[1] = p.y + pt_box; q.hi[2] = p.z + pt_box;
+  int k = 0;
+  const int base = pass ? ptr[v] : 0;
+  const bool ok = traverse(P, q, left, right, nodes, leaves, [&](int s) {
+    const int t = order[s];
+    const int a = tris[3 * t], b = tris[3 * t + 1], c = tris[3 * t + 2];
+    if (a == v || b == v || c == v) return;
+    const f3 xa = ld3(x, a), xb = ld3(x, b), xc = ld3(x, c);
+    const PTResult r = point_triangle(p, xa, xb, xc);
+    if (!(r.dist < keep)) return;
+    if (pass) {
+      SelfContact sc;
+      sc.type = 0;
+      sc.idx[0] = v; sc.idx[1] = a; sc.idx[2] = b; sc.idx[3] = c;
+      sc.gap = r.dist;
+      sc.nx = r.normal.x; sc.ny = r.normal.y; sc.nz = r.normal.z;
+      const f3 bc = triangle_bary(r.wq, xa, xb, xc);
+      sc.w[0] = 1.0f; sc.w[1] = -bc.x; sc.w[2] = -bc.y; sc.w[3] = -bc.z;
+      sc.qx = r.wq.x; sc.qy = r.wq.y; sc.qz = r.wq.z;
+      out[base + k] = sc;
+      key[base + k] = t;
+    }
+    ++k;
+  });
+  if (!ok) atomicOr(err, 1u);
+  if (pass == 0) {
+    count[v] = k;
+    return;
+  }
+  sort_hits(out, key, base, k);
+}
+
+// EE query for edge e1 against edges e2 > e1, same two passes; hits sorted by e2.
+__global__ void k_query_ee(int pass, int E, const float* __restrict__ x, float keep, const int32_t* __restrict__ edges,
+                           int P, const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+                           const float* __restrict__ nodes, const float* __restrict__ leaves,
+                           const int32_t* __restrict__ order, int32_t* __restrict__ count,
+                           const int32_t* __restrict__ ptr, SelfContact* __restrict__ out,
+                           int32_t* __restrict__ key, uint32_t* __restrict__ err) {
+  const int e1 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e1 >= E) return;
+  const int a0 = edges[2 * e1], a1 = edges[2 * e1 + 1];
+  const f3 p0 = ld3(x, a0), p1 = ld3(x, a1);
+  Box q;
+  q.lo[0] = fminf(p0.x, p1.x); q.lo[1] = fminf(p0.y, p1.y); q.lo[2] = fminf(p0.z, p1.z);
+  q.hi[0] = fmaxf(p0.x, p1.x); q.hi[1] = fmaxf(p0.y, p1.y); q.hi[2] = fmaxf(p0.z, p1.z);
+  int k = 0;
+  const int base = pass ? ptr[e1] : 0;
+  const bool ok = traverse(P, q, left, right, nodes, leaves, [&](int s) {
+    const int e2 = order[s];
+    if (e2 <= e1) return;
+    const int b0 = edges[2 * e2], b1 = edges[2 * e2 + 1];
+    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
+    float dist;
+    f3 nrm_, wp, wq;
+    edge_edge(p0, p1, ld3(x, b0), ld3(x, b1), dist, nrm_, wp, wq);
+    if (!(dist < keep)) return;
+    if (pass) {
+      SelfContact sc;
+      sc.type = 1;
+      sc.idx[0] = a0; sc.idx[1] = a1; sc.idx[2] = b0; sc.idx[3] = b1;
+      sc.gap = dist;
+      sc.nx = nrm_.x; sc.ny = nrm_.y; sc.nz = nrm_.z;
+      sc.w[0] = 0.5f; sc.w[1] = 0.5f; sc.w[2] = -0.5f; sc.w[3] = -0.5f;
+      sc.qx = wq.x; sc.qy = wq.y; sc.qz = wq.z;
+      out[base + k] = sc;
+      key[base + k] = e2;
+    }
+    ++k;
+  });
+  if (!ok) atomicOr(err, 1u);
+  if (pass == 0) {
+    count[e1] = k;
+    return;
+  }
+  sort_hits(out, key, base, k);
+}
+
+}  // namespace lbvh
+}  // namespace ando

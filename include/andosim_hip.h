@@ -186,6 +186,12 @@ int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, co
 /* With ANDO_MODE_LEARNED_PRECOND, ANDO_MODE_LEARNED_WARM_START_ONLY keeps the
  * block-Jacobi preconditioner and uses the network only for the warm start. */
 #define ANDO_MODE_LEARNED_WARM_START_ONLY 16
+/* ANDO_MODE_CORRECTED_CONTACT (new; the reference's self broad phase is always
+ * empty, collision.cpp:63-160) detects self point-triangle and edge-edge
+ * contacts with the predicate the reference intends (point box +-1e-4, edge
+ * boxes, gap < 0.01) through a GPU LBVH, and puts both barriers in the energy;
+ * ando_compute_contacts then lists them before the rigid contacts. */
+#define ANDO_MODE_CORRECTED_CONTACT 32
 int ando_set_mode(ando_ctx* ctx, int32_t flags);
 
 int ando_apply_gravity(ando_ctx* ctx, const float* gravity3, float dt);
