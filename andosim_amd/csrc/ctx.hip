@@ -209,6 +209,12 @@ struct ando_ctx {
   DBuf<int32_t> sc_key, sc_count, sc_ptr, sc_sums, sc_total_d;
   DBuf<uint32_t> sc_err;
   int sc_pt = 0, sc_total = 0;
+  DBuf<ando::SelfTerm> sterm;
+  DBuf<uint32_t> sv_keys, sv_keys2;
+  DBuf<int32_t> sv_vals, sv_vals2, sv_counts, sv_offsets, sv_sums, sv_total, sv_hist, sv_ptr;
+  DBuf<double> part_pAp2;
+  bool sv_ready = false;
+  float barrier_tol = 1e-12f;
   int learned = 0;         // learned preconditioner + warm start in the step's PCG
   nnh::Net nn;
   nnh::Work nnw;
@@ -461,6 +467,28 @@ int32_t exclusive_scan(ando_ctx* c, int n, const int32_t* in, int32_t* out, DBuf
   return t;
 }
 
+// Stable LSD radix sort of P (key, value) pairs, 8-bit digits; the result is
+// left in keys / vals (the second pair of buffers is scratch).
+void radix_sort(ando_ctx* c, int P, DBuf<uint32_t>& keys, DBuf<int32_t>& vals, DBuf<uint32_t>& keys2,
+                DBuf<int32_t>& vals2, DBuf<int32_t>& counts, DBuf<int32_t>& offsets, DBuf<int32_t>& sums,
+                DBuf<int32_t>& total) {
+  using namespace ando::lbvh;
+  hipStream_t s = c->stream;
+  const int G = nblk(P, 256), nc = 256 * G;
+  keys2.ensure(P);
+  vals2.ensure(P);
+  counts.ensure(nc);
+  offsets.ensure((size_t)nc + 1);
+  for (int shift = 0; shift < 32; shift += 8) {
+    hipLaunchKernelGGL(k_radix_count, dim3(G), dim3(256), 0, s, P, (const uint32_t*)keys.p, shift, counts.p);
+    exclusive_scan(c, nc, counts.p, offsets.p, sums, total);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(G), dim3(256), 0, s, P, (const uint32_t*)keys.p,
+                       (const int32_t*)vals.p, shift, (const int32_t*)offsets.p, keys2.p, vals2.p);
+    std::swap(keys, keys2);
+    std::swap(vals, vals2);
+  }
+}
+
 // LBVH over P primitives of `kind` vertices each (device index array prims):
 // boxes, Morton codes, stable radix sort, Karras hierarchy, node boxes.
 void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims) {
@@ -478,17 +506,7 @@ void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims)
   T.keys.ensure(P); T.keys2.ensure(P); T.vals.ensure(P); T.vals2.ensure(P);
   hipLaunchKernelGGL(k_morton, dim3(G), dim3(256), 0, s, P, (const float*)T.boxes.p, (const float*)T.bounds.p,
                      T.keys.p, T.vals.p);
-  const int nc = 256 * G;
-  T.counts.ensure(nc);
-  T.offsets.ensure((size_t)nc + 1);
-  for (int shift = 0; shift < 32; shift += 8) {
-    hipLaunchKernelGGL(k_radix_count, dim3(G), dim3(256), 0, s, P, (const uint32_t*)T.keys.p, shift, T.counts.p);
-    exclusive_scan(c, nc, T.counts.p, T.offsets.p, T.scan_sums, T.scan_total);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(G), dim3(256), 0, s, P, (const uint32_t*)T.keys.p,
-                       (const int32_t*)T.vals.p, shift, (const int32_t*)T.offsets.p, T.keys2.p, T.vals2.p);
-    std::swap(T.keys, T.keys2);
-    std::swap(T.vals, T.vals2);
-  }
+  radix_sort(c, P, T.keys, T.vals, T.keys2, T.vals2, T.counts, T.offsets, T.scan_sums, T.scan_total);
   int levels = 1;
   while ((1 << levels) <= P) ++levels;
   T.table.ensure(6 * (size_t)P * levels);
@@ -514,6 +532,7 @@ void detect_self(ando_ctx* c) {
   using namespace ando::lbvh;
   hipStream_t s = c->stream;
   c->sc_pt = c->sc_total = 0;
+  c->sv_ready = false;
   if (!c->corrected) return;
   const int n = c->n, m = c->m, E = (int)c->edges_h.size() / 2;
   lbvh_build(c, c->tri_tree, m, 3, c->d_tris.p);
@@ -566,6 +585,41 @@ void detect_self(ando_ctx* c) {
   if (err) throw Fail(ANDO_EDEVICE, "BVH traversal stack exceeded its depth");
   c->sc_pt = npt;
   c->sc_total = npt + nee;
+}
+
+// Step-constant terms of the self contacts and their vertex view (after
+// detect_self; needs the elastic-only blocks from ensure_const).
+void prepare_self(ando_ctx* c, const StepParams& P) {
+  using namespace ando::lbvh;
+  hipStream_t s = c->stream;
+  c->sv_ready = false;
+  const int nc = c->sc_total;
+  if (nc == 0) return;
+  c->sterm.ensure(nc);
+  hipLaunchKernelGGL(k_self_terms, dim3(nblk(nc)), dim3(kBlock), 0, s, nc, c->n, P,
+                     (const ando::SelfContact*)c->sc.p, (const float*)c->masses.p, (const int32_t*)c->d_row_ptr.p,
+                     (const int32_t*)c->d_cols.p, (const float*)c->Helast3.p, c->sterm.p);
+  const int cnt = 4 * nc;
+  c->sv_keys.ensure(cnt);
+  c->sv_vals.ensure(cnt);
+  hipLaunchKernelGGL(k_self_slots, dim3(nblk(cnt)), dim3(kBlock), 0, s, nc, (const ando::SelfContact*)c->sc.p,
+                     c->sv_keys.p, c->sv_vals.p);
+  radix_sort(c, cnt, c->sv_keys, c->sv_vals, c->sv_keys2, c->sv_vals2, c->sv_counts, c->sv_offsets, c->sv_sums,
+             c->sv_total);
+  c->sv_hist.ensure((size_t)c->n + 1);
+  c->sv_ptr.ensure((size_t)c->n + 1);
+  HIPCHK(hipMemsetAsync(c->sv_hist.p, 0, sizeof(int32_t) * c->n, s));
+  hipLaunchKernelGGL(k_count_keys, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, c->n, (const uint32_t*)c->sv_keys.p,
+                     c->sv_hist.p);
+  exclusive_scan(c, c->n, c->sv_hist.p, c->sv_ptr.p, c->sv_sums, c->sv_total);
+  c->part_pAp2.ensure((size_t)nblk(c->n) + 1);
+  c->sv_ready = true;
+}
+
+ando::SelfView self_view(ando_ctx* c) {
+  ando::SelfView v{nullptr, nullptr, nullptr, nullptr};
+  if (c->sv_ready) v = ando::SelfView{c->sv_ptr.p, c->sv_vals.p, c->sc.p, c->sterm.p};
+  return v;
 }
 
 // PCG on the matrix in c->H (or a caller's buffers) with b = -g; result in c->d.
@@ -643,11 +697,15 @@ PcgOut run_pcg_persistent(ando_ctx* c, int n, const float* A, const int32_t* rp,
 
 // P_given / x0_given (device, both or neither): a preconditioner of 9 floats
 // per vertex and a starting guess instead of block-Jacobi from x0 = 0.
+// sv (nullable): self-contact blocks that couple two vertices, applied after
+// the SpMV of every iteration (corrected contact mode; two-kernel row path).
 PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int32_t* cols, const int32_t* diag,
                const float* rhs, int negate, float* xsol, float tol, int maxit, SpmvMeta* sm,
-               const float* P_given = nullptr, const float* x0_given = nullptr) {
+               const float* P_given = nullptr, const float* x0_given = nullptr,
+               const ando::SelfView* sv = nullptr) {
   hipStream_t s = c->stream;
-  if (c->pcg_persistent && P_given == nullptr) {
+  const bool couple = sv != nullptr && sv->ptr != nullptr;
+  if (c->pcg_persistent && P_given == nullptr && !couple) {
     int G = persistent_grid(c, n);
     if (G > 0) {
       c->stats.pcg_kernel = 1;
@@ -655,7 +713,7 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
     }
   }
   const int R = c->spmv_rowsets;
-  const bool blk = c->spmv_blocks && sm != nullptr && sm->ok;
+  const bool blk = c->spmv_blocks && sm != nullptr && sm->ok && !couple;
   int nwg_v = nblk(n), nwg_r = blk ? sm->groups : spmv_groups(n, R);
   c->stats.pcg_kernel = blk ? 2 : 0;
   c->rz_hist.ensure((size_t)maxit + 2);
@@ -727,12 +785,16 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
           default: spmv(k_pcg_spmv<3>); break;
         }
       }
+      if (couple)
+        hipLaunchKernelGGL(ando::lbvh::k_self_couple, dim3(nwg_v), dim3(kBlock), 0, s, n, *sv, c->barrier_tol,
+                           (const PcgStatus*)c->pcg_status.p, (const float*)pcur, c->Ap.p, c->part_pAp2.p);
       hipExtLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(kBlock), 0, s, e2, e3, 0, n, it, (const float*)pcur,
                             (const float*)c->Ap.p, (const float*)(blk ? sm->extra.p : nullptr),
                             (const int32_t*)(blk ? sm->straddle.p : nullptr), Pc, xsol, c->r.p,
                             c->z.p,
                             (const double*)c->part_pAp.p, nwg_r, (const double*)c->rz_hist.p, c->part_rz.p,
-                            c->part_max.p, c->pcg_status.p);
+                            c->part_max.p, c->pcg_status.p, (const double*)(couple ? c->part_pAp2.p : nullptr),
+                            nwg_v);
     }
     HIPCHK(hipGetLastError());
     chunk = std::min(chunk * 2, 64);
@@ -791,7 +853,7 @@ void launch_gradient(ando_ctx* c, const ando_params& p, bool with_rigid) {
   hipLaunchKernelGGL(k_gradient, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, P, c->x.p, c->xhat.p, c->masses.p,
                      c->egrad.p, c->v2f_ptr.p, c->v2f.p, c->sflags.p, c->sgrad.p, rcp, c->rc.p, pinp, c->pin_t.p,
                      c->d_diag.p, c->Hconst.p, c->d_wall_n.p, c->d_wall_off.p, c->kw.p, c->xprev.p, c->g.p,
-                     c->scalars.p);
+                     c->scalars.p, self_view(c));
   HIPCHK(hipGetLastError());
 }
 
@@ -808,22 +870,33 @@ void launch_assemble(ando_ctx* c, const ando_params& p, bool with_rigid) {
   hipLaunchKernelGGL(k_assemble, dim3(nblk(c->nb)), dim3(kBlock), 0, c->stream, c->n, P, c->d_block_row.p,
                      c->d_cols.p, c->blk_cptr.p, c->blk_contrib.p, c->Hconst.p, c->sflags.p, c->shess.p, c->x.p,
                      c->masses.p, rcp, c->rc.p, pinp, c->pin_t.p, c->d_wall_n.p, c->d_wall_off.p, c->kw.p, c->xprev.p, c->H.p,
-                     c->nb);
+                     c->nb, self_view(c));
   HIPCHK(hipGetLastError());
 }
 
-// LineSearch::search over the faithful contact set: the first active wall.
+// LineSearch::search: the first active wall, and (corrected mode) the CCD of
+// the self point-triangle and edge-edge contacts.
 float line_search(ando_ctx* c) {
   hipStream_t s = c->stream;
-  if (c->num_walls == 0) return 1.0f;
-  const float* wn = c->wall_n.data();
-  f3 w = mk(wn[0], wn[1], wn[2]);
-  double sq = (double)(w.x * w.x + (w.y * w.y + w.z * w.z));
-  if (!(sq > 0.5)) return 1.0f;
+  const bool self = c->sc_total > 0;
+  bool wall = c->num_walls > 0;
+  f3 w = mk(0.0f, 0.0f, 0.0f);
+  if (wall) {
+    const float* wn = c->wall_n.data();
+    w = mk(wn[0], wn[1], wn[2]);
+    double sq = (double)(w.x * w.x + (w.y * w.y + w.z * w.z));
+    wall = sq > 0.5;
+  }
+  if (!wall && !self) return 1.0f;
   uint32_t init = 0xFFFFFu;
   HIPCHK(hipMemcpyAsync(c->scalars.p + 1, &init, sizeof(uint32_t), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_ls_mask, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, w, c->wall_off[0], 1, c->x.p, c->d.p,
-                     c->scalars.p + 1);
+  if (wall)
+    hipLaunchKernelGGL(k_ls_mask, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, w, c->wall_off[0], 1, c->x.p, c->d.p,
+                       c->scalars.p + 1);
+  if (self)
+    hipLaunchKernelGGL(ando::lbvh::k_ls_self, dim3(nblk(c->sc_total)), dim3(kBlock), 0, s, c->sc_total,
+                       (const ando::SelfContact*)c->sc.p, (const float*)c->x.p, (const float*)c->d.p, 1.25f,
+                       c->scalars.p + 1);
   HIPCHK(hipGetLastError());
   uint32_t mask = 0;
   HIPCHK(hipMemcpyAsync(&mask, c->scalars.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -845,6 +918,7 @@ float inner_newton(ando_ctx* c, const ando_params& p) {
     if (gmax < p.pcg_tol) return 1.0f;
     launch_assemble(c, p, true);
     PcgOut po;
+    const ando::SelfView sv = self_view(c);
     if (c->learned && c->nn.ready) {
       // Learned preconditioner and warm start; plain block-Jacobi PCG from
       // x0 = 0 is the fallback when it misses the tolerance.
@@ -852,19 +926,20 @@ float inner_newton(ando_ctx* c, const ando_params& p) {
                    c->stream);
       nnh::warm_start(c->nnw, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->g.p, 1, c->stream);
       po = run_pcg(c, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol,
-                   p.pcg_max_iters, &c->spmv_meta, c->learned == 2 ? c->nnw.Dinv.p : c->nnw.P.p, c->nnw.y.p);
+                   p.pcg_max_iters, &c->spmv_meta, c->learned == 2 ? c->nnw.Dinv.p : c->nnw.P.p, c->nnw.y.p,
+                   &sv);
       c->stats.learned_solves++;
       if (!po.converged) {
         c->stats.learned_fallbacks++;
         const int first = po.iters;
         po = run_pcg(c, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol,
-                     p.pcg_max_iters, &c->spmv_meta);
+                     p.pcg_max_iters, &c->spmv_meta, nullptr, nullptr, &sv);
         c->stats.fallback_iterations += po.iters;
         po.iters += first;
       }
     } else {
       po = run_pcg(c, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol,
-                   p.pcg_max_iters, &c->spmv_meta);
+                   p.pcg_max_iters, &c->spmv_meta, nullptr, nullptr, &sv);
     }
     if (c->stats.pcg_solves < ANDO_MAX_RECORDED_SOLVES) c->stats.pcg_per_solve[c->stats.pcg_solves] = po.iters;
     c->stats.pcg_solves++;
@@ -1276,7 +1351,10 @@ int ando_step(ando_ctx* c, const ando_params* params, ando_rigid_body* bodies, i
     HIPCHK(hipGetLastError());
     upload_bodies(c, bodies, num_bodies);
     detect_rigid(c, bodies, num_bodies, p.dt);
-    c->stats.contacts = c->rc_total;
+    c->barrier_tol = p.barrier_tolerance;
+    detect_self(c);
+    prepare_self(c, step_params(c, p));
+    c->stats.contacts = c->rc_total + c->sc_total;
     c->stats.rigid_contacts = c->rc_total;
     float beta = 0.0f;
     int bi = 0;
@@ -1403,6 +1481,9 @@ int ando_linearize(ando_ctx* c, const ando_params* params, const float* x_target
     c->xhat.upload(x_target, 3 * (size_t)c->n, c->stream);
     upload_bodies(c, bodies, num_bodies);
     detect_rigid(c, bodies, num_bodies, p.dt);
+    c->barrier_tol = p.barrier_tolerance;
+    detect_self(c);
+    prepare_self(c, step_params(c, p));
     launch_gradient(c, p, true);
     launch_assemble(c, p, true);
     if (gradient)

@@ -21,6 +21,37 @@ struct RigidContact {
   int32_t vertex;
 };
 
+// A self contact of the corrected mode (ContactPair, collision.h:78-103):
+// type 0 PT (idx = v, t0, t1, t2; weights 1, -b0, -b1, -b2) or 1 EE (idx =
+// e1 ends, e2 ends; weights 1/2, 1/2, -1/2, -1/2).
+struct SelfContact {
+  int type;
+  int idx[4];
+  float gap;
+  float nx, ny, nz;
+  float w[4];
+  float qx, qy, qz;
+};
+
+// Step-constant barrier terms of one self contact: in_domain flag, the
+// normalised normal, V' and V'' at the detected gap with the contact's
+// stiffness (Barrier::compute_gradient/hessian, barrier.cpp:29-65;
+// Stiffness::compute_contact_stiffness, stiffness.cpp:53-116).
+struct SelfTerm {
+  int active;
+  float dV, d2;
+  float nx, ny, nz;
+};
+
+// Self contacts seen from the vertices: for vertex i, entries
+// [ptr[i], ptr[i+1]) of code hold contact * 4 + slot in contact order.
+struct SelfView {
+  const int32_t* ptr;
+  const int32_t* code;
+  const SelfContact* sc;
+  const SelfTerm* st;
+};
+
 // Rigid body dynamic state on the device (rigid_body.h:72-85).
 struct BodyState {
   float pos[3];

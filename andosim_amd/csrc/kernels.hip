@@ -505,7 +505,7 @@ __global__ void k_gradient(int n, StepParams P, const float* __restrict__ x, con
                            const int32_t* __restrict__ diag, const float* __restrict__ Hconst,
                            const float* __restrict__ wall_n, const float* __restrict__ wall_off,
                            const float* __restrict__ kw, const float* __restrict__ xprev, float* __restrict__ g,
-                           uint32_t* __restrict__ gmax_bits) {
+                           uint32_t* __restrict__ gmax_bits, SelfView sv) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   float local = 0.0f;
   if (i < n) {
@@ -521,6 +521,15 @@ __global__ void k_gradient(int n, StepParams P, const float* __restrict__ x, con
         int fl = sflags[f];
         for (int s = 0; s < 2; ++s)
           if ((fl & (1 << s)) && (fl & (4 << s))) gi = gi + ld3(sgrad + 18 * (size_t)f + 9 * s, a);
+      }
+    // Self contacts (corrected mode) come first in the contact list.
+    if (sv.ptr)
+      for (int e = sv.ptr[i]; e < sv.ptr[i + 1]; ++e) {
+        const int code = sv.code[e], c = code >> 2, slot = code & 3;
+        const SelfTerm& t = sv.st[c];
+        if (!t.active || t.dV == 0.0f) continue;
+        const float s = t.dV * sv.sc[c].w[slot];
+        gi = gi + s * mk(t.nx, t.ny, t.nz);
       }
     if (rc_ptr)
       for (int e = rc_ptr[i]; e < rc_ptr[i + 1]; ++e) {
@@ -586,7 +595,7 @@ __global__ void k_assemble(int n, StepParams P, const int32_t* __restrict__ bloc
                            const RigidContact* __restrict__ rc, const int32_t* __restrict__ pin_ptr,
                            const float* __restrict__ pin_t, const float* __restrict__ wall_n,
                            const float* __restrict__ wall_off, const float* __restrict__ kw,
-                           const float* __restrict__ xprev, float* __restrict__ H, int nb) {
+                           const float* __restrict__ xprev, float* __restrict__ H, int nb, SelfView sv) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   float h[9];
@@ -616,6 +625,20 @@ __global__ void k_assemble(int n, StepParams P, const int32_t* __restrict__ bloc
     }
   int i = block_row[b];
   if (cols[b] == i) {
+    // Diagonal blocks of the self contacts (both slots on vertex i); the
+    // blocks coupling two vertices are applied in the PCG product.
+    if (sv.ptr)
+      for (int e = sv.ptr[i]; e < sv.ptr[i + 1]; ++e) {
+        const int code = sv.code[e], c = code >> 2, slot = code & 3;
+        const SelfTerm& t = sv.st[c];
+        if (!t.active || t.d2 == 0.0f) continue;
+        const f3 gi = sv.sc[c].w[slot] * mk(t.nx, t.ny, t.nz);
+        for (int e2 = 0; e2 < 9; ++e2) {
+          const float val = t.d2 * (comp(gi, e2 / 3) * comp(gi, e2 % 3));
+          if (fabsf(val) < P.barrier_tol) continue;
+          h[e2] = h[e2] + val;
+        }
+      }
     if (rc_ptr)
       for (int e = rc_ptr[i]; e < rc_ptr[i + 1]; ++e) {
         const RigidContact& c = rc[e];

@@ -25,18 +25,6 @@
 
 namespace ando {
 
-// A self contact (ContactPair, collision.h:78-103): type 0 PT (idx = v, t0, t1,
-// t2; weights 1, -b0, -b1, -b2) or 1 EE (idx = e1 ends, e2 ends; weights
-// 1/2, 1/2, -1/2, -1/2).
-struct SelfContact {
-  int type;
-  int idx[4];
-  float gap;
-  float nx, ny, nz;
-  float w[4];
-  float qx, qy, qz;
-};
-
 namespace lbvh {
 
 constexpr int kStack = 64;
@@ -465,6 +453,200 @@ __global__ void k_query_ee(int pass, int E, const float* __restrict__ x, float k
     return;
   }
   sort_hits(out, key, base, k);
+}
+
+// ---------------------------------------------------------------------------
+// Self-contact physics (corrected mode): step-constant barrier terms, the
+// vertex view of the contact list, the PCG product of the blocks that couple
+// two vertices, and the line-search CCD.
+// ---------------------------------------------------------------------------
+
+__device__ inline int find_block(const int32_t* rp, const int32_t* cols, int i, int j) {
+  int lo = rp[i], hi = rp[i + 1];
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cols[mid] < j) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < rp[i + 1] && cols[lo] == j) ? lo : -1;
+}
+
+// Per contact: k-bar = sum m_i w_i^2 / dt^2 + max(0, Wn^T H_el Wn) over the
+// elastic-only blocks linking its vertices (Stiffness::compute_contact_stiffness,
+// stiffness.cpp:53-116; H_el blocks are diagonal, stored as 3 floats), then
+// V' and V'' at the detected gap (integrator.cpp:315-338, 450-480).
+__global__ void k_self_terms(int nc, int n, StepParams P, const SelfContact* __restrict__ sc,
+                             const float* __restrict__ masses, const int32_t* __restrict__ rp,
+                             const int32_t* __restrict__ cols, const float* __restrict__ Hel3,
+                             SelfTerm* __restrict__ out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const SelfContact& C = sc[c];
+  float kb = 0.0f;
+  const f3 n0 = mk(C.nx, C.ny, C.nz);
+  const float nl = nrm(n0);
+  if (P.dt > 0.0f && !((double)nl < 1e-12)) {
+    const f3 nn = n0 / nl;
+    int vv[4];
+    float ww[4];
+    int bc = 0;
+    for (int i = 0; i < 4; ++i) {
+      const int idx = C.idx[i];
+      if (idx < 0 || idx >= n || C.w[i] == 0.0f) continue;
+      vv[bc] = idx;
+      ww[bc] = C.w[i];
+      ++bc;
+    }
+    if (bc > 0) {
+      float inertial = 0.0f;
+      for (int i = 0; i < bc; ++i) inertial += masses[vv[i]] * ww[i] * ww[i];
+      inertial /= (P.dt * P.dt);
+      float Wn[12], Ht[12];
+      for (int i = 0; i < bc; ++i) {
+        Wn[3 * i] = ww[i] * nn.x;
+        Wn[3 * i + 1] = ww[i] * nn.y;
+        Wn[3 * i + 2] = ww[i] * nn.z;
+      }
+      for (int i = 0; i < 3 * bc; ++i) Ht[i] = 0.0f;
+      for (int cb = 0; cb < bc; ++cb)
+        for (int rb = 0; rb < bc; ++rb) {
+          const int bi = find_block(rp, cols, vv[rb], vv[cb]);
+          if (bi < 0) continue;
+          for (int r = 0; r < 3; ++r) Ht[3 * rb + r] += Hel3[3 * (size_t)bi + r] * Wn[3 * cb + r];
+        }
+      float el = Wn[0] * Ht[0];
+      for (int i = 1; i < 3 * bc; ++i) el = el + Wn[i] * Ht[i];
+      if (el < 0.0f) el = 0.0f;
+      kb = inertial + el;
+    }
+  }
+  SelfTerm t;
+  t.active = in_domain(C.gap, P.gap_max) ? 1 : 0;
+  const f3 nn = normalize_or_default(n0, P.normal_eps);
+  t.nx = nn.x; t.ny = nn.y; t.nz = nn.z;
+  t.dV = t.active ? barrier_grad(C.gap, P.gap_max, kb, P.fp64_near) : 0.0f;
+  t.d2 = t.active ? barrier_hess(C.gap, P.gap_max, kb, P.fp64_near) : 0.0f;
+  out[c] = t;
+}
+
+// Vertex view: key = vertex of slot (c, s) or 0xffffffff when the slot carries
+// no vertex (idx < 0 or weight 0); value = c * 4 + s.  A stable sort by key
+// then groups the slots by vertex in contact order.
+__global__ void k_self_slots(int nc, const SelfContact* __restrict__ sc, uint32_t* __restrict__ keys,
+                             int32_t* __restrict__ vals) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * nc) return;
+  const int c = t >> 2, s = t & 3;
+  const int idx = sc[c].idx[s];
+  keys[t] = (idx >= 0 && sc[c].w[s] != 0.0f) ? (uint32_t)idx : 0xffffffffu;
+  vals[t] = t;
+}
+__global__ void k_count_keys(int count, int n, const uint32_t* __restrict__ keys, int32_t* __restrict__ hist) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const uint32_t k = keys[t];
+  if (k < (uint32_t)n) atomicAdd(&hist[k], 1);
+}
+
+// Ap_i += sum over i's self-contact slots (contact order) of the blocks
+// d2 (w_i n)(w_j n)^T, entries below barrier_tol dropped, times p_j for the
+// contact's other vertices j; plus the per-workgroup partial of p . (that sum).
+__global__ void k_self_couple(int n, SelfView sv, float barrier_tol, const PcgStatus* __restrict__ st,
+                              const float* __restrict__ p, float* __restrict__ Ap, double* __restrict__ part) {
+  if (st->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double pap = 0.0;
+  if (i < n && sv.ptr[i] < sv.ptr[i + 1]) {
+    float a[3] = {Ap[3 * i], Ap[3 * i + 1], Ap[3 * i + 2]};
+    float cp[3] = {0.0f, 0.0f, 0.0f};
+    for (int e = sv.ptr[i]; e < sv.ptr[i + 1]; ++e) {
+      const int code = sv.code[e], c = code >> 2, si = code & 3;
+      const SelfTerm& t = sv.st[c];
+      if (!t.active || t.d2 == 0.0f) continue;
+      const SelfContact& C = sv.sc[c];
+      const f3 nn = mk(t.nx, t.ny, t.nz);
+      const f3 gi = C.w[si] * nn;
+      for (int sj = 0; sj < 4; ++sj) {
+        if (sj == si || C.w[sj] == 0.0f || C.idx[sj] < 0) continue;
+        const f3 gj = C.w[sj] * nn;
+        float b[9];
+        bool any = false;
+        for (int k = 0; k < 9; ++k) {
+          const float val = t.d2 * (comp(gi, k / 3) * comp(gj, k % 3));
+          b[k] = fabsf(val) < barrier_tol ? 0.0f : val;
+          any = any || b[k] != 0.0f;
+        }
+        if (!any) continue;
+        const int j = C.idx[sj];
+        for (int r = 0; r < 3; ++r) {
+          float tt = 0.0f;
+          for (int q = 0; q < 3; ++q) tt += b[3 * r + q] * p[3 * j + q];
+          a[r] += tt;
+          cp[r] += tt;
+        }
+      }
+    }
+    for (int r = 0; r < 3; ++r) {
+      Ap[3 * i + r] = a[r];
+      pap += (double)p[3 * i + r] * (double)cp[r];
+    }
+  }
+  __shared__ double lds[4];
+  const double sblk = block_reduce(pap, lds, [](double u, double v) { return u + v; });
+  if (threadIdx.x == 0) part[blockIdx.x] = sblk;
+}
+
+// Line-search feasibility of the self contacts (LineSearch::is_feasible,
+// line_search.cpp:70-132, with ccd_point_triangle / ccd_edge_edge :163-247):
+// for trial t (alpha = 2^-t) the contact fails if its 10-sample CCD finds a
+// distance below 1e-6 at a sample time t' < 1.  Bit t of the AND-reduced mask
+// is set when every contact passes trial t.
+__global__ void k_ls_self(int nc, const SelfContact* __restrict__ sc, const float* __restrict__ x,
+                          const float* __restrict__ d, float ext, uint32_t* __restrict__ mask) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t m = 0xFFFFFu;
+  if (c < nc) {
+    const SelfContact& C = sc[c];
+    f3 x0[4], ed[4];
+    for (int k = 0; k < 4; ++k) {
+      x0[k] = ld3(x, C.idx[k]);
+      const f3 dk = ld3(d, C.idx[k]);
+      ed[k] = mk(ext * dk.x, ext * dk.y, ext * dk.z);
+    }
+    float alpha = 1.0f;
+    m = 0u;
+    for (int tr = 0; tr < 20; ++tr) {
+      f3 x1[4];
+      bool still = true;
+      for (int k = 0; k < 4; ++k) {
+        x1[k] = x0[k] + alpha * ed[k];
+        still = still && (double)sqn(x1[k] - x0[k]) < 1e-12;
+      }
+      bool ok = true;
+      if (!still)
+        for (int smp = 1; smp <= 10; ++smp) {
+          const float tt = (float)smp / 10.0f;
+          f3 q[4];
+          for (int k = 0; k < 4; ++k) q[k] = x0[k] + tt * (x1[k] - x0[k]);
+          float dist;
+          if (C.type == 0) {
+            dist = point_triangle(q[0], q[1], q[2], q[3]).dist;
+          } else {
+            f3 nn, wp, wq;
+            edge_edge(q[0], q[1], q[2], q[3], dist, nn, wp, wq);
+          }
+          if ((double)dist < 1e-6) {
+            ok = tt >= 1.0f;
+            break;
+          }
+        }
+      if (ok) m |= 1u << tr;
+      alpha *= 0.5f;
+    }
+  }
+  __shared__ uint32_t lds[4];
+  const uint32_t r = block_reduce(m, lds, [](uint32_t a, uint32_t b) { return a & b; });
+  if (threadIdx.x == 0 && r != 0xFFFFFu) atomicAnd(mask, r);
 }
 
 }  // namespace lbvh

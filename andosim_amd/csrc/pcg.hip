@@ -465,7 +465,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const floa
                              const int32_t* __restrict__ straddle, const float* __restrict__ Pinv,
                              float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
                              const double* __restrict__ part_pAp, int nwg_r, const double* __restrict__ rz_hist,
-                             double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st) {
+                             double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st,
+                             const double* __restrict__ part_pAp2, int n2) {
   if (st->done) return;
   __shared__ double ldsd[4];
   __shared__ float ldsf[4];
@@ -490,6 +491,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const floa
   for (int e = 0; e < 9; ++e) P[e] = Pinv[9 * (size_t)ii + e];
   const double rz_it = rz_hist[it];
   double pAp = reduce_sum_partials(part_pAp, nwg_r, ldsd);
+  if (part_pAp2) pAp += reduce_sum_partials(part_pAp2, n2, ldsd);
   if (fabs(pAp) < 1e-16) {
     if (threadIdx.x == 0) {
       st->failed = 1;

@@ -1083,6 +1083,7 @@ template <class R> struct Sim {
     }
     if (P.enable_friction && P.mu > R(0))
       for (const auto& c : contacts) {
+        if (contact_mode == 1 && c.type != RIGID_PT) continue;  // corrected mode: rigid contacts only
         R kf; V3<R> tan;
         if (!friction_data(c, kf, tan)) continue;
         for (int k = 0; k < 3; ++k) g[3 * c.idx[0] + k] += kf * tan[k];
@@ -1228,6 +1229,7 @@ template <class R> struct Sim {
     }
     if (P.enable_friction && P.mu > R(0))
       for (const auto& c : contacts) {
+        if (contact_mode == 1 && c.type != RIGID_PT) continue;
         R kf; V3<R> tan;
         if (!friction_data(c, kf, tan)) continue;
         for (int r = 0; r < 3; ++r)

@@ -73,3 +73,61 @@ def test_faithful_mode_has_no_self_contacts():
     g = native.GpuSim(jittered_stack())
     cg, cnt = g.contacts()
     assert cnt == 0
+
+
+def sync_state(o, g):
+    g.set_state(o.get("x").astype(np.float32), o.get("v").astype(np.float32))
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def resting_stack(gap=8e-5, z0=6e-4):
+    """Two jittered sheets, the lower one 0.57-0.63 mm above the ground wall,
+    the upper one `gap` above it: PT and EE contacts that change step to step."""
+    sc = jittered_stack(9, gap, 11, 2, 0.013, 3e-5)
+    sc.verts[:, 2] += np.float32(z0 - 0.2)
+    sc.walls = [(np.array([0, 0, 1], np.float32), 0.0, 1e-3)]
+    return sc
+
+
+@pytest.mark.parametrize("gap", [8e-5, 3e-5])
+def test_corrected_linearize_bit_exact(gap):
+    """Gradient and system matrix with self-contact barriers (FP64 island off):
+    bit-identical to the FP32 oracle in corrected mode."""
+    sc = resting_stack(gap)
+    o = OracleSim(sc, 32)
+    o.set_contact_mode(True)
+    g = native.GpuSim(sc, fp64_near=False)
+    g.set_mode(fp64_near=False, corrected=True)
+    sync_state(o, g)
+    x = o.get("x").astype(np.float32)
+    v = o.get("v").astype(np.float32)
+    xhat = (x + np.float32(sc.params["dt"]) * v).astype(np.float32)
+    go, Ho = o.linearize(xhat)
+    gg, Hg = g.linearize(xhat)
+    assert len(o.contacts()) > 0
+    assert np.array_equal(gg.reshape(-1), go.astype(np.float32).reshape(-1))
+    assert np.array_equal(Hg, Ho.astype(np.float32))
+
+
+def test_corrected_step_parity():
+    """Substeps from identical states with self contacts active: positions
+    within 1e-6 relative of the FP32 oracle (same contacts, same line-search
+    CCD, PCG to the same tolerance including the coupling blocks)."""
+    sc = resting_stack(8e-5)
+    o = OracleSim(sc, 32)
+    o.set_contact_mode(True)
+    errs = []
+    for k in range(8):
+        g = native.GpuSim(sc)
+        g.set_mode(corrected=True)
+        sync_state(o, g)
+        o.substep()
+        g.substep()
+        errs.append(rel_err(g.get("x"), o.get("x")))
+        assert g.stats()["contacts"] == o.stats()["contacts"]
+    assert max(errs) < 1e-6, errs
