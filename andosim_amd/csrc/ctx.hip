@@ -1537,6 +1537,46 @@ int ando_pcg(ando_ctx* c, int32_t num_rows, const int32_t* row_ptr, const int32_
   });
 }
 
+int ando_count_intersections(ando_ctx* c, const ando_rigid_body* bodies, int32_t num_bodies, int32_t* edge_triangle,
+                              int32_t* inside_rigid) {
+  return guarded([&] {
+    need(c && c->have_state, ANDO_ESTATE, "state has not been set");
+    need(num_bodies == 0 || bodies != nullptr, ANDO_EINVAL, "bodies is null");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int E = (int)c->edges_h.size() / 2;
+    int32_t et = 0, in = 0;
+    if (E > 0 && c->m > 0) {
+      lbvh_build(c, c->tri_tree, c->m, 3, c->d_tris.p);
+      c->sc_err.ensure(1);
+      HIPCHK(hipMemsetAsync(c->sc_err.p, 0, sizeof(uint32_t), s));
+      c->sc_count.ensure((size_t)std::max(c->n, E) + 1);
+      c->sc_ptr.ensure((size_t)std::max(c->n, E) + 2);
+      const LbvhTree& T = c->tri_tree;
+      hipLaunchKernelGGL(ando::lbvh::k_count_edge_tri, dim3(nblk(E)), dim3(kBlock), 0, s, E, (const float*)c->x.p,
+                         (const int32_t*)c->d_edges.p, (const int32_t*)c->d_tris.p, T.P, (const int32_t*)T.left.p,
+                         (const int32_t*)T.right.p, (const float*)T.nodes.p, (const float*)T.table.p,
+                         (const int32_t*)T.vals.p, c->sc_count.p, c->sc_err.p);
+      et = exclusive_scan(c, E, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
+      uint32_t err = 0;
+      HIPCHK(hipMemcpyAsync(&err, c->sc_err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+      sync(c);
+      if (err) throw Fail(ANDO_EDEVICE, "BVH traversal stack exceeded its depth");
+    }
+    if (num_bodies > 0) {
+      upload_bodies(c, bodies, num_bodies);
+      DetectBodies D = detect_bodies(c, bodies);
+      c->sc_count.ensure((size_t)std::max(c->n, E) + 1);
+      c->sc_ptr.ensure((size_t)std::max(c->n, E) + 2);
+      hipLaunchKernelGGL(ando::lbvh::k_count_inside, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n,
+                         (const float*)c->x.p, num_bodies, D, c->sc_count.p);
+      in = exclusive_scan(c, c->n, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
+    }
+    if (edge_triangle) *edge_triangle = et;
+    if (inside_rigid) *inside_rigid = in;
+  });
+}
+
 int32_t ando_nn_num_weights(void) { return ando::nn::kNumWeights; }
 
 int ando_set_precond_weights(ando_ctx* c, const float* weights, int32_t count) {

@@ -649,5 +649,90 @@ __global__ void k_ls_self(int nc, const SelfContact* __restrict__ sc, const floa
   if (threadIdx.x == 0 && r != 0xFFFFFu) atomicAnd(mask, r);
 }
 
+// ---------------------------------------------------------------------------
+// Intersection validator (new; the reference's CollisionValidator counts
+// gap < 0 on unsigned distances and never sees a penetration, F12).
+// ---------------------------------------------------------------------------
+
+// Sign of the volume of (a, b, c, d), in double on the float coordinates.
+__device__ inline int orient3(f3 a, f3 b, f3 c, f3 d) {
+  const double ax = (double)a.x - d.x, ay = (double)a.y - d.y, az = (double)a.z - d.z;
+  const double bx = (double)b.x - d.x, by = (double)b.y - d.y, bz = (double)b.z - d.z;
+  const double cx = (double)c.x - d.x, cy = (double)c.y - d.y, cz = (double)c.z - d.z;
+  const double v = ax * (by * cz - bz * cy) - ay * (bx * cz - bz * cx) + az * (bx * cy - by * cx);
+  return v > 0.0 ? 1 : (v < 0.0 ? -1 : 0);
+}
+
+// Segment p q properly crosses triangle a b c: p and q strictly on opposite
+// sides of the plane and the line through them strictly inside the three
+// edges (degenerate touching contacts are not counted).
+__device__ inline bool segment_crosses_triangle(f3 p, f3 q, f3 a, f3 b, f3 c) {
+  const int sp = orient3(a, b, c, p), sq = orient3(a, b, c, q);
+  if (sp == 0 || sq == 0 || sp == sq) return false;
+  const int s1 = orient3(p, q, a, b), s2 = orient3(p, q, b, c), s3 = orient3(p, q, c, a);
+  return s1 != 0 && s1 == s2 && s2 == s3;
+}
+
+// Count of (mesh edge, triangle) pairs sharing no vertex where the edge
+// crosses the triangle; one thread per edge over the triangle BVH.
+__global__ void k_count_edge_tri(int E, const float* __restrict__ x, const int32_t* __restrict__ edges,
+                                 const int32_t* __restrict__ tris, int P, const int32_t* __restrict__ left,
+                                 const int32_t* __restrict__ right, const float* __restrict__ nodes,
+                                 const float* __restrict__ leaves, const int32_t* __restrict__ order,
+                                 int32_t* __restrict__ count, uint32_t* __restrict__ err) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int a0 = edges[2 * e], a1 = edges[2 * e + 1];
+  const f3 p = ld3(x, a0), q = ld3(x, a1);
+  Box bq;
+  bq.lo[0] = fminf(p.x, q.x); bq.lo[1] = fminf(p.y, q.y); bq.lo[2] = fminf(p.z, q.z);
+  bq.hi[0] = fmaxf(p.x, q.x); bq.hi[1] = fmaxf(p.y, q.y); bq.hi[2] = fmaxf(p.z, q.z);
+  int k = 0;
+  const bool ok = traverse(P, bq, left, right, nodes, leaves, [&](int s) {
+    const int t = order[s];
+    const int a = tris[3 * t], b = tris[3 * t + 1], c = tris[3 * t + 2];
+    if (a == a0 || a == a1 || b == a0 || b == a1 || c == a0 || c == a1) return;
+    if (segment_crosses_triangle(p, q, ld3(x, a), ld3(x, b), ld3(x, c))) ++k;
+  });
+  if (!ok) atomicOr(err, 1u);
+  count[e] = k;
+}
+
+// Cloth vertices inside a closed rigid surface: generalized winding number
+// (sum of signed solid angles / 4 pi, Van Oosterom-Strackee, in double) above
+// 1/2.  One thread per vertex, all bodies' world triangles.
+__global__ void k_count_inside(int n, const float* __restrict__ x, int nbod, DetectBodies D,
+                               int32_t* __restrict__ count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const f3 p = ld3(x, i);
+  int inside = 0;
+  for (int b = 0; b < nbod; ++b) {
+    const DetectBody& B = D.b[b];
+    double w = 0.0;
+    for (int t = 0; t < B.nt; ++t) {
+      const int* T = B.tris + 3 * t;
+      double a[3], bb[3], c[3];
+      for (int k = 0; k < 3; ++k) {
+        a[k] = (double)B.world[3 * T[0] + k] - comp(p, k);
+        bb[k] = (double)B.world[3 * T[1] + k] - comp(p, k);
+        c[k] = (double)B.world[3 * T[2] + k] - comp(p, k);
+      }
+      const double la = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+      const double lb = sqrt(bb[0] * bb[0] + bb[1] * bb[1] + bb[2] * bb[2]);
+      const double lc = sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+      const double det = a[0] * (bb[1] * c[2] - bb[2] * c[1]) - a[1] * (bb[0] * c[2] - bb[2] * c[0]) +
+                         a[2] * (bb[0] * c[1] - bb[1] * c[0]);
+      const double ab = a[0] * bb[0] + a[1] * bb[1] + a[2] * bb[2];
+      const double bc = bb[0] * c[0] + bb[1] * c[1] + bb[2] * c[2];
+      const double ca = c[0] * a[0] + c[1] * a[1] + c[2] * a[2];
+      const double den = la * lb * lc + ab * lc + bc * la + ca * lb;
+      w += 2.0 * atan2(det, den);
+    }
+    if (fabs(w) / (4.0 * 3.141592653589793) > 0.5) ++inside;
+  }
+  count[i] = inside;
+}
+
 }  // namespace lbvh
 }  // namespace ando

@@ -20,7 +20,8 @@ ABI_SYMBOLS = [
     "ando_lumped_masses", "ando_set_previous_positions",
     "ando_set_state", "ando_get_state", "ando_set_pins", "ando_set_walls", "ando_set_mode", "ando_apply_gravity",
     "ando_step",
-    "ando_get_stats", "ando_compute_contacts", "ando_get_pattern", "ando_linearize", "ando_pcg",
+    "ando_get_stats", "ando_compute_contacts", "ando_count_intersections", "ando_get_pattern", "ando_linearize",
+    "ando_pcg",
     "ando_nn_num_weights", "ando_set_precond_weights", "ando_trainer_create", "ando_trainer_destroy",
     "ando_trainer_add_system", "ando_trainer_init_weights", "ando_trainer_set_weights", "ando_trainer_get_weights",
     "ando_trainer_forward", "ando_trainer_loss_grad", "ando_trainer_train", "ando_trainer_solve",
@@ -102,6 +103,7 @@ def load():
     L.ando_compute_contacts.argtypes = [C.c_void_p, C.POINTER(RigidBody), C.c_int32, C.POINTER(Contact), C.c_int32,
                                         ip]
     L.ando_get_pattern.argtypes = [C.c_void_p, ip, ip, ip]
+    L.ando_count_intersections.argtypes = [C.c_void_p, C.POINTER(RigidBody), C.c_int32, ip, ip]
     L.ando_linearize.argtypes = [C.c_void_p, C.POINTER(Params), fp, C.POINTER(RigidBody), C.c_int32, fp, fp]
     L.ando_pcg.argtypes = [C.c_void_p, C.c_int32, ip, ip, fp, fp, fp, C.c_float, C.c_int32, ip, ip]
     dp = C.POINTER(C.c_double)
@@ -346,6 +348,12 @@ class GpuSim:
         cnt = C.c_int32(0)
         check(self.L.ando_compute_contacts(self.h, self.bodies, self.nbodies, buf, cap, C.byref(cnt)))
         return [buf[i] for i in range(min(cnt.value, cap))], cnt.value
+
+    def count_intersections(self):
+        """(edge-triangle crossings, cloth vertices inside rigid bodies)."""
+        et, ins = C.c_int32(0), C.c_int32(0)
+        check(self.L.ando_count_intersections(self.h, self.bodies, self.nbodies, C.byref(et), C.byref(ins)))
+        return et.value, ins.value
 
     def set_precond_weights(self, w):
         w = _f(w).ravel()

@@ -253,7 +253,31 @@ def cyl257(seed: int = 4, s: int = 257, n_around: int = 48, n_along: int = 24) -
     return Scene("cyl257", V, T, dict(BASE_MATERIAL), base_params(), [], [], rig, steps=300)
 
 
-WORKLOADS = {"drop21": drop21, "sphere129": sphere129, "stack65x8": stack65x8, "cyl257": cyl257}
+def stack_resting(seed: int = 5, s: int = 65, sheets: int = 8, gap: float = 8e-5, jitter: float = 3e-5,
+                  z0: float = 6e-4) -> Scene:
+    """Dense self-contact: `sheets` s x s sheets (1 m) stacked `gap` apart in one
+    mesh, each shifted U(-1,1) cm and jittered U(-jitter, jitter) in z, the
+    lowest z0 above a ground wall.  Starts in contact, so the corrected mode's
+    PT/EE barriers act from the first substep (the falling stack's sheets all
+    fall at the same speed and, with the explicit gravity pre-step, pass the
+    ground together before they touch)."""
+    rng = SplitMix64(0x5E57 + seed)
+    Vs, Ts = [], []
+    for k in range(sheets):
+        V, T = grid(s, 1.0, z0 + k * gap)
+        V[:, 0] += rng.uniform(-0.01, 0.01)
+        V[:, 1] += rng.uniform(-0.01, 0.01)
+        V[:, 2] += rng.uniforms(len(V), -jitter, jitter)
+        Ts.append(T + k * s * s)
+        Vs.append(V)
+    V = _f32(np.concatenate(Vs))
+    T = np.ascontiguousarray(np.concatenate(Ts).astype(np.int32))
+    walls = [(np.array([0, 0, 1], np.float32), 0.0, 1e-3)]
+    return Scene("stack_resting", V, T, dict(BASE_MATERIAL), base_params(), [], walls, [], steps=60)
+
+
+WORKLOADS = {"drop21": drop21, "sphere129": sphere129, "stack65x8": stack65x8, "cyl257": cyl257,
+             "stack_resting": stack_resting}
 
 
 def small_sheet(s: int = 9, seed: int = 7, pins: int = 2, wall: bool = True, rigid: bool = False,

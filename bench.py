@@ -77,6 +77,41 @@ def measured_traffic(workload: str, n: int, nb: int, kernel_prefix: str):
     return None
 
 
+def corrected_contact_leg(steps: int):
+    """Dense self contact: 8 sheets of 65x65 stacked 0.08 mm apart (jittered
+    +-0.03 mm, so they start interpenetrating in places) just above a ground
+    wall, run for `steps` substeps in faithful mode (self broad phase empty, as
+    the reference) and in corrected mode (LBVH self PT/EE contacts and
+    barriers); steps/s, contacts per step, PCG iterations and the intersection
+    validator at the end of each run."""
+    from andosim_amd import native, scenes
+
+    out = {"workload": "stack_resting: 8 sheets 65x65 (33,800 v, 65,536 tris, 98,304 edges) 0.08 mm apart, "
+                       "jitter 0.03 mm, ground wall", "steps": steps}
+    for mode in ("faithful", "corrected"):
+        sc = scenes.stack_resting()
+        sim = native.GpuSim(sc)
+        sim.set_mode(fp64_near=True, corrected=(mode == "corrected"))
+        t = 0.0
+        contacts, pcg = [], []
+        done = 0
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            sim.substep()
+            t += time.perf_counter() - t0
+            st = sim.stats()
+            contacts.append(st["contacts"])
+            pcg.append(st["pcg_iterations"])
+            done += 1
+            if not finite_state(sim):
+                break
+        et, ins = sim.count_intersections()
+        out[mode] = {"steps_per_s": round(done / t, 2), "steps_run": done, "finite": finite_state(sim),
+                     "contacts_per_step": contacts, "pcg_iters_per_step": round(float(np.mean(pcg)), 1),
+                     "edge_triangle_intersections_at_end": et}
+    return out
+
+
 def learned_comparison(sim, sc, window, restore, replay, a, pers):
     """Learned preconditioner vs plain block-Jacobi PCG on the same window.
 
@@ -165,6 +200,8 @@ def main(argv=None):
     ap.add_argument("--pcg", choices=["persistent", "two-kernel"], default="two-kernel")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner comparison")
     ap.add_argument("--train-steps", type=int, default=8000)
+    ap.add_argument("--no-corrected", action="store_true", help="skip the corrected-contact leg")
+    ap.add_argument("--corrected-steps", type=int, default=20)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU design is described "
@@ -283,6 +320,10 @@ def main(argv=None):
             roof["pcg_kernels_share_of_step"] = round((t_launch[0] + t_launch[1]) * 1e-3 / ev_elapsed, 3)
         roof["event_pass_ms_per_step"] = round(1e3 * ev_elapsed / len(ev_recs), 3)
 
+    corrected = None
+    if not a.no_corrected:
+        corrected = corrected_contact_leg(a.corrected_steps)
+
     learned = None
     if not a.no_learned:
         learned = learned_comparison(sim, sc, window, restore, replay, a, pers)
@@ -341,6 +382,7 @@ def main(argv=None):
         "pcg_iters_per_step": round(pcg_iters / len(recs), 2),
         "newton_iters_per_step": round(newton / len(recs), 2),
         "learned_precond": learned,
+        "corrected_contact": corrected,
         "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu_single,
     }
     print(json.dumps(out), flush=True)

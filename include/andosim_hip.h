@@ -201,6 +201,15 @@ int ando_get_stats(ando_ctx* ctx, ando_stats* out);
 int ando_compute_contacts(ando_ctx* ctx, const ando_rigid_body* bodies, int32_t num_bodies, ando_contact* out,
                           int32_t capacity, int32_t* count);
 
+/* Intersection validator (new; the reference's CollisionValidator only counts
+ * gap < 0 on unsigned distances, collision_validator.cpp:64, so it never sees
+ * a penetration).  edge_triangle: (mesh edge, triangle) pairs sharing no
+ * vertex where the edge properly crosses the triangle (double-precision
+ * orientation tests, through the LBVH); inside_rigid: cloth vertices whose
+ * generalized winding number w.r.t. a rigid body's closed surface exceeds 1/2. */
+int ando_count_intersections(ando_ctx* ctx, const ando_rigid_body* bodies, int32_t num_bodies, int32_t* edge_triangle,
+                             int32_t* inside_rigid);
+
 int ando_get_pattern(ando_ctx* ctx, int32_t* row_ptr, int32_t* cols, int32_t* num_blocks);
 int ando_linearize(ando_ctx* ctx, const ando_params* params, const float* x_target, const ando_rigid_body* bodies,
                    int32_t num_bodies, float* gradient, float* values);

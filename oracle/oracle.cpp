@@ -1634,6 +1634,65 @@ void orc_destroy(void* h) {
 }
 
 // material: E, nu, density, thickness, bending
+// Intersection validator, brute force: (edge, triangle) pairs sharing no
+// vertex where the edge properly crosses the triangle (orientation tests in
+// double on the float coordinates), and cloth vertices with generalized
+// winding number above 1/2 w.r.t. a rigid body's surface.
+static int orient3d_sign(const double* a, const double* b, const double* c, const double* d) {
+  const double ax = a[0] - d[0], ay = a[1] - d[1], az = a[2] - d[2];
+  const double bx = b[0] - d[0], by = b[1] - d[1], bz = b[2] - d[2];
+  const double cx = c[0] - d[0], cy = c[1] - d[1], cz = c[2] - d[2];
+  const double v = ax * (by * cz - bz * cy) - ay * (bx * cz - bz * cx) + az * (bx * cy - by * cx);
+  return v > 0.0 ? 1 : (v < 0.0 ? -1 : 0);
+}
+
+int orc_count_intersections(void* h, int* edge_tri, int* inside) {
+  OSim* s = (OSim*)h;
+  int et = 0, in = 0;
+  DISPATCH(s, {
+    auto P3 = [&](int i, double* o) { for (int k = 0; k < 3; ++k) o[k] = (double)(float)S.x[3 * i + k]; };
+    for (const auto& e : S.edges)
+      for (const auto& t : S.tris) {
+        if (t[0] == e[0] || t[0] == e[1] || t[1] == e[0] || t[1] == e[1] || t[2] == e[0] || t[2] == e[1]) continue;
+        double p[3], q[3], a[3], b[3], c[3];
+        P3(e[0], p); P3(e[1], q); P3(t[0], a); P3(t[1], b); P3(t[2], c);
+        const int sp = orient3d_sign(a, b, c, p), sq = orient3d_sign(a, b, c, q);
+        if (sp == 0 || sq == 0 || sp == sq) continue;
+        const int s1 = orient3d_sign(p, q, a, b), s2 = orient3d_sign(p, q, b, c), s3 = orient3d_sign(p, q, c, a);
+        if (s1 != 0 && s1 == s2 && s2 == s3) ++et;
+      }
+    for (int i = 0; i < S.n; ++i) {
+      double p[3];
+      P3(i, p);
+      for (const auto& rb : S.rigids) {
+        auto W = rb.world();
+        double w = 0.0;
+        for (const auto& t : rb.tris) {
+          double a[3], b[3], c[3];
+          for (int k = 0; k < 3; ++k) {
+            a[k] = (double)(float)W[t[0]][k] - p[k];
+            b[k] = (double)(float)W[t[1]][k] - p[k];
+            c[k] = (double)(float)W[t[2]][k] - p[k];
+          }
+          const double la = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+          const double lb = std::sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+          const double lc = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+          const double det = a[0] * (b[1] * c[2] - b[2] * c[1]) - a[1] * (b[0] * c[2] - b[2] * c[0]) +
+                             a[2] * (b[0] * c[1] - b[1] * c[0]);
+          const double den = la * lb * lc + (a[0] * b[0] + a[1] * b[1] + a[2] * b[2]) * lc +
+                             (b[0] * c[0] + b[1] * c[1] + b[2] * c[2]) * la +
+                             (c[0] * a[0] + c[1] * a[1] + c[2] * a[2]) * lb;
+          w += 2.0 * std::atan2(det, den);
+        }
+        if (std::fabs(w) / (4.0 * 3.141592653589793) > 0.5) ++in;
+      }
+    }
+  });
+  *edge_tri = et;
+  *inside = in;
+  return 0;
+}
+
 // Contact mode: 0 faithful, 1 corrected (brute-force self PT/EE with the
 // reference's intended predicate, PT and EE barriers in the energy).
 int orc_set_contact_mode(void* h, int mode, double pt_box, double keep_gap) {

@@ -76,6 +76,7 @@ def lib():
         L.orc_set_contacts_pt.argtypes = [C.c_void_p, C.c_int, _ip]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_contact_mode.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
+        L.orc_count_intersections.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_get_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -160,6 +161,12 @@ class OracleSim:
         """Faithful (self broad phase empty) or corrected (brute-force PT/EE with
         the intended predicate, both barriers in the energy)."""
         self.L.orc_set_contact_mode(self.h, 1 if corrected else 0, pt_box, keep_gap)
+
+    def count_intersections(self):
+        """Brute-force (edge-triangle crossings, vertices inside rigid bodies)."""
+        et, ins = C.c_int(0), C.c_int(0)
+        self.L.orc_count_intersections(self.h, C.byref(et), C.byref(ins))
+        return et.value, ins.value
 
     def stats(self) -> dict:
         out = np.zeros(11)

@@ -131,3 +131,34 @@ def test_corrected_step_parity():
         errs.append(rel_err(g.get("x"), o.get("x")))
         assert g.stats()["contacts"] == o.stats()["contacts"]
     assert max(errs) < 1e-6, errs
+
+
+def crossing_sheets(s=12):
+    """A horizontal sheet and a sheet tilted 30 degrees through it."""
+    V1, T1 = scenes.grid(s, 0.2, 0.2)
+    V2, T2 = scenes.grid(s, 0.2, 0.0)
+    th = np.radians(30.0)
+    y, z = V2[:, 1].copy(), V2[:, 2].copy()
+    V2[:, 1] = y * np.cos(th) - z * np.sin(th) + 0.003
+    V2[:, 2] = 0.2 + y * np.sin(th) + z * np.cos(th)
+    V = np.concatenate([V1, V2]).astype(np.float32)
+    T = np.concatenate([T1, T2 + s * s]).astype(np.int32)
+    return scenes.Scene("cross", V, T, dict(scenes.BASE_MATERIAL), scenes.base_params(), [], [], [])
+
+
+def sheet_through_sphere():
+    sc = scenes.small_sheet(15, pins=0, wall=False, rigid=True, z=0.3)
+    sc.rigids[0].verts = (sc.rigids[0].verts + np.array([0, 0, 0.105], np.float32)).astype(np.float32)
+    return sc
+
+
+@pytest.mark.parametrize("make", [crossing_sheets, crumpled, sheet_through_sphere, lambda: jittered_stack()])
+def test_intersection_counts_match_brute_force(make):
+    sc = make()
+    o = OracleSim(sc, 32)
+    g = native.GpuSim(sc)
+    assert g.count_intersections() == o.count_intersections()
+    if make is crossing_sheets:
+        assert o.count_intersections()[0] > 0
+    if make is sheet_through_sphere:
+        assert o.count_intersections()[1] > 0

@@ -405,3 +405,25 @@ def test_corrected_step_uses_self_barrier():
     f = orc.OracleSim(sc, 64)
     f.substep()
     assert f.stats()["contacts"] == 0
+
+
+def test_intersection_validator_known_answers():
+    """Triangle B stands through triangle A: B's two slanted edges cross A,
+    A's edges miss B; a vertex at the sphere's centre is inside, one far away
+    is not (winding number 1 and 0)."""
+    V = np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0],            # A in z = 0
+                  [0, 0, -1], [0.1, 0, 1], [-0.1, 0, 1],         # B through A's interior
+                  [0, 0, 5], [3, 3, 3]], np.float32)             # free vertices
+    T = np.array([[0, 1, 2], [3, 4, 5]], np.int32)
+    SV, ST = scenes.icosphere(2, 0.5, (0.0, 0.0, 5.0))
+    sc = scenes.Scene("x", V, T, dict(scenes.BASE_MATERIAL), scenes.base_params(), [], [],
+                      [scenes.Rigid(SV.astype(np.float32), ST, 1e12)])
+    o = orc.OracleSim(sc, 64)
+    et, ins = o.count_intersections()
+    assert et == 2
+    assert ins == 1          # vertex 6 at the centre; vertex 7 and the triangles are outside
+    # Separated triangles: no crossing.
+    V2 = V.copy()
+    V2[3:6, 2] += 3.0
+    o2 = orc.OracleSim(scenes.Scene("y", V2, T, dict(scenes.BASE_MATERIAL), scenes.base_params(), [], [], []), 64)
+    assert o2.count_intersections() == (0, 0)
