@@ -213,6 +213,8 @@ struct ando_ctx {
   DBuf<uint32_t> sv_keys, sv_keys2;
   DBuf<int32_t> sv_vals, sv_vals2, sv_counts, sv_offsets, sv_sums, sv_total, sv_hist, sv_ptr;
   DBuf<double> part_pAp2;
+  DBuf<int32_t> ccol;
+  DBuf<float> cval;
   bool sv_ready = false;
   float barrier_tol = 1e-12f;
   int learned = 0;         // learned preconditioner + warm start in the step's PCG
@@ -434,7 +436,7 @@ void detect_rigid(ando_ctx* c, const ando_rigid_body* bodies, int nbod, float dt
   int nbs = (c->n + 1023) / 1024;
   hipLaunchKernelGGL(k_scan_local, dim3(nbs), dim3(kBlock), 0, s, c->n, c->rc_count.p, c->rc_ptr.p,
                      c->scan_sums.p);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, s, nbs, c->scan_sums.p, c->scan_total.p);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, nbs, c->scan_sums.p, c->scan_total.p);
   hipLaunchKernelGGL(k_scan_add, dim3(nblk(c->n + 1)), dim3(kBlock), 0, s, c->n, c->rc_ptr.p, c->scan_sums.p,
                      c->scan_total.p);
   HIPCHK(hipGetLastError());
@@ -457,7 +459,7 @@ int32_t exclusive_scan(ando_ctx* c, int n, const int32_t* in, int32_t* out, DBuf
   sums.ensure((size_t)nbs + 1);
   total.ensure(1);
   hipLaunchKernelGGL(k_scan_local, dim3(nbs), dim3(kBlock), 0, s, n, in, out, sums.p);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, s, nbs, sums.p, total.p);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, nbs, sums.p, total.p);
   hipLaunchKernelGGL(k_scan_add, dim3(nblk(n + 1)), dim3(kBlock), 0, s, n, out, (const int32_t*)sums.p,
                      (const int32_t*)total.p);
   HIPCHK(hipGetLastError());
@@ -587,6 +589,12 @@ void detect_self(ando_ctx* c) {
   c->sc_total = npt + nee;
 }
 
+ando::SelfView self_view(ando_ctx* c) {
+  ando::SelfView v{nullptr, nullptr, nullptr, nullptr};
+  if (c->sv_ready) v = ando::SelfView{c->sv_ptr.p, c->sv_vals.p, c->sc.p, c->sterm.p};
+  return v;
+}
+
 // Step-constant terms of the self contacts and their vertex view (after
 // detect_self; needs the elastic-only blocks from ensure_const).
 void prepare_self(ando_ctx* c, const StepParams& P) {
@@ -611,15 +619,15 @@ void prepare_self(ando_ctx* c, const StepParams& P) {
   HIPCHK(hipMemsetAsync(c->sv_hist.p, 0, sizeof(int32_t) * c->n, s));
   hipLaunchKernelGGL(k_count_keys, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, c->n, (const uint32_t*)c->sv_keys.p,
                      c->sv_hist.p);
-  exclusive_scan(c, c->n, c->sv_hist.p, c->sv_ptr.p, c->sv_sums, c->sv_total);
-  c->part_pAp2.ensure((size_t)nblk(c->n) + 1);
+  const int entries = exclusive_scan(c, c->n, c->sv_hist.p, c->sv_ptr.p, c->sv_sums, c->sv_total);
+  c->part_pAp2.ensure((size_t)nblk(c->n, 32) + 1);
   c->sv_ready = true;
-}
-
-ando::SelfView self_view(ando_ctx* c) {
-  ando::SelfView v{nullptr, nullptr, nullptr, nullptr};
-  if (c->sv_ready) v = ando::SelfView{c->sv_ptr.p, c->sv_vals.p, c->sc.p, c->sterm.p};
-  return v;
+  c->ccol.ensure(3 * (size_t)entries + 1);
+  c->cval.ensure(27 * (size_t)entries + 1);
+  if (entries)
+    hipLaunchKernelGGL(k_self_blocks, dim3(nblk(3LL * entries)), dim3(kBlock), 0, s, entries, self_view(c),
+                       c->barrier_tol, c->ccol.p, c->cval.p);
+  HIPCHK(hipGetLastError());
 }
 
 // PCG on the matrix in c->H (or a caller's buffers) with b = -g; result in c->d.
@@ -786,15 +794,16 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
         }
       }
       if (couple)
-        hipLaunchKernelGGL(ando::lbvh::k_self_couple, dim3(nwg_v), dim3(kBlock), 0, s, n, *sv, c->barrier_tol,
-                           (const PcgStatus*)c->pcg_status.p, (const float*)pcur, c->Ap.p, c->part_pAp2.p);
+        hipLaunchKernelGGL(ando::lbvh::k_self_spmv, dim3(nblk(n, 32)), dim3(kBlock), 0, s, n, sv->ptr,
+                           (const int32_t*)c->ccol.p, (const float*)c->cval.p, (const PcgStatus*)c->pcg_status.p,
+                           (const float*)pcur, c->Ap.p, c->part_pAp2.p);
       hipExtLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(kBlock), 0, s, e2, e3, 0, n, it, (const float*)pcur,
                             (const float*)c->Ap.p, (const float*)(blk ? sm->extra.p : nullptr),
                             (const int32_t*)(blk ? sm->straddle.p : nullptr), Pc, xsol, c->r.p,
                             c->z.p,
                             (const double*)c->part_pAp.p, nwg_r, (const double*)c->rz_hist.p, c->part_rz.p,
                             c->part_max.p, c->pcg_status.p, (const double*)(couple ? c->part_pAp2.p : nullptr),
-                            nwg_v);
+                            nblk(n, 32));
     }
     HIPCHK(hipGetLastError());
     chunk = std::min(chunk * 2, 64);

@@ -311,26 +311,69 @@ __global__ void __launch_bounds__(kBlock) k_rigid_detect(int n, int pass, const 
 // ---------------------------------------------------------------------------
 // Exclusive scan of int32 counts (three phases, 1024 items per workgroup).
 // ---------------------------------------------------------------------------
-__global__ void k_scan_local(int n, const int32_t* __restrict__ in, int32_t* __restrict__ out,
-                             int32_t* __restrict__ sums) {
-  __shared__ int32_t s[1024];
-  int base = blockIdx.x * 1024;
-  for (int k = threadIdx.x; k < 1024; k += blockDim.x) s[k] = (base + k < n) ? in[base + k] : 0;
+// Exclusive scan of one value per thread over the workgroup (blockDim a
+// multiple of 64, at most 1024): wave-inclusive scans by shuffles, then the
+// wave totals; returns the exclusive prefix and the workgroup total.
+__device__ inline int32_t block_excl_scan(int32_t v, int32_t* lds, int32_t& total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int32_t inc = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int32_t u = __shfl_up(inc, off, 64);
+    if (lane >= off) inc += u;
+  }
+  if (lane == 63) lds[wid] = inc;
   __syncthreads();
   if (threadIdx.x == 0) {
     int32_t acc = 0;
-    for (int k = 0; k < 1024; ++k) { int32_t t = s[k]; s[k] = acc; acc += t; }
-    sums[blockIdx.x] = acc;
+    for (int w = 0; w < nw; ++w) {
+      const int32_t t = lds[w];
+      lds[w] = acc;
+      acc += t;
+    }
+    lds[nw] = acc;
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < 1024; k += blockDim.x)
-    if (base + k < n) out[base + k] = s[k];
+  const int32_t excl = inc - v + lds[wid];
+  total = lds[nw];
+  __syncthreads();
+  return excl;
 }
+
+// Exclusive scan of 1024-element blocks (4 per thread at 256 threads);
+// sums[block] = the block's total.
+__global__ void k_scan_local(int n, const int32_t* __restrict__ in, int32_t* __restrict__ out,
+                             int32_t* __restrict__ sums) {
+  __shared__ int32_t lds[17];
+  const int per = 1024 / blockDim.x;
+  const int base = blockIdx.x * 1024 + threadIdx.x * per;
+  int32_t v[4] = {0, 0, 0, 0};
+  int32_t tsum = 0;
+  for (int k = 0; k < per; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    tsum += v[k];
+  }
+  int32_t total = 0;
+  int32_t acc = block_excl_scan(tsum, lds, total);
+  for (int k = 0; k < per; ++k) {
+    if (base + k < n) out[base + k] = acc;
+    acc += v[k];
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+// Exclusive scan of the block sums in place (one workgroup, chunks of
+// blockDim with a running carry); *total = the grand total.
 __global__ void k_scan_sums(int nblk, int32_t* __restrict__ sums, int32_t* __restrict__ total) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int32_t acc = 0;
-  for (int k = 0; k < nblk; ++k) { int32_t t = sums[k]; sums[k] = acc; acc += t; }
-  *total = acc;
+  __shared__ int32_t lds[17];
+  int32_t carry = 0;
+  for (int base = 0; base < nblk; base += blockDim.x) {
+    const int k = base + threadIdx.x;
+    const int32_t v = k < nblk ? sums[k] : 0;
+    int32_t t = 0;
+    const int32_t e = block_excl_scan(v, lds, t);
+    if (k < nblk) sums[k] = carry + e;
+    carry += t;
+  }
+  if (threadIdx.x == 0) *total = carry;
 }
 __global__ void k_scan_add(int n, int32_t* __restrict__ out, const int32_t* __restrict__ sums,
                            const int32_t* __restrict__ total) {

@@ -548,48 +548,70 @@ __global__ void k_count_keys(int count, int n, const uint32_t* __restrict__ keys
   if (k < (uint32_t)n) atomicAdd(&hist[k], 1);
 }
 
-// Ap_i += sum over i's self-contact slots (contact order) of the blocks
-// d2 (w_i n)(w_j n)^T, entries below barrier_tol dropped, times p_j for the
-// contact's other vertices j; plus the per-workgroup partial of p . (that sum).
-__global__ void k_self_couple(int n, SelfView sv, float barrier_tol, const PcgStatus* __restrict__ st,
-                              const float* __restrict__ p, float* __restrict__ Ap, double* __restrict__ part) {
+// The blocks coupling two vertices, materialised once per step: for vertex
+// view entry e (contact c, slot si on row vertex i) and the contact's other
+// slots sj in slot order, block 3 e + jj = d2 (w_si n)(w_sj n)^T with entries
+// below barrier_tol dropped (zero when the term is inactive or the slot has
+// no vertex); column = that slot's vertex (the row vertex when it has none).
+__global__ void k_self_blocks(int entries, SelfView sv, float barrier_tol, int32_t* __restrict__ ccol,
+                              float* __restrict__ cval) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * entries) return;
+  const int e = t / 3, jj = t - 3 * e;
+  const int code = sv.code[e], c = code >> 2, si = code & 3;
+  const int sj = jj < si ? jj : jj + 1;
+  const SelfContact& C = sv.sc[c];
+  const SelfTerm& st = sv.st[c];
+  float b[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int col = C.idx[si];
+  if (st.active && st.d2 != 0.0f && C.w[sj] != 0.0f && C.idx[sj] >= 0) {
+    const f3 nn = mk(st.nx, st.ny, st.nz);
+    const f3 gi = C.w[si] * nn, gj = C.w[sj] * nn;
+    for (int k = 0; k < 9; ++k) {
+      const float val = st.d2 * (comp(gi, k / 3) * comp(gj, k % 3));
+      b[k] = fabsf(val) < barrier_tol ? 0.0f : val;
+    }
+    col = C.idx[sj];
+  }
+  ccol[t] = col;
+  for (int k = 0; k < 9; ++k) cval[9 * (size_t)t + k] = b[k];
+}
+
+// Ap_i += sum of row i's coupling blocks times p (rows = 3 * vertex-view
+// ranges), 8 lanes per row as in the mesh SpMV, plus the per-workgroup partial
+// of p_i . (that sum).  Skipped once the solve is done.
+__global__ void __launch_bounds__(256) k_self_spmv(int n, const int32_t* __restrict__ vptr,
+                                                   const int32_t* __restrict__ ccol, const float* __restrict__ cval,
+                                                   const PcgStatus* __restrict__ st, const float* __restrict__ p,
+                                                   float* __restrict__ Ap, double* __restrict__ part) {
   if (st->done) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 7;
+  const int i = blockIdx.x * 32 + (threadIdx.x >> 3);
+  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+  int beg = 0, end = 0;
+  if (i < n) {
+    beg = 3 * vptr[i];
+    end = 3 * vptr[i + 1];
+  }
+  for (int b = beg + lane; b < end; b += 8) {
+    const int j = ccol[b];
+    const float* B = cval + 9 * (size_t)b;
+    const float p0 = p[3 * j], p1 = p[3 * j + 1], p2 = p[3 * j + 2];
+    s0 = s0 + (B[0] * p0 + (B[1] * p1 + B[2] * p2));
+    s1 = s1 + (B[3] * p0 + (B[4] * p1 + B[5] * p2));
+    s2 = s2 + (B[6] * p0 + (B[7] * p1 + B[8] * p2));
+  }
+  for (int off = 4; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 8);
+    s1 += __shfl_xor(s1, off, 8);
+    s2 += __shfl_xor(s2, off, 8);
+  }
   double pap = 0.0;
-  if (i < n && sv.ptr[i] < sv.ptr[i + 1]) {
-    float a[3] = {Ap[3 * i], Ap[3 * i + 1], Ap[3 * i + 2]};
-    float cp[3] = {0.0f, 0.0f, 0.0f};
-    for (int e = sv.ptr[i]; e < sv.ptr[i + 1]; ++e) {
-      const int code = sv.code[e], c = code >> 2, si = code & 3;
-      const SelfTerm& t = sv.st[c];
-      if (!t.active || t.d2 == 0.0f) continue;
-      const SelfContact& C = sv.sc[c];
-      const f3 nn = mk(t.nx, t.ny, t.nz);
-      const f3 gi = C.w[si] * nn;
-      for (int sj = 0; sj < 4; ++sj) {
-        if (sj == si || C.w[sj] == 0.0f || C.idx[sj] < 0) continue;
-        const f3 gj = C.w[sj] * nn;
-        float b[9];
-        bool any = false;
-        for (int k = 0; k < 9; ++k) {
-          const float val = t.d2 * (comp(gi, k / 3) * comp(gj, k % 3));
-          b[k] = fabsf(val) < barrier_tol ? 0.0f : val;
-          any = any || b[k] != 0.0f;
-        }
-        if (!any) continue;
-        const int j = C.idx[sj];
-        for (int r = 0; r < 3; ++r) {
-          float tt = 0.0f;
-          for (int q = 0; q < 3; ++q) tt += b[3 * r + q] * p[3 * j + q];
-          a[r] += tt;
-          cp[r] += tt;
-        }
-      }
-    }
-    for (int r = 0; r < 3; ++r) {
-      Ap[3 * i + r] = a[r];
-      pap += (double)p[3 * i + r] * (double)cp[r];
-    }
+  if (i < n && lane == 0 && end > beg) {
+    Ap[3 * i] += s0;
+    Ap[3 * i + 1] += s1;
+    Ap[3 * i + 2] += s2;
+    pap = (double)p[3 * i] * s0 + (double)p[3 * i + 1] * s1 + (double)p[3 * i + 2] * s2;
   }
   __shared__ double lds[4];
   const double sblk = block_reduce(pap, lds, [](double u, double v) { return u + v; });
