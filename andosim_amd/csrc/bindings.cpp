@@ -8,8 +8,11 @@
 // keeps its positions and velocities resident on the device; host copies are
 // refreshed lazily when Python reads them.
 //
-// Host-only utilities (EnergyTracker, CollisionValidator, AdaptiveTimestep,
-// Elasticity/barrier statics) are plain C++ restatements of
+// EnergyTracker and CollisionValidator.compute_metrics (the add-on's
+// per-frame consumers) run on the device when the state is resident there
+// (for the validator: when the list is the state's last contact listing);
+// otherwise they, AdaptiveTimestep and the Elasticity/barrier statics are
+// plain C++ restatements of
 // src/core/{energy_tracker,collision_validator,adaptive_timestep,elasticity,barrier}.cpp.
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -158,6 +161,8 @@ struct Session {
   uint64_t mesh_version = ~0ull;
   const void* constraints = nullptr;
   uint64_t constraints_version = ~0ull;
+  uint64_t listing = 0;     // id of the last contact listing of this session
+  int32_t listed = -1;      // its length
   ~Session() {
     if (ctx) ando_destroy(ctx);
   }
@@ -231,6 +236,11 @@ struct ContactPair {
   V3 normal{0, 0, 0}, witness_p{0, 0, 0}, witness_q{0, 0, 0};
   int vertex_count = 0;
   int rigid_body_index = -1;
+  // Set when the contact comes from a device listing: which listing, and its
+  // position in it.  A list that is exactly that listing is validated on the
+  // device without being read back.
+  uint64_t listing = 0;
+  int32_t slot = -1;
 };
 
 // RigidBody (rigid_body.h:16-86, rigid_body.cpp:20-184) on the host.
@@ -680,10 +690,61 @@ struct CollisionMetrics {
   }
 };
 
+// The device holds this state (and this mesh, when one is given) with
+// nothing newer on the host.
+bool device_resident(const Mesh* mesh, const State& state) {
+  if (!state.session || !state.session->ctx || state.host_dirty) return false;
+  if (mesh && (state.session->mesh != mesh || state.session->mesh_version != mesh->version)) return false;
+  return true;
+}
+
+// EnergyTracker::compute's sums on the device (ando_energy_diagnostics).
+bool gpu_energy(const Mesh* mesh, const State& state, ando_energy& e) {
+  if (!device_resident(mesh, state)) return false;
+  abi_check(ando_energy_diagnostics(state.session->ctx, &e));
+  return true;
+}
+
+// CollisionValidator::compute_metrics on the device when `cs` is exactly the
+// session's last contact listing and the device still holds that table and
+// the current velocities.
+bool gpu_metrics(State& state, const std::vector<ContactPair>& cs, float gap_max, bool ccd, CollisionMetrics& M) {
+  if (cs.empty() || !device_resident(nullptr, state)) return false;
+  const Session& s = *state.session;
+  if (s.listing == 0 || (int64_t)cs.size() != (int64_t)s.listed) return false;
+  for (size_t i = 0; i < cs.size(); ++i)
+    if (cs[i].listing != s.listing || cs[i].slot != (int32_t)i) return false;
+  ando_collision_metrics r{};
+  const int rc = ando_contact_metrics(s.ctx, nullptr, 0, gap_max, ccd ? 1 : 0, &r);
+  if (rc == ANDO_ESTATE) return false;  // a later detection replaced the table
+  abi_check(rc);
+  M.ccd_enabled = ccd;
+  M.num_point_triangle = r.num_point_triangle;
+  M.num_edge_edge = r.num_edge_edge;
+  M.num_wall = r.num_wall;
+  M.num_total_contacts = r.num_total_contacts;
+  M.min_gap = r.min_gap;
+  M.max_gap = r.max_gap;
+  M.avg_gap = r.avg_gap;
+  M.num_penetrations = r.num_penetrations;
+  M.max_penetration = r.max_penetration;
+  M.avg_penetration = r.avg_penetration;
+  M.has_major_penetration = r.has_major_penetration != 0;
+  M.has_tunneling = r.has_tunneling != 0;
+  M.is_stable = r.num_penetrations == 0;
+  M.num_ccd_contacts = r.num_ccd_contacts;
+  M.num_broad_phase_contacts = r.num_broad_phase_contacts;
+  M.ccd_effectiveness = r.ccd_effectiveness;
+  M.max_relative_velocity = r.max_relative_velocity;
+  M.avg_relative_velocity = r.avg_relative_velocity;
+  return true;
+}
+
 CollisionMetrics compute_metrics(const Mesh&, State& state, const std::vector<ContactPair>& cs, float gap_max,
                                  bool ccd) {
-  state.pull();
   CollisionMetrics M;
+  if (gpu_metrics(state, cs, gap_max, ccd, M)) return M;
+  state.pull();
   M.compute_from_contacts(cs, gap_max, ccd);
   std::vector<float> rel;
   auto vel = [&](int32_t i) {
@@ -747,15 +808,20 @@ float at_cfl(float vmax, float min_edge, float safety) {
   if (vmax < kStaticVel) return 1.0f;
   return safety * min_edge / vmax;
 }
-float at_next_dt(const std::vector<float>& v, const Mesh& mesh, float cur, float dmin, float dmax, float safety) {
-  float vmax = at_max_velocity(v);
+// compute_next_dt (adaptive_timestep.cpp:8-44) once max |v| is known; the
+// minimum edge length comes from min_edge() only when it is needed.
+template <class MinEdge>
+float at_next_dt_from(float vmax, MinEdge min_edge, float cur, float dmin, float dmax, float safety) {
   if (vmax < kStaticVel) return dmax;
-  float me = at_min_edge(mesh);
+  float me = min_edge();
   if (me < kMinEdge) me = kMinEdge;
   float cfl = at_cfl(vmax, me, safety);
   float target = std::clamp(cfl, dmin, dmax);
   float next = target < cur ? target : std::min(target, cur * 1.5f);
   return std::clamp(next, dmin, dmax);
+}
+float at_next_dt(const std::vector<float>& v, const Mesh& mesh, float cur, float dmin, float dmax, float safety) {
+  return at_next_dt_from(at_max_velocity(v), [&] { return at_min_edge(mesh); }, cur, dmin, dmax, safety);
 }
 
 std::vector<float> flat_floats(py::handle obj) {
@@ -798,10 +864,16 @@ std::vector<ContactPair> gpu_contacts(const Mesh& mesh, State& state, py::object
   std::vector<ando_contact> raw(std::max(count, 1));
   abi_check(ando_compute_contacts(state.session->ctx, bodies.data(), (int32_t)bodies.size(), raw.data(), count,
                                   &count));
+  static uint64_t listings = 0;
+  Session& sess = *state.session;
+  sess.listing = ++listings;
+  sess.listed = count;
   std::vector<ContactPair> cs(count);
   for (int i = 0; i < count; ++i) {
     const ando_contact& r = raw[i];
     ContactPair& c = cs[i];
+    c.listing = sess.listing;
+    c.slot = i;
     c.type = (ContactType)r.type;
     c.idx0 = r.idx[0]; c.idx1 = r.idx[1]; c.idx2 = r.idx[2]; c.idx3 = r.idx[3];
     c.gap = r.gap;
@@ -1210,15 +1282,26 @@ PYBIND11_MODULE(ando_barrier_core, m) {
       .def_static(
           "compute",
           [](const Mesh& mesh, State& state, const Constraints& constraints, const SimParams&) {
-            state.pull();
             EnergyDiagnostics d;
-            d.kinetic_energy = kinetic_energy(state);
-            d.elastic_energy = elastic_energy(mesh);
+            ando_energy e;
+            if (gpu_energy(&mesh, state, e)) {
+              d.kinetic_energy = (float)e.kinetic_energy;
+              d.elastic_energy = (float)e.elastic_energy;
+              d.linear_momentum = {(float)e.linear_momentum[0], (float)e.linear_momentum[1],
+                                   (float)e.linear_momentum[2]};
+              d.angular_momentum = {(float)e.angular_momentum[0], (float)e.angular_momentum[1],
+                                    (float)e.angular_momentum[2]};
+              d.max_velocity = (float)e.max_velocity;
+            } else {
+              state.pull();
+              d.kinetic_energy = kinetic_energy(state);
+              d.elastic_energy = elastic_energy(mesh);
+              d.linear_momentum = linear_momentum(state);
+              d.angular_momentum = angular_momentum(state);
+              d.max_velocity = max_velocity(state);
+            }
             d.barrier_energy = 0.0f;
             d.total_energy = (d.kinetic_energy + d.elastic_energy) + d.barrier_energy;
-            d.linear_momentum = linear_momentum(state);
-            d.angular_momentum = angular_momentum(state);
-            d.max_velocity = max_velocity(state);
             d.num_contacts = 0;
             d.num_pins = (int)constraints.num_active_pins();
             return d;
@@ -1228,6 +1311,8 @@ PYBIND11_MODULE(ando_barrier_core, m) {
       .def_static(
           "compute_kinetic_energy",
           [](State& s) {
+            ando_energy e;
+            if (gpu_energy(nullptr, s, e)) return (float)e.kinetic_energy;
             s.pull();
             return kinetic_energy(s);
           },
@@ -1235,6 +1320,10 @@ PYBIND11_MODULE(ando_barrier_core, m) {
       .def_static(
           "compute_linear_momentum",
           [](State& s) {
+            ando_energy e;
+            if (gpu_energy(nullptr, s, e))
+              return std::vector<float>{(float)e.linear_momentum[0], (float)e.linear_momentum[1],
+                                        (float)e.linear_momentum[2]};
             s.pull();
             V3 p = linear_momentum(s);
             return std::vector<float>{p.x, p.y, p.z};
@@ -1243,6 +1332,10 @@ PYBIND11_MODULE(ando_barrier_core, m) {
       .def_static(
           "compute_angular_momentum",
           [](State& s) {
+            ando_energy e;
+            if (gpu_energy(nullptr, s, e))
+              return std::vector<float>{(float)e.angular_momentum[0], (float)e.angular_momentum[1],
+                                        (float)e.angular_momentum[2]};
             s.pull();
             V3 p = angular_momentum(s);
             return std::vector<float>{p.x, p.y, p.z};
@@ -1251,6 +1344,8 @@ PYBIND11_MODULE(ando_barrier_core, m) {
       .def_static(
           "compute_max_velocity",
           [](State& s) {
+            ando_energy e;
+            if (gpu_energy(nullptr, s, e)) return (float)e.max_velocity;
             s.pull();
             return max_velocity(s);
           },
@@ -1310,8 +1405,24 @@ PYBIND11_MODULE(ando_barrier_core, m) {
           "compute_next_dt",
           [](py::object velocities, const Mesh& mesh, double current_dt, double dt_min, double dt_max,
              double safety) {
-            std::vector<float> v = flat_floats(velocities);
-            float dt = at_next_dt(v, mesh, (float)current_dt, (float)dt_min, (float)dt_max, (float)safety);
+            float dt;
+            if (py::isinstance<State>(velocities)) {
+              // Additive: the State itself may be passed; when it is resident on
+              // the device, both reductions run there and nothing is read back.
+              State& st = velocities.cast<State&>();
+              float vmax = 0.0f, me = 0.0f;
+              if (device_resident(&mesh, st)) {
+                abi_check(ando_cfl_inputs(st.session->ctx, &vmax, &me));
+                dt = at_next_dt_from(vmax, [&] { return me; }, (float)current_dt, (float)dt_min, (float)dt_max,
+                                     (float)safety);
+              } else {
+                st.pull();
+                dt = at_next_dt(st.velocities, mesh, (float)current_dt, (float)dt_min, (float)dt_max, (float)safety);
+              }
+            } else {
+              std::vector<float> v = flat_floats(velocities);
+              dt = at_next_dt(v, mesh, (float)current_dt, (float)dt_min, (float)dt_max, (float)safety);
+            }
             return std::clamp((double)dt, dt_min, dt_max);
           },
           py::arg("velocities"), py::arg("mesh"), py::arg("current_dt"), py::arg("dt_min"), py::arg("dt_max"),
@@ -1325,6 +1436,19 @@ PYBIND11_MODULE(ando_barrier_core, m) {
           "Compute CFL timestep from velocity and mesh resolution")
       .def_static("compute_min_edge_length", &at_min_edge, py::arg("mesh"), "Compute minimum edge length in mesh")
       .def_static(
-          "compute_max_velocity", [](py::object velocities) { return at_max_velocity(flat_floats(velocities)); },
+          "compute_max_velocity",
+          [](py::object velocities) {
+            if (py::isinstance<State>(velocities)) {
+              State& st = velocities.cast<State&>();
+              if (device_resident(nullptr, st)) {
+                float vmax = 0.0f, me = 0.0f;
+                abi_check(ando_cfl_inputs(st.session->ctx, &vmax, &me));
+                return vmax;
+              }
+              st.pull();
+              return at_max_velocity(st.velocities);
+            }
+            return at_max_velocity(flat_floats(velocities));
+          },
           py::arg("velocities"), "Compute maximum velocity magnitude");
 }

@@ -23,6 +23,7 @@
 #include "pcg.hip"
 #include "learned.hip"
 #include "lbvh.hip"
+#include "frame.hip"
 
 namespace ando {
 
@@ -220,6 +221,17 @@ struct ando_ctx {
   int learned = 0;         // learned preconditioner + warm start in the step's PCG
   nnh::Net nn;
   nnh::Work nnw;
+  // Per-frame consumers: partials, results, an uploaded contact list, and the
+  // generation of the contact table (bumped by every detection) that the last
+  // ando_compute_contacts listed.
+  DBuf<ando::EnergyAcc> fe_part;
+  DBuf<ando::MetricAcc> fm_part;
+  DBuf<ando_energy> fe_out;
+  DBuf<ando_collision_metrics> fm_out;
+  DBuf<ando_contact> fm_list;
+  DBuf<ando::ExtremaAcc> fx_part;
+  DBuf<float> fx_out;
+  uint64_t det_gen = 0, listed_gen = ~0ull;
   DBuf<uint32_t> pcg_barrier_ctr;
   int pcg_grid_cap = 0;    // co-resident workgroups (0 = not yet queried, -1 = no cooperative launch)
   std::vector<hipEvent_t> events;  // pool: start/stop pairs for timed launches
@@ -427,6 +439,7 @@ DetectBodies detect_bodies(ando_ctx* c, const ando_rigid_body* bodies) {
 
 // Rigid contacts at the current positions, ordered (vertex, body, triangle).
 void detect_rigid(ando_ctx* c, const ando_rigid_body* bodies, int nbod, float dt) {
+  ++c->det_gen;
   c->rc_total = 0;
   if (nbod == 0) return;
   hipStream_t s = c->stream;
@@ -533,6 +546,7 @@ void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims)
 void detect_self(ando_ctx* c) {
   using namespace ando::lbvh;
   hipStream_t s = c->stream;
+  ++c->det_gen;
   c->sc_pt = c->sc_total = 0;
   c->sv_ready = false;
   if (!c->corrected) return;
@@ -1416,6 +1430,7 @@ int ando_compute_contacts(ando_ctx* c, const ando_rigid_body* bodies, int32_t nu
     }
     detect_rigid(c, bodies, num_bodies, dt);
     detect_self(c);
+    c->listed_gen = c->det_gen;
     *count = c->sc_total + c->rc_total;
     if (!out || capacity <= 0) return;
     std::vector<RigidContact> hc(c->rc_total);
@@ -1583,6 +1598,96 @@ int ando_count_intersections(ando_ctx* c, const ando_rigid_body* bodies, int32_t
     }
     if (edge_triangle) *edge_triangle = et;
     if (inside_rigid) *inside_rigid = in;
+  });
+}
+
+int ando_energy_diagnostics(ando_ctx* c, ando_energy* out) {
+  return guarded([&] {
+    need(c && out, ANDO_EINVAL, "null argument");
+    need(c->have_state, ANDO_ESTATE, "state has not been set");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int G = std::max(1, std::min(ando::kFrameMaxBlocks, nblk(std::max(c->n, c->m), ando::kFrameBlock)));
+    c->fe_part.ensure((size_t)G);
+    c->fe_out.ensure(1);
+    hipLaunchKernelGGL(ando::k_frame_energy, dim3(G), dim3(ando::kFrameBlock), 0, s, c->n, c->m,
+                       (const float*)c->x.p, (const float*)c->v.p, (const float*)c->masses.p,
+                       (const int32_t*)c->d_tris.p, (const float*)c->mverts.p, (const float*)c->dm.p,
+                       (const float*)c->area.p, c->mat.thickness, c->mu, c->fe_part.p);
+    hipLaunchKernelGGL(ando::k_frame_energy_final, dim3(1), dim3(ando::kFrameBlock), 0, s, G,
+                       (const ando::EnergyAcc*)c->fe_part.p, c->fe_out.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, c->fe_out.p, sizeof(ando_energy), hipMemcpyDeviceToHost, s));
+    sync(c);
+  });
+}
+
+int ando_contact_metrics(ando_ctx* c, const ando_contact* contacts, int32_t count, float gap_max, int32_t ccd,
+                         ando_collision_metrics* out) {
+  return guarded([&] {
+    need(c && out, ANDO_EINVAL, "null argument");
+    need(c->have_state, ANDO_ESTATE, "state has not been set");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    int total = 0;
+    if (contacts) {
+      need(count >= 0, ANDO_EINVAL, "negative contact count");
+      // Indices the relative-velocity terms read (the reference would read out
+      // of bounds; here a bad index is an argument error).
+      for (int e = 0; e < count; ++e) {
+        const ando_contact& k = contacts[e];
+        const int used = k.type == 0 || k.type == 1 ? 4 : (k.type == 2 ? 1 : 0);
+        for (int q = 0; q < used; ++q)
+          need(k.idx[q] >= 0 && k.idx[q] < c->n, ANDO_EINVAL, "contact vertex index out of range");
+      }
+      total = count;
+      if (total) c->fm_list.upload(contacts, (size_t)total, s);
+    } else {
+      need(c->listed_gen == c->det_gen, ANDO_ESTATE,
+           "the contacts listed by ando_compute_contacts have been replaced by a later detection");
+      total = c->sc_total + c->rc_total;
+    }
+    const int G = std::max(1, std::min(ando::kFrameMaxBlocks, nblk(total, ando::kFrameBlock)));
+    c->fm_part.ensure((size_t)G);
+    c->fm_out.ensure(1);
+    const int flag = ccd ? 1 : 0;
+    if (contacts) {
+      ando::ListSource src{c->fm_list.p};
+      hipLaunchKernelGGL(ando::k_frame_metrics<ando::ListSource>, dim3(G), dim3(ando::kFrameBlock), 0, s, total, src,
+                         (const float*)c->v.p, gap_max, flag, c->fm_part.p);
+    } else {
+      ando::TableSource src{c->sc.p, c->sc_total, c->rc.p};
+      hipLaunchKernelGGL(ando::k_frame_metrics<ando::TableSource>, dim3(G), dim3(ando::kFrameBlock), 0, s, total,
+                         src, (const float*)c->v.p, gap_max, flag, c->fm_part.p);
+    }
+    hipLaunchKernelGGL(ando::k_frame_metrics_final, dim3(1), dim3(ando::kFrameBlock), 0, s, G,
+                       (const ando::MetricAcc*)c->fm_part.p, flag, c->fm_out.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, c->fm_out.p, sizeof(ando_collision_metrics), hipMemcpyDeviceToHost, s));
+    sync(c);
+  });
+}
+
+int ando_cfl_inputs(ando_ctx* c, float* max_velocity, float* min_edge_length) {
+  return guarded([&] {
+    need(c && max_velocity && min_edge_length, ANDO_EINVAL, "null argument");
+    need(c->have_state, ANDO_ESTATE, "state has not been set");
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int E = (int)c->edges_h.size() / 2;
+    const int G = std::max(1, std::min(ando::kFrameMaxBlocks, nblk(std::max(c->n, E), ando::kFrameBlock)));
+    c->fx_part.ensure((size_t)G);
+    c->fx_out.ensure(2);
+    hipLaunchKernelGGL(ando::k_frame_extrema, dim3(G), dim3(ando::kFrameBlock), 0, s, c->n, E, (const float*)c->v.p,
+                       (const int32_t*)c->d_edges.p, (const float*)c->mverts.p, c->fx_part.p);
+    hipLaunchKernelGGL(ando::k_frame_extrema_final, dim3(1), dim3(ando::kFrameBlock), 0, s, G,
+                       (const ando::ExtremaAcc*)c->fx_part.p, c->fx_out.p);
+    HIPCHK(hipGetLastError());
+    float out[2];
+    HIPCHK(hipMemcpyAsync(out, c->fx_out.p, sizeof(out), hipMemcpyDeviceToHost, s));
+    sync(c);
+    *max_velocity = out[0];
+    *min_edge_length = out[1];
   });
 }
 

@@ -25,6 +25,7 @@ ABI_SYMBOLS = [
     "ando_nn_num_weights", "ando_set_precond_weights", "ando_trainer_create", "ando_trainer_destroy",
     "ando_trainer_add_system", "ando_trainer_init_weights", "ando_trainer_set_weights", "ando_trainer_get_weights",
     "ando_trainer_forward", "ando_trainer_loss_grad", "ando_trainer_train", "ando_trainer_solve",
+    "ando_energy_diagnostics", "ando_contact_metrics", "ando_cfl_inputs",
 ]
 
 
@@ -57,6 +58,22 @@ class Contact(C.Structure):
     _fields_ = [("type", C.c_int32), ("idx", C.c_int32 * 4), ("rigid_body_index", C.c_int32),
                 ("vertex_count", C.c_int32), ("gap", C.c_float), ("normal", C.c_float * 3),
                 ("witness_p", C.c_float * 3), ("witness_q", C.c_float * 3), ("weights", C.c_float * 4)]
+
+
+class Energy(C.Structure):
+    _fields_ = [("kinetic_energy", C.c_double), ("elastic_energy", C.c_double),
+                ("linear_momentum", C.c_double * 3), ("angular_momentum", C.c_double * 3),
+                ("max_velocity", C.c_double)]
+
+
+class CollisionMetrics(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in (
+        "num_point_triangle", "num_edge_edge", "num_wall", "num_total_contacts", "num_penetrations",
+        "num_ccd_contacts", "num_broad_phase_contacts", "num_relative_velocities", "has_major_penetration",
+        "has_tunneling")] + [(k, C.c_float) for k in (
+            "min_gap", "max_gap", "avg_gap", "max_penetration", "avg_penetration", "ccd_effectiveness",
+            "max_relative_velocity", "avg_relative_velocity")] + [(k, C.c_double) for k in (
+                "sum_gap", "sum_penetration", "sum_relative_velocity")]
 
 
 MAX_RECORDED = 64
@@ -119,6 +136,10 @@ def load():
     L.ando_trainer_loss_grad.argtypes = [C.c_void_p, C.c_int32, dp, fp]
     L.ando_trainer_train.argtypes = [C.c_void_p, C.c_int32, C.c_float, C.c_uint64, dp]
     L.ando_trainer_solve.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_int32, ip, ip]
+    L.ando_energy_diagnostics.argtypes = [C.c_void_p, C.POINTER(Energy)]
+    L.ando_contact_metrics.argtypes = [C.c_void_p, C.POINTER(Contact), C.c_int32, C.c_float, C.c_int32,
+                                       C.POINTER(CollisionMetrics)]
+    L.ando_cfl_inputs.argtypes = [C.c_void_p, fp, fp]
     _lib = L
     return L
 
@@ -348,6 +369,32 @@ class GpuSim:
         cnt = C.c_int32(0)
         check(self.L.ando_compute_contacts(self.h, self.bodies, self.nbodies, buf, cap, C.byref(cnt)))
         return [buf[i] for i in range(min(cnt.value, cap))], cnt.value
+
+    def energy(self) -> dict:
+        """EnergyTracker::compute on the device (sums in double)."""
+        e = Energy()
+        check(self.L.ando_energy_diagnostics(self.h, C.byref(e)))
+        return {"kinetic_energy": e.kinetic_energy, "elastic_energy": e.elastic_energy,
+                "linear_momentum": np.array(e.linear_momentum[:]),
+                "angular_momentum": np.array(e.angular_momentum[:]), "max_velocity": e.max_velocity}
+
+    def contact_metrics(self, gap_max: float, ccd: bool, contacts=None) -> dict:
+        """CollisionValidator::compute_metrics on the device: over the contacts
+        the last contacts() call listed (contacts=None), or over a given list
+        of Contact structures."""
+        m = CollisionMetrics()
+        if contacts is None:
+            check(self.L.ando_contact_metrics(self.h, None, 0, gap_max, int(bool(ccd)), C.byref(m)))
+        else:
+            arr = (Contact * max(len(contacts), 1))(*contacts)
+            check(self.L.ando_contact_metrics(self.h, arr, len(contacts), gap_max, int(bool(ccd)), C.byref(m)))
+        return {k: getattr(m, k) for k, _ in CollisionMetrics._fields_}
+
+    def cfl_inputs(self):
+        """(max |v|, min edge length) reduced on the device."""
+        mv, me = C.c_float(0.0), C.c_float(0.0)
+        check(self.L.ando_cfl_inputs(self.h, C.byref(mv), C.byref(me)))
+        return mv.value, me.value
 
     def count_intersections(self):
         """(edge-triangle crossings, cloth vertices inside rigid bodies)."""

@@ -23,6 +23,10 @@
  *                                                                       src/py/bindings.cpp:469-494
  *   ando_compute_contacts  Integrator::compute_contacts                 src/core/integrator.cpp:238-244,
  *                                                                       src/py/bindings.cpp:495-509
+ *   ando_energy_diagnostics  EnergyTracker::compute                   src/core/energy_tracker.cpp:7-76
+ *   ando_contact_metrics   CollisionValidator::compute_metrics          src/core/collision_validator.cpp:133-157
+ *   ando_cfl_inputs        AdaptiveTimestep::compute_max_velocity +
+ *                          compute_min_edge_length                      src/core/adaptive_timestep.cpp:82-150
  *   ando_get_stats         (new, additive) solver statistics; the reference exposes none
  *                          (src/core/pcg_solver.h:26-27 returns bool only)
  *   ando_linearize         one Newton linearisation: Integrator::compute_gradient +
@@ -209,6 +213,66 @@ int ando_compute_contacts(ando_ctx* ctx, const ando_rigid_body* bodies, int32_t 
  * generalized winding number w.r.t. a rigid body's closed surface exceeds 1/2. */
 int ando_count_intersections(ando_ctx* ctx, const ando_rigid_body* bodies, int32_t num_bodies, int32_t* edge_triangle,
                              int32_t* inside_rigid);
+
+/* Per-frame diagnostics the add-on reads after each frame
+ * (blender_addon/operators.py:909), from the resident state:
+ * EnergyTracker::compute, src/core/energy_tracker.cpp:7-76, with the elastic
+ * energy of Elasticity::compute_energy, src/core/elasticity.cpp:5-14,115-126
+ * (F from the mesh vertices).  Each per-vertex and per-face term is the
+ * reference's float expression; the sums are taken in double (the reference
+ * accumulates in float). */
+typedef struct {
+  double kinetic_energy;
+  double elastic_energy;
+  double linear_momentum[3];
+  double angular_momentum[3];
+  double max_velocity;
+} ando_energy;
+int ando_energy_diagnostics(ando_ctx* ctx, ando_energy* out);
+
+/* CollisionValidator::compute_metrics (src/core/collision_validator.cpp:7-157,
+ * 178-230; called at blender_addon/operators.py:973) with the resident
+ * velocities.  contacts == NULL: over the contacts the last
+ * ando_compute_contacts listed, without listing them again (ANDO_ESTATE if a
+ * step or another detection has replaced them since).  Otherwise over the
+ * caller's `count` contacts.  Counts, minima and maxima are exact; the sums are
+ * taken in double and the averages are float quotients of the rounded sums.
+ * Rigid contacts (type 3) count in the totals and gap statistics but, as in
+ * the reference, carry no relative velocity. */
+typedef struct {
+  int32_t num_point_triangle;
+  int32_t num_edge_edge;
+  int32_t num_wall;
+  int32_t num_total_contacts;
+  int32_t num_penetrations;
+  int32_t num_ccd_contacts;
+  int32_t num_broad_phase_contacts;
+  int32_t num_relative_velocities;
+  int32_t has_major_penetration;
+  int32_t has_tunneling;
+  float min_gap;
+  float max_gap;
+  float avg_gap;
+  float max_penetration;
+  float avg_penetration;
+  float ccd_effectiveness;
+  float max_relative_velocity;
+  float avg_relative_velocity;
+  double sum_gap;
+  double sum_penetration;
+  double sum_relative_velocity;
+} ando_collision_metrics;
+int ando_contact_metrics(ando_ctx* ctx, const ando_contact* contacts, int32_t count, float gap_max,
+                         int32_t ccd_enabled, ando_collision_metrics* out);
+
+/* AdaptiveTimestep's two reductions on the resident state, without reading the
+ * velocities back (called through AdaptiveTimestep.compute_next_dt at
+ * blender_addon/operators.py:872-880): max_velocity as
+ * AdaptiveTimestep::compute_max_velocity (src/core/adaptive_timestep.cpp:136-150)
+ * over the device velocities, min_edge_length as compute_min_edge_length
+ * (:82-134) over the mesh edges and mesh vertices.  Both are exact (a maximum
+ * and a minimum of the reference's float expressions). */
+int ando_cfl_inputs(ando_ctx* ctx, float* max_velocity, float* min_edge_length);
 
 int ando_get_pattern(ando_ctx* ctx, int32_t* row_ptr, int32_t* cols, int32_t* num_blocks);
 int ando_linearize(ando_ctx* ctx, const ando_params* params, const float* x_target, const ando_rigid_body* bodies,

@@ -617,6 +617,53 @@ template <class R> struct Sim {
       for (int c = 0; c < 2; ++c) F[r][c] = D[r][0] * Mi[0][c] + D[r][1] * Mi[1][c];
   }
 
+  // EnergyTracker::compute (energy_tracker.cpp:7-76) on caller-given x, v, m
+  // with the elastic energy of Elasticity::compute_energy (elasticity.cpp:5-14,
+  // 115-126; F from the mesh vertices).  Order of out: kinetic, elastic,
+  // linear momentum (3), angular momentum (3), max |v|.  out[0..8] accumulate
+  // as the reference does (in R; the kinetic sum as R(double(ke) + 0.5*m*|v|^2));
+  // out[9..17] sum the same per-item R terms in double.
+  void energy(const double* xd, const double* vd, const double* md, double* out) const {
+    R ke = R(0), el = R(0), vmax = R(0);
+    V3<R> lm{R(0), R(0), R(0)}, am{R(0), R(0), R(0)};
+    double ke_d = 0.0, el_d = 0.0, lm_d[3] = {0, 0, 0}, am_d[3] = {0, 0, 0};
+    for (int i = 0; i < n; ++i) {
+      V3<R> vi{(R)vd[3 * i], (R)vd[3 * i + 1], (R)vd[3 * i + 2]};
+      V3<R> ri{(R)xd[3 * i], (R)xd[3 * i + 1], (R)xd[3 * i + 2]};
+      R mi = (R)md[i];
+      R vsq = sqn3(vi);
+      double term = 0.5 * (double)mi * (double)vsq;
+      ke = (R)((double)ke + term);
+      ke_d += term;
+      V3<R> p{mi * vi[0], mi * vi[1], mi * vi[2]};
+      V3<R> c = cross3(ri, p);
+      for (int k = 0; k < 3; ++k) {
+        lm[k] = lm[k] + p[k];
+        am[k] = am[k] + c[k];
+        lm_d[k] += (double)p[k];
+        am_d[k] += (double)c[k];
+      }
+      vmax = std::max(vmax, (R)std::sqrt(vsq));
+    }
+    R mu = mu_();
+    for (size_t f = 0; f < tris.size(); ++f) {
+      R F[2][2];
+      compute_F(mverts[tris[f][0]], mverts[tris[f][1]], mverts[tris[f][2]], f, F);
+      R k = area[f] * thick * mu;
+      R d00 = F[0][0] - R(1), d01 = F[0][1], d10 = F[1][0], d11 = F[1][1] - R(1);
+      R e = k * ((d00 * d00 + d10 * d10) + (d01 * d01 + d11 * d11));
+      el = el + e;
+      el_d += (double)e;
+    }
+    const double a[9] = {(double)ke, (double)el, (double)lm[0], (double)lm[1], (double)lm[2],
+                         (double)am[0], (double)am[1], (double)am[2], (double)vmax};
+    const double b[9] = {ke_d, el_d, lm_d[0], lm_d[1], lm_d[2], am_d[0], am_d[1], am_d[2], (double)vmax};
+    for (int k = 0; k < 9; ++k) {
+      out[k] = a[k];
+      out[9 + k] = b[k];
+    }
+  }
+
   // Elasticity::compute_gradient (elasticity.cpp:18-61).  Faithful: F from the
   // mesh buffer (mesh.vertices), which Integrator::step never updates.
   void elastic_gradient() {
@@ -1695,6 +1742,108 @@ int orc_count_intersections(void* h, int* edge_tri, int* inside) {
 
 // Contact mode: 0 faithful, 1 corrected (brute-force self PT/EE with the
 // reference's intended predicate, PT and EE barriers in the energy).
+int orc_energy(void* h, const double* x, const double* v, const double* m, double* out) {
+  OSim* s = (OSim*)h;
+  return guard([&] { DISPATCH(s, S.energy(x, v, m, out)); });
+}
+
+// CollisionValidator::compute_metrics (collision_validator.cpp:7-157) with
+// compute_relative_velocities (:178-230) on a contact list and velocities v
+// (3 per vertex).  type: 0 PT, 1 EE, 2 wall, 3 rigid; idx 4 per contact;
+// normal 3 per contact.  out: num_point_triangle, num_edge_edge, num_wall,
+// num_total, num_penetrations, num_ccd, num_broad_phase, num_rel_velocities,
+// has_major_penetration, has_tunneling, min_gap, max_gap, avg_gap,
+// max_penetration, avg_penetration, ccd_effectiveness, max_rel_velocity,
+// avg_rel_velocity (all as the reference computes them in R), then the gap,
+// penetration and relative-velocity sums of the same R terms in double.
+}  // extern "C"
+
+template <class R>
+static void contact_metrics(int count, const int* type, const int* idx, const double* gap, const double* nrm,
+                            const double* vel, double gap_max_d, int ccd, double* out) {
+  const R gap_max = (R)gap_max_d;
+  auto V = [&](int i) { return V3<R>{(R)vel[3 * i], (R)vel[3 * i + 1], (R)vel[3 * i + 2]}; };
+  int npt = 0, nee = 0, nwall = 0, npen = 0, nccd = 0, nbroad = 0, nrel = 0;
+  bool major = false, tunnel = false;
+  R min_gap = R(0), max_gap = R(0), avg_gap = R(0), max_pen = R(0), avg_pen = R(0), eff = R(0);
+  R max_rel = R(0), avg_rel = R(0);
+  double sgap_d = 0.0, spen_d = 0.0, srel_d = 0.0;
+  if (count > 0) {
+    min_gap = max_gap = (R)gap[0];
+    R sum_gap = R(0), sum_pen = R(0);
+    for (int e = 0; e < count; ++e) {
+      if (type[e] == 0) ++npt;
+      else if (type[e] == 1) ++nee;
+      else if (type[e] == 2) ++nwall;
+      R g = (R)gap[e];
+      min_gap = std::min(min_gap, g);
+      max_gap = std::max(max_gap, g);
+      sum_gap = sum_gap + g;
+      sgap_d += (double)g;
+      if (g < R(0)) {
+        ++npen;
+        R pen = -g;
+        max_pen = std::max(max_pen, pen);
+        sum_pen = sum_pen + pen;
+        spen_d += (double)pen;
+        if ((double)pen > 0.001) major = true;
+        if ((double)pen > 0.1 * (double)gap_max) tunnel = true;
+      }
+    }
+    avg_gap = sum_gap / (R)count;
+    avg_pen = npen > 0 ? sum_pen / (R)npen : R(0);
+    for (int e = 0; e < count; ++e) {
+      if (ccd && (double)(R)gap[e] < (double)gap_max * 0.5) ++nccd;
+      else ++nbroad;
+    }
+    eff = ccd ? (R)((double)((R)nccd / (R)count) * 100.0) : R(0);
+    R sum_rel = R(0);
+    for (int e = 0; e < count; ++e) {
+      const int* q = idx + 4 * e;
+      V3<R> n{(R)nrm[3 * e], (R)nrm[3 * e + 1], (R)nrm[3 * e + 2]};
+      R rel;
+      if (type[e] == 0) {
+        V3<R> vp = V(q[0]), a = V(q[1]), b = V(q[2]), d = V(q[3]);
+        V3<R> vt;
+        for (int k = 0; k < 3; ++k) vt[k] = (R)((double)((a[k] + b[k]) + d[k]) / 3.0);
+        rel = std::fabs(dot3(vp - vt, n));
+      } else if (type[e] == 1) {
+        V3<R> a = V(q[0]), b = V(q[1]), d = V(q[2]), f = V(q[3]);
+        V3<R> e0, e1;
+        for (int k = 0; k < 3; ++k) {
+          e0[k] = (R)((double)(a[k] + b[k]) / 2.0);
+          e1[k] = (R)((double)(d[k] + f[k]) / 2.0);
+        }
+        rel = std::fabs(dot3(e0 - e1, n));
+      } else if (type[e] == 2) {
+        rel = std::fabs(dot3(V(q[0]), n));
+      } else {
+        continue;
+      }
+      ++nrel;
+      max_rel = nrel == 1 ? rel : std::max(max_rel, rel);
+      sum_rel = sum_rel + rel;
+      srel_d += (double)rel;
+    }
+    avg_rel = nrel ? sum_rel / (R)nrel : R(0);
+  }
+  const double r[21] = {(double)npt, (double)nee, (double)nwall, (double)count, (double)npen, (double)nccd,
+                        (double)(count > 0 ? nbroad : 0), (double)nrel, major ? 1.0 : 0.0, tunnel ? 1.0 : 0.0,
+                        (double)min_gap, (double)max_gap, (double)avg_gap, (double)max_pen, (double)avg_pen,
+                        (double)eff, (double)max_rel, (double)avg_rel, sgap_d, spen_d, srel_d};
+  for (int k = 0; k < 21; ++k) out[k] = r[k];
+}
+
+extern "C" {
+
+int orc_contact_metrics(int prec, int count, const int* type, const int* idx, const double* gap, const double* nrm,
+                        const double* vel, double gap_max, int ccd, double* out) {
+  return guard([&] {
+    if (prec == 64) contact_metrics<double>(count, type, idx, gap, nrm, vel, gap_max, ccd, out);
+    else contact_metrics<float>(count, type, idx, gap, nrm, vel, gap_max, ccd, out);
+  });
+}
+
 int orc_set_contact_mode(void* h, int mode, double pt_box, double keep_gap) {
   OSim* s = (OSim*)h;
   DISPATCH(s, S.contact_mode = mode; S.pt_box = (decltype(S.pt_box))pt_box; S.keep_gap = (decltype(S.keep_gap))keep_gap);
@@ -1806,7 +1955,7 @@ int orc_get(void* h, int which, double* out) {
     });
   });
 }
-// which: 0 positions, 1 velocities
+// which: 0 positions, 1 velocities, 2 masses, 3 mesh vertices
 int orc_set(void* h, int which, const double* in) {
   OSim* s = (OSim*)h;
   return guard([&] {
@@ -1814,6 +1963,9 @@ int orc_set(void* h, int which, const double* in) {
       using R = std::remove_reference_t<decltype(S.E)>;
       if (which == 0) for (int i = 0; i < 3 * S.n; ++i) S.x[i] = (R)in[i];
       else if (which == 1) for (int i = 0; i < 3 * S.n; ++i) S.v[i] = (R)in[i];
+      else if (which == 2) for (int i = 0; i < S.n; ++i) S.m[i] = (R)in[i];
+      else if (which == 3)  // Mesh.vertices setter: the rest state is kept (mesh.cpp:122-124)
+        for (int i = 0; i < S.n; ++i) for (int k = 0; k < 3; ++k) S.mverts[i][k] = (R)in[3 * i + k];
       else throw std::invalid_argument("bad selector");
     });
   });

@@ -78,6 +78,8 @@ def lib():
         L.orc_set_contact_mode.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
         L.orc_count_intersections.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_get_threads.restype = C.c_int
+        L.orc_energy.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_contact_metrics.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -98,6 +100,36 @@ def _d(a):
 def _chk(rc):
     if rc != 0:
         raise RuntimeError(lib().orc_last_error().decode())
+
+
+ENERGY_KEYS = ["kinetic_energy", "elastic_energy", "linear_momentum", "angular_momentum", "max_velocity"]
+METRIC_KEYS = ["num_point_triangle", "num_edge_edge", "num_wall", "num_total_contacts", "num_penetrations",
+               "num_ccd_contacts", "num_broad_phase_contacts", "num_relative_velocities", "has_major_penetration",
+               "has_tunneling", "min_gap", "max_gap", "avg_gap", "max_penetration", "avg_penetration",
+               "ccd_effectiveness", "max_relative_velocity", "avg_relative_velocity", "sum_gap",
+               "sum_penetration", "sum_relative_velocity"]
+
+
+def _energy_dict(a):
+    return {"kinetic_energy": a[0], "elastic_energy": a[1], "linear_momentum": np.array(a[2:5]),
+            "angular_momentum": np.array(a[5:8]), "max_velocity": a[8]}
+
+
+def contact_metrics(types, idx, gaps, normals, velocities, gap_max, ccd, precision=32):
+    """CollisionValidator::compute_metrics on a contact list (see orc_contact_metrics)."""
+    t = np.ascontiguousarray(np.asarray(types, np.int32).reshape(-1))
+    k = len(t)
+    ix = np.ascontiguousarray(np.asarray(idx, np.int32).reshape(k, 4)) if k else np.zeros((1, 4), np.int32)
+    g = _d(gaps).reshape(-1) if k else np.zeros(1)
+    nr = _d(normals).reshape(-1) if k else np.zeros(3)
+    vel = _d(velocities).reshape(-1)
+    out = np.zeros(len(METRIC_KEYS))
+    _chk(lib().orc_contact_metrics(int(precision), k, t if k else np.zeros(1, np.int32), ix.ravel(), g, nr, vel,
+                                   float(gap_max), int(bool(ccd)), out))
+    d = dict(zip(METRIC_KEYS, out.tolist()))
+    for key in METRIC_KEYS[:10]:
+        d[key] = int(d[key])
+    return d
 
 
 class OracleSim:
@@ -155,12 +187,19 @@ class OracleSim:
         return out if sel == 2 else out.reshape(-1, 3)
 
     def set(self, which: str, arr):
-        _chk(self.L.orc_set(self.h, {"x": 0, "v": 1}[which], _d(arr).ravel()))
+        _chk(self.L.orc_set(self.h, {"x": 0, "v": 1, "m": 2, "mesh": 3}[which], _d(arr).ravel()))
 
     def set_contact_mode(self, corrected: bool, pt_box: float = 1e-4, keep_gap: float = 0.01):
         """Faithful (self broad phase empty) or corrected (brute-force PT/EE with
         the intended predicate, both barriers in the energy)."""
         self.L.orc_set_contact_mode(self.h, 1 if corrected else 0, pt_box, keep_gap)
+
+    def energy(self, x, v, m):
+        """EnergyTracker::compute on (x, v, m) with this mesh: (reference
+        accumulation, the same per-item terms summed in double)."""
+        out = np.zeros(18)
+        _chk(self.L.orc_energy(self.h, _d(x).ravel(), _d(v).ravel(), _d(m).ravel(), out))
+        return _energy_dict(out[:9]), _energy_dict(out[9:])
 
     def count_intersections(self):
         """Brute-force (edge-triangle crossings, vertices inside rigid bodies)."""

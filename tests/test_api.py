@@ -196,3 +196,18 @@ def test_energy_tracker_and_validator_on_host():
     met = abc.CollisionValidator.compute_metrics(mesh, state, [], 0.001, True)
     assert met.num_total_contacts == 0 and met.quality_level() == 0
     assert met.quality_description() == "Excellent - No penetrations"
+
+
+def test_adaptive_timestep_accepts_a_host_state():
+    """compute_next_dt / compute_max_velocity also take the State itself (an
+    additive form); on a state that never reached the GPU they read its host
+    velocities."""
+    mesh = abc.Mesh()
+    mesh.initialize([[0.0, 0.0, 0.0], [0.1, 0.0, 0.0], [0.0, 0.1, 0.0]], [0, 1, 2], abc.Material())
+    state = abc.State()
+    state.initialize(mesh)
+    state.apply_gravity((0.0, 0.0, -9.81), 0.01)
+    a = abc.AdaptiveTimestep.compute_next_dt(state, mesh, 0.01, 1e-4, 0.02, 0.5)
+    b = abc.AdaptiveTimestep.compute_next_dt(state.get_velocities(), mesh, 0.01, 1e-4, 0.02, 0.5)
+    assert a == b
+    assert abc.AdaptiveTimestep.compute_max_velocity(state) == pytest.approx(0.0981, rel=1e-6)

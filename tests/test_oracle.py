@@ -427,3 +427,115 @@ def test_intersection_validator_known_answers():
     V2[3:6, 2] += 3.0
     o2 = orc.OracleSim(scenes.Scene("y", V2, T, dict(scenes.BASE_MATERIAL), scenes.base_params(), [], [], []), 64)
     assert o2.count_intersections() == (0, 0)
+
+
+# ---------------------------------------------------------------------------
+# Per-frame consumers: EnergyTracker::compute and CollisionValidator.
+# ---------------------------------------------------------------------------
+def _shear_modulus(mat):
+    return mat["youngs_modulus"] / (2.0 * (1.0 + mat["poisson_ratio"]))
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_energy_closed_forms(prec):
+    """A uniformly scaled mesh has F = sI on every face, so the elastic energy
+    is 2 (s-1)^2 t mu sum(A); a uniform velocity gives KE = |v|^2 M / 2,
+    P = M v, L = (sum m r) x v and max|v| = |v|; a rotation about z gives
+    L = I w with the inertia tensor I = sum m (|r|^2 1 - r r^T)."""
+    sc = scenes.small_sheet(9, pins=0, wall=False)
+    o = orc.OracleSim(sc, prec)
+    V = np.asarray(sc.verts, np.float64)
+    m = o.get("m")
+    s = 1.01
+    o.set("mesh", V * s)
+    mat = sc.material
+    sumA = m.sum() / (mat["density"] * mat["thickness"])  # lumped masses sum to rho t sum(A)
+    rel = 2e-5 if prec == 32 else 1e-10
+    v0 = np.array([0.3, -0.2, 0.5])
+    vel = np.tile(v0, (len(V), 1))
+    faithful, summed = o.energy(V, vel, m)
+    e_el = 2.0 * (s - 1.0) ** 2 * mat["thickness"] * _shear_modulus(mat) * sumA
+    for d in (faithful, summed):
+        # Float F - I carries O(1e-7) absolute error against (s-1) = 1e-2.
+        assert d["elastic_energy"] == pytest.approx(e_el, rel=1e-4 if prec == 32 else 1e-9)
+        assert d["kinetic_energy"] == pytest.approx(0.5 * v0 @ v0 * m.sum(), rel=rel)
+        assert np.allclose(d["linear_momentum"], v0 * m.sum(), rtol=rel, atol=0)
+        L = np.cross((m[:, None] * V).sum(axis=0), v0)
+        assert np.allclose(d["angular_momentum"], L, rtol=rel, atol=rel * np.abs(L).max())
+        assert d["max_velocity"] == pytest.approx(np.linalg.norm(v0), rel=1e-6 if prec == 32 else 1e-12)
+    w = np.array([0.0, 0.0, 2.0])
+    vel = np.cross(w, V)
+    _, summed = o.energy(V, vel, m)
+    I = sum(mi * (r @ r * np.eye(3) - np.outer(r, r)) for mi, r in zip(m, V))
+    Lw = I @ w
+    assert np.allclose(summed["angular_momentum"], Lw, rtol=0, atol=(2e-6 if prec == 32 else 1e-12) * np.abs(Lw).max())
+    # An unscaled mesh is at rest: no elastic energy beyond float rounding.
+    o.set("mesh", V)
+    faithful, _ = o.energy(V, vel, m)
+    assert faithful["elastic_energy"] <= (1e-9 if prec == 32 else 1e-24) * e_el / (s - 1) ** 2
+
+
+def test_energy_faithful_sum_within_float_bound():
+    """The reference's float accumulation and the double sum of the same terms
+    differ by at most n eps sum|terms| (the textbook recursive-sum bound)."""
+    sc = scenes.small_sheet(17, pins=0, wall=False)
+    o = orc.OracleSim(sc, 32)
+    rng = np.random.default_rng(3)
+    V = np.asarray(sc.verts, np.float64)
+    vel = rng.normal(size=V.shape).astype(np.float32).astype(np.float64)
+    o.set("mesh", (V + rng.normal(scale=1e-3, size=V.shape)).astype(np.float32))
+    faithful, summed = o.energy(V, vel, o.get("m"))
+    n = len(V)
+    eps = np.finfo(np.float32).eps
+    for key in ("kinetic_energy", "elastic_energy"):
+        assert abs(faithful[key] - summed[key]) <= 2 * n * eps * abs(summed[key])
+    assert faithful["max_velocity"] == summed["max_velocity"]
+
+
+def _reference_validator_case():
+    """The contacts of test_collision_validator.cpp:54-95 (velocities :62-65)."""
+    vel = np.array([[0.0, -1.0, 0.0], [0.0, 0.25, 0.0], [0.0, -0.4, 0.0], [0.0, 0.1, 0.0]])
+    types = [0, 1, 2]
+    idx = [[0, 1, 2, 3], [0, 1, 2, 3], [2, -1, -1, -1]]
+    gaps = [-5.0e-4, 2.0e-4, -1.5e-3]
+    normals = [[0.0, 1.0, 0.0]] * 3
+    return types, idx, gaps, normals, vel
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_contact_metrics_reference_known_answers(prec):
+    types, idx, gaps, normals, vel = _reference_validator_case()
+    m = orc.contact_metrics(types, idx, gaps, normals, vel, 0.002, True, prec)
+    # The reference's assertions (test_collision_validator.cpp:97-128).
+    assert m["num_total_contacts"] == 3
+    assert (m["num_point_triangle"], m["num_edge_edge"], m["num_wall"]) == (1, 1, 1)
+    assert m["num_penetrations"] == 2
+    assert abs(m["max_penetration"] - 1.5e-3) < 1e-6
+    assert m["has_major_penetration"] == 1 and m["has_tunneling"] == 1
+    assert m["num_ccd_contacts"] == 3 and m["ccd_effectiveness"] > 99.0
+    assert m["max_relative_velocity"] > 0.5 and m["avg_relative_velocity"] > 0.0
+    # Values by hand: PT |v0 - (v1+v2+v3)/3| = 0.98333, EE |(v0+v1)/2 - (v2+v3)/2| = 0.225, wall |v2| = 0.4.
+    tol = 1e-6 if prec == 32 else 1e-12
+    assert m["max_relative_velocity"] == pytest.approx(1.0 - 0.05 / 3.0, rel=tol)
+    assert m["avg_relative_velocity"] == pytest.approx((1.0 - 0.05 / 3.0 + 0.225 + 0.4) / 3.0, rel=tol)
+    assert m["avg_gap"] == pytest.approx(-6e-4, rel=tol * 10)
+    assert m["avg_penetration"] == pytest.approx(1e-3, rel=tol * 10)
+    assert (m["min_gap"], m["max_gap"]) == (pytest.approx(-1.5e-3, rel=tol), pytest.approx(2e-4, rel=tol))
+    assert m["num_broad_phase_contacts"] == 0
+    # CCD off zeroes the CCD figures and keeps the penetration ones (:117-123).
+    m0 = orc.contact_metrics(types, idx, gaps, normals, vel, 0.002, False, prec)
+    assert m0["num_ccd_contacts"] == 0 and m0["ccd_effectiveness"] == 0.0
+    assert m0["num_broad_phase_contacts"] == 3 and m0["num_penetrations"] == 2
+    # No contacts (:30-45).
+    e = orc.contact_metrics([], [], [], [], vel, 0.002, True, prec)
+    assert e["num_total_contacts"] == 0 and e["num_penetrations"] == 0 and e["ccd_effectiveness"] == 0.0
+    assert e["num_broad_phase_contacts"] == 0
+
+
+def test_contact_metrics_rigid_contacts_have_no_relative_velocity():
+    vel = np.ones((4, 3))
+    m = orc.contact_metrics([3, 3], [[0, 1, 2, 3], [1, 0, 2, 3]], [1e-4, 6e-4], [[0, 0, 1]] * 2, vel, 1e-3, True)
+    assert m["num_total_contacts"] == 2 and m["num_relative_velocities"] == 0
+    assert (m["num_point_triangle"], m["num_edge_edge"], m["num_wall"]) == (0, 0, 0)
+    assert m["max_relative_velocity"] == 0.0 and m["avg_relative_velocity"] == 0.0
+    assert m["num_ccd_contacts"] == 1 and m["avg_gap"] == pytest.approx(3.5e-4, rel=1e-6)
