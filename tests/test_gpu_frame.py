@@ -156,7 +156,7 @@ def test_contact_metrics_reference_case_on_device():
     bad = native.Contact()
     bad.type = 0
     bad.idx[:] = [0, 1, 2, g.n]
-    with pytest.raises(ValueError):
+    with pytest.raises(RuntimeError, match="out of range"):
         g.contact_metrics(0.002, True, contacts=[bad])
 
 
