@@ -98,10 +98,60 @@ struct SpmvMeta {
   DBuf<uint32_t> meta;
   DBuf<int32_t> run_row0, straddle;
   DBuf<float> extra;
+  // Sliced-ELL layout of the same pattern (k_pcg_spmv_sell): per slice of 64
+  // rows its first slot position and width, per slot position its BSR block
+  // (-1 = padding) and column, and room for the values.
+  bool sell_ok = false;
+  int slices = 0;
+  int uniform_w = 0;  // the common slice width, or 0 when widths differ
+  DBuf<int32_t> slice_off, slice_w, sell_perm, sell_col;
+  DBuf<float> sell_vals;
 };
+
+void build_sell_meta(SpmvMeta& sm, int n, const std::vector<int32_t>& row_ptr, const std::vector<int32_t>& cols,
+                     hipStream_t s) {
+  sm.sell_ok = false;
+  if (n <= 0) return;
+  const int S = (n + 63) / 64;
+  std::vector<int32_t> off(S + 1, 0), w(S, 0);
+  for (int sl = 0; sl < S; ++sl) {
+    int W = 0;
+    for (int i = sl * 64; i < std::min(n, sl * 64 + 64); ++i) W = std::max(W, row_ptr[i + 1] - row_ptr[i]);
+    w[sl] = W;
+    const long long next = (long long)off[sl] + 64LL * W;
+    if (9 * next >= (1LL << 31)) return;
+    off[sl + 1] = (int32_t)next;
+  }
+  const size_t P = (size_t)off[S];
+  std::vector<int32_t> perm(P, -1), col(P, 0);
+  for (int sl = 0; sl < S; ++sl)
+    for (int l = 0; l < 64; ++l) {
+      const int i = sl * 64 + l;
+      for (int k = 0; k < w[sl]; ++k) {
+        const size_t pos = (size_t)off[sl] + 64 * (size_t)k + l;
+        if (i < n && row_ptr[i] + k < row_ptr[i + 1]) {
+          perm[pos] = row_ptr[i] + k;
+          col[pos] = cols[row_ptr[i] + k];
+        } else {
+          col[pos] = i < n ? i : 0;
+        }
+      }
+    }
+  sm.slice_off.upload(off, s);
+  sm.slice_w.upload(w, s);
+  sm.sell_perm.upload(perm, s);
+  sm.sell_col.upload(col, s);
+  sm.sell_vals.ensure(9 * std::max<size_t>(P, 1));
+  sm.uniform_w = w[0];
+  for (int sl = 1; sl < S; ++sl)
+    if (w[sl] != w[0]) sm.uniform_w = 0;
+  sm.slices = S;
+  sm.sell_ok = true;
+}
 
 void build_spmv_meta(SpmvMeta& sm, int n, const std::vector<int32_t>& row_ptr, const std::vector<int32_t>& cols,
                      hipStream_t s) {
+  build_sell_meta(sm, n, row_ptr, cols, s);
   const int nb = row_ptr[n];
   sm.ok = false;
   if (n <= 0 || nb <= 0 || n >= (1 << ando::kMetaColBits)) return;
@@ -189,7 +239,8 @@ struct ando_ctx {
   DBuf<ando::PcgStatus> pcg_status;
   // Small readbacks.
   DBuf<uint32_t> scalars;  // [0] gradient max bits, [1] line-search mask, [2] strain active count
-  uint32_t* host_pinned = nullptr;
+  PcgStatus* status_pinned = nullptr;  // host copy of the PCG status (pinned)
+  hipEvent_t status_ev = nullptr;
   int pcg_chunk = 8;
   ando_stats stats{};
   int fp64_near = 1;
@@ -199,6 +250,8 @@ struct ando_ctx {
   int pcg_barrier = 0;     // persistent grid barrier: 0 cooperative-groups sync, 1 counter barrier
   int spmv_rowsets = 3;    // block rows per lane group of the row SpMV (1..4)
   int spmv_blocks = 0;     // 1: block-parallel SpMV where the pattern allows it, 0: row SpMV
+  int spmv_sell = 1;       // 1: sliced-ELL SpMV (values copied once per solve), 0: BSR kernels
+  int spmv_env = -1;       // ANDO_PCG_SPMV: -1 unset, 0 row, 1 blk, 2 sell (overrides the mode flag)
   SpmvMeta spmv_meta;      // for the mesh's block pattern
   // Corrected contact mode (self point-triangle and edge-edge through the LBVH).
   int corrected = 0;
@@ -237,6 +290,8 @@ struct ando_ctx {
   std::vector<hipEvent_t> events;  // pool: start/stop pairs for timed launches
   ~ando_ctx() {
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    if (status_pinned) (void)hipHostFree(status_pinned);
+    if (status_ev) (void)hipEventDestroy(status_ev);
   }
 };
 
@@ -735,21 +790,29 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
     }
   }
   const int R = c->spmv_rowsets;
-  const bool blk = c->spmv_blocks && sm != nullptr && sm->ok && !couple;
-  int nwg_v = nblk(n), nwg_r = blk ? sm->groups : spmv_groups(n, R);
-  c->stats.pcg_kernel = blk ? 2 : 0;
+  const bool sell = c->spmv_sell && sm != nullptr && sm->sell_ok;
+  const bool blk = !sell && c->spmv_blocks && sm != nullptr && sm->ok && !couple;
+  int nwg_v = nblk(n), nwg_r = sell ? nblk(sm->slices, kBlock / 64) : (blk ? sm->groups : spmv_groups(n, R));
+  c->stats.pcg_kernel = sell ? 3 : (blk ? 2 : 0);
   c->rz_hist.ensure((size_t)maxit + 2);
   c->pcg_status.ensure(1);
   c->part_pAp.ensure((size_t)nwg_r + 1);
   const float* Pc = c->Pinv.p;
+  int pe = n, pi = 1;  // block-Jacobi blocks as one plane per entry
+  if (sell)
+    hipLaunchKernelGGL(k_sell_fill, dim3(nblk(sm->slices, kBlock / 64)), dim3(kBlock), 0, s, sm->slices,
+                       (const int32_t*)sm->slice_off.p, (const int32_t*)sm->slice_w.p, (const int32_t*)sm->sell_perm.p,
+                       A, sm->sell_vals.p);
   if (P_given) {
+    pe = 1;
+    pi = 9;
     c->part_bmax.ensure((size_t)nwg_v + 1);
     hipLaunchKernelGGL(k_pcg_init_given, dim3(nwg_v), dim3(kBlock), 0, s, n, A, rp, cols, P_given, rhs, negate,
                        x0_given, xsol, c->r.p, c->z.p, c->pbuf0.p, c->part_rz.p, c->part_max.p, c->part_bmax.p);
     Pc = P_given;
   } else {
     hipLaunchKernelGGL(k_pcg_init, dim3(nwg_v), dim3(kBlock), 0, s, n, A, diag, rhs, xsol, c->r.p, c->z.p,
-                       c->pbuf0.p, c->Pinv.p, c->part_rz.p, c->part_max.p, negate);
+                       c->pbuf0.p, c->Pinv.p, pe, pi, c->part_rz.p, c->part_max.p, negate);
   }
   hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, s, nwg_v, (const double*)c->part_rz.p,
                      (const float*)c->part_max.p, (const float*)(P_given ? c->part_bmax.p : nullptr), c->rz_hist.p,
@@ -757,6 +820,12 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
   HIPCHK(hipGetLastError());
   PcgStatus st{};
   int it = 0;
+  // The iterations go out in chunks of 8, 16, 32, then 64 iterations; after each
+  // chunk the status is copied to pinned host memory and read.  Launches
+  // queued after convergence return at the done flag (after issuing their
+  // first loads, which lets a live launch overlap the flag's latency).  Tried
+  // and slower on this path: queueing the next chunk before waiting, and
+  // sizing the first chunk from the previous step's solve.
   int chunk = std::max(1, c->pcg_chunk);
   const bool timing = c->kernel_timing != 0;
   size_t ev_used = 0;
@@ -770,17 +839,8 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
     }
     return c->events[k];
   };
-  for (;;) {
-    HIPCHK(hipMemcpyAsync(&st, c->pcg_status.p, sizeof(PcgStatus), hipMemcpyDeviceToHost, s));
-    sync(c);
-    if (st.done) break;
-    if (it >= maxit) {
-      hipLaunchKernelGGL(k_pcg_final, dim3(1), dim3(kBlock), 0, s, it, c->part_max.p, nwg_v, c->pcg_status.p, tol);
-      HIPCHK(hipMemcpyAsync(&st, c->pcg_status.p, sizeof(PcgStatus), hipMemcpyDeviceToHost, s));
-      sync(c);
-      break;
-    }
-    int end = std::min(maxit, it + chunk);
+  auto launch_chunk = [&](int count) {
+    const int end = std::min(maxit, it + count);
     for (; it < end; ++it) {
       float* pold = (it == 0) ? c->pbuf0.p : ((it - 1) & 1 ? c->pbuf1.p : c->pbuf0.p);
       float* pcur = (it & 1) ? c->pbuf1.p : c->pbuf0.p;
@@ -794,7 +854,14 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
                               (const float*)c->z.p, (const float*)pold, pcur, c->Ap.p, (const double*)c->part_rz.p,
                               (const float*)c->part_max.p, nwg_v, c->rz_hist.p, c->part_pAp.p, c->pcg_status.p, tol);
       };
-      if (blk) {
+      if (sell) {
+        hipExtLaunchKernelGGL(k_pcg_spmv_sell, dim3(nwg_r), dim3(kBlock), 0, s, e0, e1, 0, n, sm->slices, it,
+                              (const float*)sm->sell_vals.p, (const int32_t*)sm->sell_col.p,
+                              (const int32_t*)sm->slice_off.p, (const int32_t*)sm->slice_w.p, sm->uniform_w,
+                              (const float*)c->z.p,
+                              (const float*)pold, pcur, c->Ap.p, (const double*)c->part_rz.p,
+                              (const float*)c->part_max.p, nwg_v, c->rz_hist.p, c->part_pAp.p, c->pcg_status.p, tol);
+      } else if (blk) {
         hipExtLaunchKernelGGL(k_pcg_spmv_blk, dim3(nwg_r), dim3(kBlock), 0, s, e0, e1, 0, sm->nb, it, A,
                               (const uint32_t*)sm->meta.p, (const int32_t*)sm->run_row0.p, (const float*)c->z.p,
                               (const float*)pold, pcur, c->Ap.p, sm->extra.p, (const double*)c->part_rz.p,
@@ -813,15 +880,31 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
                            (const float*)pcur, c->Ap.p, c->part_pAp2.p);
       hipExtLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(kBlock), 0, s, e2, e3, 0, n, it, (const float*)pcur,
                             (const float*)c->Ap.p, (const float*)(blk ? sm->extra.p : nullptr),
-                            (const int32_t*)(blk ? sm->straddle.p : nullptr), Pc, xsol, c->r.p,
+                            (const int32_t*)(blk ? sm->straddle.p : nullptr), Pc, pe, pi, xsol, c->r.p,
                             c->z.p,
                             (const double*)c->part_pAp.p, nwg_r, (const double*)c->rz_hist.p, c->part_rz.p,
                             c->part_max.p, c->pcg_status.p, (const double*)(couple ? c->part_pAp2.p : nullptr),
                             nblk(n, 32));
     }
     HIPCHK(hipGetLastError());
+  };
+  for (;;) {
+    launch_chunk(chunk);
+    HIPCHK(hipMemcpyAsync(c->status_pinned, c->pcg_status.p, sizeof(PcgStatus), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(c->status_ev, s));
+    HIPCHK(hipEventSynchronize(c->status_ev));
+    st = *c->status_pinned;
+    if (st.done) break;
+    if (it >= maxit) {
+      hipLaunchKernelGGL(k_pcg_final, dim3(1), dim3(kBlock), 0, s, it, c->part_max.p, nwg_v, c->pcg_status.p, tol);
+      HIPCHK(hipMemcpyAsync(c->status_pinned, c->pcg_status.p, sizeof(PcgStatus), hipMemcpyDeviceToHost, s));
+      sync(c);
+      st = *c->status_pinned;
+      break;
+    }
     chunk = std::min(chunk * 2, 64);
   }
+  if (timing) sync(c);
   if (timing) {
     // Launch k of the solve ran in full iff k < iterations.
     for (size_t k = 0; 4 * k + 3 < ev_used && (int)k < st.iters; ++k) {
@@ -1137,10 +1220,20 @@ int ando_create(int32_t device, ando_ctx** out) {
       throw Fail(ANDO_EDEVICE, "hipStreamCreate failed");
     }
     c->scalars.ensure(4);
+    if (hipHostMalloc((void**)&c->status_pinned, sizeof(PcgStatus), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&c->status_ev, hipEventDisableTiming) != hipSuccess) {
+      delete c;
+      throw Fail(ANDO_EDEVICE, "pinned status or event allocation failed");
+    }
     if (const char* w = std::getenv("ANDO_PCG_WGS")) c->pcg_wgs = std::atoi(w);
+    if (const char* w = std::getenv("ANDO_PCG_CHUNK")) c->pcg_chunk = std::min(256, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_BARRIER")) c->pcg_barrier = std::atoi(w) == 1 ? 1 : 0;
     if (const char* w = std::getenv("ANDO_PCG_ROWSETS")) c->spmv_rowsets = std::min(4, std::max(1, std::atoi(w)));
-    if (const char* w = std::getenv("ANDO_PCG_SPMV")) c->spmv_blocks = std::strcmp(w, "blk") == 0 ? 1 : 0;
+    if (const char* w = std::getenv("ANDO_PCG_SPMV")) {
+      c->spmv_env = std::strcmp(w, "blk") == 0 ? 1 : (std::strcmp(w, "sell") == 0 ? 2 : 0);
+      c->spmv_blocks = c->spmv_env == 1 ? 1 : 0;
+      c->spmv_sell = c->spmv_env == 2 ? 1 : 0;
+    }
     *out = c;
   });
 }
@@ -1335,7 +1428,7 @@ int ando_set_mode(ando_ctx* c, int32_t flags) {
     need(c != nullptr, ANDO_EINVAL, "null context");
     need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING | ANDO_MODE_PCG_PERSISTENT |
                     ANDO_MODE_LEARNED_PRECOND | ANDO_MODE_LEARNED_WARM_START_ONLY |
-                    ANDO_MODE_CORRECTED_CONTACT)) == 0,
+                    ANDO_MODE_CORRECTED_CONTACT | ANDO_MODE_PCG_ROW_SPMV)) == 0,
          ANDO_EINVAL, "unknown mode flag");
     need(!(flags & ANDO_MODE_LEARNED_PRECOND) || c->nn.ready, ANDO_ESTATE,
          "learned preconditioner mode needs ando_set_precond_weights first");
@@ -1344,6 +1437,7 @@ int ando_set_mode(ando_ctx* c, int32_t flags) {
     c->pcg_persistent = (flags & ANDO_MODE_PCG_PERSISTENT) ? 1 : 0;
     c->learned = (flags & ANDO_MODE_LEARNED_PRECOND) ? ((flags & ANDO_MODE_LEARNED_WARM_START_ONLY) ? 2 : 1) : 0;
     c->corrected = (flags & ANDO_MODE_CORRECTED_CONTACT) ? 1 : 0;
+    if (c->spmv_env < 0) c->spmv_sell = (flags & ANDO_MODE_PCG_ROW_SPMV) ? 0 : 1;
   });
 }
 

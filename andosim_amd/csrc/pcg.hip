@@ -77,9 +77,12 @@ __device__ inline float cof3(const float* B, int i, int j) {
 }
 
 // Preconditioner, r = b = -g, x = 0, z = P^-1 r, p = z (pcg_solver.cpp:13-34,77-135).
+// Entry e of vertex i's preconditioner block sits at Pinv[e * pe + i * pi]
+// (pe = 1, pi = 9: block per vertex; pe = n, pi = 1: one plane per entry, which
+// the update kernel reads with coalesced loads).
 __global__ void k_pcg_init(int n, const float* __restrict__ A, const int32_t* __restrict__ diag,
                            const float* __restrict__ g, float* __restrict__ x, float* __restrict__ r,
-                           float* __restrict__ z, float* __restrict__ p, float* __restrict__ Pinv,
+                           float* __restrict__ z, float* __restrict__ p, float* __restrict__ Pinv, int pe, int pi,
                            double* __restrict__ part_rz, float* __restrict__ part_max, int negate) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0;
@@ -97,7 +100,7 @@ __global__ void k_pcg_init(int n, const float* __restrict__ A, const int32_t* __
     } else {
       for (int e = 0; e < 9; ++e) P[e] = (e % 4 == 0) ? 1.0f : 0.0f;
     }
-    for (int e = 0; e < 9; ++e) Pinv[9 * (size_t)i + e] = P[e];
+    for (int e = 0; e < 9; ++e) Pinv[(size_t)e * pe + (size_t)i * pi] = P[e];
     float rv[3];
     for (int k = 0; k < 3; ++k) {
       float bk = negate ? -g[3 * i + k] : g[3 * i + k];
@@ -334,6 +337,180 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int n, int it, const float*
   if (threadIdx.x == 0) part_pAp[g] = s;
 }
 
+// Sliced-ELL SpMV.  Rows are cut into slices of 64 (one wave, one row per
+// lane); slice s holds W_s slots (its longest row), slot k of a row being its
+// k-th block in BSR order (padding: a zero block on the row's own column).
+// Values are stored slot-major and entry-major inside a slice,
+//   vals[9 * off_s + (9 k + e) * 64 + lane],   cols[off_s + 64 k + lane],
+// so every load of the kernel's first pass is 64 consecutive words.  A row's
+// sum runs over its blocks in BSR order.  The slots of a chunk (kSellChunk)
+// are all requested before any is used; rows longer than that loop.
+constexpr int kSellChunk = 8;  // = the 8 lanes per row of k_pcg_spmv (summation order)
+
+// Copies BSR values into the sliced layout (perm: BSR block of each slot
+// position, -1 for padding); one wave per slice.
+__global__ void __launch_bounds__(kBlock) k_sell_fill(int slices, const int32_t* __restrict__ slice_off,
+                                                     const int32_t* __restrict__ slice_w,
+                                                     const int32_t* __restrict__ perm, const float* __restrict__ A,
+                                                     float* __restrict__ vals) {
+  const int sl = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (sl >= slices) return;
+  const int off = slice_off[sl], W = slice_w[sl];
+  for (int k = 0; k < W; ++k) {
+    const int b = perm[off + 64 * k + lane];
+    float* dst = vals + 9 * (size_t)off + (size_t)(9 * k) * 64 + lane;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) dst[64 * e] = b >= 0 ? A[9 * (size_t)b + e] : 0.0f;
+  }
+}
+
+// Same contract as k_pcg_spmv (convergence test, beta, p = z + beta p on the
+// fly, Ap, partial p.Ap per workgroup), over the sliced layout; a workgroup
+// covers kBlock / 64 consecutive slices, numbered XCD-contiguously.
+__global__ void __launch_bounds__(kBlock) k_pcg_spmv_sell(int n, int slices, int it, const float* __restrict__ vals,
+                           const int32_t* __restrict__ scol, const int32_t* __restrict__ slice_off,
+                           const int32_t* __restrict__ slice_w, int uniform_w, const float* __restrict__ z,
+                           const float* __restrict__ p_old, float* __restrict__ p_new, float* __restrict__ Ap,
+                           const double* __restrict__ part_rz, const float* __restrict__ part_max, int nwg_v,
+                           double* __restrict__ rz_hist, double* __restrict__ part_pAp, PcgStatus* __restrict__ st,
+                           float tol) {
+  __shared__ double ldsd[4];
+  __shared__ float ldsf[4];
+  const int g = xcd_logical_group(blockIdx.x, gridDim.x);
+  const int lane = threadIdx.x & 63;
+  // The slice index is wave-uniform; saying so makes its metadata scalar loads
+  // and the slot tests scalar branches.
+  const int sl0 = __builtin_amdgcn_readfirstlane(g * (kBlock / 64) + (int)(threadIdx.x >> 6));
+  const bool live = sl0 < slices;
+  const int sl = live ? sl0 : slices - 1;
+  const int row = sl0 * 64 + lane;
+  const int ri = row < n ? row : n - 1;
+  // A pattern whose slices all have the same width (uniform_w > 0) needs no
+  // slice metadata: offsets follow from the slice index.
+  const int off = uniform_w > 0 ? sl * 64 * uniform_w : slice_off[sl];
+  const int W = !live ? 0 : (uniform_w > 0 ? uniform_w : slice_w[sl]);
+  // First chunk of slots, the row's own z and p entries: none depend on beta.
+  int jc[kSellChunk];
+  float Ab[kSellChunk][9];
+  const float* vbase = vals + 9 * (size_t)off + lane;
+  const int32_t* cbase = scol + off + lane;
+#pragma unroll
+  for (int k = 0; k < kSellChunk; ++k) {
+    if (k < W) {
+      jc[k] = cbase[64 * k];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) Ab[k][e] = vbase[(size_t)(9 * k + e) * 64];
+    }
+  }
+  float zi[3], pi[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    zi[c] = z[3 * ri + c];
+    pi[c] = p_old[3 * ri + c];
+  }
+  float zo[kSellChunk][3], po[kSellChunk][3];
+#pragma unroll
+  for (int k = 0; k < kSellChunk; ++k) {
+    if (k < W) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        zo[k][c] = z[3 * jc[k] + c];
+        po[k][c] = p_old[3 * jc[k] + c];
+      }
+    }
+  }
+  if (st->done) return;
+  float beta = 0.0f;
+  if (it > 0) {
+    const float bmax = st->bmax;
+    const double rz_prev = rz_hist[it - 1];
+    double rz = reduce_sum_partials(part_rz, nwg_v, ldsd);
+    float rmax = reduce_max_partials(part_max, nwg_v, ldsf);
+    float rel = ((double)bmax < 1e-16) ? rmax : rmax / bmax;
+    if (rel < tol) {
+      if (threadIdx.x == 0) {
+        st->rel = rel;
+        st->iters = it;
+        st->done = 1;
+      }
+      return;
+    }
+    beta = (float)(rz / rz_prev);
+    if (g == 0 && threadIdx.x == 0) rz_hist[it] = rz;
+  }
+  // Slot k's product goes to partial t[k % 8]; the row sum combines the eight
+  // partials as ((t0 + t4) + (t2 + t6)) + ((t1 + t5) + (t3 + t7)), the order in
+  // which k_pcg_spmv's 8-lane shuffle tree adds its lanes (lane l holds blocks
+  // l, l + 8, ... of the row), so both kernels produce the same Ap bit for bit.
+  float t[kSellChunk][3];
+#pragma unroll
+  for (int k = 0; k < kSellChunk; ++k) {
+    t[k][0] = t[k][1] = t[k][2] = 0.0f;
+    if (k < W) {
+      float pv0, pv1, pv2;
+      if (it > 0) {
+        pv0 = zo[k][0] + beta * po[k][0];
+        pv1 = zo[k][1] + beta * po[k][1];
+        pv2 = zo[k][2] + beta * po[k][2];
+      } else {
+        pv0 = po[k][0]; pv1 = po[k][1]; pv2 = po[k][2];
+      }
+      t[k][0] = Ab[k][0] * pv0 + (Ab[k][1] * pv1 + Ab[k][2] * pv2);
+      t[k][1] = Ab[k][3] * pv0 + (Ab[k][4] * pv1 + Ab[k][5] * pv2);
+      t[k][2] = Ab[k][6] * pv0 + (Ab[k][7] * pv1 + Ab[k][8] * pv2);
+    }
+  }
+  // Rows longer than one chunk (slot k + u adds to partial u).
+  for (int k0 = kSellChunk; k0 < W; k0 += kSellChunk) {
+#pragma unroll
+    for (int u = 0; u < kSellChunk; ++u) {
+      const int k = k0 + u;
+      if (k < W) {
+        const int j = cbase[64 * k];
+        float B[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) B[e] = vbase[(size_t)(9 * k + e) * 64];
+        float pv0, pv1, pv2;
+        if (it > 0) {
+          pv0 = z[3 * j] + beta * p_old[3 * j];
+          pv1 = z[3 * j + 1] + beta * p_old[3 * j + 1];
+          pv2 = z[3 * j + 2] + beta * p_old[3 * j + 2];
+        } else {
+          pv0 = p_old[3 * j]; pv1 = p_old[3 * j + 1]; pv2 = p_old[3 * j + 2];
+        }
+        t[u][0] = t[u][0] + (B[0] * pv0 + (B[1] * pv1 + B[2] * pv2));
+        t[u][1] = t[u][1] + (B[3] * pv0 + (B[4] * pv1 + B[5] * pv2));
+        t[u][2] = t[u][2] + (B[6] * pv0 + (B[7] * pv1 + B[8] * pv2));
+      }
+    }
+  }
+  float sr[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) sr[c] = ((t[0][c] + t[4][c]) + (t[2][c] + t[6][c])) + ((t[1][c] + t[5][c]) + (t[3][c] + t[7][c]));
+  const float s0 = sr[0], s1 = sr[1], s2 = sr[2];
+  double pap = 0.0;
+  if (live && row < n) {
+    float q0, q1, q2;
+    if (it > 0) {
+      q0 = zi[0] + beta * pi[0];
+      q1 = zi[1] + beta * pi[1];
+      q2 = zi[2] + beta * pi[2];
+      p_new[3 * row] = q0;
+      p_new[3 * row + 1] = q1;
+      p_new[3 * row + 2] = q2;
+    } else {
+      q0 = pi[0]; q1 = pi[1]; q2 = pi[2];
+    }
+    Ap[3 * row] = s0;
+    Ap[3 * row + 1] = s1;
+    Ap[3 * row + 2] = s2;
+    pap = (double)q0 * s0 + (double)q1 * s1 + (double)q2 * s2;
+  }
+  double sp = block_reduce(pap, ldsd, [](double a, double b) { return a + b; });
+  if (threadIdx.x == 0) part_pAp[g] = sp;
+}
+
 // Block-parallel SpMV: one 3x3 block per lane, workgroups over contiguous
 // runs of kBlock blocks, so a wave reads 64 consecutive blocks (2304 bytes)
 // and no lane waits on a row pointer.  Per block, meta packs
@@ -462,17 +639,18 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_blk(int nb, int it, const f
 // Ap_extra entry holds the rest of row i (straddle == nullptr otherwise).
 __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const float* __restrict__ p,
                              const float* __restrict__ Ap, const float* __restrict__ Ap_extra,
-                             const int32_t* __restrict__ straddle, const float* __restrict__ Pinv,
-                             float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
+                             const int32_t* __restrict__ straddle, const float* __restrict__ Pinv, int pe,
+                             int pi, float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
                              const double* __restrict__ part_pAp, int nwg_r, const double* __restrict__ rz_hist,
                              double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st,
                              const double* __restrict__ part_pAp2, int n2) {
-  if (st->done) return;
   __shared__ double ldsd[4];
   __shared__ float ldsf[4];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int ii = i < n ? i : n - 1;
   float pv[3], av[3], xv[3], rv0[3], P[9];
+  // Every load that does not depend on alpha is issued before the done flag
+  // and the p.Ap partials are read, so their latencies overlap.
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     pv[k] = p[3 * ii + k];
@@ -480,6 +658,9 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const floa
     xv[k] = x[3 * ii + k];
     rv0[k] = r[3 * ii + k];
   }
+#pragma unroll
+  for (int e = 0; e < 9; ++e) P[e] = Pinv[(size_t)e * pe + (size_t)ii * pi];
+  if (st->done) return;
   if (straddle != nullptr) {
     const int w = straddle[ii];
     if (w >= 0) {
@@ -487,8 +668,6 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const floa
       for (int k = 0; k < 3; ++k) av[k] = av[k] + Ap_extra[3 * w + k];
     }
   }
-#pragma unroll
-  for (int e = 0; e < 9; ++e) P[e] = Pinv[9 * (size_t)ii + e];
   const double rz_it = rz_hist[it];
   double pAp = reduce_sum_partials(part_pAp, nwg_r, ldsd);
   if (part_pAp2) pAp += reduce_sum_partials(part_pAp2, n2, ldsd);

@@ -72,7 +72,7 @@ def measured_traffic(workload: str, n: int, nb: int, kernel_prefix: str):
     if d.get("vertices") != n or d.get("blocks") != nb:
         return None
     for name, rec in d.get("kernels", {}).items():
-        if kernel_prefix in name:
+        if name == kernel_prefix or name.endswith(" " + kernel_prefix):
             return int(round(rec["traffic_bytes"]))
     return None
 
@@ -109,6 +109,51 @@ def corrected_contact_leg(steps: int):
         out[mode] = {"steps_per_s": round(done / t, 2), "steps_run": done, "finite": finite_state(sim),
                      "contacts_per_step": contacts, "pcg_iters_per_step": round(float(np.mean(pcg)), 1),
                      "edge_triangle_intersections_at_end": et}
+    return out
+
+
+def per_frame_leg(sim, sc, reps: int = 20, with_cpu: bool = True):
+    """The add-on's per-frame consumers (operators.py:872-880, 909, 947,
+    967-975) at the state the timed window ends in: host-clock ms per call of
+    each device entry point (the call includes its small readback), and the
+    oracle's CPU restatement of the same reductions for comparison."""
+    import ctypes as C
+    from andosim_amd import native
+
+    L, h = sim.L, sim.h
+
+    def avg_ms(fn):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return round(1e3 * (time.perf_counter() - t0) / reps, 4)
+
+    cnt = C.c_int32(0)
+    x = np.zeros((sim.n, 3), np.float32)
+    v = np.zeros((sim.n, 3), np.float32)
+    out = {"energy_ms": avg_ms(sim.energy),
+           "detect_contacts_ms": avg_ms(lambda: native.check(L.ando_compute_contacts(
+               h, sim.bodies, sim.nbodies, None, 0, C.byref(cnt)))),
+           "cfl_inputs_ms": avg_ms(sim.cfl_inputs),
+           "get_state_ms": avg_ms(lambda: native.check(L.ando_get_state(h, native._fp(x), native._fp(v), sim.n)))}
+    cs, count = sim.contacts(cap=1 << 20)
+    out["contacts"] = count
+    out["metrics_ms"] = avg_ms(lambda: sim.contact_metrics(float(sc.params["contact_gap_max"]), True))
+    out["list_contacts_ms"] = avg_ms(lambda: sim.contacts(cap=max(count, 1)))
+    if with_cpu:
+        from oracle import oracle as orc
+        o = orc.OracleSim(sc, 32)
+        xs, vs = sim.get("x"), sim.get("v")
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            o.energy(xs, vs, sim.masses)
+        out["cpu_oracle_energy_ms"] = round(1e3 * (time.perf_counter() - t0) / reps, 4)
+        arrs = ([c.type for c in cs], [list(c.idx) for c in cs], [c.gap for c in cs], [list(c.normal) for c in cs])
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            orc.contact_metrics(*arrs, vs, float(sc.params["contact_gap_max"]), True, 32)
+        out["cpu_oracle_metrics_ms"] = round(1e3 * (time.perf_counter() - t0) / reps, 4)
     return out
 
 
@@ -309,9 +354,10 @@ def main(argv=None):
         avg_s = t_launch[0] / n_launch[0] * 1e-3
         by = spmv_bytes(n, nb)
         ach = by / avg_s / 1e9
+        kname = {3: "ando::k_pcg_spmv_sell", 2: "ando::k_pcg_spmv_blk"}.get(kernel, "ando::k_pcg_spmv<3>")
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(a.workload, n, nb, "k_pcg_spmv"), "kernel": "k_pcg_spmv",
+                "traffic": measured_traffic(a.workload, n, nb, kname), "kernel": kname,
                 "bytes_per_launch": by, "avg_launch_us": round(avg_s * 1e6, 2), "launches": n_launch[0]}
         if n_launch[1]:
             avg_u = t_launch[1] / n_launch[1] * 1e-3
@@ -319,6 +365,8 @@ def main(argv=None):
             roof["update_avg_launch_us"] = round(avg_u * 1e6, 2)
             roof["pcg_kernels_share_of_step"] = round((t_launch[0] + t_launch[1]) * 1e-3 / ev_elapsed, 3)
         roof["event_pass_ms_per_step"] = round(1e3 * ev_elapsed / len(ev_recs), 3)
+
+    frame = per_frame_leg(sim, sc, with_cpu=not a.no_cpu)
 
     corrected = None
     if not a.no_corrected:
@@ -376,13 +424,15 @@ def main(argv=None):
         "config": {"workload": f"{a.workload}: 257x257 cloth (66,049 v, 131,072 tris) onto two crossed cylinders "
                                "r=0.1 m (4,800 rigid tris); dt=0.01, 5% strain limit, PCG tol 1e-6, faithful mode",
                    "vertices": n, "blocks": nb, "precond": "block-Jacobi (plain PCG)",
-                   "pcg_kernel": {1: "persistent", 0: "two-kernel"}.get(kernel, "none"),
+                   "pcg_kernel": {1: "persistent", 0: "two-kernel, BSR row SpMV", 2: "two-kernel, block SpMV",
+                                  3: "two-kernel, sliced-ELL SpMV"}.get(kernel, "none"),
                    "timed_window": [window[0], window[-1]],
                    "first_nonfinite_step": last_ok + 1 if last_ok + 1 < a.max_explore else None},
         "pcg_iters_per_step": round(pcg_iters / len(recs), 2),
         "newton_iters_per_step": round(newton / len(recs), 2),
         "learned_precond": learned,
         "corrected_contact": corrected,
+        "per_frame_consumers": frame,
         "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu_single,
     }
     print(json.dumps(out), flush=True)

@@ -139,7 +139,8 @@ typedef struct {
   double timed_ms[2];               /* summed event durations of those launches */
   int32_t timed_iterations;         /* PCG iterations covered by timed persistent launches */
   int32_t pcg_kernel;               /* 0: two kernels per iteration (row SpMV), 1: persistent
-                                       single launch, 2: two kernels with the block-parallel SpMV */
+                                       single launch, 2: two kernels with the block-parallel SpMV,
+                                       3: two kernels with the sliced-ELL SpMV */
   int32_t learned_solves;           /* solves that used the learned preconditioner */
   int32_t learned_fallbacks;        /* of those, solves that fell back to plain PCG */
   int32_t fallback_iterations;      /* plain PCG iterations spent in fallbacks (included in pcg_iterations) */
@@ -196,6 +197,11 @@ int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, co
  * boxes, gap < 0.01) through a GPU LBVH, and puts both barriers in the energy;
  * ando_compute_contacts then lists them before the rigid contacts. */
 #define ANDO_MODE_CORRECTED_CONTACT 32
+/* The two-kernel PCG multiplies by the system matrix in a sliced-ELL copy of
+ * its values (slices of 64 rows, made once per solve); ANDO_MODE_PCG_ROW_SPMV
+ * reads the BSR values in place instead (8 lanes per block row).  Both sum a
+ * row's blocks in the same order and give the same product bit for bit. */
+#define ANDO_MODE_PCG_ROW_SPMV 64
 int ando_set_mode(ando_ctx* ctx, int32_t flags);
 
 int ando_apply_gravity(ando_ctx* ctx, const float* gravity3, float dt);

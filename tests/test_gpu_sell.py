@@ -1,0 +1,98 @@
+"""The sliced-ELL SpMV (default) against the BSR row SpMV (ANDO_MODE_PCG_ROW_SPMV).
+
+Both kernels sum a row's blocks in the same order, so their products agree bit
+for bit; the p.Ap partials are grouped differently (per workgroup, in double),
+which can move alpha by a rounding step, so whole solves are compared to 1e-6
+with equal iteration counts.  Irregular patterns (slices of different widths,
+rows longer than one chunk of eight blocks) go through ando_pcg.
+"""
+import numpy as np
+import pytest
+
+from andosim_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+native = pytest.importorskip("andosim_amd.native")
+
+
+def random_spd_bsr(n, seed, extra_rows=()):
+    """Symmetric block pattern with random neighbours (rows in extra_rows get
+    many), diagonally dominant so it is SPD."""
+    rng = np.random.default_rng(seed)
+    nbr = [set() for _ in range(n)]
+    for i in range(n):
+        for j in rng.choice(n, size=rng.integers(0, 5), replace=False):
+            if j != i:
+                nbr[i].add(int(j))
+                nbr[int(j)].add(i)
+    for i in extra_rows:
+        for j in rng.choice(n, size=18, replace=False):
+            if j != i:
+                nbr[i].add(int(j))
+                nbr[int(j)].add(i)
+    blocks = {}
+    for i in range(n):
+        for j in nbr[i]:
+            if (j, i) in blocks:
+                blocks[(i, j)] = blocks[(j, i)].T.copy()
+            else:
+                blocks[(i, j)] = rng.normal(scale=0.3, size=(3, 3))
+    rp, ci, vals = [0], [], []
+    for i in range(n):
+        cols = sorted(nbr[i] | {i})
+        off = sum(np.abs(blocks[(i, j)]).sum() for j in cols if j != i)
+        for j in cols:
+            if j == i:
+                M = rng.normal(scale=0.1, size=(3, 3))
+                vals.append(M @ M.T + (off + 1.0) * np.eye(3))
+            else:
+                vals.append(blocks[(i, j)])
+            ci.append(j)
+        rp.append(len(ci))
+    return (np.array(rp, np.int32), np.array(ci, np.int32), np.array(vals, np.float32),
+            rng.normal(size=3 * n).astype(np.float32))
+
+
+@pytest.mark.parametrize("n,seed", [(300, 1), (1000, 2), (4097, 3)])
+def test_sell_pcg_matches_row_and_dense(n, seed):
+    print(f"sell pcg n={n}", flush=True)
+    rp, ci, vals, b = random_spd_bsr(n, seed, extra_rows=(0, n // 2, n - 1))
+    assert np.max(np.diff(rp)) > 8  # rows longer than one chunk
+    sell, row = native.GpuSim(), native.GpuSim()
+    row.set_mode(row_spmv=True)
+    xs, its, cs = sell.pcg(rp, ci, vals, b, tol=1e-6, maxit=2000)
+    xr, itr, cr = row.pcg(rp, ci, vals, b, tol=1e-6, maxit=2000)
+    assert cs and cr and its == itr
+    assert np.max(np.abs(xs - xr)) <= 1e-6 * np.max(np.abs(xr))
+    if n <= 1000:
+        A = np.zeros((3 * n, 3 * n))
+        for i in range(n):
+            for k in range(rp[i], rp[i + 1]):
+                j = ci[k]
+                A[3 * i:3 * i + 3, 3 * j:3 * j + 3] = vals[k]
+        x = np.linalg.solve(A, b.astype(np.float64))
+        assert np.max(np.abs(A @ xs - b)) <= 1e-5 * np.max(np.abs(b))
+        assert np.max(np.abs(xs - x)) <= 1e-4 * np.max(np.abs(x))
+
+
+@pytest.mark.parametrize("name,make,steps,corrected", [
+    ("sphere129", lambda: scenes.sphere129(), 5, False),
+    ("sheet9_rigid", lambda: scenes.small_sheet(9, pins=0, wall=False, rigid=True), 8, False),
+    ("stack_resting33", lambda: scenes.stack_resting(s=33, sheets=4), 3, True),
+])
+def test_sell_steps_match_row(name, make, steps, corrected):
+    print(f"sell steps {name}", flush=True)
+    sc = make()
+    a, b = native.GpuSim(sc), native.GpuSim(sc)
+    a.set_mode(corrected=corrected)
+    b.set_mode(corrected=corrected, row_spmv=True)
+    for _ in range(steps):
+        a.substep()
+        b.substep()
+        sa, sb = a.stats(), b.stats()
+        assert sa["pcg_kernel"] == 3 and sb["pcg_kernel"] == 0
+        assert sa["newton_iterations"] == sb["newton_iterations"]
+        assert abs(sa["pcg_iterations"] - sb["pcg_iterations"]) <= max(2, sb["pcg_iterations"] // 100)
+    xa, xb = a.get("x"), b.get("x")
+    assert np.max(np.abs(xa - xb)) <= 1e-6 * max(1.0, float(np.max(np.abs(xb))))
