@@ -35,35 +35,101 @@ struct PcgStatus {
 // of a chunk back to back before summing them (a plain strided loop would wait
 // for every load in turn), then the workgroup combines the thread sums; the
 // order is fixed, so every workgroup obtains the same value.
-constexpr int kUnroll = 12;
+constexpr int kUnroll = 4;
 
-__device__ inline double reduce_sum_partials(const double* p, int count, double* lds) {
+// The per-thread parts (loads and the thread's own sum) are separate so a
+// kernel can issue them before it waits on anything else.
+// Loads are unconditional (indices past the end read the last partial and
+// are then dropped), so they issue back to back without branches; count >= 1.
+__device__ inline double thread_sum_partials(const double* p, int count) {
   double s = 0.0;
   for (int base = 0; base < count; base += kUnroll * (int)blockDim.x) {
     double v[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const int k = base + u * (int)blockDim.x + (int)threadIdx.x;
-      v[u] = k < count ? p[k] : 0.0;
+      v[u] = p[k < count ? k : count - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int k = base + u * (int)blockDim.x + (int)threadIdx.x;
+      if (k >= count) v[u] = 0.0;
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) s += v[u];
   }
-  return block_reduce(s, lds, [](double a, double b) { return a + b; });
+  return s;
 }
-__device__ inline float reduce_max_partials(const float* p, int count, float* lds) {
+__device__ inline float thread_max_partials(const float* p, int count) {
   float s = 0.0f;
   for (int base = 0; base < count; base += kUnroll * (int)blockDim.x) {
     float v[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const int k = base + u * (int)blockDim.x + (int)threadIdx.x;
-      v[u] = k < count ? p[k] : 0.0f;
+      v[u] = p[k < count ? k : count - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int k = base + u * (int)blockDim.x + (int)threadIdx.x;
+      if (k >= count) v[u] = 0.0f;
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) s = fmaxf(s, v[u]);
   }
-  return block_reduce(s, lds, [](float a, float b) { return fmaxf(a, b); });
+  return s;
+}
+// Split form of thread_sum_partials / thread_max_partials: the first
+// kUnroll * blockDim partials are loaded into registers by prefetch_partials
+// (issued before a kernel's other loads, since they head the longest chain),
+// and summed in the same order by finish_*.
+template <class T>
+__device__ inline void prefetch_partials(const T* p, int count, T (&v)[kUnroll]) {
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    const int k = u * (int)blockDim.x + (int)threadIdx.x;
+    v[u] = p[k < count ? k : count - 1];
+  }
+}
+__device__ inline double finish_sum_partials(const double* p, int count, const double (&v)[kUnroll]) {
+  double s = 0.0;
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) s += (u * (int)blockDim.x + (int)threadIdx.x) < count ? v[u] : 0.0;
+  for (int base = kUnroll * (int)blockDim.x; base < count; base += kUnroll * (int)blockDim.x) {
+    double w[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int k = base + u * (int)blockDim.x + (int)threadIdx.x;
+      w[u] = p[k < count ? k : count - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) s += (base + u * (int)blockDim.x + (int)threadIdx.x) < count ? w[u] : 0.0;
+  }
+  return s;
+}
+__device__ inline float finish_max_partials(const float* p, int count, const float (&v)[kUnroll]) {
+  float s = 0.0f;
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) s = fmaxf(s, (u * (int)blockDim.x + (int)threadIdx.x) < count ? v[u] : 0.0f);
+  for (int base = kUnroll * (int)blockDim.x; base < count; base += kUnroll * (int)blockDim.x) {
+    float w[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int k = base + u * (int)blockDim.x + (int)threadIdx.x;
+      w[u] = p[k < count ? k : count - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      s = fmaxf(s, (base + u * (int)blockDim.x + (int)threadIdx.x) < count ? w[u] : 0.0f);
+  }
+  return s;
+}
+
+__device__ inline double reduce_sum_partials(const double* p, int count, double* lds) {
+  return block_reduce(thread_sum_partials(p, count), lds, [](double a, double b) { return a + b; });
+}
+__device__ inline float reduce_max_partials(const float* p, int count, float* lds) {
+  return block_reduce(thread_max_partials(p, count), lds, [](float a, float b) { return fmaxf(a, b); });
 }
 
 // 3x3 determinant (Eigen's bruteforce formula) and cofactor inverse.
@@ -390,6 +456,14 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_sell(int n, int slices, int
   // slice metadata: offsets follow from the slice index.
   const int off = uniform_w > 0 ? sl * 64 * uniform_w : slice_off[sl];
   const int W = !live ? 0 : (uniform_w > 0 ? uniform_w : slice_w[sl]);
+  // The previous update's partials head the longest chain (reduction,
+  // convergence test, beta): requested first.
+  double pre_rz[kUnroll];
+  float pre_max[kUnroll];
+  if (it > 0) {
+    prefetch_partials(part_rz, nwg_v, pre_rz);
+    prefetch_partials(part_max, nwg_v, pre_max);
+  }
   // First chunk of slots, the row's own z and p entries: none depend on beta.
   int jc[kSellChunk];
   float Ab[kSellChunk][9];
@@ -425,8 +499,10 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_sell(int n, int slices, int
   if (it > 0) {
     const float bmax = st->bmax;
     const double rz_prev = rz_hist[it - 1];
-    double rz = reduce_sum_partials(part_rz, nwg_v, ldsd);
-    float rmax = reduce_max_partials(part_max, nwg_v, ldsf);
+    double rz = block_reduce(finish_sum_partials(part_rz, nwg_v, pre_rz), ldsd,
+                             [](double a, double b) { return a + b; });
+    float rmax = block_reduce(finish_max_partials(part_max, nwg_v, pre_max), ldsf,
+                              [](float a, float b) { return fmaxf(a, b); });
     float rel = ((double)bmax < 1e-16) ? rmax : rmax / bmax;
     if (rel < tol) {
       if (threadIdx.x == 0) {
@@ -649,8 +725,11 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const floa
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int ii = i < n ? i : n - 1;
   float pv[3], av[3], xv[3], rv0[3], P[9];
-  // Every load that does not depend on alpha is issued before the done flag
-  // and the p.Ap partials are read, so their latencies overlap.
+  // The p.Ap partials head the longest dependency chain (reduction, alpha,
+  // update): they are requested first, then every other load that does not
+  // depend on alpha.
+  double pre[kUnroll];
+  prefetch_partials(part_pAp, nwg_r, pre);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     pv[k] = p[3 * ii + k];
@@ -661,6 +740,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const floa
 #pragma unroll
   for (int e = 0; e < 9; ++e) P[e] = Pinv[(size_t)e * pe + (size_t)ii * pi];
   if (st->done) return;
+  const double pAp_t = finish_sum_partials(part_pAp, nwg_r, pre);
+  const double pAp2_t = part_pAp2 ? thread_sum_partials(part_pAp2, n2) : 0.0;
   if (straddle != nullptr) {
     const int w = straddle[ii];
     if (w >= 0) {
@@ -669,8 +750,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const floa
     }
   }
   const double rz_it = rz_hist[it];
-  double pAp = reduce_sum_partials(part_pAp, nwg_r, ldsd);
-  if (part_pAp2) pAp += reduce_sum_partials(part_pAp2, n2, ldsd);
+  double pAp = block_reduce(pAp_t, ldsd, [](double a, double b) { return a + b; });
+  if (part_pAp2) pAp += block_reduce(pAp2_t, ldsd, [](double a, double b) { return a + b; });
   if (fabs(pAp) < 1e-16) {
     if (threadIdx.x == 0) {
       st->failed = 1;
