@@ -231,6 +231,7 @@ struct ando_ctx {
   std::vector<BodyCache> bodies;
   DBuf<int32_t> rc_count, rc_ptr, scan_sums, scan_total;
   DBuf<ando::RigidContact> rc;
+  DBuf<int32_t> rc_hits;  // per vertex, the first kHitCache (body << 24 | triangle) hits of the count pass
   int rc_total = 0;
   // PCG.
   DBuf<float> r, z, pbuf0, pbuf1, Ap, Pinv;
@@ -242,6 +243,7 @@ struct ando_ctx {
   PcgStatus* status_pinned = nullptr;  // host copy of the PCG status (pinned)
   hipEvent_t status_ev = nullptr;
   int pcg_chunk = 8;
+  int rigid_group = kGroup;  // rigid triangles per culling box
   ando_stats stats{};
   int fp64_near = 1;
   int kernel_timing = 0;
@@ -441,6 +443,9 @@ void ensure_const(ando_ctx* c, const ando_params& p) {
 
 void upload_bodies(ando_ctx* c, const ando_rigid_body* bodies, int nbod) {
   need(nbod >= 0 && nbod <= kMaxBodies, ANDO_EINVAL, "too many rigid bodies (limit 16)");
+  for (int b = 0; b < nbod; ++b)
+    need(bodies[b].num_triangles >= 0 && bodies[b].num_triangles < (1 << 24), ANDO_EINVAL,
+         "a rigid body has too many triangles (limit 2^24 - 1)");
   c->bodies.resize(nbod);
   hipStream_t s = c->stream;
   for (int b = 0; b < nbod; ++b) {
@@ -459,8 +464,8 @@ void upload_bodies(ando_ctx* c, const ando_rigid_body* bodies, int nbod) {
       C.d_local.upload(C.local, s);
       C.d_tris.upload(C.tris, s);
       C.d_world.ensure(nv3);
-      C.d_gbox.ensure(6 * (size_t)((B.num_triangles + kGroup - 1) / kGroup + 1));
     }
+    C.d_gbox.ensure(6 * (size_t)((B.num_triangles + c->rigid_group - 1) / c->rigid_group + 1));
     float pose[12];
     std::memcpy(pose, B.position, 3 * sizeof(float));
     std::memcpy(pose + 3, B.rotation, 9 * sizeof(float));
@@ -468,10 +473,10 @@ void upload_bodies(ando_ctx* c, const ando_rigid_body* bodies, int nbod) {
     if (B.num_vertices)
       hipLaunchKernelGGL(k_rigid_world, dim3(nblk(B.num_vertices)), dim3(kBlock), 0, s, B.num_vertices,
                          C.d_local.p, C.d_pose.p, C.d_world.p);
-    int ng = (B.num_triangles + kGroup - 1) / kGroup;
+    int ng = (B.num_triangles + c->rigid_group - 1) / c->rigid_group;
     if (ng)
-      hipLaunchKernelGGL(k_rigid_groups, dim3(nblk(ng)), dim3(kBlock), 0, s, B.num_triangles, C.d_tris.p,
-                         C.d_world.p, C.d_gbox.p);
+      hipLaunchKernelGGL(k_rigid_groups, dim3(nblk(ng)), dim3(kBlock), 0, s, B.num_triangles, c->rigid_group,
+                         C.d_tris.p, C.d_world.p, C.d_gbox.p);
     C.d_bbox.ensure(6);
     hipLaunchKernelGGL(k_rigid_bbox, dim3(1), dim3(kBlock), 0, s, ng, C.d_gbox.p, C.d_bbox.p);
     HIPCHK(hipGetLastError());
@@ -484,6 +489,7 @@ DetectBodies detect_bodies(ando_ctx* c, const ando_rigid_body* bodies) {
   for (int b = 0; b < D.count; ++b) {
     D.b[b].nv = bodies[b].num_vertices;
     D.b[b].nt = bodies[b].num_triangles;
+    D.b[b].grp = c->rigid_group;
     D.b[b].world = c->bodies[b].d_world.p;
     D.b[b].tris = c->bodies[b].d_tris.p;
     D.b[b].gbox = c->bodies[b].d_gbox.p;
@@ -499,8 +505,9 @@ void detect_rigid(ando_ctx* c, const ando_rigid_body* bodies, int nbod, float dt
   if (nbod == 0) return;
   hipStream_t s = c->stream;
   DetectBodies D = detect_bodies(c, bodies);
+  c->rc_hits.ensure((size_t)c->n * kHitCache);
   hipLaunchKernelGGL(k_rigid_detect, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, 0, c->x.p, D, c->rc_ptr.p,
-                     c->rc_count.p, c->rc.p, c->masses.p, c->d_diag.p, c->Helast3.p, dt);
+                     c->rc_count.p, c->rc.p, c->masses.p, c->d_diag.p, c->Helast3.p, dt, c->rc_hits.p);
   int nbs = (c->n + 1023) / 1024;
   hipLaunchKernelGGL(k_scan_local, dim3(nbs), dim3(kBlock), 0, s, c->n, c->rc_count.p, c->rc_ptr.p,
                      c->scan_sums.p);
@@ -515,7 +522,7 @@ void detect_rigid(ando_ctx* c, const ando_rigid_body* bodies, int nbod, float dt
   c->rc.ensure((size_t)total + 1);
   if (total)
     hipLaunchKernelGGL(k_rigid_detect, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, 1, c->x.p, D, c->rc_ptr.p,
-                       c->rc_count.p, c->rc.p, c->masses.p, c->d_diag.p, c->Helast3.p, dt);
+                       c->rc_count.p, c->rc.p, c->masses.p, c->d_diag.p, c->Helast3.p, dt, c->rc_hits.p);
   HIPCHK(hipGetLastError());
 }
 
@@ -1227,6 +1234,7 @@ int ando_create(int32_t device, ando_ctx** out) {
     }
     if (const char* w = std::getenv("ANDO_PCG_WGS")) c->pcg_wgs = std::atoi(w);
     if (const char* w = std::getenv("ANDO_PCG_CHUNK")) c->pcg_chunk = std::min(256, std::max(1, std::atoi(w)));
+    if (const char* w = std::getenv("ANDO_RIGID_GROUP")) c->rigid_group = std::min(256, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_BARRIER")) c->pcg_barrier = std::atoi(w) == 1 ? 1 : 0;
     if (const char* w = std::getenv("ANDO_PCG_ROWSETS")) c->spmv_rowsets = std::min(4, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_SPMV")) {

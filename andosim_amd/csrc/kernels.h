@@ -6,7 +6,7 @@
 namespace ando {
 
 constexpr int kBlock = 256;   // threads per workgroup (4 x wave64)
-constexpr int kGroup = 32;    // rigid triangles per culling group
+constexpr int kGroup = 16;    // default rigid triangles per culling group
 constexpr int kMaxBodies = 16;
 
 // One detected rigid contact (ContactPair of type RIGID_POINT_TRIANGLE,
@@ -64,9 +64,10 @@ struct BodyState {
 
 struct DetectBody {
   int32_t nv, nt;
+  int32_t grp;           // triangles per culling group
   const float* world;    // (nv, 3)
   const int32_t* tris;   // (nt, 3)
-  const float* gbox;     // (ceil(nt/kGroup), 6): min xyz, max xyz, grown by 0.01 + 1e-5
+  const float* gbox;     // (ceil(nt/grp), 6): min xyz, max xyz, grown by 0.01 + 1e-5
   const float* bbox;     // (6): union of gbox
 };
 struct DetectBodies {

@@ -172,7 +172,7 @@ __global__ void k_axpy(int n3, float s, const float* __restrict__ d, float* __re
 // ---------------------------------------------------------------------------
 // Rigid-body contact detection (collision.cpp:536-580): every cloth vertex
 // against every rigid triangle, kept if the unsigned distance < 0.01.  Groups
-// of 32 consecutive triangles carry an AABB grown by 0.01 + 1e-5, so culling
+// of consecutive triangles carry an AABB grown by 0.01 + 1e-5, so culling
 // never drops a pair the distance test would keep; the surviving pairs are
 // visited in (body, triangle) order, the reference's order.
 // ---------------------------------------------------------------------------
@@ -187,13 +187,13 @@ __global__ void k_rigid_world(int nv, const float* __restrict__ local, const flo
   st3(world, i, mk(pose[0], pose[1], pose[2]) + Rv);
 }
 
-__global__ void k_rigid_groups(int nt, const int32_t* __restrict__ tris, const float* __restrict__ world,
+__global__ void k_rigid_groups(int nt, int grp, const int32_t* __restrict__ tris, const float* __restrict__ world,
                                float* __restrict__ gbox) {
   int gi = blockIdx.x * blockDim.x + threadIdx.x;
-  int ng = (nt + kGroup - 1) / kGroup;
+  int ng = (nt + grp - 1) / grp;
   if (gi >= ng) return;
   float mn[3] = {3.4e38f, 3.4e38f, 3.4e38f}, mx[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
-  for (int t = gi * kGroup; t < min(nt, (gi + 1) * kGroup); ++t)
+  for (int t = gi * grp; t < min(nt, (gi + 1) * grp); ++t)
     for (int c = 0; c < 3; ++c) {
       f3 p = ld3(world, tris[3 * t + c]);
       mn[0] = fminf(mn[0], p.x); mn[1] = fminf(mn[1], p.y); mn[2] = fminf(mn[2], p.z);
@@ -244,15 +244,31 @@ __global__ void k_rigid_bbox(int ng, const float* __restrict__ gbox, float* __re
 
 // Pass 0 counts, pass 1 writes records at ptr[i] in reference order.  Group
 // boxes are staged through LDS 256 at a time; a workgroup skips a body when
-// none of its 256 vertices lies in the body's grown box.
+// none of its 256 vertices lies in the body's grown box.  Pass 0 also keeps
+// the (body, triangle) of a vertex's first kHitCache hits in `hits`; pass 1
+// re-evaluates only those when the vertex had no more, instead of scanning
+// again (the same arithmetic on the same inputs, so the same records).
 constexpr int kGroupTile = 256;
+constexpr int kHitCache = 16;
+
+__device__ inline void write_rigid_contact(RigidContact& o, const PTResult& r, float mass, f3 E, float dt, int b,
+                                           int t, int i) {
+  o.gap = r.dist;
+  o.nx = r.normal.x; o.ny = r.normal.y; o.nz = r.normal.z;
+  o.qx = r.wq.x; o.qy = r.wq.y; o.qz = r.wq.z;
+  o.k = rigid_contact_k(mass, E, r.normal, dt);
+  o.body = b;
+  o.tri = t;
+  o.vertex = i;
+}
 
 __global__ void __launch_bounds__(kBlock) k_rigid_detect(int n, int pass, const float* __restrict__ x,
                                                          DetectBodies bodies, const int32_t* __restrict__ ptr,
                                                          int32_t* __restrict__ count, RigidContact* __restrict__ out,
                                                          const float* __restrict__ masses,
                                                          const int32_t* __restrict__ diag,
-                                                         const float* __restrict__ Helast3, float dt) {
+                                                         const float* __restrict__ Helast3, float dt,
+                                                         int32_t* __restrict__ hits) {
   __shared__ float sbox[6 * kGroupTile];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = i < n;
@@ -266,13 +282,29 @@ __global__ void __launch_bounds__(kBlock) k_rigid_detect(int n, int pass, const 
     E = mk(e[0], e[1], e[2]);
     mass = masses[i];
   }
+  // Pass 1 of a vertex whose hits all fit the cache: only those triangles.
+  const int cached = (pass && live) ? count[i] : kHitCache + 1;
+  if (pass && cached <= kHitCache) {
+    for (int h = 0; h < cached; ++h) {
+      const int code = hits[(size_t)i * kHitCache + h];
+      const int b = code >> 24, t = code & 0xffffff;
+      const DetectBody& B = bodies.b[b];
+      f3 a = ld3(B.world, B.tris[3 * t]), bq = ld3(B.world, B.tris[3 * t + 1]);
+      f3 cq = ld3(B.world, B.tris[3 * t + 2]);
+      PTResult r = point_triangle(p, a, bq, cq);
+      write_rigid_contact(out[base + h], r, mass, E, dt, b, t, i);
+    }
+  }
+  // Every thread of the workgroup takes part in the staging below; a vertex
+  // served from the cache only skips its own tests.
+  const bool scan = live && !(pass && cached <= kHitCache);
   for (int b = 0; b < bodies.count; ++b) {
     const DetectBody& B = bodies.b[b];
     const float* bb0 = B.bbox;
-    bool in_body = live && !(p.x < bb0[0] || p.y < bb0[1] || p.z < bb0[2] || p.x > bb0[3] || p.y > bb0[4] ||
+    bool in_body = scan && !(p.x < bb0[0] || p.y < bb0[1] || p.z < bb0[2] || p.x > bb0[3] || p.y > bb0[4] ||
                              p.z > bb0[5]);
     if (!__syncthreads_or(in_body)) continue;
-    const int ng = (B.nt + kGroup - 1) / kGroup;
+    const int ng = (B.nt + B.grp - 1) / B.grp;
     for (int g0 = 0; g0 < ng; g0 += kGroupTile) {
       const int gn = min(kGroupTile, ng - g0);
       for (int k = threadIdx.x; k < 6 * gn; k += blockDim.x) sbox[k] = B.gbox[6 * g0 + k];
@@ -282,22 +314,14 @@ __global__ void __launch_bounds__(kBlock) k_rigid_detect(int n, int pass, const 
           const float* bx = sbox + 6 * gl;
           if (p.x < bx[0] || p.y < bx[1] || p.z < bx[2] || p.x > bx[3] || p.y > bx[4] || p.z > bx[5]) continue;
           const int gi = g0 + gl;
-          const int t1 = min(B.nt, (gi + 1) * kGroup);
-          for (int t = gi * kGroup; t < t1; ++t) {
+          const int t1 = min(B.nt, (gi + 1) * B.grp);
+          for (int t = gi * B.grp; t < t1; ++t) {
             f3 a = ld3(B.world, B.tris[3 * t]), bq = ld3(B.world, B.tris[3 * t + 1]);
             f3 cq = ld3(B.world, B.tris[3 * t + 2]);
             PTResult r = point_triangle(p, a, bq, cq);
             if (r.dist < 0.01f) {
-              if (pass) {
-                RigidContact& o = out[base + c];
-                o.gap = r.dist;
-                o.nx = r.normal.x; o.ny = r.normal.y; o.nz = r.normal.z;
-                o.qx = r.wq.x; o.qy = r.wq.y; o.qz = r.wq.z;
-                o.k = rigid_contact_k(mass, E, r.normal, dt);
-                o.body = b;
-                o.tri = t;
-                o.vertex = i;
-              }
+              if (pass) write_rigid_contact(out[base + c], r, mass, E, dt, b, t, i);
+              else if (c < kHitCache) hits[(size_t)i * kHitCache + c] = (b << 24) | t;
               ++c;
             }
           }
