@@ -231,7 +231,8 @@ struct ando_ctx {
   std::vector<BodyCache> bodies;
   DBuf<int32_t> rc_count, rc_ptr, scan_sums, scan_total;
   DBuf<ando::RigidContact> rc;
-  DBuf<int32_t> rc_hits;  // per vertex, the first kHitCache (body << 24 | triangle) hits of the count pass
+  DBuf<int32_t> rc_hits;
+  DBuf<double> rw_part, rw_out;  // rigid coupling: per-workgroup and per-body force/torque sums  // per vertex, the first kHitCache (body << 24 | triangle) hits of the count pass
   int rc_total = 0;
   // PCG.
   DBuf<float> r, z, pbuf0, pbuf1, Ap, Pinv;
@@ -257,6 +258,7 @@ struct ando_ctx {
   SpmvMeta spmv_meta;      // for the mesh's block pattern
   // Corrected contact mode (self point-triangle and edge-edge through the LBVH).
   int corrected = 0;
+  int static_rigids = 0;   // rigid bodies are fixed colliders: no coupling, no body update
   float pt_box = 1e-4f, keep_gap = 0.01f;
   std::vector<int32_t> edges_h;
   DBuf<int32_t> d_edges;
@@ -1120,44 +1122,30 @@ void rigid_integrate(ando_rigid_body& B, const double* force, const double* torq
 
 // Integrator::apply_rigid_coupling (integrator.cpp:676-754): contacts at the
 // final positions push the bodies with -dV/dg n at the witness point.
+// The per-body sums run on the device (k_rigid_forces); only 6 doubles per
+// body come back, and the body update itself (a few dozen flops per body, on
+// the caller's ando_rigid_body) stays on the host.
 void rigid_coupling(ando_ctx* c, const ando_params& p, ando_rigid_body* bodies, int nbod) {
   upload_bodies(c, bodies, nbod);
   detect_rigid(c, bodies, nbod, p.dt);
-  std::vector<RigidContact> hc(c->rc_total);
-  if (c->rc_total)
-    HIPCHK(hipMemcpyAsync(hc.data(), c->rc.p, sizeof(RigidContact) * c->rc_total, hipMemcpyDeviceToHost,
-                          c->stream));
-  sync(c);
-  std::vector<float> F(3 * nbod, 0.0f), T(3 * nbod, 0.0f);
-  for (const auto& rcn : hc) {
-    if (rcn.body < 0 || rcn.body >= nbod) continue;
-    float nv[3] = {rcn.nx, rcn.ny, rcn.nz};
-    float nl = std::sqrt(nv[0] * nv[0] + (nv[1] * nv[1] + nv[2] * nv[2]));
-    if ((double)nl < 1e-8) continue;
-    for (float& q : nv) q = q / nl;
-    float dV = 0.0f;
-    float gmax = p.contact_gap_max, g = rcn.gap;
-    if (!(gmax <= 0.0f || g >= gmax)) {
-      float dd = gmax - g;
-      float inv = 1.0f / gmax;
-      dV = -1.5f * rcn.k * inv * dd * dd;
-    }
-    if ((double)std::fabs(dV) < 1e-12) continue;
-    const ando_rigid_body& B = bodies[rcn.body];
-    if (B.mass <= 0.0f) continue;
-    float f[3] = {-(dV * nv[0]), -(dV * nv[1]), -(dV * nv[2])};
-    float rr[3] = {rcn.qx - B.position[0], rcn.qy - B.position[1], rcn.qz - B.position[2]};
-    float* Fb = &F[3 * rcn.body];
-    float* Tb = &T[3 * rcn.body];
-    for (int k = 0; k < 3; ++k) Fb[k] = Fb[k] + f[k];
-    Tb[0] = Tb[0] + (rr[1] * f[2] - rr[2] * f[1]);
-    Tb[1] = Tb[1] + (rr[2] * f[0] - rr[0] * f[2]);
-    Tb[2] = Tb[2] + (rr[0] * f[1] - rr[1] * f[0]);
-  }
+  hipStream_t s = c->stream;
+  BodyFrames bf{};
   for (int b = 0; b < nbod; ++b) {
-    double fd[3] = {F[3 * b], F[3 * b + 1], F[3 * b + 2]}, td[3] = {T[3 * b], T[3 * b + 1], T[3 * b + 2]};
-    rigid_integrate(bodies[b], fd, td, p.dt);
+    for (int k = 0; k < 3; ++k) bf.pos[b][k] = bodies[b].position[k];
+    bf.mass[b] = bodies[b].mass;
   }
+  const int G = std::max(1, std::min(64, nblk(c->rc_total)));
+  c->rw_part.ensure((size_t)G * nbod * 6);
+  c->rw_out.ensure((size_t)nbod * 6);
+  hipLaunchKernelGGL(k_rigid_forces, dim3(G), dim3(kBlock), 0, s, c->rc_total, (const RigidContact*)c->rc.p, nbod,
+                     bf, p.contact_gap_max, c->rw_part.p);
+  hipLaunchKernelGGL(k_rigid_forces_final, dim3(1), dim3(kBlock), 0, s, G, nbod, (const double*)c->rw_part.p,
+                     c->rw_out.p);
+  HIPCHK(hipGetLastError());
+  std::vector<double> w(6 * (size_t)nbod);
+  HIPCHK(hipMemcpyAsync(w.data(), c->rw_out.p, sizeof(double) * w.size(), hipMemcpyDeviceToHost, s));
+  sync(c);
+  for (int b = 0; b < nbod; ++b) rigid_integrate(bodies[b], &w[6 * b], &w[6 * b + 3], p.dt);
 }
 
 // Integrator::apply_contact_restitution (integrator.cpp:565-674).
@@ -1436,7 +1424,7 @@ int ando_set_mode(ando_ctx* c, int32_t flags) {
     need(c != nullptr, ANDO_EINVAL, "null context");
     need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING | ANDO_MODE_PCG_PERSISTENT |
                     ANDO_MODE_LEARNED_PRECOND | ANDO_MODE_LEARNED_WARM_START_ONLY |
-                    ANDO_MODE_CORRECTED_CONTACT | ANDO_MODE_PCG_ROW_SPMV)) == 0,
+                    ANDO_MODE_CORRECTED_CONTACT | ANDO_MODE_PCG_ROW_SPMV | ANDO_MODE_STATIC_RIGIDS)) == 0,
          ANDO_EINVAL, "unknown mode flag");
     need(!(flags & ANDO_MODE_LEARNED_PRECOND) || c->nn.ready, ANDO_ESTATE,
          "learned preconditioner mode needs ando_set_precond_weights first");
@@ -1446,6 +1434,7 @@ int ando_set_mode(ando_ctx* c, int32_t flags) {
     c->learned = (flags & ANDO_MODE_LEARNED_PRECOND) ? ((flags & ANDO_MODE_LEARNED_WARM_START_ONLY) ? 2 : 1) : 0;
     c->corrected = (flags & ANDO_MODE_CORRECTED_CONTACT) ? 1 : 0;
     if (c->spmv_env < 0) c->spmv_sell = (flags & ANDO_MODE_PCG_ROW_SPMV) ? 0 : 1;
+    c->static_rigids = (flags & ANDO_MODE_STATIC_RIGIDS) ? 1 : 0;
   });
 }
 
@@ -1505,7 +1494,7 @@ int ando_step(ando_ctx* c, const ando_params* params, ando_rigid_body* bodies, i
     HIPCHK(hipMemcpyAsync(&sact, c->scalars.p + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     sync(c);
     c->stats.strain_active = (int32_t)sact;
-    if (num_bodies > 0) rigid_coupling(c, p, bodies, num_bodies);
+    if (num_bodies > 0 && !c->static_rigids) rigid_coupling(c, p, bodies, num_bodies);
     sync(c);
     c->stats.step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   });

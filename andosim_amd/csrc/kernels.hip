@@ -333,6 +333,76 @@ __global__ void __launch_bounds__(kBlock) k_rigid_detect(int n, int pass, const 
 }
 
 // ---------------------------------------------------------------------------
+// Integrator::apply_rigid_coupling's force and torque sums (integrator.cpp:
+// 676-754): each rigid contact at the final positions pushes its body with
+// f = -V'(g) n (n normalised; skipped if |n| < 1e-8, |V'| < 1e-12 or the body
+// has no mass) at the witness point.  Per-contact terms are the reference's
+// float expressions; each workgroup sums them per body in double (fixed tree
+// order), and k_rigid_forces_final adds the workgroup partials in order.
+// ---------------------------------------------------------------------------
+struct BodyFrames {
+  float pos[kMaxBodies][3];
+  float mass[kMaxBodies];
+};
+
+struct Wrench {
+  double v[6];
+  __device__ void add(const Wrench& o) {
+    for (int k = 0; k < 6; ++k) v[k] += o.v[k];
+  }
+};
+
+__global__ void __launch_bounds__(kBlock) k_rigid_forces(int count, const RigidContact* __restrict__ rc, int nbod,
+                                                         BodyFrames bf, float gmax, double* __restrict__ part) {
+  __shared__ Wrench lds[kBlock];
+  for (int b = 0; b < nbod; ++b) {
+    Wrench a;
+    for (int k = 0; k < 6; ++k) a.v[k] = 0.0;
+    for (int e = blockIdx.x * kBlock + threadIdx.x; e < count; e += gridDim.x * kBlock) {
+      const RigidContact& r = rc[e];
+      if (r.body != b) continue;
+      float nv[3] = {r.nx, r.ny, r.nz};
+      const float nl = sqrtf(nv[0] * nv[0] + (nv[1] * nv[1] + nv[2] * nv[2]));
+      if ((double)nl < 1e-8) continue;
+      for (int k = 0; k < 3; ++k) nv[k] = nv[k] / nl;
+      float dV = 0.0f;
+      if (!(gmax <= 0.0f || r.gap >= gmax)) {
+        const float dd = gmax - r.gap;
+        const float inv = 1.0f / gmax;
+        dV = -1.5f * r.k * inv * dd * dd;
+      }
+      if ((double)fabsf(dV) < 1e-12) continue;
+      if (bf.mass[b] <= 0.0f) continue;
+      const float f[3] = {-(dV * nv[0]), -(dV * nv[1]), -(dV * nv[2])};
+      const float rr[3] = {r.qx - bf.pos[b][0], r.qy - bf.pos[b][1], r.qz - bf.pos[b][2]};
+      for (int k = 0; k < 3; ++k) a.v[k] += (double)f[k];
+      a.v[3] += (double)(rr[1] * f[2] - rr[2] * f[1]);
+      a.v[4] += (double)(rr[2] * f[0] - rr[0] * f[2]);
+      a.v[5] += (double)(rr[0] * f[1] - rr[1] * f[0]);
+    }
+    const int t = threadIdx.x;
+    lds[t] = a;
+    __syncthreads();
+    for (int sft = kBlock / 2; sft > 0; sft >>= 1) {
+      if (t < sft) lds[t].add(lds[t + sft]);
+      __syncthreads();
+    }
+    if (t < 6) part[((size_t)blockIdx.x * nbod + b) * 6 + t] = lds[0].v[t];
+    __syncthreads();
+  }
+}
+
+// out[6 b + k] = sum over workgroups g (in order) of part[(g nbod + b) 6 + k].
+__global__ void k_rigid_forces_final(int G, int nbod, const double* __restrict__ part, double* __restrict__ out) {
+  const int t = threadIdx.x;
+  if (t >= 6 * nbod) return;
+  const int b = t / 6, k = t % 6;
+  double s = 0.0;
+  for (int g = 0; g < G; ++g) s += part[((size_t)g * nbod + b) * 6 + k];
+  out[t] = s;
+}
+
+// ---------------------------------------------------------------------------
 // Exclusive scan of int32 counts (three phases, 1024 items per workgroup).
 // ---------------------------------------------------------------------------
 // Exclusive scan of one value per thread over the workgroup (blockDim a

@@ -202,6 +202,12 @@ int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, co
  * reads the BSR values in place instead (8 lanes per block row).  Both sum a
  * row's blocks in the same order and give the same product bit for bit. */
 #define ANDO_MODE_PCG_ROW_SPMV 64
+/* ANDO_MODE_STATIC_RIGIDS (new): the rigid bodies are fixed colliders.  Their
+ * contacts still act on the cloth, but Integrator::apply_rigid_coupling
+ * (integrator.cpp:676-754) is skipped: no contact forces on the bodies and no
+ * RigidBody::integrate, so the bodies passed to ando_step are left unchanged.
+ * Meant for obstacles the reference models as very dense bodies. */
+#define ANDO_MODE_STATIC_RIGIDS 128
 int ando_set_mode(ando_ctx* ctx, int32_t flags);
 
 int ando_apply_gravity(ando_ctx* ctx, const float* gravity3, float dt);
