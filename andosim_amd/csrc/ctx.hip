@@ -259,6 +259,7 @@ struct ando_ctx {
   // Corrected contact mode (self point-triangle and edge-edge through the LBVH).
   int corrected = 0;
   int static_rigids = 0;   // rigid bodies are fixed colliders: no coupling, no body update
+  int additive_ccd = 0;    // corrected mode: additive CCD instead of the 10-sample test in the line search
   float pt_box = 1e-4f, keep_gap = 0.01f;
   std::vector<int32_t> edges_h;
   DBuf<int32_t> d_edges;
@@ -1011,7 +1012,7 @@ float line_search(ando_ctx* c) {
   if (self)
     hipLaunchKernelGGL(ando::lbvh::k_ls_self, dim3(nblk(c->sc_total)), dim3(kBlock), 0, s, c->sc_total,
                        (const ando::SelfContact*)c->sc.p, (const float*)c->x.p, (const float*)c->d.p, 1.25f,
-                       c->scalars.p + 1);
+                       c->additive_ccd, c->scalars.p + 1);
   HIPCHK(hipGetLastError());
   uint32_t mask = 0;
   HIPCHK(hipMemcpyAsync(&mask, c->scalars.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -1424,7 +1425,8 @@ int ando_set_mode(ando_ctx* c, int32_t flags) {
     need(c != nullptr, ANDO_EINVAL, "null context");
     need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING | ANDO_MODE_PCG_PERSISTENT |
                     ANDO_MODE_LEARNED_PRECOND | ANDO_MODE_LEARNED_WARM_START_ONLY |
-                    ANDO_MODE_CORRECTED_CONTACT | ANDO_MODE_PCG_ROW_SPMV | ANDO_MODE_STATIC_RIGIDS)) == 0,
+                    ANDO_MODE_CORRECTED_CONTACT | ANDO_MODE_PCG_ROW_SPMV | ANDO_MODE_STATIC_RIGIDS |
+                    ANDO_MODE_ADDITIVE_CCD)) == 0,
          ANDO_EINVAL, "unknown mode flag");
     need(!(flags & ANDO_MODE_LEARNED_PRECOND) || c->nn.ready, ANDO_ESTATE,
          "learned preconditioner mode needs ando_set_precond_weights first");
@@ -1435,6 +1437,7 @@ int ando_set_mode(ando_ctx* c, int32_t flags) {
     c->corrected = (flags & ANDO_MODE_CORRECTED_CONTACT) ? 1 : 0;
     if (c->spmv_env < 0) c->spmv_sell = (flags & ANDO_MODE_PCG_ROW_SPMV) ? 0 : 1;
     c->static_rigids = (flags & ANDO_MODE_STATIC_RIGIDS) ? 1 : 0;
+    c->additive_ccd = (flags & ANDO_MODE_ADDITIVE_CCD) ? 1 : 0;
   });
 }
 

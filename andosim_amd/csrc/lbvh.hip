@@ -623,8 +623,54 @@ __global__ void __launch_bounds__(256) k_self_spmv(int n, const int32_t* __restr
 // for trial t (alpha = 2^-t) the contact fails if its 10-sample CCD finds a
 // distance below 1e-6 at a sample time t' < 1.  Bit t of the AND-reduced mask
 // is set when every contact passes trial t.
+// Additive CCD (Li, Kaufman, Jiang et al., "Codimensional Incremental
+// Potential Contact", 2021, Algorithm 1), thickness 0, t_c = 1: the pair moves
+// by p (per vertex) over t in [0, 1]; with the mean displacement removed, l_p
+// bounds how fast the distance can shrink, so advancing by (1 - s) d / l_p,
+// then 0.9 d / l_p, never crosses.  Returns 1 when the whole motion keeps the
+// pair apart, otherwise the last time at which the distance was still at
+// least s times its initial value (a conservative time of impact).
+constexpr float kAccdS = 0.1f;
+constexpr int kAccdMaxIter = 256;
+
+__device__ inline float pair_distance(int type, const f3* q) {
+  if (type == 0) return point_triangle(q[0], q[1], q[2], q[3]).dist;
+  float dist;
+  f3 nn, wp, wq;
+  edge_edge(q[0], q[1], q[2], q[3], dist, nn, wp, wq);
+  return dist;
+}
+
+__device__ inline float additive_ccd(int type, const f3* x0, const f3* p) {
+  const f3 pbar = mk(((p[0].x + p[1].x) + (p[2].x + p[3].x)) * 0.25f, ((p[0].y + p[1].y) + (p[2].y + p[3].y)) * 0.25f,
+                     ((p[0].z + p[1].z) + (p[2].z + p[3].z)) * 0.25f);
+  f3 q[4], x[4];
+  for (int k = 0; k < 4; ++k) {
+    q[k] = p[k] - pbar;
+    x[k] = x0[k];
+  }
+  const float lp = type == 0 ? nrm(q[0]) + fmaxf(nrm(q[1]), fmaxf(nrm(q[2]), nrm(q[3])))
+                             : fmaxf(nrm(q[0]), nrm(q[1])) + fmaxf(nrm(q[2]), nrm(q[3]));
+  if (lp == 0.0f) return 1.0f;
+  float dist = pair_distance(type, x);
+  if (!(dist > 0.0f)) return 0.0f;
+  const float g = kAccdS * dist;
+  float t = 0.0f, tl = (1.0f - kAccdS) * dist / lp;
+  for (int it = 0; it < kAccdMaxIter; ++it) {
+    for (int k = 0; k < 4; ++k) x[k] = x[k] + tl * q[k];
+    dist = pair_distance(type, x);
+    if (t > 0.0f && dist < g) return t;
+    t = t + tl;
+    if (t > 1.0f) return 1.0f;
+    tl = 0.9f * dist / lp;
+  }
+  return t;
+}
+
+// accd = 0: the reference's 10-sample test; 1: additive CCD over each trial's
+// motion (the trial is feasible iff the whole motion is free).
 __global__ void k_ls_self(int nc, const SelfContact* __restrict__ sc, const float* __restrict__ x,
-                          const float* __restrict__ d, float ext, uint32_t* __restrict__ mask) {
+                          const float* __restrict__ d, float ext, int accd, uint32_t* __restrict__ mask) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t m = 0xFFFFFu;
   if (c < nc) {
@@ -645,7 +691,11 @@ __global__ void k_ls_self(int nc, const SelfContact* __restrict__ sc, const floa
         still = still && (double)sqn(x1[k] - x0[k]) < 1e-12;
       }
       bool ok = true;
-      if (!still)
+      if (accd) {
+        f3 pm[4];
+        for (int k = 0; k < 4; ++k) pm[k] = x1[k] - x0[k];
+        ok = still || additive_ccd(C.type, x0, pm) >= 1.0f;
+      } else if (!still)
         for (int smp = 1; smp <= 10; ++smp) {
           const float tt = (float)smp / 10.0f;
           f3 q[4];

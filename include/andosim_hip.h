@@ -208,6 +208,13 @@ int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, co
  * RigidBody::integrate, so the bodies passed to ando_step are left unchanged.
  * Meant for obstacles the reference models as very dense bodies. */
 #define ANDO_MODE_STATIC_RIGIDS 128
+/* ANDO_MODE_ADDITIVE_CCD (new; with ANDO_MODE_CORRECTED_CONTACT): the line
+ * search tests every self contact's motion over a trial step with additive
+ * CCD (a conservative time of impact, Li et al. 2021) instead of the
+ * reference's 10 distance samples (line_search.cpp:182-243), which a fast
+ * primitive can pass between.  A trial is feasible only if the whole motion
+ * keeps every contact pair apart. */
+#define ANDO_MODE_ADDITIVE_CCD 256
 int ando_set_mode(ando_ctx* ctx, int32_t flags);
 
 int ando_apply_gravity(ando_ctx* ctx, const float* gravity3, float dt);

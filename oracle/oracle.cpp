@@ -512,6 +512,7 @@ template <class R> struct Sim {
   // vertex/triangle and edge/edge pair tested by brute force with the
   // reference's intended predicate; PT and EE barriers both enter the energy).
   int contact_mode = 0;
+  int ccd_mode = 0;  // corrected mode line search: 0 the reference's 10 samples, 1 additive CCD
   R pt_box = R(1e-4), keep_gap = R(0.01);
   // Contact Hessian blocks that couple two different vertices.  The reference
   // folds them into a pattern built from the triplets; here the pattern is the
@@ -1396,7 +1397,12 @@ template <class R> struct Sim {
         V3<R> a0 = X(c.idx[1]), a1 = Xof(xn, c.idx[1]);
         V3<R> b0 = X(c.idx[2]), b1 = Xof(xn, c.idx[2]);
         V3<R> c0 = X(c.idx[3]), c1 = Xof(xn, c.idx[3]);
-        if (ccd_pt(p0, p1, a0, a1, b0, b1, c0, c1) < R(1)) return false;
+        if (ccd_mode == 1) {
+          if (!still4(p0, p1, a0, a1, b0, b1, c0, c1) && accd_pair(0, p0, p1, a0, a1, b0, b1, c0, c1) < R(1))
+            return false;
+        } else if (ccd_pt(p0, p1, a0, a1, b0, b1, c0, c1) < R(1)) {
+          return false;
+        }
         R d; V3<R> nn, q;
         narrow_pt(p1, a1, b1, c1, d, nn, q);
         if (d < gap_min) return false;
@@ -1405,7 +1411,12 @@ template <class R> struct Sim {
         V3<R> b0 = X(c.idx[1]), b1 = Xof(xn, c.idx[1]);
         V3<R> c0 = X(c.idx[2]), c1 = Xof(xn, c.idx[2]);
         V3<R> d0 = X(c.idx[3]), d1 = Xof(xn, c.idx[3]);
-        if (ccd_ee(a0, a1, b0, b1, c0, c1, d0, d1) < R(1)) return false;
+        if (ccd_mode == 1) {
+          if (!still4(a0, a1, b0, b1, c0, c1, d0, d1) && accd_pair(1, a0, a1, b0, b1, c0, c1, d0, d1) < R(1))
+            return false;
+        } else if (ccd_ee(a0, a1, b0, b1, c0, c1, d0, d1) < R(1)) {
+          return false;
+        }
         R d; V3<R> nn, wp, wq;
         narrow_ee(a1, b1, c1, d1, d, nn, wp, wq);
         if (d < gap_min) return false;
@@ -1423,6 +1434,57 @@ template <class R> struct Sim {
       if (!ok) return false;
     }
     return true;
+  }
+  // No vertex of the pair moves (the reference's test at line_search.cpp:182-201).
+  static bool still4(const V3<R>& p0, const V3<R>& p1, const V3<R>& a0, const V3<R>& a1, const V3<R>& b0,
+                     const V3<R>& b1, const V3<R>& c0, const V3<R>& c1) {
+    return (double)sqn3(p1 - p0) < 1e-12 && (double)sqn3(a1 - a0) < 1e-12 && (double)sqn3(b1 - b0) < 1e-12 &&
+           (double)sqn3(c1 - c0) < 1e-12;
+  }
+  // Unsigned distance of a PT (type 0: point, triangle) or EE (type 1) pair.
+  static R pair_distance(int type, const V3<R>* q) {
+    R d;
+    V3<R> nn, w1, w2;
+    if (type == 0) narrow_pt(q[0], q[1], q[2], q[3], d, nn, w1);
+    else narrow_ee(q[0], q[1], q[2], q[3], d, nn, w1, w2);
+    return d;
+  }
+  // Additive CCD (Li, Kaufman, Jiang et al., "Codimensional Incremental
+  // Potential Contact", 2021, Algorithm 1) with thickness 0, s = 0.1, t_c = 1,
+  // on the motion x0 -> x1 of the pair's four vertices: 1 if the whole motion
+  // keeps the pair apart, else a conservative time of impact.  Not part of the
+  // reference (its line search samples 10 distances); corrected mode option.
+  static R additive_ccd(int type, const V3<R>* x0, const V3<R>* p) {
+    const R s = R(0.1f);
+    V3<R> pbar;
+    for (int k = 0; k < 3; ++k) pbar[k] = ((p[0][k] + p[1][k]) + (p[2][k] + p[3][k])) * R(0.25f);
+    V3<R> q[4], x[4];
+    for (int k = 0; k < 4; ++k) {
+      q[k] = p[k] - pbar;
+      x[k] = x0[k];
+    }
+    const R lp = type == 0 ? norm3(q[0]) + std::max(norm3(q[1]), std::max(norm3(q[2]), norm3(q[3])))
+                           : std::max(norm3(q[0]), norm3(q[1])) + std::max(norm3(q[2]), norm3(q[3]));
+    if (lp == R(0)) return R(1);
+    R dist = pair_distance(type, x);
+    if (!(dist > R(0))) return R(0);
+    const R g = s * dist;
+    R t = R(0), tl = (R(1) - s) * dist / lp;
+    for (int it = 0; it < 256; ++it) {
+      for (int k = 0; k < 4; ++k) x[k] = x[k] + tl * q[k];
+      dist = pair_distance(type, x);
+      if (t > R(0) && dist < g) return t;
+      t = t + tl;
+      if (t > R(1)) return R(1);
+      tl = R(0.9f) * dist / lp;
+    }
+    return t;
+  }
+  static R accd_pair(int type, const V3<R>& a0, const V3<R>& a1, const V3<R>& b0, const V3<R>& b1, const V3<R>& c0,
+                     const V3<R>& c1, const V3<R>& d0, const V3<R>& d1) {
+    const V3<R> x0[4] = {a0, b0, c0, d0};
+    const V3<R> p[4] = {a1 - a0, b1 - b0, c1 - c0, d1 - d0};
+    return additive_ccd(type, x0, p);
   }
   R ccd_pt(const V3<R>& p0, const V3<R>& p1, const V3<R>& a0, const V3<R>& a1, const V3<R>& b0, const V3<R>& b1,
            const V3<R>& c0, const V3<R>& c1) const {
@@ -1842,6 +1904,28 @@ int orc_contact_metrics(int prec, int count, const int* type, const int* idx, co
     if (prec == 64) contact_metrics<double>(count, type, idx, gap, nrm, vel, gap_max, ccd, out);
     else contact_metrics<float>(count, type, idx, gap, nrm, vel, gap_max, ccd, out);
   });
+}
+
+int orc_set_ccd_mode(void* h, int mode) {
+  OSim* s = (OSim*)h;
+  return guard([&] { DISPATCH(s, S.ccd_mode = mode == 1 ? 1 : 0); });
+}
+
+// Additive CCD on one pair: type 0 PT / 1 EE, x0 and p 12 doubles each.
+double orc_additive_ccd(int prec, int type, const double* x0, const double* p) {
+  double r = 0.0;
+  auto run = [&](auto tag) {
+    using R = decltype(tag);
+    V3<R> xs[4], ps[4];
+    for (int k = 0; k < 4; ++k)
+      for (int c = 0; c < 3; ++c) {
+        xs[k][c] = (R)x0[3 * k + c];
+        ps[k][c] = (R)p[3 * k + c];
+      }
+    r = (double)Sim<R>::additive_ccd(type, xs, ps);
+  };
+  if (prec == 64) run(0.0); else run(0.0f);
+  return r;
 }
 
 int orc_set_contact_mode(void* h, int mode, double pt_box, double keep_gap) {

@@ -79,6 +79,9 @@ def lib():
         L.orc_count_intersections.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.orc_get_threads.restype = C.c_int
         L.orc_energy.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_set_ccd_mode.argtypes = [C.c_void_p, C.c_int]
+        L.orc_additive_ccd.restype = C.c_double
+        L.orc_additive_ccd.argtypes = [C.c_int, C.c_int, _dp, _dp]
         L.orc_contact_metrics.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, _dp]
         _lib = L
     return _lib
@@ -113,6 +116,14 @@ METRIC_KEYS = ["num_point_triangle", "num_edge_edge", "num_wall", "num_total_con
 def _energy_dict(a):
     return {"kinetic_energy": a[0], "elastic_energy": a[1], "linear_momentum": np.array(a[2:5]),
             "angular_momentum": np.array(a[5:8]), "max_velocity": a[8]}
+
+
+def additive_ccd(kind: str, x0, p, precision=32) -> float:
+    """Additive CCD of one PT ("pt": point, then triangle) or EE ("ee") pair
+    moving by p over [0, 1]: 1 if the motion is free, else a conservative
+    time of impact."""
+    return float(lib().orc_additive_ccd(int(precision), 0 if kind == "pt" else 1, _d(x0).reshape(-1),
+                                        _d(p).reshape(-1)))
 
 
 def contact_metrics(types, idx, gaps, normals, velocities, gap_max, ccd, precision=32):
@@ -193,6 +204,11 @@ class OracleSim:
         """Faithful (self broad phase empty) or corrected (brute-force PT/EE with
         the intended predicate, both barriers in the energy)."""
         self.L.orc_set_contact_mode(self.h, 1 if corrected else 0, pt_box, keep_gap)
+
+    def set_ccd_mode(self, additive: bool):
+        """Corrected mode line search: additive CCD (True) or the reference's
+        10 distance samples (False)."""
+        _chk(self.L.orc_set_ccd_mode(self.h, 1 if additive else 0))
 
     def energy(self, x, v, m):
         """EnergyTracker::compute on (x, v, m) with this mesh: (reference

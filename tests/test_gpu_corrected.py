@@ -114,22 +114,33 @@ def test_corrected_linearize_bit_exact(gap):
     assert np.array_equal(Hg, Ho.astype(np.float32))
 
 
-def test_corrected_step_parity():
+@pytest.mark.parametrize("additive", [False, True])
+def test_corrected_step_parity(additive):
     """Substeps from identical states with self contacts active: positions
     within 1e-6 relative of the FP32 oracle (same contacts, same line-search
-    CCD, PCG to the same tolerance including the coupling blocks)."""
+    CCD — the reference's samples or additive CCD —, PCG to the same tolerance
+    including the coupling blocks)."""
+    print(f"corrected step parity additive={additive}", flush=True)
     sc = resting_stack(8e-5)
     o = OracleSim(sc, 32)
     o.set_contact_mode(True)
-    errs = []
+    o.set_ccd_mode(additive)
+    if additive:
+        # The upper sheet moves down 1 mm per substep onto the lower one (0.08 mm
+        # below), so contact pairs close within a step.
+        v = o.get("v")
+        v[len(v) // 2:, 2] = -0.1
+        o.set("v", v)
+    errs, alphas = [], []
     for k in range(8):
         g = native.GpuSim(sc)
-        g.set_mode(corrected=True)
+        g.set_mode(corrected=True, additive_ccd=additive)
         sync_state(o, g)
         o.substep()
         g.substep()
         errs.append(rel_err(g.get("x"), o.get("x")))
         assert g.stats()["contacts"] == o.stats()["contacts"]
+        alphas.append(g.stats()["last_alpha"])
     assert max(errs) < 1e-6, errs
 
 

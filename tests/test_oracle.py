@@ -539,3 +539,92 @@ def test_contact_metrics_rigid_contacts_have_no_relative_velocity():
     assert (m["num_point_triangle"], m["num_edge_edge"], m["num_wall"]) == (0, 0, 0)
     assert m["max_relative_velocity"] == 0.0 and m["avg_relative_velocity"] == 0.0
     assert m["num_ccd_contacts"] == 1 and m["avg_gap"] == pytest.approx(3.5e-4, rel=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# Additive CCD (corrected-mode option).
+# ---------------------------------------------------------------------------
+def _ee_head_on(d0, drop):
+    """Two parallel edges (crossing in plan), the upper one moving down by drop."""
+    x0 = np.array([[-1.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, -1.0, d0], [0.0, 1.0, d0]])
+    p = np.zeros((4, 3))
+    p[2:, 2] = -drop
+    return x0, p
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_additive_ccd_closed_forms(prec):
+    d0 = 0.01
+    # Head-on approach to contact at t = 0.5: relative displacement 2 d0 with
+    # l_p = 2 d0, so the first advance is 0.9 d0 / l_p = 0.45 (distance 0.1 d0),
+    # and the next one falls below 0.1 d0: the time of impact bound is 0.45.
+    x0, p = _ee_head_on(d0, 2 * d0)
+    assert orc.additive_ccd("ee", x0, p, prec) == pytest.approx(0.45, rel=1e-5)
+    # Half that motion touches exactly at t = 1: not free.
+    x0, p = _ee_head_on(d0, d0)
+    assert orc.additive_ccd("ee", x0, p, prec) == pytest.approx(0.9, rel=1e-5)
+    # A quarter keeps half the distance: free.
+    x0, p = _ee_head_on(d0, 0.5 * d0)
+    assert orc.additive_ccd("ee", x0, p, prec) == 1.0
+    # Separating and rigidly translating pairs are free.
+    x0, p = _ee_head_on(d0, -d0)
+    assert orc.additive_ccd("ee", x0, p, prec) == 1.0
+    assert orc.additive_ccd("ee", x0, np.tile([0.3, -0.2, 5.0], (4, 1)), prec) == 1.0
+    # Touching pairs admit no motion.
+    x0, p = _ee_head_on(0.0, d0)
+    assert orc.additive_ccd("ee", x0, p, prec) == 0.0
+
+
+def _pt_through(z0, z1):
+    """A point above the unit right triangle moving from z0 to z1."""
+    x0 = np.array([[0.25, 0.25, z0], [0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    p = np.zeros((4, 3))
+    p[0, 2] = z1 - z0
+    return x0, p
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_additive_ccd_catches_what_sampling_misses(prec):
+    """A point crossing the triangle between the reference's samples (t = 0.55,
+    samples at 0.1 k): every sample is 5 mm away, so the 10-sample test passes
+    the step; additive CCD stops before the crossing."""
+    z0, z1 = 0.055, -0.045
+    x0, p = _pt_through(z0, z1)
+    ts = np.arange(1, 11) / 10.0
+    assert np.all(np.abs(z0 + ts * (z1 - z0)) > 1e-6)  # sampled distances (the point stays over the face)
+    t = orc.additive_ccd("pt", x0, p, prec)
+    assert 0.0 < t < 0.55
+    # Conservative: the pair is apart on [0, t] (dense check of the signed height).
+    tt = np.linspace(0.0, t, 2001)
+    assert np.min(z0 + tt * (z1 - z0)) > 0.0
+
+
+def test_additive_ccd_is_conservative_on_random_pairs():
+    """Brute force: on random moving PT and EE pairs the distance, sampled
+    densely in double on [0, t_ccd], stays above zero."""
+    rng = np.random.default_rng(11)
+    hits = 0
+    for trial in range(80):
+        kind = "pt" if trial % 2 == 0 else "ee"
+        x0 = rng.uniform(-0.05, 0.05, size=(4, 3))
+        p = rng.uniform(-0.05, 0.05, size=(4, 3))
+        if trial % 4 < 2:
+            # Aim a collision at t = 0.5: the point (or edge 2's midpoint) passes
+            # through a point of the triangle (edge 1) there.
+            if kind == "pt":
+                w = rng.dirichlet([1.0, 1.0, 1.0])
+                target = w @ (x0[1:] + 0.5 * p[1:])
+                p[0] = 2.0 * (target - x0[0])
+            else:
+                u = rng.uniform(0.2, 0.8)
+                target = (1 - u) * (x0[0] + 0.5 * p[0]) + u * (x0[1] + 0.5 * p[1])
+                shift = 2.0 * (target - 0.5 * (x0[2] + x0[3])) - 0.5 * (p[2] + p[3])
+                p[2] += shift
+                p[3] += shift
+        t = orc.additive_ccd(kind, x0, p, 64)
+        hits += t < 1.0
+        for tt in np.linspace(0.0, t, 400):
+            q = x0 + tt * p
+            dist = (orc.narrow_pt if kind == "pt" else orc.narrow_ee)(q[0], q[1], q[2], q[3], 64)[0]
+            assert dist > 0.0, (trial, tt, t)
+    assert hits >= 40  # every aimed motion collides
