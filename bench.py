@@ -81,19 +81,21 @@ def corrected_contact_leg(steps: int):
     """Dense self contact: 8 sheets of 65x65 stacked 0.08 mm apart (jittered
     +-0.03 mm, so they start interpenetrating in places) just above a ground
     wall, run for `steps` substeps in faithful mode (self broad phase empty, as
-    the reference) and in corrected mode (LBVH self PT/EE contacts and
-    barriers); steps/s, contacts per step, PCG iterations and the intersection
-    validator at the end of each run."""
+    the reference), in corrected mode (LBVH self PT/EE contacts and barriers,
+    the reference's sampled CCD in the line search) and in corrected mode with
+    additive CCD; steps/s, contacts per step, PCG iterations and the
+    intersection validator at the end of each run."""
     from andosim_amd import native, scenes
 
     out = {"workload": "stack_resting: 8 sheets 65x65 (33,800 v, 65,536 tris, 98,304 edges) 0.08 mm apart, "
                        "jitter 0.03 mm, ground wall", "steps": steps}
-    for mode in ("faithful", "corrected"):
+    for mode in ("faithful", "corrected", "corrected_additive_ccd"):
         sc = scenes.stack_resting()
         sim = native.GpuSim(sc)
-        sim.set_mode(fp64_near=True, corrected=(mode == "corrected"))
+        sim.set_mode(fp64_near=True, corrected=(mode != "faithful"), additive_ccd=(mode == "corrected_additive_ccd"))
         t = 0.0
         contacts, pcg = [], []
+        ls_fail = 0
         done = 0
         for _ in range(steps):
             t0 = time.perf_counter()
@@ -102,12 +104,14 @@ def corrected_contact_leg(steps: int):
             st = sim.stats()
             contacts.append(st["contacts"])
             pcg.append(st["pcg_iterations"])
+            ls_fail += st["line_search_failures"]
             done += 1
             if not finite_state(sim):
                 break
         et, ins = sim.count_intersections()
         out[mode] = {"steps_per_s": round(done / t, 2), "steps_run": done, "finite": finite_state(sim),
                      "contacts_per_step": contacts, "pcg_iters_per_step": round(float(np.mean(pcg)), 1),
+                     "line_search_failures": ls_fail,
                      "edge_triangle_intersections_at_end": et}
     return out
 
