@@ -512,6 +512,10 @@ template <class R> struct Sim {
   // vertex/triangle and edge/edge pair tested by brute force with the
   // reference's intended predicate; PT and EE barriers both enter the energy).
   int contact_mode = 0;
+  // Test hook: the system matrix and right-hand side of solve number
+  // capture_solve (counted within a step) are kept for inspection.
+  int capture_solve = -1;
+  std::vector<double> cap_H, cap_b;
   int ccd_mode = 0;  // corrected mode line search: 0 the reference's 10 samples, 1 additive CCD
   R pt_box = R(1e-4), keep_gap = R(0.01);
   // Contact Hessian blocks that couple two different vertices.  The reference
@@ -1541,6 +1545,10 @@ template <class R> struct Sim {
       d.assign(3 * n, R(0));
       std::vector<R> b(3 * n);
       for (int i = 0; i < 3 * n; ++i) b[i] = -g[i];
+      if (st.pcg_solves == capture_solve) {
+        cap_H.assign(H.begin(), H.end());
+        cap_b.assign(b.begin(), b.end());
+      }
       int iters = 0;
       bool ok = pcg(n, rp, ci, diag, H, b, d, P.pcg_tol, P.pcg_max_iters, iters, &couplings);
       st.pcg_iters += iters;
@@ -1904,6 +1912,24 @@ int orc_contact_metrics(int prec, int count, const int* type, const int* idx, co
     if (prec == 64) contact_metrics<double>(count, type, idx, gap, nrm, vel, gap_max, ccd, out);
     else contact_metrics<float>(count, type, idx, gap, nrm, vel, gap_max, ccd, out);
   });
+}
+
+int orc_set_capture(void* h, int solve) {
+  OSim* s = (OSim*)h;
+  return guard([&] { DISPATCH(s, S.capture_solve = solve; S.cap_H.clear(); S.cap_b.clear()); });
+}
+// Copies the captured system (9 nb values, 3 n rhs); returns 1 if one was captured.
+int orc_get_capture(void* h, double* H, double* b) {
+  OSim* s = (OSim*)h;
+  int got = 0;
+  DISPATCH(s, {
+    if (!S.cap_H.empty()) {
+      std::copy(S.cap_H.begin(), S.cap_H.end(), H);
+      std::copy(S.cap_b.begin(), S.cap_b.end(), b);
+      got = 1;
+    }
+  });
+  return got;
 }
 
 int orc_set_ccd_mode(void* h, int mode) {

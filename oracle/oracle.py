@@ -80,6 +80,8 @@ def lib():
         L.orc_get_threads.restype = C.c_int
         L.orc_energy.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_set_ccd_mode.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_capture.argtypes = [C.c_void_p, C.c_int]
+        L.orc_get_capture.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_additive_ccd.restype = C.c_double
         L.orc_additive_ccd.argtypes = [C.c_int, C.c_int, _dp, _dp]
         L.orc_contact_metrics.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, _dp]
@@ -204,6 +206,19 @@ class OracleSim:
         """Faithful (self broad phase empty) or corrected (brute-force PT/EE with
         the intended predicate, both barriers in the energy)."""
         self.L.orc_set_contact_mode(self.h, 1 if corrected else 0, pt_box, keep_gap)
+
+    def capture_solve(self, k: int):
+        """Keep the system of the k-th PCG solve of the following steps."""
+        _chk(self.L.orc_set_capture(self.h, int(k)))
+
+    def captured(self):
+        """(BSR values (nb, 3, 3), rhs) of the captured solve, or None."""
+        nb = self.L.orc_num_blocks(self.h)
+        H = np.zeros(9 * nb)
+        b = np.zeros(3 * self.n)
+        if not self.L.orc_get_capture(self.h, H, b):
+            return None
+        return H.reshape(nb, 3, 3), b
 
     def set_ccd_mode(self, additive: bool):
         """Corrected mode line search: additive CCD (True) or the reference's
