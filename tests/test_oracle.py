@@ -628,3 +628,18 @@ def test_additive_ccd_is_conservative_on_random_pairs():
             dist = (orc.narrow_pt if kind == "pt" else orc.narrow_ee)(q[0], q[1], q[2], q[3], 64)[0]
             assert dist > 0.0, (trial, tt, t)
     assert hits >= 40  # every aimed motion collides
+
+
+def test_capture_hook_returns_the_solved_system():
+    """The captured system is the one the step solved: the standalone oracle
+    PCG on it takes the same number of iterations the step recorded."""
+    sc = scenes.small_sheet(9, pins=2, wall=True)
+    o = orc.OracleSim(sc, 32)
+    o.substep()
+    o.capture_solve(0)
+    o.substep()
+    H, b = o.captured()
+    rp, ci = o.pattern()
+    assert H.shape == (len(ci), 3, 3) and b.shape == (3 * o.n,)
+    _, iters, conv = orc.pcg(rp, ci, H, b, tol=sc.params["pcg_tol"], maxit=sc.params["pcg_max_iters"])
+    assert iters == o.stats()["pcg_per_solve"][0]
