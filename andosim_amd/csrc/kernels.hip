@@ -318,6 +318,13 @@ __global__ void __launch_bounds__(kBlock) k_rigid_detect(int n, int pass, const 
           for (int t = gi * B.grp; t < t1; ++t) {
             f3 a = ld3(B.world, B.tris[3 * t]), bq = ld3(B.world, B.tris[3 * t + 1]);
             f3 cq = ld3(B.world, B.tris[3 * t + 2]);
+            // The triangle's own box, grown like the group boxes: a vertex
+            // outside it is farther than 0.01 from the triangle.
+            const float gr = 0.01f + 1e-5f;
+            if (p.x < fminf(a.x, fminf(bq.x, cq.x)) - gr || p.x > fmaxf(a.x, fmaxf(bq.x, cq.x)) + gr ||
+                p.y < fminf(a.y, fminf(bq.y, cq.y)) - gr || p.y > fmaxf(a.y, fmaxf(bq.y, cq.y)) + gr ||
+                p.z < fminf(a.z, fminf(bq.z, cq.z)) - gr || p.z > fmaxf(a.z, fmaxf(bq.z, cq.z)) + gr)
+              continue;
             PTResult r = point_triangle(p, a, bq, cq);
             if (r.dist < 0.01f) {
               if (pass) write_rigid_contact(out[base + c], r, mass, E, dt, b, t, i);
