@@ -157,4 +157,29 @@ __device__ inline T block_reduce(T v, T* lds, Op op) {
   return r;
 }
 
+// A sum (double) and a maximum (float) reduced together: one shuffle tree
+// and one LDS round instead of two.  Each result equals what block_reduce
+// returns for that value alone (same association).
+__device__ inline void block_reduce_sum_max(double& sum, float& mx, double* ldsd, float* ldsf) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int off = 32; off > 0; off >>= 1) {
+    sum += __shfl_down(sum, off, 64);
+    mx = fmaxf(mx, __shfl_down(mx, off, 64));
+  }
+  if (lane == 0) {
+    ldsd[wid] = sum;
+    ldsf[wid] = mx;
+  }
+  __syncthreads();
+  double s = ldsd[0];
+  float m = ldsf[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+    s = s + ldsd[w];
+    m = fmaxf(m, ldsf[w]);
+  }
+  __syncthreads();
+  sum = s;
+  mx = m;
+}
+
 }  // namespace ando

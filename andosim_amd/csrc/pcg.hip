@@ -499,10 +499,9 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_sell(int n, int slices, int
   if (it > 0) {
     const float bmax = st->bmax;
     const double rz_prev = rz_hist[it - 1];
-    double rz = block_reduce(finish_sum_partials(part_rz, nwg_v, pre_rz), ldsd,
-                             [](double a, double b) { return a + b; });
-    float rmax = block_reduce(finish_max_partials(part_max, nwg_v, pre_max), ldsf,
-                              [](float a, float b) { return fmaxf(a, b); });
+    double rz = finish_sum_partials(part_rz, nwg_v, pre_rz);
+    float rmax = finish_max_partials(part_max, nwg_v, pre_max);
+    block_reduce_sum_max(rz, rmax, ldsd, ldsf);
     float rel = ((double)bmax < 1e-16) ? rmax : rmax / bmax;
     if (rel < tol) {
       if (threadIdx.x == 0) {
@@ -779,8 +778,9 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const floa
     }
     if (rm != rm) rm = __int_as_float(0x7f800000);
   }
-  double s = block_reduce(rz, ldsd, [](double a, double b) { return a + b; });
-  float mx = block_reduce(rm, ldsf, [](float a, float b) { return fmaxf(a, b); });
+  double s = rz;
+  float mx = rm;
+  block_reduce_sum_max(s, mx, ldsd, ldsf);
   if (threadIdx.x == 0) {
     part_rz[blockIdx.x] = s;
     part_max[blockIdx.x] = mx;
