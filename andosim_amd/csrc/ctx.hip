@@ -83,6 +83,11 @@ struct BodyCache {
   std::vector<int32_t> tris;
   DBuf<float> d_local, d_world, d_gbox, d_pose, d_bbox;
   DBuf<int32_t> d_tris;
+  // Pose and culling-group size the device world vertices and boxes were made
+  // for; an unchanged body is not transformed again.
+  bool placed = false;
+  float pose[12] = {};
+  int group = 0;
 };
 
 }  // namespace ando
@@ -467,12 +472,17 @@ void upload_bodies(ando_ctx* c, const ando_rigid_body* bodies, int nbod) {
       C.d_local.upload(C.local, s);
       C.d_tris.upload(C.tris, s);
       C.d_world.ensure(nv3);
+      C.placed = false;
     }
-    C.d_gbox.ensure(6 * (size_t)((B.num_triangles + c->rigid_group - 1) / c->rigid_group + 1));
     float pose[12];
     std::memcpy(pose, B.position, 3 * sizeof(float));
     std::memcpy(pose + 3, B.rotation, 9 * sizeof(float));
+    if (C.placed && C.group == c->rigid_group && std::memcmp(pose, C.pose, sizeof(pose)) == 0) continue;
+    C.d_gbox.ensure(6 * (size_t)((B.num_triangles + c->rigid_group - 1) / c->rigid_group + 1));
     C.d_pose.upload(pose, 12, s);
+    std::memcpy(C.pose, pose, sizeof(pose));
+    C.group = c->rigid_group;
+    C.placed = true;
     if (B.num_vertices)
       hipLaunchKernelGGL(k_rigid_world, dim3(nblk(B.num_vertices)), dim3(kBlock), 0, s, B.num_vertices,
                          C.d_local.p, C.d_pose.p, C.d_world.p);
