@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -15,6 +16,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/andosim_hip.h"
@@ -1703,6 +1705,42 @@ int ando_count_intersections(ando_ctx* c, const ando_rigid_body* bodies, int32_t
     if (edge_triangle) *edge_triangle = et;
     if (inside_rigid) *inside_rigid = in;
   });
+}
+
+int ando_step_batch(ando_ctx* const* ctxs, const ando_params* params, ando_rigid_body* const* bodies,
+                    const int32_t* num_bodies, int32_t count, int32_t threads) {
+  std::vector<int> rc;
+  std::vector<std::string> err;
+  const int status = guarded([&] {
+    need(count >= 0 && (count == 0 || (ctxs && params)), ANDO_EINVAL, "null argument");
+    std::vector<const ando_ctx*> seen(ctxs, ctxs + count);
+    std::sort(seen.begin(), seen.end());
+    need(std::adjacent_find(seen.begin(), seen.end()) == seen.end(), ANDO_EINVAL,
+         "a context appears twice in the batch");
+    for (int k = 0; k < count; ++k) need(ctxs[k] != nullptr, ANDO_EINVAL, "null context in the batch");
+    rc.assign(count, ANDO_OK);
+    err.assign(count, std::string());
+    int T = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
+    T = std::max(1, std::min({T, count, 64}));
+    std::atomic<int> next{0};
+    auto work = [&] {
+      for (int k = next++; k < count; k = next++) {
+        rc[k] = ando_step(ctxs[k], &params[k], bodies ? bodies[k] : nullptr, num_bodies ? num_bodies[k] : 0);
+        if (rc[k] != ANDO_OK) err[k] = ando_last_error();
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < T; ++t) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
+  });
+  if (status != ANDO_OK) return status;
+  for (int k = 0; k < (int)rc.size(); ++k)
+    if (rc[k] != ANDO_OK) {
+      set_error("batch entry " + std::to_string(k) + ": " + err[k]);
+      return rc[k];
+    }
+  return ANDO_OK;
 }
 
 int ando_energy_diagnostics(ando_ctx* c, ando_energy* out) {

@@ -25,7 +25,7 @@ ABI_SYMBOLS = [
     "ando_nn_num_weights", "ando_set_precond_weights", "ando_trainer_create", "ando_trainer_destroy",
     "ando_trainer_add_system", "ando_trainer_init_weights", "ando_trainer_set_weights", "ando_trainer_get_weights",
     "ando_trainer_forward", "ando_trainer_loss_grad", "ando_trainer_train", "ando_trainer_solve",
-    "ando_energy_diagnostics", "ando_contact_metrics", "ando_cfl_inputs",
+    "ando_energy_diagnostics", "ando_contact_metrics", "ando_cfl_inputs", "ando_step_batch",
 ]
 
 
@@ -140,6 +140,8 @@ def load():
     L.ando_contact_metrics.argtypes = [C.c_void_p, C.POINTER(Contact), C.c_int32, C.c_float, C.c_int32,
                                        C.POINTER(CollisionMetrics)]
     L.ando_cfl_inputs.argtypes = [C.c_void_p, fp, fp]
+    L.ando_step_batch.argtypes = [C.POINTER(C.c_void_p), C.POINTER(Params), C.POINTER(C.POINTER(RigidBody)),
+                                  C.POINTER(C.c_int32), C.c_int32, C.c_int32]
     _lib = L
     return L
 
@@ -417,6 +419,19 @@ class GpuSim:
         check(self.L.ando_pcg(self.h, n, _ip(rp_), _ip(ci_), _fp(v_), _fp(b_), _fp(x), tol, maxit, C.byref(it),
                               C.byref(conv)))
         return x, it.value, bool(conv.value)
+
+
+def step_batch(sims, threads: int = 0):
+    """ando_step on every GpuSim of `sims` at once (each has its own context
+    and stream); apply_gravity is each caller's, as in GpuSim.substep."""
+    L = load()
+    k = len(sims)
+    ctxs = (C.c_void_p * max(k, 1))(*[s.h.value for s in sims])
+    params = (Params * max(k, 1))(*[s.params for s in sims])
+    bodies = (C.POINTER(RigidBody) * max(k, 1))(*[C.cast(s.bodies, C.POINTER(RigidBody)) if s.nbodies else None
+                                                    for s in sims])
+    nb = (C.c_int32 * max(k, 1))(*[s.nbodies for s in sims])
+    check(L.ando_step_batch(ctxs, params, bodies, nb, k, threads))
 
 
 def nn_num_weights() -> int:

@@ -161,6 +161,34 @@ def per_frame_leg(sim, sc, reps: int = 20, with_cpu: bool = True):
     return out
 
 
+def batched_leg(steps: int, count: int = 256):
+    """The batched workload (`count` cloths of side U{17..65}, 1-4 pins, wind,
+    ground wall; about 481k vertices for 256): cloth-steps/s stepping the
+    contexts one after another (the reference's only way) and through
+    ando_step_batch (one host thread per context group, a stream per cloth)."""
+    from andosim_amd import native, scenes
+
+    out = {"workload": f"batch{count}: {count} independent cloths, sides U{{17..65}}", "substeps": steps}
+    for mode in ("sequential", "batched"):
+        sims = [native.GpuSim(sc) for sc in scenes.batch256(count=count)]
+        for g in sims:  # warm-up substep (constants, allocations)
+            g.substep()
+        t = 0.0
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            if mode == "sequential":
+                for g in sims:
+                    g.substep()
+            else:
+                for g in sims:
+                    g.apply_gravity()
+                native.step_batch(sims, threads=16)
+            t += time.perf_counter() - t0
+        out[mode] = {"cloth_steps_per_s": round(count * steps / t, 1),
+                     "vertices": int(sum(g.n for g in sims))}
+    return out
+
+
 def learned_comparison(sim, sc, window, restore, replay, a, pers):
     """Learned preconditioner vs plain block-Jacobi PCG on the same window.
 
@@ -251,6 +279,8 @@ def main(argv=None):
     ap.add_argument("--train-steps", type=int, default=8000)
     ap.add_argument("--no-corrected", action="store_true", help="skip the corrected-contact leg")
     ap.add_argument("--corrected-steps", type=int, default=20)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-workload leg")
+    ap.add_argument("--batched-steps", type=int, default=5)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU design is described "
@@ -372,6 +402,8 @@ def main(argv=None):
 
     frame = per_frame_leg(sim, sc, with_cpu=not a.no_cpu)
 
+    batched = None if a.no_batched else batched_leg(a.batched_steps)
+
     corrected = None
     if not a.no_corrected:
         corrected = corrected_contact_leg(a.corrected_steps)
@@ -437,6 +469,7 @@ def main(argv=None):
         "learned_precond": learned,
         "corrected_contact": corrected,
         "per_frame_consumers": frame,
+        "batched": batched,
         "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu_single,
     }
     print(json.dumps(out), flush=True)

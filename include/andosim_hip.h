@@ -27,6 +27,8 @@
  *   ando_contact_metrics   CollisionValidator::compute_metrics          src/core/collision_validator.cpp:133-157
  *   ando_cfl_inputs        AdaptiveTimestep::compute_max_velocity +
  *                          compute_min_edge_length                      src/core/adaptive_timestep.cpp:82-150
+ *   ando_step_batch        (new, additive) ando_step on many contexts at once; the reference
+ *                          steps independent cloths one after another
  *   ando_get_stats         (new, additive) solver statistics; the reference exposes none
  *                          (src/core/pcg_solver.h:26-27 returns bool only)
  *   ando_linearize         one Newton linearisation: Integrator::compute_gradient +
@@ -220,6 +222,16 @@ int ando_set_mode(ando_ctx* ctx, int32_t flags);
 int ando_apply_gravity(ando_ctx* ctx, const float* gravity3, float dt);
 int ando_step(ando_ctx* ctx, const ando_params* params, ando_rigid_body* bodies, int32_t num_bodies);
 int ando_get_stats(ando_ctx* ctx, ando_stats* out);
+
+/* Batched stepping (new; the reference has no batched API and steps
+ * independent cloths one after another): ando_step on `count` distinct
+ * contexts, each on its own stream, from up to `threads` host threads
+ * (0 = the host's hardware concurrency), so small cloths overlap on the GPU.
+ * params[k], bodies[k] / num_bodies[k] (both may be NULL: no bodies) belong to
+ * ctxs[k].  Each context's result equals a plain ando_step.  Returns the first
+ * failing entry's code, with its index in the message; the others still run. */
+int ando_step_batch(ando_ctx* const* ctxs, const ando_params* params, ando_rigid_body* const* bodies,
+                    const int32_t* num_bodies, int32_t count, int32_t threads);
 
 int ando_compute_contacts(ando_ctx* ctx, const ando_rigid_body* bodies, int32_t num_bodies, ando_contact* out,
                           int32_t capacity, int32_t* count);
