@@ -273,6 +273,7 @@ struct ando_ctx {
   LbvhTree tri_tree, edge_tree;
   DBuf<ando::SelfContact> sc;
   DBuf<int32_t> sc_key, sc_count, sc_ptr, sc_sums, sc_total_d;
+  DBuf<int32_t> sc_cache;  // per query, the first kQueryCache hits of the count pass
   DBuf<uint32_t> sc_err;
   int sc_pt = 0, sc_total = 0;
   DBuf<ando::SelfTerm> sterm;
@@ -635,19 +636,21 @@ void detect_self(ando_ctx* c) {
   const int q = std::max(n, E);
   c->sc_count.ensure((size_t)q + 1);
   c->sc_ptr.ensure((size_t)q + 2);
+  c->sc_cache.ensure((size_t)q * ando::lbvh::kQueryCache + 1);
   const LbvhTree& T = c->tri_tree;
   const LbvhTree& Q = c->edge_tree;
   auto pt = [&](int pass, ando::SelfContact* out, int32_t* key) {
     hipLaunchKernelGGL(k_query_pt, dim3(nblk(n)), dim3(kBlock), 0, s, pass, n, (const float*)c->x.p, c->pt_box,
                        c->keep_gap, (const int32_t*)c->d_tris.p, T.P, (const int32_t*)T.left.p,
                        (const int32_t*)T.right.p, (const float*)T.nodes.p, (const float*)T.table.p,
-                       (const int32_t*)T.vals.p, c->sc_count.p, (const int32_t*)c->sc_ptr.p, out, key, c->sc_err.p);
+                       (const int32_t*)T.vals.p, c->sc_count.p, (const int32_t*)c->sc_ptr.p, out, key, c->sc_err.p,
+                       c->sc_cache.p);
   };
   auto ee = [&](int pass, ando::SelfContact* out, int32_t* key) {
     hipLaunchKernelGGL(k_query_ee, dim3(nblk(E)), dim3(kBlock), 0, s, pass, E, (const float*)c->x.p, c->keep_gap,
                        (const int32_t*)c->d_edges.p, Q.P, (const int32_t*)Q.left.p, (const int32_t*)Q.right.p,
                        (const float*)Q.nodes.p, (const float*)Q.table.p, (const int32_t*)Q.vals.p, c->sc_count.p,
-                       (const int32_t*)c->sc_ptr.p, out, key, c->sc_err.p);
+                       (const int32_t*)c->sc_ptr.p, out, key, c->sc_err.p, c->sc_cache.p);
   };
   int npt = 0, nee = 0;
   if (m > 0) {

@@ -363,12 +363,16 @@ __device__ inline void sort_hits(SelfContact* out, int32_t* key, int base, int k
 
 // PT query, pass 0 counts hits per vertex, pass 1 writes them at ptr[v] and
 // sorts the vertex's hits by triangle index.
+// Hits a query keeps from its count pass for its fill pass.
+constexpr int kQueryCache = 16;
+
 __global__ void k_query_pt(int pass, int n, const float* __restrict__ x, float pt_box, float keep,
                            const int32_t* __restrict__ tris, int P, const int32_t* __restrict__ left,
                            const int32_t* __restrict__ right, const float* __restrict__ nodes,
                            const float* __restrict__ leaves, const int32_t* __restrict__ order,
                            int32_t* __restrict__ count, const int32_t* __restrict__ ptr,
-                           SelfContact* __restrict__ out, int32_t* __restrict__ key, uint32_t* __restrict__ err) {
+                           SelfContact* __restrict__ out, int32_t* __restrict__ key, uint32_t* __restrict__ err,
+                           int32_t* __restrict__ cache) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n || P == 0) {
     if (pass == 0 && v < n) count[v] = 0;
@@ -380,13 +384,13 @@ __global__ void k_query_pt(int pass, int n, const float* __restrict__ x, float p
   q.hi[0] = p.x + pt_box; q.hi[1] = p.y + pt_box; q.hi[2] = p.z + pt_box;
   int k = 0;
   const int base = pass ? ptr[v] : 0;
-  const bool ok = traverse(P, q, left, right, nodes, leaves, [&](int s) {
-    const int t = order[s];
+  auto visit = [&](int t) {
     const int a = tris[3 * t], b = tris[3 * t + 1], c = tris[3 * t + 2];
     if (a == v || b == v || c == v) return;
     const f3 xa = ld3(x, a), xb = ld3(x, b), xc = ld3(x, c);
     const PTResult r = point_triangle(p, xa, xb, xc);
     if (!(r.dist < keep)) return;
+    if (!pass && k < kQueryCache) cache[(size_t)v * kQueryCache + k] = t;
     if (pass) {
       SelfContact sc;
       sc.type = 0;
@@ -400,7 +404,16 @@ __global__ void k_query_pt(int pass, int n, const float* __restrict__ x, float p
       key[base + k] = t;
     }
     ++k;
-  });
+  };
+  // The fill pass of a query whose hits all fit the cache re-evaluates only
+  // those, in the order the count pass found them (the same records).
+  const int cached = pass ? count[v] : kQueryCache + 1;
+  bool ok = true;
+  if (cached <= kQueryCache) {
+    for (int h = 0; h < cached; ++h) visit(cache[(size_t)v * kQueryCache + h]);
+  } else {
+    ok = traverse(P, q, left, right, nodes, leaves, [&](int s) { visit(order[s]); });
+  }
   if (!ok) atomicOr(err, 1u);
   if (pass == 0) {
     count[v] = k;
@@ -415,7 +428,7 @@ __global__ void k_query_ee(int pass, int E, const float* __restrict__ x, float k
                            const float* __restrict__ nodes, const float* __restrict__ leaves,
                            const int32_t* __restrict__ order, int32_t* __restrict__ count,
                            const int32_t* __restrict__ ptr, SelfContact* __restrict__ out,
-                           int32_t* __restrict__ key, uint32_t* __restrict__ err) {
+                           int32_t* __restrict__ key, uint32_t* __restrict__ err, int32_t* __restrict__ cache) {
   const int e1 = blockIdx.x * blockDim.x + threadIdx.x;
   if (e1 >= E) return;
   const int a0 = edges[2 * e1], a1 = edges[2 * e1 + 1];
@@ -425,8 +438,7 @@ __global__ void k_query_ee(int pass, int E, const float* __restrict__ x, float k
   q.hi[0] = fmaxf(p0.x, p1.x); q.hi[1] = fmaxf(p0.y, p1.y); q.hi[2] = fmaxf(p0.z, p1.z);
   int k = 0;
   const int base = pass ? ptr[e1] : 0;
-  const bool ok = traverse(P, q, left, right, nodes, leaves, [&](int s) {
-    const int e2 = order[s];
+  auto visit = [&](int e2) {
     if (e2 <= e1) return;
     const int b0 = edges[2 * e2], b1 = edges[2 * e2 + 1];
     if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
@@ -434,6 +446,7 @@ __global__ void k_query_ee(int pass, int E, const float* __restrict__ x, float k
     f3 nrm_, wp, wq;
     edge_edge(p0, p1, ld3(x, b0), ld3(x, b1), dist, nrm_, wp, wq);
     if (!(dist < keep)) return;
+    if (!pass && k < kQueryCache) cache[(size_t)e1 * kQueryCache + k] = e2;
     if (pass) {
       SelfContact sc;
       sc.type = 1;
@@ -446,7 +459,14 @@ __global__ void k_query_ee(int pass, int E, const float* __restrict__ x, float k
       key[base + k] = e2;
     }
     ++k;
-  });
+  };
+  const int cached = pass ? count[e1] : kQueryCache + 1;
+  bool ok = true;
+  if (cached <= kQueryCache) {
+    for (int h = 0; h < cached; ++h) visit(cache[(size_t)e1 * kQueryCache + h]);
+  } else {
+    ok = traverse(P, q, left, right, nodes, leaves, [&](int s) { visit(order[s]); });
+  }
   if (!ok) atomicOr(err, 1u);
   if (pass == 0) {
     count[e1] = k;
