@@ -613,13 +613,37 @@ __global__ void __launch_bounds__(256) k_self_spmv(int n, const int32_t* __restr
     beg = 3 * vptr[i];
     end = 3 * vptr[i + 1];
   }
-  for (int b = beg + lane; b < end; b += 8) {
-    const int j = ccol[b];
-    const float* B = cval + 9 * (size_t)b;
-    const float p0 = p[3 * j], p1 = p[3 * j + 1], p2 = p[3 * j + 2];
-    s0 = s0 + (B[0] * p0 + (B[1] * p1 + B[2] * p2));
-    s1 = s1 + (B[3] * p0 + (B[4] * p1 + B[5] * p2));
-    s2 = s2 + (B[6] * p0 + (B[7] * p1 + B[8] * p2));
+  // Lane l takes blocks l, l + 8, l + 16, ... of the row; four of them per
+  // pass are requested before any is used (the sums keep that block order).
+  constexpr int kU = 4;
+  for (int b0 = beg + lane; b0 < end; b0 += 8 * kU) {
+    int j[kU];
+    float B[kU][9], pv[kU][3];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int b = b0 + 8 * u;
+      if (b < end) {
+        j[u] = ccol[b];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) B[u][e] = cval[9 * (size_t)b + e];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (b0 + 8 * u < end) {
+        pv[u][0] = p[3 * j[u]];
+        pv[u][1] = p[3 * j[u] + 1];
+        pv[u][2] = p[3 * j[u] + 2];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (b0 + 8 * u < end) {
+        s0 = s0 + (B[u][0] * pv[u][0] + (B[u][1] * pv[u][1] + B[u][2] * pv[u][2]));
+        s1 = s1 + (B[u][3] * pv[u][0] + (B[u][4] * pv[u][1] + B[u][5] * pv[u][2]));
+        s2 = s2 + (B[u][6] * pv[u][0] + (B[u][7] * pv[u][1] + B[u][8] * pv[u][2]));
+      }
+    }
   }
   for (int off = 4; off > 0; off >>= 1) {
     s0 += __shfl_xor(s0, off, 8);
