@@ -595,7 +595,7 @@ void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims)
   T.bpart.ensure(6 * (size_t)G);
   T.bounds.ensure(6);
   hipLaunchKernelGGL(k_centroid_bounds, dim3(G), dim3(256), 0, s, P, (const float*)T.boxes.p, T.bpart.p);
-  hipLaunchKernelGGL(k_centroid_bounds_final, dim3(1), dim3(64), 0, s, G, (const float*)T.bpart.p, T.bounds.p);
+  hipLaunchKernelGGL(k_centroid_bounds_final, dim3(1), dim3(256), 0, s, G, (const float*)T.bpart.p, T.bounds.p);
   T.keys.ensure(P); T.keys2.ensure(P); T.vals.ensure(P); T.vals2.ensure(P);
   hipLaunchKernelGGL(k_morton, dim3(G), dim3(256), 0, s, P, (const float*)T.boxes.p, (const float*)T.bounds.p,
                      T.keys.p, T.vals.p);

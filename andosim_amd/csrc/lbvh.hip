@@ -106,12 +106,31 @@ __global__ void k_centroid_bounds(int P, const float* __restrict__ boxes, float*
   }
   if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
 }
-__global__ void k_centroid_bounds_final(int G, const float* __restrict__ part, float* __restrict__ out) {
-  if (threadIdx.x >= 6) return;
-  const int k = threadIdx.x;
-  float v = k < 3 ? INFINITY : -INFINITY;
-  for (int g = 0; g < G; ++g) v = k < 3 ? fminf(v, part[6 * g + k]) : fmaxf(v, part[6 * g + k]);
-  out[k] = v;
+// One 256-thread workgroup: each thread folds a strided share of the partials,
+// then a tree in LDS (minima and maxima are exact in any order).
+__global__ void __launch_bounds__(256) k_centroid_bounds_final(int G, const float* __restrict__ part,
+                                                              float* __restrict__ out) {
+  __shared__ float s[6][256];
+  float c[3] = {INFINITY, INFINITY, INFINITY}, d[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int g = threadIdx.x; g < G; g += blockDim.x)
+    for (int k = 0; k < 3; ++k) {
+      c[k] = fminf(c[k], part[6 * g + k]);
+      d[k] = fmaxf(d[k], part[6 * g + 3 + k]);
+    }
+  for (int k = 0; k < 3; ++k) {
+    s[k][threadIdx.x] = c[k];
+    s[3 + k][threadIdx.x] = d[k];
+  }
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int k = 0; k < 3; ++k) {
+        s[k][threadIdx.x] = fminf(s[k][threadIdx.x], s[k][threadIdx.x + st]);
+        s[3 + k][threadIdx.x] = fmaxf(s[3 + k][threadIdx.x], s[3 + k][threadIdx.x + st]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
 __device__ inline uint32_t spread10(uint32_t v) {
