@@ -261,6 +261,7 @@ struct ando_ctx {
   int spmv_rowsets = 3;    // block rows per lane group of the row SpMV (1..4)
   int spmv_blocks = 0;     // 1: block-parallel SpMV where the pattern allows it, 0: row SpMV
   int spmv_sell = 1;       // 1: sliced-ELL SpMV (values copied once per solve), 0: BSR kernels
+  int sell_lanes = 2;      // ANDO_SELL_LANES: lanes per row of the sliced-ELL SpMV (1, 2 or 4)
   int spmv_env = -1;       // ANDO_PCG_SPMV: -1 unset, 0 row, 1 blk, 2 sell (overrides the mode flag)
   SpmvMeta spmv_meta;      // for the mesh's block pattern
   // Corrected contact mode (self point-triangle and edge-edge through the LBVH).
@@ -542,9 +543,10 @@ void detect_rigid(ando_ctx* c, const ando_rigid_body* bodies, int nbod, float dt
   HIPCHK(hipGetLastError());
 }
 
-// Exclusive scan of n int32 (out has n + 1 entries); returns the total.
+// Exclusive scan of n int32 (out has n + 1 entries); returns the total, read
+// back with a stream synchronisation unless read_total is false (then -1).
 int32_t exclusive_scan(ando_ctx* c, int n, const int32_t* in, int32_t* out, DBuf<int32_t>& sums,
-                       DBuf<int32_t>& total) {
+                       DBuf<int32_t>& total, bool read_total = true) {
   hipStream_t s = c->stream;
   const int nbs = std::max(1, (n + 1023) / 1024);
   sums.ensure((size_t)nbs + 1);
@@ -554,6 +556,7 @@ int32_t exclusive_scan(ando_ctx* c, int n, const int32_t* in, int32_t* out, DBuf
   hipLaunchKernelGGL(k_scan_add, dim3(nblk(n + 1)), dim3(kBlock), 0, s, n, out, (const int32_t*)sums.p,
                      (const int32_t*)total.p);
   HIPCHK(hipGetLastError());
+  if (!read_total) return -1;
   int32_t t = 0;
   HIPCHK(hipMemcpyAsync(&t, total.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
   sync(c);
@@ -574,7 +577,7 @@ void radix_sort(ando_ctx* c, int P, DBuf<uint32_t>& keys, DBuf<int32_t>& vals, D
   offsets.ensure((size_t)nc + 1);
   for (int shift = 0; shift < 32; shift += 8) {
     hipLaunchKernelGGL(k_radix_count, dim3(G), dim3(256), 0, s, P, (const uint32_t*)keys.p, shift, counts.p);
-    exclusive_scan(c, nc, counts.p, offsets.p, sums, total);
+    exclusive_scan(c, nc, counts.p, offsets.p, sums, total, false);
     hipLaunchKernelGGL(k_radix_scatter, dim3(G), dim3(256), 0, s, P, (const uint32_t*)keys.p,
                        (const int32_t*)vals.p, shift, (const int32_t*)offsets.p, keys2.p, vals2.p);
     std::swap(keys, keys2);
@@ -705,7 +708,7 @@ void prepare_self(ando_ctx* c, const StepParams& P) {
   c->sv_keys.ensure(cnt);
   c->sv_vals.ensure(cnt);
   hipLaunchKernelGGL(k_self_slots, dim3(nblk(cnt)), dim3(kBlock), 0, s, nc, (const ando::SelfContact*)c->sc.p,
-                     c->sv_keys.p, c->sv_vals.p);
+                     (const ando::SelfTerm*)c->sterm.p, c->sv_keys.p, c->sv_vals.p);
   radix_sort(c, cnt, c->sv_keys, c->sv_vals, c->sv_keys2, c->sv_vals2, c->sv_counts, c->sv_offsets, c->sv_sums,
              c->sv_total);
   c->sv_hist.ensure((size_t)c->n + 1);
@@ -819,6 +822,7 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
   const bool blk = !sell && c->spmv_blocks && sm != nullptr && sm->ok && !couple;
   int nwg_v = nblk(n), nwg_r = sell ? nblk(sm->slices, kBlock / 64) : (blk ? sm->groups : spmv_groups(n, R));
   c->stats.pcg_kernel = sell ? 3 : (blk ? 2 : 0);
+  c->stats.spmv_lanes = sell ? c->sell_lanes : 0;
   c->rz_hist.ensure((size_t)maxit + 2);
   c->pcg_status.ensure(1);
   c->part_pAp.ensure((size_t)nwg_r + 1);
@@ -880,12 +884,17 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
                               (const float*)c->part_max.p, nwg_v, c->rz_hist.p, c->part_pAp.p, c->pcg_status.p, tol);
       };
       if (sell) {
-        hipExtLaunchKernelGGL(k_pcg_spmv_sell, dim3(nwg_r), dim3(kBlock), 0, s, e0, e1, 0, n, sm->slices, it,
-                              (const float*)sm->sell_vals.p, (const int32_t*)sm->sell_col.p,
-                              (const int32_t*)sm->slice_off.p, (const int32_t*)sm->slice_w.p, sm->uniform_w,
-                              (const float*)c->z.p,
-                              (const float*)pold, pcur, c->Ap.p, (const double*)c->part_rz.p,
-                              (const float*)c->part_max.p, nwg_v, c->rz_hist.p, c->part_pAp.p, c->pcg_status.p, tol);
+        auto sell_launch = [&](auto kernel, int threads) {
+          hipExtLaunchKernelGGL(kernel, dim3(nwg_r), dim3(threads), 0, s, e0, e1, 0, n, sm->slices, it,
+                                (const float*)sm->sell_vals.p, (const int32_t*)sm->sell_col.p,
+                                (const int32_t*)sm->slice_off.p, (const int32_t*)sm->slice_w.p, sm->uniform_w,
+                                (const float*)c->z.p, (const float*)pold, pcur, c->Ap.p,
+                                (const double*)c->part_rz.p, (const float*)c->part_max.p, nwg_v, c->rz_hist.p,
+                                c->part_pAp.p, c->pcg_status.p, tol);
+        };
+        if (c->sell_lanes == 2) sell_launch(k_pcg_spmv_sell_split<2>, 2 * kBlock);
+        else if (c->sell_lanes == 4) sell_launch(k_pcg_spmv_sell_split<4>, 4 * kBlock);
+        else sell_launch(k_pcg_spmv_sell, kBlock);
       } else if (blk) {
         hipExtLaunchKernelGGL(k_pcg_spmv_blk, dim3(nwg_r), dim3(kBlock), 0, s, e0, e1, 0, sm->nb, it, A,
                               (const uint32_t*)sm->meta.p, (const int32_t*)sm->run_row0.p, (const float*)c->z.p,
@@ -1240,6 +1249,10 @@ int ando_create(int32_t device, ando_ctx** out) {
     if (const char* w = std::getenv("ANDO_PCG_CHUNK")) c->pcg_chunk = std::min(256, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_RIGID_GROUP")) c->rigid_group = std::min(256, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_BARRIER")) c->pcg_barrier = std::atoi(w) == 1 ? 1 : 0;
+    if (const char* w = std::getenv("ANDO_SELL_LANES")) {
+      const int l = std::atoi(w);
+      c->sell_lanes = (l == 1 || l == 4) ? l : 2;
+    }
     if (const char* w = std::getenv("ANDO_PCG_ROWSETS")) c->spmv_rowsets = std::min(4, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_SPMV")) {
       c->spmv_env = std::strcmp(w, "blk") == 0 ? 1 : (std::strcmp(w, "sell") == 0 ? 2 : 0);

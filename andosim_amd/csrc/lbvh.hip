@@ -569,15 +569,19 @@ __global__ void k_self_terms(int nc, int n, StepParams P, const SelfContact* __r
 }
 
 // Vertex view: key = vertex of slot (c, s) or 0xffffffff when the slot carries
-// no vertex (idx < 0 or weight 0); value = c * 4 + s.  A stable sort by key
+// no vertex (idx < 0 or weight 0) or the contact adds nothing (term inactive,
+// or V' and V'' both zero: the gradient, the diagonal blocks and the coupling
+// blocks all skip such a contact); value = c * 4 + s.  A stable sort by key
 // then groups the slots by vertex in contact order.
-__global__ void k_self_slots(int nc, const SelfContact* __restrict__ sc, uint32_t* __restrict__ keys,
-                             int32_t* __restrict__ vals) {
+__global__ void k_self_slots(int nc, const SelfContact* __restrict__ sc, const SelfTerm* __restrict__ st,
+                             uint32_t* __restrict__ keys, int32_t* __restrict__ vals) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 4 * nc) return;
   const int c = t >> 2, s = t & 3;
   const int idx = sc[c].idx[s];
-  keys[t] = (idx >= 0 && sc[c].w[s] != 0.0f) ? (uint32_t)idx : 0xffffffffu;
+  const SelfTerm& T = st[c];
+  const bool adds = T.active && (T.dV != 0.0f || T.d2 != 0.0f);
+  keys[t] = (adds && idx >= 0 && sc[c].w[s] != 0.0f) ? (uint32_t)idx : 0xffffffffu;
   vals[t] = t;
 }
 __global__ void k_count_keys(int count, int n, const uint32_t* __restrict__ keys, int32_t* __restrict__ hist) {

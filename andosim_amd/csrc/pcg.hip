@@ -83,46 +83,58 @@ __device__ inline float thread_max_partials(const float* p, int count) {
 // kUnroll * blockDim partials are loaded into registers by prefetch_partials
 // (issued before a kernel's other loads, since they head the longest chain),
 // and summed in the same order by finish_*.
+// (tid, nt): this thread's index among the nt threads that share the
+// partials; the plain forms use the whole workgroup.
 template <class T>
-__device__ inline void prefetch_partials(const T* p, int count, T (&v)[kUnroll]) {
+__device__ inline void prefetch_partials(const T* p, int count, T (&v)[kUnroll], int tid, int nt) {
 #pragma unroll
   for (int u = 0; u < kUnroll; ++u) {
-    const int k = u * (int)blockDim.x + (int)threadIdx.x;
+    const int k = u * nt + tid;
     v[u] = p[k < count ? k : count - 1];
   }
 }
-__device__ inline double finish_sum_partials(const double* p, int count, const double (&v)[kUnroll]) {
+template <class T>
+__device__ inline void prefetch_partials(const T* p, int count, T (&v)[kUnroll]) {
+  prefetch_partials(p, count, v, (int)threadIdx.x, (int)blockDim.x);
+}
+__device__ inline double finish_sum_partials(const double* p, int count, const double (&v)[kUnroll], int tid,
+                                             int nt) {
   double s = 0.0;
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u) s += (u * (int)blockDim.x + (int)threadIdx.x) < count ? v[u] : 0.0;
-  for (int base = kUnroll * (int)blockDim.x; base < count; base += kUnroll * (int)blockDim.x) {
+  for (int u = 0; u < kUnroll; ++u) s += (u * nt + tid) < count ? v[u] : 0.0;
+  for (int base = kUnroll * nt; base < count; base += kUnroll * nt) {
     double w[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      const int k = base + u * (int)blockDim.x + (int)threadIdx.x;
+      const int k = base + u * nt + tid;
       w[u] = p[k < count ? k : count - 1];
     }
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) s += (base + u * (int)blockDim.x + (int)threadIdx.x) < count ? w[u] : 0.0;
+    for (int u = 0; u < kUnroll; ++u) s += (base + u * nt + tid) < count ? w[u] : 0.0;
+  }
+  return s;
+}
+__device__ inline double finish_sum_partials(const double* p, int count, const double (&v)[kUnroll]) {
+  return finish_sum_partials(p, count, v, (int)threadIdx.x, (int)blockDim.x);
+}
+__device__ inline float finish_max_partials(const float* p, int count, const float (&v)[kUnroll], int tid, int nt) {
+  float s = 0.0f;
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) s = fmaxf(s, (u * nt + tid) < count ? v[u] : 0.0f);
+  for (int base = kUnroll * nt; base < count; base += kUnroll * nt) {
+    float w[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int k = base + u * nt + tid;
+      w[u] = p[k < count ? k : count - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) s = fmaxf(s, (base + u * nt + tid) < count ? w[u] : 0.0f);
   }
   return s;
 }
 __device__ inline float finish_max_partials(const float* p, int count, const float (&v)[kUnroll]) {
-  float s = 0.0f;
-#pragma unroll
-  for (int u = 0; u < kUnroll; ++u) s = fmaxf(s, (u * (int)blockDim.x + (int)threadIdx.x) < count ? v[u] : 0.0f);
-  for (int base = kUnroll * (int)blockDim.x; base < count; base += kUnroll * (int)blockDim.x) {
-    float w[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const int k = base + u * (int)blockDim.x + (int)threadIdx.x;
-      w[u] = p[k < count ? k : count - 1];
-    }
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u)
-      s = fmaxf(s, (base + u * (int)blockDim.x + (int)threadIdx.x) < count ? w[u] : 0.0f);
-  }
-  return s;
+  return finish_max_partials(p, count, v, (int)threadIdx.x, (int)blockDim.x);
 }
 
 __device__ inline double reduce_sum_partials(const double* p, int count, double* lds) {
@@ -584,6 +596,182 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_sell(int n, int slices, int
   }
   double sp = block_reduce(pap, ldsd, [](double a, double b) { return a + b; });
   if (threadIdx.x == 0) part_pAp[g] = sp;
+}
+
+// k_pcg_spmv_sell with L lanes per row (L = 2 or 4), for more waves in flight
+// per slot; same contract and the same results bit for bit.  A workgroup of
+// 256 L threads still covers four slices (L waves per slice, 64 / L rows per
+// wave), so the partial counts are unchanged.  Lane group q of a row takes the
+// slots k with k % L == q; their products go to the partials t[k % 8] as in
+// the one-lane kernel, and the groups' sums meet in that kernel's tree order
+// ((t0 + t4) + (t2 + t6)) + ((t1 + t5) + (t3 + t7)) through shuffles.  The
+// r.z / max|r| reduction runs on the first 256 threads (the rest add exact
+// zeros) and the p.Ap partial is reduced from the one-lane layout restored in
+// LDS, so both associate as in the one-lane kernel.
+template <int L>
+__global__ void __launch_bounds__(256 * L) k_pcg_spmv_sell_split(int n, int slices, int it,
+                           const float* __restrict__ vals, const int32_t* __restrict__ scol,
+                           const int32_t* __restrict__ slice_off, const int32_t* __restrict__ slice_w, int uniform_w,
+                           const float* __restrict__ z, const float* __restrict__ p_old, float* __restrict__ p_new,
+                           float* __restrict__ Ap, const double* __restrict__ part_rz,
+                           const float* __restrict__ part_max, int nwg_v, double* __restrict__ rz_hist,
+                           double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol) {
+  static_assert(L == 2 || L == 4, "two or four lanes per row");
+  constexpr int R = 64 / L;             // rows per wave
+  constexpr int KC = kSellChunk / L;    // slots per lane per chunk
+  __shared__ double ldsd[4 * L];
+  __shared__ float ldsf[4 * L];
+  __shared__ double ldsp[4][64];
+  const int g = xcd_logical_group(blockIdx.x, gridDim.x);
+  const int tid = threadIdx.x;
+  const int wv = tid >> 6, lane = tid & 63;
+  const int q = lane / R, rl = lane % R;
+  const int wslice = wv / L, sub = wv % L;
+  const int sl0 = __builtin_amdgcn_readfirstlane(g * 4 + wslice);
+  const bool live = sl0 < slices;
+  const int sl = live ? sl0 : slices - 1;
+  const int rs = sub * R + rl;  // row within the slice
+  const int row = sl0 * 64 + rs;
+  const int ri = row < n ? row : n - 1;
+  const int off = uniform_w > 0 ? sl * 64 * uniform_w : slice_off[sl];
+  const int W = !live ? 0 : (uniform_w > 0 ? uniform_w : slice_w[sl]);
+  const bool red = tid < 256;
+  double pre_rz[kUnroll];
+  float pre_max[kUnroll];
+  if (it > 0 && red) {
+    prefetch_partials(part_rz, nwg_v, pre_rz, tid, 256);
+    prefetch_partials(part_max, nwg_v, pre_max, tid, 256);
+  }
+  int jc[KC];
+  float Ab[KC][9];
+  const float* vbase = vals + 9 * (size_t)off + rs;
+  const int32_t* cbase = scol + off + rs;
+#pragma unroll
+  for (int m = 0; m < KC; ++m) {
+    const int k = L * m + q;
+    if (k < W) {
+      jc[m] = cbase[64 * k];
+#pragma unroll
+      for (int e = 0; e < 9; ++e) Ab[m][e] = vbase[(size_t)(9 * k + e) * 64];
+    }
+  }
+  float zi[3], pi[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    zi[c] = z[3 * ri + c];
+    pi[c] = p_old[3 * ri + c];
+  }
+  float zo[KC][3], po[KC][3];
+#pragma unroll
+  for (int m = 0; m < KC; ++m) {
+    if (L * m + q < W) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        zo[m][c] = z[3 * jc[m] + c];
+        po[m][c] = p_old[3 * jc[m] + c];
+      }
+    }
+  }
+  if (st->done) return;
+  float beta = 0.0f;
+  if (it > 0) {
+    const float bmax = st->bmax;
+    const double rz_prev = rz_hist[it - 1];
+    double rz = red ? finish_sum_partials(part_rz, nwg_v, pre_rz, tid, 256) : 0.0;
+    float rmax = red ? finish_max_partials(part_max, nwg_v, pre_max, tid, 256) : 0.0f;
+    block_reduce_sum_max(rz, rmax, ldsd, ldsf);
+    float rel = ((double)bmax < 1e-16) ? rmax : rmax / bmax;
+    if (rel < tol) {
+      if (tid == 0) {
+        st->rel = rel;
+        st->iters = it;
+        st->done = 1;
+      }
+      return;
+    }
+    beta = (float)(rz / rz_prev);
+    if (g == 0 && tid == 0) rz_hist[it] = rz;
+  }
+  // T[m] is partial t[L m + q] of the one-lane kernel.
+  float T[KC][3];
+#pragma unroll
+  for (int m = 0; m < KC; ++m) {
+    T[m][0] = T[m][1] = T[m][2] = 0.0f;
+    if (L * m + q < W) {
+      float pv0, pv1, pv2;
+      if (it > 0) {
+        pv0 = zo[m][0] + beta * po[m][0];
+        pv1 = zo[m][1] + beta * po[m][1];
+        pv2 = zo[m][2] + beta * po[m][2];
+      } else {
+        pv0 = po[m][0]; pv1 = po[m][1]; pv2 = po[m][2];
+      }
+      T[m][0] = Ab[m][0] * pv0 + (Ab[m][1] * pv1 + Ab[m][2] * pv2);
+      T[m][1] = Ab[m][3] * pv0 + (Ab[m][4] * pv1 + Ab[m][5] * pv2);
+      T[m][2] = Ab[m][6] * pv0 + (Ab[m][7] * pv1 + Ab[m][8] * pv2);
+    }
+  }
+  for (int k0 = kSellChunk; k0 < W; k0 += kSellChunk) {
+#pragma unroll
+    for (int m = 0; m < KC; ++m) {
+      const int k = k0 + L * m + q;
+      if (k < W) {
+        const int j = cbase[64 * k];
+        float B[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) B[e] = vbase[(size_t)(9 * k + e) * 64];
+        float pv0, pv1, pv2;
+        if (it > 0) {
+          pv0 = z[3 * j] + beta * p_old[3 * j];
+          pv1 = z[3 * j + 1] + beta * p_old[3 * j + 1];
+          pv2 = z[3 * j + 2] + beta * p_old[3 * j + 2];
+        } else {
+          pv0 = p_old[3 * j]; pv1 = p_old[3 * j + 1]; pv2 = p_old[3 * j + 2];
+        }
+        T[m][0] = T[m][0] + (B[0] * pv0 + (B[1] * pv1 + B[2] * pv2));
+        T[m][1] = T[m][1] + (B[3] * pv0 + (B[4] * pv1 + B[5] * pv2));
+        T[m][2] = T[m][2] + (B[6] * pv0 + (B[7] * pv1 + B[8] * pv2));
+      }
+    }
+  }
+  float sr[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    float h;
+    if constexpr (L == 2) {
+      // q = 0 holds t0, t2, t4, t6 and q = 1 holds t1, t3, t5, t7.
+      h = (T[0][c] + T[2][c]) + (T[1][c] + T[3][c]);
+      sr[c] = h + __shfl_down(h, 32, 64);
+    } else {
+      // q holds t[q] and t[q + 4]; pairs (q0 + q2) and (q1 + q3), then their sum.
+      h = T[0][c] + T[1][c];
+      h = h + __shfl_down(h, 32, 64);
+      sr[c] = h + __shfl_down(h, 16, 64);
+    }
+  }
+  double pap = 0.0;
+  if (q == 0 && live && row < n) {
+    float q0, q1, q2;
+    if (it > 0) {
+      q0 = zi[0] + beta * pi[0];
+      q1 = zi[1] + beta * pi[1];
+      q2 = zi[2] + beta * pi[2];
+      p_new[3 * row] = q0;
+      p_new[3 * row + 1] = q1;
+      p_new[3 * row + 2] = q2;
+    } else {
+      q0 = pi[0]; q1 = pi[1]; q2 = pi[2];
+    }
+    Ap[3 * row] = sr[0];
+    Ap[3 * row + 1] = sr[1];
+    Ap[3 * row + 2] = sr[2];
+    pap = (double)q0 * sr[0] + (double)q1 * sr[1] + (double)q2 * sr[2];
+  }
+  if (q == 0) ldsp[wslice][rs] = pap;
+  __syncthreads();
+  const double pv = red ? ldsp[wv][lane] : 0.0;
+  double sp = block_reduce(pv, ldsd, [](double a, double b) { return a + b; });
+  if (tid == 0) part_pAp[g] = sp;
 }
 
 // Block-parallel SpMV: one 3x3 block per lane, workgroups over contiguous

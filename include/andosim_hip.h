@@ -146,6 +146,7 @@ typedef struct {
   int32_t learned_solves;           /* solves that used the learned preconditioner */
   int32_t learned_fallbacks;        /* of those, solves that fell back to plain PCG */
   int32_t fallback_iterations;      /* plain PCG iterations spent in fallbacks (included in pcg_iterations) */
+  int32_t spmv_lanes;               /* pcg_kernel 3: lanes per row of the sliced-ELL SpMV (1, 2 or 4) */
 } ando_stats;
 
 int32_t ando_abi_version(void);

@@ -96,3 +96,31 @@ def test_sell_steps_match_row(name, make, steps, corrected):
         assert abs(sa["pcg_iterations"] - sb["pcg_iterations"]) <= max(2, sb["pcg_iterations"] // 100)
     xa, xb = a.get("x"), b.get("x")
     assert np.max(np.abs(xa - xb)) <= 1e-6 * max(1.0, float(np.max(np.abs(xb))))
+
+
+@pytest.mark.parametrize("lanes", [2, 4])
+def test_sell_lane_split_bit_identical(lanes, monkeypatch):
+    """Two (the default) or four lanes per row (ANDO_SELL_LANES) sum in the
+    one-lane kernel's order and reduce its partials the same way: solves and
+    steps agree bit for bit."""
+    print(f"sell lanes={lanes}", flush=True)
+    rp, ci, vals, b = random_spd_bsr(4097, 3, extra_rows=(0, 2048, 4096))
+    monkeypatch.setenv("ANDO_SELL_LANES", "1")
+    one = native.GpuSim()
+    monkeypatch.setenv("ANDO_SELL_LANES", str(lanes))
+    split = native.GpuSim()
+    x1, i1, c1 = one.pcg(rp, ci, vals, b, tol=1e-6, maxit=2000)
+    x2, i2, c2 = split.pcg(rp, ci, vals, b, tol=1e-6, maxit=2000)
+    assert c1 and c2 and i1 == i2
+    assert np.array_equal(x1, x2)
+    sc = scenes.sphere129()
+    monkeypatch.setenv("ANDO_SELL_LANES", "1")
+    a = native.GpuSim(sc)
+    monkeypatch.setenv("ANDO_SELL_LANES", str(lanes))
+    s = native.GpuSim(sc)
+    monkeypatch.delenv("ANDO_SELL_LANES")
+    for _ in range(4):
+        a.substep()
+        s.substep()
+        assert a.stats()["pcg_iterations"] == s.stats()["pcg_iterations"]
+    assert np.array_equal(a.get("x"), s.get("x"))
