@@ -283,6 +283,7 @@ struct ando_ctx {
   DBuf<double> part_pAp2;
   DBuf<int32_t> ccol;
   DBuf<float> cval;
+  DBuf<int32_t> pt_of;     // faithful one-triangle mesh: PT contact of each point vertex, or -1
   bool sv_ready = false;
   float barrier_tol = 1e-12f;
   int learned = 0;         // learned preconditioner + warm start in the step's PCG
@@ -622,15 +623,23 @@ void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims)
   HIPCHK(hipGetLastError());
 }
 
-// Corrected self detection at the current positions: c->sc holds the PT
-// contacts (vertex, triangle order) followed by the EE contacts (e1, e2).
+// Self contacts exist in corrected mode, and in faithful mode on a mesh of
+// exactly one triangle: the reference's BVH then has a single primitive-bearing
+// leaf, and its broad and narrow phases find the PT contacts of the vertices
+// not on the triangle (collision.cpp:78-86,198-244,279-387) -- the corrected
+// predicate restricted to one triangle; its three edges share vertices, so
+// there are no EE contacts.
+bool self_active(const ando_ctx* c) { return c->corrected || c->m == 1; }
+
+// Self detection at the current positions: c->sc holds the PT contacts
+// (vertex, triangle order) followed by the EE contacts (e1, e2).
 void detect_self(ando_ctx* c) {
   using namespace ando::lbvh;
   hipStream_t s = c->stream;
   ++c->det_gen;
   c->sc_pt = c->sc_total = 0;
   c->sv_ready = false;
-  if (!c->corrected) return;
+  if (!self_active(c)) return;
   const int n = c->n, m = c->m, E = (int)c->edges_h.size() / 2;
   lbvh_build(c, c->tri_tree, m, 3, c->d_tris.p);
   lbvh_build(c, c->edge_tree, E, 2, c->d_edges.p);
@@ -688,7 +697,8 @@ void detect_self(ando_ctx* c) {
 
 ando::SelfView self_view(ando_ctx* c) {
   ando::SelfView v{nullptr, nullptr, nullptr, nullptr};
-  if (c->sv_ready) v = ando::SelfView{c->sv_ptr.p, c->sv_vals.p, c->sc.p, c->sterm.p};
+  if (c->sv_ready)
+    v = ando::SelfView{c->sv_ptr.p, c->sv_vals.p, c->sc.p, c->sterm.p, c->corrected ? nullptr : c->pt_of.p};
   return v;
 }
 
@@ -718,6 +728,13 @@ void prepare_self(ando_ctx* c, const StepParams& P) {
                      c->sv_hist.p);
   const int entries = exclusive_scan(c, c->n, c->sv_hist.p, c->sv_ptr.p, c->sv_sums, c->sv_total);
   c->part_pAp2.ensure((size_t)nblk(c->n, 32) + 1);
+  if (!c->corrected) {
+    c->pt_of.ensure((size_t)c->n);
+    HIPCHK(hipMemsetAsync(c->pt_of.p, 0xff, sizeof(int32_t) * c->n, s));
+    if (c->sc_pt > 0)
+      hipLaunchKernelGGL(k_pt_of, dim3(nblk(c->sc_pt)), dim3(kBlock), 0, s, c->sc_pt, c->n,
+                         (const ando::SelfContact*)c->sc.p, c->pt_of.p);
+  }
   c->sv_ready = true;
   c->ccol.ensure(3 * (size_t)entries + 1);
   c->cval.ensure(27 * (size_t)entries + 1);
@@ -1178,6 +1195,30 @@ void restitution(ando_ctx* c, const ando_params& p, ando_rigid_body* bodies, int
   float e = std::min(std::max(p.contact_restitution, 0.0f), 1.0f);
   if (e <= 0.0f) return;
   hipStream_t s = c->stream;
+  // Self contacts first (they head the contact list), detected again at the
+  // end-of-step positions.
+  if (self_active(c)) {
+    detect_self(c);
+    const int nc = c->sc_total, cnt = 4 * nc;
+    if (nc > 0) {
+      using namespace ando::lbvh;
+      c->sv_keys.ensure(cnt);
+      c->sv_vals.ensure(cnt);
+      hipLaunchKernelGGL(k_rest_slots, dim3(nblk(cnt)), dim3(kBlock), 0, s, nc, c->n, (const ando::SelfContact*)c->sc.p,
+                         std::max(p.contact_gap_max, 1e-5f), c->sv_keys.p, c->sv_vals.p);
+      radix_sort(c, cnt, c->sv_keys, c->sv_vals, c->sv_keys2, c->sv_vals2, c->sv_counts, c->sv_offsets, c->sv_sums,
+                 c->sv_total);
+      c->sv_hist.ensure((size_t)c->n + 1);
+      c->sv_ptr.ensure((size_t)c->n + 1);
+      HIPCHK(hipMemsetAsync(c->sv_hist.p, 0, sizeof(int32_t) * c->n, s));
+      hipLaunchKernelGGL(k_count_keys, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, c->n, (const uint32_t*)c->sv_keys.p,
+                         c->sv_hist.p);
+      exclusive_scan(c, c->n, c->sv_hist.p, c->sv_ptr.p, c->sv_sums, c->sv_total, false);
+      hipLaunchKernelGGL(k_rest_self, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, (const int32_t*)c->sv_ptr.p,
+                         (const int32_t*)c->sv_vals.p, (const ando::SelfContact*)c->sc.p, e, c->v.p);
+      HIPCHK(hipGetLastError());
+    }
+  }
   if (nbod > 0) {
     upload_bodies(c, bodies, nbod);
     detect_rigid(c, bodies, nbod, p.dt);
@@ -1636,6 +1677,18 @@ int ando_linearize(ando_ctx* c, const ando_params* params, const float* x_target
       HIPCHK(hipMemcpyAsync(gradient, c->g.p, 3 * sizeof(float) * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
     if (values)
       HIPCHK(hipMemcpyAsync(values, c->H.p, 9 * sizeof(float) * (size_t)c->nb, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+  });
+}
+
+int ando_apply_restitution(ando_ctx* c, const ando_params* params, ando_rigid_body* bodies, int32_t num_bodies) {
+  return guarded([&] {
+    need(c && c->have_state, ANDO_ESTATE, "state has not been set");
+    check_params(params);
+    need(num_bodies >= 0 && (num_bodies == 0 || bodies != nullptr), ANDO_EINVAL, "bodies is null");
+    HIPCHK(hipSetDevice(c->device));
+    ensure_const(c, *params);
+    restitution(c, *params, bodies, num_bodies);
     sync(c);
   });
 }

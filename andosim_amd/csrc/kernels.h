@@ -41,15 +41,19 @@ struct SelfTerm {
   int active;
   float dV, d2;
   float nx, ny, nz;
+  float kb;  // contact stiffness (also the friction normal force scale)
 };
 
 // Self contacts seen from the vertices: for vertex i, entries
 // [ptr[i], ptr[i+1]) of code hold contact * 4 + slot in contact order.
+// pt_of (faithful mode on a one-triangle mesh, else null): for vertex i the
+// PT contact whose point it is, or -1; such contacts also carry friction.
 struct SelfView {
   const int32_t* ptr;
   const int32_t* code;
   const SelfContact* sc;
   const SelfTerm* st;
+  const int32_t* pt_of;
 };
 
 // Rigid body dynamic state on the device (rigid_body.h:72-85).
@@ -93,18 +97,20 @@ struct StepParams {
 // Friction of one contact on its first vertex (integrator.cpp:28-66,
 // friction.cpp:6-90): tangential slip since the state's initial positions,
 // stiffness mu |k gap| / max(eps, |slip|) capped at 1e8.
-__device__ inline bool friction_data(const RigidContact& c, f3 disp, const StepParams& P, float& kf, f3& tan) {
-  f3 n = mk(c.nx, c.ny, c.nz);
+__device__ inline bool friction_data(f3 n, float k, float gap, f3 disp, const StepParams& P, float& kf, f3& tan) {
   float nc = dot(disp, n);
   tan = disp - nc * n;
   float tn = nrm(tan);
   if (!(tn > P.friction_tangent)) return false;
-  float nf = c.k * fabsf(c.gap);
+  float nf = k * fabsf(gap);
   float den = fmaxf(P.friction_eps, fabsf(tn));
   if (den <= 0.0f) den = P.friction_eps;
   float s = P.mu * fabsf(nf) / den;
   kf = fminf(s, 1e8f);
   return true;
+}
+__device__ inline bool friction_data(const RigidContact& c, f3 disp, const StepParams& P, float& kf, f3& tan) {
+  return friction_data(mk(c.nx, c.ny, c.nz), c.k, c.gap, disp, P, kf, tan);
 }
 
 struct StrainParams {

@@ -706,14 +706,24 @@ __global__ void k_gradient(int n, StepParams P, const float* __restrict__ x, con
       float dV = barrier_grad(gap, P.gap_max, kb, P.fp64_near);
       gi = gi + dV * nn;
     }
-    if (P.friction && rc_ptr) {
+    // Friction in contact order: the self PT contact of a one-triangle mesh
+    // (faithful mode), then the rigid contacts.
+    if (P.friction && (rc_ptr || sv.pt_of)) {
       f3 disp = xi - ld3(xprev, i);
-      for (int e = rc_ptr[i]; e < rc_ptr[i + 1]; ++e) {
+      const int pc = sv.pt_of ? sv.pt_of[i] : -1;
+      if (pc >= 0) {
+        const SelfContact& C = sv.sc[pc];
         float kf;
         f3 tan;
-        if (!friction_data(rc[e], disp, P, kf, tan)) continue;
-        gi = gi + kf * tan;
+        if (friction_data(mk(C.nx, C.ny, C.nz), sv.st[pc].kb, C.gap, disp, P, kf, tan)) gi = gi + kf * tan;
       }
+      if (rc_ptr)
+        for (int e = rc_ptr[i]; e < rc_ptr[i + 1]; ++e) {
+          float kf;
+          f3 tan;
+          if (!friction_data(rc[e], disp, P, kf, tan)) continue;
+          gi = gi + kf * tan;
+        }
     }
     st3(g, i, gi);
     local = fmaxf(fabsf(gi.x), fmaxf(fabsf(gi.y), fabsf(gi.z)));
@@ -834,14 +844,11 @@ __global__ void k_assemble(int n, StepParams P, const int32_t* __restrict__ bloc
           }
       }
     }
-    if (P.friction && rc_ptr) {
-      // k_f (I - n n^T) + 1e-8 I, entries above 1e-12 (integrator.cpp:515-536, friction.cpp:54-72).
+    if (P.friction && (rc_ptr || sv.pt_of)) {
+      // k_f (I - n n^T) + 1e-8 I, entries above 1e-12 (integrator.cpp:515-536, friction.cpp:54-72),
+      // in contact order: the one-triangle self PT contact, then the rigid ones.
       f3 disp = ld3(x, i) - ld3(xprev, i);
-      for (int e = rc_ptr[i]; e < rc_ptr[i + 1]; ++e) {
-        float kf;
-        f3 tan;
-        if (!friction_data(rc[e], disp, P, kf, tan)) continue;
-        f3 nn = mk(rc[e].nx, rc[e].ny, rc[e].nz);
+      auto add_friction = [&](f3 nn, float kf) {
         for (int r = 0; r < 3; ++r)
           for (int c = 0; c < 3; ++c) {
             float val = r == c ? kf : 0.0f;
@@ -850,7 +857,22 @@ __global__ void k_assemble(int n, StepParams P, const int32_t* __restrict__ bloc
             if (!((double)fabsf(val) > 1e-12)) continue;
             h[3 * r + c] = h[3 * r + c] + val;
           }
+      };
+      const int pc = sv.pt_of ? sv.pt_of[i] : -1;
+      if (pc >= 0) {
+        const SelfContact& C = sv.sc[pc];
+        const f3 nn = mk(C.nx, C.ny, C.nz);
+        float kf;
+        f3 tan;
+        if (friction_data(nn, sv.st[pc].kb, C.gap, disp, P, kf, tan)) add_friction(nn, kf);
       }
+      if (rc_ptr)
+        for (int e = rc_ptr[i]; e < rc_ptr[i + 1]; ++e) {
+          float kf;
+          f3 tan;
+          if (!friction_data(rc[e], disp, P, kf, tan)) continue;
+          add_friction(mk(rc[e].nx, rc[e].ny, rc[e].nz), kf);
+        }
     }
   }
   float* o = H + 9 * (size_t)b;

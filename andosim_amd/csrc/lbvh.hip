@@ -560,6 +560,7 @@ __global__ void k_self_terms(int nc, int n, StepParams P, const SelfContact* __r
     }
   }
   SelfTerm t;
+  t.kb = kb;
   t.active = in_domain(C.gap, P.gap_max) ? 1 : 0;
   const f3 nn = normalize_or_default(n0, P.normal_eps);
   t.nx = nn.x; t.ny = nn.y; t.nz = nn.z;
@@ -589,6 +590,56 @@ __global__ void k_count_keys(int count, int n, const uint32_t* __restrict__ keys
   if (t >= count) return;
   const uint32_t k = keys[t];
   if (k < (uint32_t)n) atomicAdd(&hist[k], 1);
+}
+
+// Faithful mode on a one-triangle mesh: each vertex is the point of at most
+// one PT contact (there is one triangle); pt_of[v] = that contact, else -1
+// (cleared by the caller).
+__global__ void k_pt_of(int npt, int n, const SelfContact* __restrict__ sc, int32_t* __restrict__ pt_of) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= npt || sc[c].type != 0) return;
+  const int v = sc[c].idx[0];
+  if (v >= 0 && v < n) pt_of[v] = c;
+}
+
+// Restitution of the self contacts (Integrator::apply_contact_restitution,
+// integrator.cpp:565-674): every contact within gap_limit, in list order,
+// gives its impulse vertices -- a PT contact's point, an EE contact's four
+// vertices, the second edge's with the normal negated -- the velocity
+// v - (1 + e) (v.n) n when v.n < 0.  Impulses on different vertices are
+// independent and those on one vertex run in contact order, so slots are
+// grouped by vertex (key = vertex of an impulse slot, else 0xffffffff; a
+// stable sort keeps contact order) and vertices run in parallel.
+__global__ void k_rest_slots(int nc, int n, const SelfContact* __restrict__ sc, float gap_limit,
+                             uint32_t* __restrict__ keys, int32_t* __restrict__ vals) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * nc) return;
+  const int c = t >> 2, s = t & 3;
+  const SelfContact& C = sc[c];
+  const bool use = !(C.gap > gap_limit) && (C.type == 1 || s == 0);
+  const int idx = C.idx[s];
+  keys[t] = (use && idx >= 0 && idx < n) ? (uint32_t)idx : 0xffffffffu;
+  vals[t] = t;
+}
+__global__ void k_rest_self(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ code,
+                            const SelfContact* __restrict__ sc, float e, float* __restrict__ v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int beg = ptr[i], end = ptr[i + 1];
+  if (beg == end) return;
+  f3 vel = ld3(v, i);
+  for (int k = beg; k < end; ++k) {
+    const int c = code[k] >> 2, s = code[k] & 3;
+    const SelfContact& C = sc[c];
+    f3 nn = mk(C.nx, C.ny, C.nz);
+    if (C.type == 1 && s >= 2) nn = -nn;
+    const float nl = nrm(nn);
+    if ((double)nl < 1e-8) continue;
+    nn = nn / nl;
+    const float vn = dot(vel, nn);
+    if (vn < 0.0f) vel = vel - ((1.0f + e) * vn) * nn;
+  }
+  st3(v, i, vel);
 }
 
 // The blocks coupling two vertices, materialised once per step: for vertex

@@ -21,7 +21,7 @@ ABI_SYMBOLS = [
     "ando_set_state", "ando_get_state", "ando_set_pins", "ando_set_walls", "ando_set_mode", "ando_apply_gravity",
     "ando_step",
     "ando_get_stats", "ando_compute_contacts", "ando_count_intersections", "ando_get_pattern", "ando_linearize",
-    "ando_pcg",
+    "ando_pcg", "ando_apply_restitution",
     "ando_nn_num_weights", "ando_set_precond_weights", "ando_trainer_create", "ando_trainer_destroy",
     "ando_trainer_add_system", "ando_trainer_init_weights", "ando_trainer_set_weights", "ando_trainer_get_weights",
     "ando_trainer_forward", "ando_trainer_loss_grad", "ando_trainer_train", "ando_trainer_solve",
@@ -122,6 +122,7 @@ def load():
     L.ando_get_pattern.argtypes = [C.c_void_p, ip, ip, ip]
     L.ando_count_intersections.argtypes = [C.c_void_p, C.POINTER(RigidBody), C.c_int32, ip, ip]
     L.ando_linearize.argtypes = [C.c_void_p, C.POINTER(Params), fp, C.POINTER(RigidBody), C.c_int32, fp, fp]
+    L.ando_apply_restitution.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(RigidBody), C.c_int32]
     L.ando_pcg.argtypes = [C.c_void_p, C.c_int32, ip, ip, fp, fp, fp, C.c_float, C.c_int32, ip, ip]
     dp = C.POINTER(C.c_double)
     L.ando_nn_num_weights.restype = C.c_int32
@@ -367,6 +368,10 @@ class GpuSim:
         check(self.L.ando_linearize(self.h, C.byref(self.params), _fp(xh), self.bodies, self.nbodies, _fp(g),
                                     _fp(H)))
         return g, H.reshape(-1, 3, 3)
+
+    def apply_restitution(self):
+        """Integrator::apply_contact_restitution on the current state."""
+        check(self.L.ando_apply_restitution(self.h, C.byref(self.params), self.bodies, self.nbodies))
 
     def contacts(self, cap=1 << 16):
         buf = (Contact * cap)()

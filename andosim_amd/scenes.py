@@ -280,6 +280,19 @@ WORKLOADS = {"drop21": drop21, "sphere129": sphere129, "stack65x8": stack65x8, "
              "stack_resting": stack_resting}
 
 
+def one_triangle(**params) -> Scene:
+    """One triangle in the plane z = 0.3 plus isolated vertices: the mesh shape
+    for which the reference's self broad phase is not empty (a single
+    primitive-bearing leaf).  Vertices 3 and 4 lie over the interior within the
+    1e-4 point box (above and below), vertex 5 over the interior 2 mm away,
+    vertex 6 far off."""
+    V = np.array([[0, 0, 0.3], [0.1, 0, 0.3], [0, 0.1, 0.3],
+                  [0.02, 0.03, 0.3 + 5e-5], [0.03, 0.02, 0.3 - 4e-5], [0.025, 0.025, 0.302], [0.5, 0.5, 0.3]],
+                 np.float32)
+    T = np.array([[0, 1, 2]], np.int32)
+    return Scene("one_triangle", V, T, dict(BASE_MATERIAL), base_params(**params), [], [], [])
+
+
 def small_sheet(s: int = 9, seed: int = 7, pins: int = 2, wall: bool = True, rigid: bool = False,
                 strain: bool = True, z: float = 0.3) -> Scene:
     """Tiny scenes for fast parity tests (same material and parameters as drop21)."""

@@ -34,6 +34,7 @@
  *   ando_linearize         one Newton linearisation: Integrator::compute_gradient +
  *                          assemble_system_matrix                      src/core/integrator.cpp:246-544
  *   ando_pcg               PCGSolver::solve on a caller-supplied BSR   src/core/pcg_solver.cpp:8-146
+ *   ando_apply_restitution Integrator::apply_contact_restitution       src/core/integrator.cpp:565-674
  *   ando_get_pattern       block sparsity of the system matrix (the CSR pattern that
  *                          setFromTriplets builds at integrator.cpp:539)
  */
@@ -311,6 +312,11 @@ int ando_linearize(ando_ctx* ctx, const ando_params* params, const float* x_targ
                    int32_t num_bodies, float* gradient, float* values);
 int ando_pcg(ando_ctx* ctx, int32_t num_rows, const int32_t* row_ptr, const int32_t* cols, const float* values,
              const float* b, float* x, float tol, int32_t max_iters, int32_t* iterations, int32_t* converged);
+/* The restitution pass that ando_step runs after its velocity update when
+ * contact_restitution > 0, on the current state: contacts detected at the
+ * current positions (self contacts, then rigid), impulses in list order
+ * (self, then rigid, which also change the bodies' velocities), then walls. */
+int ando_apply_restitution(ando_ctx* ctx, const ando_params* params, ando_rigid_body* bodies, int32_t num_bodies);
 
 /* ---- Learned preconditioner (new; no counterpart in the reference, whose PCG
  * is block-Jacobi from x0 = 0: pcg_solver.cpp:8-75, integrator.cpp:187). ----

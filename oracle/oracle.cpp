@@ -2155,6 +2155,13 @@ int orc_linearize(void* h, const double* xhat, double* g_out, double* H_out) {
   });
 }
 
+// Integrator::apply_contact_restitution on the current state (the pass step
+// runs after its velocity update when contact_restitution > 0).
+int orc_restitution(void* h) {
+  OSim* s = (OSim*)h;
+  return guard([&] { DISPATCH(s, S.restitution()); });
+}
+
 // Block-Jacobi PCG on a BSR matrix (9 values per block).
 int orc_pcg(int prec, int nv, const int32_t* rp, const int32_t* ci, const double* vals, const double* b,
             double* x, double tol, int maxit, int* iters) {

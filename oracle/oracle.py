@@ -80,6 +80,7 @@ def lib():
         L.orc_get_threads.restype = C.c_int
         L.orc_energy.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_set_ccd_mode.argtypes = [C.c_void_p, C.c_int]
+        L.orc_restitution.argtypes = [C.c_void_p]
         L.orc_set_capture.argtypes = [C.c_void_p, C.c_int]
         L.orc_get_capture.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_additive_ccd.restype = C.c_double
@@ -274,6 +275,10 @@ class OracleSim:
         H = np.zeros(9 * nb)
         _chk(self.L.orc_linearize(self.h, _d(xhat).ravel(), g, H))
         return g, H.reshape(nb, 3, 3)
+
+    def restitution(self):
+        """Integrator::apply_contact_restitution on the current state."""
+        _chk(self.L.orc_restitution(self.h))
 
     def contacts(self) -> np.ndarray:
         cap = 1 << 16

@@ -173,3 +173,96 @@ def test_intersection_counts_match_brute_force(make):
         assert o.count_intersections()[0] > 0
     if make is sheet_through_sphere:
         assert o.count_intersections()[1] > 0
+
+
+def test_one_triangle_faithful_contacts_match_oracle():
+    """Faithful mode on a one-triangle mesh lists the reference's PT contacts
+    (its BVH has one primitive-bearing leaf): identical to the oracle's list."""
+    sc = scenes.one_triangle()
+    o = OracleSim(sc, 32)
+    co = o.contacts()
+    g = native.GpuSim(sc)
+    cg, cnt = g.contacts()
+    assert cnt == len(co) == 2
+    assert np.array_equal(contact_table_oracle(co), contact_table_gpu(cg))
+
+
+@pytest.mark.parametrize("friction", [False, True])
+def test_one_triangle_faithful_linearize_bit_exact(friction):
+    """Gradient and matrix with the one-triangle PT barrier and (faithful mode)
+    its friction on the point, the points having slid since the initial
+    positions: bit-identical to the FP32 oracle."""
+    sc = scenes.one_triangle(enable_friction=friction, friction_mu=0.5)
+    o = OracleSim(sc, 32)
+    x = o.get("x")
+    x[3, 0] += 2e-3
+    x[4, 1] -= 1e-3
+    o.set("x", x)
+    v = o.get("v")
+    v[3] = [0.01, 0.0, -0.002]
+    o.set("v", v)
+    g = native.GpuSim(sc, fp64_near=False)
+    g.set_mode(fp64_near=False)
+    sync_state(o, g)
+    xhat = (x + np.float32(sc.params["dt"]) * v).astype(np.float32)
+    go, Ho = o.linearize(xhat)
+    gg, Hg = g.linearize(xhat)
+    assert len(o.contacts()) == 2
+    assert np.array_equal(gg.reshape(-1), go.astype(np.float32).reshape(-1))
+    assert np.array_equal(Hg, Ho.astype(np.float32))
+    if friction:
+        o0 = OracleSim(scenes.one_triangle(), 32)
+        o0.set("x", x)
+        o0.set("v", v)
+        g0, _ = o0.linearize(xhat)
+        assert not np.array_equal(g0.reshape(-1, 3)[3], go.reshape(-1, 3)[3])   # friction acts on the point
+
+
+def test_one_triangle_faithful_steps_match_oracle():
+    """Substeps of the one-triangle mesh with friction and restitution from
+    identical states: positions within 1e-6 and velocities within 1e-5 of the
+    FP32 oracle, same contact counts."""
+    sc = scenes.one_triangle(enable_friction=True, friction_mu=0.5, contact_restitution=0.5)
+    o = OracleSim(sc, 32)
+    v = o.get("v")
+    v[3] = [0.01, 0.0, -0.002]
+    v[4] = [0.0, -0.01, 0.002]
+    o.set("v", v)
+    for k in range(4):
+        g = native.GpuSim(sc)
+        sync_state(o, g)
+        o.substep()
+        g.substep()
+        assert g.stats()["contacts"] == o.stats()["contacts"]
+        assert rel_err(g.get("x"), o.get("x")) < 1e-6, k
+        dv = np.max(np.abs(g.get("v").astype(np.float64) - o.get("v")))
+        assert dv <= 1e-5 * max(1.0, float(np.max(np.abs(o.get("v"))))), (k, dv)
+
+
+@pytest.mark.parametrize("name,make,corrected", [
+    ("stack", lambda: jittered_stack(), True),
+    ("crumpled", lambda: crumpled(), True),
+    ("crossing", lambda: crossing_sheets(), True),
+    ("one_triangle", lambda: scenes.one_triangle(), False),
+])
+def test_self_restitution_bit_exact(name, make, corrected):
+    """Integrator::apply_contact_restitution with self contacts (PT points, EE
+    ends with the second edge's normal negated; sequential per vertex in list
+    order): velocities bit-identical to the FP32 oracle."""
+    print(f"self restitution {name}", flush=True)
+    sc = make()
+    sc.params["contact_restitution"] = 0.5
+    o = OracleSim(sc, 32)
+    o.set_contact_mode(corrected)
+    rng = np.random.default_rng(11)
+    v0 = rng.normal(scale=0.1, size=(len(sc.verts), 3)).astype(np.float32)
+    o.set("v", v0)
+    g = native.GpuSim(sc)
+    g.set_mode(corrected=corrected)
+    sync_state(o, g)
+    assert len(o.contacts()) > 0
+    o.restitution()
+    g.apply_restitution()
+    vo = o.get("v").astype(np.float32)
+    assert not np.array_equal(vo, v0)
+    assert np.array_equal(g.get("v"), vo)

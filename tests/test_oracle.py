@@ -643,3 +643,70 @@ def test_capture_hook_returns_the_solved_system():
     assert H.shape == (len(ci), 3, 3) and b.shape == (3 * o.n,)
     _, iters, conv = orc.pcg(rp, ci, H, b, tol=sc.params["pcg_tol"], maxit=sc.params["pcg_max_iters"])
     assert iters == o.stats()["pcg_per_solve"][0]
+
+
+def test_one_triangle_faithful_contacts_known_answer():
+    """A one-triangle mesh has a single primitive-bearing leaf, so faithful mode
+    lists the PT contacts of the vertices within the point box: vertices 3 and 4,
+    at their heights above/below the plane, normal +-z toward the point, weights
+    {1, -b} with the barycentrics of the projection."""
+    o = orc.OracleSim(scenes.one_triangle(), 64)
+    cs = o.contacts()
+    assert cs[:, 1].astype(int).tolist() == [3, 4]
+    assert np.all(cs[:, 0] == 0) and np.all(cs[:, 2:5] == [0, 1, 2])
+    V = scenes.one_triangle().verts.astype(np.float64)
+    np.testing.assert_allclose(cs[:, 6], np.abs(V[3:5, 2] - V[0, 2]), rtol=1e-9)
+    np.testing.assert_allclose(cs[:, 7:10], [[0, 0, 1], [0, 0, -1]], atol=1e-12)
+    for row, v in zip(cs, (3, 4)):
+        b1, b2 = V[v, 0] / 0.1, V[v, 1] / 0.1
+        np.testing.assert_allclose(row[13:17], [1, -(1 - b1 - b2), -b1, -b2], atol=1e-6)
+
+
+@pytest.mark.parametrize("e", [0.5, 1.0])
+def test_self_restitution_known_answer(e):
+    """Restitution on a one-triangle mesh: the points in contact move into the
+    triangle at |v_z| = 1 and leave at e (normal exactly +-z); the point moving
+    away, the out-of-range point and the triangle keep their velocities."""
+    o = orc.OracleSim(scenes.one_triangle(contact_restitution=e), 32)
+    v = np.zeros((7, 3))
+    v[3] = [0.25, -0.5, -1.0]    # above, moving down
+    v[4] = [0.0, 0.125, 1.0]     # below, moving up
+    v[5] = [0.0, 0.0, -1.0]      # 2 mm above: not a contact
+    v[0:3, 2] = -0.5             # the triangle moves down too: a PT contact gives its triangle no impulse
+    o.set("v", v)
+    o.restitution()
+    w = o.get("v")
+    np.testing.assert_array_equal(w[3], [0.25, -0.5, e])
+    np.testing.assert_array_equal(w[4], [0.0, 0.125, -e])
+    np.testing.assert_array_equal(w[[0, 1, 2, 5, 6]], v[[0, 1, 2, 5, 6]])
+    # A point already leaving (v.n > 0) is untouched.
+    v2 = v.copy()
+    v2[3, 2] = 0.75
+    o.set("v", v2)
+    o.restitution()
+    np.testing.assert_array_equal(o.get("v")[3], v2[3])
+
+
+@pytest.mark.parametrize("e", [0.5, 1.0])
+def test_self_restitution_edge_edge_signs(e):
+    """Corrected mode, the crossing edges of the edge-edge known answer (normal
+    about -z, from the second edge to the first): the first edge's ends get the
+    impulse along n, the second's along -n, so edges closing at |v_z| = 1 both
+    bounce back to about e; vertices only on the triangles keep their velocity."""
+    V = np.array([[-1, 0, -0.01], [1, 0, 0.01], [0.5, 5, 0.0],
+                  [0, -1, 0.001], [0, 1, 0.001], [-5, 0.5, 9.0]], np.float32)
+    T = np.array([[0, 1, 2], [3, 4, 5]], np.int32)
+    sc = scenes.Scene("ee", V, T, dict(scenes.BASE_MATERIAL), scenes.base_params(contact_restitution=e), [], [], [])
+    o = orc.OracleSim(sc, 64)
+    o.set_contact_mode(True, pt_box=1e-6)
+    v = np.zeros((6, 3))
+    v[[0, 1], 2] = 1.0     # first edge moves up into the second
+    v[[3, 4], 2] = -1.0    # second edge moves down into the first
+    v[[2, 5], 2] = -1.0
+    o.set("v", v)
+    o.restitution()
+    w = o.get("v")
+    np.testing.assert_allclose(w[[0, 1], 2], -e, atol=1e-3)
+    np.testing.assert_allclose(w[[3, 4], 2], e, atol=1e-3)
+    assert np.all(np.abs(w[[0, 1, 3, 4]][:, :2]) < 0.03)
+    np.testing.assert_array_equal(w[[2, 5]], v[[2, 5]])
