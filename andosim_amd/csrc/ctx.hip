@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -264,6 +265,7 @@ struct ando_ctx {
   int sell_lanes = 2;      // ANDO_SELL_LANES: lanes per row of the sliced-ELL SpMV (1, 2 or 4)
   int spmv_env = -1;       // ANDO_PCG_SPMV: -1 unset, 0 row, 1 blk, 2 sell (overrides the mode flag)
   SpmvMeta spmv_meta;      // for the mesh's block pattern
+  std::mutex call_mutex;   // held by every entry point for the whole call (CtxLock)
   // Corrected contact mode (self point-triangle and edge-edge through the LBVH).
   int corrected = 0;
   int static_rigids = 0;   // rigid bodies are fixed colliders: no coupling, no body update
@@ -330,6 +332,17 @@ int guarded(F&& fn) {
     return ANDO_EDEVICE;
   }
 }
+
+// Calls on one context are serialised: the Python module releases the GIL
+// while a step runs, so another thread can reach the same context meanwhile.
+// Every entry point that takes a context holds its mutex for the whole call;
+// distinct contexts run concurrently (ando_step_batch relies on that).
+struct CtxLock {
+  std::unique_lock<std::mutex> lock;
+  explicit CtxLock(ando_ctx* c) {
+    if (c) lock = std::unique_lock<std::mutex>(c->call_mutex);
+  }
+};
 
 void need(bool cond, int code, const char* msg) {
   if (!cond) throw Fail(code, msg);
@@ -1317,6 +1330,7 @@ int ando_destroy(ando_ctx* c) {
 int ando_set_mesh(ando_ctx* c, const float* vertices, int32_t n, const int32_t* triangles, int32_t m,
                   const ando_material* material) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && vertices && material && (m == 0 || triangles), ANDO_EINVAL, "null argument");
     need(n > 0 && m >= 0, ANDO_EINVAL, "mesh needs at least one vertex");
     HIPCHK(hipSetDevice(c->device));
@@ -1365,6 +1379,7 @@ int ando_set_mesh(ando_ctx* c, const float* vertices, int32_t n, const int32_t* 
 
 int ando_set_mesh_vertices(ando_ctx* c, const float* vertices, int32_t n) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && vertices, ANDO_EINVAL, "null argument");
     need(c->have_mesh && n == c->n, ANDO_EINVAL, "vertex count does not match the mesh");
     HIPCHK(hipSetDevice(c->device));
@@ -1401,6 +1416,7 @@ int ando_lumped_masses(const float* vertices, int32_t n, const int32_t* triangle
 
 int ando_set_state(ando_ctx* c, const float* positions, const float* velocities, const float* masses, int32_t n) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && c->have_mesh, ANDO_ESTATE, "mesh must be set before the state");
     need(n == c->n, ANDO_EINVAL, "vertex count does not match the mesh");
     HIPCHK(hipSetDevice(c->device));
@@ -1423,6 +1439,7 @@ int ando_set_state(ando_ctx* c, const float* positions, const float* velocities,
 
 int ando_set_previous_positions(ando_ctx* c, const float* prev, int32_t n) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && c->have_mesh && prev, ANDO_ESTATE, "mesh must be set before the state");
     need(n == c->n, ANDO_EINVAL, "vertex count does not match the mesh");
     HIPCHK(hipSetDevice(c->device));
@@ -1434,6 +1451,7 @@ int ando_set_previous_positions(ando_ctx* c, const float* prev, int32_t n) {
 
 int ando_get_state(ando_ctx* c, float* positions, float* velocities, int32_t n) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && c->have_state, ANDO_ESTATE, "state has not been set");
     need(n == c->n, ANDO_EINVAL, "vertex count does not match the state");
     HIPCHK(hipSetDevice(c->device));
@@ -1447,6 +1465,7 @@ int ando_get_state(ando_ctx* c, float* positions, float* velocities, int32_t n) 
 
 int ando_set_pins(ando_ctx* c, const int32_t* vertices, const float* targets, int32_t count) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && c->have_mesh, ANDO_ESTATE, "mesh must be set before pins");
     need(count >= 0 && (count == 0 || (vertices && targets)), ANDO_EINVAL, "bad pin arrays");
     for (int k = 0; k < count; ++k)
@@ -1473,6 +1492,7 @@ int ando_set_pins(ando_ctx* c, const int32_t* vertices, const float* targets, in
 
 int ando_set_walls(ando_ctx* c, const float* normals, const float* offsets, const float* gaps, int32_t count) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && c->have_mesh, ANDO_ESTATE, "mesh must be set before walls");
     need(count >= 0 && (count == 0 || (normals && offsets && gaps)), ANDO_EINVAL, "bad wall arrays");
     HIPCHK(hipSetDevice(c->device));
@@ -1491,6 +1511,7 @@ int ando_set_walls(ando_ctx* c, const float* normals, const float* offsets, cons
 
 int ando_set_mode(ando_ctx* c, int32_t flags) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c != nullptr, ANDO_EINVAL, "null context");
     need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING | ANDO_MODE_PCG_PERSISTENT |
                     ANDO_MODE_LEARNED_PRECOND | ANDO_MODE_LEARNED_WARM_START_ONLY |
@@ -1512,6 +1533,7 @@ int ando_set_mode(ando_ctx* c, int32_t flags) {
 
 int ando_apply_gravity(ando_ctx* c, const float* g3, float dt) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && c->have_state, ANDO_ESTATE, "State has not been initialised");
     need(g3 != nullptr, ANDO_EINVAL, "gravity is null");
     HIPCHK(hipSetDevice(c->device));
@@ -1523,6 +1545,7 @@ int ando_apply_gravity(ando_ctx* c, const float* g3, float dt) {
 
 int ando_step(ando_ctx* c, const ando_params* params, ando_rigid_body* bodies, int32_t num_bodies) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && c->have_state, ANDO_ESTATE, "state has not been set");
     check_params(params);
     need(num_bodies == 0 || bodies != nullptr, ANDO_EINVAL, "bodies is null");
@@ -1574,6 +1597,7 @@ int ando_step(ando_ctx* c, const ando_params* params, ando_rigid_body* bodies, i
 
 int ando_get_stats(ando_ctx* c, ando_stats* out) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && out, ANDO_EINVAL, "null argument");
     *out = c->stats;
   });
@@ -1582,6 +1606,7 @@ int ando_get_stats(ando_ctx* c, ando_stats* out) {
 int ando_compute_contacts(ando_ctx* c, const ando_rigid_body* bodies, int32_t num_bodies, ando_contact* out,
                           int32_t capacity, int32_t* count) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && c->have_state && count, ANDO_ESTATE, "state has not been set");
     need(num_bodies == 0 || bodies != nullptr, ANDO_EINVAL, "bodies is null");
     HIPCHK(hipSetDevice(c->device));
@@ -1649,6 +1674,7 @@ int ando_compute_contacts(ando_ctx* c, const ando_rigid_body* bodies, int32_t nu
 
 int ando_get_pattern(ando_ctx* c, int32_t* row_ptr, int32_t* cols, int32_t* num_blocks) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && c->have_mesh && num_blocks, ANDO_ESTATE, "mesh has not been set");
     *num_blocks = c->nb;
     if (row_ptr) std::memcpy(row_ptr, c->row_ptr.data(), sizeof(int32_t) * (c->n + 1));
@@ -1659,6 +1685,7 @@ int ando_get_pattern(ando_ctx* c, int32_t* row_ptr, int32_t* cols, int32_t* num_
 int ando_linearize(ando_ctx* c, const ando_params* params, const float* x_target, const ando_rigid_body* bodies,
                    int32_t num_bodies, float* gradient, float* values) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && c->have_state, ANDO_ESTATE, "state has not been set");
     check_params(params);
     need(x_target != nullptr, ANDO_EINVAL, "x_target is null");
@@ -1683,6 +1710,7 @@ int ando_linearize(ando_ctx* c, const ando_params* params, const float* x_target
 
 int ando_apply_restitution(ando_ctx* c, const ando_params* params, ando_rigid_body* bodies, int32_t num_bodies) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && c->have_state, ANDO_ESTATE, "state has not been set");
     check_params(params);
     need(num_bodies >= 0 && (num_bodies == 0 || bodies != nullptr), ANDO_EINVAL, "bodies is null");
@@ -1696,6 +1724,7 @@ int ando_apply_restitution(ando_ctx* c, const ando_params* params, ando_rigid_bo
 int ando_pcg(ando_ctx* c, int32_t num_rows, const int32_t* row_ptr, const int32_t* cols, const float* values,
              const float* b, float* x, float tol, int32_t max_iters, int32_t* iterations, int32_t* converged) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && row_ptr && cols && values && b && x, ANDO_EINVAL, "null argument");
     need(num_rows > 0 && max_iters >= 0, ANDO_EINVAL, "bad sizes");
     HIPCHK(hipSetDevice(c->device));
@@ -1739,6 +1768,7 @@ int ando_pcg(ando_ctx* c, int32_t num_rows, const int32_t* row_ptr, const int32_
 int ando_count_intersections(ando_ctx* c, const ando_rigid_body* bodies, int32_t num_bodies, int32_t* edge_triangle,
                               int32_t* inside_rigid) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && c->have_state, ANDO_ESTATE, "state has not been set");
     need(num_bodies == 0 || bodies != nullptr, ANDO_EINVAL, "bodies is null");
     HIPCHK(hipSetDevice(c->device));
@@ -1814,6 +1844,7 @@ int ando_step_batch(ando_ctx* const* ctxs, const ando_params* params, ando_rigid
 
 int ando_energy_diagnostics(ando_ctx* c, ando_energy* out) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && out, ANDO_EINVAL, "null argument");
     need(c->have_state, ANDO_ESTATE, "state has not been set");
     HIPCHK(hipSetDevice(c->device));
@@ -1836,6 +1867,7 @@ int ando_energy_diagnostics(ando_ctx* c, ando_energy* out) {
 int ando_contact_metrics(ando_ctx* c, const ando_contact* contacts, int32_t count, float gap_max, int32_t ccd,
                          ando_collision_metrics* out) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && out, ANDO_EINVAL, "null argument");
     need(c->have_state, ANDO_ESTATE, "state has not been set");
     HIPCHK(hipSetDevice(c->device));
@@ -1881,6 +1913,7 @@ int ando_contact_metrics(ando_ctx* c, const ando_contact* contacts, int32_t coun
 
 int ando_cfl_inputs(ando_ctx* c, float* max_velocity, float* min_edge_length) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && max_velocity && min_edge_length, ANDO_EINVAL, "null argument");
     need(c->have_state, ANDO_ESTATE, "state has not been set");
     HIPCHK(hipSetDevice(c->device));
@@ -1906,6 +1939,7 @@ int32_t ando_nn_num_weights(void) { return ando::nn::kNumWeights; }
 
 int ando_set_precond_weights(ando_ctx* c, const float* weights, int32_t count) {
   return guarded([&] {
+    CtxLock lock(c);
     need(c && weights, ANDO_EINVAL, "null argument");
     need(count == ando::nn::kNumWeights, ANDO_EINVAL, "weight count differs from ando_nn_num_weights()");
     HIPCHK(hipSetDevice(c->device));

@@ -6,6 +6,8 @@
  * opaque context, and returns 0 on success or a negative ANDO_E* code with a
  * message readable through ando_last_error().  Solver non-convergence is not
  * an error (the reference only prints it); it is reported in ando_stats.
+ * Calls on one context are serialised (each holds the context's lock for the
+ * whole call); distinct contexts may be used from different threads at once.
  *
  * Reference interfaces replaced (paths relative to the AndoSim source tree):
  *   ando_set_mesh          Mesh::initialize + compute_rest_state       src/core/mesh.cpp:8-75

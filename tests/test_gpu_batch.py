@@ -45,3 +45,30 @@ def test_batch_with_rigid_bodies_and_errors():
     with pytest.raises(RuntimeError, match="twice"):
         native.step_batch([bat[0], bat[0]])
     native.step_batch([])
+
+
+def test_concurrent_calls_on_one_context_are_serialised():
+    """ctypes releases the GIL during a foreign call, so two threads stepping
+    the same context really call ando_step at once; the context lock
+    serialises them, and the result equals two steps one after the other."""
+    import threading
+    sc = scenes.small_sheet(17, pins=2, wall=True)
+    a, b = native.GpuSim(sc), native.GpuSim(sc)
+    errors = []
+
+    def run():
+        try:
+            a.step()
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(e)
+
+    ts = [threading.Thread(target=run) for _ in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
+    b.step()
+    b.step()
+    assert np.array_equal(a.get("x"), b.get("x"))
+    assert np.array_equal(a.get("v"), b.get("v"))
