@@ -1216,6 +1216,11 @@ PYBIND11_MODULE(ando_barrier_core, m) {
       .def_readonly("beta", &ando_stats::beta)
       .def_readonly("last_alpha", &ando_stats::last_alpha)
       .def_readonly("step_ms", &ando_stats::step_ms)
+      .def_readonly("pcg_kernel", &ando_stats::pcg_kernel)
+      .def_readonly("spmv_lanes", &ando_stats::spmv_lanes)
+      .def_readonly("learned_solves", &ando_stats::learned_solves)
+      .def_readonly("learned_fallbacks", &ando_stats::learned_fallbacks)
+      .def_readonly("fallback_iterations", &ando_stats::fallback_iterations)
       .def_property_readonly("pcg_per_solve", [](const ando_stats& s) {
         int k = std::min(s.pcg_solves, (int32_t)ANDO_MAX_RECORDED_SOLVES);
         return std::vector<int32_t>(s.pcg_per_solve, s.pcg_per_solve + k);

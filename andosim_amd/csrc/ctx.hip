@@ -252,6 +252,7 @@ struct ando_ctx {
   PcgStatus* status_pinned = nullptr;  // host copy of the PCG status (pinned)
   hipEvent_t status_ev = nullptr;
   int pcg_chunk = 8;
+  int pcg_chunk_max = 256;  // ANDO_PCG_CHUNK_MAX: largest chunk (chunks double from pcg_chunk)
   int rigid_group = kGroup;  // rigid triangles per culling box
   ando_stats stats{};
   int fp64_near = 1;
@@ -879,8 +880,8 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
   HIPCHK(hipGetLastError());
   PcgStatus st{};
   int it = 0;
-  // The iterations go out in chunks of 8, 16, 32, then 64 iterations; after each
-  // chunk the status is copied to pinned host memory and read.  Launches
+  // The iterations go out in chunks of 8, 16, 32, 64, 128, then 256 iterations;
+  // after each chunk the status is copied to pinned host memory and read.  Launches
   // queued after convergence return at the done flag (after issuing their
   // first loads, which lets a live launch overlap the flag's latency).  Tried
   // and slower on this path: queueing the next chunk before waiting, and
@@ -966,7 +967,7 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
       st = *c->status_pinned;
       break;
     }
-    chunk = std::min(chunk * 2, 64);
+    chunk = std::min(chunk * 2, c->pcg_chunk_max);
   }
   if (timing) sync(c);
   if (timing) {
@@ -1301,6 +1302,8 @@ int ando_create(int32_t device, ando_ctx** out) {
     }
     if (const char* w = std::getenv("ANDO_PCG_WGS")) c->pcg_wgs = std::atoi(w);
     if (const char* w = std::getenv("ANDO_PCG_CHUNK")) c->pcg_chunk = std::min(256, std::max(1, std::atoi(w)));
+    if (const char* w = std::getenv("ANDO_PCG_CHUNK_MAX"))
+      c->pcg_chunk_max = std::min(1024, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_RIGID_GROUP")) c->rigid_group = std::min(256, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_BARRIER")) c->pcg_barrier = std::atoi(w) == 1 ? 1 : 0;
     if (const char* w = std::getenv("ANDO_SELL_LANES")) {

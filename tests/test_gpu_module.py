@@ -150,6 +150,7 @@ def test_module_step_matches_oracle(make):
     assert worst < 1e-4, worst
     st = abc.Integrator.last_stats()
     assert st.pcg_solves >= 1 and st.newton_iterations >= 1
+    assert st.pcg_kernel == 3 and st.spmv_lanes == 2 and st.learned_solves == 0
     if bodies:
         assert np.allclose(bodies[0].position, o.rigid_state(0)["position"], atol=1e-6)
     # The per-frame consumers of the add-on still work after GPU steps.
