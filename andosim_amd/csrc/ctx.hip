@@ -264,6 +264,8 @@ struct ando_ctx {
   int spmv_blocks = 0;     // 1: block-parallel SpMV where the pattern allows it, 0: row SpMV
   int spmv_sell = 1;       // 1: sliced-ELL SpMV (values copied once per solve), 0: BSR kernels
   int sell_lanes = 2;      // ANDO_SELL_LANES: lanes per row of the sliced-ELL SpMV (1, 2 or 4)
+  int sell_wg_slices = 1;  // ANDO_SELL_WG_SLICES: slices per workgroup with two lanes per row (0: a wave, 1, 2, 4)
+  int pcg_vec_block = kBlock;  // ANDO_PCG_VEC_BLOCK: threads per workgroup of the PCG vector kernels (64, 128, 256)
   int spmv_env = -1;       // ANDO_PCG_SPMV: -1 unset, 0 row, 1 blk, 2 sell (overrides the mode flag)
   SpmvMeta spmv_meta;      // for the mesh's block pattern
   std::mutex call_mutex;   // held by every entry point for the whole call (CtxLock)
@@ -851,9 +853,16 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
   const int R = c->spmv_rowsets;
   const bool sell = c->spmv_sell && sm != nullptr && sm->sell_ok;
   const bool blk = !sell && c->spmv_blocks && sm != nullptr && sm->ok && !couple;
-  int nwg_v = nblk(n), nwg_r = sell ? nblk(sm->slices, kBlock / 64) : (blk ? sm->groups : spmv_groups(n, R));
+  const int sell_slices = (c->sell_lanes == 2) ? c->sell_wg_slices : 4;  // slices per SpMV workgroup (0: a wave)
+  const int vb = c->pcg_vec_block;  // threads per workgroup of the vector kernels (init, update)
+  int nwg_v = nblk(n, vb);
+  int nwg_r = sell ? (sell_slices == 0 ? 2 * sm->slices : nblk(sm->slices, sell_slices))
+                   : (blk ? sm->groups : spmv_groups(n, R));
+  c->part_rz.ensure((size_t)nwg_v + 1);
+  c->part_max.ensure((size_t)nwg_v + 1);
   c->stats.pcg_kernel = sell ? 3 : (blk ? 2 : 0);
   c->stats.spmv_lanes = sell ? c->sell_lanes : 0;
+  c->stats.spmv_wg_slices = sell ? sell_slices : 0;
   c->rz_hist.ensure((size_t)maxit + 2);
   c->pcg_status.ensure(1);
   c->part_pAp.ensure((size_t)nwg_r + 1);
@@ -867,11 +876,11 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
     pe = 1;
     pi = 9;
     c->part_bmax.ensure((size_t)nwg_v + 1);
-    hipLaunchKernelGGL(k_pcg_init_given, dim3(nwg_v), dim3(kBlock), 0, s, n, A, rp, cols, P_given, rhs, negate,
+    hipLaunchKernelGGL(k_pcg_init_given, dim3(nwg_v), dim3(vb), 0, s, n, A, rp, cols, P_given, rhs, negate,
                        x0_given, xsol, c->r.p, c->z.p, c->pbuf0.p, c->part_rz.p, c->part_max.p, c->part_bmax.p);
     Pc = P_given;
   } else {
-    hipLaunchKernelGGL(k_pcg_init, dim3(nwg_v), dim3(kBlock), 0, s, n, A, diag, rhs, xsol, c->r.p, c->z.p,
+    hipLaunchKernelGGL(k_pcg_init, dim3(nwg_v), dim3(vb), 0, s, n, A, diag, rhs, xsol, c->r.p, c->z.p,
                        c->pbuf0.p, c->Pinv.p, pe, pi, c->part_rz.p, c->part_max.p, negate);
   }
   hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, s, nwg_v, (const double*)c->part_rz.p,
@@ -923,7 +932,10 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
                                 (const double*)c->part_rz.p, (const float*)c->part_max.p, nwg_v, c->rz_hist.p,
                                 c->part_pAp.p, c->pcg_status.p, tol);
         };
-        if (c->sell_lanes == 2) sell_launch(k_pcg_spmv_sell_split<2>, 2 * kBlock);
+        if (c->sell_lanes == 2 && sell_slices == 0) sell_launch(k_pcg_spmv_sell_split<2, 0>, 64);
+        else if (c->sell_lanes == 2 && sell_slices == 1) sell_launch(k_pcg_spmv_sell_split<2, 1>, 128);
+        else if (c->sell_lanes == 2 && sell_slices == 2) sell_launch(k_pcg_spmv_sell_split<2, 2>, 256);
+        else if (c->sell_lanes == 2) sell_launch(k_pcg_spmv_sell_split<2>, 2 * kBlock);
         else if (c->sell_lanes == 4) sell_launch(k_pcg_spmv_sell_split<4>, 4 * kBlock);
         else sell_launch(k_pcg_spmv_sell, kBlock);
       } else if (blk) {
@@ -943,7 +955,7 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
         hipLaunchKernelGGL(ando::lbvh::k_self_spmv, dim3(nblk(n, 32)), dim3(kBlock), 0, s, n, sv->ptr,
                            (const int32_t*)c->ccol.p, (const float*)c->cval.p, (const PcgStatus*)c->pcg_status.p,
                            (const float*)pcur, c->Ap.p, c->part_pAp2.p);
-      hipExtLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(kBlock), 0, s, e2, e3, 0, n, it, (const float*)pcur,
+      hipExtLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(vb), 0, s, e2, e3, 0, n, it, (const float*)pcur,
                             (const float*)c->Ap.p, (const float*)(blk ? sm->extra.p : nullptr),
                             (const int32_t*)(blk ? sm->straddle.p : nullptr), Pc, pe, pi, xsol, c->r.p,
                             c->z.p,
@@ -1306,6 +1318,14 @@ int ando_create(int32_t device, ando_ctx** out) {
       c->pcg_chunk_max = std::min(1024, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_RIGID_GROUP")) c->rigid_group = std::min(256, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_BARRIER")) c->pcg_barrier = std::atoi(w) == 1 ? 1 : 0;
+    if (const char* w = std::getenv("ANDO_PCG_VEC_BLOCK")) {
+      const int k = std::atoi(w);
+      c->pcg_vec_block = (k == 64 || k == 128) ? k : kBlock;
+    }
+    if (const char* w = std::getenv("ANDO_SELL_WG_SLICES")) {
+      const int k = std::atoi(w);
+      c->sell_wg_slices = (k == 0 || k == 2 || k == 4) ? k : 1;
+    }
     if (const char* w = std::getenv("ANDO_SELL_LANES")) {
       const int l = std::atoi(w);
       c->sell_lanes = (l == 1 || l == 4) ? l : 2;

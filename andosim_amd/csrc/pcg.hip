@@ -85,24 +85,27 @@ __device__ inline float thread_max_partials(const float* p, int count) {
 // and summed in the same order by finish_*.
 // (tid, nt): this thread's index among the nt threads that share the
 // partials; the plain forms use the whole workgroup.
-template <class T>
-__device__ inline void prefetch_partials(const T* p, int count, T (&v)[kUnroll], int tid, int nt) {
+// The prefetch holds N partials per thread (N = kUnroll, or more where a
+// kernel reads many); the sums add them in index order either way, so the
+// result does not depend on N.
+template <class T, int N>
+__device__ inline void prefetch_partials(const T* p, int count, T (&v)[N], int tid, int nt) {
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u) {
+  for (int u = 0; u < N; ++u) {
     const int k = u * nt + tid;
     v[u] = p[k < count ? k : count - 1];
   }
 }
-template <class T>
-__device__ inline void prefetch_partials(const T* p, int count, T (&v)[kUnroll]) {
+template <class T, int N>
+__device__ inline void prefetch_partials(const T* p, int count, T (&v)[N]) {
   prefetch_partials(p, count, v, (int)threadIdx.x, (int)blockDim.x);
 }
-__device__ inline double finish_sum_partials(const double* p, int count, const double (&v)[kUnroll], int tid,
-                                             int nt) {
+template <int N>
+__device__ inline double finish_sum_partials(const double* p, int count, const double (&v)[N], int tid, int nt) {
   double s = 0.0;
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u) s += (u * nt + tid) < count ? v[u] : 0.0;
-  for (int base = kUnroll * nt; base < count; base += kUnroll * nt) {
+  for (int u = 0; u < N; ++u) s += (u * nt + tid) < count ? v[u] : 0.0;
+  for (int base = N * nt; base < count; base += kUnroll * nt) {
     double w[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
@@ -114,14 +117,16 @@ __device__ inline double finish_sum_partials(const double* p, int count, const d
   }
   return s;
 }
-__device__ inline double finish_sum_partials(const double* p, int count, const double (&v)[kUnroll]) {
+template <int N>
+__device__ inline double finish_sum_partials(const double* p, int count, const double (&v)[N]) {
   return finish_sum_partials(p, count, v, (int)threadIdx.x, (int)blockDim.x);
 }
-__device__ inline float finish_max_partials(const float* p, int count, const float (&v)[kUnroll], int tid, int nt) {
+template <int N>
+__device__ inline float finish_max_partials(const float* p, int count, const float (&v)[N], int tid, int nt) {
   float s = 0.0f;
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u) s = fmaxf(s, (u * nt + tid) < count ? v[u] : 0.0f);
-  for (int base = kUnroll * nt; base < count; base += kUnroll * nt) {
+  for (int u = 0; u < N; ++u) s = fmaxf(s, (u * nt + tid) < count ? v[u] : 0.0f);
+  for (int base = N * nt; base < count; base += kUnroll * nt) {
     float w[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
@@ -133,7 +138,8 @@ __device__ inline float finish_max_partials(const float* p, int count, const flo
   }
   return s;
 }
-__device__ inline float finish_max_partials(const float* p, int count, const float (&v)[kUnroll]) {
+template <int N>
+__device__ inline float finish_max_partials(const float* p, int count, const float (&v)[N]) {
   return finish_max_partials(p, count, v, (int)threadIdx.x, (int)blockDim.x);
 }
 
@@ -608,8 +614,12 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_sell(int n, int slices, int
 // r.z / max|r| reduction runs on the first 256 threads (the rest add exact
 // zeros) and the p.Ap partial is reduced from the one-lane layout restored in
 // LDS, so both associate as in the one-lane kernel.
-template <int L>
-__global__ void __launch_bounds__(256 * L) k_pcg_spmv_sell_split(int n, int slices, int it,
+// S (slices per workgroup, default 4) below 4 gives a finer grid, S = 0 one
+// wave per workgroup (64 / L rows of a slice); the r.z / max|r| and p.Ap
+// reductions then run over the workgroup's threads, another association than
+// the one-lane kernel's.
+template <int L, int S = 4>
+__global__ void __launch_bounds__(64 * (S == 0 ? 1 : S * L)) k_pcg_spmv_sell_split(int n, int slices, int it,
                            const float* __restrict__ vals, const int32_t* __restrict__ scol,
                            const int32_t* __restrict__ slice_off, const int32_t* __restrict__ slice_w, int uniform_w,
                            const float* __restrict__ z, const float* __restrict__ p_old, float* __restrict__ p_new,
@@ -617,17 +627,20 @@ __global__ void __launch_bounds__(256 * L) k_pcg_spmv_sell_split(int n, int slic
                            const float* __restrict__ part_max, int nwg_v, double* __restrict__ rz_hist,
                            double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol) {
   static_assert(L == 2 || L == 4, "two or four lanes per row");
+  static_assert(S == 0 || S == 1 || S == 2 || S == 4, "a wave, or one, two or four slices per workgroup");
   constexpr int R = 64 / L;             // rows per wave
   constexpr int KC = kSellChunk / L;    // slots per lane per chunk
-  __shared__ double ldsd[4 * L];
-  __shared__ float ldsf[4 * L];
-  __shared__ double ldsp[4][64];
+  constexpr int WPW = S == 0 ? 1 : S * L;         // waves per workgroup
+  constexpr int RT = 64 * WPW < 256 ? 64 * WPW : 256;  // threads that reduce the partials
+  __shared__ double ldsd[WPW];
+  __shared__ float ldsf[WPW];
+  __shared__ double ldsp[S == 0 ? 1 : S][64];
   const int g = xcd_logical_group(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   const int wv = tid >> 6, lane = tid & 63;
   const int q = lane / R, rl = lane % R;
-  const int wslice = wv / L, sub = wv % L;
-  const int sl0 = __builtin_amdgcn_readfirstlane(g * 4 + wslice);
+  const int wslice = S == 0 ? 0 : wv / L, sub = S == 0 ? g % L : wv % L;
+  const int sl0 = __builtin_amdgcn_readfirstlane(S == 0 ? g / L : g * S + wslice);
   const bool live = sl0 < slices;
   const int sl = live ? sl0 : slices - 1;
   const int rs = sub * R + rl;  // row within the slice
@@ -635,12 +648,12 @@ __global__ void __launch_bounds__(256 * L) k_pcg_spmv_sell_split(int n, int slic
   const int ri = row < n ? row : n - 1;
   const int off = uniform_w > 0 ? sl * 64 * uniform_w : slice_off[sl];
   const int W = !live ? 0 : (uniform_w > 0 ? uniform_w : slice_w[sl]);
-  const bool red = tid < 256;
+  const bool red = tid < RT;
   double pre_rz[kUnroll];
   float pre_max[kUnroll];
   if (it > 0 && red) {
-    prefetch_partials(part_rz, nwg_v, pre_rz, tid, 256);
-    prefetch_partials(part_max, nwg_v, pre_max, tid, 256);
+    prefetch_partials(part_rz, nwg_v, pre_rz, tid, RT);
+    prefetch_partials(part_max, nwg_v, pre_max, tid, RT);
   }
   int jc[KC];
   float Ab[KC][9];
@@ -677,8 +690,8 @@ __global__ void __launch_bounds__(256 * L) k_pcg_spmv_sell_split(int n, int slic
   if (it > 0) {
     const float bmax = st->bmax;
     const double rz_prev = rz_hist[it - 1];
-    double rz = red ? finish_sum_partials(part_rz, nwg_v, pre_rz, tid, 256) : 0.0;
-    float rmax = red ? finish_max_partials(part_max, nwg_v, pre_max, tid, 256) : 0.0f;
+    double rz = red ? finish_sum_partials(part_rz, nwg_v, pre_rz, tid, RT) : 0.0;
+    float rmax = red ? finish_max_partials(part_max, nwg_v, pre_max, tid, RT) : 0.0f;
     block_reduce_sum_max(rz, rmax, ldsd, ldsf);
     float rel = ((double)bmax < 1e-16) ? rmax : rmax / bmax;
     if (rel < tol) {
@@ -767,9 +780,12 @@ __global__ void __launch_bounds__(256 * L) k_pcg_spmv_sell_split(int n, int slic
     Ap[3 * row + 2] = sr[2];
     pap = (double)q0 * sr[0] + (double)q1 * sr[1] + (double)q2 * sr[2];
   }
-  if (q == 0) ldsp[wslice][rs] = pap;
-  __syncthreads();
-  const double pv = red ? ldsp[wv][lane] : 0.0;
+  double pv = pap;
+  if constexpr (S != 0) {
+    if (q == 0) ldsp[wslice][rs] = pap;
+    __syncthreads();
+    pv = tid < 64 * S ? ldsp[wv][lane] : 0.0;
+  }
   double sp = block_reduce(pv, ldsd, [](double a, double b) { return a + b; });
   if (tid == 0) part_pAp[g] = sp;
 }
@@ -915,7 +931,9 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const floa
   // The p.Ap partials head the longest dependency chain (reduction, alpha,
   // update): they are requested first, then every other load that does not
   // depend on alpha.
-  double pre[kUnroll];
+  // Eight per thread: with one SpMV workgroup per slice there are about
+  // n / 64 partials, which then take one pass.
+  double pre[2 * kUnroll];
   prefetch_partials(part_pAp, nwg_r, pre);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {

@@ -360,6 +360,7 @@ def main(argv=None):
     newton = sum(r[1]["newton_iterations"] for r in recs)
     kernel = recs[-1][1]["pcg_kernel"]
     lanes = recs[-1][1].get("spmv_lanes", 0)
+    wg_slices = recs[-1][1].get("spmv_wg_slices", 4)
 
     # Event pass: the same window again, every PCG launch bracketed by events.
     ev_recs = []
@@ -391,7 +392,7 @@ def main(argv=None):
         ach = by / avg_s / 1e9
         kname = {3: "ando::k_pcg_spmv_sell", 2: "ando::k_pcg_spmv_blk"}.get(kernel, "ando::k_pcg_spmv<3>")
         if kernel == 3 and lanes in (2, 4):
-            kname = f"ando::k_pcg_spmv_sell_split<{lanes}>"
+            kname = f"ando::k_pcg_spmv_sell_split<{lanes}, {wg_slices}>"
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(ach / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(a.workload, n, nb, kname), "kernel": kname,
@@ -464,7 +465,8 @@ def main(argv=None):
                                "r=0.1 m (4,800 rigid tris); dt=0.01, 5% strain limit, PCG tol 1e-6, faithful mode",
                    "vertices": n, "blocks": nb, "precond": "block-Jacobi (plain PCG)",
                    "pcg_kernel": {1: "persistent", 0: "two-kernel, BSR row SpMV", 2: "two-kernel, block SpMV",
-                                  3: f"two-kernel, sliced-ELL SpMV ({lanes} lane{'s' if lanes > 1 else ''} per row)"
+                                  3: f"two-kernel, sliced-ELL SpMV ({lanes} lane{'s' if lanes > 1 else ''} per row, "
+                                     f"{wg_slices} slice{'s' if wg_slices != 1 else ''} per workgroup)"
                                   }.get(kernel, "none"),
                    "timed_window": [window[0], window[-1]],
                    "first_nonfinite_step": last_ok + 1 if last_ok + 1 < a.max_explore else None},

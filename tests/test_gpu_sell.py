@@ -100,11 +100,12 @@ def test_sell_steps_match_row(name, make, steps, corrected):
 
 @pytest.mark.parametrize("lanes", [2, 4])
 def test_sell_lane_split_bit_identical(lanes, monkeypatch):
-    """Two (the default) or four lanes per row (ANDO_SELL_LANES) sum in the
-    one-lane kernel's order and reduce its partials the same way: solves and
-    steps agree bit for bit."""
+    """Two (the default) or four lanes per row (ANDO_SELL_LANES) on the
+    one-lane kernel's grid of four slices per workgroup sum in its order and
+    reduce its partials the same way: solves and steps agree bit for bit."""
     print(f"sell lanes={lanes}", flush=True)
     rp, ci, vals, b = random_spd_bsr(4097, 3, extra_rows=(0, 2048, 4096))
+    monkeypatch.setenv("ANDO_SELL_WG_SLICES", "4")   # the one-lane kernel's grid
     monkeypatch.setenv("ANDO_SELL_LANES", "1")
     one = native.GpuSim()
     monkeypatch.setenv("ANDO_SELL_LANES", str(lanes))
@@ -119,8 +120,39 @@ def test_sell_lane_split_bit_identical(lanes, monkeypatch):
     monkeypatch.setenv("ANDO_SELL_LANES", str(lanes))
     s = native.GpuSim(sc)
     monkeypatch.delenv("ANDO_SELL_LANES")
+    monkeypatch.delenv("ANDO_SELL_WG_SLICES")
     for _ in range(4):
         a.substep()
         s.substep()
         assert a.stats()["pcg_iterations"] == s.stats()["pcg_iterations"]
     assert np.array_equal(a.get("x"), s.get("x"))
+
+
+@pytest.mark.parametrize("slices,vb", [(4, 64), (2, 256), (1, 256), (1, 64), (0, 256), (0, 128)])
+def test_sell_finer_grid_matches(slices, vb, monkeypatch):
+    """Fewer slices per SpMV workgroup (ANDO_SELL_WG_SLICES; 0 = one wave) and
+    smaller vector-kernel workgroups (ANDO_PCG_VEC_BLOCK): the same products,
+    partial sums over other workgroups; solves agree with the four-slice,
+    256-thread grid to 1e-6 with iteration counts within 1%, and steps to 1e-6."""
+    print(f"sell slices per workgroup={slices} vector block={vb}", flush=True)
+    rp, ci, vals, b = random_spd_bsr(4097, 3, extra_rows=(0, 2048, 4096))
+    monkeypatch.setenv("ANDO_SELL_WG_SLICES", "4")
+    monkeypatch.setenv("ANDO_PCG_VEC_BLOCK", "256")
+    ref = native.GpuSim()
+    sc = scenes.sphere129()
+    a = native.GpuSim(sc)
+    monkeypatch.setenv("ANDO_SELL_WG_SLICES", str(slices))
+    monkeypatch.setenv("ANDO_PCG_VEC_BLOCK", str(vb))
+    fine = native.GpuSim()
+    s = native.GpuSim(sc)
+    monkeypatch.delenv("ANDO_SELL_WG_SLICES")
+    monkeypatch.delenv("ANDO_PCG_VEC_BLOCK")
+    x1, i1, c1 = ref.pcg(rp, ci, vals, b, tol=1e-6, maxit=2000)
+    x2, i2, c2 = fine.pcg(rp, ci, vals, b, tol=1e-6, maxit=2000)
+    assert c1 and c2 and abs(i1 - i2) <= max(1, i1 // 100)
+    assert np.max(np.abs(x1 - x2)) <= 1e-6 * np.max(np.abs(x1))
+    for _ in range(4):
+        a.substep()
+        s.substep()
+        assert abs(a.stats()["pcg_iterations"] - s.stats()["pcg_iterations"]) <= max(2, a.stats()["pcg_iterations"] // 100)
+    assert np.max(np.abs(a.get("x") - s.get("x"))) <= 1e-6 * max(1.0, float(np.max(np.abs(a.get("x")))))
