@@ -156,3 +156,31 @@ def test_sell_finer_grid_matches(slices, vb, monkeypatch):
         s.substep()
         assert abs(a.stats()["pcg_iterations"] - s.stats()["pcg_iterations"]) <= max(2, a.stats()["pcg_iterations"] // 100)
     assert np.max(np.abs(a.get("x") - s.get("x"))) <= 1e-6 * max(1.0, float(np.max(np.abs(a.get("x")))))
+
+
+def _friction_sheet():
+    sc = scenes.small_sheet(9, pins=0, wall=False, rigid=True)
+    sc.params.update(enable_friction=True, friction_mu=0.5)
+    return sc
+
+
+def test_assembled_matrix_is_bit_symmetric():
+    """The assembled system equals its block transpose bit for bit (every
+    contribution is symmetric and both orientations are summed in the same
+    order), on a sheet with wall and pins, a sphere drape, and friction on a
+    rigid body."""
+    for sc in (scenes.small_sheet(9), scenes.sphere129(), _friction_sheet()):
+        g = native.GpuSim(sc, fp64_near=False)
+        for _ in range(3):
+            g.substep()
+        x = g.get("x")
+        xhat = (x + np.float32(sc.params["dt"]) * g.get("v")).astype(np.float32)
+        _, H = g.linearize(xhat)
+        rp, ci = g.pattern()
+        for i in range(len(rp) - 1):
+            for k in range(rp[i], rp[i + 1]):
+                j = ci[k]
+                if j <= i:
+                    continue
+                kk = rp[j] + int(np.searchsorted(ci[rp[j]:rp[j + 1]], i))
+                assert ci[kk] == i and np.array_equal(H[k], H[kk].T), (sc.name, i, j)
