@@ -10,10 +10,12 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -269,6 +271,7 @@ struct ando_ctx {
   int spmv_env = -1;       // ANDO_PCG_SPMV: -1 unset, 0 row, 1 blk, 2 sell (overrides the mode flag)
   SpmvMeta spmv_meta;      // for the mesh's block pattern
   std::mutex call_mutex;   // held by every entry point for the whole call (CtxLock)
+  hipEvent_t batch_ev = nullptr;  // a batched solve's set-up is done on the context's stream
   // Corrected contact mode (self point-triangle and edge-edge through the LBVH).
   int corrected = 0;
   int static_rigids = 0;   // rigid bodies are fixed colliders: no coupling, no body update
@@ -766,6 +769,190 @@ struct PcgOut {
   bool converged;
 };
 
+// Batched PCG for ando_step_batch.  The worker threads step their contexts as
+// usual; a solve that qualifies (sliced-ELL SpMV on the default grid, no
+// coupling blocks, no learned preconditioner, no kernel timing) is set up on
+// the context's stream and handed over here.  The first waiting thread leads:
+// once every active worker waits (or 100 us after it arrived) it takes all
+// waiting solves with its own iteration cap, runs their iterations together
+// (k_pcg_spmv_sell_batch + k_pcg_update_batch, one status read-back per chunk
+// for all of them) on the batch stream, and releases them; the contexts'
+// streams wait for the batch stream before going on.  Each system's
+// arithmetic is that of its own launch, so results do not depend on batching.
+struct BatchJob {
+  ando_ctx* c = nullptr;
+  SpmvMeta* sm = nullptr;
+  float* xsol = nullptr;
+  int n = 0, nwg_v = 0, nwg_r = 0, maxit = 0;
+  float tol = 0.0f;
+  hipEvent_t ready = nullptr;
+  PcgOut out{0, false};
+  bool done = false;
+  std::string error;
+};
+
+// The batch the calling thread steps its contexts for (ando_step_batch), or null.
+struct BatchPcg;
+thread_local BatchPcg* t_batch = nullptr;
+
+struct BatchPcg {
+  std::mutex mu;
+  std::condition_variable cv;
+  int active = 0;       // workers that still have contexts to step
+  bool leading = false;
+  std::vector<BatchJob*> waiting;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  DBuf<ando::PcgSys> sys;
+  DBuf<int32_t> wg_r, wg_v;
+  DBuf<PcgStatus> st_all;
+  PcgStatus* st_host = nullptr;  // pinned
+  int st_cap = 0;
+  int chunk0 = 8, chunk_max = 256;
+  int gather_us = 100;  // how long a solve waits for others before leading a round
+  long rounds = 0, solves = 0;
+
+  ~BatchPcg() {
+    if (st_host) (void)hipHostFree(st_host);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void worker_done() {
+    std::lock_guard<std::mutex> lk(mu);
+    --active;
+    cv.notify_all();
+  }
+
+  PcgOut solve(BatchJob& job) {
+    HIPCHK(hipEventRecord(job.ready, job.c->stream));
+    const auto deadline = std::chrono::steady_clock::now() + std::chrono::microseconds(gather_us);
+    std::unique_lock<std::mutex> lk(mu);
+    waiting.push_back(&job);
+    cv.notify_all();
+    while (!job.done) {
+      if (leading) {
+        cv.wait(lk);
+        continue;
+      }
+      if ((int)waiting.size() < active && std::chrono::steady_clock::now() < deadline) {
+        cv.wait_until(lk, deadline);
+        continue;
+      }
+      // Lead a round: every waiting solve with this solve's iteration cap.
+      leading = true;
+      std::vector<BatchJob*> round, rest;
+      for (BatchJob* j : waiting) (j->maxit == job.maxit ? round : rest).push_back(j);
+      waiting.swap(rest);
+      lk.unlock();
+      std::string err;
+      try {
+        run_round(round);
+      } catch (const std::exception& e) {
+        err = e.what();
+      }
+      lk.lock();
+      for (BatchJob* j : round) {
+        j->error = err;
+        j->done = true;
+      }
+      leading = false;
+      cv.notify_all();
+    }
+    lk.unlock();
+    if (!job.error.empty()) throw Fail(ANDO_EDEVICE, "batched PCG: " + job.error);
+    // The round ended with a synchronisation of the batch stream, so the
+    // solution is in place before the context's stream goes on.
+    return job.out;
+  }
+
+  void run_round(std::vector<BatchJob*>& jobs) {
+    HIPCHK(hipSetDevice(device));
+    const int ns = (int)jobs.size();
+    std::vector<ando::PcgSys> h(ns);
+    std::vector<int32_t> hr, hv;
+    for (int k = 0; k < ns; ++k) {
+      BatchJob& j = *jobs[k];
+      ando_ctx* c = j.c;
+      HIPCHK(hipStreamWaitEvent(stream, j.ready, 0));
+      ando::PcgSys& S = h[k];
+      S.n = j.n;
+      S.slices = j.sm->slices;
+      S.uniform_w = j.sm->uniform_w;
+      S.nwg_v = j.nwg_v;
+      S.nwg_r = j.nwg_r;
+      S.wg0_r = (int)hr.size();
+      S.wg0_v = (int)hv.size();
+      S.vals = j.sm->sell_vals.p;
+      S.scol = j.sm->sell_col.p;
+      S.slice_off = j.sm->slice_off.p;
+      S.slice_w = j.sm->slice_w.p;
+      S.Pinv = c->Pinv.p;
+      S.pe = j.n;
+      S.pi = 1;
+      S.z = c->z.p;
+      S.p0 = c->pbuf0.p;
+      S.p1 = c->pbuf1.p;
+      S.Ap = c->Ap.p;
+      S.x = j.xsol;
+      S.r = c->r.p;
+      S.part_rz = c->part_rz.p;
+      S.part_max = c->part_max.p;
+      S.rz_hist = c->rz_hist.p;
+      S.part_pAp = c->part_pAp.p;
+      S.st = c->pcg_status.p;
+      S.tol = j.tol;
+      hr.insert(hr.end(), j.nwg_r, k);
+      hv.insert(hv.end(), j.nwg_v, k);
+    }
+    sys.upload(h, stream);
+    wg_r.upload(hr, stream);
+    wg_v.upload(hv, stream);
+    st_all.ensure((size_t)ns);
+    if (st_cap < ns) {
+      if (st_host) HIPCHK(hipHostFree(st_host));
+      st_host = nullptr;
+      HIPCHK(hipHostMalloc((void**)&st_host, sizeof(PcgStatus) * ns));
+      st_cap = ns;
+    }
+    // The tables are read by the kernels below; the uploads finish first.
+    HIPCHK(hipStreamSynchronize(stream));
+    const int maxit = jobs[0]->maxit;
+    auto read_status = [&] {
+      hipLaunchKernelGGL(ando::k_pcg_status_gather, dim3((ns + 63) / 64), dim3(64), 0, stream, ns,
+                         (const ando::PcgSys*)sys.p, st_all.p);
+      HIPCHK(hipMemcpyAsync(st_host, st_all.p, sizeof(PcgStatus) * ns, hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    };
+    int it = 0, chunk = chunk0;
+    for (;;) {
+      const int end = std::min(maxit, it + chunk);
+      for (; it < end; ++it) {
+        hipLaunchKernelGGL(ando::k_pcg_spmv_sell_batch, dim3((int)hr.size()), dim3(128), 0, stream,
+                           (const ando::PcgSys*)sys.p, (const int32_t*)wg_r.p, it);
+        hipLaunchKernelGGL(ando::k_pcg_update_batch, dim3((int)hv.size()), dim3(kBlock), 0, stream,
+                           (const ando::PcgSys*)sys.p, (const int32_t*)wg_v.p, it);
+      }
+      HIPCHK(hipGetLastError());
+      read_status();
+      bool all = true;
+      for (int k = 0; k < ns; ++k) all = all && st_host[k].done;
+      if (all) break;
+      if (it >= maxit) {
+        for (int k = 0; k < ns; ++k)
+          if (!st_host[k].done)
+            hipLaunchKernelGGL(ando::k_pcg_final, dim3(1), dim3(kBlock), 0, stream, it, h[k].part_max, h[k].nwg_v,
+                               h[k].st, h[k].tol);
+        read_status();
+        break;
+      }
+      chunk = std::min(chunk * 2, chunk_max);
+    }
+    for (int k = 0; k < ns; ++k) jobs[k]->out = PcgOut{st_host[k].iters, !st_host[k].failed};
+    ++rounds;
+    solves += ns;
+  }
+};
+
 // Number of co-resident workgroups for the persistent PCG, a multiple of 8 so
 // that the XCD-contiguous row mapping holds.
 int persistent_grid(ando_ctx* c, int n) {
@@ -887,6 +1074,22 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
                      (const float*)c->part_max.p, (const float*)(P_given ? c->part_bmax.p : nullptr), c->rz_hist.p,
                      c->pcg_status.p, tol);
   HIPCHK(hipGetLastError());
+  if (t_batch && t_batch->device == c->device && sell && !couple && P_given == nullptr && c->kernel_timing == 0 &&
+      c->sell_lanes == 2 && sell_slices == 1) {
+    BatchJob job;
+    job.c = c;
+    job.sm = sm;
+    job.xsol = xsol;
+    job.n = n;
+    job.nwg_v = nwg_v;
+    job.nwg_r = nwg_r;
+    job.maxit = maxit;
+    job.tol = tol;
+    if (!c->batch_ev) HIPCHK(hipEventCreateWithFlags(&c->batch_ev, hipEventDisableTiming));
+    job.ready = c->batch_ev;
+    c->stats.batched_solves++;
+    return t_batch->solve(job);
+  }
   PcgStatus st{};
   int it = 0;
   // The iterations go out in chunks of 8, 16, 32, 64, 128, then 256 iterations;
@@ -1843,18 +2046,39 @@ int ando_step_batch(ando_ctx* const* ctxs, const ando_params* params, ando_rigid
     rc.assign(count, ANDO_OK);
     err.assign(count, std::string());
     int T = threads > 0 ? threads : (int)std::max(1u, std::thread::hardware_concurrency());
-    T = std::max(1, std::min({T, count, 64}));
+    T = std::max(1, std::min({T, count, 256}));
+    // Batched PCG when every context is on one device (ANDO_BATCH_PCG=0 turns it off).
+    std::unique_ptr<BatchPcg> batch;
+    bool same_device = count > 0;
+    for (int k = 1; k < count && same_device; ++k) same_device = ctxs[k]->device == ctxs[0]->device;
+    const char* bp = std::getenv("ANDO_BATCH_PCG");
+    if (same_device && T > 1 && !(bp && std::atoi(bp) == 0)) {
+      batch.reset(new BatchPcg);
+      batch->device = ctxs[0]->device;
+      batch->active = T;
+      batch->chunk0 = std::max(1, ctxs[0]->pcg_chunk);
+      batch->chunk_max = ctxs[0]->pcg_chunk_max;
+      if (const char* w = std::getenv("ANDO_BATCH_GATHER_US")) batch->gather_us = std::max(0, std::atoi(w));
+      HIPCHK(hipSetDevice(batch->device));
+      HIPCHK(hipStreamCreateWithFlags(&batch->stream, hipStreamNonBlocking));
+    }
     std::atomic<int> next{0};
     auto work = [&] {
+      t_batch = batch.get();
       for (int k = next++; k < count; k = next++) {
         rc[k] = ando_step(ctxs[k], &params[k], bodies ? bodies[k] : nullptr, num_bodies ? num_bodies[k] : 0);
         if (rc[k] != ANDO_OK) err[k] = ando_last_error();
       }
+      t_batch = nullptr;
+      if (batch) batch->worker_done();
     };
     std::vector<std::thread> pool;
     for (int t = 1; t < T; ++t) pool.emplace_back(work);
     work();
     for (auto& th : pool) th.join();
+    if (batch && std::getenv("ANDO_BATCH_REPORT"))
+      std::fprintf(stderr, "ando_step_batch: %d contexts on %d threads, %ld solves in %ld batched rounds\n", count, T,
+                   batch->solves, batch->rounds);
   });
   if (status != ANDO_OK) return status;
   for (int k = 0; k < (int)rc.size(); ++k)

@@ -618,8 +618,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_sell(int n, int slices, int
 // wave per workgroup (64 / L rows of a slice); the r.z / max|r| and p.Ap
 // reductions then run over the workgroup's threads, another association than
 // the one-lane kernel's.
-template <int L, int S = 4>
-__global__ void __launch_bounds__(64 * (S == 0 ? 1 : S * L)) k_pcg_spmv_sell_split(int n, int slices, int it,
+template <int L, int S>
+__device__ __forceinline__ void sell_split_body(int g, int n, int slices, int it,
                            const float* __restrict__ vals, const int32_t* __restrict__ scol,
                            const int32_t* __restrict__ slice_off, const int32_t* __restrict__ slice_w, int uniform_w,
                            const float* __restrict__ z, const float* __restrict__ p_old, float* __restrict__ p_new,
@@ -635,7 +635,6 @@ __global__ void __launch_bounds__(64 * (S == 0 ? 1 : S * L)) k_pcg_spmv_sell_spl
   __shared__ double ldsd[WPW];
   __shared__ float ldsf[WPW];
   __shared__ double ldsp[S == 0 ? 1 : S][64];
-  const int g = xcd_logical_group(blockIdx.x, gridDim.x);
   const int tid = threadIdx.x;
   const int wv = tid >> 6, lane = tid & 63;
   const int q = lane / R, rl = lane % R;
@@ -790,6 +789,17 @@ __global__ void __launch_bounds__(64 * (S == 0 ? 1 : S * L)) k_pcg_spmv_sell_spl
   if (tid == 0) part_pAp[g] = sp;
 }
 
+template <int L, int S = 4>
+__global__ void __launch_bounds__(64 * (S == 0 ? 1 : S * L)) k_pcg_spmv_sell_split(int n, int slices, int it,
+                           const float* __restrict__ vals, const int32_t* __restrict__ scol,
+                           const int32_t* __restrict__ slice_off, const int32_t* __restrict__ slice_w, int uniform_w,
+                           const float* __restrict__ z, const float* __restrict__ p_old, float* __restrict__ p_new,
+                           float* __restrict__ Ap, const double* __restrict__ part_rz,
+                           const float* __restrict__ part_max, int nwg_v, double* __restrict__ rz_hist,
+                           double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol) {
+  sell_split_body<L, S>(xcd_logical_group(blockIdx.x, gridDim.x), n, slices, it, vals, scol, slice_off, slice_w, uniform_w, z, p_old, p_new, Ap, part_rz, part_max, nwg_v, rz_hist, part_pAp, st, tol);
+}
+
 // Block-parallel SpMV: one 3x3 block per lane, workgroups over contiguous
 // runs of kBlock blocks, so a wave reads 64 consecutive blocks (2304 bytes)
 // and no lane waits on a row pointer.  Per block, meta packs
@@ -916,7 +926,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_blk(int nb, int it, const f
 // vertex.  The vertex's operands are loaded before the p.Ap reduction.
 // With the block-parallel SpMV, straddle[i] >= 0 names the wave whose
 // Ap_extra entry holds the rest of row i (straddle == nullptr otherwise).
-__global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const float* __restrict__ p,
+__device__ __forceinline__ void update_body(int wg, int n, int it, const float* __restrict__ p,
                              const float* __restrict__ Ap, const float* __restrict__ Ap_extra,
                              const int32_t* __restrict__ straddle, const float* __restrict__ Pinv, int pe,
                              int pi, float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
@@ -925,7 +935,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const floa
                              const double* __restrict__ part_pAp2, int n2) {
   __shared__ double ldsd[4];
   __shared__ float ldsf[4];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = wg * blockDim.x + threadIdx.x;
   const int ii = i < n ? i : n - 1;
   float pv[3], av[3], xv[3], rv0[3], P[9];
   // The p.Ap partials head the longest dependency chain (reduction, alpha,
@@ -988,9 +998,19 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const floa
   float mx = rm;
   block_reduce_sum_max(s, mx, ldsd, ldsf);
   if (threadIdx.x == 0) {
-    part_rz[blockIdx.x] = s;
-    part_max[blockIdx.x] = mx;
+    part_rz[wg] = s;
+    part_max[wg] = mx;
   }
+}
+
+__global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const float* __restrict__ p,
+                             const float* __restrict__ Ap, const float* __restrict__ Ap_extra,
+                             const int32_t* __restrict__ straddle, const float* __restrict__ Pinv, int pe,
+                             int pi, float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
+                             const double* __restrict__ part_pAp, int nwg_r, const double* __restrict__ rz_hist,
+                             double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st,
+                             const double* __restrict__ part_pAp2, int n2) {
+  update_body(blockIdx.x, n, it, p, Ap, Ap_extra, straddle, Pinv, pe, pi, x, r, z, part_pAp, nwg_r, rz_hist, part_rz, part_max, st, part_pAp2, n2);
 }
 
 // ---------------------------------------------------------------------------
@@ -1256,6 +1276,62 @@ __global__ void __launch_bounds__(kBlock) k_pcg_persistent(PcgPersistentArgs a) 
 }
 
 // Convergence test after the last permitted iteration (pcg_solver.cpp:55-74).
+// ---------------------------------------------------------------------------
+// Batched PCG (ando_step_batch): the iterations of many independent systems in
+// one SpMV launch and one update launch.  Workgroup w of a batched grid runs
+// workgroup w - wg0 of system wg_sys[w] with that system's buffers, the same
+// row range and partial slot as in the system's own launch, so every system's
+// results equal those of solving it alone, bit for bit.  All systems of a
+// batch start together, so the iteration index is common; a converged system's
+// workgroups return at its done flag.
+// ---------------------------------------------------------------------------
+struct PcgSys {
+  int n, slices, uniform_w, nwg_v, nwg_r;
+  int wg0_r, wg0_v;  // first workgroup of the system in the SpMV / update grids
+  const float* vals;
+  const int32_t* scol;
+  const int32_t* slice_off;
+  const int32_t* slice_w;
+  const float* Pinv;
+  int pe, pi;
+  float* z;
+  float* p0;
+  float* p1;
+  float* Ap;
+  float* x;
+  float* r;
+  double* part_rz;
+  float* part_max;
+  double* rz_hist;
+  double* part_pAp;
+  PcgStatus* st;
+  float tol;
+};
+
+__global__ void __launch_bounds__(128) k_pcg_spmv_sell_batch(const PcgSys* __restrict__ sys,
+                                                            const int32_t* __restrict__ wg_sys, int it) {
+  const PcgSys& S = sys[wg_sys[blockIdx.x]];
+  float* pold = it == 0 ? S.p0 : (((it - 1) & 1) ? S.p1 : S.p0);
+  float* pcur = (it & 1) ? S.p1 : S.p0;
+  sell_split_body<2, 1>((int)blockIdx.x - S.wg0_r, S.n, S.slices, it, S.vals, S.scol, S.slice_off, S.slice_w,
+                        S.uniform_w, S.z, pold, pcur, S.Ap, S.part_rz, S.part_max, S.nwg_v, S.rz_hist, S.part_pAp,
+                        S.st, S.tol);
+}
+
+__global__ void __launch_bounds__(kBlock) k_pcg_update_batch(const PcgSys* __restrict__ sys,
+                                                            const int32_t* __restrict__ wg_sys, int it) {
+  const PcgSys& S = sys[wg_sys[blockIdx.x]];
+  float* pcur = (it & 1) ? S.p1 : S.p0;
+  update_body((int)blockIdx.x - S.wg0_v, S.n, it, pcur, S.Ap, nullptr, nullptr, S.Pinv, S.pe, S.pi, S.x, S.r, S.z,
+              S.part_pAp, S.nwg_r, S.rz_hist, S.part_rz, S.part_max, S.st, nullptr, 0);
+}
+
+// The systems' status words side by side, for one read-back per chunk.
+__global__ void k_pcg_status_gather(int nsys, const PcgSys* __restrict__ sys, PcgStatus* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < nsys) out[s] = *sys[s].st;
+}
+
 __global__ void k_pcg_final(int it, const float* __restrict__ part_max, int nwg_v, PcgStatus* __restrict__ st,
                             float tol) {
   if (st->done) return;

@@ -165,7 +165,8 @@ def batched_leg(steps: int, count: int = 256):
     """The batched workload (`count` cloths of side U{17..65}, 1-4 pins, wind,
     ground wall; about 481k vertices for 256): cloth-steps/s stepping the
     contexts one after another (the reference's only way) and through
-    ando_step_batch (one host thread per context group, a stream per cloth)."""
+    ando_step_batch (64 host threads, a stream per cloth, the solves of
+    threads that reach PCG together run in batched launches)."""
     from andosim_amd import native, scenes
 
     out = {"workload": f"batch{count}: {count} independent cloths, sides U{{17..65}}", "substeps": steps}
@@ -182,10 +183,14 @@ def batched_leg(steps: int, count: int = 256):
             else:
                 for g in sims:
                     g.apply_gravity()
-                native.step_batch(sims, threads=16)
+                native.step_batch(sims, threads=64)
             t += time.perf_counter() - t0
         out[mode] = {"cloth_steps_per_s": round(count * steps / t, 1),
                      "vertices": int(sum(g.n for g in sims))}
+        if mode == "batched":
+            out[mode]["host_threads"] = 64
+            out[mode]["batched_solve_share"] = round(
+                sum(g.stats()["batched_solves"] for g in sims) / max(1, sum(g.stats()["pcg_solves"] for g in sims)), 3)
     return out
 
 

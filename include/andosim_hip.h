@@ -151,6 +151,7 @@ typedef struct {
   int32_t fallback_iterations;      /* plain PCG iterations spent in fallbacks (included in pcg_iterations) */
   int32_t spmv_lanes;               /* pcg_kernel 3: lanes per row of the sliced-ELL SpMV (1, 2 or 4) */
   int32_t spmv_wg_slices;           /* pcg_kernel 3: slices of 64 rows per SpMV workgroup (0: half a slice) */
+  int32_t batched_solves;           /* solves run inside a batched PCG launch (ando_step_batch) */
 } ando_stats;
 
 int32_t ando_abi_version(void);

@@ -27,6 +27,31 @@ def test_batch_equals_sequential():
         assert np.array_equal(a.get("v"), b.get("v"))
         sa, sb = a.stats(), b.stats()
         assert sa["pcg_iterations"] == sb["pcg_iterations"] and sa["newton_iterations"] == sb["newton_iterations"]
+        # The batched steps ran their solves inside batched PCG launches.
+        assert sa["batched_solves"] == 0 and sb["batched_solves"] == sb["pcg_solves"] > 0
+
+
+def test_batch_mixed_modes_one_thread_per_cloth():
+    """Twenty cloths on twenty threads (large batched rounds), with a
+    corrected-mode context among them whose solves keep their own launches
+    (coupling blocks): every result equals sequential stepping."""
+    print("batch of 20 with a corrected-mode context", flush=True)
+    scs = scenes.batch256(count=19) + [scenes.stack_resting(s=17, sheets=2)]
+    seq = [native.GpuSim(sc) for sc in scs]
+    bat = [native.GpuSim(sc) for sc in scs]
+    for g in (seq[-1], bat[-1]):
+        g.set_mode(corrected=True)
+    for _ in range(3):
+        for g in seq:
+            g.substep()
+        for g in bat:
+            g.apply_gravity()
+        native.step_batch(bat, threads=len(bat))
+    for a, b in zip(seq, bat):
+        assert np.array_equal(a.get("x"), b.get("x"))
+        assert np.array_equal(a.get("v"), b.get("v"))
+    assert bat[0].stats()["batched_solves"] > 0
+    assert bat[-1].stats()["batched_solves"] == 0
 
 
 def test_batch_with_rigid_bodies_and_errors():
