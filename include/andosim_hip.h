@@ -232,7 +232,9 @@ int ando_get_stats(ando_ctx* ctx, ando_stats* out);
 /* Batched stepping (new; the reference has no batched API and steps
  * independent cloths one after another): ando_step on `count` distinct
  * contexts, each on its own stream, from up to `threads` host threads
- * (0 = the host's hardware concurrency), so small cloths overlap on the GPU.
+ * (0 = the host's hardware concurrency; at most 256), so small cloths overlap
+ * on the GPU; when all contexts are on one device, the PCG solves that the
+ * threads reach together run as one batched launch pair per iteration.
  * params[k], bodies[k] / num_bodies[k] (both may be NULL: no bodies) belong to
  * ctxs[k].  Each context's result equals a plain ando_step.  Returns the first
  * failing entry's code, with its index in the message; the others still run. */
