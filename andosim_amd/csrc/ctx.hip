@@ -785,6 +785,9 @@ struct BatchJob {
   float* xsol = nullptr;
   int n = 0, nwg_v = 0, nwg_r = 0, maxit = 0;
   float tol = 0.0f;
+  const float* A = nullptr;
+  const int32_t* diag = nullptr;
+  const float* rhs = nullptr;
   hipEvent_t ready = nullptr;
   PcgOut out{0, false};
   bool done = false;
@@ -804,7 +807,7 @@ struct BatchPcg {
   int device = 0;
   hipStream_t stream = nullptr;
   DBuf<ando::PcgSys> sys;
-  DBuf<int32_t> wg_r, wg_v;
+  DBuf<int32_t> wg_r, wg_v, wg_f;
   DBuf<PcgStatus> st_all;
   PcgStatus* st_host = nullptr;  // pinned
   int st_cap = 0;
@@ -869,7 +872,7 @@ struct BatchPcg {
     HIPCHK(hipSetDevice(device));
     const int ns = (int)jobs.size();
     std::vector<ando::PcgSys> h(ns);
-    std::vector<int32_t> hr, hv;
+    std::vector<int32_t> hr, hv, hf;
     for (int k = 0; k < ns; ++k) {
       BatchJob& j = *jobs[k];
       ando_ctx* c = j.c;
@@ -882,6 +885,11 @@ struct BatchPcg {
       S.nwg_r = j.nwg_r;
       S.wg0_r = (int)hr.size();
       S.wg0_v = (int)hv.size();
+      S.wg0_f = (int)hf.size();
+      S.A = j.A;
+      S.diag = j.diag;
+      S.rhs = j.rhs;
+      S.perm = j.sm->sell_perm.p;
       S.vals = j.sm->sell_vals.p;
       S.scol = j.sm->sell_col.p;
       S.slice_off = j.sm->slice_off.p;
@@ -903,10 +911,12 @@ struct BatchPcg {
       S.tol = j.tol;
       hr.insert(hr.end(), j.nwg_r, k);
       hv.insert(hv.end(), j.nwg_v, k);
+      hf.insert(hf.end(), nblk(j.sm->slices, kBlock / 64), k);
     }
     sys.upload(h, stream);
     wg_r.upload(hr, stream);
     wg_v.upload(hv, stream);
+    wg_f.upload(hf, stream);
     st_all.ensure((size_t)ns);
     if (st_cap < ns) {
       if (st_host) HIPCHK(hipHostFree(st_host));
@@ -917,6 +927,12 @@ struct BatchPcg {
     // The tables are read by the kernels below; the uploads finish first.
     HIPCHK(hipStreamSynchronize(stream));
     const int maxit = jobs[0]->maxit;
+    // Set-up of all systems (what run_pcg does per system on its own stream).
+    hipLaunchKernelGGL(ando::k_sell_fill_batch, dim3((int)hf.size()), dim3(kBlock), 0, stream,
+                       (const ando::PcgSys*)sys.p, (const int32_t*)wg_f.p);
+    hipLaunchKernelGGL(ando::k_pcg_init_batch, dim3((int)hv.size()), dim3(kBlock), 0, stream,
+                       (const ando::PcgSys*)sys.p, (const int32_t*)wg_v.p);
+    hipLaunchKernelGGL(ando::k_pcg_start_batch, dim3(ns), dim3(kBlock), 0, stream, (const ando::PcgSys*)sys.p);
     auto read_status = [&] {
       hipLaunchKernelGGL(ando::k_pcg_status_gather, dim3((ns + 63) / 64), dim3(64), 0, stream, ns,
                          (const ando::PcgSys*)sys.p, st_all.p);
@@ -1053,6 +1069,28 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
   c->rz_hist.ensure((size_t)maxit + 2);
   c->pcg_status.ensure(1);
   c->part_pAp.ensure((size_t)nwg_r + 1);
+  // Batched solves (ando_step_batch) are set up by the batch: the Newton
+  // system (b = -g) on the default sliced-ELL grid, without coupling blocks,
+  // learned preconditioner or kernel timing.
+  if (t_batch && t_batch->device == c->device && sell && !couple && P_given == nullptr && x0_given == nullptr &&
+      negate == 1 && c->kernel_timing == 0 && c->sell_lanes == 2 && sell_slices == 1 && vb == kBlock) {
+    BatchJob job;
+    job.c = c;
+    job.sm = sm;
+    job.xsol = xsol;
+    job.n = n;
+    job.nwg_v = nwg_v;
+    job.nwg_r = nwg_r;
+    job.maxit = maxit;
+    job.tol = tol;
+    job.A = A;
+    job.diag = diag;
+    job.rhs = rhs;
+    if (!c->batch_ev) HIPCHK(hipEventCreateWithFlags(&c->batch_ev, hipEventDisableTiming));
+    job.ready = c->batch_ev;
+    c->stats.batched_solves++;
+    return t_batch->solve(job);
+  }
   const float* Pc = c->Pinv.p;
   int pe = n, pi = 1;  // block-Jacobi blocks as one plane per entry
   if (sell)
@@ -1074,22 +1112,6 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
                      (const float*)c->part_max.p, (const float*)(P_given ? c->part_bmax.p : nullptr), c->rz_hist.p,
                      c->pcg_status.p, tol);
   HIPCHK(hipGetLastError());
-  if (t_batch && t_batch->device == c->device && sell && !couple && P_given == nullptr && c->kernel_timing == 0 &&
-      c->sell_lanes == 2 && sell_slices == 1) {
-    BatchJob job;
-    job.c = c;
-    job.sm = sm;
-    job.xsol = xsol;
-    job.n = n;
-    job.nwg_v = nwg_v;
-    job.nwg_r = nwg_r;
-    job.maxit = maxit;
-    job.tol = tol;
-    if (!c->batch_ev) HIPCHK(hipEventCreateWithFlags(&c->batch_ev, hipEventDisableTiming));
-    job.ready = c->batch_ev;
-    c->stats.batched_solves++;
-    return t_batch->solve(job);
-  }
   PcgStatus st{};
   int it = 0;
   // The iterations go out in chunks of 8, 16, 32, 64, 128, then 256 iterations;

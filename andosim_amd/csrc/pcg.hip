@@ -164,11 +164,11 @@ __device__ inline float cof3(const float* B, int i, int j) {
 // Entry e of vertex i's preconditioner block sits at Pinv[e * pe + i * pi]
 // (pe = 1, pi = 9: block per vertex; pe = n, pi = 1: one plane per entry, which
 // the update kernel reads with coalesced loads).
-__global__ void k_pcg_init(int n, const float* __restrict__ A, const int32_t* __restrict__ diag,
+__device__ __forceinline__ void pcg_init_body(int wg, int n, const float* __restrict__ A, const int32_t* __restrict__ diag,
                            const float* __restrict__ g, float* __restrict__ x, float* __restrict__ r,
                            float* __restrict__ z, float* __restrict__ p, float* __restrict__ Pinv, int pe, int pi,
                            double* __restrict__ part_rz, float* __restrict__ part_max, int negate) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = wg * blockDim.x + threadIdx.x;
   double rz = 0.0;
   float bm = 0.0f;
   if (i < n) {
@@ -205,9 +205,16 @@ __global__ void k_pcg_init(int n, const float* __restrict__ A, const int32_t* __
   double s = block_reduce(rz, ldsd, [](double a, double b) { return a + b; });
   float mx = block_reduce(bm, ldsf, [](float a, float b) { return fmaxf(a, b); });
   if (threadIdx.x == 0) {
-    part_rz[blockIdx.x] = s;
-    part_max[blockIdx.x] = mx;
+    part_rz[wg] = s;
+    part_max[wg] = mx;
   }
+}
+
+__global__ void k_pcg_init(int n, const float* __restrict__ A, const int32_t* __restrict__ diag,
+                           const float* __restrict__ g, float* __restrict__ x, float* __restrict__ r,
+                           float* __restrict__ z, float* __restrict__ p, float* __restrict__ Pinv, int pe, int pi,
+                           double* __restrict__ part_rz, float* __restrict__ part_max, int negate) {
+  pcg_init_body(blockIdx.x, n, A, diag, g, x, r, z, p, Pinv, pe, pi, part_rz, part_max, negate);
 }
 
 // With a starting guess, part_bmax holds the max|b| partials and part_max the
@@ -263,9 +270,9 @@ __global__ void k_pcg_init_given(int n, const float* __restrict__ A, const int32
   }
 }
 
-__global__ void k_pcg_start(int nwg, const double* __restrict__ part_rz, const float* __restrict__ part_max,
-                            const float* __restrict__ part_bmax, double* __restrict__ rz_hist,
-                            PcgStatus* __restrict__ st, float tol) {
+__device__ __forceinline__ void pcg_start_body(int nwg, const double* __restrict__ part_rz,
+                                               const float* __restrict__ part_max, const float* __restrict__ part_bmax,
+                                               double* __restrict__ rz_hist, PcgStatus* __restrict__ st, float tol) {
   __shared__ double ldsd[4];
   __shared__ float ldsf[4];
   double rz = reduce_sum_partials(part_rz, nwg, ldsd);
@@ -280,6 +287,11 @@ __global__ void k_pcg_start(int nwg, const double* __restrict__ part_rz, const f
     st->iters = 0;
     st->done = rel < tol ? 1 : 0;
   }
+}
+__global__ void k_pcg_start(int nwg, const double* __restrict__ part_rz, const float* __restrict__ part_max,
+                            const float* __restrict__ part_bmax, double* __restrict__ rz_hist,
+                            PcgStatus* __restrict__ st, float tol) {
+  pcg_start_body(nwg, part_rz, part_max, part_bmax, rz_hist, st, tol);
 }
 
 // Workgroup numbering that gives each XCD a contiguous run of logical
@@ -433,11 +445,11 @@ constexpr int kSellChunk = 8;  // = the 8 lanes per row of k_pcg_spmv (summation
 
 // Copies BSR values into the sliced layout (perm: BSR block of each slot
 // position, -1 for padding); one wave per slice.
-__global__ void __launch_bounds__(kBlock) k_sell_fill(int slices, const int32_t* __restrict__ slice_off,
+__device__ __forceinline__ void sell_fill_body(int wg, int slices, const int32_t* __restrict__ slice_off,
                                                      const int32_t* __restrict__ slice_w,
                                                      const int32_t* __restrict__ perm, const float* __restrict__ A,
                                                      float* __restrict__ vals) {
-  const int sl = blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6);
+  const int sl = wg * (kBlock / 64) + (int)(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (sl >= slices) return;
   const int off = slice_off[sl], W = slice_w[sl];
@@ -447,6 +459,13 @@ __global__ void __launch_bounds__(kBlock) k_sell_fill(int slices, const int32_t*
 #pragma unroll
     for (int e = 0; e < 9; ++e) dst[64 * e] = b >= 0 ? A[9 * (size_t)b + e] : 0.0f;
   }
+}
+
+__global__ void __launch_bounds__(kBlock) k_sell_fill(int slices, const int32_t* __restrict__ slice_off,
+                                                     const int32_t* __restrict__ slice_w,
+                                                     const int32_t* __restrict__ perm, const float* __restrict__ A,
+                                                     float* __restrict__ vals) {
+  sell_fill_body(blockIdx.x, slices, slice_off, slice_w, perm, A, vals);
 }
 
 // Same contract as k_pcg_spmv (convergence test, beta, p = z + beta p on the
@@ -1287,12 +1306,16 @@ __global__ void __launch_bounds__(kBlock) k_pcg_persistent(PcgPersistentArgs a) 
 // ---------------------------------------------------------------------------
 struct PcgSys {
   int n, slices, uniform_w, nwg_v, nwg_r;
-  int wg0_r, wg0_v;  // first workgroup of the system in the SpMV / update grids
-  const float* vals;
+  int wg0_r, wg0_v, wg0_f;  // first workgroup of the system in the SpMV, vector and fill grids
+  const float* A;           // the system's BSR values
+  const int32_t* diag;      // BSR index of each diagonal block
+  const float* rhs;         // b = -rhs
+  const int32_t* perm;      // BSR block of each sliced slot
+  float* vals;
   const int32_t* scol;
   const int32_t* slice_off;
   const int32_t* slice_w;
-  const float* Pinv;
+  float* Pinv;
   int pe, pi;
   float* z;
   float* p0;
@@ -1324,6 +1347,25 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update_batch(const PcgSys* __res
   float* pcur = (it & 1) ? S.p1 : S.p0;
   update_body((int)blockIdx.x - S.wg0_v, S.n, it, pcur, S.Ap, nullptr, nullptr, S.Pinv, S.pe, S.pi, S.x, S.r, S.z,
               S.part_pAp, S.nwg_r, S.rz_hist, S.part_rz, S.part_max, S.st, nullptr, 0);
+}
+
+// Set-up of every system of a batch: sliced-ELL values, then x0 = 0, r = b,
+// z = p = P^-1 r with the block-Jacobi inverse and the partials, then the
+// initial residual test (one workgroup per system).
+__global__ void __launch_bounds__(kBlock) k_sell_fill_batch(const PcgSys* __restrict__ sys,
+                                                           const int32_t* __restrict__ wg_sys) {
+  const PcgSys& S = sys[wg_sys[blockIdx.x]];
+  sell_fill_body((int)blockIdx.x - S.wg0_f, S.slices, S.slice_off, S.slice_w, S.perm, S.A, S.vals);
+}
+__global__ void __launch_bounds__(kBlock) k_pcg_init_batch(const PcgSys* __restrict__ sys,
+                                                          const int32_t* __restrict__ wg_sys) {
+  const PcgSys& S = sys[wg_sys[blockIdx.x]];
+  pcg_init_body((int)blockIdx.x - S.wg0_v, S.n, S.A, S.diag, S.rhs, S.x, S.r, S.z, S.p0, S.Pinv, S.pe, S.pi,
+                S.part_rz, S.part_max, 1);
+}
+__global__ void __launch_bounds__(kBlock) k_pcg_start_batch(const PcgSys* __restrict__ sys) {
+  const PcgSys& S = sys[blockIdx.x];
+  pcg_start_body(S.nwg_v, S.part_rz, S.part_max, nullptr, S.rz_hist, S.st, S.tol);
 }
 
 // The systems' status words side by side, for one read-back per chunk.
