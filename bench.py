@@ -9,8 +9,8 @@ Integrator.step, with the state resident in HBM.
 
 Timed window.  The faithful FP32 solver diverges a few steps after first
 contact on this workload (the strain barrier's m/gap^2 stiffness makes PCG
-stall at its 1000-iteration cap; the FP32 oracle diverges the same way, see
-DESIGN.md).  A run past that point measures NaN arithmetic, so the bench
+stall at its 1000-iteration cap; the FP32 oracle diverges the same way).  A
+run past that point measures NaN arithmetic, so the bench
 first replays the drop untimed to find the window [first contact, last finite
 step], keeps the device state at the window start, and then times --steps
 substeps by replaying that window, restoring the saved state (untimed)
@@ -288,8 +288,8 @@ def main(argv=None):
     ap.add_argument("--batched-steps", type=int, default=5)
     a = ap.parse_args(argv)
     if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); the multi-GPU design is described "
-              "in DESIGN.md and is not built.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); sharding the solve over several GPUs "
+              "is not built.", file=sys.stderr)
         return 2
 
     from andosim_amd import native, scenes
