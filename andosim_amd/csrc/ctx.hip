@@ -198,7 +198,7 @@ void build_spmv_meta(SpmvMeta& sm, int n, const std::vector<int32_t>& row_ptr, c
 // One linear BVH (triangles or edges) and its build scratch.
 struct LbvhTree {
   int P = 0;
-  DBuf<float> boxes, table, nodes, bpart, bounds;
+  DBuf<float> boxes, table, nodes, cbox, bpart, bounds;
   DBuf<uint32_t> keys, keys2;
   DBuf<int32_t> vals, vals2, counts, offsets, left, right, first, last, scan_sums, scan_total;
 };
@@ -638,6 +638,9 @@ void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims)
                        T.right.p, T.first.p, T.last.p);
     hipLaunchKernelGGL(k_node_boxes, dim3(nblk(P - 1)), dim3(256), 0, s, P, (const int32_t*)T.first.p,
                        (const int32_t*)T.last.p, (const float*)T.table.p, T.nodes.p);
+    T.cbox.ensure(12 * (size_t)(P - 1));
+    hipLaunchKernelGGL(k_child_boxes, dim3(nblk(P - 1)), dim3(256), 0, s, P, (const int32_t*)T.left.p,
+                       (const int32_t*)T.right.p, (const float*)T.nodes.p, (const float*)T.table.p, T.cbox.p);
   }
   HIPCHK(hipGetLastError());
 }
@@ -673,14 +676,14 @@ void detect_self(ando_ctx* c) {
   auto pt = [&](int pass, ando::SelfContact* out, int32_t* key) {
     hipLaunchKernelGGL(k_query_pt, dim3(nblk(n)), dim3(kBlock), 0, s, pass, n, (const float*)c->x.p, c->pt_box,
                        c->keep_gap, (const int32_t*)c->d_tris.p, T.P, (const int32_t*)T.left.p,
-                       (const int32_t*)T.right.p, (const float*)T.nodes.p, (const float*)T.table.p,
+                       (const int32_t*)T.right.p, (const float*)T.cbox.p, (const float*)T.table.p,
                        (const int32_t*)T.vals.p, c->sc_count.p, (const int32_t*)c->sc_ptr.p, out, key, c->sc_err.p,
                        c->sc_cache.p);
   };
   auto ee = [&](int pass, ando::SelfContact* out, int32_t* key) {
     hipLaunchKernelGGL(k_query_ee, dim3(nblk(E)), dim3(kBlock), 0, s, pass, E, (const float*)c->x.p, c->keep_gap,
                        (const int32_t*)c->d_edges.p, Q.P, (const int32_t*)Q.left.p, (const int32_t*)Q.right.p,
-                       (const float*)Q.nodes.p, (const float*)Q.table.p, (const int32_t*)Q.vals.p, c->sc_count.p,
+                       (const float*)Q.cbox.p, (const float*)Q.table.p, (const int32_t*)Q.vals.p, c->sc_count.p,
                        (const int32_t*)c->sc_ptr.p, out, key, c->sc_err.p, c->sc_cache.p);
   };
   int npt = 0, nee = 0;
@@ -2032,7 +2035,7 @@ int ando_count_intersections(ando_ctx* c, const ando_rigid_body* bodies, int32_t
       const LbvhTree& T = c->tri_tree;
       hipLaunchKernelGGL(ando::lbvh::k_count_edge_tri, dim3(nblk(E)), dim3(kBlock), 0, s, E, (const float*)c->x.p,
                          (const int32_t*)c->d_edges.p, (const int32_t*)c->d_tris.p, T.P, (const int32_t*)T.left.p,
-                         (const int32_t*)T.right.p, (const float*)T.nodes.p, (const float*)T.table.p,
+                         (const int32_t*)T.right.p, (const float*)T.cbox.p, (const float*)T.table.p,
                          (const int32_t*)T.vals.p, c->sc_count.p, c->sc_err.p);
       et = exclusive_scan(c, E, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
       uint32_t err = 0;

@@ -271,6 +271,19 @@ __global__ void k_node_boxes(int P, const int32_t* __restrict__ first, const int
   store_box(nodes, i, unite(load_box(Tk, a), load_box(Tk, b - (1 << k) + 1)));
 }
 
+// Boxes of both children of every internal node side by side (12 floats per
+// node: left, right; a child that is a leaf takes its primitive's box), so a
+// traversal tests both children of a node with one contiguous read.
+__global__ void k_child_boxes(int P, const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+                              const float* __restrict__ nodes, const float* __restrict__ leaves,
+                              float* __restrict__ cbox) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P - 1) return;
+  const int l = left[i], r = right[i];
+  store_box(cbox, 2 * i, l < 0 ? load_box(leaves, ~l) : load_box(nodes, l));
+  store_box(cbox, 2 * i + 1, r < 0 ? load_box(leaves, ~r) : load_box(nodes, r));
+}
+
 // Triangle barycentrics of p (collision.cpp:11-43): clamped, renormalised.
 __device__ inline f3 triangle_bary(f3 p, f3 a, f3 b, f3 c) {
   const f3 v0 = b - a, v1 = c - a, v2 = p - a;
@@ -339,26 +352,29 @@ __device__ inline void edge_edge(f3 p0, f3 p1, f3 q0, f3 q1, float& dist, f3& no
 // overlapping leaf.  Returns false if the stack would overflow (the caller
 // reports an error; nothing is written out of bounds).
 template <class F>
-__device__ inline bool traverse(int P, const Box& q, const int32_t* left, const int32_t* right, const float* nodes,
+__device__ inline bool traverse(int P, const Box& q, const int32_t* left, const int32_t* right, const float* cbox,
                                 const float* leaves, F&& visit) {
   if (P == 1) {
     if (overlaps(q, load_box(leaves, 0))) visit(0);
     return true;
   }
+  // A popped node's two children are tested together (cbox: k_child_boxes):
+  // overlapping leaves are visited at once, overlapping internal children are
+  // pushed (right first, so the left subtree is entered first).  Callers sort
+  // each query's hits, so the visit order does not reach any result.
   int stack[kStack];
   int sp = 0;
   stack[sp++] = 0;
   while (sp > 0) {
     const int node = stack[--sp];
-    if (node < 0) {
-      const int s = ~node;
-      if (overlaps(q, load_box(leaves, s))) visit(s);
-      continue;
-    }
-    if (!overlaps(q, load_box(nodes, node))) continue;
+    const int l = left[node], r = right[node];
+    const bool ol = overlaps(q, load_box(cbox, 2 * node));
+    const bool orr = overlaps(q, load_box(cbox, 2 * node + 1));
+    if (ol && l < 0) visit(~l);
+    if (orr && r < 0) visit(~r);
     if (sp + 2 > kStack) return false;
-    stack[sp++] = right[node];
-    stack[sp++] = left[node];
+    if (orr && r >= 0) stack[sp++] = r;
+    if (ol && l >= 0) stack[sp++] = l;
   }
   return true;
 }

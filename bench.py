@@ -194,6 +194,42 @@ def batched_leg(steps: int, count: int = 256):
     return out
 
 
+def fp64_accuracy(sim, sc, saved, restore, window, orc, threads, steps: int = 3):
+    """Accuracy of the measured result: the first `steps` substeps of the timed
+    window from its start state on the GPU (FP32 with its FP64 islands), in the
+    FP32 oracle and in the FP64 oracle; after each substep, max |x - x64| /
+    max |x64| for the GPU and for the FP32 oracle against the FP64 oracle."""
+    k = min(steps, len(window))
+    restore()
+    gpu = []
+    for _ in range(k):
+        sim.substep()
+        gpu.append(sim.get("x").astype(np.float64))
+    orc.set_threads(threads)
+
+    def run(bits):
+        o = orc.OracleSim(sc, bits)
+        x0, v0, rig = saved
+        o.set("x", x0.astype(np.float64))
+        o.set("v", v0.astype(np.float64))
+        for b, rs in enumerate(rig):
+            o.set_rigid_pose(b, rs)
+        xs = []
+        for _ in range(k):
+            o.substep()
+            xs.append(o.get("x").astype(np.float64))
+        return xs
+
+    o32, o64 = run(32), run(64)
+
+    def rel(a, b):
+        return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-30))
+
+    return {"reference": "oracle in FP64 on the host, same start state", "substeps": k,
+            "gpu_max_rel_err_per_step": [float(f"{rel(g, r):.3e}") for g, r in zip(gpu, o64)],
+            "fp32_oracle_max_rel_err_per_step": [float(f"{rel(c, r):.3e}") for c, r in zip(o32, o64)]}
+
+
 def learned_comparison(sim, sc, window, restore, replay, a, pers):
     """Learned preconditioner vs plain block-Jacobi PCG on the same window.
 
@@ -421,7 +457,7 @@ def main(argv=None):
     if not a.no_learned:
         learned = learned_comparison(sim, sc, window, restore, replay, a, pers)
 
-    cpu = cpu_single = None
+    cpu = cpu_single = accuracy = None
     if not a.no_cpu:
         from oracle import oracle as orc
 
@@ -461,6 +497,7 @@ def main(argv=None):
         cpu_single = {"value": round(k1 / cs1, 4), "unit": "steps/s", "cores": 1, "kind": "port",
                       "sample": f"first {k1} substeps of the timed window, oracle FP32 scan-free, 1 thread "
                                 f"pinned to one core, {cs1:.1f} s"}
+        accuracy = fp64_accuracy(sim, sc, saved, restore, window, orc, threads)
 
     out = {
         "metric": METRIC, "value": round(steps_per_s, 3), "unit": "steps/s", "n_gpus": 1, "steps": len(recs),
@@ -482,6 +519,7 @@ def main(argv=None):
         "per_frame_consumers": frame,
         "batched": batched,
         "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu_single,
+        "accuracy_vs_fp64": accuracy,
     }
     print(json.dumps(out), flush=True)
     return 0
