@@ -674,14 +674,16 @@ void detect_self(ando_ctx* c) {
   const LbvhTree& T = c->tri_tree;
   const LbvhTree& Q = c->edge_tree;
   auto pt = [&](int pass, ando::SelfContact* out, int32_t* key) {
-    hipLaunchKernelGGL(k_query_pt, dim3(nblk(n)), dim3(kBlock), 0, s, pass, n, (const float*)c->x.p, c->pt_box,
+    // Traversals: one thread per query, 64-thread workgroups so that even
+    // ~30k queries spread over every CU.
+    hipLaunchKernelGGL(k_query_pt, dim3(nblk(n, 64)), dim3(64), 0, s, pass, n, (const float*)c->x.p, c->pt_box,
                        c->keep_gap, (const int32_t*)c->d_tris.p, T.P, (const int32_t*)T.left.p,
                        (const int32_t*)T.right.p, (const float*)T.cbox.p, (const float*)T.table.p,
                        (const int32_t*)T.vals.p, c->sc_count.p, (const int32_t*)c->sc_ptr.p, out, key, c->sc_err.p,
                        c->sc_cache.p);
   };
   auto ee = [&](int pass, ando::SelfContact* out, int32_t* key) {
-    hipLaunchKernelGGL(k_query_ee, dim3(nblk(E)), dim3(kBlock), 0, s, pass, E, (const float*)c->x.p, c->keep_gap,
+    hipLaunchKernelGGL(k_query_ee, dim3(nblk(E, 64)), dim3(64), 0, s, pass, E, (const float*)c->x.p, c->keep_gap,
                        (const int32_t*)c->d_edges.p, Q.P, (const int32_t*)Q.left.p, (const int32_t*)Q.right.p,
                        (const float*)Q.cbox.p, (const float*)Q.table.p, (const int32_t*)Q.vals.p, c->sc_count.p,
                        (const int32_t*)c->sc_ptr.p, out, key, c->sc_err.p, c->sc_cache.p);
@@ -2033,7 +2035,7 @@ int ando_count_intersections(ando_ctx* c, const ando_rigid_body* bodies, int32_t
       c->sc_count.ensure((size_t)std::max(c->n, E) + 1);
       c->sc_ptr.ensure((size_t)std::max(c->n, E) + 2);
       const LbvhTree& T = c->tri_tree;
-      hipLaunchKernelGGL(ando::lbvh::k_count_edge_tri, dim3(nblk(E)), dim3(kBlock), 0, s, E, (const float*)c->x.p,
+      hipLaunchKernelGGL(ando::lbvh::k_count_edge_tri, dim3(nblk(E, 64)), dim3(64), 0, s, E, (const float*)c->x.p,
                          (const int32_t*)c->d_edges.p, (const int32_t*)c->d_tris.p, T.P, (const int32_t*)T.left.p,
                          (const int32_t*)T.right.p, (const float*)T.cbox.p, (const float*)T.table.p,
                          (const int32_t*)T.vals.p, c->sc_count.p, c->sc_err.p);

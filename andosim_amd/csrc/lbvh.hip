@@ -401,7 +401,7 @@ __device__ inline void sort_hits(SelfContact* out, int32_t* key, int base, int k
 // Hits a query keeps from its count pass for its fill pass.
 constexpr int kQueryCache = 16;
 
-__global__ void k_query_pt(int pass, int n, const float* __restrict__ x, float pt_box, float keep,
+__global__ void __launch_bounds__(64) k_query_pt(int pass, int n, const float* __restrict__ x, float pt_box, float keep,
                            const int32_t* __restrict__ tris, int P, const int32_t* __restrict__ left,
                            const int32_t* __restrict__ right, const float* __restrict__ nodes,
                            const float* __restrict__ leaves, const int32_t* __restrict__ order,
@@ -458,7 +458,7 @@ __global__ void k_query_pt(int pass, int n, const float* __restrict__ x, float p
 }
 
 // EE query for edge e1 against edges e2 > e1, same two passes; hits sorted by e2.
-__global__ void k_query_ee(int pass, int E, const float* __restrict__ x, float keep, const int32_t* __restrict__ edges,
+__global__ void __launch_bounds__(64) k_query_ee(int pass, int E, const float* __restrict__ x, float keep, const int32_t* __restrict__ edges,
                            int P, const int32_t* __restrict__ left, const int32_t* __restrict__ right,
                            const float* __restrict__ nodes, const float* __restrict__ leaves,
                            const int32_t* __restrict__ order, int32_t* __restrict__ count,
@@ -881,7 +881,7 @@ __device__ inline bool segment_crosses_triangle(f3 p, f3 q, f3 a, f3 b, f3 c) {
 
 // Count of (mesh edge, triangle) pairs sharing no vertex where the edge
 // crosses the triangle; one thread per edge over the triangle BVH.
-__global__ void k_count_edge_tri(int E, const float* __restrict__ x, const int32_t* __restrict__ edges,
+__global__ void __launch_bounds__(64) k_count_edge_tri(int E, const float* __restrict__ x, const int32_t* __restrict__ edges,
                                  const int32_t* __restrict__ tris, int P, const int32_t* __restrict__ left,
                                  const int32_t* __restrict__ right, const float* __restrict__ nodes,
                                  const float* __restrict__ leaves, const int32_t* __restrict__ order,
