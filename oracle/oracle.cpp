@@ -22,11 +22,15 @@
 //     accumulators, then (l0+l2)+(l1+l3) for float and l0+l1 for double;
 //   * sparse row-major SpMV sums each row left to right in column order;
 //   * setFromTriplets sums duplicates in triplet-list order.
-// Bit-level agreement with the real reference build is therefore unpinned
-// wherever Eigen chooses the order.  The SVD (JacobiSVD), the general 3x3
-// eigen-solver and AngleAxis are restated with different algorithms (see
-// svd32 and spd_general below) and agree with Eigen only to rounding.
-// Checks that pin this file are in tests/test_oracle_*.py.
+// Parity unpinned: bit-level agreement with the real reference build wherever
+// Eigen chooses the order.  The SVD (JacobiSVD), the general 3x3 eigen-solver
+// and AngleAxis are restated with different algorithms (see svd32 and
+// spd_general below) and agree with Eigen only to rounding; they are pinned by
+// identities (U S V^T = F, ordering, derivatives by finite differences), not
+// bit for bit.  Everything else is pinned by the checks in tests/test_oracle.py
+// (the reference's known answers in tests/golden/, closed forms, brute force,
+// finite differences, invariants); the learned-preconditioner oracle
+// (oracle/learned.py) by tests/test_learned_oracle.py.
 
 #include <algorithm>
 #include <array>
