@@ -268,6 +268,10 @@ struct ando_ctx {
   int sell_lanes = 2;      // ANDO_SELL_LANES: lanes per row of the sliced-ELL SpMV (1, 2 or 4)
   int sell_wg_slices = 1;  // ANDO_SELL_WG_SLICES: slices per workgroup with two lanes per row (0: a wave, 1, 2, 4)
   int pcg_vec_block = kBlock;  // ANDO_PCG_VEC_BLOCK: threads per workgroup of the PCG vector kernels (64, 128, 256)
+  int pcg_autotune = 0;    // ANDO_PCG_AUTOTUNE: 1 pick the sliced-ELL launch shape by timing, 2 also print the table
+  int tuned_n = -1;        // block rows the launch shape was tuned (or looked up) for
+  int autotuned = 0;       // the launch shape in use came from the autotuner
+  bool tuning = false;     // inside a trial solve of the autotuner
   int spmv_env = -1;       // ANDO_PCG_SPMV: -1 unset, 0 row, 1 blk, 2 sell (overrides the mode flag)
   SpmvMeta spmv_meta;      // for the mesh's block pattern
   std::mutex call_mutex;   // held by every entry point for the whole call (CtxLock)
@@ -1041,6 +1045,9 @@ PcgOut run_pcg_persistent(ando_ctx* c, int n, const float* A, const int32_t* rp,
   return PcgOut{st.iters, !st.failed};
 }
 
+void autotune_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int32_t* cols, const int32_t* diag,
+                  const float* rhs, int negate, float* xsol, int maxit, SpmvMeta* sm, const ando::SelfView* sv);
+
 // P_given / x0_given (device, both or neither): a preconditioner of 9 floats
 // per vertex and a starting guess instead of block-Jacobi from x0 = 0.
 // sv (nullable): self-contact blocks that couple two vertices, applied after
@@ -1061,6 +1068,9 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
   const int R = c->spmv_rowsets;
   const bool sell = c->spmv_sell && sm != nullptr && sm->sell_ok;
   const bool blk = !sell && c->spmv_blocks && sm != nullptr && sm->ok && !couple;
+  if (c->pcg_autotune && !c->tuning && sell && c->tuned_n != n && P_given == nullptr && x0_given == nullptr &&
+      !(t_batch && t_batch->device == c->device))
+    autotune_pcg(c, n, A, rp, cols, diag, rhs, negate, xsol, maxit, sm, sv);
   const int sell_slices = (c->sell_lanes == 2) ? c->sell_wg_slices : 4;  // slices per SpMV workgroup (0: a wave)
   const int vb = c->pcg_vec_block;  // threads per workgroup of the vector kernels (init, update)
   int nwg_v = nblk(n, vb);
@@ -1071,21 +1081,36 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
   c->stats.pcg_kernel = sell ? 3 : (blk ? 2 : 0);
   c->stats.spmv_lanes = sell ? c->sell_lanes : 0;
   c->stats.spmv_wg_slices = sell ? sell_slices : 0;
+  c->stats.pcg_vec_block = vb;
+  c->stats.pcg_autotuned = sell ? c->autotuned : 0;
   c->rz_hist.ensure((size_t)maxit + 2);
   c->pcg_status.ensure(1);
   c->part_pAp.ensure((size_t)nwg_r + 1);
   // Batched solves (ando_step_batch) are set up by the batch: the Newton
   // system (b = -g) on the default sliced-ELL grid, without coupling blocks,
-  // learned preconditioner or kernel timing.
+  // learned preconditioner or kernel timing.  A shape chosen through the
+  // environment keeps the solve out of the batch; one chosen by the autotuner
+  // gives way to the batch's grid.
+  const bool default_shape = c->sell_lanes == 2 && sell_slices == 1 && vb == kBlock;
   if (t_batch && t_batch->device == c->device && sell && !couple && P_given == nullptr && x0_given == nullptr &&
-      negate == 1 && c->kernel_timing == 0 && c->sell_lanes == 2 && sell_slices == 1 && vb == kBlock) {
+      negate == 1 && c->kernel_timing == 0 && (default_shape || c->autotuned)) {
+    c->stats.spmv_lanes = 2;
+    c->stats.spmv_wg_slices = 1;
+    c->stats.pcg_vec_block = kBlock;
+    c->stats.pcg_autotuned = 0;
+    // The batch's grid: one slice per SpMV workgroup, 256-thread vector
+    // workgroups (the partial buffers above were sized for the context's shape).
+    const int bv = nblk(n, kBlock), br = sm->slices;
+    c->part_rz.ensure((size_t)bv + 1);
+    c->part_max.ensure((size_t)bv + 1);
+    c->part_pAp.ensure((size_t)br + 1);
     BatchJob job;
     job.c = c;
     job.sm = sm;
     job.xsol = xsol;
     job.n = n;
-    job.nwg_v = nwg_v;
-    job.nwg_r = nwg_r;
+    job.nwg_v = bv;
+    job.nwg_r = br;
     job.maxit = maxit;
     job.tol = tol;
     job.A = A;
@@ -1225,6 +1250,94 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
     }
   }
   return PcgOut{st.iters, !st.failed};
+}
+
+// Launch-shape autotuner (ANDO_PCG_AUTOTUNE).  On the first sliced-ELL solve of
+// a system size, a short trial solve (32 iterations, tolerance 0) runs with each
+// candidate shape -- lanes per row, slices of 64 rows per SpMV workgroup, threads
+// per vector workgroup -- on the system at hand, and every SpMV and update launch
+// is timed with events recorded by the dispatch.  The shape with the least kernel
+// time per iteration (best of two trials) is kept for the context and, through a
+// process-wide table keyed by (device, block rows), for every later context of
+// that size.  The default shape stays unless another is at least 3% faster, so
+// timing noise between near-equal shapes does not move it.  The lanes alone
+// change no result; the grid and the vector block change the association of the
+// p.Ap and r.z sums, so results agree with the default shape to rounding.  The
+// trial solves write only the solve's own work vectors and x, which the real
+// solve then initialises again; their iterations are not counted in the stats.
+struct PcgShape {
+  int lanes, slices, vb;
+};
+constexpr PcgShape kPcgShapes[] = {{2, 1, kBlock}, {2, 0, kBlock}, {2, 2, kBlock}, {2, 4, kBlock},
+                                   {4, 4, kBlock}, {1, 4, kBlock}, {2, 1, 128},    {2, 1, 64}};
+std::mutex g_tune_mu;
+std::vector<std::pair<std::pair<int, int>, PcgShape>> g_tune_table;
+
+void set_shape(ando_ctx* c, const PcgShape& sh) {
+  c->sell_lanes = sh.lanes;
+  c->sell_wg_slices = sh.slices;
+  c->pcg_vec_block = sh.vb;
+}
+
+void autotune_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int32_t* cols, const int32_t* diag,
+                  const float* rhs, int negate, float* xsol, int maxit, SpmvMeta* sm, const ando::SelfView* sv) {
+  const std::pair<int, int> key(c->device, n);
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    for (const auto& e : g_tune_table)
+      if (e.first == key) {
+        set_shape(c, e.second);
+        c->tuned_n = n;
+        c->autotuned = 1;
+        return;
+      }
+  }
+  const int K = std::min(maxit, 32);
+  if (K < 4) return;
+  constexpr int NS = (int)(sizeof(kPcgShapes) / sizeof(kPcgShapes[0]));
+  const int saved_timing = c->kernel_timing, saved_chunk = c->pcg_chunk;
+  const int32_t tl[2] = {c->stats.timed_launches[0], c->stats.timed_launches[1]};
+  const double tm[2] = {c->stats.timed_ms[0], c->stats.timed_ms[1]};
+  c->tuning = true;
+  c->kernel_timing = 1;
+  c->pcg_chunk = K;  // one chunk: no status read-back inside the trial
+  double us[NS];
+  for (int k = 0; k < NS; ++k) {
+    set_shape(c, kPcgShapes[k]);
+    us[k] = INFINITY;
+    for (int rep = 0; rep < 2; ++rep) {
+      c->stats.timed_launches[0] = c->stats.timed_launches[1] = 0;
+      c->stats.timed_ms[0] = c->stats.timed_ms[1] = 0.0;
+      run_pcg(c, n, A, rp, cols, diag, rhs, negate, xsol, 0.0f, K, sm, nullptr, nullptr, sv);
+      if (c->stats.timed_launches[0] > 0)
+        us[k] = std::min(us[k], 1e3 * (c->stats.timed_ms[0] + c->stats.timed_ms[1]) / c->stats.timed_launches[0]);
+    }
+  }
+  c->tuning = false;
+  c->kernel_timing = saved_timing;
+  c->pcg_chunk = saved_chunk;
+  c->stats.timed_launches[0] = tl[0];
+  c->stats.timed_launches[1] = tl[1];
+  c->stats.timed_ms[0] = tm[0];
+  c->stats.timed_ms[1] = tm[1];
+  int best = 0;
+  for (int k = 1; k < NS; ++k)
+    if (us[k] < us[best]) best = k;
+  if (!(us[best] < 0.97 * us[0])) best = 0;
+  set_shape(c, kPcgShapes[best]);
+  if (!std::isfinite(us[0])) return;  // nothing ran (b = 0): try again on the next solve
+  c->tuned_n = n;
+  c->autotuned = 1;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune_table.push_back({key, kPcgShapes[best]});
+  }
+  if (c->pcg_autotune == 2) {
+    std::fprintf(stderr, "ando autotune n=%d:", n);
+    for (int k = 0; k < NS; ++k)
+      std::fprintf(stderr, " (%d,%d,%d)=%.2fus", kPcgShapes[k].lanes, kPcgShapes[k].slices, kPcgShapes[k].vb, us[k]);
+    std::fprintf(stderr, " -> (%d,%d,%d)\n", kPcgShapes[best].lanes, kPcgShapes[best].slices, kPcgShapes[best].vb);
+  }
 }
 
 float read_gmax(ando_ctx* c) {
@@ -1560,6 +1673,7 @@ int ando_create(int32_t device, ando_ctx** out) {
       const int l = std::atoi(w);
       c->sell_lanes = (l == 1 || l == 4) ? l : 2;
     }
+    if (const char* w = std::getenv("ANDO_PCG_AUTOTUNE")) c->pcg_autotune = std::min(2, std::max(0, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_ROWSETS")) c->spmv_rowsets = std::min(4, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_SPMV")) {
       c->spmv_env = std::strcmp(w, "blk") == 0 ? 1 : (std::strcmp(w, "sell") == 0 ? 2 : 0);

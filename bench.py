@@ -322,12 +322,16 @@ def main(argv=None):
     ap.add_argument("--corrected-steps", type=int, default=20)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-workload leg")
     ap.add_argument("--batched-steps", type=int, default=5)
+    ap.add_argument("--autotune", action="store_true",
+                    help="every context picks its PCG launch shape by timing (ANDO_PCG_AUTOTUNE=1)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); sharding the solve over several GPUs "
               "is not built.", file=sys.stderr)
         return 2
 
+    if a.autotune:
+        os.environ.setdefault("ANDO_PCG_AUTOTUNE", "1")
     from andosim_amd import native, scenes
 
     sc = scenes.WORKLOADS[a.workload]()
@@ -402,6 +406,8 @@ def main(argv=None):
     kernel = recs[-1][1]["pcg_kernel"]
     lanes = recs[-1][1].get("spmv_lanes", 0)
     wg_slices = recs[-1][1].get("spmv_wg_slices", 4)
+    vec_block = recs[-1][1].get("pcg_vec_block", 256)
+    tuned = recs[-1][1].get("pcg_autotuned", 0) == 1
 
     # Event pass: the same window again, every PCG launch bracketed by events.
     ev_recs = []
@@ -508,7 +514,8 @@ def main(argv=None):
                    "vertices": n, "blocks": nb, "precond": "block-Jacobi (plain PCG)",
                    "pcg_kernel": {1: "persistent", 0: "two-kernel, BSR row SpMV", 2: "two-kernel, block SpMV",
                                   3: f"two-kernel, sliced-ELL SpMV ({lanes} lane{'s' if lanes > 1 else ''} per row, "
-                                     f"{wg_slices} slice{'s' if wg_slices != 1 else ''} per workgroup)"
+                                     f"{wg_slices} slice{'s' if wg_slices != 1 else ''} per workgroup, "
+                                     f"{vec_block}-thread vector kernels{', autotuned' if tuned else ''})"
                                   }.get(kernel, "none"),
                    "timed_window": [window[0], window[-1]],
                    "first_nonfinite_step": last_ok + 1 if last_ok + 1 < a.max_explore else None},

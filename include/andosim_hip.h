@@ -152,6 +152,9 @@ typedef struct {
   int32_t spmv_lanes;               /* pcg_kernel 3: lanes per row of the sliced-ELL SpMV (1, 2 or 4) */
   int32_t spmv_wg_slices;           /* pcg_kernel 3: slices of 64 rows per SpMV workgroup (0: half a slice) */
   int32_t batched_solves;           /* solves run inside a batched PCG launch (ando_step_batch) */
+  int32_t pcg_vec_block;            /* pcg_kernel 0, 2, 3: threads per workgroup of the PCG vector kernels */
+  int32_t pcg_autotuned;            /* 1: the launch shape above was picked by the autotuner
+                                       (environment ANDO_PCG_AUTOTUNE=1, read by ando_create) */
 } ando_stats;
 
 int32_t ando_abi_version(void);

@@ -31,6 +31,38 @@ def test_batch_equals_sequential():
         assert sa["batched_solves"] == 0 and sb["batched_solves"] == sb["pcg_solves"] > 0
 
 
+def test_batch_after_autotune(monkeypatch):
+    """Contexts whose launch shape came from the autotuner (ANDO_PCG_AUTOTUNE=1,
+    tuned on a single step) still join batched solves, on the batch's own
+    grid: from the same state, the batched steps equal stepping contexts of the
+    default shape one by one, bit for bit."""
+    print("batch of 8 autotuned cloths", flush=True)
+    scs = scenes.batch256(count=8)
+    seq = [native.GpuSim(sc) for sc in scs]
+    monkeypatch.setenv("ANDO_PCG_AUTOTUNE", "1")
+    bat = [native.GpuSim(sc) for sc in scs]
+    monkeypatch.delenv("ANDO_PCG_AUTOTUNE")
+    start = [(g.get("x").copy(), g.get("v").copy()) for g in bat]
+    for g in bat:
+        g.substep()
+        assert g.stats()["pcg_autotuned"] == 1
+    for g, (x, v) in zip(bat, start):
+        g.set_state(x, v)
+    for _ in range(3):
+        for g in seq:
+            g.substep()
+        for g in bat:
+            g.apply_gravity()
+        native.step_batch(bat, threads=4)
+    for a, b in zip(seq, bat):
+        assert np.array_equal(a.get("x"), b.get("x"))
+        assert np.array_equal(a.get("v"), b.get("v"))
+        sa, sb = a.stats(), b.stats()
+        assert sa["pcg_iterations"] == sb["pcg_iterations"]
+        assert sb["batched_solves"] == sb["pcg_solves"] > 0 and sb["pcg_autotuned"] == 0
+        assert (sb["spmv_lanes"], sb["spmv_wg_slices"], sb["pcg_vec_block"]) == (2, 1, 256)
+
+
 def test_batch_mixed_modes_one_thread_per_cloth():
     """Twenty cloths on twenty threads (large batched rounds), with a
     corrected-mode context among them whose solves keep their own launches

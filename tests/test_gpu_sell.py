@@ -158,6 +158,56 @@ def test_sell_finer_grid_matches(slices, vb, monkeypatch):
     assert np.max(np.abs(a.get("x") - s.get("x"))) <= 1e-6 * max(1.0, float(np.max(np.abs(a.get("x")))))
 
 
+# Launch shapes the autotuner may pick, as the stats report them:
+# (lanes per row, slices per SpMV workgroup, threads per vector workgroup).
+AUTOTUNE_SHAPES = {(2, 1, 256), (2, 0, 256), (2, 2, 256), (2, 4, 256), (4, 4, 256), (1, 4, 256), (2, 1, 128),
+                   (2, 1, 64)}
+
+
+def test_pcg_autotune_picks_a_tested_shape(monkeypatch):
+    """ANDO_PCG_AUTOTUNE=1: the first solve times the candidate launch shapes
+    on its own system and keeps the fastest.  The stats report the shape and
+    that it was tuned; each candidate is a grid whose solves agree with the
+    four-slice grid to 1e-6 (the tests above), so steps agree with the default
+    shape to 2e-6 with iteration counts within 1%.  A second context of the
+    same size takes the tuned shape from the process-wide table."""
+    sc = scenes.sphere129()
+    a = native.GpuSim(sc)
+    monkeypatch.setenv("ANDO_PCG_AUTOTUNE", "1")
+    t = native.GpuSim(sc)
+    t2 = native.GpuSim(sc)
+    monkeypatch.delenv("ANDO_PCG_AUTOTUNE")
+    shape = None
+    for k in range(4):
+        a.substep()
+        t.substep()
+        sa, st = a.stats(), t.stats()
+        print(f"substep {k}: tuned shape", (st["spmv_lanes"], st["spmv_wg_slices"], st["pcg_vec_block"]), flush=True)
+        assert sa["pcg_autotuned"] == 0 and (sa["spmv_lanes"], sa["spmv_wg_slices"], sa["pcg_vec_block"]) == (2, 1, 256)
+        assert st["pcg_autotuned"] == 1 and st["pcg_kernel"] == 3
+        shape = (st["spmv_lanes"], st["spmv_wg_slices"], st["pcg_vec_block"])
+        assert shape in AUTOTUNE_SHAPES
+        assert st["pcg_solves"] == sa["pcg_solves"]
+        assert abs(sa["pcg_iterations"] - st["pcg_iterations"]) <= max(2, sa["pcg_iterations"] // 100)
+    assert np.max(np.abs(a.get("x") - t.get("x"))) <= 2e-6 * max(1.0, float(np.max(np.abs(a.get("x")))))
+    t2.substep()
+    s2 = t2.stats()
+    assert s2["pcg_autotuned"] == 1 and (s2["spmv_lanes"], s2["spmv_wg_slices"], s2["pcg_vec_block"]) == shape
+    assert np.array_equal(t2.get("x"), _one_step_with_shape(sc, shape, monkeypatch))
+
+
+def _one_step_with_shape(sc, shape, monkeypatch):
+    lanes, slices, vb = shape
+    monkeypatch.setenv("ANDO_SELL_LANES", str(lanes))
+    monkeypatch.setenv("ANDO_SELL_WG_SLICES", str(slices))
+    monkeypatch.setenv("ANDO_PCG_VEC_BLOCK", str(vb))
+    g = native.GpuSim(sc)
+    for k in ("ANDO_SELL_LANES", "ANDO_SELL_WG_SLICES", "ANDO_PCG_VEC_BLOCK"):
+        monkeypatch.delenv(k)
+    g.substep()
+    return g.get("x")
+
+
 def _friction_sheet():
     sc = scenes.small_sheet(9, pins=0, wall=False, rigid=True)
     sc.params.update(enable_friction=True, friction_mu=0.5)
