@@ -23,8 +23,10 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "andosim_hip.h"
@@ -176,6 +178,64 @@ void abi_check(int rc) {
   }
 }
 
+// Page-locked buffers behind the arrays get_positions / get_velocities return
+// while the device holds the newer state: the device copies into the buffer by
+// DMA and the array owns it.  A released array hands its buffer back to a pool
+// (at most four free buffers per size), so steady per-frame reads allocate
+// nothing.  Live buffers are capped at kPinnedCap bytes; beyond that the reads
+// copy through the state's pageable host arrays.  The pool lives until the
+// process ends.
+constexpr size_t kPinnedCap = size_t(256) << 20;
+struct PinnedPool {
+  std::mutex mu;
+  std::vector<std::pair<size_t, void*>> idle;  // (bytes, buffer)
+  std::unordered_map<void*, size_t> live;      // buffer -> bytes
+  size_t live_bytes = 0;
+};
+PinnedPool& pinned_pool() {
+  static PinnedPool* pool = new PinnedPool();
+  return *pool;
+}
+void* pinned_take(size_t bytes) {
+  PinnedPool& pool = pinned_pool();
+  {
+    std::lock_guard<std::mutex> g(pool.mu);
+    if (pool.live_bytes + bytes > kPinnedCap) return nullptr;
+    for (size_t k = 0; k < pool.idle.size(); ++k)
+      if (pool.idle[k].first == bytes) {
+        void* p = pool.idle[k].second;
+        pool.idle.erase(pool.idle.begin() + (std::ptrdiff_t)k);
+        pool.live[p] = bytes;
+        pool.live_bytes += bytes;
+        return p;
+      }
+  }
+  void* p = nullptr;
+  if (ando_host_alloc(bytes, &p) != ANDO_OK) return nullptr;
+  std::lock_guard<std::mutex> g(pinned_pool().mu);
+  pool.live[p] = bytes;
+  pool.live_bytes += bytes;
+  return p;
+}
+void pinned_give(void* p) {
+  PinnedPool& pool = pinned_pool();
+  {
+    std::lock_guard<std::mutex> g(pool.mu);
+    auto it = pool.live.find(p);
+    if (it == pool.live.end()) return;
+    const size_t bytes = it->second;
+    pool.live.erase(it);
+    pool.live_bytes -= bytes;
+    int same = 0;
+    for (const auto& e : pool.idle) same += e.first == bytes;
+    if (same < 4) {
+      pool.idle.push_back({bytes, p});
+      return;
+    }
+  }
+  (void)ando_host_free(p);
+}
+
 struct State {
   std::vector<float> positions, velocities, masses, positions_prev;
   std::shared_ptr<Session> session;
@@ -201,6 +261,28 @@ struct State {
       abi_check(ando_get_state(session->ctx, positions.data(), velocities.data(), (int32_t)num_vertices()));
       device_dirty = false;
     }
+  }
+  // get_positions (pos) / get_velocities: a new (N, 3) array, as the
+  // reference's copy out of its Eigen vector (bindings.cpp:248-267).  While the
+  // device holds the newer state, only the one array is copied, by DMA into a
+  // pinned buffer that the returned array owns; the host copies stay stale.
+  py::array_t<float> read(bool pos) {
+    const size_t n = num_vertices();
+    if (device_dirty && session && session->ctx && n > 0) {
+      if (void* buf = pinned_take(3 * n * sizeof(float))) {
+        float* f = static_cast<float*>(buf);
+        const int rc = ando_get_state(session->ctx, pos ? f : nullptr, pos ? nullptr : f, (int32_t)n);
+        if (rc != ANDO_OK) {
+          pinned_give(buf);
+          abi_check(rc);
+        }
+        py::capsule owner(buf, [](void* p) { pinned_give(p); });
+        return py::array_t<float>({py::ssize_t(n), py::ssize_t(3)},
+                                  {py::ssize_t(3 * sizeof(float)), py::ssize_t(sizeof(float))}, f, owner);
+      }
+    }
+    pull();
+    return nx3(pos ? positions.data() : velocities.data(), n);
   }
 };
 
@@ -1004,16 +1086,8 @@ PYBIND11_MODULE(ando_barrier_core, m) {
       .def(py::init<>())
       .def("initialize", &State::initialize)
       .def("num_vertices", &State::num_vertices)
-      .def("get_positions",
-           [](State& s) {
-             s.pull();
-             return nx3(s.positions.data(), s.num_vertices());
-           })
-      .def("get_velocities",
-           [](State& s) {
-             s.pull();
-             return nx3(s.velocities.data(), s.num_vertices());
-           })
+      .def("get_positions", [](State& s) { return s.read(true); })
+      .def("get_velocities", [](State& s) { return s.read(false); })
       .def("set_velocities",
            [](State& s, FArr velocities) {
              if (velocities.ndim() != 2 || velocities.shape(1) < 3)

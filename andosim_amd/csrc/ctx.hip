@@ -1830,6 +1830,20 @@ int ando_get_state(ando_ctx* c, float* positions, float* velocities, int32_t n) 
   });
 }
 
+int ando_host_alloc(size_t bytes, void** out) {
+  return guarded([&] {
+    need(out != nullptr && bytes > 0, ANDO_EINVAL, "bad pinned allocation request");
+    *out = nullptr;
+    if (hipHostMalloc(out, bytes, hipHostMallocDefault) != hipSuccess) throw Fail(ANDO_EDEVICE, "hipHostMalloc failed");
+  });
+}
+
+int ando_host_free(void* ptr) {
+  return guarded([&] {
+    if (ptr) HIPCHK(hipHostFree(ptr));
+  });
+}
+
 int ando_set_pins(ando_ctx* c, const int32_t* vertices, const float* targets, int32_t count) {
   return guarded([&] {
     CtxLock lock(c);

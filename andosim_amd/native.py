@@ -26,6 +26,7 @@ ABI_SYMBOLS = [
     "ando_trainer_add_system", "ando_trainer_init_weights", "ando_trainer_set_weights", "ando_trainer_get_weights",
     "ando_trainer_forward", "ando_trainer_loss_grad", "ando_trainer_train", "ando_trainer_solve",
     "ando_energy_diagnostics", "ando_contact_metrics", "ando_cfl_inputs", "ando_step_batch",
+    "ando_host_alloc", "ando_host_free",
 ]
 
 
@@ -112,6 +113,8 @@ def load():
     L.ando_lumped_masses.argtypes = [fp, C.c_int32, ip, C.c_int32, C.POINTER(Material), fp]
     L.ando_set_state.argtypes = [C.c_void_p, fp, fp, fp, C.c_int32]
     L.ando_get_state.argtypes = [C.c_void_p, fp, fp, C.c_int32]
+    L.ando_host_alloc.argtypes = [C.c_size_t, C.POINTER(C.c_void_p)]
+    L.ando_host_free.argtypes = [C.c_void_p]
     L.ando_set_previous_positions.argtypes = [C.c_void_p, fp, C.c_int32]
     L.ando_set_pins.argtypes = [C.c_void_p, ip, fp, C.c_int32]
     L.ando_set_walls.argtypes = [C.c_void_p, fp, fp, fp, C.c_int32]

@@ -140,7 +140,14 @@ def per_frame_leg(sim, sc, reps: int = 20, with_cpu: bool = True):
            "detect_contacts_ms": avg_ms(lambda: native.check(L.ando_compute_contacts(
                h, sim.bodies, sim.nbodies, None, 0, C.byref(cnt)))),
            "cfl_inputs_ms": avg_ms(sim.cfl_inputs),
-           "get_state_ms": avg_ms(lambda: native.check(L.ando_get_state(h, native._fp(x), native._fp(v), sim.n)))}
+           "get_state_ms": avg_ms(lambda: native.check(L.ando_get_state(h, native._fp(x), native._fp(v), sim.n))),
+           "get_positions_ms": avg_ms(lambda: native.check(L.ando_get_state(h, native._fp(x), None, sim.n)))}
+    # The module's get_positions: positions only, by DMA into a pinned buffer.
+    buf = C.c_void_p()
+    native.check(L.ando_host_alloc(12 * sim.n, C.byref(buf)))
+    pinned = C.cast(buf, C.POINTER(C.c_float))
+    out["get_positions_pinned_ms"] = avg_ms(lambda: native.check(L.ando_get_state(h, pinned, None, sim.n)))
+    native.check(L.ando_host_free(buf))
     cs, count = sim.contacts(cap=1 << 20)
     out["contacts"] = count
     out["metrics_ms"] = avg_ms(lambda: sim.contact_metrics(float(sc.params["contact_gap_max"]), True))

@@ -29,6 +29,10 @@
  *   ando_contact_metrics   CollisionValidator::compute_metrics          src/core/collision_validator.cpp:133-157
  *   ando_cfl_inputs        AdaptiveTimestep::compute_max_velocity +
  *                          compute_min_edge_length                      src/core/adaptive_timestep.cpp:82-150
+ *   ando_host_alloc/_free  (new, additive) pinned host buffers: ando_get_state into one is a
+ *                          direct DMA; the module returns get_positions / get_velocities
+ *                          as arrays over such buffers (the reference copies out of an
+ *                          Eigen vector, src/py/bindings.cpp:248-267)
  *   ando_step_batch        (new, additive) ando_step on many contexts at once; the reference
  *                          steps independent cloths one after another
  *   ando_get_stats         (new, additive) solver statistics; the reference exposes none
@@ -43,6 +47,7 @@
 #ifndef ANDOSIM_HIP_H
 #define ANDOSIM_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -173,6 +178,11 @@ int ando_lumped_masses(const float* vertices, int32_t num_vertices, const int32_
 int ando_set_state(ando_ctx* ctx, const float* positions, const float* velocities, const float* masses,
                    int32_t num_vertices);
 int ando_get_state(ando_ctx* ctx, float* positions, float* velocities, int32_t num_vertices);
+/* Page-locked host memory (new, additive).  ando_get_state copies into such a
+ * buffer by DMA, without the staging copy that pageable memory needs.  Needs no
+ * context; ando_host_free(NULL) does nothing. */
+int ando_host_alloc(size_t bytes, void** out);
+int ando_host_free(void* ptr);
 /* State::positions_prev (src/core/state.h:18): set once by State::initialize
  * (state.cpp:13) and read by friction (integrator.cpp:39,384).  Defaults to the
  * positions of the first complete ando_set_state. */

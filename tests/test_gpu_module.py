@@ -176,6 +176,35 @@ def test_module_host_edits_reach_device():
     assert np.max(np.abs(state.get_positions() - x0)) < 1e-6
 
 
+def test_module_readback_arrays_are_independent():
+    """get_positions / get_velocities while the device holds the newer state:
+    each call returns a new (N, 3) float32 array over its own pinned buffer;
+    arrays from earlier calls keep their values through later steps and later
+    reads (buffers go back to the pool only when their array is released), and
+    the values equal the pageable copy of the module's host path."""
+    abc = pytest.importorskip("ando_barrier_core")
+    sc = scenes.small_sheet(9, pins=0, wall=True)
+    mesh, state, cons, p, _ = build_module_scene(abc, sc)
+    n = mesh.num_vertices()
+    kept = []
+    for _ in range(6):
+        state.apply_gravity((0.0, 0.0, -9.81), p.dt)
+        abc.Integrator.step(mesh, state, cons, p)
+        x, v = state.get_positions(), state.get_velocities()
+        assert x.shape == (n, 3) and x.dtype == np.float32 and x.flags.c_contiguous and x.flags.writeable
+        x2 = state.get_positions()
+        assert np.array_equal(x, x2) and not np.shares_memory(x, x2)
+        x2[:] = 0.0
+        del x2
+        kept.append((x.copy(), x, v.copy(), v))
+    for xc, x, vc, v in kept:
+        assert np.array_equal(xc, x) and np.array_equal(vc, v)
+    assert not np.array_equal(kept[0][0], kept[-1][0])
+    xs, vs = kept[-1][0], kept[-1][2]
+    state.set_velocities(vs)  # pulls both arrays into the host copies first
+    assert np.array_equal(state.get_positions(), xs) and np.array_equal(state.get_velocities(), vs)
+
+
 def light_sphere_scene(density=50.0):
     """The 9x9 sheet falling on a sphere light enough to be pushed around."""
     sc = scenes.small_sheet(9, pins=0, wall=False, rigid=True)
