@@ -78,6 +78,30 @@ struct DBuf {
   void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
 };
 
+// Page-locked host buffer for small read-backs: a copy into it is a direct DMA,
+// where a pageable destination costs the runtime a staging copy.  Grows on
+// demand; each use ends with a stream synchronisation before the next.
+struct PinnedStage {
+  void* p = nullptr;
+  size_t n = 0;
+  PinnedStage() = default;
+  PinnedStage(const PinnedStage&) = delete;
+  PinnedStage& operator=(const PinnedStage&) = delete;
+  ~PinnedStage() {
+    if (p) (void)hipHostFree(p);
+  }
+  void* ensure(size_t bytes) {
+    if (bytes <= n && p) return p;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+    const size_t b = std::max<size_t>(bytes, 256);
+    HIPCHK(hipHostMalloc(&p, b, hipHostMallocDefault));
+    n = b;
+    return p;
+  }
+};
+
 static inline int nblk(long long work, int bs = kBlock) { return (int)((work + bs - 1) / bs); }
 // Workgroups of the block-row SpMV with R rows per lane group
 // (R * kRowsPerPass block rows per workgroup); R = 1 is the largest count.
@@ -252,6 +276,8 @@ struct ando_ctx {
   // Small readbacks.
   DBuf<uint32_t> scalars;  // [0] gradient max bits, [1] line-search mask, [2] strain active count
   PcgStatus* status_pinned = nullptr;  // host copy of the PCG status (pinned)
+  uint32_t* sact_pinned = nullptr;     // host copy of the strain-active count (pinned)
+  PinnedStage hstage;                  // small read-backs (read_small)
   hipEvent_t status_ev = nullptr;
   int pcg_chunk = 8;
   int pcg_chunk_max = 256;  // ANDO_PCG_CHUNK_MAX: largest chunk (chunks double from pcg_chunk)
@@ -318,6 +344,7 @@ struct ando_ctx {
   ~ando_ctx() {
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     if (status_pinned) (void)hipHostFree(status_pinned);
+    if (sact_pinned) (void)hipHostFree(sact_pinned);
     if (status_ev) (void)hipEventDestroy(status_ev);
   }
 };
@@ -359,6 +386,16 @@ void need(bool cond, int code, const char* msg) {
 }
 
 void sync(ando_ctx* c) { HIPCHK(hipStreamSynchronize(c->stream)); }
+
+// Copies count values from device memory through the context's pinned stage
+// and waits for them (everything queued before on the stream finishes first).
+template <class T>
+void read_small(ando_ctx* c, T* dst, const T* src, size_t count) {
+  T* h = static_cast<T*>(c->hstage.ensure(count * sizeof(T)));
+  HIPCHK(hipMemcpyAsync(h, src, count * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  std::memcpy(dst, h, count * sizeof(T));
+}
 
 // Block pattern, gather lists and per-face block slots.
 void build_topology(ando_ctx* c, const std::vector<int32_t>& edges) {
@@ -557,8 +594,7 @@ void detect_rigid(ando_ctx* c, const ando_rigid_body* bodies, int nbod, float dt
                      c->scan_total.p);
   HIPCHK(hipGetLastError());
   int32_t total = 0;
-  HIPCHK(hipMemcpyAsync(&total, c->scan_total.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  sync(c);
+  read_small(c, &total, (const int32_t*)c->scan_total.p, 1);
   c->rc_total = total;
   c->rc.ensure((size_t)total + 1);
   if (total)
@@ -582,8 +618,7 @@ int32_t exclusive_scan(ando_ctx* c, int n, const int32_t* in, int32_t* out, DBuf
   HIPCHK(hipGetLastError());
   if (!read_total) return -1;
   int32_t t = 0;
-  HIPCHK(hipMemcpyAsync(&t, total.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  sync(c);
+  read_small(c, &t, (const int32_t*)total.p, 1);
   return t;
 }
 
@@ -716,8 +751,7 @@ void detect_self(ando_ctx* c) {
   }
   HIPCHK(hipGetLastError());
   uint32_t err = 0;
-  HIPCHK(hipMemcpyAsync(&err, c->sc_err.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  sync(c);
+  read_small(c, &err, (const uint32_t*)c->sc_err.p, 1);
   if (err) throw Fail(ANDO_EDEVICE, "BVH traversal stack exceeded its depth");
   c->sc_pt = npt;
   c->sc_total = npt + nee;
@@ -1342,8 +1376,7 @@ void autotune_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const i
 
 float read_gmax(ando_ctx* c) {
   uint32_t bits = 0;
-  HIPCHK(hipMemcpyAsync(&bits, c->scalars.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-  sync(c);
+  read_small(c, &bits, (const uint32_t*)c->scalars.p, 1);
   float f;
   std::memcpy(&f, &bits, sizeof(float));
   return f;
@@ -1414,8 +1447,7 @@ float line_search(ando_ctx* c) {
     wall = sq > 0.5;
   }
   if (!wall && !self) return 1.0f;
-  uint32_t init = 0xFFFFFu;
-  HIPCHK(hipMemcpyAsync(c->scalars.p + 1, &init, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemsetD32Async(c->scalars.p + 1, 0xFFFFF, 1, s));  // bit t: alpha = 2^-t still feasible
   if (wall)
     hipLaunchKernelGGL(k_ls_mask, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, w, c->wall_off[0], 1, c->x.p, c->d.p,
                        c->scalars.p + 1);
@@ -1425,8 +1457,7 @@ float line_search(ando_ctx* c) {
                        c->additive_ccd, c->scalars.p + 1);
   HIPCHK(hipGetLastError());
   uint32_t mask = 0;
-  HIPCHK(hipMemcpyAsync(&mask, c->scalars.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-  sync(c);
+  read_small(c, &mask, (const uint32_t*)(c->scalars.p + 1), 1);
   for (int t = 0; t < 20; ++t)
     if (mask & (1u << t)) return std::ldexp(1.0f, -t);
   return 0.0f;
@@ -1554,8 +1585,7 @@ void rigid_coupling(ando_ctx* c, const ando_params& p, ando_rigid_body* bodies, 
                      c->rw_out.p);
   HIPCHK(hipGetLastError());
   std::vector<double> w(6 * (size_t)nbod);
-  HIPCHK(hipMemcpyAsync(w.data(), c->rw_out.p, sizeof(double) * w.size(), hipMemcpyDeviceToHost, s));
-  sync(c);
+  read_small(c, w.data(), (const double*)c->rw_out.p, w.size());
   for (int b = 0; b < nbod; ++b) rigid_integrate(bodies[b], &w[6 * b], &w[6 * b + 3], p.dt);
 }
 
@@ -1605,8 +1635,7 @@ void restitution(ando_ctx* c, const ando_params& p, ando_rigid_body* bodies, int
     if (c->rc_total)
       hipLaunchKernelGGL(k_restitution_rigid, dim3(1), dim3(64), 0, s, c->rc_total, c->rc.p, c->v.p, c->masses.p,
                          c->body_state.p, nbod, e, std::max(p.contact_gap_max, 1e-5f));
-    HIPCHK(hipMemcpyAsync(bs.data(), c->body_state.p, sizeof(BodyState) * nbod, hipMemcpyDeviceToHost, s));
-    sync(c);
+    read_small(c, bs.data(), (const BodyState*)c->body_state.p, (size_t)nbod);
     for (int b = 0; b < nbod; ++b) {
       std::memcpy(bodies[b].linear_velocity, bs[b].lin, sizeof(bs[b].lin));
       std::memcpy(bodies[b].angular_velocity, bs[b].ang, sizeof(bs[b].ang));
@@ -1651,6 +1680,7 @@ int ando_create(int32_t device, ando_ctx** out) {
     }
     c->scalars.ensure(4);
     if (hipHostMalloc((void**)&c->status_pinned, sizeof(PcgStatus), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&c->sact_pinned, sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&c->status_ev, hipEventDisableTiming) != hipSuccess) {
       delete c;
       throw Fail(ANDO_EDEVICE, "pinned status or event allocation failed");
@@ -1966,12 +1996,11 @@ int ando_step(ando_ctx* c, const ando_params* params, ando_rigid_body* bodies, i
       HIPCHK(hipGetLastError());
       if ((double)p.contact_restitution > 0.0) restitution(c, p, bodies, num_bodies);
     }
-    uint32_t sact = 0;
-    HIPCHK(hipMemcpyAsync(&sact, c->scalars.p + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    sync(c);
-    c->stats.strain_active = (int32_t)sact;
+    // Read with the step's last synchronisation.
+    HIPCHK(hipMemcpyAsync(c->sact_pinned, c->scalars.p + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     if (num_bodies > 0 && !c->static_rigids) rigid_coupling(c, p, bodies, num_bodies);
     sync(c);
+    c->stats.strain_active = (int32_t)*c->sact_pinned;
     c->stats.step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   });
 }
