@@ -258,7 +258,7 @@ struct ando_ctx {
   DBuf<float> pin_t;
   int num_walls = 0;
   std::vector<float> wall_n, wall_off, wall_gap;
-  DBuf<float> d_wall_n, d_wall_off, kw;
+  DBuf<float> d_wall_n, d_wall_off, d_wall_gap, kw;
   bool kw_valid = false;
   float kw_wall_gap = -1.0f, kw_min_gap = -1.0f;
   // Rigid contacts.
@@ -740,12 +740,15 @@ void detect_self(ando_ctx* c) {
     nee = exclusive_scan(c, E, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
   }
   if (nee) {
-    // Grow (keeping the PT part) and append the EE part.
-    DBuf<ando::SelfContact> all;
-    all.ensure((size_t)npt + nee);
-    if (npt)
-      HIPCHK(hipMemcpyAsync(all.p, c->sc.p, sizeof(ando::SelfContact) * npt, hipMemcpyDeviceToDevice, s));
-    c->sc = std::move(all);
+    // Append the EE part, growing the list (keeping the PT part, with a
+    // quarter of headroom so later detections reuse it) only when it is full.
+    if (c->sc.n < (size_t)npt + nee) {
+      DBuf<ando::SelfContact> all;
+      all.ensure(((size_t)npt + nee) * 5 / 4 + 1);
+      if (npt)
+        HIPCHK(hipMemcpyAsync(all.p, c->sc.p, sizeof(ando::SelfContact) * npt, hipMemcpyDeviceToDevice, s));
+      c->sc = std::move(all);
+    }
     c->sc_key.ensure((size_t)npt + nee);
     ee(1, c->sc.p + npt, c->sc_key.p + npt);
   }
@@ -1641,13 +1644,9 @@ void restitution(ando_ctx* c, const ando_params& p, ando_rigid_body* bodies, int
       std::memcpy(bodies[b].angular_velocity, bs[b].ang, sizeof(bs[b].ang));
     }
   }
-  if (c->num_walls) {
-    DBuf<float> gaps;
-    gaps.upload(c->wall_gap, s);
+  if (c->num_walls)
     hipLaunchKernelGGL(k_restitution_walls, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, c->num_walls,
-                       c->d_wall_n.p, c->d_wall_off.p, gaps.p, p.contact_gap_max, e, c->x.p, c->v.p);
-    sync(c);
-  }
+                       c->d_wall_n.p, c->d_wall_off.p, c->d_wall_gap.p, p.contact_gap_max, e, c->x.p, c->v.p);
   HIPCHK(hipGetLastError());
 }
 
@@ -1914,6 +1913,7 @@ int ando_set_walls(ando_ctx* c, const float* normals, const float* offsets, cons
     if (count) {
       c->d_wall_n.upload(c->wall_n, c->stream);
       c->d_wall_off.upload(c->wall_off, c->stream);
+      c->d_wall_gap.upload(c->wall_gap, c->stream);
     }
     c->kw_valid = false;
     sync(c);
