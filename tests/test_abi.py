@@ -52,3 +52,26 @@ def test_errors_are_reported_not_raised():
     rc = L.ando_step(None, None, None, 0)
     assert rc != 0
     assert len(L.ando_last_error()) > 0
+
+
+def _header_struct_fields(name):
+    text = open(os.path.join(ROOT, "include", "andosim_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    m = re.search(r"typedef struct \{([^{}]*)\}\s*" + name + r"\s*;", text)
+    assert m, name
+    return [f.group(1) for f in re.finditer(r"([a-z_0-9]+)\s*(?:\[[A-Z_0-9]+\])?\s*;", m.group(1))]
+
+
+@pytest.mark.parametrize("cname,pyname", [("ando_stats", "Stats"), ("ando_params", "Params"),
+                                          ("ando_material", "Material"), ("ando_contact", "Contact"),
+                                          ("ando_rigid_body", "RigidBody"), ("ando_energy", "Energy"),
+                                          ("ando_collision_metrics", "CollisionMetrics")])
+def test_ctypes_structs_follow_the_header(cname, pyname):
+    """The ctypes mirrors in native.py list the header's struct fields in the
+    header's order, so the layouts agree (a field added to the header but not
+    to the mirror would shift every later one)."""
+    native = pytest.importorskip("andosim_amd.native")
+    cls = getattr(native, pyname, None)
+    if cls is None:
+        pytest.skip(f"native.py has no {pyname} mirror")
+    assert [f[0] for f in cls._fields_] == _header_struct_fields(cname)
