@@ -2069,7 +2069,7 @@ int orc_get(void* h, int which, double* out) {
     });
   });
 }
-// which: 0 positions, 1 velocities, 2 masses, 3 mesh vertices
+// which: 0 positions, 1 velocities, 2 masses, 3 mesh vertices, 4 positions_prev
 int orc_set(void* h, int which, const double* in) {
   OSim* s = (OSim*)h;
   return guard([&] {
@@ -2080,6 +2080,8 @@ int orc_set(void* h, int which, const double* in) {
       else if (which == 2) for (int i = 0; i < S.n; ++i) S.m[i] = (R)in[i];
       else if (which == 3)  // Mesh.vertices setter: the rest state is kept (mesh.cpp:122-124)
         for (int i = 0; i < S.n; ++i) for (int k = 0; k < 3; ++k) S.mverts[i][k] = (R)in[3 * i + k];
+      else if (which == 4)  // State::positions_prev (state.h:18), read by friction (integrator.cpp:39)
+        for (int i = 0; i < 3 * S.n; ++i) S.xprev[i] = (R)in[i];
       else throw std::invalid_argument("bad selector");
     });
   });
@@ -2125,6 +2127,21 @@ int orc_set_rigid_state(void* h, int rb, const double* in /*pos3, lin3, ang3, ro
       auto& b = S.rigids.at(rb);
       for (int k = 0; k < 3; ++k) { b.pos[k] = (R)in[k]; b.lin[k] = (R)in[3 + k]; b.ang[k] = (R)in[6 + k]; }
       for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) b.rot.m[r][c] = (R)in[9 + 3 * r + c];
+    });
+  });
+}
+
+// RigidBody::apply_force accumulation then integrate (rigid_body.cpp:129-176)
+// on body rb with the given total force and torque (test hook).
+int orc_rigid_integrate(void* h, int rb, const double* force, const double* torque, double dt) {
+  OSim* s = (OSim*)h;
+  return guard([&] {
+    DISPATCH(s, {
+      using R = std::remove_reference_t<decltype(S.E)>;
+      auto& b = S.rigids.at(rb);
+      b.force = V3<R>((R)force[0], (R)force[1], (R)force[2]);
+      b.torque = V3<R>((R)torque[0], (R)torque[1], (R)torque[2]);
+      b.integrate((R)dt);
     });
   });
 }

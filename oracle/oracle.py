@@ -59,6 +59,7 @@ def lib():
         L.orc_pcg_per_solve.argtypes = [C.c_void_p, _ip, C.c_int]
         L.orc_rigid_state.argtypes = [C.c_void_p, C.c_int, _dp]
         L.orc_set_rigid_state.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.orc_rigid_integrate.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_double]
         L.orc_pattern.argtypes = [C.c_void_p, _ip, _ip]
         L.orc_linearize.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.orc_pcg.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, C.POINTER(C.c_int)]
@@ -201,7 +202,7 @@ class OracleSim:
         return out if sel == 2 else out.reshape(-1, 3)
 
     def set(self, which: str, arr):
-        _chk(self.L.orc_set(self.h, {"x": 0, "v": 1, "m": 2, "mesh": 3}[which], _d(arr).ravel()))
+        _chk(self.L.orc_set(self.h, {"x": 0, "v": 1, "m": 2, "mesh": 3, "xprev": 4}[which], _d(arr).ravel()))
 
     def set_contact_mode(self, corrected: bool, pt_box: float = 1e-4, keep_gap: float = 0.01):
         """Faithful (self broad phase empty) or corrected (brute-force PT/EE with
@@ -261,6 +262,10 @@ class OracleSim:
                              np.asarray(rs["angular_velocity"], np.float64),
                              np.asarray(rs["rotation"], np.float64).ravel()])
         _chk(self.L.orc_set_rigid_state(self.h, rb, _d(st)))
+
+    def rigid_integrate(self, rb: int, force, torque, dt: float):
+        """RigidBody::integrate of body rb with the given force and torque."""
+        _chk(self.L.orc_rigid_integrate(self.h, rb, _d(force), _d(torque), float(dt)))
 
     def pattern(self):
         nb = self.L.orc_num_blocks(self.h)
