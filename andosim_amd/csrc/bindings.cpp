@@ -182,15 +182,16 @@ void abi_check(int rc) {
 // while the device holds the newer state: the device copies into the buffer by
 // DMA and the array owns it.  A released array hands its buffer back to a pool
 // (at most four free buffers per size), so steady per-frame reads allocate
-// nothing.  Live buffers are capped at kPinnedCap bytes; beyond that the reads
-// copy through the state's pageable host arrays.  The pool lives until the
-// process ends.
+// nothing.  Live and idle buffers together are capped at kPinnedCap bytes: a
+// request that would pass the cap first frees idle buffers, oldest first, and
+// if the live buffers alone leave no room the read copies through the state's
+// pageable host arrays instead.  The pool lives until the process ends.
 constexpr size_t kPinnedCap = size_t(256) << 20;
 struct PinnedPool {
   std::mutex mu;
-  std::vector<std::pair<size_t, void*>> idle;  // (bytes, buffer)
+  std::vector<std::pair<size_t, void*>> idle;  // (bytes, buffer), oldest first
   std::unordered_map<void*, size_t> live;      // buffer -> bytes
-  size_t live_bytes = 0;
+  size_t live_bytes = 0, idle_bytes = 0;
 };
 PinnedPool& pinned_pool() {
   static PinnedPool* pool = new PinnedPool();
@@ -198,23 +199,36 @@ PinnedPool& pinned_pool() {
 }
 void* pinned_take(size_t bytes) {
   PinnedPool& pool = pinned_pool();
+  std::vector<void*> evict;
   {
     std::lock_guard<std::mutex> g(pool.mu);
-    if (pool.live_bytes + bytes > kPinnedCap) return nullptr;
     for (size_t k = 0; k < pool.idle.size(); ++k)
       if (pool.idle[k].first == bytes) {
         void* p = pool.idle[k].second;
         pool.idle.erase(pool.idle.begin() + (std::ptrdiff_t)k);
+        pool.idle_bytes -= bytes;
         pool.live[p] = bytes;
         pool.live_bytes += bytes;
         return p;
       }
+    if (pool.live_bytes + bytes > kPinnedCap) return nullptr;
+    while (!pool.idle.empty() && pool.live_bytes + pool.idle_bytes + bytes > kPinnedCap) {
+      evict.push_back(pool.idle.front().second);
+      pool.idle_bytes -= pool.idle.front().first;
+      pool.idle.erase(pool.idle.begin());
+    }
+    // Reserve the bytes before allocating outside the lock.
+    pool.live_bytes += bytes;
   }
+  for (void* q : evict) (void)ando_host_free(q);
   void* p = nullptr;
-  if (ando_host_alloc(bytes, &p) != ANDO_OK) return nullptr;
-  std::lock_guard<std::mutex> g(pinned_pool().mu);
+  if (ando_host_alloc(bytes, &p) != ANDO_OK) {
+    std::lock_guard<std::mutex> g(pool.mu);
+    pool.live_bytes -= bytes;
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> g(pool.mu);
   pool.live[p] = bytes;
-  pool.live_bytes += bytes;
   return p;
 }
 void pinned_give(void* p) {
@@ -228,12 +242,19 @@ void pinned_give(void* p) {
     pool.live_bytes -= bytes;
     int same = 0;
     for (const auto& e : pool.idle) same += e.first == bytes;
-    if (same < 4) {
+    if (same < 4 && pool.live_bytes + pool.idle_bytes + bytes <= kPinnedCap) {
       pool.idle.push_back({bytes, p});
+      pool.idle_bytes += bytes;
       return;
     }
   }
   (void)ando_host_free(p);
+}
+// Bytes the pool holds (live, idle); test hook.
+std::pair<size_t, size_t> pinned_pool_bytes() {
+  PinnedPool& pool = pinned_pool();
+  std::lock_guard<std::mutex> g(pool.mu);
+  return {pool.live_bytes, pool.idle_bytes};
 }
 
 struct State {
@@ -974,6 +995,8 @@ PYBIND11_MODULE(ando_barrier_core, m) {
   m.doc() = "Ando 2024 Cubic Barrier with Elasticity-Inclusive Dynamic Stiffness (MI355X HIP backend)";
   m.def("version", []() { return std::string("ando_barrier_core v1.0.0"); }, "Get version string");
   m.def("backend", []() { return std::string("hip-gfx950"); }, "Name of the compute backend");
+  m.def("_pinned_pool_bytes", &pinned_pool_bytes,
+        "(live, idle) bytes of page-locked buffers behind get_positions / get_velocities arrays (test hook)");
 
   py::class_<Material>(m, "Material")
       .def(py::init<>())

@@ -123,21 +123,16 @@ struct BodyCache {
 
 using namespace ando;
 
-// Metadata of the block-parallel SpMV for one block pattern (see
-// k_pcg_spmv_blk): packed column / row offset / row-start per block, the first
-// row of each workgroup run, and for each row the wave holding its remainder.
+// Sliced-ELL metadata of one block pattern.
 struct SpmvMeta {
-  bool ok = false;  // pattern fits the packing (n < 2^23, rows of at most 64 blocks)
-  int groups = 0, nb = 0;
-  DBuf<uint32_t> meta;
-  DBuf<int32_t> run_row0, straddle;
-  DBuf<float> extra;
-  // Sliced-ELL layout of the same pattern (k_pcg_spmv_sell): per slice of 64
+  // Sliced-ELL layout of the pattern (k_pcg_spmv_sell): per slice of 64
   // rows its first slot position and width, per slot position its BSR block
   // (-1 = padding) and column, and room for the values.
   bool sell_ok = false;
   int slices = 0;
   int uniform_w = 0;  // the common slice width, or 0 when widths differ
+  long long slots = 0;        // slot positions over all slices (padding included)
+  uint64_t layout_hash = 0;   // FNV-1a of the slice offsets and widths
   DBuf<int32_t> slice_off, slice_w, sell_perm, sell_col;
   DBuf<float> sell_vals;
 };
@@ -180,41 +175,25 @@ void build_sell_meta(SpmvMeta& sm, int n, const std::vector<int32_t>& row_ptr, c
   for (int sl = 1; sl < S; ++sl)
     if (w[sl] != w[0]) sm.uniform_w = 0;
   sm.slices = S;
+  sm.slots = (long long)P;
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&h](int32_t v) {
+    for (int b = 0; b < 4; ++b) {
+      h ^= (uint64_t)((uint32_t)v >> (8 * b) & 0xffu);
+      h *= 1099511628211ull;
+    }
+  };
+  for (int sl = 0; sl < S; ++sl) {
+    mix(off[sl]);
+    mix(w[sl]);
+  }
+  sm.layout_hash = h;
   sm.sell_ok = true;
 }
 
 void build_spmv_meta(SpmvMeta& sm, int n, const std::vector<int32_t>& row_ptr, const std::vector<int32_t>& cols,
                      hipStream_t s) {
   build_sell_meta(sm, n, row_ptr, cols, s);
-  const int nb = row_ptr[n];
-  sm.ok = false;
-  if (n <= 0 || nb <= 0 || n >= (1 << ando::kMetaColBits)) return;
-  for (int i = 0; i < n; ++i)
-    if (row_ptr[i + 1] - row_ptr[i] > 64 || row_ptr[i + 1] <= row_ptr[i]) return;
-  const int G = nblk(nb, kBlock), waves = G * (kBlock / 64);
-  std::vector<int32_t> brow(nb);
-  for (int i = 0; i < n; ++i)
-    for (int b = row_ptr[i]; b < row_ptr[i + 1]; ++b) brow[b] = i;
-  std::vector<uint32_t> meta(nb);
-  std::vector<int32_t> row0(G), straddle(n, -1);
-  for (int g = 0; g < G; ++g) row0[g] = brow[(size_t)g * kBlock];
-  for (int b = 0; b < nb; ++b) {
-    const int delta = brow[b] - row0[b / kBlock];
-    if (delta > 255) return;
-    meta[b] = (uint32_t)cols[b] | ((uint32_t)delta << ando::kMetaColBits) |
-              (b == row_ptr[brow[b]] ? 0x80000000u : 0u);
-  }
-  for (int w = 0; w < waves; ++w) {
-    const int b = 64 * w;
-    if (b > 0 && b < nb && brow[b] == brow[b - 1]) straddle[brow[b]] = w;
-  }
-  sm.meta.upload(meta, s);
-  sm.run_row0.upload(row0, s);
-  sm.straddle.upload(straddle, s);
-  sm.extra.ensure(3 * (size_t)waves);
-  sm.groups = G;
-  sm.nb = nb;
-  sm.ok = true;
 }
 
 #include "learned_host.hip"
@@ -285,11 +264,7 @@ struct ando_ctx {
   ando_stats stats{};
   int fp64_near = 1;
   int kernel_timing = 0;
-  int pcg_persistent = 0;  // 1: one cooperative launch per solve, 0: two kernels per iteration
-  int pcg_wgs = 0;         // requested workgroups (0 = default)
-  int pcg_barrier = 0;     // persistent grid barrier: 0 cooperative-groups sync, 1 counter barrier
   int spmv_rowsets = 3;    // block rows per lane group of the row SpMV (1..4)
-  int spmv_blocks = 0;     // 1: block-parallel SpMV where the pattern allows it, 0: row SpMV
   int spmv_sell = 1;       // 1: sliced-ELL SpMV (values copied once per solve), 0: BSR kernels
   int sell_lanes = 2;      // ANDO_SELL_LANES: lanes per row of the sliced-ELL SpMV (1, 2 or 4)
   int sell_wg_slices = 1;  // ANDO_SELL_WG_SLICES: slices per workgroup with two lanes per row (0: a wave, 1, 2, 4)
@@ -298,7 +273,7 @@ struct ando_ctx {
   int tuned_n = -1;        // block rows the launch shape was tuned (or looked up) for
   int autotuned = 0;       // the launch shape in use came from the autotuner
   bool tuning = false;     // inside a trial solve of the autotuner
-  int spmv_env = -1;       // ANDO_PCG_SPMV: -1 unset, 0 row, 1 blk, 2 sell (overrides the mode flag)
+  int spmv_env = -1;       // ANDO_PCG_SPMV: -1 unset, 0 row, 2 sell (overrides the mode flag)
   SpmvMeta spmv_meta;      // for the mesh's block pattern
   std::mutex call_mutex;   // held by every entry point for the whole call (CtxLock)
   hipEvent_t batch_ev = nullptr;  // a batched solve's set-up is done on the context's stream
@@ -338,8 +313,6 @@ struct ando_ctx {
   DBuf<ando::ExtremaAcc> fx_part;
   DBuf<float> fx_out;
   uint64_t det_gen = 0, listed_gen = ~0ull;
-  DBuf<uint32_t> pcg_barrier_ctr;
-  int pcg_grid_cap = 0;    // co-resident workgroups (0 = not yet queried, -1 = no cooperative launch)
   std::vector<hipEvent_t> events;  // pool: start/stop pairs for timed launches
   ~ando_ctx() {
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
@@ -1015,73 +988,6 @@ struct BatchPcg {
   }
 };
 
-// Number of co-resident workgroups for the persistent PCG, a multiple of 8 so
-// that the XCD-contiguous row mapping holds.
-int persistent_grid(ando_ctx* c, int n) {
-  if (c->pcg_grid_cap == 0) {
-    hipDeviceProp_t prop;
-    HIPCHK(hipGetDeviceProperties(&prop, c->device));
-    int per_cu = 0;
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_pcg_persistent),
-                                                        kBlock, 0));
-    c->pcg_grid_cap = std::max(1, per_cu) * prop.multiProcessorCount;
-    if (!prop.cooperativeLaunch) c->pcg_grid_cap = -1;
-  }
-  if (c->pcg_grid_cap < 0) return 0;
-  int want = c->pcg_wgs > 0 ? c->pcg_wgs : 512;
-  int rows_per_wg_min = 32;
-  want = std::min(want, (n + rows_per_wg_min - 1) / rows_per_wg_min);
-  want = std::min(want, c->pcg_grid_cap);
-  if (want >= 8) want -= want % 8;
-  return std::max(want, 1);
-}
-
-PcgOut run_pcg_persistent(ando_ctx* c, int n, const float* A, const int32_t* rp, const int32_t* cols,
-                          const int32_t* diag, const float* rhs, int negate, float* xsol, float tol, int maxit,
-                          int G) {
-  hipStream_t s = c->stream;
-  c->part_rz.ensure(G + 1);
-  c->part_pAp.ensure(G + 1);
-  c->part_max.ensure(G + 1);
-  c->pcg_status.ensure(1);
-  c->pcg_barrier_ctr.ensure(1);
-  HIPCHK(hipMemsetAsync(c->pcg_barrier_ctr.p, 0, sizeof(uint32_t), s));
-  PcgPersistentArgs a;
-  a.barrier = c->pcg_barrier_ctr.p;
-  a.barrier_kind = c->pcg_barrier;
-  a.n = n; a.A = A; a.row_ptr = rp; a.cols = cols; a.diag = diag; a.rhs = rhs; a.negate = negate;
-  a.x = xsol; a.r = c->r.p; a.z = c->z.p; a.p0 = c->pbuf0.p; a.p1 = c->pbuf1.p; a.Ap = c->Ap.p; a.Pinv = c->Pinv.p;
-  a.part_a = c->part_rz.p; a.part_b = c->part_pAp.p; a.part_max = c->part_max.p; a.st = c->pcg_status.p;
-  a.tol = tol; a.maxit = maxit;
-  void* args[] = {&a};
-  const bool timing = c->kernel_timing != 0;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (timing) {
-    while (c->events.size() < 2) {
-      hipEvent_t e;
-      HIPCHK(hipEventCreate(&e));
-      c->events.push_back(e);
-    }
-    e0 = c->events[0];
-    e1 = c->events[1];
-    HIPCHK(hipEventRecord(e0, s));
-  }
-  HIPCHK(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pcg_persistent), dim3(G), dim3(kBlock), args, 0,
-                                    s));
-  if (timing) HIPCHK(hipEventRecord(e1, s));
-  PcgStatus st{};
-  HIPCHK(hipMemcpyAsync(&st, c->pcg_status.p, sizeof(PcgStatus), hipMemcpyDeviceToHost, s));
-  sync(c);
-  if (timing) {
-    float ms = 0.0f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    c->stats.timed_launches[0]++;
-    c->stats.timed_ms[0] += ms;
-    c->stats.timed_iterations += st.iters;
-  }
-  return PcgOut{st.iters, !st.failed};
-}
-
 void autotune_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int32_t* cols, const int32_t* diag,
                   const float* rhs, int negate, float* xsol, int maxit, SpmvMeta* sm, const ando::SelfView* sv);
 
@@ -1095,27 +1001,18 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
                const ando::SelfView* sv = nullptr) {
   hipStream_t s = c->stream;
   const bool couple = sv != nullptr && sv->ptr != nullptr;
-  if (c->pcg_persistent && P_given == nullptr && !couple) {
-    int G = persistent_grid(c, n);
-    if (G > 0) {
-      c->stats.pcg_kernel = 1;
-      return run_pcg_persistent(c, n, A, rp, cols, diag, rhs, negate, xsol, tol, maxit, G);
-    }
-  }
   const int R = c->spmv_rowsets;
   const bool sell = c->spmv_sell && sm != nullptr && sm->sell_ok;
-  const bool blk = !sell && c->spmv_blocks && sm != nullptr && sm->ok && !couple;
   if (c->pcg_autotune && !c->tuning && sell && c->tuned_n != n && P_given == nullptr && x0_given == nullptr &&
       !(t_batch && t_batch->device == c->device))
     autotune_pcg(c, n, A, rp, cols, diag, rhs, negate, xsol, maxit, sm, sv);
   const int sell_slices = (c->sell_lanes == 2) ? c->sell_wg_slices : 4;  // slices per SpMV workgroup (0: a wave)
   const int vb = c->pcg_vec_block;  // threads per workgroup of the vector kernels (init, update)
   int nwg_v = nblk(n, vb);
-  int nwg_r = sell ? (sell_slices == 0 ? 2 * sm->slices : nblk(sm->slices, sell_slices))
-                   : (blk ? sm->groups : spmv_groups(n, R));
+  int nwg_r = sell ? (sell_slices == 0 ? 2 * sm->slices : nblk(sm->slices, sell_slices)) : spmv_groups(n, R);
   c->part_rz.ensure((size_t)nwg_v + 1);
   c->part_max.ensure((size_t)nwg_v + 1);
-  c->stats.pcg_kernel = sell ? 3 : (blk ? 2 : 0);
+  c->stats.pcg_kernel = sell ? 3 : 0;
   c->stats.spmv_lanes = sell ? c->sell_lanes : 0;
   c->stats.spmv_wg_slices = sell ? sell_slices : 0;
   c->stats.pcg_vec_block = vb;
@@ -1230,11 +1127,6 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
         else if (c->sell_lanes == 2) sell_launch(k_pcg_spmv_sell_split<2>, 2 * kBlock);
         else if (c->sell_lanes == 4) sell_launch(k_pcg_spmv_sell_split<4>, 4 * kBlock);
         else sell_launch(k_pcg_spmv_sell, kBlock);
-      } else if (blk) {
-        hipExtLaunchKernelGGL(k_pcg_spmv_blk, dim3(nwg_r), dim3(kBlock), 0, s, e0, e1, 0, sm->nb, it, A,
-                              (const uint32_t*)sm->meta.p, (const int32_t*)sm->run_row0.p, (const float*)c->z.p,
-                              (const float*)pold, pcur, c->Ap.p, sm->extra.p, (const double*)c->part_rz.p,
-                              (const float*)c->part_max.p, nwg_v, c->rz_hist.p, c->part_pAp.p, c->pcg_status.p, tol);
       } else {
         switch (R) {
           case 1: spmv(k_pcg_spmv<1>); break;
@@ -1248,9 +1140,7 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
                            (const int32_t*)c->ccol.p, (const float*)c->cval.p, (const PcgStatus*)c->pcg_status.p,
                            (const float*)pcur, c->Ap.p, c->part_pAp2.p);
       hipExtLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(vb), 0, s, e2, e3, 0, n, it, (const float*)pcur,
-                            (const float*)c->Ap.p, (const float*)(blk ? sm->extra.p : nullptr),
-                            (const int32_t*)(blk ? sm->straddle.p : nullptr), Pc, pe, pi, xsol, c->r.p,
-                            c->z.p,
+                            (const float*)c->Ap.p, Pc, pe, pi, xsol, c->r.p, c->z.p,
                             (const double*)c->part_pAp.p, nwg_r, (const double*)c->rz_hist.p, c->part_rz.p,
                             c->part_max.p, c->pcg_status.p, (const double*)(couple ? c->part_pAp2.p : nullptr),
                             nblk(n, 32));
@@ -1295,8 +1185,8 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
 // per vector workgroup -- on the system at hand, and every SpMV and update launch
 // is timed with events recorded by the dispatch.  The shape with the least kernel
 // time per iteration (best of two trials) is kept for the context and, through a
-// process-wide table keyed by (device, block rows), for every later context of
-// that size.  The default shape stays unless another is at least 3% faster, so
+// process-wide table keyed by the device and the sliced-ELL structure, for every
+// later context with the same structure.  The default shape stays unless another is at least 3% faster, so
 // timing noise between near-equal shapes does not move it.  The lanes alone
 // change no result; the grid and the vector block change the association of the
 // p.Ap and r.z sums, so results agree with the default shape to rounding.  The
@@ -1307,8 +1197,18 @@ struct PcgShape {
 };
 constexpr PcgShape kPcgShapes[] = {{2, 1, kBlock}, {2, 0, kBlock}, {2, 2, kBlock}, {2, 4, kBlock},
                                    {4, 4, kBlock}, {1, 4, kBlock}, {2, 1, 128},    {2, 1, 64}};
+// A tuned shape is reused only for the same device and the same sliced-ELL
+// structure (rows, slices, slot count and slice layout), not merely the same size.
+struct TuneKey {
+  int device, n, slices;
+  long long slots;
+  uint64_t layout;
+  bool operator==(const TuneKey& o) const {
+    return device == o.device && n == o.n && slices == o.slices && slots == o.slots && layout == o.layout;
+  }
+};
 std::mutex g_tune_mu;
-std::vector<std::pair<std::pair<int, int>, PcgShape>> g_tune_table;
+std::vector<std::pair<TuneKey, PcgShape>> g_tune_table;
 
 void set_shape(ando_ctx* c, const PcgShape& sh) {
   c->sell_lanes = sh.lanes;
@@ -1318,7 +1218,7 @@ void set_shape(ando_ctx* c, const PcgShape& sh) {
 
 void autotune_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int32_t* cols, const int32_t* diag,
                   const float* rhs, int negate, float* xsol, int maxit, SpmvMeta* sm, const ando::SelfView* sv) {
-  const std::pair<int, int> key(c->device, n);
+  const TuneKey key{c->device, n, sm->slices, sm->slots, sm->layout_hash};
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
     for (const auto& e : g_tune_table)
@@ -1332,9 +1232,27 @@ void autotune_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const i
   const int K = std::min(maxit, 32);
   if (K < 4) return;
   constexpr int NS = (int)(sizeof(kPcgShapes) / sizeof(kPcgShapes[0]));
-  const int saved_timing = c->kernel_timing, saved_chunk = c->pcg_chunk;
-  const int32_t tl[2] = {c->stats.timed_launches[0], c->stats.timed_launches[1]};
-  const double tm[2] = {c->stats.timed_ms[0], c->stats.timed_ms[1]};
+  // The trials switch the context into timing mode; this guard puts the
+  // context back (mode, chunk, timed statistics, shape) on every exit,
+  // including a trial that throws.
+  struct Restore {
+    ando_ctx* c;
+    int timing, chunk;
+    int32_t tl[2];
+    double tm[2];
+    PcgShape shape;
+    ~Restore() {
+      c->tuning = false;
+      c->kernel_timing = timing;
+      c->pcg_chunk = chunk;
+      c->stats.timed_launches[0] = tl[0];
+      c->stats.timed_launches[1] = tl[1];
+      c->stats.timed_ms[0] = tm[0];
+      c->stats.timed_ms[1] = tm[1];
+      set_shape(c, shape);
+    }
+  } restore{c, c->kernel_timing, c->pcg_chunk, {c->stats.timed_launches[0], c->stats.timed_launches[1]},
+            {c->stats.timed_ms[0], c->stats.timed_ms[1]}, PcgShape{c->sell_lanes, c->sell_wg_slices, c->pcg_vec_block}};
   c->tuning = true;
   c->kernel_timing = 1;
   c->pcg_chunk = K;  // one chunk: no status read-back inside the trial
@@ -1350,18 +1268,11 @@ void autotune_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const i
         us[k] = std::min(us[k], 1e3 * (c->stats.timed_ms[0] + c->stats.timed_ms[1]) / c->stats.timed_launches[0]);
     }
   }
-  c->tuning = false;
-  c->kernel_timing = saved_timing;
-  c->pcg_chunk = saved_chunk;
-  c->stats.timed_launches[0] = tl[0];
-  c->stats.timed_launches[1] = tl[1];
-  c->stats.timed_ms[0] = tm[0];
-  c->stats.timed_ms[1] = tm[1];
   int best = 0;
   for (int k = 1; k < NS; ++k)
     if (us[k] < us[best]) best = k;
   if (!(us[best] < 0.97 * us[0])) best = 0;
-  set_shape(c, kPcgShapes[best]);
+  restore.shape = kPcgShapes[best];
   if (!std::isfinite(us[0])) return;  // nothing ran (b = 0): try again on the next solve
   c->tuned_n = n;
   c->autotuned = 1;
@@ -1684,12 +1595,10 @@ int ando_create(int32_t device, ando_ctx** out) {
       delete c;
       throw Fail(ANDO_EDEVICE, "pinned status or event allocation failed");
     }
-    if (const char* w = std::getenv("ANDO_PCG_WGS")) c->pcg_wgs = std::atoi(w);
     if (const char* w = std::getenv("ANDO_PCG_CHUNK")) c->pcg_chunk = std::min(256, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_CHUNK_MAX"))
       c->pcg_chunk_max = std::min(1024, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_RIGID_GROUP")) c->rigid_group = std::min(256, std::max(1, std::atoi(w)));
-    if (const char* w = std::getenv("ANDO_PCG_BARRIER")) c->pcg_barrier = std::atoi(w) == 1 ? 1 : 0;
     if (const char* w = std::getenv("ANDO_PCG_VEC_BLOCK")) {
       const int k = std::atoi(w);
       c->pcg_vec_block = (k == 64 || k == 128) ? k : kBlock;
@@ -1705,8 +1614,7 @@ int ando_create(int32_t device, ando_ctx** out) {
     if (const char* w = std::getenv("ANDO_PCG_AUTOTUNE")) c->pcg_autotune = std::min(2, std::max(0, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_ROWSETS")) c->spmv_rowsets = std::min(4, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_SPMV")) {
-      c->spmv_env = std::strcmp(w, "blk") == 0 ? 1 : (std::strcmp(w, "sell") == 0 ? 2 : 0);
-      c->spmv_blocks = c->spmv_env == 1 ? 1 : 0;
+      c->spmv_env = std::strcmp(w, "sell") == 0 ? 2 : 0;
       c->spmv_sell = c->spmv_env == 2 ? 1 : 0;
     }
     *out = c;
@@ -1924,7 +1832,7 @@ int ando_set_mode(ando_ctx* c, int32_t flags) {
   return guarded([&] {
     CtxLock lock(c);
     need(c != nullptr, ANDO_EINVAL, "null context");
-    need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING | ANDO_MODE_PCG_PERSISTENT |
+    need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING |
                     ANDO_MODE_LEARNED_PRECOND | ANDO_MODE_LEARNED_WARM_START_ONLY |
                     ANDO_MODE_CORRECTED_CONTACT | ANDO_MODE_PCG_ROW_SPMV | ANDO_MODE_STATIC_RIGIDS |
                     ANDO_MODE_ADDITIVE_CCD)) == 0,
@@ -1933,7 +1841,6 @@ int ando_set_mode(ando_ctx* c, int32_t flags) {
          "learned preconditioner mode needs ando_set_precond_weights first");
     c->fp64_near = (flags & ANDO_MODE_FP64_NEAR_CONTACT) ? 1 : 0;
     c->kernel_timing = (flags & ANDO_MODE_KERNEL_TIMING) ? 1 : 0;
-    c->pcg_persistent = (flags & ANDO_MODE_PCG_PERSISTENT) ? 1 : 0;
     c->learned = (flags & ANDO_MODE_LEARNED_PRECOND) ? ((flags & ANDO_MODE_LEARNED_WARM_START_ONLY) ? 2 : 1) : 0;
     c->corrected = (flags & ANDO_MODE_CORRECTED_CONTACT) ? 1 : 0;
     if (c->spmv_env < 0) c->spmv_sell = (flags & ANDO_MODE_PCG_ROW_SPMV) ? 0 : 1;

@@ -13,7 +13,6 @@
 // Every workgroup reduces the partial arrays itself in a fixed order, so all
 // workgroups reach the same scalar and the same decision without atomics.
 // A device-side done flag turns the remaining launches of a chunk into no-ops.
-#include <hip/hip_cooperative_groups.h>
 
 #include "kernels.h"
 
@@ -819,135 +818,10 @@ __global__ void __launch_bounds__(64 * (S == 0 ? 1 : S * L)) k_pcg_spmv_sell_spl
   sell_split_body<L, S>(xcd_logical_group(blockIdx.x, gridDim.x), n, slices, it, vals, scol, slice_off, slice_w, uniform_w, z, p_old, p_new, Ap, part_rz, part_max, nwg_v, rz_hist, part_pAp, st, tol);
 }
 
-// Block-parallel SpMV: one 3x3 block per lane, workgroups over contiguous
-// runs of kBlock blocks, so a wave reads 64 consecutive blocks (2304 bytes)
-// and no lane waits on a row pointer.  Per block, meta packs
-//   bits 0..22  column j,
-//   bits 23..30 row minus the first row of the workgroup's run,
-//   bit  31     the block is the first of its row.
-// A lane's three products are summed per row by a segmented suffix scan
-// inside the wave (rows are runs of lanes).  The lane that heads a segment
-// holds its sum: a row's first segment goes to Ap[row]; a row that continues
-// from the previous wave (at most one per wave, in lane 0; rows have at most
-// 64 blocks) stores its remainder in Ap_extra[wave] and the update kernel adds
-// it.  Workgroups are numbered XCD-contiguously (hardware places workgroup w
-// on XCD w % 8), so neighbouring runs share an L2.
-constexpr int kMetaColBits = 23;
-constexpr uint32_t kMetaColMask = (1u << kMetaColBits) - 1u;
-
-__global__ void __launch_bounds__(kBlock) k_pcg_spmv_blk(int nb, int it, const float* __restrict__ A,
-                           const uint32_t* __restrict__ meta, const int32_t* __restrict__ run_row0,
-                           const float* __restrict__ z, const float* __restrict__ p_old, float* __restrict__ p_new,
-                           float* __restrict__ Ap, float* __restrict__ Ap_extra, const double* __restrict__ part_rz,
-                           const float* __restrict__ part_max, int nwg_v, double* __restrict__ rz_hist,
-                           double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol) {
-  __shared__ double ldsd[4];
-  __shared__ float ldsf[4];
-  const int g = xcd_logical_group(blockIdx.x, gridDim.x);
-  const int b = g * kBlock + threadIdx.x;
-  const bool act = b < nb;
-  const int bl = act ? b : nb - 1;
-  const uint32_t mt = meta[bl];  // in flight while the done flag is read
-  const int row0 = run_row0[g];
-  if (st->done) return;
-  float Ab[9];
-#pragma unroll
-  for (int e = 0; e < 9; ++e) Ab[e] = A[9 * (size_t)bl + e];
-  const int j = (int)(mt & kMetaColMask);
-  const int row = row0 + (int)((mt >> kMetaColBits) & 0xffu);
-  const bool first = (mt >> 31) != 0u;
-  float po[3], zo[3], pr[3], zr[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    po[c] = p_old[3 * j + c];
-    pr[c] = p_old[3 * row + c];
-    zo[c] = it > 0 ? z[3 * j + c] : 0.0f;
-    zr[c] = it > 0 ? z[3 * row + c] : 0.0f;
-  }
-  float beta = 0.0f;
-  if (it > 0) {
-    const float bmax = st->bmax;
-    const double rz_prev = rz_hist[it - 1];
-    double rz = reduce_sum_partials(part_rz, nwg_v, ldsd);
-    float rmax = reduce_max_partials(part_max, nwg_v, ldsf);
-    float rel = ((double)bmax < 1e-16) ? rmax : rmax / bmax;
-    if (rel < tol) {
-      if (threadIdx.x == 0) {
-        st->rel = rel;
-        st->iters = it;
-        st->done = 1;
-      }
-      return;
-    }
-    beta = (float)(rz / rz_prev);
-    if (g == 0 && threadIdx.x == 0) rz_hist[it] = rz;
-  }
-  float pv0, pv1, pv2;
-  if (it > 0) {
-    pv0 = zo[0] + beta * po[0];
-    pv1 = zo[1] + beta * po[1];
-    pv2 = zo[2] + beta * po[2];
-  } else {
-    pv0 = po[0]; pv1 = po[1]; pv2 = po[2];
-  }
-  float s0 = Ab[0] * pv0 + (Ab[1] * pv1 + Ab[2] * pv2);
-  float s1 = Ab[3] * pv0 + (Ab[4] * pv1 + Ab[5] * pv2);
-  float s2 = Ab[6] * pv0 + (Ab[7] * pv1 + Ab[8] * pv2);
-  // Inactive tail lanes get distinct negative rows, so they join no segment.
-  const int lane = threadIdx.x & 63;
-  const int key = act ? row : -1 - lane;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const float u0 = __shfl_down(s0, off, 64);
-    const float u1 = __shfl_down(s1, off, 64);
-    const float u2 = __shfl_down(s2, off, 64);
-    const int ku = __shfl_down(key, off, 64);
-    if (lane + off < 64 && ku == key) {
-      s0 += u0;
-      s1 += u1;
-      s2 += u2;
-    }
-  }
-  const int kprev = __shfl_up(key, 1, 64);
-  const bool head = act && (lane == 0 || kprev != key);
-  double pap = 0.0;
-  if (head) {
-    float q0, q1, q2;
-    if (it > 0) {
-      q0 = zr[0] + beta * pr[0];
-      q1 = zr[1] + beta * pr[1];
-      q2 = zr[2] + beta * pr[2];
-    } else {
-      q0 = pr[0]; q1 = pr[1]; q2 = pr[2];
-    }
-    if (first) {
-      Ap[3 * row] = s0;
-      Ap[3 * row + 1] = s1;
-      Ap[3 * row + 2] = s2;
-      if (it > 0) {
-        p_new[3 * row] = q0;
-        p_new[3 * row + 1] = q1;
-        p_new[3 * row + 2] = q2;
-      }
-    } else {
-      const int wave = g * (kBlock / 64) + (int)(threadIdx.x >> 6);
-      Ap_extra[3 * wave] = s0;
-      Ap_extra[3 * wave + 1] = s1;
-      Ap_extra[3 * wave + 2] = s2;
-    }
-    pap = (double)q0 * s0 + (double)q1 * s1 + (double)q2 * s2;
-  }
-  double s = block_reduce(pap, ldsd, [](double a, double b2) { return a + b2; });
-  if (threadIdx.x == 0) part_pAp[g] = s;
-}
-
 // alpha = r.z / p.Ap, x += alpha p, r -= alpha Ap, z = P^-1 r; one thread per
 // vertex.  The vertex's operands are loaded before the p.Ap reduction.
-// With the block-parallel SpMV, straddle[i] >= 0 names the wave whose
-// Ap_extra entry holds the rest of row i (straddle == nullptr otherwise).
 __device__ __forceinline__ void update_body(int wg, int n, int it, const float* __restrict__ p,
-                             const float* __restrict__ Ap, const float* __restrict__ Ap_extra,
-                             const int32_t* __restrict__ straddle, const float* __restrict__ Pinv, int pe,
+                             const float* __restrict__ Ap, const float* __restrict__ Pinv, int pe,
                              int pi, float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
                              const double* __restrict__ part_pAp, int nwg_r, const double* __restrict__ rz_hist,
                              double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st,
@@ -976,13 +850,6 @@ __device__ __forceinline__ void update_body(int wg, int n, int it, const float* 
   if (st->done) return;
   const double pAp_t = finish_sum_partials(part_pAp, nwg_r, pre);
   const double pAp2_t = part_pAp2 ? thread_sum_partials(part_pAp2, n2) : 0.0;
-  if (straddle != nullptr) {
-    const int w = straddle[ii];
-    if (w >= 0) {
-#pragma unroll
-      for (int k = 0; k < 3; ++k) av[k] = av[k] + Ap_extra[3 * w + k];
-    }
-  }
   const double rz_it = rz_hist[it];
   double pAp = block_reduce(pAp_t, ldsd, [](double a, double b) { return a + b; });
   if (part_pAp2) pAp += block_reduce(pAp2_t, ldsd, [](double a, double b) { return a + b; });
@@ -1023,278 +890,14 @@ __device__ __forceinline__ void update_body(int wg, int n, int it, const float* 
 }
 
 __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const float* __restrict__ p,
-                             const float* __restrict__ Ap, const float* __restrict__ Ap_extra,
-                             const int32_t* __restrict__ straddle, const float* __restrict__ Pinv, int pe,
+                             const float* __restrict__ Ap, const float* __restrict__ Pinv, int pe,
                              int pi, float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
                              const double* __restrict__ part_pAp, int nwg_r, const double* __restrict__ rz_hist,
                              double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st,
                              const double* __restrict__ part_pAp2, int n2) {
-  update_body(blockIdx.x, n, it, p, Ap, Ap_extra, straddle, Pinv, pe, pi, x, r, z, part_pAp, nwg_r, rz_hist, part_rz, part_max, st, part_pAp2, n2);
+  update_body(blockIdx.x, n, it, p, Ap, Pinv, pe, pi, x, r, z, part_pAp, nwg_r, rz_hist, part_rz, part_max, st, part_pAp2, n2);
 }
 
-// ---------------------------------------------------------------------------
-// Persistent PCG: the whole solve in one cooperative launch.
-//
-// Every workgroup owns a contiguous range of block rows (and the same range of
-// vertices in the vector update), numbered so that consecutive ranges sit on
-// the same XCD: hardware dispatches workgroup w to XCD w % 8, and the logical
-// range index is (w % 8) * (G / 8) + w / 8.  Neighbouring rows read the same p
-// and z entries, so an XCD's L2 serves most of those reads.
-//
-// A block row is handled by 8 lanes, one 3x3 block per lane per pass, so the
-// 36-byte blocks of a row are read as one contiguous run; the three row sums
-// are combined by an xor-shuffle tree inside the 8-lane group.
-// Per iteration: SpMV + partial p.Ap | grid barrier | alpha, x/r/z update,
-// partial r.z and max|r| | grid barrier | convergence test and beta.
-// ---------------------------------------------------------------------------
-struct PcgPersistentArgs {
-  unsigned int* barrier;  // monotonically increasing arrival counter, zeroed before the launch
-  int barrier_kind;       // 0: cooperative-groups grid sync, 1: counter barrier
-  int n;
-  const float* A;
-  const int32_t* row_ptr;
-  const int32_t* cols;
-  const int32_t* diag;
-  const float* rhs;
-  int negate;
-  float* x;
-  float* r;
-  float* z;
-  float* p0;
-  float* p1;
-  float* Ap;
-  float* Pinv;
-  double* part_a;   // r.z partials (one per workgroup)
-  double* part_b;   // p.Ap partials
-  float* part_max;  // max |r| partials
-  PcgStatus* st;
-  float tol;
-  int maxit;
-};
-
-
-__device__ inline void wg_range(int n, int& lo, int& hi) {
-  const int G = gridDim.x, w = blockIdx.x;
-  const int xcds = (G % 8 == 0) ? 8 : 1;
-  const int logical = (w % xcds) * (G / xcds) + w / xcds;
-  const long long per = (n + (long long)G - 1) / G;
-  lo = (int)min((long long)n, logical * per);
-  hi = (int)min((long long)n, lo + per);
-}
-
-// Grid barrier for a co-resident grid (the launch is cooperative, so every
-// workgroup is resident and every wait ends).  After the workgroup barrier
-// (which waits for the workgroup's stores), thread 0 counts the workgroup in
-// with an agent-scope release, spins with relaxed loads until all G workgroups
-// of this generation have arrived, and then takes one agent-scope acquire
-// fence; the counter only grows within a launch.
-__device__ inline void counter_barrier(unsigned int* counter, unsigned int target) {
-  __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have reached L2
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target)
-      __builtin_amdgcn_s_sleep(1);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  }
-  __syncthreads();
-}
-
-__global__ void __launch_bounds__(kBlock) k_pcg_persistent(PcgPersistentArgs a) {
-  cooperative_groups::grid_group grid = cooperative_groups::this_grid();
-  __shared__ double ldsd[4];
-  __shared__ float ldsf[4];
-  const int G = gridDim.x;
-  unsigned int generation = 0;
-  auto grid_sync = [&]() {
-    if (a.barrier_kind == 1) {
-      ++generation;
-      counter_barrier(a.barrier, generation * (unsigned int)G);
-    } else {
-      grid.sync();
-    }
-  };
-  int lo, hi;
-  wg_range(a.n, lo, hi);
-
-  // Initialisation: preconditioner, r = b, x = 0, z = P^-1 r, p = z.
-  double rz_part = 0.0;
-  float bmax_part = 0.0f;
-  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-    const float* B = a.A + 9 * (size_t)a.diag[i];
-    float P[9];
-    float det = det3(B);
-    if ((double)fabsf(det) > 1e-16) {
-      float c0 = cof3(B, 0, 0), c1 = cof3(B, 1, 0), c2 = cof3(B, 2, 0);
-      float d = c0 * B[0] + (c1 * B[3] + c2 * B[6]);
-      float inv = 1.0f / d;
-      for (int q = 0; q < 3; ++q)
-        for (int b = 0; b < 3; ++b) P[3 * q + b] = cof3(B, b, q) * inv;
-    } else {
-      for (int e = 0; e < 9; ++e) P[e] = (e % 4 == 0) ? 1.0f : 0.0f;
-    }
-    for (int e = 0; e < 9; ++e) a.Pinv[9 * (size_t)i + e] = P[e];
-    float rv[3];
-    for (int k = 0; k < 3; ++k) {
-      float bk = a.negate ? -a.rhs[3 * i + k] : a.rhs[3 * i + k];
-      rv[k] = bk - 0.0f;
-      a.r[3 * i + k] = rv[k];
-      a.x[3 * i + k] = 0.0f;
-      bmax_part = fmaxf(bmax_part, fabsf(bk));
-    }
-    for (int k = 0; k < 3; ++k) {
-      float zk = P[3 * k] * rv[0] + (P[3 * k + 1] * rv[1] + P[3 * k + 2] * rv[2]);
-      a.z[3 * i + k] = zk;
-      a.p0[3 * i + k] = zk;
-      rz_part += (double)rv[k] * (double)zk;
-    }
-  }
-  {
-    double s = block_reduce(rz_part, ldsd, [](double u, double v) { return u + v; });
-    float m = block_reduce(bmax_part, ldsf, [](float u, float v) { return fmaxf(u, v); });
-    if (threadIdx.x == 0) {
-      a.part_a[blockIdx.x] = s;
-      a.part_max[blockIdx.x] = m;
-    }
-  }
-  grid_sync();
-  double rz = reduce_sum_partials(a.part_a, G, ldsd);
-  const float bmax = reduce_max_partials(a.part_max, G, ldsf);
-  {
-    float rel = ((double)bmax < 1e-16) ? bmax : bmax / bmax;
-    if (rel < a.tol) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.st->bmax = bmax; a.st->rel = rel; a.st->iters = 0; a.st->failed = 0; a.st->done = 1;
-      }
-      return;
-    }
-  }
-
-  const int lane = threadIdx.x % kLanesPerRow;
-  const int slot = threadIdx.x / kLanesPerRow;
-  float beta = 0.0f;
-  float* pold = a.p0;
-  float* pnew = a.p1;
-  for (int it = 0; it < a.maxit; ++it) {
-    const float* pcur = it > 0 ? pnew : a.p0;
-    // SpMV with p = z + beta p_old formed on the fly (it > 0).
-    double pap_part = 0.0;
-    for (int base = lo; base < hi; base += kRowsPerPass) {
-      int i = base + slot;
-      float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
-      if (i < hi) {
-        for (int bb = a.row_ptr[i] + lane; bb < a.row_ptr[i + 1]; bb += kLanesPerRow) {
-          int j = a.cols[bb];
-          float pv0, pv1, pv2;
-          if (it > 0) {
-            pv0 = a.z[3 * j] + beta * pold[3 * j];
-            pv1 = a.z[3 * j + 1] + beta * pold[3 * j + 1];
-            pv2 = a.z[3 * j + 2] + beta * pold[3 * j + 2];
-            if (j == i) {
-              pnew[3 * i] = pv0;
-              pnew[3 * i + 1] = pv1;
-              pnew[3 * i + 2] = pv2;
-            }
-          } else {
-            pv0 = pold[3 * j];
-            pv1 = pold[3 * j + 1];
-            pv2 = pold[3 * j + 2];
-          }
-          const float* Ab = a.A + 9 * (size_t)bb;
-          s0 = s0 + (Ab[0] * pv0 + (Ab[1] * pv1 + Ab[2] * pv2));
-          s1 = s1 + (Ab[3] * pv0 + (Ab[4] * pv1 + Ab[5] * pv2));
-          s2 = s2 + (Ab[6] * pv0 + (Ab[7] * pv1 + Ab[8] * pv2));
-        }
-      }
-      for (int off = kLanesPerRow / 2; off > 0; off >>= 1) {
-        s0 += __shfl_xor(s0, off, kLanesPerRow);
-        s1 += __shfl_xor(s1, off, kLanesPerRow);
-        s2 += __shfl_xor(s2, off, kLanesPerRow);
-      }
-      if (i < hi && lane == 0) {
-        a.Ap[3 * i] = s0;
-        a.Ap[3 * i + 1] = s1;
-        a.Ap[3 * i + 2] = s2;
-        float q0, q1, q2;
-        if (it > 0) {
-          q0 = a.z[3 * i] + beta * pold[3 * i];
-          q1 = a.z[3 * i + 1] + beta * pold[3 * i + 1];
-          q2 = a.z[3 * i + 2] + beta * pold[3 * i + 2];
-        } else {
-          q0 = pold[3 * i]; q1 = pold[3 * i + 1]; q2 = pold[3 * i + 2];
-        }
-        pap_part += (double)q0 * s0 + (double)q1 * s1 + (double)q2 * s2;
-      }
-    }
-    {
-      double s = block_reduce(pap_part, ldsd, [](double u, double v) { return u + v; });
-      if (threadIdx.x == 0) a.part_b[blockIdx.x] = s;
-    }
-    grid_sync();
-    double pAp = reduce_sum_partials(a.part_b, G, ldsd);
-    if (fabs(pAp) < 1e-16) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.st->bmax = bmax; a.st->iters = it + 1; a.st->failed = 1; a.st->done = 1;
-      }
-      return;
-    }
-    const float alpha = (float)(rz / pAp);
-    const float* p = it > 0 ? pnew : a.p0;
-    double rz_p = 0.0;
-    float rm = 0.0f;
-    for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-      float rv[3];
-      for (int k = 0; k < 3; ++k) {
-        int e = 3 * i + k;
-        a.x[e] = a.x[e] + alpha * p[e];
-        rv[k] = a.r[e] - alpha * a.Ap[e];
-        a.r[e] = rv[k];
-        rm = fmaxf(rm, fabsf(rv[k]));
-      }
-      const float* P = a.Pinv + 9 * (size_t)i;
-      for (int k = 0; k < 3; ++k) {
-        float zk = P[3 * k] * rv[0] + (P[3 * k + 1] * rv[1] + P[3 * k + 2] * rv[2]);
-        a.z[3 * i + k] = zk;
-        rz_p += (double)rv[k] * (double)zk;
-      }
-      if (rm != rm) rm = __int_as_float(0x7f800000);
-    }
-    {
-      double s = block_reduce(rz_p, ldsd, [](double u, double v) { return u + v; });
-      float m = block_reduce(rm, ldsf, [](float u, float v) { return fmaxf(u, v); });
-      if (threadIdx.x == 0) {
-        a.part_a[blockIdx.x] = s;
-        a.part_max[blockIdx.x] = m;
-      }
-    }
-    grid_sync();
-    double rz_new = reduce_sum_partials(a.part_a, G, ldsd);
-    float rmax = reduce_max_partials(a.part_max, G, ldsf);
-    float rel = ((double)bmax < 1e-16) ? rmax : rmax / bmax;
-    if (rel < a.tol) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.st->bmax = bmax; a.st->rel = rel; a.st->iters = it + 1; a.st->failed = 0; a.st->done = 1;
-      }
-      return;
-    }
-    beta = (float)(rz_new / rz);
-    rz = rz_new;
-    // p_new of this iteration becomes p_old of the next.
-    if (it > 0) {
-      float* t = pold;
-      pold = pnew;
-      pnew = t;
-    } else {
-      pold = a.p0;
-      pnew = a.p1;
-    }
-  }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    a.st->bmax = bmax; a.st->iters = a.maxit; a.st->failed = 1; a.st->done = 1;
-  }
-}
-
-// Convergence test after the last permitted iteration (pcg_solver.cpp:55-74).
 // ---------------------------------------------------------------------------
 // Batched PCG (ando_step_batch): the iterations of many independent systems in
 // one SpMV launch and one update launch.  Workgroup w of a batched grid runs
@@ -1345,7 +948,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update_batch(const PcgSys* __res
                                                             const int32_t* __restrict__ wg_sys, int it) {
   const PcgSys& S = sys[wg_sys[blockIdx.x]];
   float* pcur = (it & 1) ? S.p1 : S.p0;
-  update_body((int)blockIdx.x - S.wg0_v, S.n, it, pcur, S.Ap, nullptr, nullptr, S.Pinv, S.pe, S.pi, S.x, S.r, S.z,
+  update_body((int)blockIdx.x - S.wg0_v, S.n, it, pcur, S.Ap, S.Pinv, S.pe, S.pi, S.x, S.r, S.z,
               S.part_pAp, S.nwg_r, S.rz_hist, S.part_rz, S.part_max, S.st, nullptr, 0);
 }
 
@@ -1374,6 +977,7 @@ __global__ void k_pcg_status_gather(int nsys, const PcgSys* __restrict__ sys, Pc
   if (s < nsys) out[s] = *sys[s].st;
 }
 
+// Convergence test after the last permitted iteration (pcg_solver.cpp:55-74).
 __global__ void k_pcg_final(int it, const float* __restrict__ part_max, int nwg_v, PcgStatus* __restrict__ st,
                             float tol) {
   if (st->done) return;

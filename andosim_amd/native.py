@@ -7,8 +7,10 @@ to a CPU implementation.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -93,6 +95,14 @@ class Stats(C.Structure):
 
 
 _lib = None
+# Every live context and trainer, released by _close_all at interpreter exit
+# (before module teardown), so no device object outlives the interpreter.
+_live = weakref.WeakSet()
+
+
+def _close_all():
+    for obj in list(_live):
+        obj.close()
 
 
 def load():
@@ -149,6 +159,7 @@ def load():
     L.ando_step_batch.argtypes = [C.POINTER(C.c_void_p), C.POINTER(Params), C.POINTER(C.POINTER(RigidBody)),
                                   C.POINTER(C.c_int32), C.c_int32, C.c_int32]
     _lib = L
+    atexit.register(_close_all)
     return L
 
 
@@ -272,18 +283,26 @@ class GpuSim:
 
     def __init__(self, scene=None, device: int = 0, fp64_near: bool = True, kernel_timing: bool = False):
         self.L = load()
+        self.h = None
         h = C.c_void_p()
         check(self.L.ando_create(device, C.byref(h)))
         self.h = h
+        _live.add(self)
         self.bodies = None
         self.nbodies = 0
         self.set_mode(fp64_near, kernel_timing)
         if scene is not None:
             self.load(scene)
 
+    def close(self):
+        """Release the device context (idempotent)."""
+        h, self.h = getattr(self, "h", None), None
+        if h is not None:
+            self.L.ando_destroy(h)
+
     def __del__(self):
         try:
-            self.L.ando_destroy(self.h)
+            self.close()
         except Exception:
             pass
 
@@ -313,10 +332,10 @@ class GpuSim:
         self.gravity = _f(sc.gravity)
         self.dt = float(sc.params["dt"])
 
-    def set_mode(self, fp64_near: bool = True, kernel_timing: bool = False, pcg_persistent: bool = False,
+    def set_mode(self, fp64_near: bool = True, kernel_timing: bool = False,
                  learned: bool = False, warm_start_only: bool = False, corrected: bool = False,
                  row_spmv: bool = False, static_rigids: bool = False, additive_ccd: bool = False):
-        flags = ((1 if fp64_near else 0) | (2 if kernel_timing else 0) | (4 if pcg_persistent else 0) |
+        flags = ((1 if fp64_near else 0) | (2 if kernel_timing else 0) |
                  (8 if learned else 0) | (16 if learned and warm_start_only else 0) | (32 if corrected else 0) |
                  (64 if row_spmv else 0) | (128 if static_rigids else 0) | (256 if additive_ccd else 0))
         check(self.L.ando_set_mode(self.h, flags))
@@ -453,16 +472,25 @@ class Trainer:
 
     def __init__(self, device: int = 0):
         self.L = load()
+        self.h = None
         h = C.c_void_p()
         check(self.L.ando_trainer_create(device, C.byref(h)))
         self.h = h
+        _live.add(self)
         self.nw = nn_num_weights()
         self.sizes = []
 
+    def close(self):
+        """Release the trainer's device memory (idempotent)."""
+        h, self.h = getattr(self, "h", None), None
+        if h is not None:
+            self.L.ando_trainer_destroy(h)
+
     def __del__(self):
-        if getattr(self, "h", None) is not None and self.L is not None:
-            self.L.ando_trainer_destroy(self.h)
-            self.h = None
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def add_system(self, rp, ci, vals, b) -> int:
         rp_, ci_, v_, b_ = _i(rp), _i(ci), _f(vals).ravel(), _f(b).ravel()

@@ -54,11 +54,6 @@ def update_bytes(n: int) -> int:
     return 120 * n
 
 
-def init_bytes(n: int) -> int:
-    """PCG set-up: diagonal blocks (36 B) + index read, rhs read; P^-1 (36 B), r, x, z, p written."""
-    return 40 * n + 12 * n + 36 * n + 4 * 12 * n
-
-
 def measured_traffic(workload: str, n: int, nb: int, kernel_prefix: str):
     """HBM bytes per launch of the kernel from the committed rocprofv3 PMC
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel
@@ -237,7 +232,7 @@ def fp64_accuracy(sim, sc, saved, restore, window, orc, threads, steps: int = 3)
             "fp32_oracle_max_rel_err_per_step": [float(f"{rel(c, r):.3e}") for c, r in zip(o32, o64)]}
 
 
-def learned_comparison(sim, sc, window, restore, replay, a, pers):
+def learned_comparison(sim, sc, window, restore, replay, a):
     """Learned preconditioner vs plain block-Jacobi PCG on the same window.
 
     The network is trained here, on the GPU, on Newton systems of the batched
@@ -272,7 +267,7 @@ def learned_comparison(sim, sc, window, restore, replay, a, pers):
     sim.set_precond_weights(w)
 
     def steps(warm_only):
-        sim.set_mode(fp64_near=True, kernel_timing=False, pcg_persistent=pers, learned=True,
+        sim.set_mode(fp64_near=True, kernel_timing=False, learned=True,
                      warm_start_only=warm_only)
         restore()
         recs, nonfinite = [], None
@@ -283,7 +278,7 @@ def learned_comparison(sim, sc, window, restore, replay, a, pers):
             if not finite_state(sim):
                 nonfinite = k
                 break
-        sim.set_mode(fp64_near=True, kernel_timing=False, pcg_persistent=pers, learned=False)
+        sim.set_mode(fp64_near=True, kernel_timing=False, learned=False)
         el = sum(r[0] for r in recs)
         return {"pcg_iters_per_step": round(sum(r[1]["pcg_iterations"] for r in recs) / len(recs), 2),
                 "steps_per_s": round(len(recs) / el, 3), "nonfinite_at_step": nonfinite,
@@ -322,7 +317,6 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="per CPU baseline run")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip the kernel-event pass")
-    ap.add_argument("--pcg", choices=["persistent", "two-kernel"], default="two-kernel")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner comparison")
     ap.add_argument("--train-steps", type=int, default=8000)
     ap.add_argument("--no-corrected", action="store_true", help="skip the corrected-contact leg")
@@ -342,9 +336,8 @@ def main(argv=None):
     from andosim_amd import native, scenes
 
     sc = scenes.WORKLOADS[a.workload]()
-    pers = a.pcg == "persistent"
     sim = native.GpuSim(sc, fp64_near=True)
-    sim.set_mode(fp64_near=True, kernel_timing=False, pcg_persistent=pers)
+    sim.set_mode(fp64_near=True, kernel_timing=False)
     n = len(sc.verts)
     rp, ci = sim.pattern()
     nb = len(ci)
@@ -419,32 +412,19 @@ def main(argv=None):
     # Event pass: the same window again, every PCG launch bracketed by events.
     ev_recs = []
     if not a.no_timing:
-        sim.set_mode(fp64_near=True, kernel_timing=True, pcg_persistent=pers)
+        sim.set_mode(fp64_near=True, kernel_timing=True)
         ev_recs = replay(a.steps, True)
-        sim.set_mode(fp64_near=True, kernel_timing=False, pcg_persistent=pers)
-    timed_iters = sum(r[1]["timed_iterations"] for r in ev_recs)
+        sim.set_mode(fp64_near=True, kernel_timing=False)
     t_launch = [sum(r[1]["timed_ms"][k] for r in ev_recs) for k in range(2)]
     n_launch = [sum(r[1]["timed_launches"][k] for r in ev_recs) for k in range(2)]
     ev_elapsed = sum(r[0] for r in ev_recs)
 
     roof = None
-    if n_launch[0] and kernel == 1:
-        # One launch = one whole PCG solve: algorithmic bytes of its iterations + set-up.
-        by_total = timed_iters * (spmv_bytes(n, nb) + update_bytes(n)) + n_launch[0] * init_bytes(n)
-        secs = t_launch[0] * 1e-3
-        ach = by_total / secs / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_pcg_persistent",
-                "bytes_per_launch": int(by_total / n_launch[0]), "avg_launch_us": round(secs / n_launch[0] * 1e6, 2),
-                "launches": n_launch[0], "iterations_per_launch": round(timed_iters / n_launch[0], 2),
-                "us_per_iteration": round(secs / max(timed_iters, 1) * 1e6, 2),
-                "bytes_per_iteration": spmv_bytes(n, nb) + update_bytes(n),
-                "pcg_share_of_step": round(secs / ev_elapsed, 3)}
-    elif n_launch[0]:
+    if n_launch[0]:
         avg_s = t_launch[0] / n_launch[0] * 1e-3
         by = spmv_bytes(n, nb)
         ach = by / avg_s / 1e9
-        kname = {3: "ando::k_pcg_spmv_sell", 2: "ando::k_pcg_spmv_blk"}.get(kernel, "ando::k_pcg_spmv<3>")
+        kname = {3: "ando::k_pcg_spmv_sell"}.get(kernel, "ando::k_pcg_spmv<3>")
         if kernel == 3 and lanes in (2, 4):
             kname = f"ando::k_pcg_spmv_sell_split<{lanes}, {wg_slices}>"
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -468,7 +448,7 @@ def main(argv=None):
 
     learned = None
     if not a.no_learned:
-        learned = learned_comparison(sim, sc, window, restore, replay, a, pers)
+        learned = learned_comparison(sim, sc, window, restore, replay, a)
 
     cpu = cpu_single = accuracy = None
     if not a.no_cpu:
@@ -519,7 +499,7 @@ def main(argv=None):
         "config": {"workload": f"{a.workload}: 257x257 cloth (66,049 v, 131,072 tris) onto two crossed cylinders "
                                "r=0.1 m (4,800 rigid tris); dt=0.01, 5% strain limit, PCG tol 1e-6, faithful mode",
                    "vertices": n, "blocks": nb, "precond": "block-Jacobi (plain PCG)",
-                   "pcg_kernel": {1: "persistent", 0: "two-kernel, BSR row SpMV", 2: "two-kernel, block SpMV",
+                   "pcg_kernel": {0: "two-kernel, BSR row SpMV",
                                   3: f"two-kernel, sliced-ELL SpMV ({lanes} lane{'s' if lanes > 1 else ''} per row, "
                                      f"{wg_slices} slice{'s' if wg_slices != 1 else ''} per workgroup, "
                                      f"{vec_block}-thread vector kernels{', autotuned' if tuned else ''})"

@@ -147,17 +147,16 @@ typedef struct {
   double step_ms;                   /* wall time of the step, host clock */
   int32_t timed_launches[2];        /* ANDO_MODE_KERNEL_TIMING: launches that ran (not skipped) */
   double timed_ms[2];               /* summed event durations of those launches */
-  int32_t timed_iterations;         /* PCG iterations covered by timed persistent launches */
-  int32_t pcg_kernel;               /* 0: two kernels per iteration (row SpMV), 1: persistent
-                                       single launch, 2: two kernels with the block-parallel SpMV,
-                                       3: two kernels with the sliced-ELL SpMV */
+  int32_t timed_iterations;         /* reserved (0) */
+  int32_t pcg_kernel;               /* two kernels per iteration with 0: the BSR row SpMV,
+                                       3: the sliced-ELL SpMV (1 and 2 are not used) */
   int32_t learned_solves;           /* solves that used the learned preconditioner */
   int32_t learned_fallbacks;        /* of those, solves that fell back to plain PCG */
   int32_t fallback_iterations;      /* plain PCG iterations spent in fallbacks (included in pcg_iterations) */
   int32_t spmv_lanes;               /* pcg_kernel 3: lanes per row of the sliced-ELL SpMV (1, 2 or 4) */
   int32_t spmv_wg_slices;           /* pcg_kernel 3: slices of 64 rows per SpMV workgroup (0: half a slice) */
   int32_t batched_solves;           /* solves run inside a batched PCG launch (ando_step_batch) */
-  int32_t pcg_vec_block;            /* pcg_kernel 0, 2, 3: threads per workgroup of the PCG vector kernels */
+  int32_t pcg_vec_block;            /* threads per workgroup of the PCG vector kernels */
   int32_t pcg_autotuned;            /* 1: the launch shape above was picked by the autotuner
                                        (environment ANDO_PCG_AUTOTUNE=1, read by ando_create) */
 } ando_stats;
@@ -197,13 +196,8 @@ int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, co
 #define ANDO_MODE_FP64_NEAR_CONTACT 1
 /* ANDO_MODE_KERNEL_TIMING brackets every PCG launch with HIP events on the
  * context's stream and accumulates their durations into ando_stats.timed_ms
- * (two-kernel PCG: index 0 SpMV kernel, 1 update kernel; persistent PCG:
- * index 0 the whole-solve launch). */
+ * (index 0 the SpMV kernel, 1 the update kernel).  Flag value 4 is not used. */
 #define ANDO_MODE_KERNEL_TIMING 2
-/* ANDO_MODE_PCG_PERSISTENT runs each PCG solve as one cooperative launch with
- * grid barriers instead of the default two kernels per iteration (host check
- * every few iterations); same arithmetic. */
-#define ANDO_MODE_PCG_PERSISTENT 4
 /* ANDO_MODE_LEARNED_PRECOND (needs ando_set_precond_weights) solves each
  * Newton system with the learned block preconditioner and the warm start it
  * predicts; a solve that misses the tolerance is redone with plain

@@ -4,6 +4,8 @@ import ctypes
 import glob
 import os
 import re
+import subprocess
+import tempfile
 
 import numpy as np
 import pytest
@@ -75,3 +77,43 @@ def test_ctypes_structs_follow_the_header(cname, pyname):
     if cls is None:
         pytest.skip(f"native.py has no {pyname} mirror")
     assert [f[0] for f in cls._fields_] == _header_struct_fields(cname)
+
+
+STRUCTS = [("ando_stats", "Stats"), ("ando_params", "Params"), ("ando_material", "Material"),
+           ("ando_contact", "Contact"), ("ando_rigid_body", "RigidBody"), ("ando_energy", "Energy"),
+           ("ando_collision_metrics", "CollisionMetrics")]
+
+
+def _c_layout():
+    """sizeof and (offsetof, sizeof) of every field of the header's structs,
+    from a probe compiled by the host C compiler."""
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "andosim_hip.h"', "int main(void) {"]
+    for cname, _ in STRUCTS:
+        lines.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in _header_struct_fields(cname):
+            lines.append(f'  printf("{cname}.{f} %zu %zu\\n", offsetof({cname}, {f}), sizeof((({cname}*)0)->{f}));')
+    lines += ["  return 0;", "}"]
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "probe.c"), os.path.join(d, "probe")
+        open(src, "w").write("\n".join(lines) + "\n")
+        subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), "-o", exe, src], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    layout = {}
+    for ln in out.splitlines():
+        key, *vals = ln.split()
+        layout[key] = tuple(int(v) for v in vals)
+    return layout
+
+
+@pytest.mark.parametrize("cname,pyname", STRUCTS)
+def test_ctypes_struct_offsets_and_sizes_match_the_c_compiler(cname, pyname):
+    """Names alone would miss a type drift (a float for a double, int32 for
+    int64): every field's offset and size and the struct's size must equal
+    what the C compiler lays out for the header."""
+    native = pytest.importorskip("andosim_amd.native")
+    cls = getattr(native, pyname)
+    layout = _c_layout()
+    assert ctypes.sizeof(cls) == layout[cname][0], (cname, ctypes.sizeof(cls), layout[cname])
+    for f, _ in cls._fields_:
+        d = getattr(cls, f)
+        assert (d.offset, d.size) == layout[f"{cname}.{f}"], (cname, f)

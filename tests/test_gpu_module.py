@@ -205,6 +205,30 @@ def test_module_readback_arrays_are_independent():
     assert np.array_equal(state.get_positions(), xs) and np.array_equal(state.get_velocities(), vs)
 
 
+def test_module_pinned_pool_is_bounded():
+    """Reads of states of several sizes: released arrays return their buffers
+    to the pool (at most four idle per size), nothing stays live once every
+    array is gone, and live plus idle bytes never pass the 256 MB cap."""
+    abc = pytest.importorskip("ando_barrier_core")
+    import gc
+    gc.collect()
+    cap = 256 << 20
+    live0, idle0 = abc._pinned_pool_bytes()  # earlier tests may leave idle buffers
+    for s in (5, 9, 13, 17):
+        sc = scenes.small_sheet(s, pins=0, wall=True)
+        mesh, state, cons, p, _ = build_module_scene(abc, sc)
+        state.apply_gravity((0.0, 0.0, -9.81), p.dt)
+        abc.Integrator.step(mesh, state, cons, p)
+        arrays = [state.get_positions() for _ in range(7)]
+        live, idle = abc._pinned_pool_bytes()
+        assert live >= 7 * 12 * s * s and live + idle <= cap
+        del arrays
+        live, idle = abc._pinned_pool_bytes()
+        assert live + idle <= cap and idle - idle0 <= 4 * 12 * (5 * 5 + 9 * 9 + 13 * 13 + 17 * 17)
+    gc.collect()
+    assert abc._pinned_pool_bytes()[0] == live0
+
+
 def light_sphere_scene(density=50.0):
     """The 9x9 sheet falling on a sphere light enough to be pushed around."""
     sc = scenes.small_sheet(9, pins=0, wall=False, rigid=True)

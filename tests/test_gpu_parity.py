@@ -183,34 +183,3 @@ def test_free_fall_closed_form():
         assert rel_err(g.get("x"), o.get("x")) < 1e-6
         # Velocities agree to a few position ulps divided by dt.
         assert np.max(np.abs(g.get("v") - o.get("v"))) < 4 * np.spacing(np.float32(1.0)) / sc.params["dt"]
-
-
-@pytest.mark.parametrize("name,make,nb", SMALL)
-def test_persistent_and_two_kernel_pcg_agree(name, make, nb):
-    """The single-launch cooperative PCG and the two-kernels-per-iteration PCG
-    solve the same system to the same tolerance (they differ only in the SpMV
-    row-sum association)."""
-    sc = make()
-    o = OracleSim(sc, 32)
-    for _ in range(12):
-        o.substep()
-    x, v = o.get("x"), o.get("v")
-    xhat = (x + sc.params["dt"] * v).astype(np.float32)
-    g_, H = o.linearize(xhat)
-    rp, ci = o.pattern()
-    b = (-g_).astype(np.float32)
-    gs = gpu(sc)
-    gs.set_mode(fp64_near=False, pcg_persistent=True)
-    x1, it1, ok1 = gs.pcg(rp, ci, H.astype(np.float32), b, tol=1e-6, maxit=1000)
-    gs.set_mode(fp64_near=False, pcg_persistent=False)
-    x2, it2, ok2 = gs.pcg(rp, ci, H.astype(np.float32), b, tol=1e-6, maxit=1000)
-    x32, it32, ok32 = oracle_pcg(rp, ci, H, -g_, tol=1e-6, maxit=1000, precision=32)
-    # Both variants reach the same verdict as the FP32 oracle (some of these
-    # states are too stiff for FP32 PCG to reach 1e-6 within the cap).
-    assert ok1 == ok32 and ok2 == ok32
-    if ok32:
-        assert abs(it1 - it2) <= max(2, it2 // 10)
-    x64, _, _ = oracle_pcg(rp, ci, H, -g_, tol=1e-12, maxit=5000, precision=64)
-    e32 = rel_err(x32, x64)
-    assert rel_err(x1, x64) <= max(2.0 * e32, 1e-6)
-    assert rel_err(x2, x64) <= max(2.0 * e32, 1e-6)
