@@ -2082,6 +2082,176 @@ int ando_pcg(ando_ctx* c, int32_t num_rows, const int32_t* row_ptr, const int32_
   });
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free PCG (ando_pcg_noalloc).  Its work vectors, preconditioner,
+// partial sums, history, status and error word live in one file-scope device
+// array, `scratch`, sized for kNoAllocMaxRows block rows and kNoAllocMaxIters
+// iterations.  Each GPU holds its own copy for the life of the process; host
+// code never names the array in a launch or copy, only its per-device address
+// from hipGetSymbolAddress.  Calls on one GPU share the array: the library
+// serialises them with a per-device mutex, and every buffer that must start at
+// zero is cleared by a kernel at the start of each call.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kNoAllocMaxRows = 1 << 17;
+constexpr int kNoAllocMaxIters = 8192;
+constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+struct NoAllocLayout {
+  size_t r, z, p0, p1, Ap, Pinv, diag, part_rz, part_max, part_pAp, rz_hist, status, err, total;
+};
+constexpr NoAllocLayout noalloc_layout() {
+  NoAllocLayout L{};
+  const size_t N = kNoAllocMaxRows;
+  const size_t vec = align256(12 * N);
+  const size_t wg_v = (N + kBlock - 1) / kBlock + 1;
+  const size_t wg_r = (N + ando::kRowsPerPass - 1) / ando::kRowsPerPass + 1;
+  size_t o = 0;
+  L.r = o; o += vec;
+  L.z = o; o += vec;
+  L.p0 = o; o += vec;
+  L.p1 = o; o += vec;
+  L.Ap = o; o += vec;
+  L.Pinv = o; o += align256(36 * N);
+  L.diag = o; o += align256(4 * N);
+  L.part_rz = o; o += align256(8 * wg_v);
+  L.part_max = o; o += align256(4 * wg_v);
+  L.part_pAp = o; o += align256(8 * wg_r);
+  L.rz_hist = o; o += align256(8 * (size_t)(kNoAllocMaxIters + 2));
+  L.status = o; o += align256(sizeof(ando::PcgStatus));
+  L.err = o; o += align256(sizeof(uint32_t));
+  L.total = o;
+  return L;
+}
+constexpr NoAllocLayout kNoAlloc = noalloc_layout();
+
+static __device__ __align__(256) unsigned char scratch[kNoAlloc.total];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu[kMaxDevices];
+unsigned char* g_noalloc_ptr[kMaxDevices] = {};  // device address of `scratch`, per device
+
+// Checks a BSR pattern before any kernel follows it: row_ptr starts at 0, is
+// monotone and ends at num_blocks, every column is a row index, and every row
+// has its diagonal block (whose position goes to diag).  Violations set bits
+// of *err: 1 row pointers, 2 columns, 4 a missing diagonal.
+__global__ void k_check_pattern(int n, int nb, const int32_t* __restrict__ rp, const int32_t* __restrict__ cols,
+                                int32_t* __restrict__ diag, uint32_t* __restrict__ err) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int a = rp[i], b = rp[i + 1];
+  if ((i == 0 && a != 0) || (i == n - 1 && b != nb) || a < 0 || b < a || b > nb) {
+    atomicOr(err, 1u);
+    diag[i] = -1;
+    return;
+  }
+  int d = -1;
+  for (int k = a; k < b; ++k) {
+    const int c = cols[k];
+    if (c < 0 || c >= n) atomicOr(err, 2u);
+    if (c == i) d = k;
+  }
+  if (d < 0) atomicOr(err, 4u);
+  diag[i] = d;
+}
+
+__global__ void k_zero_u32(uint32_t* __restrict__ p, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0u;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ando_pcg_noalloc(int32_t device, int32_t num_rows, int32_t num_blocks, const int32_t* row_ptr, const int32_t* cols,
+                     const float* values, const float* b, float* x, float tol, int32_t max_iters, void* stream,
+                     int32_t* iterations, int32_t* converged) {
+  return guarded([&] {
+    need(row_ptr && cols && values && b && x, ANDO_EINVAL, "null argument");
+    need(device >= 0 && device < kMaxDevices, ANDO_EINVAL, "device index out of range");
+    need(num_rows > 0 && num_blocks >= num_rows, ANDO_EINVAL, "bad sizes");
+    need(num_rows <= kNoAllocMaxRows, ANDO_EINVAL,
+         "ando_pcg_noalloc: more than 131072 block rows (its fixed scratch holds at most 131072)");
+    need(max_iters >= 0 && max_iters <= kNoAllocMaxIters, ANDO_EINVAL,
+         "ando_pcg_noalloc: max_iters must be within 0..8192 (its fixed scratch holds that history)");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu[device]);
+    HIPCHK(hipSetDevice(device));
+    if (!g_noalloc_ptr[device]) {
+      void* ptr = nullptr;
+      HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+      need(ptr != nullptr, ANDO_EDEVICE, "scratch has no device address");
+      g_noalloc_ptr[device] = static_cast<unsigned char*>(ptr);
+    }
+    unsigned char* base = g_noalloc_ptr[device];
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int n = num_rows;
+    float* r = reinterpret_cast<float*>(base + kNoAlloc.r);
+    float* z = reinterpret_cast<float*>(base + kNoAlloc.z);
+    float* p0 = reinterpret_cast<float*>(base + kNoAlloc.p0);
+    float* p1 = reinterpret_cast<float*>(base + kNoAlloc.p1);
+    float* Ap = reinterpret_cast<float*>(base + kNoAlloc.Ap);
+    float* Pinv = reinterpret_cast<float*>(base + kNoAlloc.Pinv);
+    int32_t* diag = reinterpret_cast<int32_t*>(base + kNoAlloc.diag);
+    double* part_rz = reinterpret_cast<double*>(base + kNoAlloc.part_rz);
+    float* part_max = reinterpret_cast<float*>(base + kNoAlloc.part_max);
+    double* part_pAp = reinterpret_cast<double*>(base + kNoAlloc.part_pAp);
+    double* rz_hist = reinterpret_cast<double*>(base + kNoAlloc.rz_hist);
+    PcgStatus* st = reinterpret_cast<PcgStatus*>(base + kNoAlloc.status);
+    uint32_t* err = reinterpret_cast<uint32_t*>(base + kNoAlloc.err);
+    // The pattern first: nothing below reads a column or row pointer it has not vetted.
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, err, 1);
+    hipLaunchKernelGGL(k_check_pattern, dim3(nblk(n)), dim3(kBlock), 0, s, n, num_blocks, row_ptr, cols, diag, err);
+    HIPCHK(hipGetLastError());
+    uint32_t e = 0;
+    HIPCHK(hipMemcpyAsync(&e, err, sizeof(e), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    need(!(e & 1u), ANDO_EINVAL, "row_ptr must start at 0, be monotone and end at num_blocks");
+    need(!(e & 2u), ANDO_EINVAL, "column index out of range");
+    need(!(e & 4u), ANDO_EINVAL, "every row needs its diagonal block");
+    // PCGSolver::solve (pcg_solver.cpp:8-146) with the row SpMV, as ando_pcg
+    // runs it in ANDO_MODE_PCG_ROW_SPMV: same kernels, grid and chunk schedule.
+    constexpr int R = 3;
+    const int nwg_v = nblk(n), nwg_r = spmv_groups(n, R);
+    hipLaunchKernelGGL(k_pcg_init, dim3(nwg_v), dim3(kBlock), 0, s, n, values, (const int32_t*)diag, b, x, r, z, p0,
+                       Pinv, n, 1, part_rz, part_max, 0);
+    hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, s, nwg_v, (const double*)part_rz,
+                       (const float*)part_max, (const float*)nullptr, rz_hist, st, tol);
+    HIPCHK(hipGetLastError());
+    PcgStatus hs{};
+    int it = 0, chunk = 8;
+    for (;;) {
+      const int end = std::min(max_iters, it + chunk);
+      for (; it < end; ++it) {
+        float* pold = (it == 0) ? p0 : ((it - 1) & 1 ? p1 : p0);
+        float* pcur = (it & 1) ? p1 : p0;
+        hipLaunchKernelGGL(k_pcg_spmv<R>, dim3(nwg_r), dim3(kBlock), 0, s, n, it, values, row_ptr, cols,
+                           (const float*)z, (const float*)pold, pcur, Ap, (const double*)part_rz,
+                           (const float*)part_max, nwg_v, rz_hist, part_pAp, st, tol);
+        hipLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(kBlock), 0, s, n, it, (const float*)pcur,
+                           (const float*)Ap, (const float*)Pinv, n, 1, x, r, z, (const double*)part_pAp, nwg_r,
+                           (const double*)rz_hist, part_rz, part_max, st, (const double*)nullptr, 0);
+      }
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (hs.done) break;
+      if (it >= max_iters) {
+        hipLaunchKernelGGL(k_pcg_final, dim3(1), dim3(kBlock), 0, s, it, part_max, nwg_v, st, tol);
+        HIPCHK(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        break;
+      }
+      chunk = std::min(chunk * 2, 256);
+    }
+    if (iterations) *iterations = hs.iters;
+    if (converged) *converged = hs.failed ? 0 : 1;
+  });
+}
+
 int ando_count_intersections(ando_ctx* c, const ando_rigid_body* bodies, int32_t num_bodies, int32_t* edge_triangle,
                               int32_t* inside_rigid) {
   return guarded([&] {

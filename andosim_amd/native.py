@@ -28,7 +28,7 @@ ABI_SYMBOLS = [
     "ando_trainer_add_system", "ando_trainer_init_weights", "ando_trainer_set_weights", "ando_trainer_get_weights",
     "ando_trainer_forward", "ando_trainer_loss_grad", "ando_trainer_train", "ando_trainer_solve",
     "ando_energy_diagnostics", "ando_contact_metrics", "ando_cfl_inputs", "ando_step_batch",
-    "ando_host_alloc", "ando_host_free",
+    "ando_host_alloc", "ando_host_free", "ando_pcg_noalloc",
 ]
 
 
@@ -139,6 +139,8 @@ def load():
     L.ando_linearize.argtypes = [C.c_void_p, C.POINTER(Params), fp, C.POINTER(RigidBody), C.c_int32, fp, fp]
     L.ando_apply_restitution.argtypes = [C.c_void_p, C.POINTER(Params), C.POINTER(RigidBody), C.c_int32]
     L.ando_pcg.argtypes = [C.c_void_p, C.c_int32, ip, ip, fp, fp, fp, C.c_float, C.c_int32, ip, ip]
+    L.ando_pcg_noalloc.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_float, C.c_int32, C.c_void_p, ip, ip]
     dp = C.POINTER(C.c_double)
     L.ando_nn_num_weights.restype = C.c_int32
     L.ando_set_precond_weights.argtypes = [C.c_void_p, fp, C.c_int32]
@@ -448,6 +450,18 @@ class GpuSim:
         check(self.L.ando_pcg(self.h, n, _ip(rp_), _ip(ci_), _fp(v_), _fp(b_), _fp(x), tol, maxit, C.byref(it),
                               C.byref(conv)))
         return x, it.value, bool(conv.value)
+
+
+def pcg_noalloc(device: int, num_rows: int, num_blocks: int, row_ptr, cols, values, b, x, tol: float = 1e-6,
+                maxit: int = 1000, stream=None):
+    """ando_pcg_noalloc on device buffers given as integer addresses (for
+    example torch tensors' data_ptr()); returns (iterations, converged)."""
+    it = C.c_int32(0)
+    conv = C.c_int32(0)
+    check(load().ando_pcg_noalloc(device, num_rows, num_blocks, C.c_void_p(row_ptr), C.c_void_p(cols),
+                                  C.c_void_p(values), C.c_void_p(b), C.c_void_p(x), tol, maxit,
+                                  C.c_void_p(stream) if stream else None, C.byref(it), C.byref(conv)))
+    return it.value, bool(conv.value)
 
 
 def step_batch(sims, threads: int = 0):

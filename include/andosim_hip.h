@@ -325,6 +325,26 @@ int ando_linearize(ando_ctx* ctx, const ando_params* params, const float* x_targ
                    int32_t num_bodies, float* gradient, float* values);
 int ando_pcg(ando_ctx* ctx, int32_t num_rows, const int32_t* row_ptr, const int32_t* cols, const float* values,
              const float* b, float* x, float tol, int32_t max_iters, int32_t* iterations, int32_t* converged);
+/* Allocation-free PCG (new; no counterpart in the reference).  For callers
+ * that cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  The same solve as ando_pcg
+ * in ANDO_MODE_PCG_ROW_SPMV (block-Jacobi PCG, pcg_solver.cpp:8-146; the two
+ * return the same x and iteration count bit for bit), on a BSR system already
+ * in device memory of `device`: row_ptr (num_rows + 1), cols (num_blocks),
+ * values (9 per block, row-major 3x3), b and x (3 * num_rows) are device
+ * pointers; x receives the solution from x0 = 0.  iterations / converged
+ * (host pointers, nullable) report the outcome.  stream: the hipStream_t to
+ * run on (NULL: the null stream); the call waits for it before returning.
+ * The library allocates nothing in this call: its work memory is one fixed
+ * device array per GPU (about 13 MB, held until the process ends), sized for
+ * num_rows <= 131072 and max_iters <= 8192; larger requests fail with
+ * ANDO_EINVAL.  Calls on one GPU share that array and are serialised by the
+ * library within a process.  The pattern is checked on the device before any
+ * kernel follows it (ANDO_EINVAL on a bad row_ptr, a column out of range or a
+ * missing diagonal block). */
+int ando_pcg_noalloc(int32_t device, int32_t num_rows, int32_t num_blocks, const int32_t* row_ptr, const int32_t* cols,
+                     const float* values, const float* b, float* x, float tol, int32_t max_iters, void* stream,
+                     int32_t* iterations, int32_t* converged);
 /* The restitution pass that ando_step runs after its velocity update when
  * contact_restitution > 0, on the current state: contacts detected at the
  * current positions (self contacts, then rigid), impulses in list order
