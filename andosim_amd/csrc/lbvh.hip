@@ -861,12 +861,18 @@ __global__ void k_ls_self(int nc, const SelfContact* __restrict__ sc, const floa
 // ---------------------------------------------------------------------------
 
 // Sign of the volume of (a, b, c, d), in double on the float coordinates.
+// 0 when rounding could have decided the sign: Shewchuk's static error bound
+// for orient3d (o3derrboundA = (7 + 56 eps) eps, eps = 2^-53), so a
+// near-coplanar configuration counts as degenerate, not as a crossing.
 __device__ inline int orient3(f3 a, f3 b, f3 c, f3 d) {
   const double ax = (double)a.x - d.x, ay = (double)a.y - d.y, az = (double)a.z - d.z;
   const double bx = (double)b.x - d.x, by = (double)b.y - d.y, bz = (double)b.z - d.z;
   const double cx = (double)c.x - d.x, cy = (double)c.y - d.y, cz = (double)c.z - d.z;
   const double v = ax * (by * cz - bz * cy) - ay * (bx * cz - bz * cx) + az * (bx * cy - by * cx);
-  return v > 0.0 ? 1 : (v < 0.0 ? -1 : 0);
+  const double perm = fabs(ax) * (fabs(by * cz) + fabs(bz * cy)) + fabs(ay) * (fabs(bx * cz) + fabs(bz * cx)) +
+                      fabs(az) * (fabs(bx * cy) + fabs(by * cx));
+  const double bound = 7.771561172376103e-16 * perm;
+  return v > bound ? 1 : (v < -bound ? -1 : 0);
 }
 
 // Segment p q properly crosses triangle a b c: p and q strictly on opposite

@@ -290,12 +290,14 @@ void narrow_pt(const V3<R>& p, const V3<R>& a, const V3<R>& b, const V3<R>& c, R
 }
 
 // Edge-edge narrow phase (collision.cpp:389-452).
+// s_out / t_out (optional): the closest points' parameters on the two edges
+// (0 for both when both edges are degenerate).
 template <class R>
 void narrow_ee(const V3<R>& p0, const V3<R>& p1, const V3<R>& q0, const V3<R>& q1, R& dist, V3<R>& normal,
-               V3<R>& wp, V3<R>& wq) {
+               V3<R>& wp, V3<R>& wq, R* s_out = nullptr, R* t_out = nullptr) {
   V3<R> d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
   R a = dot3(d1, d1), e = dot3(d2, d2), f = dot3(d2, r);
-  R s, t;
+  R s = R(0), t = R(0);
   const R eps = R(1e-10);
   auto cl = [](R x) { return std::clamp(x, R(0), R(1)); };
   if (a <= eps && e <= eps) {
@@ -319,6 +321,8 @@ void narrow_ee(const V3<R>& p0, const V3<R>& p1, const V3<R>& q0, const V3<R>& q
     wp = p0 + s * d1;
     wq = q0 + t * d2;
   }
+  if (s_out) *s_out = s;
+  if (t_out) *t_out = t;
   V3<R> diff = wp - wq;
   dist = norm3(diff);
   normal = (double)dist > 1e-10 ? diff / dist : V3<R>(0, 0, 1);
@@ -527,6 +531,60 @@ template <class R> struct Sim {
   // mesh's, so they are kept beside it and applied in the PCG product.
   struct Coupling { int i, j; R b[9]; };
   std::vector<Coupling> couplings;
+  // Corrected physics (new; the intended behaviour that the reference's F3,
+  // F7, F11 and F12 findings leave out).  With cp set:
+  //   * apply_gravity changes only v; the step predicts x_hat = x + dt v from
+  //     the positions of the previous step's end, which the previous line
+  //     search left free of penetrations;
+  //   * the membrane gradient reads the current positions (the reference's
+  //     formula, elasticity.cpp:18-61, on x instead of mesh.vertices), its
+  //     Hessian is the Gauss-Newton form 2k (B B^T)_ab (I - n n^T) of the
+  //     face at the current x (B = S Dm_inv, S the edge-difference map);
+  //   * pins attract: E = (pin_k m / dt^2) |x - p|^2 / 2;
+  //   * contact candidates come from per-vertex boxes: PT pairs (a vertex box
+  //     grown by gap_max against the union box of a triangle not incident to
+  //     the vertex), EE pairs (the union box of one edge grown by gap_max
+  //     against that of another sharing no vertex) and rigid PT pairs (a
+  //     vertex box grown by gap_max against a rigid triangle's box).  At the
+  //     start of a step the boxes are the points x, so the candidates are the
+  //     pairs within gap_max (per axis); after every PCG solve they are the
+  //     swept boxes of the trial motion [x, x + 1.25 d], which serve the line
+  //     search and then the barrier at the next iterate (every iterate lies on
+  //     that motion, so each pair within gap_max of it is a candidate);
+  //   * every Newton iteration refreshes each candidate's gap, normal and
+  //     weights at the current x (PT: 1, -b; EE: 1 - s, s, -(1 - t), -t from
+  //     the closest points), so the barrier follows the iterate;
+  //   * contact stiffness takes the wall's inertial scale, sum m w^2 /
+  //     max(gap_max, min_gap)^2 (stiffness.cpp:146-172), in place of
+  //     sum m w^2 / dt^2: the cubic barrier's force is bounded by
+  //     1.5 k gap_max, which at m / dt^2 cannot stop a vertex faster than
+  //     about 0.15 m/s within a step;
+  //   * strain-limit stiffness takes the band width as its scale,
+  //     m_face / eps^2, in place of m_face / max(|gap|, min_gap)^2, which
+  //     grows without bound at the limit and stalls FP32 PCG;
+  //   * the velocity is (x - x_old) / dt after the step's passes (the
+  //     reference divides by beta dt, and beta = 0.5 after a capped pass
+  //     doubles the velocity);
+  //   * contact, wall and strain stiffness read the isotropic elastic
+  //     stiffness sum_f 2k (B B^T)_ab I (an upper bound on the projected
+  //     Hessian along any direction) in place of the clamped faithful blocks;
+  //   * the line search accepts alpha only if the first wall is no more
+  //     violated than at the iterate and every candidate pair's additive-CCD
+  //     time of impact over the full trial motion (1.25 d) is at least alpha;
+  //     pairs outside the candidate set cannot meet during that motion;
+  //   * rigid coupling pushes the body away from the cloth (the reference
+  //     applies -dV/dg n, toward it).
+  bool cp = false;
+  R pin_k = R(1e4);
+  std::vector<std::array<R, 6>> gk, fpi;  // per face: 2k (B B^T) and I - n n^T, entries 00 01 02 11 12 22
+  std::vector<R> Hiso, Hbase_iso;         // 9 per block: isotropic elastic stiffness; plus mass / dt^2
+  std::vector<R> vbox;                    // 6 per vertex: candidate boxes (lo xyz, hi xyz)
+  std::vector<std::vector<V3<R>>> rworld; // rigid world vertices during the step
+  static int sym6(int a, int b) {
+    static const int t[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    return t[a][b];
+  }
+  const std::vector<R>& Hel() const { return cp ? Hiso : Helast; }
   bool assembled_symmetric = true;
   long asym_count = 0;
 
@@ -788,7 +846,15 @@ template <class R> struct Sim {
     if (vv.empty()) return R(0);
     R inertial = R(0);
     for (size_t i = 0; i < vv.size(); ++i) inertial += m[vv[i]] * ww[i] * ww[i];
-    inertial /= (dt * dt);
+    if (cp) {
+      // Corrected physics: the wall's inertial scale m / gap_max^2
+      // (stiffness.cpp:146-172) instead of m / dt^2, so the barrier's force
+      // near contact (1.5 k gap_max) stops what the wall stops.
+      const R gh = std::max(std::max(P.gap_max, P.min_gap), R(1e-12));
+      inertial /= (gh * gh);
+    } else {
+      inertial /= (dt * dt);
+    }
     int bc = (int)vv.size();
     std::vector<R> Wn(3 * bc), Ht(3 * bc, R(0));
     for (int i = 0; i < bc; ++i) for (int k = 0; k < 3; ++k) Wn[3 * i + k] = ww[i] * nn[k];
@@ -798,7 +864,7 @@ template <class R> struct Sim {
         int bi = blk(vv[rb], vv[cb]);
         if (bi < 0) continue;
         for (int r = 0; r < 3; ++r)
-          for (int k = 0; k < 3; ++k) Ht[3 * rb + r] += Helast[9 * bi + 3 * r + k] * Wn[3 * cb + k];
+          for (int k = 0; k < 3; ++k) Ht[3 * rb + r] += Hel()[9 * bi + 3 * r + k] * Wn[3 * cb + k];
       }
     R el = Wn[0] * Ht[0];
     for (int i = 1; i < 3 * bc; ++i) el = el + Wn[i] * Ht[i];
@@ -919,7 +985,7 @@ template <class R> struct Sim {
           R s = R(0);
           for (int b = 0; b < 3; ++b) {
             int bi = blk(tr[a], tr[b]);
-            for (int l = 0; l < 3; ++l) s += Helast[9 * bi + 3 * k + l] * w[3 * b + l];
+            for (int l = 0; l < 3; ++l) s += Hel()[9 * bi + 3 * k + l] * w[3 * b + l];
           }
           Hw[3 * a + k] = s;
         }
@@ -931,7 +997,9 @@ template <class R> struct Sim {
       for (int s = 0; s < 2; ++s) {
         R gap = (R(1) + tau + eps) - sig[s];
         if (!in_domain(gap, eps)) continue;
-        R gc = std::max(std::fabs(gap), min_gap);
+        // Corrected physics: the band width as the stiffness scale (as the
+        // wall uses its gap), so the stiffness stays bounded at the limit.
+        R gc = cp ? eps : std::max(std::fabs(gap), min_gap);
         S.active[s] = 1;
         S.k[s] = fmass / (gc * gc) + el;
         S.gap[s] = gap;
@@ -1093,6 +1161,7 @@ template <class R> struct Sim {
       R mf = m[i] / (dt * dt);
       for (int j = 0; j < 3; ++j) g[3 * i + j] += mf * (x[3 * i + j] - xt[3 * i + j]);
     }
+    if (cp) elastic_gradient_x();
     for (int i = 0; i < 3 * n; ++i) g[i] += egrad[i];
     strain_faces();
     for (size_t f = 0; f < tris.size(); ++f)
@@ -1118,6 +1187,11 @@ template <class R> struct Sim {
     for (const auto& p : pins) {
       if (!p.active) continue;
       V3<R> off = X(p.v) - p.t;
+      if (cp) {
+        const R kp = pin_k * (m[p.v] / (dt * dt));
+        for (int k = 0; k < 3; ++k) g[p.v * 3 + k] += kp * off[k];
+        continue;
+      }
       R kb = pin_stiffness(m[p.v], off, block_of(Hconst, p.v));
       R gap = norm3(off);
       if (!in_domain(gap, P.gap_max) || gap <= P.normal_eps) continue;
@@ -1129,7 +1203,7 @@ template <class R> struct Sim {
       if (!w.active) continue;
 #pragma omp parallel for schedule(static)
       for (int i = 0; i < n; ++i) {
-        R kb = wall_stiffness(m[i], w.n, block_of(Hconst, i));
+        R kb = wall_stiffness(m[i], w.n, block_of(cp ? Hbase_iso : Hconst, i));
         V3<R> nn = normalize_or_default(w.n);
         R gap = dot3(nn, X(i)) - w.off;
         if (!in_domain(gap, P.gap_max)) continue;
@@ -1179,13 +1253,24 @@ template <class R> struct Sim {
       add(row, row, val);
     }
     const R eps = R(1e-8);
-    for (size_t f = 0; f < tris.size(); ++f)
-      for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
-          R cc = ec[f][a * 3 + b];
-          R vals[9] = {cc, 0, 0, 0, cc, 0, 0, 0, eps};
-          for (int e = 0; e < 9; ++e) add(tris[f][a] * 3 + e / 3, tris[f][b] * 3 + e % 3, vals[e]);
-        }
+    if (cp) {
+      cp_projectors();
+      for (size_t f = 0; f < tris.size(); ++f)
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            const R gab = gk[f][sym6(a, b)];
+            for (int e = 0; e < 9; ++e)
+              add(tris[f][a] * 3 + e / 3, tris[f][b] * 3 + e % 3, gab * fpi[f][sym6(e / 3, e % 3)]);
+          }
+    } else {
+      for (size_t f = 0; f < tris.size(); ++f)
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            R cc = ec[f][a * 3 + b];
+            R vals[9] = {cc, 0, 0, 0, cc, 0, 0, 0, eps};
+            for (int e = 0; e < 9; ++e) add(tris[f][a] * 3 + e / 3, tris[f][b] * 3 + e % 3, vals[e]);
+          }
+    }
     const R tiny = R(1e-12);
     if (P.enable_strain)
       for (size_t f = 0; f < tris.size(); ++f)
@@ -1252,6 +1337,11 @@ template <class R> struct Sim {
     for (const auto& p : pins) {
       if (!p.active) continue;
       V3<R> off = X(p.v) - p.t;
+      if (cp) {
+        const R kp = pin_k * (m[p.v] / (dt * dt));
+        for (int k = 0; k < 3; ++k) add(p.v * 3 + k, p.v * 3 + k, kp);
+        continue;
+      }
       R kb = pin_stiffness(m[p.v], off, block_of(Hconst, p.v));
       R gap = norm3(off);
       if (!in_domain(gap, P.gap_max) || gap <= P.normal_eps) continue;
@@ -1270,7 +1360,7 @@ template <class R> struct Sim {
     for (const auto& w : walls) {
       if (!w.active) continue;
       for (int i = 0; i < n; ++i) {
-        R kb = wall_stiffness(m[i], w.n, block_of(Hconst, i));
+        R kb = wall_stiffness(m[i], w.n, block_of(cp ? Hbase_iso : Hconst, i));
         V3<R> nn = normalize_or_default(w.n);
         R gap = dot3(nn, X(i)) - w.off;
         if (!in_domain(gap, P.gap_max)) continue;
@@ -1535,6 +1625,297 @@ template <class R> struct Sim {
     return alpha;
   }
 
+  // ---- Corrected physics ---------------------------------------------------
+  // Per-face 2k (B B^T) and the isotropic stiffness folds (once per step, like
+  // the faithful constants).
+  void cp_constants() {
+    const R mu = mu_();
+    const size_t M = tris.size();
+    gk.assign(M, {});
+    for (size_t f = 0; f < M; ++f) {
+      const R k = area[f] * thick * mu;
+      const R twok = (R)(2.0 * (double)k);
+      const R Mi[2][2] = {{Dm[0][f], Dm[1][f]}, {Dm[2][f], Dm[3][f]}};
+      R B[3][2];
+      for (int c = 0; c < 2; ++c) {
+        B[0][c] = -(Mi[0][c] + Mi[1][c]);
+        B[1][c] = Mi[0][c];
+        B[2][c] = Mi[1][c];
+      }
+      for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b) gk[f][sym6(a, b)] = twok * (B[a][0] * B[b][0] + B[a][1] * B[b][1]);
+    }
+    const size_t nb = ci.size();
+    Hiso.assign(9 * nb, R(0));
+    Hbase_iso.assign(9 * nb, R(0));
+    const R dtinv2 = R(1) / (P.dt * P.dt);
+    for (int i = 0; i < n; ++i) {
+      const R mf = m[i] * dtinv2;
+      if (mf != R(0))
+        for (int k = 0; k < 3; ++k) Hbase_iso[9 * diag[i] + 4 * k] += mf;
+    }
+    for (size_t f = 0; f < M; ++f)
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          const int bi = blk(tris[f][a], tris[f][b]);
+          const R g = gk[f][sym6(a, b)];
+          for (int k = 0; k < 3; ++k) {
+            Hiso[9 * bi + 4 * k] += g;
+            Hbase_iso[9 * bi + 4 * k] += g;
+          }
+        }
+  }
+
+  // I - n n^T of every face at the current positions.
+  void cp_projectors() {
+    fpi.assign(tris.size(), {});
+    for (size_t f = 0; f < tris.size(); ++f) {
+      const V3<R> x0 = X(tris[f][0]);
+      const V3<R> nn = normalized3(cross3(X(tris[f][1]) - x0, X(tris[f][2]) - x0));
+      for (int a = 0; a < 3; ++a)
+        for (int b = a; b < 3; ++b) fpi[f][sym6(a, b)] = (a == b ? R(1) : R(0)) - nn[a] * nn[b];
+    }
+  }
+
+  // The membrane gradient of elastic_gradient with F from the current positions.
+  void elastic_gradient_x() {
+    egrad.assign(3 * n, R(0));
+    R mu = mu_();
+    for (size_t f = 0; f < tris.size(); ++f) {
+      const auto& tr = tris[f];
+      V3<R> v0 = X(tr[0]), v1 = X(tr[1]), v2 = X(tr[2]);
+      R F[2][2];
+      compute_F(v0, v1, v2, f, F);
+      R k = area[f] * thick * mu;
+      R twok = (R)(2.0 * (double)k);
+      R Pm[2][2] = {{twok * (F[0][0] - R(1)), twok * (F[0][1] - R(0))}, {twok * (F[1][0] - R(0)), twok * (F[1][1] - R(1))}};
+      R Mi[2][2] = {{Dm[0][f], Dm[1][f]}, {Dm[2][f], Dm[3][f]}};
+      R Hm[2][2];
+      for (int r = 0; r < 2; ++r)
+        for (int c = 0; c < 2; ++c) Hm[r][c] = Pm[r][0] * Mi[c][0] + Pm[r][1] * Mi[c][1];
+      V3<R> e1 = v1 - v0, e2 = v2 - v0;
+      V3<R> nn = normalized3(cross3(e1, e2));
+      V3<R> t1 = normalized3(e1);
+      V3<R> t2 = cross3(nn, t1);
+      V3<R> f1 = Hm[0][0] * t1 + Hm[1][0] * t2;
+      V3<R> f2 = Hm[0][1] * t1 + Hm[1][1] * t2;
+      V3<R> f0 = -(f1 + f2);
+      for (int k2 = 0; k2 < 3; ++k2) {
+        egrad[tr[0] * 3 + k2] += f0[k2];
+        egrad[tr[1] * 3 + k2] += f1[k2];
+        egrad[tr[2] * 3 + k2] += f2[k2];
+      }
+    }
+  }
+
+  // Per-vertex boxes of the motion x -> x + ed (ed null: the points x).
+  void cp_boxes(const std::vector<R>* ed) {
+    vbox.assign(6 * (size_t)n, R(0));
+    for (int i = 0; i < n; ++i)
+      for (int k = 0; k < 3; ++k) {
+        const R a = x[3 * i + k];
+        const R e = ed ? a + (*ed)[3 * i + k] : a;
+        vbox[6 * (size_t)i + k] = std::min(a, e);
+        vbox[6 * (size_t)i + 3 + k] = std::max(a, e);
+      }
+  }
+  static bool box_overlap(const R* a, const R* b) {
+    return (a[0] <= b[3] && a[3] >= b[0]) && (a[1] <= b[4] && a[4] >= b[1]) && (a[2] <= b[5] && a[5] >= b[2]);
+  }
+  // Union box of vertex boxes.
+  void union_box(const int* vs, int cnt, R* o) const {
+    for (int k = 0; k < 3; ++k) {
+      o[k] = vbox[6 * (size_t)vs[0] + k];
+      o[3 + k] = vbox[6 * (size_t)vs[0] + 3 + k];
+    }
+    for (int c = 1; c < cnt; ++c)
+      for (int k = 0; k < 3; ++k) {
+        o[k] = std::min(o[k], vbox[6 * (size_t)vs[c] + k]);
+        o[3 + k] = std::max(o[3 + k], vbox[6 * (size_t)vs[c] + 3 + k]);
+      }
+  }
+  // Every (q, t) with overlapping boxes (6 R per box), t ascending for each q
+  // in ascending q.  A uniform grid (cell = the largest box extent) only
+  // enumerates candidates; the predicate is the exact box overlap, so the
+  // pairs are those of testing all of them.
+  template <class F>
+  static void grid_pairs(size_t nq, const std::vector<R>& qb, size_t nt, const std::vector<R>& tb, F visit) {
+    if (nq == 0 || nt == 0) return;
+    double h = 0.0, org[3] = {1e300, 1e300, 1e300};
+    for (size_t t = 0; t < nt; ++t)
+      for (int k = 0; k < 3; ++k) {
+        h = std::max(h, (double)tb[6 * t + 3 + k] - (double)tb[6 * t + k]);
+        org[k] = std::min(org[k], (double)tb[6 * t + k]);
+      }
+    for (size_t q = 0; q < nq; ++q)
+      for (int k = 0; k < 3; ++k) h = std::max(h, (double)qb[6 * q + 3 + k] - (double)qb[6 * q + k]);
+    if (!(h > 0.0)) h = 1.0;
+    auto cell = [&](double v, int k) { return (int64_t)std::floor((v - org[k]) / h); };
+    auto key = [](int64_t a, int64_t b, int64_t c) {
+      return ((uint64_t)(a & 0x1fffff) << 42) | ((uint64_t)(b & 0x1fffff) << 21) | (uint64_t)(c & 0x1fffff);
+    };
+    std::unordered_map<uint64_t, std::vector<int>> grid;
+    for (size_t t = 0; t < nt; ++t) {
+      int64_t lo[3], hi[3];
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = cell(tb[6 * t + k], k);
+        hi[k] = cell(tb[6 * t + 3 + k], k);
+      }
+      for (int64_t a = lo[0]; a <= hi[0]; ++a)
+        for (int64_t b = lo[1]; b <= hi[1]; ++b)
+          for (int64_t c = lo[2]; c <= hi[2]; ++c) grid[key(a, b, c)].push_back((int)t);
+    }
+    std::vector<int> cand;
+    for (size_t q = 0; q < nq; ++q) {
+      cand.clear();
+      int64_t lo[3], hi[3];
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = cell(qb[6 * q + k], k);
+        hi[k] = cell(qb[6 * q + 3 + k], k);
+      }
+      for (int64_t a = lo[0]; a <= hi[0]; ++a)
+        for (int64_t b = lo[1]; b <= hi[1]; ++b)
+          for (int64_t c = lo[2]; c <= hi[2]; ++c) {
+            auto it = grid.find(key(a, b, c));
+            if (it != grid.end()) cand.insert(cand.end(), it->second.begin(), it->second.end());
+          }
+      std::sort(cand.begin(), cand.end());
+      cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+      for (int t : cand)
+        if (box_overlap(&qb[6 * q], &tb[6 * (size_t)t])) visit((int)q, t);
+    }
+  }
+  // The step's candidate pairs (PT by vertex then triangle, EE by e1 then e2,
+  // rigid PT by body, vertex, triangle); gaps come from refresh_contacts.
+  void detect_cp(const std::vector<R>* ed, std::vector<Contact<R>>& out) {
+    out.clear();
+    cp_boxes(ed);
+    const R gm = P.gap_max;
+    auto grow = [gm](std::vector<R>& b) {
+      for (size_t i = 0; i < b.size() / 6; ++i)
+        for (int k = 0; k < 3; ++k) {
+          b[6 * i + k] = b[6 * i + k] - gm;
+          b[6 * i + 3 + k] = b[6 * i + 3 + k] + gm;
+        }
+    };
+    std::vector<R> qb(vbox), tb(6 * tris.size()), eb(6 * edges.size());
+    grow(qb);
+    for (size_t t = 0; t < tris.size(); ++t) union_box(tris[t].data(), 3, &tb[6 * t]);
+    for (size_t e = 0; e < edges.size(); ++e) union_box(edges[e].data(), 2, &eb[6 * e]);
+    std::vector<R> eq(eb);
+    grow(eq);
+    grid_pairs(n, qb, tris.size(), tb, [&](int v, int t) {
+      const auto& tr = tris[t];
+      if (tr[0] == v || tr[1] == v || tr[2] == v) return;
+      Contact<R> c;
+      c.type = PT; c.idx[0] = v; c.idx[1] = tr[0]; c.idx[2] = tr[1]; c.idx[3] = tr[2];
+      c.vcount = 4;
+      out.push_back(c);
+    });
+    grid_pairs(edges.size(), eq, edges.size(), eb, [&](int e1, int e2) {
+      if (e2 <= e1) return;
+      const auto& A = edges[e1];
+      const auto& B = edges[e2];
+      if (A[0] == B[0] || A[0] == B[1] || A[1] == B[0] || A[1] == B[1]) return;
+      Contact<R> c;
+      c.type = EE; c.idx[0] = A[0]; c.idx[1] = A[1]; c.idx[2] = B[0]; c.idx[3] = B[1];
+      c.vcount = 4;
+      out.push_back(c);
+    });
+    if (rworld.size() != rigids.size()) {
+      rworld.clear();
+      for (size_t rb = 0; rb < rigids.size(); ++rb) rworld.push_back(rigids[rb].world());
+    }
+    for (size_t rb = 0; rb < rigids.size(); ++rb) {
+      const auto& W = rworld[rb];
+      const auto& RT = rigids[rb].tris;
+      std::vector<R> rtb(6 * RT.size());
+      for (size_t t = 0; t < RT.size(); ++t)
+        for (int k = 0; k < 3; ++k) {
+          rtb[6 * t + k] = std::min(std::min(W[RT[t][0]][k], W[RT[t][1]][k]), W[RT[t][2]][k]);
+          rtb[6 * t + 3 + k] = std::max(std::max(W[RT[t][0]][k], W[RT[t][1]][k]), W[RT[t][2]][k]);
+        }
+      grid_pairs(n, qb, RT.size(), rtb, [&](int v, int t) {
+        Contact<R> c;
+        c.type = RIGID_PT; c.idx[0] = v; c.idx[1] = RT[t][0]; c.idx[2] = RT[t][1]; c.idx[3] = RT[t][2];
+        c.rigid = (int)rb;
+        c.vcount = 1;
+        out.push_back(c);
+      });
+    }
+  }
+  // Gap, normal, witness and weights of every candidate at the current x.
+  void refresh_contacts(std::vector<Contact<R>>& cs) const {
+#pragma omp parallel for schedule(static)
+    for (size_t k = 0; k < cs.size(); ++k) {
+      Contact<R>& c = cs[k];
+      if (c.type == PT) {
+        const V3<R> p = X(c.idx[0]), a = X(c.idx[1]), b = X(c.idx[2]), cc = X(c.idx[3]);
+        narrow_pt(p, a, b, cc, c.gap, c.normal, c.wq);
+        c.wp = p;
+        c.bary = triangle_barycentric(c.wq, a, b, cc);
+        c.w[0] = R(1); c.w[1] = -c.bary[0]; c.w[2] = -c.bary[1]; c.w[3] = -c.bary[2];
+      } else if (c.type == EE) {
+        R s_ = R(0), t_ = R(0);
+        narrow_ee(X(c.idx[0]), X(c.idx[1]), X(c.idx[2]), X(c.idx[3]), c.gap, c.normal, c.wp, c.wq, &s_, &t_);
+        c.w[0] = R(1) - s_; c.w[1] = s_; c.w[2] = -(R(1) - t_); c.w[3] = -t_;
+      } else if (c.type == RIGID_PT) {
+        const auto& W = rworld[c.rigid];
+        const V3<R> p = X(c.idx[0]);
+        narrow_pt(p, W[c.idx[1]], W[c.idx[2]], W[c.idx[3]], c.gap, c.normal, c.wq);
+        c.wp = p;
+        c.w[0] = R(1);
+      }
+    }
+  }
+  // LineSearch::search with the corrected feasibility rule; the candidates of
+  // the trial motion x -> x + 1.25 d replace `contacts`.
+  R line_search_cp(const std::vector<R>& d, const V3<R>& wn, R woff) {
+    std::vector<R> ed(3 * n);
+    for (int i = 0; i < 3 * n; ++i) ed[i] = R(1.25) * d[i];
+    detect_cp(&ed, contacts);
+    R toi = R(1);
+#pragma omp parallel for schedule(dynamic, 256) reduction(min : toi)
+    for (size_t k = 0; k < contacts.size(); ++k) {
+      const Contact<R>& c = contacts[k];
+      V3<R> x0[4], p[4];
+      if (c.type == RIGID_PT) {
+        const auto& W = rworld[c.rigid];
+        x0[0] = X(c.idx[0]);
+        p[0] = Xof(ed, c.idx[0]);
+        for (int j = 1; j < 4; ++j) {
+          x0[j] = W[c.idx[j]];
+          p[j] = V3<R>(0, 0, 0);
+        }
+      } else {
+        for (int j = 0; j < 4; ++j) {
+          x0[j] = X(c.idx[j]);
+          p[j] = Xof(ed, c.idx[j]);
+        }
+      }
+      const R t = additive_ccd(c.type == EE ? 1 : 0, x0, p);
+      toi = std::min(toi, t);
+    }
+    const bool wall = (double)sqn3(wn) > 0.5;
+    R alpha = R(1);
+    for (int it = 0; it < 20; ++it) {
+      bool ok = alpha <= toi;
+      if (ok && wall) {
+#pragma omp parallel for schedule(static) reduction(&& : ok)
+        for (int i = 0; i < n; ++i) {
+          V3<R> xi(x[3 * i] + alpha * ed[3 * i], x[3 * i + 1] + alpha * ed[3 * i + 1],
+                   x[3 * i + 2] + alpha * ed[3 * i + 2]);
+          const R g1 = dot3(wn, xi) - woff, g0 = dot3(wn, X(i)) - woff;
+          if (!(g1 >= std::min(R(0), g0))) ok = false;
+        }
+      }
+      if (ok) return alpha;
+      alpha *= R(0.5);
+      if (alpha < R(1e-6)) return R(0);
+    }
+    return alpha;
+  }
+
   // Integrator::inner_newton_step (integrator.cpp:152-236).
   R inner_newton(const std::vector<R>& xt) {
     int maxit = P.max_newton;
@@ -1542,6 +1923,7 @@ template <class R> struct Sim {
     std::vector<R> g, d;
     for (int it = 0; it < maxit; ++it) {
       ++st.newton_iters;
+      if (cp) refresh_contacts(contacts);
       compute_gradient(xt, g);
       R gn = vinf(g.data(), g.size());
       if (gn < P.pcg_tol) return R(1);
@@ -1563,7 +1945,7 @@ template <class R> struct Sim {
       R woff = R(0);
       for (const auto& w : walls)
         if (w.active) { wn = w.n; woff = w.off; break; }
-      R alpha = line_search(d, wn, woff, R(1.25));
+      R alpha = cp ? line_search_cp(d, wn, woff) : line_search(d, wn, woff, R(1.25));
       st.alphas.push_back(alpha);
       if ((double)alpha < 1e-8) { st.line_search_failures += 1; return R(0); }
       R s = (R)((double)alpha * 1.25);
@@ -1580,7 +1962,7 @@ template <class R> struct Sim {
     for (int i = 0; i < n; ++i)
       for (int k = 0; k < 3; ++k) {
         v[3 * i + k] += gd[k];
-        x[3 * i + k] += v[3 * i + k] * dt;
+        if (!cp) x[3 * i + k] += v[3 * i + k] * dt;  // corrected physics: x_hat carries it
       }
   }
 
@@ -1588,6 +1970,7 @@ template <class R> struct Sim {
     elastic_blocks();
     build_const();
     elastic_gradient();
+    if (cp) cp_constants();
   }
 
   // Integrator::step (integrator.cpp:70-150).
@@ -1597,7 +1980,12 @@ template <class R> struct Sim {
     refresh_constants();
     std::vector<R> xold = x, xt(3 * n);
     for (int i = 0; i < 3 * n; ++i) xt[i] = x[i] + dt * v[i];
-    detect(contacts);
+    if (cp) {
+      rworld.clear();
+      detect_cp(nullptr, contacts);
+    } else {
+      detect(contacts);
+    }
     st.contacts = (int)contacts.size();
     for (auto& c : contacts) if (c.type == RIGID_PT) ++st.rigid_contacts;
     R beta = R(0);
@@ -1610,7 +1998,17 @@ template <class R> struct Sim {
     }
     if ((double)beta > 1e-6) inner_newton(xt);
     st.beta = beta;
-    if ((double)beta > 1e-6) {
+    if (cp) {
+      // Corrected physics: the implicit-Euler velocity of the positions
+      // reached, whatever the Newton passes returned (the reference divides by
+      // beta dt, which doubles the velocity after a pass that hit its cap).
+      for (int i = 0; i < 3 * n; ++i) v[i] = (x[i] - xold[i]) / dt;
+      if ((double)P.damping > 0.0) {
+        R sc = R(1) - std::clamp(P.damping, R(0), R(1));
+        for (auto& vv : v) vv *= sc;
+      }
+      if ((double)P.restitution > 0.0) restitution();
+    } else if ((double)beta > 1e-6) {
       R bdt = beta * dt;
       for (int i = 0; i < 3 * n; ++i) v[i] = (x[i] - xold[i]) / bdt;
       if ((double)P.damping > 0.0) {
@@ -1689,6 +2087,25 @@ template <class R> struct Sim {
   // Integrator::apply_rigid_coupling (integrator.cpp:676-754).
   void rigid_coupling() {
     std::vector<Contact<R>> cs;
+    if (cp) {
+      // The step's candidates at the final positions; the body is pushed
+      // away from the cloth.
+      refresh_contacts(contacts);
+      for (auto& b : rigids) b.force = b.torque = V3<R>();
+      for (const auto& c : contacts) {
+        if (c.type != RIGID_PT || c.rigid < 0 || c.rigid >= (int)rigids.size()) continue;
+        R kb = contact_stiffness(c);
+        V3<R> nn = c.normal;
+        R nl = norm3(nn);
+        if ((double)nl < 1e-8) continue;
+        nn = nn / nl;
+        R dV = barrier_grad(c.gap, P.gap_max, kb);
+        if ((double)std::fabs(dV) < 1e-12) continue;
+        rigids[c.rigid].apply_force(c.wq, dV * nn);
+      }
+      for (auto& b : rigids) b.integrate(P.dt);
+      return;
+    }
     detect(cs);
     for (auto& b : rigids) b.force = b.torque = V3<R>();
     for (const auto& c : cs) {
@@ -1759,12 +2176,21 @@ void orc_destroy(void* h) {
 // vertex where the edge properly crosses the triangle (orientation tests in
 // double on the float coordinates), and cloth vertices with generalized
 // winding number above 1/2 w.r.t. a rigid body's surface.
+// Sign of orient3d with Shewchuk's static error bound ("Adaptive Precision
+// Floating-Point Arithmetic and Fast Robust Geometric Predicates", 1997,
+// o3derrboundA = (7 + 56 eps) eps, eps = 2^-53): 0 when rounding could have
+// decided the sign, so near-coplanar configurations count as degenerate
+// rather than as crossings.
 static int orient3d_sign(const double* a, const double* b, const double* c, const double* d) {
   const double ax = a[0] - d[0], ay = a[1] - d[1], az = a[2] - d[2];
   const double bx = b[0] - d[0], by = b[1] - d[1], bz = b[2] - d[2];
   const double cx = c[0] - d[0], cy = c[1] - d[1], cz = c[2] - d[2];
   const double v = ax * (by * cz - bz * cy) - ay * (bx * cz - bz * cx) + az * (bx * cy - by * cx);
-  return v > 0.0 ? 1 : (v < 0.0 ? -1 : 0);
+  const double perm = std::fabs(ax) * (std::fabs(by * cz) + std::fabs(bz * cy)) +
+                      std::fabs(ay) * (std::fabs(bx * cz) + std::fabs(bz * cx)) +
+                      std::fabs(az) * (std::fabs(bx * cy) + std::fabs(by * cx));
+  const double bound = 7.771561172376103e-16 * perm;
+  return v > bound ? 1 : (v < -bound ? -1 : 0);
 }
 
 int orc_count_intersections(void* h, int* edge_tri, int* inside) {
@@ -1934,6 +2360,17 @@ int orc_get_capture(void* h, double* H, double* b) {
     }
   });
   return got;
+}
+
+// Corrected physics on (1) or off (0); on implies the corrected contact mode.
+int orc_set_physics(void* h, int on) {
+  OSim* s = (OSim*)h;
+  return guard([&] {
+    DISPATCH(s, {
+      S.cp = on != 0;
+      if (S.cp) S.contact_mode = 1;
+    });
+  });
 }
 
 int orc_set_ccd_mode(void* h, int mode) {
@@ -2165,9 +2602,15 @@ int orc_linearize(void* h, const double* xhat, double* g_out, double* H_out) {
     DISPATCH(s, {
       using R = std::remove_reference_t<decltype(S.E)>;
       S.refresh_constants();
-      S.detect(S.contacts);
       std::vector<R> xt(3 * S.n), g;
       for (int i = 0; i < 3 * S.n; ++i) xt[i] = (R)xhat[i];
+      if (S.cp) {
+        S.rworld.clear();
+        S.detect_cp(nullptr, S.contacts);
+        S.refresh_contacts(S.contacts);
+      } else {
+        S.detect(S.contacts);
+      }
       S.compute_gradient(xt, g);
       S.assemble();
       for (int i = 0; i < 3 * S.n; ++i) g_out[i] = g[i];

@@ -81,6 +81,7 @@ def lib():
         L.orc_get_threads.restype = C.c_int
         L.orc_energy.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_set_ccd_mode.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_physics.argtypes = [C.c_void_p, C.c_int]
         L.orc_restitution.argtypes = [C.c_void_p]
         L.orc_set_capture.argtypes = [C.c_void_p, C.c_int]
         L.orc_get_capture.argtypes = [C.c_void_p, _dp, _dp]
@@ -221,6 +222,13 @@ class OracleSim:
         if not self.L.orc_get_capture(self.h, H, b):
             return None
         return H.reshape(nb, 3, 3), b
+
+    def set_physics(self, corrected: bool):
+        """Corrected physics (implicit gravity, membrane force from the current
+        positions, attracting pins, per-iteration contact barriers over the
+        step's swept-box candidates, CCD line search); implies the corrected
+        contact mode."""
+        _chk(self.L.orc_set_physics(self.h, 1 if corrected else 0))
 
     def set_ccd_mode(self, additive: bool):
         """Corrected mode line search: additive CCD (True) or the reference's
