@@ -29,6 +29,7 @@
 #include "learned.hip"
 #include "lbvh.hip"
 #include "frame.hip"
+#include "physics.hip"
 
 namespace ando {
 
@@ -281,6 +282,14 @@ struct ando_ctx {
   int corrected = 0;
   int static_rigids = 0;   // rigid bodies are fixed colliders: no coupling, no body update
   int additive_ccd = 0;    // corrected mode: additive CCD instead of the 10-sample test in the line search
+  // Corrected physics (ANDO_MODE_CORRECTED_PHYSICS; physics.hip).
+  int cp = 0;
+  float pin_k = 1e4f;      // attracting pin stiffness in units of m / dt^2
+  DBuf<float> fgk, fpi;    // per face: 2k B B^T and I - n n^T (6 floats each)
+  DBuf<float> Hiso3, Hbase;  // per block: isotropic elastic stiffness (3), plus mass / dt^2 (9)
+  DBuf<float> vbox, ed;    // per vertex: candidate boxes (6), trial motion 1.25 d (3)
+  DBuf<float> cfe, cegrad; // membrane gradient at x: per face (9), per vertex (3)
+  ando::DetectBodies cpD{};  // the step's rigid bodies (they do not move during a step)
   float pt_box = 1e-4f, keep_gap = 0.01f;
   std::vector<int32_t> edges_h;
   DBuf<int32_t> d_edges;
@@ -462,6 +471,8 @@ StepParams step_params(ando_ctx* c, const ando_params& p) {
   P.mu = p.friction_mu;
   P.friction_eps = p.friction_epsilon;
   P.friction_tangent = p.friction_tangent_threshold;
+  P.cp = c->cp;
+  P.pin_k = c->pin_k;
   return P;
 }
 
@@ -471,6 +482,19 @@ void ensure_const(ando_ctx* c, const ando_params& p) {
     float dtinv2 = 1.0f / (p.dt * p.dt);
     hipLaunchKernelGGL(k_fold_const, dim3(c->n), dim3(64), 0, s, c->n, c->d_row_ptr.p, c->d_cols.p, c->blk_cptr.p,
                        c->blk_contrib.p, c->ec.p, c->masses.p, dtinv2, c->Hconst.p, c->Helast3.p);
+    if (c->cp) {
+      c->fgk.ensure(6 * (size_t)std::max(c->m, 1));
+      c->fpi.ensure(6 * (size_t)std::max(c->m, 1));
+      c->Hiso3.ensure(3 * (size_t)c->nb);
+      c->Hbase.ensure(9 * (size_t)c->nb);
+      if (c->m)
+        hipLaunchKernelGGL(ando::cp::k_face_gk, dim3(nblk(c->m)), dim3(kBlock), 0, s, c->m, (const float*)c->dm.p,
+                           (const float*)c->area.p, c->mat.thickness, c->mu, c->fgk.p);
+      hipLaunchKernelGGL(ando::cp::k_iso_fold, dim3(c->n), dim3(64), 0, s, c->n, (const int32_t*)c->d_row_ptr.p,
+                         (const int32_t*)c->d_cols.p, (const int32_t*)c->blk_cptr.p,
+                         (const int32_t*)c->blk_contrib.p, (const float*)c->fgk.p, (const float*)c->masses.p, dtinv2,
+                         c->Hbase.p, c->Hiso3.p);
+    }
     HIPCHK(hipGetLastError());
     c->const_valid = true;
     c->const_dt = p.dt;
@@ -479,7 +503,7 @@ void ensure_const(ando_ctx* c, const ando_params& p) {
   if (c->num_walls && (!c->kw_valid || c->kw_wall_gap != p.wall_gap || c->kw_min_gap != p.min_gap)) {
     c->kw.ensure((size_t)c->num_walls * c->n);
     hipLaunchKernelGGL(k_wall_stiffness, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, c->num_walls, c->d_diag.p,
-                       c->Hconst.p, c->masses.p, c->d_wall_n.p, p.wall_gap, p.min_gap, c->kw.p);
+                       c->cp ? c->Hbase.p : c->Hconst.p, c->masses.p, c->d_wall_n.p, p.wall_gap, p.min_gap, c->kw.p);
     HIPCHK(hipGetLastError());
     c->kw_valid = true;
     c->kw_wall_gap = p.wall_gap;
@@ -619,14 +643,19 @@ void radix_sort(ando_ctx* c, int P, DBuf<uint32_t>& keys, DBuf<int32_t>& vals, D
 
 // LBVH over P primitives of `kind` vertices each (device index array prims):
 // boxes, Morton codes, stable radix sort, Karras hierarchy, node boxes.
-void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims) {
+// vbox (optional): per-vertex boxes whose unions are the primitive boxes;
+// otherwise the primitives' own boxes at the current positions.
+void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims, const float* vbox = nullptr) {
   using namespace ando::lbvh;
   hipStream_t s = c->stream;
   T.P = P;
   if (P == 0) return;
   const int G = nblk(P);
   T.boxes.ensure(6 * (size_t)P);
-  hipLaunchKernelGGL(k_prim_boxes, dim3(G), dim3(256), 0, s, P, kind, prims, (const float*)c->x.p, T.boxes.p);
+  if (vbox)
+    hipLaunchKernelGGL(k_prim_boxes_union, dim3(G), dim3(256), 0, s, P, kind, prims, vbox, T.boxes.p);
+  else
+    hipLaunchKernelGGL(k_prim_boxes, dim3(G), dim3(256), 0, s, P, kind, prims, (const float*)c->x.p, T.boxes.p);
   T.bpart.ensure(6 * (size_t)G);
   T.bounds.ensure(6);
   hipLaunchKernelGGL(k_centroid_bounds, dim3(G), dim3(256), 0, s, P, (const float*)T.boxes.p, T.bpart.p);
@@ -663,7 +692,7 @@ void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims)
 // not on the triangle (collision.cpp:78-86,198-244,279-387) -- the corrected
 // predicate restricted to one triangle; its three edges share vertices, so
 // there are no EE contacts.
-bool self_active(const ando_ctx* c) { return c->corrected || c->m == 1; }
+bool self_active(const ando_ctx* c) { return c->corrected || c->cp || c->m == 1; }
 
 // Self detection at the current positions: c->sc holds the PT contacts
 // (vertex, triangle order) followed by the EE contacts (e1, e2).
@@ -733,6 +762,97 @@ void detect_self(ando_ctx* c) {
   c->sc_total = npt + nee;
 }
 
+// Corrected physics: the candidate pairs of the per-vertex boxes of the
+// motion x -> x + ed (ed null: the points x), grown by gm on the query side.
+// Self candidates (PT by vertex then triangle, then EE by e1 then e2) replace
+// c->sc, rigid ones (per vertex by body then triangle) replace c->rc; both
+// carry indices only until refresh_cp.
+void detect_cp_rigid(ando_ctx* c, const ando::DetectBodies& D, float gm);
+void detect_cp(ando_ctx* c, const ando::DetectBodies& D, float gm, const float* ed) {
+  using namespace ando::lbvh;
+  hipStream_t s = c->stream;
+  ++c->det_gen;
+  c->sv_ready = false;
+  c->sc_pt = c->sc_total = 0;
+  c->rc_total = 0;
+  const int n = c->n, m = c->m, E = (int)c->edges_h.size() / 2;
+  c->vbox.ensure(6 * (size_t)n);
+  hipLaunchKernelGGL(ando::cp::k_vertex_boxes, dim3(nblk(n)), dim3(kBlock), 0, s, n, (const float*)c->x.p, ed,
+                     c->vbox.p);
+  HIPCHK(hipGetLastError());
+  lbvh_build(c, c->tri_tree, m, 3, c->d_tris.p, c->vbox.p);
+  lbvh_build(c, c->edge_tree, E, 2, c->d_edges.p, c->vbox.p);
+  c->sc_err.ensure(1);
+  HIPCHK(hipMemsetAsync(c->sc_err.p, 0, sizeof(uint32_t), s));
+  const int q = std::max(n, E);
+  c->sc_count.ensure((size_t)q + 1);
+  c->sc_ptr.ensure((size_t)q + 2);
+  c->sc_cache.ensure((size_t)q * ando::lbvh::kQueryCache + 1);
+  const LbvhTree& T = c->tri_tree;
+  const LbvhTree& Q = c->edge_tree;
+  auto pt = [&](int pass, ando::SelfContact* out, int32_t* key) {
+    hipLaunchKernelGGL(k_query_pt_box, dim3(nblk(n, 64)), dim3(64), 0, s, pass, n, (const float*)c->vbox.p, gm,
+                       (const int32_t*)c->d_tris.p, T.P, (const int32_t*)T.left.p, (const int32_t*)T.right.p,
+                       (const float*)T.cbox.p, (const float*)T.table.p, (const int32_t*)T.vals.p, c->sc_count.p,
+                       (const int32_t*)c->sc_ptr.p, out, key, c->sc_err.p, c->sc_cache.p);
+  };
+  auto ee = [&](int pass, ando::SelfContact* out, int32_t* key) {
+    hipLaunchKernelGGL(k_query_ee_box, dim3(nblk(E, 64)), dim3(64), 0, s, pass, E, (const float*)Q.boxes.p, gm,
+                       (const int32_t*)c->d_edges.p, Q.P, (const int32_t*)Q.left.p, (const int32_t*)Q.right.p,
+                       (const float*)Q.cbox.p, (const float*)Q.table.p, (const int32_t*)Q.vals.p, c->sc_count.p,
+                       (const int32_t*)c->sc_ptr.p, out, key, c->sc_err.p, c->sc_cache.p);
+  };
+  int npt = 0, nee = 0;
+  if (m > 0) {
+    pt(0, nullptr, nullptr);
+    npt = exclusive_scan(c, n, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
+  }
+  c->sc.ensure((size_t)npt + 1);
+  c->sc_key.ensure((size_t)npt + 1);
+  if (npt) pt(1, c->sc.p, c->sc_key.p);
+  if (E > 1) {
+    ee(0, nullptr, nullptr);
+    nee = exclusive_scan(c, E, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
+  }
+  if (nee) {
+    if (c->sc.n < (size_t)npt + nee) {
+      DBuf<ando::SelfContact> all;
+      all.ensure(((size_t)npt + nee) * 5 / 4 + 1);
+      if (npt)
+        HIPCHK(hipMemcpyAsync(all.p, c->sc.p, sizeof(ando::SelfContact) * npt, hipMemcpyDeviceToDevice, s));
+      c->sc = std::move(all);
+    }
+    c->sc_key.ensure((size_t)npt + nee);
+    ee(1, c->sc.p + npt, c->sc_key.p + npt);
+  }
+  HIPCHK(hipGetLastError());
+  uint32_t err = 0;
+  read_small(c, &err, (const uint32_t*)c->sc_err.p, 1);
+  if (err) throw Fail(ANDO_EDEVICE, "BVH traversal stack exceeded its depth");
+  c->sc_pt = npt;
+  c->sc_total = npt + nee;
+  detect_cp_rigid(c, D, gm);
+}
+
+// The rigid candidates of the vertex boxes in c->vbox (see detect_cp).
+void detect_cp_rigid(ando_ctx* c, const ando::DetectBodies& D, float gm) {
+  hipStream_t s = c->stream;
+  const int n = c->n;
+  c->rc_total = 0;
+  if (D.count > 0) {
+    c->rc_hits.ensure((size_t)n * kHitCache);
+    hipLaunchKernelGGL(ando::cp::k_rigid_cand, dim3(nblk(n)), dim3(kBlock), 0, s, n, 0, (const float*)c->vbox.p, gm, D,
+                       (const int32_t*)c->rc_ptr.p, c->rc_count.p, c->rc.p, c->rc_hits.p);
+    const int32_t total = exclusive_scan(c, n, c->rc_count.p, c->rc_ptr.p, c->scan_sums, c->scan_total);
+    c->rc_total = total;
+    c->rc.ensure((size_t)total + 1);
+    if (total)
+      hipLaunchKernelGGL(ando::cp::k_rigid_cand, dim3(nblk(n)), dim3(kBlock), 0, s, n, 1, (const float*)c->vbox.p, gm,
+                         D, (const int32_t*)c->rc_ptr.p, c->rc_count.p, c->rc.p, c->rc_hits.p);
+    HIPCHK(hipGetLastError());
+  }
+}
+
 ando::SelfView self_view(ando_ctx* c) {
   ando::SelfView v{nullptr, nullptr, nullptr, nullptr};
   if (c->sv_ready)
@@ -751,7 +871,7 @@ void prepare_self(ando_ctx* c, const StepParams& P) {
   c->sterm.ensure(nc);
   hipLaunchKernelGGL(k_self_terms, dim3(nblk(nc)), dim3(kBlock), 0, s, nc, c->n, P,
                      (const ando::SelfContact*)c->sc.p, (const float*)c->masses.p, (const int32_t*)c->d_row_ptr.p,
-                     (const int32_t*)c->d_cols.p, (const float*)c->Helast3.p, c->sterm.p);
+                     (const int32_t*)c->d_cols.p, (const float*)(c->cp ? c->Hiso3.p : c->Helast3.p), c->sterm.p);
   const int cnt = 4 * nc;
   c->sv_keys.ensure(cnt);
   c->sv_vals.ensure(cnt);
@@ -780,6 +900,24 @@ void prepare_self(ando_ctx* c, const StepParams& P) {
     hipLaunchKernelGGL(k_self_blocks, dim3(nblk(3LL * entries)), dim3(kBlock), 0, s, entries, self_view(c),
                        c->barrier_tol, c->ccol.p, c->cval.p);
   HIPCHK(hipGetLastError());
+}
+
+// Corrected physics: the candidates' gaps, normals, weights and (rigid)
+// stiffness at the current positions, then the self terms and their vertex
+// view (the set of contacts that add something changes with the gaps).
+void refresh_cp(ando_ctx* c, const ando_params& p) {
+  hipStream_t s = c->stream;
+  if (c->sc_total > 0)
+    hipLaunchKernelGGL(ando::cp::k_self_refresh, dim3(nblk(c->sc_total)), dim3(kBlock), 0, s, c->sc_total,
+                       (const float*)c->x.p, c->sc.p);
+  if (c->rc_total > 0) {
+    const float ks = std::max(std::max(p.contact_gap_max, p.min_gap), 1e-12f);
+    hipLaunchKernelGGL(ando::cp::k_rigid_refresh, dim3(nblk(c->rc_total)), dim3(kBlock), 0, s, c->rc_total,
+                       (const float*)c->x.p, c->cpD, c->rc.p, (const float*)c->masses.p, (const int32_t*)c->d_diag.p,
+                       (const float*)c->Hiso3.p, p.dt, ks);
+  }
+  HIPCHK(hipGetLastError());
+  prepare_self(c, step_params(c, p));
 }
 
 // PCG on the matrix in c->H (or a caller's buffers) with b = -g; result in c->d.
@@ -1316,15 +1454,29 @@ void launch_gradient(ando_ctx* c, const ando_params& p, bool with_rigid) {
       sp.thick = c->mat.thickness;
       sp.rho = c->mat.density;
       sp.fp64_near = c->fp64_near;
+      sp.cp = c->cp;
       hipLaunchKernelGGL(k_strain_face, dim3(nblk(c->m)), dim3(kBlock), 0, s, c->m, sp, c->d_tris.p, c->x.p,
-                         c->dm.p, c->area.p, c->face_blk.p, c->Helast3.p, c->sflags.p, c->sgrad.p, c->shess.p,
-                         c->scalars.p + 2);
+                         c->dm.p, c->area.p, c->face_blk.p, c->cp ? c->Hiso3.p : c->Helast3.p, c->sflags.p,
+                         c->sgrad.p, c->shess.p, c->scalars.p + 2);
     }
+  }
+  const float* egrad = c->egrad.p;
+  if (c->cp) {
+    // Membrane gradient and face projectors at the current positions.
+    c->cfe.ensure(9 * (size_t)std::max(c->m, 1));
+    c->cegrad.ensure(3 * (size_t)c->n);
+    if (c->m)
+      hipLaunchKernelGGL(ando::cp::k_face_x, dim3(nblk(c->m)), dim3(kBlock), 0, s, c->m, (const int32_t*)c->d_tris.p,
+                         (const float*)c->x.p, (const float*)c->dm.p, (const float*)c->area.p, c->mat.thickness, c->mu,
+                         c->cfe.p, c->fpi.p);
+    hipLaunchKernelGGL(k_egrad_gather, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, c->v2f_ptr.p, c->v2f.p, c->cfe.p,
+                       c->cegrad.p);
+    egrad = c->cegrad.p;
   }
   const int32_t* rcp = (with_rigid && c->rc_total > 0) ? c->rc_ptr.p : nullptr;
   const int32_t* pinp = c->num_pins ? c->pin_ptr.p : nullptr;
   hipLaunchKernelGGL(k_gradient, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, P, c->x.p, c->xhat.p, c->masses.p,
-                     c->egrad.p, c->v2f_ptr.p, c->v2f.p, c->sflags.p, c->sgrad.p, rcp, c->rc.p, pinp, c->pin_t.p,
+                     egrad, c->v2f_ptr.p, c->v2f.p, c->sflags.p, c->sgrad.p, rcp, c->rc.p, pinp, c->pin_t.p,
                      c->d_diag.p, c->Hconst.p, c->d_wall_n.p, c->d_wall_off.p, c->kw.p, c->xprev.p, c->g.p,
                      c->scalars.p, self_view(c));
   HIPCHK(hipGetLastError());
@@ -1343,7 +1495,7 @@ void launch_assemble(ando_ctx* c, const ando_params& p, bool with_rigid) {
   hipLaunchKernelGGL(k_assemble, dim3(nblk(c->nb)), dim3(kBlock), 0, c->stream, c->n, P, c->d_block_row.p,
                      c->d_cols.p, c->blk_cptr.p, c->blk_contrib.p, c->Hconst.p, c->sflags.p, c->shess.p, c->x.p,
                      c->masses.p, rcp, c->rc.p, pinp, c->pin_t.p, c->d_wall_n.p, c->d_wall_off.p, c->kw.p, c->xprev.p, c->H.p,
-                     c->nb, self_view(c));
+                     c->nb, self_view(c), (const float*)c->fgk.p, (const float*)c->fpi.p);
   HIPCHK(hipGetLastError());
 }
 
@@ -1377,6 +1529,45 @@ float line_search(ando_ctx* c) {
   return 0.0f;
 }
 
+// Corrected physics line search: the candidates of the trial motion
+// x -> x + 1.25 d replace the step's candidates; alpha_t = 2^-t is feasible
+// when it is at most every candidate's additive-CCD time of impact and no
+// vertex ends up more violated at the first wall than it is (g1 >= min(0, g0)).
+float line_search_cp(ando_ctx* c, const ando_params& p) {
+  hipStream_t s = c->stream;
+  const int n3 = 3 * c->n;
+  c->ed.ensure((size_t)n3);
+  hipLaunchKernelGGL(ando::cp::k_scale, dim3(nblk(n3)), dim3(kBlock), 0, s, n3, 1.25f, (const float*)c->d.p, c->ed.p);
+  HIPCHK(hipGetLastError());
+  detect_cp(c, c->cpD, p.contact_gap_max, c->ed.p);
+  bool wall = c->num_walls > 0;
+  f3 w = mk(0.0f, 0.0f, 0.0f);
+  if (wall) {
+    const float* wn = c->wall_n.data();
+    w = mk(wn[0], wn[1], wn[2]);
+    double sq = (double)(w.x * w.x + (w.y * w.y + w.z * w.z));
+    wall = sq > 0.5;
+  }
+  HIPCHK(hipMemsetD32Async(c->scalars.p + 1, 0xFFFFF, 1, s));
+  if (c->sc_total > 0)
+    hipLaunchKernelGGL(ando::cp::k_ls_toi_self, dim3(nblk(c->sc_total)), dim3(kBlock), 0, s, c->sc_total,
+                       (const ando::SelfContact*)c->sc.p, (const float*)c->x.p, (const float*)c->ed.p,
+                       c->scalars.p + 1);
+  if (c->rc_total > 0)
+    hipLaunchKernelGGL(ando::cp::k_ls_toi_rigid, dim3(nblk(c->rc_total)), dim3(kBlock), 0, s, c->rc_total,
+                       (const RigidContact*)c->rc.p, c->cpD, (const float*)c->x.p, (const float*)c->ed.p,
+                       c->scalars.p + 1);
+  if (wall)
+    hipLaunchKernelGGL(ando::cp::k_ls_wall, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, w, c->wall_off[0],
+                       (const float*)c->x.p, (const float*)c->ed.p, c->scalars.p + 1);
+  HIPCHK(hipGetLastError());
+  uint32_t mask = 0;
+  read_small(c, &mask, (const uint32_t*)(c->scalars.p + 1), 1);
+  for (int t = 0; t < 20; ++t)
+    if (mask & (1u << t)) return std::ldexp(1.0f, -t);
+  return 0.0f;
+}
+
 // Integrator::inner_newton_step (integrator.cpp:152-236).
 float inner_newton(ando_ctx* c, const ando_params& p) {
   int maxit = p.max_newton_steps;
@@ -1384,6 +1575,7 @@ float inner_newton(ando_ctx* c, const ando_params& p) {
   hipStream_t s = c->stream;
   for (int it = 0; it < maxit; ++it) {
     c->stats.newton_iterations++;
+    if (c->cp) refresh_cp(c, p);
     launch_gradient(c, p, true);
     float gmax = read_gmax(c);
     if (gmax < p.pcg_tol) return 1.0f;
@@ -1416,7 +1608,7 @@ float inner_newton(ando_ctx* c, const ando_params& p) {
     c->stats.pcg_solves++;
     c->stats.pcg_iterations += po.iters;
     if (!po.converged) c->stats.pcg_failures++;
-    float alpha = line_search(c);
+    float alpha = c->cp ? line_search_cp(c, p) : line_search(c);
     c->stats.last_alpha = alpha;
     if ((double)alpha < 1e-8) {
       c->stats.line_search_failures++;
@@ -1481,10 +1673,29 @@ void rigid_integrate(ando_rigid_body& B, const double* force, const double* torq
 // The per-body sums run on the device (k_rigid_forces); only 6 doubles per
 // body come back, and the body update itself (a few dozen flops per body, on
 // the caller's ando_rigid_body) stays on the host.
+// Corrected physics: the contacts within gap_max of the final positions (the
+// point boxes grown by gap_max) with the corrected stiffness, and the force
+// V' n, which pushes the body away from the cloth.
 void rigid_coupling(ando_ctx* c, const ando_params& p, ando_rigid_body* bodies, int nbod) {
   upload_bodies(c, bodies, nbod);
-  detect_rigid(c, bodies, nbod, p.dt);
   hipStream_t s = c->stream;
+  if (c->cp) {
+    c->cpD = detect_bodies(c, bodies);
+    c->vbox.ensure(6 * (size_t)c->n);
+    hipLaunchKernelGGL(ando::cp::k_vertex_boxes, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, (const float*)c->x.p,
+                       (const float*)nullptr, c->vbox.p);
+    ++c->det_gen;
+    detect_cp_rigid(c, c->cpD, p.contact_gap_max);
+    if (c->rc_total > 0) {
+      const float ks = std::max(std::max(p.contact_gap_max, p.min_gap), 1e-12f);
+      hipLaunchKernelGGL(ando::cp::k_rigid_refresh, dim3(nblk(c->rc_total)), dim3(kBlock), 0, s, c->rc_total,
+                         (const float*)c->x.p, c->cpD, c->rc.p, (const float*)c->masses.p,
+                         (const int32_t*)c->d_diag.p, (const float*)c->Hiso3.p, p.dt, ks);
+    }
+    HIPCHK(hipGetLastError());
+  } else {
+    detect_rigid(c, bodies, nbod, p.dt);
+  }
   BodyFrames bf{};
   for (int b = 0; b < nbod; ++b) {
     for (int k = 0; k < 3; ++k) bf.pos[b][k] = bodies[b].position[k];
@@ -1494,7 +1705,7 @@ void rigid_coupling(ando_ctx* c, const ando_params& p, ando_rigid_body* bodies, 
   c->rw_part.ensure((size_t)G * nbod * 6);
   c->rw_out.ensure((size_t)nbod * 6);
   hipLaunchKernelGGL(k_rigid_forces, dim3(G), dim3(kBlock), 0, s, c->rc_total, (const RigidContact*)c->rc.p, nbod,
-                     bf, p.contact_gap_max, c->rw_part.p);
+                     bf, p.contact_gap_max, c->cp ? 1.0f : -1.0f, c->rw_part.p);
   hipLaunchKernelGGL(k_rigid_forces_final, dim3(1), dim3(kBlock), 0, s, G, nbod, (const double*)c->rw_part.p,
                      c->rw_out.p);
   HIPCHK(hipGetLastError());
@@ -1835,14 +2046,20 @@ int ando_set_mode(ando_ctx* c, int32_t flags) {
     need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING |
                     ANDO_MODE_LEARNED_PRECOND | ANDO_MODE_LEARNED_WARM_START_ONLY |
                     ANDO_MODE_CORRECTED_CONTACT | ANDO_MODE_PCG_ROW_SPMV | ANDO_MODE_STATIC_RIGIDS |
-                    ANDO_MODE_ADDITIVE_CCD)) == 0,
+                    ANDO_MODE_ADDITIVE_CCD | ANDO_MODE_CORRECTED_PHYSICS)) == 0,
          ANDO_EINVAL, "unknown mode flag");
     need(!(flags & ANDO_MODE_LEARNED_PRECOND) || c->nn.ready, ANDO_ESTATE,
          "learned preconditioner mode needs ando_set_precond_weights first");
     c->fp64_near = (flags & ANDO_MODE_FP64_NEAR_CONTACT) ? 1 : 0;
     c->kernel_timing = (flags & ANDO_MODE_KERNEL_TIMING) ? 1 : 0;
     c->learned = (flags & ANDO_MODE_LEARNED_PRECOND) ? ((flags & ANDO_MODE_LEARNED_WARM_START_ONLY) ? 2 : 1) : 0;
-    c->corrected = (flags & ANDO_MODE_CORRECTED_CONTACT) ? 1 : 0;
+    const int cp = (flags & ANDO_MODE_CORRECTED_PHYSICS) ? 1 : 0;
+    if (cp != c->cp) {
+      c->const_valid = false;
+      c->kw_valid = false;
+    }
+    c->cp = cp;
+    c->corrected = ((flags & ANDO_MODE_CORRECTED_CONTACT) || cp) ? 1 : 0;
     if (c->spmv_env < 0) c->spmv_sell = (flags & ANDO_MODE_PCG_ROW_SPMV) ? 0 : 1;
     c->static_rigids = (flags & ANDO_MODE_STATIC_RIGIDS) ? 1 : 0;
     c->additive_ccd = (flags & ANDO_MODE_ADDITIVE_CCD) ? 1 : 0;
@@ -1855,8 +2072,9 @@ int ando_apply_gravity(ando_ctx* c, const float* g3, float dt) {
     need(c && c->have_state, ANDO_ESTATE, "State has not been initialised");
     need(g3 != nullptr, ANDO_EINVAL, "gravity is null");
     HIPCHK(hipSetDevice(c->device));
+    // Corrected physics: velocity only (the step's x_hat = x + dt v carries it).
     hipLaunchKernelGGL(k_gravity, dim3(nblk(c->n)), dim3(kBlock), 0, c->stream, c->n, g3[0], g3[1], g3[2], dt,
-                       c->x.p, c->v.p);
+                       c->x.p, c->v.p, c->cp ? 0 : 1);
     HIPCHK(hipGetLastError());
   });
 }
@@ -1877,10 +2095,15 @@ int ando_step(ando_ctx* c, const ando_params* params, ando_rigid_body* bodies, i
                        c->xold.p, c->xhat.p);
     HIPCHK(hipGetLastError());
     upload_bodies(c, bodies, num_bodies);
-    detect_rigid(c, bodies, num_bodies, p.dt);
     c->barrier_tol = p.barrier_tolerance;
-    detect_self(c);
-    prepare_self(c, step_params(c, p));
+    if (c->cp) {
+      c->cpD = detect_bodies(c, bodies);
+      detect_cp(c, c->cpD, p.contact_gap_max, nullptr);
+    } else {
+      detect_rigid(c, bodies, num_bodies, p.dt);
+      detect_self(c);
+      prepare_self(c, step_params(c, p));
+    }
     c->stats.contacts = c->rc_total + c->sc_total;
     c->stats.rigid_contacts = c->rc_total;
     float beta = 0.0f;
@@ -1893,7 +2116,16 @@ int ando_step(ando_ctx* c, const ando_params* params, ando_rigid_body* bodies, i
     }
     if ((double)beta > 1e-6) inner_newton(c, p);
     c->stats.beta = beta;
-    if ((double)beta > 1e-6) {
+    if (c->cp) {
+      // The implicit-Euler velocity of the positions reached.
+      float damp = p.velocity_damping;
+      int do_damp = (double)damp > 0.0 ? 1 : 0;
+      float scale = 1.0f - std::min(std::max(damp, 0.0f), 1.0f);
+      hipLaunchKernelGGL(ando::cp::k_velocity_dt, dim3(nblk(3LL * c->n)), dim3(kBlock), 0, s, 3 * c->n, p.dt, scale,
+                         do_damp, (const float*)c->x.p, (const float*)c->xold.p, c->v.p);
+      HIPCHK(hipGetLastError());
+      if ((double)p.contact_restitution > 0.0) restitution(c, p, bodies, num_bodies);
+    } else if ((double)beta > 1e-6) {
       float bdt = beta * p.dt;
       float damp = p.velocity_damping;
       int do_damp = (double)damp > 0.0 ? 1 : 0;
@@ -2011,10 +2243,16 @@ int ando_linearize(ando_ctx* c, const ando_params* params, const float* x_target
     ensure_const(c, p);
     c->xhat.upload(x_target, 3 * (size_t)c->n, c->stream);
     upload_bodies(c, bodies, num_bodies);
-    detect_rigid(c, bodies, num_bodies, p.dt);
     c->barrier_tol = p.barrier_tolerance;
-    detect_self(c);
-    prepare_self(c, step_params(c, p));
+    if (c->cp) {
+      c->cpD = detect_bodies(c, bodies);
+      detect_cp(c, c->cpD, p.contact_gap_max, nullptr);
+      refresh_cp(c, p);
+    } else {
+      detect_rigid(c, bodies, num_bodies, p.dt);
+      detect_self(c);
+      prepare_self(c, step_params(c, p));
+    }
     launch_gradient(c, p, true);
     launch_assemble(c, p, true);
     if (gradient)

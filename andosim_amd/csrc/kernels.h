@@ -92,6 +92,11 @@ struct StepParams {
   float mu;
   float friction_eps;
   float friction_tangent;
+  // Corrected physics (ANDO_MODE_CORRECTED_PHYSICS): contact stiffness on the
+  // wall's inertial scale m / max(gap_max, min_gap)^2, and attracting pins of
+  // stiffness pin_k m / dt^2.
+  int32_t cp;
+  float pin_k;
 };
 
 // Friction of one contact on its first vertex (integrator.cpp:28-66,
@@ -113,10 +118,17 @@ __device__ inline bool friction_data(const RigidContact& c, f3 disp, const StepP
   return friction_data(mk(c.nx, c.ny, c.nz), c.k, c.gap, disp, P, kf, tan);
 }
 
+// Index of entry (a, b) of a symmetric 3x3 stored as 00 01 02 11 12 22.
+__host__ __device__ inline int sym6(int a, int b) {
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  return lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+}
+
 struct StrainParams {
   float tau, eps, min_gap, svd_eps;
   float thick, rho;
   int32_t fp64_near;
+  int32_t cp;  // corrected physics: stiffness scale eps instead of max(|gap|, min_gap)
 };
 
 }  // namespace ando

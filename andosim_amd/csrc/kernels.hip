@@ -14,6 +14,31 @@
 
 namespace ando {
 
+// Elasticity::compute_gradient for one face (elasticity.cpp:18-61): F = Ds
+// Dm_inv in the local frame (t1 along e1, t2 = n x t1) of the vertices given,
+// P = 2k (F - I), forces of P Dm_inv^T mapped back through the frame; writes
+// the three vertex contributions to out[0..8] and the unit normal to nn.
+__device__ inline void membrane_grad(f3 v0, f3 v1, f3 v2, float M00, float M01, float M10, float M11, float k,
+                                     float* __restrict__ out, f3& nn) {
+  f3 e1 = v1 - v0, e2 = v2 - v0;
+  nn = normalized(cross(e1, e2));
+  f3 t1 = normalized(e1);
+  f3 t2 = cross(nn, t1);
+  float D00 = dot(e1, t1), D01 = dot(e2, t1), D10 = dot(e1, t2), D11 = dot(e2, t2);
+  float F00 = D00 * M00 + D01 * M10, F01 = D00 * M01 + D01 * M11;
+  float F10 = D10 * M00 + D11 * M10, F11 = D10 * M01 + D11 * M11;
+  float twok = (float)(2.0 * (double)k);
+  float P00 = twok * (F00 - 1.0f), P01 = twok * (F01 - 0.0f), P10 = twok * (F10 - 0.0f), P11 = twok * (F11 - 1.0f);
+  float H00 = P00 * M00 + P01 * M01, H01 = P00 * M10 + P01 * M11;
+  float H10 = P10 * M00 + P11 * M01, H11 = P10 * M10 + P11 * M11;
+  f3 f1 = H00 * t1 + H10 * t2;
+  f3 f2 = H01 * t1 + H11 * t2;
+  f3 f0 = -(f1 + f2);
+  st3(out, 0, f0);
+  st3(out, 1, f1);
+  st3(out, 2, f2);
+}
+
 // ---------------------------------------------------------------------------
 // Setup: per-face elastic constants and the rest-state elastic gradient.
 // Elasticity::face_hessian + enforce_spd (elasticity.cpp:63-91,135-181) and
@@ -46,24 +71,8 @@ __global__ void k_elastic_face(int m, const int32_t* __restrict__ tris, const fl
   }
   // Rest-state gradient: F = Ds Dm_inv in the face's local frame.
   int i0 = tris[3 * f], i1 = tris[3 * f + 1], i2 = tris[3 * f + 2];
-  f3 v0 = ld3(mverts, i0), v1 = ld3(mverts, i1), v2 = ld3(mverts, i2);
-  f3 e1 = v1 - v0, e2 = v2 - v0;
-  f3 nn = normalized(cross(e1, e2));
-  f3 t1 = normalized(e1);
-  f3 t2 = cross(nn, t1);
-  float D00 = dot(e1, t1), D01 = dot(e2, t1), D10 = dot(e1, t2), D11 = dot(e2, t2);
-  float F00 = D00 * M00 + D01 * M10, F01 = D00 * M01 + D01 * M11;
-  float F10 = D10 * M00 + D11 * M10, F11 = D10 * M01 + D11 * M11;
-  float twok = (float)(2.0 * (double)k);
-  float P00 = twok * (F00 - 1.0f), P01 = twok * (F01 - 0.0f), P10 = twok * (F10 - 0.0f), P11 = twok * (F11 - 1.0f);
-  float H00 = P00 * M00 + P01 * M01, H01 = P00 * M10 + P01 * M11;
-  float H10 = P10 * M00 + P11 * M01, H11 = P10 * M10 + P11 * M11;
-  f3 f1 = H00 * t1 + H10 * t2;
-  f3 f2 = H01 * t1 + H11 * t2;
-  f3 f0 = -(f1 + f2);
-  st3(fe, 3 * f + 0, f0);
-  st3(fe, 3 * f + 1, f1);
-  st3(fe, 3 * f + 2, f2);
+  f3 nn;
+  membrane_grad(ld3(mverts, i0), ld3(mverts, i1), ld3(mverts, i2), M00, M01, M10, M11, k, fe + 9 * (size_t)f, nn);
 }
 
 // egrad[i] = sum of the face forces on vertex i in face order.
@@ -131,15 +140,17 @@ __global__ void k_wall_stiffness(int n, int nw, const int32_t* __restrict__ diag
 // ---------------------------------------------------------------------------
 // Step prologue / epilogue.
 // ---------------------------------------------------------------------------
-// State.apply_gravity (bindings.cpp:284-300): v += g dt, x += v dt.
+// State.apply_gravity (bindings.cpp:284-300): v += g dt, x += v dt.  With
+// move_x = 0 (corrected physics) only v changes; the step's prediction
+// x_hat = x + dt v carries the position update.
 __global__ void k_gravity(int n, float gx, float gy, float gz, float dt, float* __restrict__ x,
-                          float* __restrict__ v) {
+                          float* __restrict__ v, int move_x) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   f3 gd = mk(gx * dt, gy * dt, gz * dt);
   f3 vi = ld3(v, i) + gd;
   st3(v, i, vi);
-  st3(x, i, ld3(x, i) + dt * vi);
+  if (move_x) st3(x, i, ld3(x, i) + dt * vi);
 }
 
 // x_old = x; x_hat = x + dt v (integrator.cpp:78-92).
@@ -207,14 +218,15 @@ __global__ void k_rigid_groups(int nt, int grp, const int32_t* __restrict__ tris
 }
 
 // Contact stiffness of a one-vertex rigid contact (stiffness.cpp:53-116):
-// m/dt^2 + max(0, n^T E n), E the elastic-only diagonal block of the vertex.
-__device__ inline float rigid_contact_k(float mass, f3 E, f3 normal, float dt) {
+// m/s^2 + max(0, n^T E n), E the elastic-only diagonal block of the vertex,
+// s = dt (the reference) or max(gap_max, min_gap) (corrected physics).
+__device__ inline float rigid_contact_k(float mass, f3 E, f3 normal, float dt, float s) {
   if (dt <= 0.0f) return 0.0f;
   float nl = nrm(normal);
   if ((double)nl < 1e-12) return 0.0f;
   f3 nn = normal / nl;
   float inertial = 0.0f + mass * 1.0f * 1.0f;
-  inertial /= (dt * dt);
+  inertial /= (s * s);
   float h0 = E.x * nn.x, h1 = E.y * nn.y, h2 = E.z * nn.z;
   float el = nn.x * h0;
   el = el + nn.y * h1;
@@ -256,7 +268,7 @@ __device__ inline void write_rigid_contact(RigidContact& o, const PTResult& r, f
   o.gap = r.dist;
   o.nx = r.normal.x; o.ny = r.normal.y; o.nz = r.normal.z;
   o.qx = r.wq.x; o.qy = r.wq.y; o.qz = r.wq.z;
-  o.k = rigid_contact_k(mass, E, r.normal, dt);
+  o.k = rigid_contact_k(mass, E, r.normal, dt, dt);
   o.body = b;
   o.tri = t;
   o.vertex = i;
@@ -359,8 +371,11 @@ struct Wrench {
   }
 };
 
+// sign: -1 for the reference's -V' n, +1 (corrected physics) for V' n, which
+// pushes the body away from the cloth.
 __global__ void __launch_bounds__(kBlock) k_rigid_forces(int count, const RigidContact* __restrict__ rc, int nbod,
-                                                         BodyFrames bf, float gmax, double* __restrict__ part) {
+                                                         BodyFrames bf, float gmax, float sign,
+                                                         double* __restrict__ part) {
   __shared__ Wrench lds[kBlock];
   for (int b = 0; b < nbod; ++b) {
     Wrench a;
@@ -380,7 +395,7 @@ __global__ void __launch_bounds__(kBlock) k_rigid_forces(int count, const RigidC
       }
       if ((double)fabsf(dV) < 1e-12) continue;
       if (bf.mass[b] <= 0.0f) continue;
-      const float f[3] = {-(dV * nv[0]), -(dV * nv[1]), -(dV * nv[2])};
+      const float f[3] = {sign * (dV * nv[0]), sign * (dV * nv[1]), sign * (dV * nv[2])};
       const float rr[3] = {r.qx - bf.pos[b][0], r.qy - bf.pos[b][1], r.qz - bf.pos[b][2]};
       for (int k = 0; k < 3; ++k) a.v[k] += (double)f[k];
       a.v[3] += (double)(rr[1] * f[2] - rr[2] * f[1]);
@@ -574,7 +589,7 @@ __global__ void k_strain_face(int m, StrainParams sp, const int32_t* __restrict_
         for (int s = 0; s < 2; ++s) {
           float gap = (1.0f + sp.tau + sp.eps) - sig[s];
           if (!in_domain(gap, sp.eps)) continue;
-          float gc = fmaxf(fabsf(gap), sp.min_gap);
+          float gc = sp.cp ? sp.eps : fmaxf(fabsf(gap), sp.min_gap);
           float kk = fmass / (gc * gc) + el;
           fl |= 1 << s;
           ++local_active;
@@ -685,7 +700,11 @@ __global__ void k_gradient(int n, StepParams P, const float* __restrict__ x, con
         float s = dV * 1.0f;
         gi = gi + s * nn;
       }
-    if (pin_ptr && pin_ptr[i] < pin_ptr[i + 1]) {
+    if (pin_ptr && pin_ptr[i] < pin_ptr[i + 1] && P.cp) {
+      // Corrected physics: attracting pins, (pin_k m / dt^2) (x - target).
+      const float kp = P.pin_k * (masses[i] / (P.dt * P.dt));
+      for (int e = pin_ptr[i]; e < pin_ptr[i + 1]; ++e) gi = gi + kp * (xi - ld3(pin_t, e));
+    } else if (pin_ptr && pin_ptr[i] < pin_ptr[i + 1]) {
       const float* B = Hconst + 9 * (size_t)diag[i];
       f3 Hd = mk(B[0], B[4], B[8]);
       for (int e = pin_ptr[i]; e < pin_ptr[i + 1]; ++e) {
@@ -749,13 +768,37 @@ __global__ void k_assemble(int n, StepParams P, const int32_t* __restrict__ bloc
                            const RigidContact* __restrict__ rc, const int32_t* __restrict__ pin_ptr,
                            const float* __restrict__ pin_t, const float* __restrict__ wall_n,
                            const float* __restrict__ wall_off, const float* __restrict__ kw,
-                           const float* __restrict__ xprev, float* __restrict__ H, int nb, SelfView sv) {
+                           const float* __restrict__ xprev, float* __restrict__ H, int nb, SelfView sv,
+                           const float* __restrict__ fgk, const float* __restrict__ fpi) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   float h[9];
   const float* hc = Hconst + 9 * (size_t)b;
+  if (P.cp) {
+    // Corrected physics: mass / dt^2, then each face's 2k G_ab (I - n n^T) in
+    // face order.
 #pragma unroll
-  for (int e = 0; e < 9; ++e) h[e] = hc[e];
+    for (int e = 0; e < 9; ++e) h[e] = 0.0f;
+    if (cols[b] == block_row[b]) {
+      const float mf = masses[block_row[b]] * (1.0f / (P.dt * P.dt));
+      if (mf != 0.0f) {
+        h[0] = h[0] + mf;
+        h[4] = h[4] + mf;
+        h[8] = h[8] + mf;
+      }
+    }
+    for (int e = blk_cptr[b]; e < blk_cptr[b + 1]; ++e) {
+      const int slot = blk_contrib[e];
+      const int f = slot / 9, ab = slot - 9 * f;
+      const float g = fgk[6 * (size_t)f + sym6(ab / 3, ab % 3)];
+      const float* pi = fpi + 6 * (size_t)f;
+#pragma unroll
+      for (int e2 = 0; e2 < 9; ++e2) h[e2] = h[e2] + g * pi[sym6(e2 / 3, e2 % 3)];
+    }
+  } else {
+#pragma unroll
+    for (int e = 0; e < 9; ++e) h[e] = hc[e];
+  }
   if (P.strain)
     for (int e = blk_cptr[b]; e < blk_cptr[b + 1]; ++e) {
       int slot = blk_contrib[e];
@@ -807,7 +850,14 @@ __global__ void k_assemble(int n, StepParams P, const int32_t* __restrict__ bloc
           h[e2] = h[e2] + val;
         }
       }
-    if (pin_ptr && pin_ptr[i] < pin_ptr[i + 1]) {
+    if (pin_ptr && pin_ptr[i] < pin_ptr[i + 1] && P.cp) {
+      const float kp = P.pin_k * (masses[i] / (P.dt * P.dt));
+      for (int e = pin_ptr[i]; e < pin_ptr[i + 1]; ++e) {
+        h[0] = h[0] + kp;
+        h[4] = h[4] + kp;
+        h[8] = h[8] + kp;
+      }
+    } else if (pin_ptr && pin_ptr[i] < pin_ptr[i + 1]) {
       f3 xi = ld3(x, i);
       f3 Hd = mk(hc[0], hc[4], hc[8]);
       for (int e = pin_ptr[i]; e < pin_ptr[i + 1]; ++e) {

