@@ -1318,6 +1318,8 @@ PYBIND11_MODULE(ando_barrier_core, m) {
       .def_readonly("spmv_wg_slices", &ando_stats::spmv_wg_slices)
       .def_readonly("pcg_vec_block", &ando_stats::pcg_vec_block)
       .def_readonly("pcg_autotuned", &ando_stats::pcg_autotuned)
+      .def_readonly("long_hit_sorts", &ando_stats::long_hit_sorts)
+      .def_readonly("max_candidates", &ando_stats::max_candidates)
       .def_readonly("learned_solves", &ando_stats::learned_solves)
       .def_readonly("learned_fallbacks", &ando_stats::learned_fallbacks)
       .def_readonly("fallback_iterations", &ando_stats::fallback_iterations)

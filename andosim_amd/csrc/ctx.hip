@@ -298,6 +298,9 @@ struct ando_ctx {
   DBuf<int32_t> sc_key, sc_count, sc_ptr, sc_sums, sc_total_d;
   DBuf<int32_t> sc_cache;  // per query, the first kQueryCache hits of the count pass
   DBuf<uint32_t> sc_err;
+  DBuf<unsigned long long> cand64;  // 64-bit total of a detection's hit counts
+  DBuf<ando::SelfContact> sc_tmp;   // segmented sort of long hit lists: gathered records
+  DBuf<int32_t> sc_ktmp;            // and their keys
   int sc_pt = 0, sc_total = 0;
   DBuf<ando::SelfTerm> sterm;
   DBuf<uint32_t> sv_keys, sv_keys2;
@@ -641,6 +644,69 @@ void radix_sort(ando_ctx* c, int P, DBuf<uint32_t>& keys, DBuf<int32_t>& vals, D
   }
 }
 
+// Candidates one detection may hold: beyond this a trial motion spans so
+// much of the mesh that the lists would take tens of GB.
+constexpr unsigned long long kMaxCandidates = 1ull << 27;
+
+// Exclusive scan of a detection pass's hit counts, the total checked in 64
+// bits against kMaxCandidates (the int32 scan would wrap).
+int32_t candidate_scan(ando_ctx* c, int n, const int32_t* in, int32_t* out, DBuf<int32_t>& sums,
+                       DBuf<int32_t>& total) {
+  hipStream_t s = c->stream;
+  c->cand64.ensure(1);
+  HIPCHK(hipMemsetAsync(c->cand64.p, 0, sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(ando::lbvh::k_sum_counts64, dim3(std::min(nblk(n), 1024)), dim3(256), 0, s, n, in,
+                     c->cand64.p);
+  exclusive_scan(c, n, in, out, sums, total, false);
+  unsigned long long t = 0;
+  read_small(c, &t, (const unsigned long long*)c->cand64.p, 1);
+  if (t > kMaxCandidates)
+    throw Fail(ANDO_EDEVICE, "contact candidates of one detection exceed 2^27 (" + std::to_string(t) +
+                                 "): a trial motion spans too much of the mesh");
+  c->stats.max_candidates = std::max(c->stats.max_candidates, (int32_t)t);
+  return (int32_t)t;
+}
+
+// The `count` hits of one pass at sc[base..], grouped by query (ptr: the
+// queries' exclusive scan, nq + 1 entries), sorted by (query, key) when a
+// query had more hits than it sorts itself.
+void sort_long_hits(ando_ctx* c, int base, int count, int nq, const int32_t* ptr) {
+  using namespace ando::lbvh;
+  if (count <= 1) return;
+  hipStream_t s = c->stream;
+  ++c->stats.long_hit_sorts;
+  c->sv_keys.ensure(count);
+  c->sv_vals.ensure(count);
+  hipLaunchKernelGGL(k_hit_keys, dim3(nblk(count)), dim3(kBlock), 0, s, count, (const int32_t*)(c->sc_key.p + base),
+                     c->sv_keys.p, c->sv_vals.p);
+  radix_sort(c, count, c->sv_keys, c->sv_vals, c->sv_keys2, c->sv_vals2, c->sv_counts, c->sv_offsets, c->sv_sums,
+             c->sv_total);
+  hipLaunchKernelGGL(k_hit_queries, dim3(nblk(count)), dim3(kBlock), 0, s, count, nq, ptr,
+                     (const int32_t*)c->sv_vals.p, c->sv_keys.p);
+  radix_sort(c, count, c->sv_keys, c->sv_vals, c->sv_keys2, c->sv_vals2, c->sv_counts, c->sv_offsets, c->sv_sums,
+             c->sv_total);
+  c->sc_tmp.ensure(count);
+  c->sc_ktmp.ensure(count);
+  hipLaunchKernelGGL(k_gather_hits, dim3(nblk(count)), dim3(kBlock), 0, s, count, (const int32_t*)c->sv_vals.p,
+                     (const ando::SelfContact*)(c->sc.p + base), (const int32_t*)(c->sc_key.p + base), c->sc_tmp.p,
+                     c->sc_ktmp.p);
+  HIPCHK(hipMemcpyAsync(c->sc.p + base, c->sc_tmp.p, sizeof(ando::SelfContact) * count, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipMemcpyAsync(c->sc_key.p + base, c->sc_ktmp.p, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
+  HIPCHK(hipGetLastError());
+}
+
+// After a fill pass: bit 0 of the error flags is a traversal stack overflow,
+// bit 1 a query that left its hits unsorted (cleared here once sorted).
+void finish_pass(ando_ctx* c, int base, int count, int nq) {
+  uint32_t err = 0;
+  read_small(c, &err, (const uint32_t*)c->sc_err.p, 1);
+  if (err & 1u) throw Fail(ANDO_EDEVICE, "BVH traversal stack exceeded its depth");
+  if (err & 2u) {
+    sort_long_hits(c, base, count, nq, c->sc_ptr.p);
+    HIPCHK(hipMemsetAsync(c->sc_err.p, 0, sizeof(uint32_t), c->stream));
+  }
+}
+
 // LBVH over P primitives of `kind` vertices each (device index array prims):
 // boxes, Morton codes, stable radix sort, Karras hierarchy, node boxes.
 // vbox (optional): per-vertex boxes whose unions are the primitive boxes;
@@ -732,14 +798,17 @@ void detect_self(ando_ctx* c) {
   int npt = 0, nee = 0;
   if (m > 0) {
     pt(0, nullptr, nullptr);
-    npt = exclusive_scan(c, n, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
+    npt = candidate_scan(c, n, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
   }
   c->sc.ensure((size_t)npt + 1);
   c->sc_key.ensure((size_t)npt + 1);
-  if (npt) pt(1, c->sc.p, c->sc_key.p);
+  if (npt) {
+    pt(1, c->sc.p, c->sc_key.p);
+    finish_pass(c, 0, npt, n);
+  }
   if (E > 1) {
     ee(0, nullptr, nullptr);
-    nee = exclusive_scan(c, E, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
+    nee = candidate_scan(c, E, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
   }
   if (nee) {
     // Append the EE part, growing the list (keeping the PT part, with a
@@ -753,11 +822,10 @@ void detect_self(ando_ctx* c) {
     }
     c->sc_key.ensure((size_t)npt + nee);
     ee(1, c->sc.p + npt, c->sc_key.p + npt);
+    finish_pass(c, npt, nee, E);
   }
   HIPCHK(hipGetLastError());
-  uint32_t err = 0;
-  read_small(c, &err, (const uint32_t*)c->sc_err.p, 1);
-  if (err) throw Fail(ANDO_EDEVICE, "BVH traversal stack exceeded its depth");
+  if (!npt && !nee) finish_pass(c, 0, 0, 0);
   c->sc_pt = npt;
   c->sc_total = npt + nee;
 }
@@ -805,14 +873,17 @@ void detect_cp(ando_ctx* c, const ando::DetectBodies& D, float gm, const float* 
   int npt = 0, nee = 0;
   if (m > 0) {
     pt(0, nullptr, nullptr);
-    npt = exclusive_scan(c, n, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
+    npt = candidate_scan(c, n, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
   }
   c->sc.ensure((size_t)npt + 1);
   c->sc_key.ensure((size_t)npt + 1);
-  if (npt) pt(1, c->sc.p, c->sc_key.p);
+  if (npt) {
+    pt(1, c->sc.p, c->sc_key.p);
+    finish_pass(c, 0, npt, n);
+  }
   if (E > 1) {
     ee(0, nullptr, nullptr);
-    nee = exclusive_scan(c, E, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
+    nee = candidate_scan(c, E, c->sc_count.p, c->sc_ptr.p, c->sc_sums, c->sc_total_d);
   }
   if (nee) {
     if (c->sc.n < (size_t)npt + nee) {
@@ -824,11 +895,10 @@ void detect_cp(ando_ctx* c, const ando::DetectBodies& D, float gm, const float* 
     }
     c->sc_key.ensure((size_t)npt + nee);
     ee(1, c->sc.p + npt, c->sc_key.p + npt);
+    finish_pass(c, npt, nee, E);
   }
   HIPCHK(hipGetLastError());
-  uint32_t err = 0;
-  read_small(c, &err, (const uint32_t*)c->sc_err.p, 1);
-  if (err) throw Fail(ANDO_EDEVICE, "BVH traversal stack exceeded its depth");
+  if (!npt && !nee) finish_pass(c, 0, 0, 0);
   c->sc_pt = npt;
   c->sc_total = npt + nee;
   detect_cp_rigid(c, D, gm);
@@ -843,7 +913,7 @@ void detect_cp_rigid(ando_ctx* c, const ando::DetectBodies& D, float gm) {
     c->rc_hits.ensure((size_t)n * kHitCache);
     hipLaunchKernelGGL(ando::cp::k_rigid_cand, dim3(nblk(n)), dim3(kBlock), 0, s, n, 0, (const float*)c->vbox.p, gm, D,
                        (const int32_t*)c->rc_ptr.p, c->rc_count.p, c->rc.p, c->rc_hits.p);
-    const int32_t total = exclusive_scan(c, n, c->rc_count.p, c->rc_ptr.p, c->scan_sums, c->scan_total);
+    const int32_t total = candidate_scan(c, n, c->rc_count.p, c->rc_ptr.p, c->scan_sums, c->scan_total);
     c->rc_total = total;
     c->rc.ensure((size_t)total + 1);
     if (total)

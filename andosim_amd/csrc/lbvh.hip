@@ -402,7 +402,18 @@ __device__ inline bool traverse(int P, const Box& q, const int32_t* left, const 
 
 // Insertion sort of one query's hits out[base .. base+k) by key (the
 // traversal visits them in Morton order; the lists are short).
-__device__ inline void sort_hits(SelfContact* out, int32_t* key, int base, int k) {
+// A query sorts up to kSortLocal hits itself (insertion sort).  A longer list
+// (a swept box over a large part of the mesh can hit 10^5 primitives, and an
+// insertion sort of that many records by one thread runs for minutes) stays
+// in traversal order and sets bit 1 of err; the caller then sorts all the
+// hits of the pass by (query, key) with sort_long_hits.
+constexpr int kSortLocal = 64;
+
+__device__ inline void sort_hits(SelfContact* out, int32_t* key, int base, int k, uint32_t* err) {
+  if (k > kSortLocal) {
+    atomicOr(err, 2u);
+    return;
+  }
   for (int a = 1; a < k; ++a) {
     const SelfContact cur = out[base + a];
     const int ck = key[base + a];
@@ -475,7 +486,7 @@ __global__ void __launch_bounds__(64) k_query_pt(int pass, int n, const float* _
     count[v] = k;
     return;
   }
-  sort_hits(out, key, base, k);
+  sort_hits(out, key, base, k, err);
 }
 
 // EE query for edge e1 against edges e2 > e1, same two passes; hits sorted by e2.
@@ -528,7 +539,7 @@ __global__ void __launch_bounds__(64) k_query_ee(int pass, int E, const float* _
     count[e1] = k;
     return;
   }
-  sort_hits(out, key, base, k);
+  sort_hits(out, key, base, k, err);
 }
 
 // Corrected-physics candidates: box overlap alone (no distance filter; the
@@ -582,7 +593,7 @@ __global__ void __launch_bounds__(64) k_query_pt_box(int pass, int n, const floa
     count[v] = k;
     return;
   }
-  sort_hits(out, key, base, k);
+  sort_hits(out, key, base, k, err);
 }
 
 __global__ void __launch_bounds__(64) k_query_ee_box(int pass, int E, const float* __restrict__ eboxes, float gm,
@@ -629,7 +640,7 @@ __global__ void __launch_bounds__(64) k_query_ee_box(int pass, int E, const floa
     count[e1] = k;
     return;
   }
-  sort_hits(out, key, base, k);
+  sort_hits(out, key, base, k, err);
 }
 
 // ---------------------------------------------------------------------------
@@ -889,6 +900,52 @@ __global__ void __launch_bounds__(256) k_self_spmv(int n, const int32_t* __restr
 // then 0.9 d / l_p, never crosses.  Returns 1 when the whole motion keeps the
 // pair apart, otherwise the last time at which the distance was still at
 // least s times its initial value (a conservative time of impact).
+// Segmented sort of a pass's hits by (query, key): two stable radix sorts of
+// positions, by key and then by query.  k_hit_keys: key of position j;
+// k_hit_queries: the query of the position each sorted slot holds (the q with
+// ptr[q] <= pos < ptr[q + 1], by bisection); k_gather_hits: the records in
+// sorted order.
+__global__ void k_hit_keys(int count, const int32_t* __restrict__ key, uint32_t* __restrict__ keys,
+                           int32_t* __restrict__ pos) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  keys[j] = (uint32_t)key[j];
+  pos[j] = j;
+}
+__global__ void k_hit_queries(int count, int nq, const int32_t* __restrict__ ptr, const int32_t* __restrict__ pos,
+                              uint32_t* __restrict__ keys) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  const int e = pos[j];
+  int lo = 0, hi = nq - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ptr[mid] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  keys[j] = (uint32_t)lo;
+}
+__global__ void k_gather_hits(int count, const int32_t* __restrict__ pos, const SelfContact* __restrict__ in,
+                              const int32_t* __restrict__ kin, SelfContact* __restrict__ out,
+                              int32_t* __restrict__ kout) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count) return;
+  out[j] = in[pos[j]];
+  kout[j] = kin[pos[j]];
+}
+
+// Sum of n hit counts in 64 bits (the int32 scan of a runaway detection would
+// wrap); *out must start at zero.
+__global__ void __launch_bounds__(256) k_sum_counts64(int n, const int32_t* __restrict__ count,
+                                                      unsigned long long* __restrict__ out) {
+  unsigned long long v = 0ull;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    v += (unsigned long long)(uint32_t)count[i];
+  __shared__ unsigned long long lds[4];
+  const unsigned long long r = block_reduce(v, lds, [](unsigned long long a, unsigned long long b) { return a + b; });
+  if (threadIdx.x == 0 && r) atomicAdd(out, r);
+}
+
 constexpr float kAccdS = 0.1f;
 constexpr int kAccdMaxIter = 256;
 

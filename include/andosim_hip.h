@@ -159,6 +159,9 @@ typedef struct {
   int32_t pcg_vec_block;            /* threads per workgroup of the PCG vector kernels */
   int32_t pcg_autotuned;            /* 1: the launch shape above was picked by the autotuner
                                        (environment ANDO_PCG_AUTOTUNE=1, read by ando_create) */
+  int32_t long_hit_sorts;           /* detection passes in which a query had more than 64 hits,
+                                       sorted by a segmented radix sort instead of by the query */
+  int32_t max_candidates;           /* largest candidate list of one detection pass (PT, EE or rigid) */
 } ando_stats;
 
 int32_t ando_abi_version(void);

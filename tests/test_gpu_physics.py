@@ -88,6 +88,33 @@ def test_physics_linearize_bit_exact(name):
     assert not np.allclose(gf, go)
 
 
+def dense_stack(s=21, sheets=10, spacing=2e-3):
+    """Ten s x s sheets 0.08 mm apart with 2 mm spacing: a vertex's box grown
+    by gap_max overlaps up to 74 triangles (more than a query sorts itself)."""
+    sc = scenes.stack_resting(s=s, sheets=sheets, gap=8e-5)
+    c = sc.verts[:, :2].mean(0)
+    sc.verts[:, :2] = ((sc.verts[:, :2] - c) * np.float32(spacing * (s - 1))).astype(np.float32)
+    return sc
+
+
+def test_physics_linearize_long_hit_lists():
+    """Candidate lists whose queries have more hits than they sort themselves
+    go through the segmented radix sort: the per-vertex contact order, and so
+    the gradient and matrix, stay bit-identical to the oracle's."""
+    sc = dense_stack()
+    o = oracle_sim(sc)
+    x = o.get("x").astype(np.float32)
+    v = o.get("v").astype(np.float32)
+    g = gpu_sim(sc, fp64_near=False)
+    g.set_state(x, v)
+    xhat = (x + np.float32(sc.params["dt"]) * v).astype(np.float32)
+    go, Ho = o.linearize(xhat)
+    gg, Hg = g.linearize(xhat)
+    assert g.stats()["long_hit_sorts"] > 0
+    assert np.array_equal(gg.reshape(-1), go.astype(np.float32).reshape(-1)), rel_err(gg, go.reshape(-1))
+    assert np.array_equal(Hg, Ho.astype(np.float32)), rel_err(Hg, Ho)
+
+
 def fp32_envelope(sc, x0, v0, x64, v64, tries=3):
     """How far FP32 substeps land from the FP64 oracle's: the largest distance
     (positions, velocities) of the FP32 oracle's substep from x0 and from x0
