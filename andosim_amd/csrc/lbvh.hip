@@ -888,18 +888,6 @@ __global__ void __launch_bounds__(256) k_self_spmv(int n, const int32_t* __restr
   if (threadIdx.x == 0) part[blockIdx.x] = sblk;
 }
 
-// Line-search feasibility of the self contacts (LineSearch::is_feasible,
-// line_search.cpp:70-132, with ccd_point_triangle / ccd_edge_edge :163-247):
-// for trial t (alpha = 2^-t) the contact fails if its 10-sample CCD finds a
-// distance below 1e-6 at a sample time t' < 1.  Bit t of the AND-reduced mask
-// is set when every contact passes trial t.
-// Additive CCD (Li, Kaufman, Jiang et al., "Codimensional Incremental
-// Potential Contact", 2021, Algorithm 1), thickness 0, t_c = 1: the pair moves
-// by p (per vertex) over t in [0, 1]; with the mean displacement removed, l_p
-// bounds how fast the distance can shrink, so advancing by (1 - s) d / l_p,
-// then 0.9 d / l_p, never crosses.  Returns 1 when the whole motion keeps the
-// pair apart, otherwise the last time at which the distance was still at
-// least s times its initial value (a conservative time of impact).
 // Segmented sort of a pass's hits by (query, key): two stable radix sorts of
 // positions, by key and then by query.  k_hit_keys: key of position j;
 // k_hit_queries: the query of the position each sorted slot holds (the q with
@@ -946,6 +934,18 @@ __global__ void __launch_bounds__(256) k_sum_counts64(int n, const int32_t* __re
   if (threadIdx.x == 0 && r) atomicAdd(out, r);
 }
 
+// Line-search feasibility of the self contacts (LineSearch::is_feasible,
+// line_search.cpp:70-132, with ccd_point_triangle / ccd_edge_edge :163-247):
+// for trial t (alpha = 2^-t) the contact fails if its 10-sample CCD finds a
+// distance below 1e-6 at a sample time t' < 1.  Bit t of the AND-reduced mask
+// is set when every contact passes trial t.
+// Additive CCD (Li, Kaufman, Jiang et al., "Codimensional Incremental
+// Potential Contact", 2021, Algorithm 1), thickness 0, t_c = 1: the pair moves
+// by p (per vertex) over t in [0, 1]; with the mean displacement removed, l_p
+// bounds how fast the distance can shrink, so advancing by (1 - s) d / l_p,
+// then 0.9 d / l_p, never crosses.  Returns 1 when the whole motion keeps the
+// pair apart, otherwise the last time at which the distance was still at
+// least s times its initial value (a conservative time of impact).
 constexpr float kAccdS = 0.1f;
 constexpr int kAccdMaxIter = 256;
 

@@ -12,7 +12,8 @@
 //     gap_max on the query side; their gaps, normals and closest-point weights
 //     are refreshed at every Newton iteration;
 //   * line search: every candidate pair's additive-CCD time of impact over the
-//     trial motion bounds alpha, and the first wall may not get more violated.
+//     trial motion (FP64, keeping a separation of 32 FP32 ulps) bounds alpha,
+//     and the first wall may not get more violated.
 // Layout: per face 6 floats for 2k B B^T (entries 00 01 02 11 12 22, `gk`)
 // and for I - n n^T (`fpi`); per block 3 floats of isotropic elastic
 // stiffness (`Hiso3`, the same sum on the three diagonal entries) and a 9-float
@@ -239,15 +240,133 @@ __global__ void __launch_bounds__(kBlock) k_rigid_cand(int n, int pass, const fl
   if (!pass && live) count[i] = c;
 }
 
+// ---------------------------------------------------------------------------
+// Line-search time of impact in FP64 with a minimum separation.  Positions
+// are FP32: an additive CCD with thickness 0 lets a pair pushed together over
+// several Newton iterations close geometrically (s = 0.1 of the gap each
+// time) until the gap is a few ulps of the coordinates, where rounding of the
+// positions decides the side.  Here the pair keeps xi = 32 FP32 ulps of its
+// largest coordinate (an upper bound 2^-23 m on ulp(m)): the accepted motion
+// never brings it within xi, and the update x + alpha ed, rounded to FP32
+// (alpha a power of two, so alpha ed is exact), moves each vertex by at most
+// half an ulp, far less than xi.  A pair already within xi keeps half its gap
+// as the margin.
+// ---------------------------------------------------------------------------
+struct d3 {
+  double x, y, z;
+};
+__device__ inline d3 dmk(double a, double b, double c) { return d3{a, b, c}; }
+__device__ inline d3 dsub(d3 a, d3 b) { return dmk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline d3 dadd(d3 a, d3 b) { return dmk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline d3 dscale(double s, d3 a) { return dmk(s * a.x, s * a.y, s * a.z); }
+__device__ inline double ddot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ inline double dnorm(d3 a) { return sqrt(ddot(a, a)); }
+__device__ inline d3 dld(const float* v, int i) { return dmk(v[3 * i], v[3 * i + 1], v[3 * i + 2]); }
+
+// Closest points of segments p1 q1 and p2 q2 (Ericson, Real-Time Collision
+// Detection, 5.1.9); the distance between them.
+__device__ inline double seg_seg_dist(d3 p1, d3 q1, d3 p2, d3 q2) {
+  const d3 d1 = dsub(q1, p1), d2 = dsub(q2, p2), r = dsub(p1, p2);
+  const double a = ddot(d1, d1), e = ddot(d2, d2), f = ddot(d2, r);
+  double s = 0.0, t = 0.0;
+  if (a <= 0.0 && e <= 0.0) {
+  } else if (a <= 0.0) {
+    t = fmin(fmax(f / e, 0.0), 1.0);
+  } else {
+    const double c = ddot(d1, r);
+    if (e <= 0.0) {
+      s = fmin(fmax(-c / a, 0.0), 1.0);
+    } else {
+      const double b = ddot(d1, d2), den = a * e - b * b;
+      s = den != 0.0 ? fmin(fmax((b * f - c * e) / den, 0.0), 1.0) : 0.0;
+      t = (b * s + f) / e;
+      if (t < 0.0) {
+        t = 0.0;
+        s = fmin(fmax(-c / a, 0.0), 1.0);
+      } else if (t > 1.0) {
+        t = 1.0;
+        s = fmin(fmax((b - c) / a, 0.0), 1.0);
+      }
+    }
+  }
+  return dnorm(dsub(dadd(p1, dscale(s, d1)), dadd(p2, dscale(t, d2))));
+}
+
+// Distance from p to triangle a b c (Ericson 5.1.5, Voronoi regions); a
+// degenerate triangle falls back to its edges.
+__device__ inline double pt_tri_dist(d3 p, d3 a, d3 b, d3 c) {
+  const d3 ab = dsub(b, a), ac = dsub(c, a), ap = dsub(p, a);
+  const double d1 = ddot(ab, ap), d2 = ddot(ac, ap);
+  if (d1 <= 0.0 && d2 <= 0.0) return dnorm(ap);
+  const d3 bp = dsub(p, b);
+  const double d3_ = ddot(ab, bp), d4 = ddot(ac, bp);
+  if (d3_ >= 0.0 && d4 <= d3_) return dnorm(bp);
+  const double vc = d1 * d4 - d3_ * d2;
+  if (vc <= 0.0 && d1 >= 0.0 && d3_ <= 0.0) return dnorm(dsub(p, dadd(a, dscale(d1 / (d1 - d3_), ab))));
+  const d3 cp = dsub(p, c);
+  const double d5 = ddot(ab, cp), d6 = ddot(ac, cp);
+  if (d6 >= 0.0 && d5 <= d6) return dnorm(cp);
+  const double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) return dnorm(dsub(p, dadd(a, dscale(d2 / (d2 - d6), ac))));
+  const double va = d3_ * d6 - d5 * d4;
+  if (va <= 0.0 && (d4 - d3_) >= 0.0 && (d5 - d6) >= 0.0)
+    return dnorm(dsub(p, dadd(b, dscale((d4 - d3_) / ((d4 - d3_) + (d5 - d6)), dsub(c, b)))));
+  const double sum = va + vb + vc;
+  if (!(sum > 0.0))
+    return fmin(fmin(seg_seg_dist(p, p, a, b), seg_seg_dist(p, p, b, c)), seg_seg_dist(p, p, c, a));
+  const double v = vb / sum, w = vc / sum;
+  return dnorm(dsub(p, dadd(a, dadd(dscale(v, ab), dscale(w, ac)))));
+}
+
+__device__ inline double pair_dist_d(int type, const d3* q) {
+  return type == 0 ? pt_tri_dist(q[0], q[1], q[2], q[3]) : seg_seg_dist(q[0], q[1], q[2], q[3]);
+}
+
+constexpr double kSepUlps = 32.0 * 1.1920928955078125e-07;  // 32 ulps per unit of coordinate magnitude
+
+// Additive CCD (Li, Kaufman, Jiang et al., "Codimensional Incremental
+// Potential Contact", 2021, Algorithm 1) in FP64 with thickness xi, s = 0.1,
+// t_c = 1, at most 256 advances: 1 when the whole motion x0 -> x0 + p keeps
+// the pair farther than xi, else a time up to which it provably does.
+__device__ inline double accd_sep(int type, const d3* x0, const d3* p) {
+  const d3 pbar = dscale(0.25, dadd(dadd(p[0], p[1]), dadd(p[2], p[3])));
+  d3 q[4], x[4];
+  double m = 0.0;
+  for (int k = 0; k < 4; ++k) {
+    q[k] = dsub(p[k], pbar);
+    x[k] = x0[k];
+    const d3 e = dadd(x0[k], p[k]);
+    m = fmax(m, fmax(fmax(fabs(x0[k].x), fabs(x0[k].y)), fmax(fabs(x0[k].z), fmax(fabs(e.x), fmax(fabs(e.y), fabs(e.z))))));
+  }
+  const double lp = type == 0 ? dnorm(q[0]) + fmax(dnorm(q[1]), fmax(dnorm(q[2]), dnorm(q[3])))
+                              : fmax(dnorm(q[0]), dnorm(q[1])) + fmax(dnorm(q[2]), dnorm(q[3]));
+  if (lp == 0.0) return 1.0;
+  double d = pair_dist_d(type, x);
+  if (!(d > 0.0)) return 0.0;
+  double xi = kSepUlps * m;
+  if (d <= xi) xi = 0.5 * d;
+  const double g = 0.1 * (d - xi);
+  double t = 0.0, tl = 0.9 * (d - xi) / lp;
+  for (int it = 0; it < 256; ++it) {
+    for (int k = 0; k < 4; ++k) x[k] = dadd(x[k], dscale(tl, q[k]));
+    d = pair_dist_d(type, x);
+    if (t > 0.0 && d - xi < g) return t;
+    t = t + tl;
+    if (t > 1.0) return 1.0;
+    tl = 0.9 * (d - xi) / lp;
+  }
+  return t;
+}
+
 // Line search: bit t of a contact's mask is set when alpha_t = 2^-t is at most
-// its additive-CCD time of impact over the whole trial motion (x -> x + ed;
+// its time of impact (accd_sep) over the whole trial motion (x -> x + ed;
 // rigid triangles stay put); masks are AND-reduced into *mask.
-__device__ inline uint32_t toi_bits(float toi) {
+__device__ inline uint32_t toi_bits(double toi) {
   uint32_t m = 0u;
-  float alpha = 1.0f;
+  double alpha = 1.0;
   for (int t = 0; t < 20; ++t) {
     if (alpha <= toi) m |= 1u << t;
-    alpha *= 0.5f;
+    alpha *= 0.5;
   }
   return m;
 }
@@ -258,12 +377,12 @@ __global__ void k_ls_toi_self(int nc, const SelfContact* __restrict__ sc, const 
   uint32_t m = 0xFFFFFu;
   if (c < nc) {
     const SelfContact& C = sc[c];
-    f3 x0[4], p[4];
+    d3 x0[4], p[4];
     for (int k = 0; k < 4; ++k) {
-      x0[k] = ld3(x, C.idx[k]);
-      p[k] = ld3(ed, C.idx[k]);
+      x0[k] = dld(x, C.idx[k]);
+      p[k] = dld(ed, C.idx[k]);
     }
-    m = toi_bits(lbvh::additive_ccd(C.type, x0, p));
+    m = toi_bits(accd_sep(C.type, x0, p));
   }
   __shared__ uint32_t lds[4];
   const uint32_t r = block_reduce(m, lds, [](uint32_t a, uint32_t b) { return a & b; });
@@ -277,14 +396,14 @@ __global__ void k_ls_toi_rigid(int total, const RigidContact* __restrict__ rc, D
   if (e < total) {
     const RigidContact& r = rc[e];
     const DetectBody& B = D.b[r.body];
-    f3 x0[4], p[4];
-    x0[0] = ld3(x, r.vertex);
-    p[0] = ld3(ed, r.vertex);
+    d3 x0[4], p[4];
+    x0[0] = dld(x, r.vertex);
+    p[0] = dld(ed, r.vertex);
     for (int k = 1; k < 4; ++k) {
-      x0[k] = ld3(B.world, B.tris[3 * r.tri + k - 1]);
-      p[k] = mk(0.0f, 0.0f, 0.0f);
+      x0[k] = dld(B.world, B.tris[3 * r.tri + k - 1]);
+      p[k] = dmk(0.0, 0.0, 0.0);
     }
-    m = toi_bits(lbvh::additive_ccd(0, x0, p));
+    m = toi_bits(accd_sep(0, x0, p));
   }
   __shared__ uint32_t lds[4];
   const uint32_t rr = block_reduce(m, lds, [](uint32_t a, uint32_t b) { return a & b; });

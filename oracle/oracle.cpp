@@ -235,6 +235,118 @@ template <class R> R barrier_hess(R g, R gmax, R k) {
 template <class R> bool in_domain(R g, R gmax) { return gmax > R(0) && g <= gmax; }
 
 // ---------------------------------------------------------------------------
+// Corrected physics line search: time of impact in FP64 with a minimum
+// separation of 32 FP32 ulps of the pair's largest coordinate (new; the
+// reference samples 10 distances, line_search.cpp:163-247).  With thickness 0
+// a pair pushed together over several Newton iterations closes geometrically
+// until FP32 rounding of the positions decides its side.  Same operations in
+// the same order as the device code, so both pick the same trial.
+// ---------------------------------------------------------------------------
+struct D3 {
+  double x, y, z;
+};
+static inline D3 dsub(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline D3 dadd(D3 a, D3 b) { return D3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+static inline D3 dscale(double s, D3 a) { return D3{s * a.x, s * a.y, s * a.z}; }
+static inline double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline double dnorm(D3 a) { return std::sqrt(ddot(a, a)); }
+
+// Segment-segment distance (Ericson, Real-Time Collision Detection, 5.1.9).
+static double seg_seg_dist(D3 p1, D3 q1, D3 p2, D3 q2) {
+  const D3 d1 = dsub(q1, p1), d2 = dsub(q2, p2), r = dsub(p1, p2);
+  const double a = ddot(d1, d1), e = ddot(d2, d2), f = ddot(d2, r);
+  double s = 0.0, t = 0.0;
+  if (a <= 0.0 && e <= 0.0) {
+  } else if (a <= 0.0) {
+    t = std::fmin(std::fmax(f / e, 0.0), 1.0);
+  } else {
+    const double c = ddot(d1, r);
+    if (e <= 0.0) {
+      s = std::fmin(std::fmax(-c / a, 0.0), 1.0);
+    } else {
+      const double b = ddot(d1, d2), den = a * e - b * b;
+      s = den != 0.0 ? std::fmin(std::fmax((b * f - c * e) / den, 0.0), 1.0) : 0.0;
+      t = (b * s + f) / e;
+      if (t < 0.0) {
+        t = 0.0;
+        s = std::fmin(std::fmax(-c / a, 0.0), 1.0);
+      } else if (t > 1.0) {
+        t = 1.0;
+        s = std::fmin(std::fmax((b - c) / a, 0.0), 1.0);
+      }
+    }
+  }
+  return dnorm(dsub(dadd(p1, dscale(s, d1)), dadd(p2, dscale(t, d2))));
+}
+
+// Point-triangle distance (Ericson 5.1.5); a degenerate triangle falls back
+// to its edges.
+static double pt_tri_dist(D3 p, D3 a, D3 b, D3 c) {
+  const D3 ab = dsub(b, a), ac = dsub(c, a), ap = dsub(p, a);
+  const double d1 = ddot(ab, ap), d2 = ddot(ac, ap);
+  if (d1 <= 0.0 && d2 <= 0.0) return dnorm(ap);
+  const D3 bp = dsub(p, b);
+  const double d3 = ddot(ab, bp), d4 = ddot(ac, bp);
+  if (d3 >= 0.0 && d4 <= d3) return dnorm(bp);
+  const double vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) return dnorm(dsub(p, dadd(a, dscale(d1 / (d1 - d3), ab))));
+  const D3 cp = dsub(p, c);
+  const double d5 = ddot(ab, cp), d6 = ddot(ac, cp);
+  if (d6 >= 0.0 && d5 <= d6) return dnorm(cp);
+  const double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) return dnorm(dsub(p, dadd(a, dscale(d2 / (d2 - d6), ac))));
+  const double va = d3 * d6 - d5 * d4;
+  if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0)
+    return dnorm(dsub(p, dadd(b, dscale((d4 - d3) / ((d4 - d3) + (d5 - d6)), dsub(c, b)))));
+  const double sum = va + vb + vc;
+  if (!(sum > 0.0))
+    return std::fmin(std::fmin(seg_seg_dist(p, p, a, b), seg_seg_dist(p, p, b, c)), seg_seg_dist(p, p, c, a));
+  const double v = vb / sum, w = vc / sum;
+  return dnorm(dsub(p, dadd(a, dadd(dscale(v, ab), dscale(w, ac)))));
+}
+
+static double pair_dist_d(int type, const D3* q) {
+  return type == 0 ? pt_tri_dist(q[0], q[1], q[2], q[3]) : seg_seg_dist(q[0], q[1], q[2], q[3]);
+}
+
+// Additive CCD (Li, Kaufman, Jiang et al. 2021, Algorithm 1) in FP64 with
+// thickness xi = 32 * 2^-23 * (largest coordinate of the pair over the motion),
+// half the initial gap when that is smaller; s = 0.1, t_c = 1, at most 256
+// advances.  1 when the whole motion x0 -> x0 + p keeps the pair farther than
+// xi, else a time up to which it provably does.
+static double accd_sep(int type, const D3* x0, const D3* p) {
+  const D3 pbar = dscale(0.25, dadd(dadd(p[0], p[1]), dadd(p[2], p[3])));
+  D3 q[4], x[4];
+  double m = 0.0;
+  for (int k = 0; k < 4; ++k) {
+    q[k] = dsub(p[k], pbar);
+    x[k] = x0[k];
+    const D3 e = dadd(x0[k], p[k]);
+    m = std::fmax(m, std::fmax(std::fmax(std::fabs(x0[k].x), std::fabs(x0[k].y)),
+                               std::fmax(std::fabs(x0[k].z),
+                                         std::fmax(std::fabs(e.x), std::fmax(std::fabs(e.y), std::fabs(e.z))))));
+  }
+  const double lp = type == 0 ? dnorm(q[0]) + std::fmax(dnorm(q[1]), std::fmax(dnorm(q[2]), dnorm(q[3])))
+                              : std::fmax(dnorm(q[0]), dnorm(q[1])) + std::fmax(dnorm(q[2]), dnorm(q[3]));
+  if (lp == 0.0) return 1.0;
+  double d = pair_dist_d(type, x);
+  if (!(d > 0.0)) return 0.0;
+  double xi = 32.0 * 1.1920928955078125e-07 * m;
+  if (d <= xi) xi = 0.5 * d;
+  const double g = 0.1 * (d - xi);
+  double t = 0.0, tl = 0.9 * (d - xi) / lp;
+  for (int it = 0; it < 256; ++it) {
+    for (int k = 0; k < 4; ++k) x[k] = dadd(x[k], dscale(tl, q[k]));
+    d = pair_dist_d(type, x);
+    if (t > 0.0 && d - xi < g) return t;
+    t = t + tl;
+    if (t > 1.0) return 1.0;
+    tl = 0.9 * (d - xi) / lp;
+  }
+  return t;
+}
+
+// ---------------------------------------------------------------------------
 // Point-triangle narrow phase (collision.cpp:279-387, Ericson regions).
 // ---------------------------------------------------------------------------
 template <class R>
@@ -1874,32 +1986,33 @@ template <class R> struct Sim {
     std::vector<R> ed(3 * n);
     for (int i = 0; i < 3 * n; ++i) ed[i] = R(1.25) * d[i];
     detect_cp(&ed, contacts);
-    R toi = R(1);
+    double toi = 1.0;
+    auto d3of = [](const V3<R>& v) { return D3{(double)v[0], (double)v[1], (double)v[2]}; };
 #pragma omp parallel for schedule(dynamic, 256) reduction(min : toi)
     for (size_t k = 0; k < contacts.size(); ++k) {
       const Contact<R>& c = contacts[k];
-      V3<R> x0[4], p[4];
+      D3 x0[4], p[4];
       if (c.type == RIGID_PT) {
         const auto& W = rworld[c.rigid];
-        x0[0] = X(c.idx[0]);
-        p[0] = Xof(ed, c.idx[0]);
+        x0[0] = d3of(X(c.idx[0]));
+        p[0] = d3of(Xof(ed, c.idx[0]));
         for (int j = 1; j < 4; ++j) {
-          x0[j] = W[c.idx[j]];
-          p[j] = V3<R>(0, 0, 0);
+          x0[j] = d3of(W[c.idx[j]]);
+          p[j] = D3{0.0, 0.0, 0.0};
         }
       } else {
         for (int j = 0; j < 4; ++j) {
-          x0[j] = X(c.idx[j]);
-          p[j] = Xof(ed, c.idx[j]);
+          x0[j] = d3of(X(c.idx[j]));
+          p[j] = d3of(Xof(ed, c.idx[j]));
         }
       }
-      const R t = additive_ccd(c.type == EE ? 1 : 0, x0, p);
+      const double t = accd_sep(c.type == EE ? 1 : 0, x0, p);
       toi = std::min(toi, t);
     }
     const bool wall = (double)sqn3(wn) > 0.5;
     R alpha = R(1);
     for (int it = 0; it < 20; ++it) {
-      bool ok = alpha <= toi;
+      bool ok = (double)alpha <= toi;
       if (ok && wall) {
 #pragma omp parallel for schedule(static) reduction(&& : ok)
         for (int i = 0; i < n; ++i) {
@@ -2379,6 +2492,16 @@ int orc_set_ccd_mode(void* h, int mode) {
 }
 
 // Additive CCD on one pair: type 0 PT / 1 EE, x0 and p 12 doubles each.
+// The corrected physics line search's FP64 time of impact with its minimum
+// separation (accd_sep).
+double orc_accd_sep(int type, const double* x0, const double* p) {
+  D3 xs[4], ps[4];
+  for (int k = 0; k < 4; ++k) {
+    xs[k] = D3{x0[3 * k], x0[3 * k + 1], x0[3 * k + 2]};
+    ps[k] = D3{p[3 * k], p[3 * k + 1], p[3 * k + 2]};
+  }
+  return accd_sep(type, xs, ps);
+}
 double orc_additive_ccd(int prec, int type, const double* x0, const double* p) {
   double r = 0.0;
   auto run = [&](auto tag) {

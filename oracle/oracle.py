@@ -87,6 +87,8 @@ def lib():
         L.orc_get_capture.argtypes = [C.c_void_p, _dp, _dp]
         L.orc_additive_ccd.restype = C.c_double
         L.orc_additive_ccd.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.orc_accd_sep.restype = C.c_double
+        L.orc_accd_sep.argtypes = [C.c_int, _dp, _dp]
         L.orc_contact_metrics.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, _dp, _dp, C.c_double, C.c_int, _dp]
         _lib = L
     return _lib
@@ -129,6 +131,13 @@ def additive_ccd(kind: str, x0, p, precision=32) -> float:
     time of impact."""
     return float(lib().orc_additive_ccd(int(precision), 0 if kind == "pt" else 1, _d(x0).reshape(-1),
                                         _d(p).reshape(-1)))
+
+
+def accd_sep(kind: str, x0, p) -> float:
+    """The corrected physics line search's time of impact (FP64, keeping 32
+    FP32 ulps of the pair's largest coordinate, or half the initial gap when
+    that is smaller) of one PT ("pt") or EE ("ee") pair moving by p."""
+    return float(lib().orc_accd_sep(0 if kind == "pt" else 1, _d(x0).reshape(-1), _d(p).reshape(-1)))
 
 
 def contact_metrics(types, idx, gaps, normals, velocities, gap_max, ccd, precision=32):

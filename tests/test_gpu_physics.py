@@ -212,21 +212,23 @@ def test_physics_flag_switches_back_to_faithful():
     assert np.array_equal(a.get("x"), b.get("x"))
 
 
-@pytest.mark.parametrize("s,steps", [(65, 100)])
-def test_physics_cylinder_drape_stays_intersection_free(s, steps):
-    """The two-cylinder drape at s x s: every substep finite, no PCG solve
-    lost, and no intersections every 20 substeps."""
-    print(f"physics drape s={s}", flush=True)
-    sc = scenes.cyl257(s=s)
+@pytest.mark.parametrize("name,make,steps", [
+    ("cyl65", lambda: scenes.cyl257(s=65), 100),
+    # A vertex of this drape was pushed through a triangle over eight Newton
+    # iterations at substep 78 by a thickness-0 FP32 line-search CCD.
+    ("sphere129", scenes.sphere129, 100),
+])
+def test_physics_drape_stays_intersection_free(name, make, steps):
+    """Drapes: every substep finite, no edge through a triangle and no vertex
+    inside a body, checked every 10 substeps."""
+    print(f"physics drape {name}", flush=True)
+    sc = make()
     g = gpu_sim(sc)
     fails = 0
     for k in range(steps):
         g.substep()
-        st = g.stats()
-        fails += st["pcg_failures"]
-        if k % 20 == 19 or k == steps - 1:
-            x = g.get("x")
-            assert np.all(np.isfinite(x)), k
-            assert g.count_intersections() == (0, 0), k
-    assert float(g.get("x")[:, 2].min()) < 0.2   # it draped over the cylinders
-    print(f"pcg failures {fails}", flush=True)
+        fails += g.stats()["pcg_failures"]
+        if k % 10 == 9:
+            assert np.all(np.isfinite(g.get("x"))), k
+            assert g.count_intersections() == (0, 0), (name, k)
+    print(f"{name}: pcg failures {fails}", flush=True)

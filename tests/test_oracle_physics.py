@@ -22,7 +22,7 @@ import numpy as np
 import pytest
 
 from andosim_amd import scenes
-from oracle.oracle import OracleSim, barrier
+from oracle.oracle import OracleSim, accd_sep, barrier
 
 MU = scenes.BASE_MATERIAL["youngs_modulus"] / (2.0 * (1.0 + scenes.BASE_MATERIAL["poisson_ratio"]))
 THICK = scenes.BASE_MATERIAL["thickness"]
@@ -239,3 +239,62 @@ def test_light_plate_is_pushed_away_from_the_cloth():
         o.step()
         vz.append(o.rigid_state(0)["linear_velocity"][2])
     assert vz[0] < 0.0 < vz[1]
+
+
+TRI = np.array([[-0.1, -0.1, 0.0], [0.3, -0.1, 0.0], [-0.1, 0.3, 0.0]])
+
+
+@pytest.mark.parametrize("h,D", [(1e-3, 4e-3), (1e-4, 1.0), (2e-5, 3e-5)])
+def test_separation_ccd_head_on(h, D):
+    """A point h above a static triangle moving D straight down: the time of
+    impact leaves at least the separation xi = 32 * 2^-23 * max|coordinate|
+    (plus a tenth of the remaining gap), and never passes the plane."""
+    x0 = np.vstack([[0.0, 0.0, h], TRI])
+    p = np.zeros((4, 3))
+    p[0, 2] = -D
+    t = accd_sep("pt", x0, p)
+    xi = 32 * 2.0 ** -23 * 0.3
+    if D < h:
+        assert t == 1.0
+        return
+    end = h - t * D
+    assert end >= xi + 0.1 * (h - xi) - 1e-15
+    assert end <= h and t > 0.0
+    # Conservative, not timid: it stops within the last 10% of the free gap.
+    assert end <= xi + 0.1 * (h - xi) + 0.9 * 0.1 * (h - xi) * 1.5
+
+
+def test_separation_ccd_resolves_fp32_coordinates():
+    """A point 5.6 FP32 ulps above a triangle at z = 0.2 moving 4 mm through
+    it: the accepted motion, rounded to FP32 as the step rounds it, keeps the
+    point strictly above the plane (a thickness-0 FP32 CCD let exactly this
+    pair through on the 129 x 129 drape)."""
+    z = np.float32(0.2)
+    pz = np.float32(z + np.float32(5.6 * 1.4901161e-08))
+    tri = np.array([[0.1, 0.1, z], [0.3, 0.1, z], [0.1, 0.3, z]], np.float32)
+    x0 = np.vstack([[0.15, 0.15, pz], tri]).astype(np.float64)
+    ed = np.zeros((4, 3), np.float32)
+    ed[0, 2] = np.float32(-4e-3)
+    t = accd_sep("pt", x0, ed.astype(np.float64))
+    assert 0.0 < t < 1.0
+    alpha = 1.0
+    while alpha > t:
+        alpha *= 0.5
+    zn = np.float32(pz + np.float32(alpha) * ed[0, 2])
+    assert zn > z
+    # The same pair far from contact moves freely.
+    x1 = x0.copy()
+    x1[0, 2] = 0.2 + 0.01
+    assert accd_sep("pt", x1, ed.astype(np.float64)) == 1.0
+
+
+def test_separation_ccd_edge_edge_crossing():
+    """Two crossing edges 1e-4 apart closing at 1 mm: the time of impact keeps
+    their distance above the separation; separating motion is free."""
+    x0 = np.array([[-1.0, 0.0, 1e-4], [1.0, 0.0, 1e-4], [0.0, -1.0, 0.0], [0.0, 1.0, 0.0]])
+    p = np.zeros((4, 3))
+    p[0:2, 2] = -1e-3
+    t = accd_sep("ee", x0, p)
+    xi = 32 * 2.0 ** -23 * 1.0
+    assert 1e-4 - t * 1e-3 >= xi + 0.1 * (1e-4 - xi) - 1e-15
+    assert accd_sep("ee", x0, -p) == 1.0
