@@ -290,6 +290,8 @@ struct ando_ctx {
   DBuf<float> vbox, ed;    // per vertex: candidate boxes (6), trial motion 1.25 d (3)
   DBuf<float> cfe, cegrad; // membrane gradient at x: per face (9), per vertex (3)
   ando::DetectBodies cpD{};  // the step's rigid bodies (they do not move during a step)
+  double mean_edge = 0.0;    // mean rest edge length (FP64, edge order)
+  DBuf<unsigned long long> motion2;  // line search: longest |d_i|^2 and |x_hat_i - x_i|^2 (double bits)
   float pt_box = 1e-4f, keep_gap = 0.01f;
   std::vector<int32_t> edges_h;
   DBuf<int32_t> d_edges;
@@ -1599,15 +1601,38 @@ float line_search(ando_ctx* c) {
   return 0.0f;
 }
 
-// Corrected physics line search: the candidates of the trial motion
-// x -> x + 1.25 d replace the step's candidates; alpha_t = 2^-t is feasible
-// when it is at most every candidate's additive-CCD time of impact and no
-// vertex ends up more violated at the first wall than it is (g1 >= min(0, g0)).
+// Corrected physics: the first line-search trial, the smallest t0 (at most 19)
+// with 1.25 * 2^-t0 * max |d_i| <= L = max(4 * mean rest edge, 2 * max
+// |x_hat_i - x_i|); arguments are the squared maxima (FP64).
+int cp_first_trial(double mean_edge, double md2, double mh2) {
+  const double L = std::max(4.0 * mean_edge, 2.0 * std::sqrt(mh2));
+  const double len = 1.25 * std::sqrt(md2);
+  int t0 = 0;
+  while (t0 < 19 && std::ldexp(len, -t0) > L) ++t0;
+  return t0;
+}
+
+// Corrected physics line search: trials start at t0 (cp_first_trial); the
+// candidates of the trial motion x -> x + ed, ed = 1.25 * 2^-t0 d, replace
+// the step's candidates; alpha_t = 2^-t (t >= t0) is feasible when 2^-(t - t0)
+// is at most every candidate's time of impact over ed and no vertex ends up
+// more violated at the first wall than it is (g1 >= min(0, g0)).
 float line_search_cp(ando_ctx* c, const ando_params& p) {
   hipStream_t s = c->stream;
   const int n3 = 3 * c->n;
   c->ed.ensure((size_t)n3);
-  hipLaunchKernelGGL(ando::cp::k_scale, dim3(nblk(n3)), dim3(kBlock), 0, s, n3, 1.25f, (const float*)c->d.p, c->ed.p);
+  c->motion2.ensure(2);
+  HIPCHK(hipMemsetAsync(c->motion2.p, 0, 2 * sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(ando::cp::k_motion_max, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, (const float*)c->d.p,
+                     (const float*)c->x.p, (const float*)c->xhat.p, c->motion2.p);
+  unsigned long long mb[2] = {0, 0};
+  read_small(c, mb, (const unsigned long long*)c->motion2.p, 2);
+  double md2, mh2;
+  std::memcpy(&md2, &mb[0], sizeof(double));
+  std::memcpy(&mh2, &mb[1], sizeof(double));
+  const int t0 = cp_first_trial(c->mean_edge, md2, mh2);
+  hipLaunchKernelGGL(ando::cp::k_scale, dim3(nblk(n3)), dim3(kBlock), 0, s, n3, std::ldexp(1.25f, -t0),
+                     (const float*)c->d.p, c->ed.p);
   HIPCHK(hipGetLastError());
   detect_cp(c, c->cpD, p.contact_gap_max, c->ed.p);
   bool wall = c->num_walls > 0;
@@ -1618,17 +1643,17 @@ float line_search_cp(ando_ctx* c, const ando_params& p) {
     double sq = (double)(w.x * w.x + (w.y * w.y + w.z * w.z));
     wall = sq > 0.5;
   }
-  HIPCHK(hipMemsetD32Async(c->scalars.p + 1, 0xFFFFF, 1, s));
+  HIPCHK(hipMemsetD32Async(c->scalars.p + 1, 0xFFFFFu & ~((1u << t0) - 1u), 1, s));
   if (c->sc_total > 0)
-    hipLaunchKernelGGL(ando::cp::k_ls_toi_self, dim3(nblk(c->sc_total)), dim3(kBlock), 0, s, c->sc_total,
+    hipLaunchKernelGGL(ando::cp::k_ls_toi_self, dim3(nblk(c->sc_total)), dim3(kBlock), 0, s, c->sc_total, t0,
                        (const ando::SelfContact*)c->sc.p, (const float*)c->x.p, (const float*)c->ed.p,
                        c->scalars.p + 1);
   if (c->rc_total > 0)
-    hipLaunchKernelGGL(ando::cp::k_ls_toi_rigid, dim3(nblk(c->rc_total)), dim3(kBlock), 0, s, c->rc_total,
+    hipLaunchKernelGGL(ando::cp::k_ls_toi_rigid, dim3(nblk(c->rc_total)), dim3(kBlock), 0, s, c->rc_total, t0,
                        (const RigidContact*)c->rc.p, c->cpD, (const float*)c->x.p, (const float*)c->ed.p,
                        c->scalars.p + 1);
   if (wall)
-    hipLaunchKernelGGL(ando::cp::k_ls_wall, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, w, c->wall_off[0],
+    hipLaunchKernelGGL(ando::cp::k_ls_wall, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, t0, w, c->wall_off[0],
                        (const float*)c->x.p, (const float*)c->ed.p, c->scalars.p + 1);
   HIPCHK(hipGetLastError());
   uint32_t mask = 0;
@@ -1932,6 +1957,18 @@ int ando_set_mesh(ando_ctx* c, const float* vertices, int32_t n, const int32_t* 
     build_topology(c, rs.edges);
     c->edges_h = rs.edges;
     c->d_edges.upload(c->edges_h, c->stream);
+    {
+      double sum = 0.0;
+      const size_t E = c->edges_h.size() / 2;
+      for (size_t e = 0; e < E; ++e) {
+        const float* a = vertices + 3 * (size_t)c->edges_h[2 * e];
+        const float* b = vertices + 3 * (size_t)c->edges_h[2 * e + 1];
+        const double dx = (double)b[0] - (double)a[0], dy = (double)b[1] - (double)a[1],
+                     dz = (double)b[2] - (double)a[2];
+        sum += std::sqrt(dx * dx + dy * dy + dz * dz);
+      }
+      c->mean_edge = E ? sum / (double)E : 0.0;
+    }
     alloc_state(c);
     hipStream_t s = c->stream;
     c->mverts.upload(vertices, 3 * (size_t)n, s);

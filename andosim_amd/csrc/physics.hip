@@ -11,9 +11,14 @@
 //     step, the swept trial motion x -> x + 1.25 d after every solve), grown by
 //     gap_max on the query side; their gaps, normals and closest-point weights
 //     are refreshed at every Newton iteration;
-//   * line search: every candidate pair's additive-CCD time of impact over the
-//     trial motion (FP64, keeping a separation of 32 FP32 ulps) bounds alpha,
-//     and the first wall may not get more violated.
+//   * line search: it starts at the largest alpha = 2^-t0 whose trial motion
+//     1.25 alpha d stays within L = max(4 x the mean rest edge length, 2 x the
+//     longest remaining inertial displacement |x_hat - x|) -- the membrane
+//     Hessian has no out-of-plane stiffness, so a contact force on a taut
+//     region can give a direction tens of centimetres long whose swept boxes
+//     cover most of the mesh; every candidate pair's additive-CCD time of
+//     impact over the trial motion (FP64, keeping a separation of 32 FP32
+//     ulps) bounds alpha, and the first wall may not get more violated.
 // Layout: per face 6 floats for 2k B B^T (entries 00 01 02 11 12 22, `gk`)
 // and for I - n n^T (`fpi`); per block 3 floats of isotropic elastic
 // stiffness (`Hiso3`, the same sum on the three diagonal entries) and a 9-float
@@ -361,17 +366,42 @@ __device__ inline double accd_sep(int type, const d3* x0, const d3* p) {
 // Line search: bit t of a contact's mask is set when alpha_t = 2^-t is at most
 // its time of impact (accd_sep) over the whole trial motion (x -> x + ed;
 // rigid triangles stay put); masks are AND-reduced into *mask.
-__device__ inline uint32_t toi_bits(double toi) {
+// The line search starts at trial t0: its motion x -> x + ed covers
+// alpha = 2^-t0, and trial t >= t0 moves 2^-(t - t0) of it.
+__device__ inline uint32_t toi_bits(double toi, int t0) {
   uint32_t m = 0u;
   double alpha = 1.0;
-  for (int t = 0; t < 20; ++t) {
+  for (int t = t0; t < 20; ++t) {
     if (alpha <= toi) m |= 1u << t;
     alpha *= 0.5;
   }
   return m;
 }
 
-__global__ void k_ls_toi_self(int nc, const SelfContact* __restrict__ sc, const float* __restrict__ x,
+// Squared lengths, in FP64, of the longest Newton displacement d_i and of the
+// longest remaining inertial displacement x_hat_i - x_i; maxima kept as the
+// bit patterns of the (non-negative) doubles in out[0], out[1] (start at 0).
+__global__ void k_motion_max(int n, const float* __restrict__ d, const float* __restrict__ x,
+                             const float* __restrict__ xhat, unsigned long long* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double dd = 0.0, hh = 0.0;
+  if (i < n) {
+    const double d0 = d[3 * i], d1 = d[3 * i + 1], d2 = d[3 * i + 2];
+    dd = d0 * d0 + d1 * d1 + d2 * d2;
+    const double h0 = (double)xhat[3 * i] - (double)x[3 * i], h1 = (double)xhat[3 * i + 1] - (double)x[3 * i + 1],
+                 h2 = (double)xhat[3 * i + 2] - (double)x[3 * i + 2];
+    hh = h0 * h0 + h1 * h1 + h2 * h2;
+  }
+  __shared__ double lds[4];
+  const double md = block_reduce(dd, lds, [](double a, double b) { return fmax(a, b); });
+  const double mh = block_reduce(hh, lds, [](double a, double b) { return fmax(a, b); });
+  if (threadIdx.x == 0) {
+    atomicMax(out, (unsigned long long)__double_as_longlong(md));
+    atomicMax(out + 1, (unsigned long long)__double_as_longlong(mh));
+  }
+}
+
+__global__ void k_ls_toi_self(int nc, int t0, const SelfContact* __restrict__ sc, const float* __restrict__ x,
                               const float* __restrict__ ed, uint32_t* __restrict__ mask) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t m = 0xFFFFFu;
@@ -382,14 +412,14 @@ __global__ void k_ls_toi_self(int nc, const SelfContact* __restrict__ sc, const 
       x0[k] = dld(x, C.idx[k]);
       p[k] = dld(ed, C.idx[k]);
     }
-    m = toi_bits(accd_sep(C.type, x0, p));
+    m = toi_bits(accd_sep(C.type, x0, p), t0);
   }
   __shared__ uint32_t lds[4];
   const uint32_t r = block_reduce(m, lds, [](uint32_t a, uint32_t b) { return a & b; });
   if (threadIdx.x == 0 && r != 0xFFFFFu) atomicAnd(mask, r);
 }
 
-__global__ void k_ls_toi_rigid(int total, const RigidContact* __restrict__ rc, DetectBodies D,
+__global__ void k_ls_toi_rigid(int total, int t0, const RigidContact* __restrict__ rc, DetectBodies D,
                                const float* __restrict__ x, const float* __restrict__ ed, uint32_t* __restrict__ mask) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t m = 0xFFFFFu;
@@ -403,16 +433,16 @@ __global__ void k_ls_toi_rigid(int total, const RigidContact* __restrict__ rc, D
       x0[k] = dld(B.world, B.tris[3 * r.tri + k - 1]);
       p[k] = dmk(0.0, 0.0, 0.0);
     }
-    m = toi_bits(accd_sep(0, x0, p));
+    m = toi_bits(accd_sep(0, x0, p), t0);
   }
   __shared__ uint32_t lds[4];
   const uint32_t rr = block_reduce(m, lds, [](uint32_t a, uint32_t b) { return a & b; });
   if (threadIdx.x == 0 && rr != 0xFFFFFu) atomicAnd(mask, rr);
 }
 
-// The first wall: trial t is feasible for vertex i when its signed distance at
-// x + alpha_t ed is at least min(0, its distance at x).
-__global__ void k_ls_wall(int n, f3 wn, float woff, const float* __restrict__ x, const float* __restrict__ ed,
+// The first wall: trial t >= t0 is feasible for vertex i when its signed
+// distance at x + 2^-(t - t0) ed is at least min(0, its distance at x).
+__global__ void k_ls_wall(int n, int t0, f3 wn, float woff, const float* __restrict__ x, const float* __restrict__ ed,
                           uint32_t* __restrict__ mask) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t m = 0xFFFFFu;
@@ -422,7 +452,7 @@ __global__ void k_ls_wall(int n, f3 wn, float woff, const float* __restrict__ x,
     const float lim = fminf(0.0f, g0);
     float alpha = 1.0f;
     m = 0u;
-    for (int t = 0; t < 20; ++t) {
+    for (int t = t0; t < 20; ++t) {
       const f3 xn = mk(xi.x + alpha * ei.x, xi.y + alpha * ei.y, xi.z + alpha * ei.z);
       if (dot(wn, xn) - woff >= lim) m |= 1u << t;
       alpha *= 0.5f;

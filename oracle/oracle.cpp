@@ -314,6 +314,19 @@ static double pair_dist_d(int type, const D3* q) {
 // half the initial gap when that is smaller; s = 0.1, t_c = 1, at most 256
 // advances.  1 when the whole motion x0 -> x0 + p keeps the pair farther than
 // xi, else a time up to which it provably does.
+// Corrected physics: the first line-search trial, the smallest t0 (at most 19)
+// with 1.25 * 2^-t0 * max |d_i| <= L = max(4 * mean rest edge, 2 * max
+// |x_hat_i - x_i|) (squared maxima given, FP64): the membrane Hessian has no
+// out-of-plane stiffness, so a contact force on a taut region can give a
+// direction tens of centimetres long whose swept boxes cover most of the mesh.
+static int cp_first_trial(double mean_edge, double md2, double mh2) {
+  const double L = std::max(4.0 * mean_edge, 2.0 * std::sqrt(mh2));
+  const double len = 1.25 * std::sqrt(md2);
+  int t0 = 0;
+  while (t0 < 19 && std::ldexp(len, -t0) > L) ++t0;
+  return t0;
+}
+
 static double accd_sep(int type, const D3* x0, const D3* p) {
   const D3 pbar = dscale(0.25, dadd(dadd(p[0], p[1]), dadd(p[2], p[3])));
   D3 q[4], x[4];
@@ -692,6 +705,7 @@ template <class R> struct Sim {
   std::vector<R> Hiso, Hbase_iso;         // 9 per block: isotropic elastic stiffness; plus mass / dt^2
   std::vector<R> vbox;                    // 6 per vertex: candidate boxes (lo xyz, hi xyz)
   std::vector<std::vector<V3<R>>> rworld; // rigid world vertices during the step
+  double mean_edge = 0.0;                 // mean rest edge length (FP64, edge order)
   static int sym6(int a, int b) {
     static const int t[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
     return t[a][b];
@@ -721,6 +735,14 @@ template <class R> struct Sim {
         uint64_t key = a < b ? (uint64_t(uint32_t(a)) << 32) | uint32_t(b) : (uint64_t(uint32_t(b)) << 32) | uint32_t(a);
         if (es.insert(key).second) edges.push_back({std::min(a, b), std::max(a, b)});
       }
+    mean_edge = 0.0;
+    for (const auto& e : edges) {
+      const double dx = (double)mverts[e[1]][0] - (double)mverts[e[0]][0];
+      const double dy = (double)mverts[e[1]][1] - (double)mverts[e[0]][1];
+      const double dz = (double)mverts[e[1]][2] - (double)mverts[e[0]][2];
+      mean_edge += std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    if (!edges.empty()) mean_edge /= (double)edges.size();
     int nv = (int)mverts.size();
     v2f.assign(nv, {});
     for (size_t f = 0; f < tris.size(); ++f)
@@ -1982,9 +2004,21 @@ template <class R> struct Sim {
   }
   // LineSearch::search with the corrected feasibility rule; the candidates of
   // the trial motion x -> x + 1.25 d replace `contacts`.
-  R line_search_cp(const std::vector<R>& d, const V3<R>& wn, R woff) {
+  // The first trial t0 (cp_first_trial) from the longest |d_i| and the
+  // longest remaining inertial displacement |xt_i - x_i|, both in FP64.
+  R line_search_cp(const std::vector<R>& d, const V3<R>& wn, R woff, const std::vector<R>& xt) {
+    double md2 = 0.0, mh2 = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double d0 = d[3 * i], d1 = d[3 * i + 1], d2 = d[3 * i + 2];
+      md2 = std::fmax(md2, d0 * d0 + d1 * d1 + d2 * d2);
+      const double h0 = (double)xt[3 * i] - (double)x[3 * i], h1 = (double)xt[3 * i + 1] - (double)x[3 * i + 1],
+                   h2 = (double)xt[3 * i + 2] - (double)x[3 * i + 2];
+      mh2 = std::fmax(mh2, h0 * h0 + h1 * h1 + h2 * h2);
+    }
+    const int t0 = cp_first_trial(mean_edge, md2, mh2);
+    const R c0 = (R)std::ldexp(1.25, -t0);
     std::vector<R> ed(3 * n);
-    for (int i = 0; i < 3 * n; ++i) ed[i] = R(1.25) * d[i];
+    for (int i = 0; i < 3 * n; ++i) ed[i] = c0 * d[i];
     detect_cp(&ed, contacts);
     double toi = 1.0;
     auto d3of = [](const V3<R>& v) { return D3{(double)v[0], (double)v[1], (double)v[2]}; };
@@ -2010,8 +2044,8 @@ template <class R> struct Sim {
       toi = std::min(toi, t);
     }
     const bool wall = (double)sqn3(wn) > 0.5;
-    R alpha = R(1);
-    for (int it = 0; it < 20; ++it) {
+    R alpha = R(1);  // of the motion ed
+    for (int it = t0; it < 20; ++it) {
       bool ok = (double)alpha <= toi;
       if (ok && wall) {
 #pragma omp parallel for schedule(static) reduction(&& : ok)
@@ -2022,11 +2056,10 @@ template <class R> struct Sim {
           if (!(g1 >= std::min(R(0), g0))) ok = false;
         }
       }
-      if (ok) return alpha;
+      if (ok) return (R)std::ldexp((double)alpha, -t0);
       alpha *= R(0.5);
-      if (alpha < R(1e-6)) return R(0);
     }
-    return alpha;
+    return R(0);
   }
 
   // Integrator::inner_newton_step (integrator.cpp:152-236).
@@ -2058,7 +2091,7 @@ template <class R> struct Sim {
       R woff = R(0);
       for (const auto& w : walls)
         if (w.active) { wn = w.n; woff = w.off; break; }
-      R alpha = cp ? line_search_cp(d, wn, woff) : line_search(d, wn, woff, R(1.25));
+      R alpha = cp ? line_search_cp(d, wn, woff, xt) : line_search(d, wn, woff, R(1.25));
       st.alphas.push_back(alpha);
       if ((double)alpha < 1e-8) { st.line_search_failures += 1; return R(0); }
       R s = (R)((double)alpha * 1.25);
@@ -2881,7 +2914,7 @@ double orc_line_search(void* h, const double* d, double ext) {
     for (const auto& w : S.walls) if (w.active) { wn = w.n; woff = w.off; break; }
     // Corrected physics: its own rule over the trial motion x -> x + 1.25 d
     // (ext is not used), with the candidates of that motion.
-    r = S.cp ? S.line_search_cp(dd, wn, woff) : S.line_search(dd, wn, woff, (R)ext);
+    r = S.cp ? S.line_search_cp(dd, wn, woff, S.x) : S.line_search(dd, wn, woff, (R)ext);
   });
   return r;
 }

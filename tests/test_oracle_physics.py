@@ -298,3 +298,23 @@ def test_separation_ccd_edge_edge_crossing():
     xi = 32 * 2.0 ** -23 * 1.0
     assert 1e-4 - t * 1e-3 >= xi + 0.1 * (1e-4 - xi) - 1e-15
     assert accd_sep("ee", x0, -p) == 1.0
+
+
+def test_line_search_first_trial_is_bounded():
+    """The trial motion 1.25 alpha d of the first trial stays within L = max(4 x
+    mean rest edge, 2 x max |x_hat - x|): a 1 m direction on a sheet of 5 cm
+    edges starts at alpha = 1/8, a 1 cm one at alpha = 1."""
+    V, T = scenes.grid(3, 0.1, 0.3)
+    sc = scenes.Scene("w", V.astype(np.float32), T, dict(scenes.BASE_MATERIAL), scenes.base_params(), [], [], [])
+    o = physics(sc)
+    X = sc.verts.astype(np.float64)
+    lengths = [np.linalg.norm(X[a] - X[b]) for a, b in {tuple(sorted((t[i], t[(i + 1) % 3]))) for t in T
+                                                          for i in range(3)}]
+    L = 4.0 * float(np.mean(lengths))
+    d = np.zeros((9, 3))
+    d[4, 2] = 1.0
+    a = o.line_search(d)
+    assert a * 1.25 <= L < 2 * a * 1.25
+    assert a == 0.125
+    d[4, 2] = 0.01
+    assert o.line_search(d) == 1.0
