@@ -1,29 +1,25 @@
 """Benchmark: AndoSim per-step cloth solve on one MI355X.
 
-Workload: cyl257, a 257x257 cloth (66,049 vertices, 131,072 triangles)
-falling onto two crossed cylinders (r = 0.1 m, 4,800 triangles); material and
-parameters of the 21x21 drop (E=3e5, nu=0.3, t=1 mm, rho=200, dt=0.01,
-gap 1 mm, 5% strain limit, PCG tol 1e-6), faithful mode.  One "step" is one
-add-on substep (operators.py:900-905): State.apply_gravity followed by
-Integrator.step, with the state resident in HBM.
-
-Timed window.  The faithful FP32 solver diverges a few steps after first
-contact on this workload (the strain barrier's m/gap^2 stiffness makes PCG
-stall at its 1000-iteration cap; the FP32 oracle diverges the same way).  A
-run past that point measures NaN arithmetic, so the bench
-first replays the drop untimed to find the window [first contact, last finite
-step], keeps the device state at the window start, and then times --steps
-substeps by replaying that window, restoring the saved state (untimed)
-between replays.  Every timed step is checked finite afterwards; a
-non-finite state aborts the bench.
+Headline workload: cyl257, a 257x257 cloth (66,049 vertices, 131,072
+triangles) falling onto two crossed cylinders (r = 0.1 m, 4,800 triangles);
+material and parameters of the 21x21 drop (E=3e5, nu=0.3, t=1 mm, rho=200,
+dt=0.01, gap 1 mm, 5% strain limit, PCG tol 1e-6), run with the corrected
+physics (ANDO_MODE_CORRECTED_PHYSICS: membrane force from the current
+positions, self and rigid contact candidates from swept boxes, FP64 CCD line
+search).  One "step" is one add-on substep (operators.py:900-905):
+State.apply_gravity followed by Integrator.step, with the state resident in
+HBM.  --warmup substeps from the initial state run untimed (first contact is
+at substep 9), then --steps contiguous substeps are timed; every timed state
+is checked finite and the intersection validator runs on the last one.
 
 The steps/s value comes from a pass without per-launch events.  A second
-pass over the same window records HIP events around every PCG launch on the
-library's stream; the roofline entry for the dominant kernel (the PCG SpMV)
-comes from it.  The CPU baseline is the oracle on a bounded sample of the same
-window, with its element-wise loops on all host threads OpenMP is given; the
-same oracle on one pinned core (the reference is single-threaded) is printed
-beside it.
+pass from the same start state records HIP events around every PCG launch on
+the library's stream; the roofline entry for the dominant kernel (the PCG
+SpMV) comes from it.  The CPU baseline is the oracle (the same corrected
+physics, FP32) on the first timed substep from the same state, its
+element-wise loops on all host threads OpenMP is given; the FP64 oracle on
+that substep gives the accuracy figures.  The faithful mode (the reference's
+step as written) is reported beside it on the window where it stays finite.
 """
 from __future__ import annotations
 
@@ -77,20 +73,22 @@ def corrected_contact_leg(steps: int):
     +-0.03 mm, so they start interpenetrating in places) just above a ground
     wall, run for `steps` substeps in faithful mode (self broad phase empty, as
     the reference), in corrected mode (LBVH self PT/EE contacts and barriers,
-    the reference's sampled CCD in the line search) and in corrected mode with
-    additive CCD; steps/s, contacts per step, PCG iterations and the
-    intersection validator at the end of each run."""
+    the reference's sampled CCD in the line search), in corrected mode with
+    additive CCD and with the corrected physics; steps/s, contacts per step,
+    PCG iterations and failures and the intersection validator at the end of
+    each run."""
     from andosim_amd import native, scenes
 
     out = {"workload": "stack_resting: 8 sheets 65x65 (33,800 v, 65,536 tris, 98,304 edges) 0.08 mm apart, "
                        "jitter 0.03 mm, ground wall", "steps": steps}
-    for mode in ("faithful", "corrected", "corrected_additive_ccd"):
+    for mode in ("faithful", "corrected", "corrected_additive_ccd", "corrected_physics"):
         sc = scenes.stack_resting()
         sim = native.GpuSim(sc)
-        sim.set_mode(fp64_near=True, corrected=(mode != "faithful"), additive_ccd=(mode == "corrected_additive_ccd"))
+        sim.set_mode(fp64_near=True, corrected=(mode != "faithful"), additive_ccd=(mode == "corrected_additive_ccd"),
+                     corrected_physics=(mode == "corrected_physics"))
         t = 0.0
         contacts, pcg = [], []
-        ls_fail = 0
+        ls_fail = pcg_fail = 0
         done = 0
         for _ in range(steps):
             t0 = time.perf_counter()
@@ -100,13 +98,14 @@ def corrected_contact_leg(steps: int):
             contacts.append(st["contacts"])
             pcg.append(st["pcg_iterations"])
             ls_fail += st["line_search_failures"]
+            pcg_fail += st["pcg_failures"]
             done += 1
             if not finite_state(sim):
                 break
         et, ins = sim.count_intersections()
         out[mode] = {"steps_per_s": round(done / t, 2), "steps_run": done, "finite": finite_state(sim),
                      "contacts_per_step": contacts, "pcg_iters_per_step": round(float(np.mean(pcg)), 1),
-                     "line_search_failures": ls_fail,
+                     "line_search_failures": ls_fail, "pcg_failures": pcg_fail,
                      "edge_triangle_intersections_at_end": et}
     return out
 
@@ -165,7 +164,7 @@ def per_frame_leg(sim, sc, reps: int = 20, with_cpu: bool = True):
 
 def batched_leg(steps: int, count: int = 256):
     """The batched workload (`count` cloths of side U{17..65}, 1-4 pins, wind,
-    ground wall; about 481k vertices for 256): cloth-steps/s stepping the
+    ground wall; about 481k vertices for 256), corrected physics: cloth-steps/s stepping the
     contexts one after another (the reference's only way) and through
     ando_step_batch (64 host threads, a stream per cloth, the solves of
     threads that reach PCG together run in batched launches)."""
@@ -174,6 +173,8 @@ def batched_leg(steps: int, count: int = 256):
     out = {"workload": f"batch{count}: {count} independent cloths, sides U{{17..65}}", "substeps": steps}
     for mode in ("sequential", "batched"):
         sims = [native.GpuSim(sc) for sc in scenes.batch256(count=count)]
+        for g in sims:
+            g.set_mode(fp64_near=True, corrected_physics=True)
         for g in sims:  # warm-up substep (constants, allocations)
             g.substep()
         t = 0.0
@@ -188,7 +189,9 @@ def batched_leg(steps: int, count: int = 256):
                 native.step_batch(sims, threads=64)
             t += time.perf_counter() - t0
         out[mode] = {"cloth_steps_per_s": round(count * steps / t, 1),
-                     "vertices": int(sum(g.n for g in sims))}
+                     "vertices": int(sum(g.n for g in sims)),
+                     "pcg_failures_last_substep": int(sum(g.stats()["pcg_failures"] for g in sims)),
+                     "finite": all(finite_state(g) for g in sims)}
         if mode == "batched":
             out[mode]["host_threads"] = 64
             out[mode]["batched_solve_share"] = round(
@@ -196,40 +199,142 @@ def batched_leg(steps: int, count: int = 256):
     return out
 
 
-def fp64_accuracy(sim, sc, saved, restore, window, orc, threads, steps: int = 3):
-    """Accuracy of the measured result: the first `steps` substeps of the timed
-    window from its start state on the GPU (FP32 with its FP64 islands), in the
-    FP32 oracle and in the FP64 oracle; after each substep, max |x - x64| /
-    max |x64| for the GPU and for the FP32 oracle against the FP64 oracle."""
-    k = min(steps, len(window))
-    restore()
-    gpu = []
-    for _ in range(k):
+def oracle_substep(sc, state, bits: int, physics: bool):
+    """One oracle substep (FP32 or FP64) from a saved (x, v, rigid poses) state;
+    returns the positions and the wall-clock seconds."""
+    from oracle import oracle as orc
+    o = orc.OracleSim(sc, bits)
+    o.set_physics(physics)
+    x0, v0, rig = state
+    o.set("x", x0.astype(np.float64))
+    o.set("v", v0.astype(np.float64))
+    for b, rs in enumerate(rig):
+        o.set_rigid_pose(b, rs)
+    t0 = time.perf_counter()
+    o.substep()
+    return o.get("x").astype(np.float64), time.perf_counter() - t0, o.stats()
+
+
+def rel_err(a, b) -> float:
+    return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-30))
+
+
+def accuracy_vs_fp64(sc, state, x_gpu, x_fp32=None, physics: bool = True):
+    """max |x - x64| / max |x64| after one substep from `state` for the GPU and
+    for the FP32 oracle, against the FP64 oracle (host threads)."""
+    x64, t64, _ = oracle_substep(sc, state, 64, physics)
+    if x_fp32 is None:
+        x_fp32, _, _ = oracle_substep(sc, state, 32, physics)
+    return {"reference": "oracle in FP64 on the host, same start state, one substep",
+            "gpu_max_rel_err": float(f"{rel_err(x_gpu, x64):.3e}"),
+            "fp32_oracle_max_rel_err": float(f"{rel_err(x_fp32, x64):.3e}"),
+            "fp64_oracle_seconds": round(t64, 1)}
+
+
+def snapshot(sim):
+    return (sim.get("x").copy(), sim.get("v").copy(), [sim.rigid_state(b) for b in range(sim.nbodies)])
+
+
+def restore_state(sim, state):
+    x0, v0, rig = state
+    sim.set_state(x0, v0)
+    for b, rs in enumerate(rig):
+        st = sim.bodies[b]
+        for k in range(3):
+            st.position[k] = float(rs["position"][k])
+            st.linear_velocity[k] = float(rs["linear_velocity"][k])
+            st.angular_velocity[k] = float(rs["angular_velocity"][k])
+        for k in range(9):
+            st.rotation[k] = float(rs["rotation"].ravel()[k])
+
+
+def run_steps(sim, count: int, check_every: int = 0):
+    """`count` substeps; per-step (seconds, stats); stops at a non-finite
+    state.  Returns (records, finite)."""
+    recs = []
+    for k in range(count):
+        t0 = time.perf_counter()
         sim.substep()
-        gpu.append(sim.get("x").astype(np.float64))
-    orc.set_threads(threads)
+        recs.append((time.perf_counter() - t0, sim.stats()))
+        if (check_every and k % check_every == check_every - 1) or k == count - 1:
+            if not finite_state(sim):
+                return recs, False
+    return recs, True
 
-    def run(bits):
-        o = orc.OracleSim(sc, bits)
-        x0, v0, rig = saved
-        o.set("x", x0.astype(np.float64))
-        o.set("v", v0.astype(np.float64))
-        for b, rs in enumerate(rig):
-            o.set_rigid_pose(b, rs)
-        xs = []
-        for _ in range(k):
-            o.substep()
-            xs.append(o.get("x").astype(np.float64))
-        return xs
 
-    o32, o64 = run(32), run(64)
+def summarize(recs):
+    el = sum(r[0] for r in recs)
+    n = max(1, len(recs))
+    st = [r[1] for r in recs]
+    return {"steps_per_s": round(len(recs) / el, 3) if el > 0 else None,
+            "ms_per_step": round(1e3 * el / n, 2),
+            "pcg_iters_per_step": round(sum(t["pcg_iterations"] for t in st) / n, 1),
+            "pcg_solves": sum(t["pcg_solves"] for t in st),
+            "pcg_failures": sum(t["pcg_failures"] for t in st),
+            "line_search_failures": sum(t["line_search_failures"] for t in st),
+            "newton_iters_per_step": round(sum(t["newton_iterations"] for t in st) / n, 2),
+            "max_candidates": max((t["max_candidates"] for t in st), default=0)}
 
-    def rel(a, b):
-        return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-30))
 
-    return {"reference": "oracle in FP64 on the host, same start state", "substeps": k,
-            "gpu_max_rel_err_per_step": [float(f"{rel(g, r):.3e}") for g, r in zip(gpu, o64)],
-            "fp32_oracle_max_rel_err_per_step": [float(f"{rel(c, r):.3e}") for c, r in zip(o32, o64)]}
+def physics_leg(name: str, steps: int, accuracy: bool):
+    """A workload run from its initial state with the corrected physics for
+    `steps` substeps: throughput, solver counters, finiteness and the
+    intersection validator at the end; optionally the FP64-oracle error of
+    the GPU's substep from the state half-way through."""
+    from andosim_amd import native, scenes
+    sc = scenes.WORKLOADS[name]()
+    sim = native.GpuSim(sc, fp64_near=True)
+    sim.set_mode(fp64_near=True, corrected_physics=True)
+    half = steps // 2
+    recs, ok = run_steps(sim, half, 10)
+    state = snapshot(sim)
+    x_mid = None
+    if ok:
+        sim.substep()
+        x_mid = sim.get("x").astype(np.float64)
+        restore_state(sim, state)
+        more, ok = run_steps(sim, steps - half, 10)
+        recs += more
+    out = {"workload": name, "vertices": int(len(sc.verts)), "steps_run": len(recs), "finite": ok,
+           **summarize(recs)}
+    et, ins = sim.count_intersections()
+    out["intersections_at_end"] = {"edge_triangle": et, "vertices_inside_bodies": ins}
+    if accuracy and x_mid is not None:
+        out["accuracy_vs_fp64"] = accuracy_vs_fp64(sc, state, x_mid)
+        out["accuracy_vs_fp64"]["at_substep"] = half
+    return out
+
+
+def faithful_leg(name: str, max_explore: int, steps: int):
+    """The reference's step as written (faithful mode) on the same workload:
+    it diverges a few substeps after first contact (the strain barrier's
+    m/gap^2 stiffness stalls PCG at its cap; the FP32 oracle does the same), so
+    it is timed on the substeps from first contact to the last finite one."""
+    from andosim_amd import native, scenes
+    sc = scenes.WORKLOADS[name]()
+    sim = native.GpuSim(sc, fp64_near=True)
+    first, saved, last_ok = None, None, -1
+    for k in range(max_explore):
+        snap = snapshot(sim) if first is None else None
+        sim.substep()
+        if not finite_state(sim):
+            break
+        last_ok = k
+        if first is None and sim.stats()["contacts"] > 0:
+            first, saved = k, snap
+    if first is None:
+        return {"workload": name, "error": "no contact within the explored substeps"}
+    window = last_ok - first + 1
+    recs = []
+    while len(recs) < steps:
+        restore_state(sim, saved)
+        more, ok = run_steps(sim, min(window, steps - len(recs)))
+        recs += more
+        if not ok:
+            break
+    return {"workload": name, "mode": "faithful", "window": [first, last_ok],
+            "first_nonfinite_step": last_ok + 1 if last_ok + 1 < max_explore else None,
+            "steps_timed": len(recs), **summarize(recs)}
 
 
 def learned_comparison(sim, sc, window, restore, replay, a):
@@ -268,7 +373,7 @@ def learned_comparison(sim, sc, window, restore, replay, a):
 
     def steps(warm_only):
         sim.set_mode(fp64_near=True, kernel_timing=False, learned=True,
-                     warm_start_only=warm_only)
+                     warm_start_only=warm_only, corrected_physics=True)
         restore()
         recs, nonfinite = [], None
         for k in window:
@@ -278,7 +383,7 @@ def learned_comparison(sim, sc, window, restore, replay, a):
             if not finite_state(sim):
                 nonfinite = k
                 break
-        sim.set_mode(fp64_near=True, kernel_timing=False, learned=False)
+        sim.set_mode(fp64_near=True, kernel_timing=False, learned=False, corrected_physics=True)
         el = sum(r[0] for r in recs)
         return {"pcg_iters_per_step": round(sum(r[1]["pcg_iterations"] for r in recs) / len(recs), 2),
                 "steps_per_s": round(len(recs) / el, 3), "nonfinite_at_step": nonfinite,
@@ -310,18 +415,19 @@ def finite_state(sim) -> bool:
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30, help="timed substeps")
-    ap.add_argument("--warmup", type=int, default=3, help="untimed substeps replayed before timing")
+    ap.add_argument("--steps", type=int, default=100, help="timed contiguous substeps")
+    ap.add_argument("--warmup", type=int, default=10, help="untimed substeps from the initial state")
     ap.add_argument("--workload", default="cyl257")
-    ap.add_argument("--max-explore", type=int, default=300, help="substeps searched for the valid window")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="per CPU baseline run")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--event-steps", type=int, default=30, help="substeps of the kernel-event pass")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the FP64 accuracy")
     ap.add_argument("--no-timing", action="store_true", help="skip the kernel-event pass")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner comparison")
+    ap.add_argument("--learned-steps", type=int, default=30)
     ap.add_argument("--train-steps", type=int, default=8000)
-    ap.add_argument("--no-corrected", action="store_true", help="skip the corrected-contact leg")
+    ap.add_argument("--no-legs", action="store_true", help="skip the other workloads")
+    ap.add_argument("--leg-accuracy", action="store_true",
+                    help="FP64-oracle error of a mid-run substep for every workload leg (minutes of host time)")
     ap.add_argument("--corrected-steps", type=int, default=20)
-    ap.add_argument("--no-batched", action="store_true", help="skip the batched-workload leg")
     ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--autotune", action="store_true",
                     help="every context picks its PCG launch shape by timing (ANDO_PCG_AUTOTUNE=1)")
@@ -330,190 +436,112 @@ def main(argv=None):
         print(f"bench.py runs on one GPU only (asked for --gpus {a.gpus}); sharding the solve over several GPUs "
               "is not built.", file=sys.stderr)
         return 2
-
     if a.autotune:
         os.environ.setdefault("ANDO_PCG_AUTOTUNE", "1")
     from andosim_amd import native, scenes
 
     sc = scenes.WORKLOADS[a.workload]()
     sim = native.GpuSim(sc, fp64_near=True)
-    sim.set_mode(fp64_near=True, kernel_timing=False)
+    sim.set_mode(fp64_near=True, corrected_physics=True)
     n = len(sc.verts)
     rp, ci = sim.pattern()
     nb = len(ci)
 
-    # Untimed exploration: first contact and last finite step.
-    first_contact = None
-    saved = None
-    last_ok = -1
-    for s in range(a.max_explore):
-        if first_contact is None:
-            snap = (sim.get("x").copy(), sim.get("v").copy(), [sim.rigid_state(b) for b in range(sim.nbodies)])
-        sim.substep()
-        st = sim.stats()
-        if not finite_state(sim):
-            break
-        last_ok = s
-        if first_contact is None and st["contacts"] > 0:
-            first_contact = s
-            saved = snap
-    if first_contact is None:
-        first_contact, saved = 0, None
-        sim = native.GpuSim(sc, fp64_near=True)
-        saved = (sim.get("x").copy(), sim.get("v").copy(), [sim.rigid_state(b) for b in range(sim.nbodies)])
-    window = list(range(first_contact, last_ok + 1))
-    if not window:
-        print(json.dumps({"metric": METRIC, "error": "no finite step after first contact"}), flush=True)
+    warm, ok = run_steps(sim, a.warmup, 1)
+    if not ok:
+        print(json.dumps({"metric": METRIC, "error": "non-finite state during warm-up"}), flush=True)
         return 1
+    saved = snapshot(sim)
+    recs, ok = run_steps(sim, a.steps, 1)
+    if not ok:
+        print(json.dumps({"metric": METRIC, "error": f"non-finite state at timed substep {len(recs)}"}), flush=True)
+        return 1
+    head = summarize(recs)
+    et, ins = sim.count_intersections()
+    last = recs[-1][1]
+    kernel, lanes = last["pcg_kernel"], last.get("spmv_lanes", 0)
+    wg_slices, vec_block = last.get("spmv_wg_slices", 4), last.get("pcg_vec_block", 256)
+    tuned = last.get("pcg_autotuned", 0) == 1
 
-    def restore():
-        x0, v0, rig = saved
-        sim.set_state(x0, v0)
-        for b, rs in enumerate(rig):
-            s = sim.bodies[b]
-            for k in range(3):
-                s.position[k] = float(rs["position"][k])
-                s.linear_velocity[k] = float(rs["linear_velocity"][k])
-                s.angular_velocity[k] = float(rs["angular_velocity"][k])
-            for k in range(9):
-                s.rotation[k] = float(rs["rotation"].ravel()[k])
-
-    def replay(count, timed):
-        """Run `count` substeps cycling through the window; returns per-step records."""
-        recs = []
-        done = 0
-        while done < count:
-            restore()
-            for _ in window:
-                if done >= count:
-                    break
-                t0 = time.perf_counter()
-                sim.substep()
-                dt_s = time.perf_counter() - t0
-                st = sim.stats()
-                if not finite_state(sim):
-                    raise RuntimeError("non-finite state inside the timed window")
-                recs.append((dt_s, st))
-                done += 1
-        return recs
-
-    replay(a.warmup, False)
-    recs = replay(a.steps, True)
-    elapsed = sum(r[0] for r in recs)
-    steps_per_s = len(recs) / elapsed
-    pcg_iters = sum(r[1]["pcg_iterations"] for r in recs)
-    newton = sum(r[1]["newton_iterations"] for r in recs)
-    kernel = recs[-1][1]["pcg_kernel"]
-    lanes = recs[-1][1].get("spmv_lanes", 0)
-    wg_slices = recs[-1][1].get("spmv_wg_slices", 4)
-    vec_block = recs[-1][1].get("pcg_vec_block", 256)
-    tuned = recs[-1][1].get("pcg_autotuned", 0) == 1
-
-    # Event pass: the same window again, every PCG launch bracketed by events.
-    ev_recs = []
-    if not a.no_timing:
-        sim.set_mode(fp64_near=True, kernel_timing=True)
-        ev_recs = replay(a.steps, True)
-        sim.set_mode(fp64_near=True, kernel_timing=False)
-    t_launch = [sum(r[1]["timed_ms"][k] for r in ev_recs) for k in range(2)]
-    n_launch = [sum(r[1]["timed_launches"][k] for r in ev_recs) for k in range(2)]
-    ev_elapsed = sum(r[0] for r in ev_recs)
-
+    # Event pass: from the same start state, every PCG launch bracketed by events.
     roof = None
-    if n_launch[0]:
-        avg_s = t_launch[0] / n_launch[0] * 1e-3
-        by = spmv_bytes(n, nb)
-        ach = by / avg_s / 1e9
-        kname = {3: "ando::k_pcg_spmv_sell"}.get(kernel, "ando::k_pcg_spmv<3>")
-        if kernel == 3 and lanes in (2, 4):
-            kname = f"ando::k_pcg_spmv_sell_split<{lanes}, {wg_slices}>"
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(a.workload, n, nb, kname), "kernel": kname,
-                "bytes_per_launch": by, "avg_launch_us": round(avg_s * 1e6, 2), "launches": n_launch[0]}
-        if n_launch[1]:
-            avg_u = t_launch[1] / n_launch[1] * 1e-3
-            roof["update_kernel_gbs"] = round(update_bytes(n) / avg_u / 1e9, 1)
-            roof["update_avg_launch_us"] = round(avg_u * 1e6, 2)
-            roof["pcg_kernels_share_of_step"] = round((t_launch[0] + t_launch[1]) * 1e-3 / ev_elapsed, 3)
-        roof["event_pass_ms_per_step"] = round(1e3 * ev_elapsed / len(ev_recs), 3)
+    if not a.no_timing:
+        restore_state(sim, saved)
+        sim.set_mode(fp64_near=True, kernel_timing=True, corrected_physics=True)
+        ev_recs, _ = run_steps(sim, min(a.event_steps, a.steps), 1)
+        sim.set_mode(fp64_near=True, kernel_timing=False, corrected_physics=True)
+        t_launch = [sum(r[1]["timed_ms"][k] for r in ev_recs) for k in range(2)]
+        n_launch = [sum(r[1]["timed_launches"][k] for r in ev_recs) for k in range(2)]
+        ev_elapsed = sum(r[0] for r in ev_recs)
+        if n_launch[0]:
+            avg_s = t_launch[0] / n_launch[0] * 1e-3
+            by = spmv_bytes(n, nb)
+            ach = by / avg_s / 1e9
+            kname = {3: "ando::k_pcg_spmv_sell"}.get(kernel, "ando::k_pcg_spmv<3>")
+            if kernel == 3 and lanes in (2, 4):
+                kname = f"ando::k_pcg_spmv_sell_split<{lanes}, {wg_slices}>"
+            roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4),
+                    "traffic": measured_traffic(a.workload, n, nb, kname), "kernel": kname,
+                    "bytes_per_launch": by, "avg_launch_us": round(avg_s * 1e6, 2), "launches": n_launch[0]}
+            if n_launch[1]:
+                avg_u = t_launch[1] / n_launch[1] * 1e-3
+                roof["update_kernel_gbs"] = round(update_bytes(n) / avg_u / 1e9, 1)
+                roof["update_avg_launch_us"] = round(avg_u * 1e6, 2)
+                roof["pcg_kernels_share_of_step"] = round((t_launch[0] + t_launch[1]) * 1e-3 / ev_elapsed, 3)
+            roof["event_pass_ms_per_step"] = round(1e3 * ev_elapsed / len(ev_recs), 3)
 
     frame = per_frame_leg(sim, sc, with_cpu=not a.no_cpu)
 
-    batched = None if a.no_batched else batched_leg(a.batched_steps)
-
-    corrected = None
-    if not a.no_corrected:
-        corrected = corrected_contact_leg(a.corrected_steps)
-
     learned = None
     if not a.no_learned:
-        learned = learned_comparison(sim, sc, window, restore, replay, a)
+        window = list(range(a.warmup, a.warmup + min(a.learned_steps, a.steps)))
+        learned = learned_comparison(sim, sc, window, lambda: restore_state(sim, saved), None, a)
 
-    cpu = cpu_single = accuracy = None
+    legs = None
+    if not a.no_legs:
+        legs = {"sphere129_corrected_physics": physics_leg("sphere129", 200, a.leg_accuracy),
+                "stack65x8_corrected_physics": physics_leg("stack65x8", 300, a.leg_accuracy),
+                "cyl257_faithful": faithful_leg(a.workload, 60, 20),
+                "batched": batched_leg(a.batched_steps),
+                "stack_resting_modes": corrected_contact_leg(a.corrected_steps)}
+
+    cpu = accuracy = None
     if not a.no_cpu:
         from oracle import oracle as orc
-
-        def cpu_run(threads: int):
-            """Oracle steps/s over the first substeps of the timed window, from
-            the same start state, stopping after --cpu-seconds."""
-            orc.set_threads(threads)
-            o = orc.OracleSim(sc, 32)
-            x0, v0, rig = saved
-            o.set("x", x0.astype(np.float64))
-            o.set("v", v0.astype(np.float64))
-            for b, rs in enumerate(rig):
-                o.set_rigid_pose(b, rs)
-            k = 0
-            c0 = time.perf_counter()
-            while k < len(window):
-                o.substep()
-                k += 1
-                if time.perf_counter() - c0 > a.cpu_seconds:
-                    break
-            cs = time.perf_counter() - c0
-            return k, cs
-
         threads = orc.get_threads()
-        k, cs = cpu_run(threads)
-        cpu = {"value": round(k / cs, 4), "unit": "steps/s", "cores": threads, "kind": "port",
-               "sample": f"first {k} substeps of the timed window (steps {window[0]}..{window[0] + k - 1} of "
-                         f"{a.workload}), oracle FP32 scan-free, OpenMP loops on {threads} threads "
-                         f"(results identical to 1 thread), {cs:.1f} s"}
-        # The reference itself is single-threaded: the same oracle on one pinned core.
-        aff = os.sched_getaffinity(0)
-        try:
-            os.sched_setaffinity(0, {min(aff)})
-            k1, cs1 = cpu_run(1)
-        finally:
-            os.sched_setaffinity(0, aff)
-        cpu_single = {"value": round(k1 / cs1, 4), "unit": "steps/s", "cores": 1, "kind": "port",
-                      "sample": f"first {k1} substeps of the timed window, oracle FP32 scan-free, 1 thread "
-                                f"pinned to one core, {cs1:.1f} s"}
-        accuracy = fp64_accuracy(sim, sc, saved, restore, window, orc, threads)
+        orc.set_threads(threads)
+        x32, cs, st32 = oracle_substep(sc, saved, 32, True)
+        cpu = {"value": round(1.0 / cs, 5), "unit": "steps/s", "cores": threads, "kind": "port",
+               "sample": f"substep {a.warmup} of {a.workload} (the first timed one) from the same state, "
+                         f"oracle FP32 with the corrected physics, OpenMP loops on {threads} threads (results "
+                         f"identical to 1 thread), {cs:.1f} s, {int(st32['pcg_iters'])} PCG iterations"}
+        restore_state(sim, saved)
+        sim.substep()
+        accuracy = accuracy_vs_fp64(sc, saved, sim.get("x").astype(np.float64), x32)
+        accuracy["at_substep"] = a.warmup
 
     out = {
-        "metric": METRIC, "value": round(steps_per_s, 3), "unit": "steps/s", "n_gpus": 1, "steps": len(recs),
-        "warmup": a.warmup, "ms_per_step": round(1e3 * elapsed / len(recs), 3), "higher_is_better": True,
+        "metric": METRIC, "value": head["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": len(recs),
+        "warmup": a.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{a.workload}: 257x257 cloth (66,049 v, 131,072 tris) onto two crossed cylinders "
-                               "r=0.1 m (4,800 rigid tris); dt=0.01, 5% strain limit, PCG tol 1e-6, faithful mode",
+                               "r=0.1 m (4,800 rigid tris); dt=0.01, 5% strain limit, PCG tol 1e-6, "
+                               "corrected physics with self contact",
                    "vertices": n, "blocks": nb, "precond": "block-Jacobi (plain PCG)",
+                   "timed_substeps": [a.warmup, a.warmup + len(recs) - 1],
                    "pcg_kernel": {0: "two-kernel, BSR row SpMV",
                                   3: f"two-kernel, sliced-ELL SpMV ({lanes} lane{'s' if lanes > 1 else ''} per row, "
                                      f"{wg_slices} slice{'s' if wg_slices != 1 else ''} per workgroup, "
                                      f"{vec_block}-thread vector kernels{', autotuned' if tuned else ''})"
-                                  }.get(kernel, "none"),
-                   "timed_window": [window[0], window[-1]],
-                   "first_nonfinite_step": last_ok + 1 if last_ok + 1 < a.max_explore else None},
-        "pcg_iters_per_step": round(pcg_iters / len(recs), 2),
-        "newton_iters_per_step": round(newton / len(recs), 2),
-        "learned_precond": learned,
-        "corrected_contact": corrected,
-        "per_frame_consumers": frame,
-        "batched": batched,
-        "roofline": roof, "cpu_baseline": cpu, "cpu_baseline_single_thread": cpu_single,
-        "accuracy_vs_fp64": accuracy,
+                                  }.get(kernel, "none")},
+        "pcg_iters_per_step": head["pcg_iters_per_step"],
+        "newton_iters_per_step": head["newton_iters_per_step"],
+        "pcg_solves": head["pcg_solves"], "pcg_failures": head["pcg_failures"],
+        "line_search_failures": head["line_search_failures"], "max_candidates": head["max_candidates"],
+        "intersections_at_end": {"edge_triangle": et, "vertices_inside_bodies": ins},
+        "roofline": roof, "cpu_baseline": cpu, "accuracy_vs_fp64": accuracy,
+        "learned_precond": learned, "legs": legs, "per_frame_consumers": frame,
     }
     print(json.dumps(out), flush=True)
     return 0
