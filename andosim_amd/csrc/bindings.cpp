@@ -1320,6 +1320,7 @@ PYBIND11_MODULE(ando_barrier_core, m) {
       .def_readonly("pcg_autotuned", &ando_stats::pcg_autotuned)
       .def_readonly("long_hit_sorts", &ando_stats::long_hit_sorts)
       .def_readonly("max_candidates", &ando_stats::max_candidates)
+      .def_readonly("held_iterations", &ando_stats::held_iterations)
       .def_readonly("learned_solves", &ando_stats::learned_solves)
       .def_readonly("learned_fallbacks", &ando_stats::learned_fallbacks)
       .def_readonly("fallback_iterations", &ando_stats::fallback_iterations)

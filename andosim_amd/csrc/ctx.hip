@@ -292,6 +292,8 @@ struct ando_ctx {
   ando::DetectBodies cpD{};  // the step's rigid bodies (they do not move during a step)
   double mean_edge = 0.0;    // mean rest edge length (FP64, edge order)
   DBuf<unsigned long long> motion2;  // line search: longest |d_i|^2 and |x_hat_i - x_i|^2 (double bits)
+  DBuf<uint8_t> hold0, hold1;        // line search: vertices held this iteration (double-buffered passes)
+  DBuf<uint32_t> hold_changed;
   float pt_box = 1e-4f, keep_gap = 0.01f;
   std::vector<int32_t> edges_h;
   DBuf<int32_t> d_edges;
@@ -307,7 +309,6 @@ struct ando_ctx {
   DBuf<ando::SelfTerm> sterm;
   DBuf<uint32_t> sv_keys, sv_keys2;
   DBuf<int32_t> sv_vals, sv_vals2, sv_counts, sv_offsets, sv_sums, sv_total, sv_hist, sv_ptr;
-  DBuf<double> part_pAp2;
   DBuf<int32_t> ccol;
   DBuf<float> cval;
   DBuf<int32_t> pt_of;     // faithful one-triangle mesh: PT contact of each point vertex, or -1
@@ -957,7 +958,6 @@ void prepare_self(ando_ctx* c, const StepParams& P) {
   hipLaunchKernelGGL(k_count_keys, dim3(nblk(cnt)), dim3(kBlock), 0, s, cnt, c->n, (const uint32_t*)c->sv_keys.p,
                      c->sv_hist.p);
   const int entries = exclusive_scan(c, c->n, c->sv_hist.p, c->sv_ptr.p, c->sv_sums, c->sv_total);
-  c->part_pAp2.ensure((size_t)nblk(c->n, 32) + 1);
   if (!c->corrected) {
     c->pt_of.ensure((size_t)c->n);
     HIPCHK(hipMemsetAsync(c->pt_of.p, 0xff, sizeof(int32_t) * c->n, s));
@@ -1203,8 +1203,8 @@ void autotune_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const i
 
 // P_given / x0_given (device, both or neither): a preconditioner of 9 floats
 // per vertex and a starting guess instead of block-Jacobi from x0 = 0.
-// sv (nullable): self-contact blocks that couple two vertices, applied after
-// the SpMV of every iteration (corrected contact mode; two-kernel row path).
+// sv (nullable): self-contact blocks that couple two vertices, multiplied
+// inside every SpMV (corrected modes).
 PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int32_t* cols, const int32_t* diag,
                const float* rhs, int negate, float* xsol, float tol, int maxit, SpmvMeta* sm,
                const float* P_given = nullptr, const float* x0_given = nullptr,
@@ -1296,6 +1296,10 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
   // sizing the first chunk from the previous step's solve.
   int chunk = std::max(1, c->pcg_chunk);
   const bool timing = c->kernel_timing != 0;
+  // Coupling blocks of the self contacts, multiplied inside the SpMV.
+  const int32_t* cptr = couple ? sv->ptr : nullptr;
+  const int32_t* ccol = couple ? (const int32_t*)c->ccol.p : nullptr;
+  const float* cval = couple ? (const float*)c->cval.p : nullptr;
   size_t ev_used = 0;
   // Timed launches record their start and end with the dispatch itself
   // (hipExtLaunchKernelGGL), so the events add no packets between kernels.
@@ -1320,7 +1324,8 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
       auto spmv = [&](auto kernel) {
         hipExtLaunchKernelGGL(kernel, dim3(nwg_r), dim3(kBlock), 0, s, e0, e1, 0, n, it, A, rp, cols,
                               (const float*)c->z.p, (const float*)pold, pcur, c->Ap.p, (const double*)c->part_rz.p,
-                              (const float*)c->part_max.p, nwg_v, c->rz_hist.p, c->part_pAp.p, c->pcg_status.p, tol);
+                              (const float*)c->part_max.p, nwg_v, c->rz_hist.p, c->part_pAp.p, c->pcg_status.p, tol,
+                              cptr, ccol, cval);
       };
       if (sell) {
         auto sell_launch = [&](auto kernel, int threads) {
@@ -1329,7 +1334,7 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
                                 (const int32_t*)sm->slice_off.p, (const int32_t*)sm->slice_w.p, sm->uniform_w,
                                 (const float*)c->z.p, (const float*)pold, pcur, c->Ap.p,
                                 (const double*)c->part_rz.p, (const float*)c->part_max.p, nwg_v, c->rz_hist.p,
-                                c->part_pAp.p, c->pcg_status.p, tol);
+                                c->part_pAp.p, c->pcg_status.p, tol, cptr, ccol, cval);
         };
         if (c->sell_lanes == 2 && sell_slices == 0) sell_launch(k_pcg_spmv_sell_split<2, 0>, 64);
         else if (c->sell_lanes == 2 && sell_slices == 1) sell_launch(k_pcg_spmv_sell_split<2, 1>, 128);
@@ -1345,15 +1350,10 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
           default: spmv(k_pcg_spmv<3>); break;
         }
       }
-      if (couple)
-        hipLaunchKernelGGL(ando::lbvh::k_self_spmv, dim3(nblk(n, 32)), dim3(kBlock), 0, s, n, sv->ptr,
-                           (const int32_t*)c->ccol.p, (const float*)c->cval.p, (const PcgStatus*)c->pcg_status.p,
-                           (const float*)pcur, c->Ap.p, c->part_pAp2.p);
       hipExtLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(vb), 0, s, e2, e3, 0, n, it, (const float*)pcur,
                             (const float*)c->Ap.p, Pc, pe, pi, xsol, c->r.p, c->z.p,
                             (const double*)c->part_pAp.p, nwg_r, (const double*)c->rz_hist.p, c->part_rz.p,
-                            c->part_max.p, c->pcg_status.p, (const double*)(couple ? c->part_pAp2.p : nullptr),
-                            nblk(n, 32));
+                            c->part_max.p, c->pcg_status.p);
     }
     HIPCHK(hipGetLastError());
   };
@@ -1635,6 +1635,44 @@ float line_search_cp(ando_ctx* c, const ando_params& p) {
                      (const float*)c->d.p, c->ed.p);
   HIPCHK(hipGetLastError());
   detect_cp(c, c->cpD, p.contact_gap_max, c->ed.p);
+  // Pairs within their separation that would not move apart hold their cloth
+  // vertices: up to four passes (a hold changes its neighbours' motion), then
+  // every such pair's vertices.
+  if (c->sc_total > 0 || c->rc_total > 0) {
+    const int n = c->n;
+    c->hold0.ensure((size_t)n);
+    c->hold1.ensure((size_t)n);
+    c->hold_changed.ensure(1);
+    HIPCHK(hipMemsetAsync(c->hold0.p, 0, (size_t)n, s));
+    uint8_t* fin = c->hold0.p;
+    uint8_t* fout = c->hold1.p;
+    bool any = false;
+    for (int pass = 0; pass < 5; ++pass) {
+      HIPCHK(hipMemcpyAsync(fout, fin, (size_t)n, hipMemcpyDeviceToDevice, s));
+      HIPCHK(hipMemsetAsync(c->hold_changed.p, 0, sizeof(uint32_t), s));
+      const int all = pass == 4 ? 1 : 0;
+      if (c->sc_total > 0)
+        hipLaunchKernelGGL(ando::cp::k_band_self, dim3(nblk(c->sc_total)), dim3(kBlock), 0, s, c->sc_total,
+                           (const ando::SelfContact*)c->sc.p, (const float*)c->x.p, (const float*)c->ed.p,
+                           (const uint8_t*)fin, fout, c->hold_changed.p, all);
+      if (c->rc_total > 0)
+        hipLaunchKernelGGL(ando::cp::k_band_rigid, dim3(nblk(c->rc_total)), dim3(kBlock), 0, s, c->rc_total,
+                           (const RigidContact*)c->rc.p, c->cpD, (const float*)c->x.p, (const float*)c->ed.p,
+                           (const uint8_t*)fin, fout, c->hold_changed.p, all);
+      HIPCHK(hipGetLastError());
+      uint32_t changed = 0;
+      read_small(c, &changed, (const uint32_t*)c->hold_changed.p, 1);
+      std::swap(fin, fout);
+      if (!changed) break;
+      any = true;
+    }
+    if (any) {
+      hipLaunchKernelGGL(ando::cp::k_apply_hold, dim3(nblk(n)), dim3(kBlock), 0, s, n, (const uint8_t*)fin, c->d.p,
+                         c->ed.p);
+      HIPCHK(hipGetLastError());
+      ++c->stats.held_iterations;
+    }
+  }
   bool wall = c->num_walls > 0;
   f3 w = mk(0.0f, 0.0f, 0.0f);
   if (wall) {
@@ -2575,10 +2613,11 @@ int ando_pcg_noalloc(int32_t device, int32_t num_rows, int32_t num_blocks, const
         float* pcur = (it & 1) ? p1 : p0;
         hipLaunchKernelGGL(k_pcg_spmv<R>, dim3(nwg_r), dim3(kBlock), 0, s, n, it, values, row_ptr, cols,
                            (const float*)z, (const float*)pold, pcur, Ap, (const double*)part_rz,
-                           (const float*)part_max, nwg_v, rz_hist, part_pAp, st, tol);
+                           (const float*)part_max, nwg_v, rz_hist, part_pAp, st, tol, (const int32_t*)nullptr,
+                           (const int32_t*)nullptr, (const float*)nullptr);
         hipLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(kBlock), 0, s, n, it, (const float*)pcur,
                            (const float*)Ap, (const float*)Pinv, n, 1, x, r, z, (const double*)part_pAp, nwg_r,
-                           (const double*)rz_hist, part_rz, part_max, st, (const double*)nullptr, 0);
+                           (const double*)rz_hist, part_rz, part_max, st);
       }
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));

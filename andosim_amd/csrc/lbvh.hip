@@ -823,70 +823,6 @@ __global__ void k_self_blocks(int entries, SelfView sv, float barrier_tol, int32
   for (int k = 0; k < 9; ++k) cval[9 * (size_t)t + k] = b[k];
 }
 
-// Ap_i += sum of row i's coupling blocks times p (rows = 3 * vertex-view
-// ranges), 8 lanes per row as in the mesh SpMV, plus the per-workgroup partial
-// of p_i . (that sum).  Skipped once the solve is done.
-__global__ void __launch_bounds__(256) k_self_spmv(int n, const int32_t* __restrict__ vptr,
-                                                   const int32_t* __restrict__ ccol, const float* __restrict__ cval,
-                                                   const PcgStatus* __restrict__ st, const float* __restrict__ p,
-                                                   float* __restrict__ Ap, double* __restrict__ part) {
-  if (st->done) return;
-  const int lane = threadIdx.x & 7;
-  const int i = blockIdx.x * 32 + (threadIdx.x >> 3);
-  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
-  int beg = 0, end = 0;
-  if (i < n) {
-    beg = 3 * vptr[i];
-    end = 3 * vptr[i + 1];
-  }
-  // Lane l takes blocks l, l + 8, l + 16, ... of the row; four of them per
-  // pass are requested before any is used (the sums keep that block order).
-  constexpr int kU = 4;
-  for (int b0 = beg + lane; b0 < end; b0 += 8 * kU) {
-    int j[kU];
-    float B[kU][9], pv[kU][3];
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      const int b = b0 + 8 * u;
-      if (b < end) {
-        j[u] = ccol[b];
-#pragma unroll
-        for (int e = 0; e < 9; ++e) B[u][e] = cval[9 * (size_t)b + e];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      if (b0 + 8 * u < end) {
-        pv[u][0] = p[3 * j[u]];
-        pv[u][1] = p[3 * j[u] + 1];
-        pv[u][2] = p[3 * j[u] + 2];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      if (b0 + 8 * u < end) {
-        s0 = s0 + (B[u][0] * pv[u][0] + (B[u][1] * pv[u][1] + B[u][2] * pv[u][2]));
-        s1 = s1 + (B[u][3] * pv[u][0] + (B[u][4] * pv[u][1] + B[u][5] * pv[u][2]));
-        s2 = s2 + (B[u][6] * pv[u][0] + (B[u][7] * pv[u][1] + B[u][8] * pv[u][2]));
-      }
-    }
-  }
-  for (int off = 4; off > 0; off >>= 1) {
-    s0 += __shfl_xor(s0, off, 8);
-    s1 += __shfl_xor(s1, off, 8);
-    s2 += __shfl_xor(s2, off, 8);
-  }
-  double pap = 0.0;
-  if (i < n && lane == 0 && end > beg) {
-    Ap[3 * i] += s0;
-    Ap[3 * i + 1] += s1;
-    Ap[3 * i + 2] += s2;
-    pap = (double)p[3 * i] * s0 + (double)p[3 * i + 1] * s1 + (double)p[3 * i + 2] * s2;
-  }
-  __shared__ double lds[4];
-  const double sblk = block_reduce(pap, lds, [](double u, double v) { return u + v; });
-  if (threadIdx.x == 0) part[blockIdx.x] = sblk;
-}
 
 // Segmented sort of a pass's hits by (query, key): two stable radix sorts of
 // positions, by key and then by query.  k_hit_keys: key of position j;

@@ -302,6 +302,38 @@ __device__ inline int xcd_logical_group(int w, int G) {
   return x * q + min(x, rem) + w / 8;
 }
 
+// Self-contact coupling blocks of a row (corrected modes; cptr null
+// otherwise): blocks [cb, ce) = [3 cptr[i], 3 cptr[i + 1]) with columns ccol
+// and 3x3 values cval, multiplied by p = z + beta p_old formed on the fly.
+// Virtual lane v (0..7) sums blocks cb + v, cb + v + 8, ... in order; a row's
+// L lanes share the virtual lanes (lane q takes v = q, q + L, ...) and the
+// eight sums meet as ((s0 + s4) + (s2 + s6)) + ((s1 + s5) + (s3 + s7)) in
+// every SpMV kernel, which then adds them to the row's mesh sum, so all of
+// them produce the same Ap bit for bit.
+__device__ __forceinline__ void couple_vlane(int v, int cb, int ce, const int32_t* __restrict__ ccol,
+                                             const float* __restrict__ cval, const float* __restrict__ z,
+                                             const float* __restrict__ p_old, float beta, int it, float* sv) {
+  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+  for (int b = cb + v; b < ce; b += 8) {
+    const int j = ccol[b];
+    const float* B = cval + 9 * (size_t)b;
+    float pv0, pv1, pv2;
+    if (it > 0) {
+      pv0 = z[3 * j] + beta * p_old[3 * j];
+      pv1 = z[3 * j + 1] + beta * p_old[3 * j + 1];
+      pv2 = z[3 * j + 2] + beta * p_old[3 * j + 2];
+    } else {
+      pv0 = p_old[3 * j]; pv1 = p_old[3 * j + 1]; pv2 = p_old[3 * j + 2];
+    }
+    s0 = s0 + (B[0] * pv0 + (B[1] * pv1 + B[2] * pv2));
+    s1 = s1 + (B[3] * pv0 + (B[4] * pv1 + B[5] * pv2));
+    s2 = s2 + (B[6] * pv0 + (B[7] * pv1 + B[8] * pv2));
+  }
+  sv[0] = s0;
+  sv[1] = s1;
+  sv[2] = s2;
+}
+
 // One block row per group of kLanesPerRow lanes, one 3x3 block per lane per
 // pass, so a row's blocks (contiguous in BSR order) are read as one run; the
 // three row sums are combined by an xor-shuffle tree inside the lane group.
@@ -318,7 +350,9 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int n, int it, const float*
                            const float* __restrict__ z, const float* __restrict__ p_old, float* __restrict__ p_new,
                            float* __restrict__ Ap, const double* __restrict__ part_rz,
                            const float* __restrict__ part_max, int nwg_v, double* __restrict__ rz_hist,
-                           double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol) {
+                           double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol,
+                           const int32_t* __restrict__ cptr, const int32_t* __restrict__ ccol,
+                           const float* __restrict__ cval) {
   constexpr int kRowsPerGroup = kRowsPerPass * R;
   __shared__ double ldsd[4];
   __shared__ float ldsf[4];
@@ -409,6 +443,19 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int n, int it, const float*
       s1 += __shfl_xor(s1, off, kLanesPerRow);
       s2 += __shfl_xor(s2, off, kLanesPerRow);
     }
+    if (cptr) {
+      const int cb = row[r] < n ? 3 * cptr[ri[r]] : 0, ce = row[r] < n ? 3 * cptr[ri[r] + 1] : 0;
+      float c3[3];
+      couple_vlane(lane, cb, ce, ccol, cval, z, p_old, beta, it, c3);
+      for (int off = kLanesPerRow / 2; off > 0; off >>= 1) {
+        c3[0] += __shfl_xor(c3[0], off, kLanesPerRow);
+        c3[1] += __shfl_xor(c3[1], off, kLanesPerRow);
+        c3[2] += __shfl_xor(c3[2], off, kLanesPerRow);
+      }
+      s0 = s0 + c3[0];
+      s1 = s1 + c3[1];
+      s2 = s2 + c3[2];
+    }
     const int i = row[r];
     if (i < n && lane == 0) {
       float q0, q1, q2;
@@ -476,7 +523,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_sell(int n, int slices, int
                            const float* __restrict__ p_old, float* __restrict__ p_new, float* __restrict__ Ap,
                            const double* __restrict__ part_rz, const float* __restrict__ part_max, int nwg_v,
                            double* __restrict__ rz_hist, double* __restrict__ part_pAp, PcgStatus* __restrict__ st,
-                           float tol) {
+                           float tol, const int32_t* __restrict__ cptr, const int32_t* __restrict__ ccol,
+                           const float* __restrict__ cval) {
   __shared__ double ldsd[4];
   __shared__ float ldsf[4];
   const int g = xcd_logical_group(blockIdx.x, gridDim.x);
@@ -599,6 +647,15 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_sell(int n, int slices, int
   float sr[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) sr[c] = ((t[0][c] + t[4][c]) + (t[2][c] + t[6][c])) + ((t[1][c] + t[5][c]) + (t[3][c] + t[7][c]));
+  if (cptr) {
+    const int cb = live && row < n ? 3 * cptr[row] : 0, ce = live && row < n ? 3 * cptr[row + 1] : 0;
+    float cs[8][3];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) couple_vlane(v, cb, ce, ccol, cval, z, p_old, beta, it, cs[v]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      sr[c] = sr[c] + (((cs[0][c] + cs[4][c]) + (cs[2][c] + cs[6][c])) + ((cs[1][c] + cs[5][c]) + (cs[3][c] + cs[7][c])));
+  }
   const float s0 = sr[0], s1 = sr[1], s2 = sr[2];
   double pap = 0.0;
   if (live && row < n) {
@@ -643,7 +700,9 @@ __device__ __forceinline__ void sell_split_body(int g, int n, int slices, int it
                            const float* __restrict__ z, const float* __restrict__ p_old, float* __restrict__ p_new,
                            float* __restrict__ Ap, const double* __restrict__ part_rz,
                            const float* __restrict__ part_max, int nwg_v, double* __restrict__ rz_hist,
-                           double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol) {
+                           double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol,
+                           const int32_t* __restrict__ cptr, const int32_t* __restrict__ ccol,
+                           const float* __restrict__ cval) {
   static_assert(L == 2 || L == 4, "two or four lanes per row");
   static_assert(S == 0 || S == 1 || S == 2 || S == 4, "a wave, or one, two or four slices per workgroup");
   constexpr int R = 64 / L;             // rows per wave
@@ -779,6 +838,27 @@ __device__ __forceinline__ void sell_split_body(int g, int n, int slices, int it
       sr[c] = h + __shfl_down(h, 16, 64);
     }
   }
+  if (cptr) {
+    // Lane group q holds the virtual lanes q, q + L, ...; the same tree as the
+    // mesh part above.
+    const int cb = live && row < n ? 3 * cptr[row] : 0, ce = live && row < n ? 3 * cptr[row + 1] : 0;
+    float cs[8 / L][3];
+#pragma unroll
+    for (int m = 0; m < 8 / L; ++m) couple_vlane(q + L * m, cb, ce, ccol, cval, z, p_old, beta, it, cs[m]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      float h, tot;
+      if constexpr (L == 2) {
+        h = (cs[0][c] + cs[2][c]) + (cs[1][c] + cs[3][c]);
+        tot = h + __shfl_down(h, 32, 64);
+      } else {
+        h = cs[0][c] + cs[1][c];
+        h = h + __shfl_down(h, 32, 64);
+        tot = h + __shfl_down(h, 16, 64);
+      }
+      sr[c] = sr[c] + tot;
+    }
+  }
   double pap = 0.0;
   if (q == 0 && live && row < n) {
     float q0, q1, q2;
@@ -814,8 +894,12 @@ __global__ void __launch_bounds__(64 * (S == 0 ? 1 : S * L)) k_pcg_spmv_sell_spl
                            const float* __restrict__ z, const float* __restrict__ p_old, float* __restrict__ p_new,
                            float* __restrict__ Ap, const double* __restrict__ part_rz,
                            const float* __restrict__ part_max, int nwg_v, double* __restrict__ rz_hist,
-                           double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol) {
-  sell_split_body<L, S>(xcd_logical_group(blockIdx.x, gridDim.x), n, slices, it, vals, scol, slice_off, slice_w, uniform_w, z, p_old, p_new, Ap, part_rz, part_max, nwg_v, rz_hist, part_pAp, st, tol);
+                           double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol,
+                           const int32_t* __restrict__ cptr, const int32_t* __restrict__ ccol,
+                           const float* __restrict__ cval) {
+  sell_split_body<L, S>(xcd_logical_group(blockIdx.x, gridDim.x), n, slices, it, vals, scol, slice_off, slice_w,
+                        uniform_w, z, p_old, p_new, Ap, part_rz, part_max, nwg_v, rz_hist, part_pAp, st, tol, cptr,
+                        ccol, cval);
 }
 
 // alpha = r.z / p.Ap, x += alpha p, r -= alpha Ap, z = P^-1 r; one thread per
@@ -824,8 +908,7 @@ __device__ __forceinline__ void update_body(int wg, int n, int it, const float* 
                              const float* __restrict__ Ap, const float* __restrict__ Pinv, int pe,
                              int pi, float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
                              const double* __restrict__ part_pAp, int nwg_r, const double* __restrict__ rz_hist,
-                             double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st,
-                             const double* __restrict__ part_pAp2, int n2) {
+                             double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st) {
   __shared__ double ldsd[4];
   __shared__ float ldsf[4];
   const int i = wg * blockDim.x + threadIdx.x;
@@ -849,10 +932,8 @@ __device__ __forceinline__ void update_body(int wg, int n, int it, const float* 
   for (int e = 0; e < 9; ++e) P[e] = Pinv[(size_t)e * pe + (size_t)ii * pi];
   if (st->done) return;
   const double pAp_t = finish_sum_partials(part_pAp, nwg_r, pre);
-  const double pAp2_t = part_pAp2 ? thread_sum_partials(part_pAp2, n2) : 0.0;
   const double rz_it = rz_hist[it];
-  double pAp = block_reduce(pAp_t, ldsd, [](double a, double b) { return a + b; });
-  if (part_pAp2) pAp += block_reduce(pAp2_t, ldsd, [](double a, double b) { return a + b; });
+  const double pAp = block_reduce(pAp_t, ldsd, [](double a, double b) { return a + b; });
   if (fabs(pAp) < 1e-16) {
     if (threadIdx.x == 0) {
       st->failed = 1;
@@ -893,9 +974,8 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const floa
                              const float* __restrict__ Ap, const float* __restrict__ Pinv, int pe,
                              int pi, float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
                              const double* __restrict__ part_pAp, int nwg_r, const double* __restrict__ rz_hist,
-                             double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st,
-                             const double* __restrict__ part_pAp2, int n2) {
-  update_body(blockIdx.x, n, it, p, Ap, Pinv, pe, pi, x, r, z, part_pAp, nwg_r, rz_hist, part_rz, part_max, st, part_pAp2, n2);
+                             double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st) {
+  update_body(blockIdx.x, n, it, p, Ap, Pinv, pe, pi, x, r, z, part_pAp, nwg_r, rz_hist, part_rz, part_max, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -941,7 +1021,7 @@ __global__ void __launch_bounds__(128) k_pcg_spmv_sell_batch(const PcgSys* __res
   float* pcur = (it & 1) ? S.p1 : S.p0;
   sell_split_body<2, 1>((int)blockIdx.x - S.wg0_r, S.n, S.slices, it, S.vals, S.scol, S.slice_off, S.slice_w,
                         S.uniform_w, S.z, pold, pcur, S.Ap, S.part_rz, S.part_max, S.nwg_v, S.rz_hist, S.part_pAp,
-                        S.st, S.tol);
+                        S.st, S.tol, nullptr, nullptr, nullptr);
 }
 
 __global__ void __launch_bounds__(kBlock) k_pcg_update_batch(const PcgSys* __restrict__ sys,
@@ -949,7 +1029,7 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update_batch(const PcgSys* __res
   const PcgSys& S = sys[wg_sys[blockIdx.x]];
   float* pcur = (it & 1) ? S.p1 : S.p0;
   update_body((int)blockIdx.x - S.wg0_v, S.n, it, pcur, S.Ap, S.Pinv, S.pe, S.pi, S.x, S.r, S.z,
-              S.part_pAp, S.nwg_r, S.rz_hist, S.part_rz, S.part_max, S.st, nullptr, 0);
+              S.part_pAp, S.nwg_r, S.rz_hist, S.part_rz, S.part_max, S.st);
 }
 
 // Set-up of every system of a batch: sliced-ELL values, then x0 = 0, r = b,

@@ -254,8 +254,8 @@ __global__ void __launch_bounds__(kBlock) k_rigid_cand(int n, int pass, const fl
 // largest coordinate (an upper bound 2^-23 m on ulp(m)): the accepted motion
 // never brings it within xi, and the update x + alpha ed, rounded to FP32
 // (alpha a power of two, so alpha ed is exact), moves each vertex by at most
-// half an ulp, far less than xi.  A pair already within xi keeps half its gap
-// as the margin.
+// half an ulp, far less than xi.  A pair that rounding has put within xi may
+// only move apart (band_ok); otherwise its vertices are held.
 // ---------------------------------------------------------------------------
 struct d3 {
   double x, y, z;
@@ -329,27 +329,34 @@ __device__ inline double pair_dist_d(int type, const d3* q) {
 
 constexpr double kSepUlps = 32.0 * 1.1920928955078125e-07;  // 32 ulps per unit of coordinate magnitude
 
+// The separation a pair keeps: 32 * 2^-23 * its largest coordinate over the
+// motion x0 -> x0 + p.
+__device__ inline double sep_xi(const d3* x0, const d3* p) {
+  double m = 0.0;
+  for (int k = 0; k < 4; ++k) {
+    const d3 e = dadd(x0[k], p[k]);
+    m = fmax(m, fmax(fmax(fabs(x0[k].x), fabs(x0[k].y)), fmax(fabs(x0[k].z), fmax(fabs(e.x), fmax(fabs(e.y), fabs(e.z))))));
+  }
+  return kSepUlps * m;
+}
+
 // Additive CCD (Li, Kaufman, Jiang et al., "Codimensional Incremental
 // Potential Contact", 2021, Algorithm 1) in FP64 with thickness xi, s = 0.1,
 // t_c = 1, at most 256 advances: 1 when the whole motion x0 -> x0 + p keeps
-// the pair farther than xi, else a time up to which it provably does.
-__device__ inline double accd_sep(int type, const d3* x0, const d3* p) {
+// the pair farther than xi, else a time up to which it provably does (0 when
+// it starts within xi).
+__device__ inline double accd_sep(int type, const d3* x0, const d3* p, double xi) {
   const d3 pbar = dscale(0.25, dadd(dadd(p[0], p[1]), dadd(p[2], p[3])));
   d3 q[4], x[4];
-  double m = 0.0;
   for (int k = 0; k < 4; ++k) {
     q[k] = dsub(p[k], pbar);
     x[k] = x0[k];
-    const d3 e = dadd(x0[k], p[k]);
-    m = fmax(m, fmax(fmax(fabs(x0[k].x), fabs(x0[k].y)), fmax(fabs(x0[k].z), fmax(fabs(e.x), fmax(fabs(e.y), fabs(e.z))))));
   }
   const double lp = type == 0 ? dnorm(q[0]) + fmax(dnorm(q[1]), fmax(dnorm(q[2]), dnorm(q[3])))
                               : fmax(dnorm(q[0]), dnorm(q[1])) + fmax(dnorm(q[2]), dnorm(q[3]));
   if (lp == 0.0) return 1.0;
   double d = pair_dist_d(type, x);
-  if (!(d > 0.0)) return 0.0;
-  double xi = kSepUlps * m;
-  if (d <= xi) xi = 0.5 * d;
+  if (!(d > xi)) return 0.0;
   const double g = 0.1 * (d - xi);
   double t = 0.0, tl = 0.9 * (d - xi) / lp;
   for (int it = 0; it < 256; ++it) {
@@ -361,6 +368,77 @@ __device__ inline double accd_sep(int type, const d3* x0, const d3* p) {
     tl = 0.9 * (d - xi) / lp;
   }
   return t;
+}
+
+// A pair already within its separation xi (rounding of the positions can put
+// it there) may only move apart: the whole motion keeps it farther than half
+// its gap and it ends no closer.  Otherwise its cloth vertices are held (their
+// direction is zeroed for this iteration): a pair pushed together iteration
+// after iteration would otherwise close geometrically until rounding decides
+// its side, and with one alpha for the whole cloth a blocking test would stop
+// every vertex instead of these.
+__device__ inline bool band_ok(int type, const d3* x0, const d3* p, double d0) {
+  d3 x1[4];
+  for (int k = 0; k < 4; ++k) x1[k] = dadd(x0[k], p[k]);
+  return pair_dist_d(type, x1) >= d0 && accd_sep(type, x0, p, 0.5 * d0) >= 1.0;
+}
+
+// One hold pass: band pairs that fail band_ok under the motion with the
+// vertices held in fin set fout for their cloth vertices (fout starts as a
+// copy of fin) and *changed.  pass_all: hold every band pair's vertices.
+__global__ void k_band_self(int nc, const SelfContact* __restrict__ sc, const float* __restrict__ x,
+                            const float* __restrict__ ed, const uint8_t* __restrict__ fin, uint8_t* __restrict__ fout,
+                            uint32_t* __restrict__ changed, int pass_all) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const SelfContact& C = sc[c];
+  d3 x0[4], p[4];
+  for (int k = 0; k < 4; ++k) {
+    x0[k] = dld(x, C.idx[k]);
+    p[k] = fin[C.idx[k]] ? dmk(0.0, 0.0, 0.0) : dld(ed, C.idx[k]);
+  }
+  const double d0 = pair_dist_d(C.type, x0);
+  if (!(d0 < sep_xi(x0, p))) return;
+  if (!pass_all && band_ok(C.type, x0, p, d0)) return;
+  for (int k = 0; k < 4; ++k)
+    if (!fin[C.idx[k]]) {
+      fout[C.idx[k]] = 1;
+      atomicOr(changed, 1u);
+    }
+}
+
+__global__ void k_band_rigid(int total, const RigidContact* __restrict__ rc, DetectBodies D,
+                             const float* __restrict__ x, const float* __restrict__ ed,
+                             const uint8_t* __restrict__ fin, uint8_t* __restrict__ fout,
+                             uint32_t* __restrict__ changed, int pass_all) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const RigidContact& r = rc[e];
+  const DetectBody& B = D.b[r.body];
+  d3 x0[4], p[4];
+  x0[0] = dld(x, r.vertex);
+  p[0] = fin[r.vertex] ? dmk(0.0, 0.0, 0.0) : dld(ed, r.vertex);
+  for (int k = 1; k < 4; ++k) {
+    x0[k] = dld(B.world, B.tris[3 * r.tri + k - 1]);
+    p[k] = dmk(0.0, 0.0, 0.0);
+  }
+  const double d0 = pair_dist_d(0, x0);
+  if (!(d0 < sep_xi(x0, p))) return;
+  if (!pass_all && band_ok(0, x0, p, d0)) return;
+  if (!fin[r.vertex]) {
+    fout[r.vertex] = 1;
+    atomicOr(changed, 1u);
+  }
+}
+
+// Held vertices: d = ed = 0.
+__global__ void k_apply_hold(int n, const uint8_t* __restrict__ hold, float* __restrict__ d, float* __restrict__ ed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !hold[i]) return;
+  for (int k = 0; k < 3; ++k) {
+    d[3 * i + k] = 0.0f;
+    ed[3 * i + k] = 0.0f;
+  }
 }
 
 // Line search: bit t of a contact's mask is set when alpha_t = 2^-t is at most
@@ -412,7 +490,9 @@ __global__ void k_ls_toi_self(int nc, int t0, const SelfContact* __restrict__ sc
       x0[k] = dld(x, C.idx[k]);
       p[k] = dld(ed, C.idx[k]);
     }
-    m = toi_bits(accd_sep(C.type, x0, p), t0);
+    const double xi = sep_xi(x0, p);
+    // A pair within its separation passed band_ok or has its vertices held.
+    m = pair_dist_d(C.type, x0) < xi ? toi_bits(1.0, t0) : toi_bits(accd_sep(C.type, x0, p, xi), t0);
   }
   __shared__ uint32_t lds[4];
   const uint32_t r = block_reduce(m, lds, [](uint32_t a, uint32_t b) { return a & b; });
@@ -433,7 +513,8 @@ __global__ void k_ls_toi_rigid(int total, int t0, const RigidContact* __restrict
       x0[k] = dld(B.world, B.tris[3 * r.tri + k - 1]);
       p[k] = dmk(0.0, 0.0, 0.0);
     }
-    m = toi_bits(accd_sep(0, x0, p), t0);
+    const double xi = sep_xi(x0, p);
+    m = pair_dist_d(0, x0) < xi ? toi_bits(1.0, t0) : toi_bits(accd_sep(0, x0, p, xi), t0);
   }
   __shared__ uint32_t lds[4];
   const uint32_t rr = block_reduce(m, lds, [](uint32_t a, uint32_t b) { return a & b; });

@@ -91,7 +91,8 @@ class Stats(C.Structure):
                 ("pcg_kernel", C.c_int32), ("learned_solves", C.c_int32), ("learned_fallbacks", C.c_int32),
                 ("fallback_iterations", C.c_int32), ("spmv_lanes", C.c_int32),
                 ("spmv_wg_slices", C.c_int32), ("batched_solves", C.c_int32), ("pcg_vec_block", C.c_int32),
-                ("pcg_autotuned", C.c_int32), ("long_hit_sorts", C.c_int32), ("max_candidates", C.c_int32)]
+                ("pcg_autotuned", C.c_int32), ("long_hit_sorts", C.c_int32), ("max_candidates", C.c_int32),
+                ("held_iterations", C.c_int32)]
 
 
 _lib = None

@@ -162,6 +162,8 @@ typedef struct {
   int32_t long_hit_sorts;           /* detection passes in which a query had more than 64 hits,
                                        sorted by a segmented radix sort instead of by the query */
   int32_t max_candidates;           /* largest candidate list of one detection pass (PT, EE or rigid) */
+  int32_t held_iterations;          /* corrected physics: Newton iterations that held the vertices of
+                                       pairs within their separation that would not move apart */
 } ando_stats;
 
 int32_t ando_abi_version(void);

@@ -309,11 +309,6 @@ static double pair_dist_d(int type, const D3* q) {
   return type == 0 ? pt_tri_dist(q[0], q[1], q[2], q[3]) : seg_seg_dist(q[0], q[1], q[2], q[3]);
 }
 
-// Additive CCD (Li, Kaufman, Jiang et al. 2021, Algorithm 1) in FP64 with
-// thickness xi = 32 * 2^-23 * (largest coordinate of the pair over the motion),
-// half the initial gap when that is smaller; s = 0.1, t_c = 1, at most 256
-// advances.  1 when the whole motion x0 -> x0 + p keeps the pair farther than
-// xi, else a time up to which it provably does.
 // Corrected physics: the first line-search trial, the smallest t0 (at most 19)
 // with 1.25 * 2^-t0 * max |d_i| <= L = max(4 * mean rest edge, 2 * max
 // |x_hat_i - x_i|) (squared maxima given, FP64): the membrane Hessian has no
@@ -327,25 +322,35 @@ static int cp_first_trial(double mean_edge, double md2, double mh2) {
   return t0;
 }
 
-static double accd_sep(int type, const D3* x0, const D3* p) {
-  const D3 pbar = dscale(0.25, dadd(dadd(p[0], p[1]), dadd(p[2], p[3])));
-  D3 q[4], x[4];
+// The separation a pair keeps: 32 * 2^-23 * its largest coordinate over the
+// motion x0 -> x0 + p.
+static double sep_xi(const D3* x0, const D3* p) {
   double m = 0.0;
   for (int k = 0; k < 4; ++k) {
-    q[k] = dsub(p[k], pbar);
-    x[k] = x0[k];
     const D3 e = dadd(x0[k], p[k]);
     m = std::fmax(m, std::fmax(std::fmax(std::fabs(x0[k].x), std::fabs(x0[k].y)),
                                std::fmax(std::fabs(x0[k].z),
                                          std::fmax(std::fabs(e.x), std::fmax(std::fabs(e.y), std::fabs(e.z))))));
   }
+  return 32.0 * 1.1920928955078125e-07 * m;
+}
+
+// Additive CCD (Li, Kaufman, Jiang et al. 2021, Algorithm 1) in FP64 with
+// thickness xi, s = 0.1, t_c = 1, at most 256 advances: 1 when the whole
+// motion x0 -> x0 + p keeps the pair farther than xi, else a time up to which
+// it provably does (0 when it starts within xi).
+static double accd_sep(int type, const D3* x0, const D3* p, double xi) {
+  const D3 pbar = dscale(0.25, dadd(dadd(p[0], p[1]), dadd(p[2], p[3])));
+  D3 q[4], x[4];
+  for (int k = 0; k < 4; ++k) {
+    q[k] = dsub(p[k], pbar);
+    x[k] = x0[k];
+  }
   const double lp = type == 0 ? dnorm(q[0]) + std::fmax(dnorm(q[1]), std::fmax(dnorm(q[2]), dnorm(q[3])))
                               : std::fmax(dnorm(q[0]), dnorm(q[1])) + std::fmax(dnorm(q[2]), dnorm(q[3]));
   if (lp == 0.0) return 1.0;
   double d = pair_dist_d(type, x);
-  if (!(d > 0.0)) return 0.0;
-  double xi = 32.0 * 1.1920928955078125e-07 * m;
-  if (d <= xi) xi = 0.5 * d;
+  if (!(d > xi)) return 0.0;
   const double g = 0.1 * (d - xi);
   double t = 0.0, tl = 0.9 * (d - xi) / lp;
   for (int it = 0; it < 256; ++it) {
@@ -357,6 +362,16 @@ static double accd_sep(int type, const D3* x0, const D3* p) {
     tl = 0.9 * (d - xi) / lp;
   }
   return t;
+}
+
+// A pair already within its separation (rounding of the positions can put it
+// there) may only move apart: the whole motion keeps it farther than half its
+// gap and it ends no closer; otherwise the line search holds its cloth
+// vertices for the iteration.
+static bool band_ok(int type, const D3* x0, const D3* p, double d0) {
+  D3 x1[4];
+  for (int k = 0; k < 4; ++k) x1[k] = dadd(x0[k], p[k]);
+  return pair_dist_d(type, x1) >= d0 && accd_sep(type, x0, p, 0.5 * d0) >= 1.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -2006,7 +2021,29 @@ template <class R> struct Sim {
   // the trial motion x -> x + 1.25 d replace `contacts`.
   // The first trial t0 (cp_first_trial) from the longest |d_i| and the
   // longest remaining inertial displacement |xt_i - x_i|, both in FP64.
-  R line_search_cp(const std::vector<R>& d, const V3<R>& wn, R woff, const std::vector<R>& xt) {
+  // The endpoints of a candidate over the motion ed with the vertices in
+  // `hold` not moving (rigid triangles stay put); returns the pair type.
+  int pair_motion(const Contact<R>& c, const std::vector<R>& ed, const std::vector<uint8_t>& hold, D3* x0,
+                  D3* p) const {
+    auto d3of = [](const V3<R>& v) { return D3{(double)v[0], (double)v[1], (double)v[2]}; };
+    auto mot = [&](int v) { return hold[v] ? D3{0.0, 0.0, 0.0} : d3of(Xof(ed, v)); };
+    if (c.type == RIGID_PT) {
+      const auto& W = rworld[c.rigid];
+      x0[0] = d3of(X(c.idx[0]));
+      p[0] = mot(c.idx[0]);
+      for (int j = 1; j < 4; ++j) {
+        x0[j] = d3of(W[c.idx[j]]);
+        p[j] = D3{0.0, 0.0, 0.0};
+      }
+      return 0;
+    }
+    for (int j = 0; j < 4; ++j) {
+      x0[j] = d3of(X(c.idx[j]));
+      p[j] = mot(c.idx[j]);
+    }
+    return c.type == EE ? 1 : 0;
+  }
+  R line_search_cp(std::vector<R>& d, const V3<R>& wn, R woff, const std::vector<R>& xt) {
     double md2 = 0.0, mh2 = 0.0;
     for (int i = 0; i < n; ++i) {
       const double d0 = d[3 * i], d1 = d[3 * i + 1], d2 = d[3 * i + 2];
@@ -2020,27 +2057,48 @@ template <class R> struct Sim {
     std::vector<R> ed(3 * n);
     for (int i = 0; i < 3 * n; ++i) ed[i] = c0 * d[i];
     detect_cp(&ed, contacts);
+    // Pairs within their separation that would not move apart hold their
+    // cloth vertices: up to four passes (each reads the holds of the last),
+    // then every such pair's vertices.
+    std::vector<uint8_t> hold(n, 0), next;
+    bool any = false;
+    for (int pass = 0; pass < 5; ++pass) {
+      const bool all = pass == 4;
+      next = hold;
+      bool changed = false;
+      for (const auto& c : contacts) {
+        D3 x0[4], p[4];
+        const int type = pair_motion(c, ed, hold, x0, p);
+        const double d0 = pair_dist_d(type, x0);
+        if (!(d0 < sep_xi(x0, p))) continue;
+        if (!all && band_ok(type, x0, p, d0)) continue;
+        const int nv = c.type == RIGID_PT ? 1 : 4;
+        for (int j = 0; j < nv; ++j)
+          if (!hold[c.idx[j]]) {
+            next[c.idx[j]] = 1;
+            changed = true;
+          }
+      }
+      hold.swap(next);
+      if (!changed) break;
+      any = true;
+    }
+    if (any)
+      for (int i = 0; i < n; ++i)
+        if (hold[i])
+          for (int k = 0; k < 3; ++k) {
+            d[3 * i + k] = R(0);
+            ed[3 * i + k] = R(0);
+          }
+    std::fill(hold.begin(), hold.end(), 0);
     double toi = 1.0;
-    auto d3of = [](const V3<R>& v) { return D3{(double)v[0], (double)v[1], (double)v[2]}; };
 #pragma omp parallel for schedule(dynamic, 256) reduction(min : toi)
     for (size_t k = 0; k < contacts.size(); ++k) {
-      const Contact<R>& c = contacts[k];
       D3 x0[4], p[4];
-      if (c.type == RIGID_PT) {
-        const auto& W = rworld[c.rigid];
-        x0[0] = d3of(X(c.idx[0]));
-        p[0] = d3of(Xof(ed, c.idx[0]));
-        for (int j = 1; j < 4; ++j) {
-          x0[j] = d3of(W[c.idx[j]]);
-          p[j] = D3{0.0, 0.0, 0.0};
-        }
-      } else {
-        for (int j = 0; j < 4; ++j) {
-          x0[j] = d3of(X(c.idx[j]));
-          p[j] = d3of(Xof(ed, c.idx[j]));
-        }
-      }
-      const double t = accd_sep(c.type == EE ? 1 : 0, x0, p);
+      const int type = pair_motion(contacts[k], ed, hold, x0, p);
+      const double xi = sep_xi(x0, p);
+      // A pair within its separation passed band_ok or has its vertices held.
+      const double t = pair_dist_d(type, x0) < xi ? 1.0 : accd_sep(type, x0, p, xi);
       toi = std::min(toi, t);
     }
     const bool wall = (double)sqn3(wn) > 0.5;
@@ -2533,7 +2591,7 @@ double orc_accd_sep(int type, const double* x0, const double* p) {
     xs[k] = D3{x0[3 * k], x0[3 * k + 1], x0[3 * k + 2]};
     ps[k] = D3{p[3 * k], p[3 * k + 1], p[3 * k + 2]};
   }
-  return accd_sep(type, xs, ps);
+  return accd_sep(type, xs, ps, sep_xi(xs, ps));
 }
 double orc_additive_ccd(int prec, int type, const double* x0, const double* p) {
   double r = 0.0;
@@ -2901,8 +2959,9 @@ int orc_svd32(int prec, const double* F6, double* U6, double* s2, double* V4) {
   if (prec == 64) run(double()); else run(float());
   return ok;
 }
-// Line search on the current state with a given direction; returns alpha.
-double orc_line_search(void* h, const double* d, double ext) {
+// Line search on the current state with a given direction; returns alpha and
+// (d_out, nullable) the direction after the corrected physics' holds.
+double orc_line_search(void* h, const double* d, double ext, double* d_out) {
   OSim* s = (OSim*)h;
   double r = 0;
   DISPATCH(s, {
@@ -2915,6 +2974,8 @@ double orc_line_search(void* h, const double* d, double ext) {
     // Corrected physics: its own rule over the trial motion x -> x + 1.25 d
     // (ext is not used), with the candidates of that motion.
     r = S.cp ? S.line_search_cp(dd, wn, woff, S.x) : S.line_search(dd, wn, woff, (R)ext);
+    if (d_out)
+      for (int i = 0; i < 3 * S.n; ++i) d_out[i] = (double)dd[i];
   });
   return r;
 }

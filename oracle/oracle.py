@@ -73,7 +73,7 @@ def lib():
         L.orc_pin_stiffness.argtypes = [C.c_int, C.c_double, _dp, _dp, C.c_double]
         L.orc_svd32.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
         L.orc_line_search.restype = C.c_double
-        L.orc_line_search.argtypes = [C.c_void_p, _dp, C.c_double]
+        L.orc_line_search.argtypes = [C.c_void_p, _dp, C.c_double, C.c_void_p]
         L.orc_set_contacts_pt.argtypes = [C.c_void_p, C.c_int, _ip]
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_contact_mode.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double]
@@ -312,7 +312,14 @@ class OracleSim:
         return buf[:17 * cnt].reshape(cnt, 17)
 
     def line_search(self, d, ext=1.25) -> float:
-        return self.L.orc_line_search(self.h, _d(d).ravel(), ext)
+        return self.L.orc_line_search(self.h, _d(d).ravel(), ext, None)
+
+    def line_search_held(self, d):
+        """Corrected physics: (alpha, the direction after the line search held
+        the vertices of pairs within their separation that would not move apart)."""
+        out = np.zeros(3 * self.n)
+        a = self.L.orc_line_search(self.h, _d(d).ravel(), 1.25, out.ctypes.data_as(C.c_void_p))
+        return a, out.reshape(-1, 3)
 
     def set_pt_contacts(self, idx4):
         idx = np.ascontiguousarray(np.asarray(idx4, dtype=np.int32).reshape(-1, 4))

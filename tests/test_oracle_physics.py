@@ -264,28 +264,33 @@ def test_separation_ccd_head_on(h, D):
     assert end <= xi + 0.1 * (h - xi) + 0.9 * 0.1 * (h - xi) * 1.5
 
 
-def test_separation_ccd_resolves_fp32_coordinates():
-    """A point 5.6 FP32 ulps above a triangle at z = 0.2 moving 4 mm through
-    it: the accepted motion, rounded to FP32 as the step rounds it, keeps the
-    point strictly above the plane (a thickness-0 FP32 CCD let exactly this
-    pair through on the 129 x 129 drape)."""
+def test_separation_band_holds_approaching_vertex():
+    """A vertex 5.6 FP32 ulps above a plate at z = 0.2 (inside its 32-ulp
+    separation, where rounding of the positions would decide its side):
+    moving 4 mm down it is held (its direction zeroed) while the rest of the
+    cloth moves with alpha = 1; moving up it is free.  The separation CCD
+    itself reports 0 for such a pair."""
     z = np.float32(0.2)
     pz = np.float32(z + np.float32(5.6 * 1.4901161e-08))
-    tri = np.array([[0.1, 0.1, z], [0.3, 0.1, z], [0.1, 0.3, z]], np.float32)
-    x0 = np.vstack([[0.15, 0.15, pz], tri]).astype(np.float64)
-    ed = np.zeros((4, 3), np.float32)
-    ed[0, 2] = np.float32(-4e-3)
-    t = accd_sep("pt", x0, ed.astype(np.float64))
-    assert 0.0 < t < 1.0
-    alpha = 1.0
-    while alpha > t:
-        alpha *= 0.5
-    zn = np.float32(pz + np.float32(alpha) * ed[0, 2])
-    assert zn > z
-    # The same pair far from contact moves freely.
-    x1 = x0.copy()
-    x1[0, 2] = 0.2 + 0.01
-    assert accd_sep("pt", x1, ed.astype(np.float64)) == 1.0
+    V, T = scenes.grid(3, 0.1, 0.3, center=(2.0, 2.0))
+    V = np.concatenate([V, [[0.15, 0.15, pz]]]).astype(np.float32)
+    sc = scenes.Scene("free", V, T, dict(scenes.BASE_MATERIAL), scenes.base_params(), [], [],
+                      [plate(float(z))])
+    o = physics(sc)
+    d = np.zeros((len(V), 3))
+    d[-1, 2] = -4e-3 / 1.25
+    d[0, 0] = 1e-3
+    a, held = o.line_search_held(d)
+    assert a == 1.0
+    assert np.all(held[-1] == 0.0)
+    assert held[0, 0] == 1e-3
+    a, held = o.line_search_held(-d)
+    assert a == 1.0 and held[-1, 2] == 4e-3 / 1.25
+    tri = np.array([[-0.5, -0.5, z], [0.5, -0.5, z], [0.5, 0.5, z]], np.float64)
+    x0 = np.vstack([[0.15, -0.15, pz], tri])
+    p = np.zeros((4, 3))
+    p[0, 2] = -4e-3
+    assert accd_sep("pt", x0, p) == 0.0
 
 
 def test_separation_ccd_edge_edge_crossing():
