@@ -311,6 +311,9 @@ struct ando_ctx {
   DBuf<int32_t> sv_vals, sv_vals2, sv_counts, sv_offsets, sv_sums, sv_total, sv_hist, sv_ptr;
   DBuf<int32_t> ccol;
   DBuf<float> cval;
+  DBuf<int32_t> crow, crow_flag, crow_pos;  // rows with coupling blocks (ncrow of them)
+  int ncrow = 0;
+  DBuf<float> Ap2;                          // their coupling products during a solve
   DBuf<int32_t> pt_of;     // faithful one-triangle mesh: PT contact of each point vertex, or -1
   bool sv_ready = false;
   float barrier_tol = 1e-12f;
@@ -968,9 +971,19 @@ void prepare_self(ando_ctx* c, const StepParams& P) {
   c->sv_ready = true;
   c->ccol.ensure(3 * (size_t)entries + 1);
   c->cval.ensure(27 * (size_t)entries + 1);
-  if (entries)
+  c->ncrow = 0;
+  if (entries) {
     hipLaunchKernelGGL(k_self_blocks, dim3(nblk(3LL * entries)), dim3(kBlock), 0, s, entries, self_view(c),
                        c->barrier_tol, c->ccol.p, c->cval.p);
+    c->crow_flag.ensure((size_t)c->n + 1);
+    c->crow_pos.ensure((size_t)c->n + 2);
+    hipLaunchKernelGGL(k_couple_row_flags, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, (const int32_t*)c->sv_ptr.p,
+                       c->crow_flag.p);
+    c->ncrow = exclusive_scan(c, c->n, c->crow_flag.p, c->crow_pos.p, c->sv_sums, c->sv_total);
+    c->crow.ensure((size_t)c->ncrow + 1);
+    hipLaunchKernelGGL(k_couple_row_fill, dim3(nblk(c->n)), dim3(kBlock), 0, s, c->n, (const int32_t*)c->sv_ptr.p,
+                       (const int32_t*)c->crow_pos.p, c->crow.p);
+  }
   HIPCHK(hipGetLastError());
 }
 
@@ -1296,10 +1309,25 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
   // sizing the first chunk from the previous step's solve.
   int chunk = std::max(1, c->pcg_chunk);
   const bool timing = c->kernel_timing != 0;
-  // Coupling blocks of the self contacts, multiplied inside the SpMV.
-  const int32_t* cptr = couple ? sv->ptr : nullptr;
-  const int32_t* ccol = couple ? (const int32_t*)c->ccol.p : nullptr;
-  const float* cval = couple ? (const float*)c->cval.p : nullptr;
+  // Coupling blocks of the self contacts: workgroups appended to the SpMV
+  // grid (8 lanes per row with blocks) write them to Ap2; their partials
+  // follow the mesh workgroups'.
+  const int spmv_threads = sell ? (c->sell_lanes == 2 && sell_slices == 0   ? 64
+                                   : c->sell_lanes == 2 && sell_slices == 1 ? 128
+                                   : c->sell_lanes == 2 && sell_slices == 2 ? 256
+                                   : c->sell_lanes == 2                      ? 2 * kBlock
+                                   : c->sell_lanes == 4                      ? 4 * kBlock
+                                                                             : kBlock)
+                                : kBlock;
+  ando::Coupling cpl{0, nwg_r, 0, nullptr, nullptr, nullptr, nullptr, nullptr};
+  int nwg_c = 0;
+  if (couple && c->ncrow > 0) {
+    c->Ap2.ensure(3 * (size_t)n);
+    HIPCHK(hipMemsetAsync(c->Ap2.p, 0, 3 * sizeof(float) * (size_t)n, s));
+    nwg_c = nblk(c->ncrow, spmv_threads / 8);
+    cpl = ando::Coupling{c->ncrow, nwg_r, nwg_c, c->crow.p, sv->ptr, c->ccol.p, c->cval.p, c->Ap2.p};
+    c->part_pAp.ensure((size_t)nwg_r + nwg_c + 1);
+  }
   size_t ev_used = 0;
   // Timed launches record their start and end with the dispatch itself
   // (hipExtLaunchKernelGGL), so the events add no packets between kernels.
@@ -1322,19 +1350,19 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
         ev_used += 4;
       }
       auto spmv = [&](auto kernel) {
-        hipExtLaunchKernelGGL(kernel, dim3(nwg_r), dim3(kBlock), 0, s, e0, e1, 0, n, it, A, rp, cols,
+        hipExtLaunchKernelGGL(kernel, dim3(nwg_r + nwg_c), dim3(kBlock), 0, s, e0, e1, 0, n, it, A, rp, cols,
                               (const float*)c->z.p, (const float*)pold, pcur, c->Ap.p, (const double*)c->part_rz.p,
                               (const float*)c->part_max.p, nwg_v, c->rz_hist.p, c->part_pAp.p, c->pcg_status.p, tol,
-                              cptr, ccol, cval);
+                              cpl);
       };
       if (sell) {
         auto sell_launch = [&](auto kernel, int threads) {
-          hipExtLaunchKernelGGL(kernel, dim3(nwg_r), dim3(threads), 0, s, e0, e1, 0, n, sm->slices, it,
+          hipExtLaunchKernelGGL(kernel, dim3(nwg_r + nwg_c), dim3(threads), 0, s, e0, e1, 0, n, sm->slices, it,
                                 (const float*)sm->sell_vals.p, (const int32_t*)sm->sell_col.p,
                                 (const int32_t*)sm->slice_off.p, (const int32_t*)sm->slice_w.p, sm->uniform_w,
                                 (const float*)c->z.p, (const float*)pold, pcur, c->Ap.p,
                                 (const double*)c->part_rz.p, (const float*)c->part_max.p, nwg_v, c->rz_hist.p,
-                                c->part_pAp.p, c->pcg_status.p, tol, cptr, ccol, cval);
+                                c->part_pAp.p, c->pcg_status.p, tol, cpl);
         };
         if (c->sell_lanes == 2 && sell_slices == 0) sell_launch(k_pcg_spmv_sell_split<2, 0>, 64);
         else if (c->sell_lanes == 2 && sell_slices == 1) sell_launch(k_pcg_spmv_sell_split<2, 1>, 128);
@@ -1351,8 +1379,8 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
         }
       }
       hipExtLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(vb), 0, s, e2, e3, 0, n, it, (const float*)pcur,
-                            (const float*)c->Ap.p, Pc, pe, pi, xsol, c->r.p, c->z.p,
-                            (const double*)c->part_pAp.p, nwg_r, (const double*)c->rz_hist.p, c->part_rz.p,
+                            (const float*)c->Ap.p, (const float*)cpl.Ap2, Pc, pe, pi, xsol, c->r.p, c->z.p,
+                            (const double*)c->part_pAp.p, nwg_r + nwg_c, (const double*)c->rz_hist.p, c->part_rz.p,
                             c->part_max.p, c->pcg_status.p);
     }
     HIPCHK(hipGetLastError());
@@ -2613,10 +2641,10 @@ int ando_pcg_noalloc(int32_t device, int32_t num_rows, int32_t num_blocks, const
         float* pcur = (it & 1) ? p1 : p0;
         hipLaunchKernelGGL(k_pcg_spmv<R>, dim3(nwg_r), dim3(kBlock), 0, s, n, it, values, row_ptr, cols,
                            (const float*)z, (const float*)pold, pcur, Ap, (const double*)part_rz,
-                           (const float*)part_max, nwg_v, rz_hist, part_pAp, st, tol, (const int32_t*)nullptr,
-                           (const int32_t*)nullptr, (const float*)nullptr);
+                           (const float*)part_max, nwg_v, rz_hist, part_pAp, st, tol,
+                           ando::Coupling{0, nwg_r, 0, nullptr, nullptr, nullptr, nullptr, nullptr});
         hipLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(kBlock), 0, s, n, it, (const float*)pcur,
-                           (const float*)Ap, (const float*)Pinv, n, 1, x, r, z, (const double*)part_pAp, nwg_r,
+                           (const float*)Ap, (const float*)nullptr, (const float*)Pinv, n, 1, x, r, z, (const double*)part_pAp, nwg_r,
                            (const double*)rz_hist, part_rz, part_max, st);
       }
       HIPCHK(hipGetLastError());

@@ -744,6 +744,18 @@ __global__ void k_count_keys(int count, int n, const uint32_t* __restrict__ keys
   if (k < (uint32_t)n) atomicAdd(&hist[k], 1);
 }
 
+// Rows with self-contact coupling blocks (vertex view entries), listed in
+// ascending order: flags, then the rows at their exclusive-scan positions.
+__global__ void k_couple_row_flags(int n, const int32_t* __restrict__ vptr, int32_t* __restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) flag[i] = vptr[i + 1] > vptr[i] ? 1 : 0;
+}
+__global__ void k_couple_row_fill(int n, const int32_t* __restrict__ vptr, const int32_t* __restrict__ pos,
+                                  int32_t* __restrict__ crow) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && vptr[i + 1] > vptr[i]) crow[pos[i]] = i;
+}
+
 // Faithful mode on a one-triangle mesh: each vertex is the point of at most
 // one PT contact (there is one triangle); pt_of[v] = that contact, else -1
 // (cleared by the caller).

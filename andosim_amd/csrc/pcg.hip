@@ -302,36 +302,108 @@ __device__ inline int xcd_logical_group(int w, int G) {
   return x * q + min(x, rem) + w / 8;
 }
 
-// Self-contact coupling blocks of a row (corrected modes; cptr null
-// otherwise): blocks [cb, ce) = [3 cptr[i], 3 cptr[i + 1]) with columns ccol
-// and 3x3 values cval, multiplied by p = z + beta p_old formed on the fly.
-// Virtual lane v (0..7) sums blocks cb + v, cb + v + 8, ... in order; a row's
-// L lanes share the virtual lanes (lane q takes v = q, q + L, ...) and the
-// eight sums meet as ((s0 + s4) + (s2 + s6)) + ((s1 + s5) + (s3 + s7)) in
-// every SpMV kernel, which then adds them to the row's mesh sum, so all of
-// them produce the same Ap bit for bit.
-__device__ __forceinline__ void couple_vlane(int v, int cb, int ce, const int32_t* __restrict__ ccol,
-                                             const float* __restrict__ cval, const float* __restrict__ z,
-                                             const float* __restrict__ p_old, float beta, int it, float* sv) {
-  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
-  for (int b = cb + v; b < ce; b += 8) {
-    const int j = ccol[b];
-    const float* B = cval + 9 * (size_t)b;
-    float pv0, pv1, pv2;
-    if (it > 0) {
-      pv0 = z[3 * j] + beta * p_old[3 * j];
-      pv1 = z[3 * j + 1] + beta * p_old[3 * j + 1];
-      pv2 = z[3 * j + 2] + beta * p_old[3 * j + 2];
-    } else {
-      pv0 = p_old[3 * j]; pv1 = p_old[3 * j + 1]; pv2 = p_old[3 * j + 2];
-    }
-    s0 = s0 + (B[0] * pv0 + (B[1] * pv1 + B[2] * pv2));
-    s1 = s1 + (B[3] * pv0 + (B[4] * pv1 + B[5] * pv2));
-    s2 = s2 + (B[6] * pv0 + (B[7] * pv1 + B[8] * pv2));
+// Self-contact coupling blocks (corrected modes).  They change with every
+// Newton iteration and cluster on a few rows, so they are not part of the
+// sliced layout: workgroups added to every SpMV grid (dispatched before the
+// mesh workgroups) take the rows that have such blocks (crow, ncrow of them), 8
+// lanes per row, and write the row's coupling product to Ap2 (zero on the
+// other rows); k_pcg_update adds Ap + Ap2, as the row's mesh sum plus its
+// coupling sum.  Row i's blocks are [3 cptr[i], 3 cptr[i + 1]) with columns
+// ccol and 3x3 values cval; lane l sums blocks l, l + 8, ... in order and the
+// lanes meet by an xor-shuffle tree.  Their p.Ap2 partials follow the mesh
+// workgroups' partials.
+struct Coupling {
+  int ncrow;              // rows with coupling blocks (0: none)
+  int nwg_mesh;           // mesh workgroups of the grid (their partials come first)
+  int nwg_couple;         // coupling workgroups, dispatched first: blockIdx < nwg_couple
+  const int32_t* crow;    // those rows
+  const int32_t* cptr;
+  const int32_t* ccol;
+  const float* cval;
+  float* Ap2;
+};
+
+__device__ __noinline__ void couple_rows_body(int w, const Coupling& C, int n, int it, const float* __restrict__ z,
+                                              const float* __restrict__ p_old, const double* __restrict__ part_rz,
+                                              const float* __restrict__ part_max, int nwg_v,
+                                              const double* __restrict__ rz_hist, double* __restrict__ part_pAp,
+                                              PcgStatus* __restrict__ st, float tol) {
+  __shared__ double ldsd[16];
+  __shared__ float ldsf[16];
+  const int lane = threadIdx.x & 7;
+  const int k = w * (int)(blockDim.x >> 3) + (int)(threadIdx.x >> 3);
+  const int i = k < C.ncrow ? C.crow[k] : -1;
+  const int cb = i >= 0 ? 3 * C.cptr[i] : 0, ce = i >= 0 ? 3 * C.cptr[i + 1] : 0;
+  if (st->done) return;
+  float beta = 0.0f;
+  if (it > 0) {
+    const float bmax = st->bmax;
+    const double rz_prev = rz_hist[it - 1];
+    double rz = reduce_sum_partials(part_rz, nwg_v, ldsd);
+    float rmax = reduce_max_partials(part_max, nwg_v, ldsf);
+    float rel = ((double)bmax < 1e-16) ? rmax : rmax / bmax;
+    if (rel < tol) return;  // the mesh workgroups record the verdict
+    beta = (float)(rz / rz_prev);
   }
-  sv[0] = s0;
-  sv[1] = s1;
-  sv[2] = s2;
+  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+  // Four blocks per lane are requested before any is used (the sums keep
+  // the block order).
+  constexpr int kU = 4;
+  for (int b0 = cb + lane; b0 < ce; b0 += 8 * kU) {
+    int j[kU];
+    float B[kU][9], pv[kU][3];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int b = b0 + 8 * u;
+      if (b < ce) {
+        j[u] = C.ccol[b];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) B[u][e] = C.cval[9 * (size_t)b + e];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (b0 + 8 * u < ce) {
+        if (it > 0) {
+          pv[u][0] = z[3 * j[u]] + beta * p_old[3 * j[u]];
+          pv[u][1] = z[3 * j[u] + 1] + beta * p_old[3 * j[u] + 1];
+          pv[u][2] = z[3 * j[u] + 2] + beta * p_old[3 * j[u] + 2];
+        } else {
+          pv[u][0] = p_old[3 * j[u]]; pv[u][1] = p_old[3 * j[u] + 1]; pv[u][2] = p_old[3 * j[u] + 2];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (b0 + 8 * u < ce) {
+        s0 = s0 + (B[u][0] * pv[u][0] + (B[u][1] * pv[u][1] + B[u][2] * pv[u][2]));
+        s1 = s1 + (B[u][3] * pv[u][0] + (B[u][4] * pv[u][1] + B[u][5] * pv[u][2]));
+        s2 = s2 + (B[u][6] * pv[u][0] + (B[u][7] * pv[u][1] + B[u][8] * pv[u][2]));
+      }
+    }
+  }
+  for (int off = 4; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 8);
+    s1 += __shfl_xor(s1, off, 8);
+    s2 += __shfl_xor(s2, off, 8);
+  }
+  double pap = 0.0;
+  if (i >= 0 && lane == 0) {
+    float q0, q1, q2;
+    if (it > 0) {
+      q0 = z[3 * i] + beta * p_old[3 * i];
+      q1 = z[3 * i + 1] + beta * p_old[3 * i + 1];
+      q2 = z[3 * i + 2] + beta * p_old[3 * i + 2];
+    } else {
+      q0 = p_old[3 * i]; q1 = p_old[3 * i + 1]; q2 = p_old[3 * i + 2];
+    }
+    C.Ap2[3 * i] = s0;
+    C.Ap2[3 * i + 1] = s1;
+    C.Ap2[3 * i + 2] = s2;
+    pap = (double)q0 * s0 + (double)q1 * s1 + (double)q2 * s2;
+  }
+  const double sp = block_reduce(pap, ldsd, [](double a, double b) { return a + b; });
+  if (threadIdx.x == 0) part_pAp[C.nwg_mesh + w] = sp;
 }
 
 // One block row per group of kLanesPerRow lanes, one 3x3 block per lane per
@@ -351,13 +423,16 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int n, int it, const float*
                            float* __restrict__ Ap, const double* __restrict__ part_rz,
                            const float* __restrict__ part_max, int nwg_v, double* __restrict__ rz_hist,
                            double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol,
-                           const int32_t* __restrict__ cptr, const int32_t* __restrict__ ccol,
-                           const float* __restrict__ cval) {
+                           Coupling cpl) {
   constexpr int kRowsPerGroup = kRowsPerPass * R;
+  if ((int)blockIdx.x < cpl.nwg_couple) {
+    couple_rows_body((int)blockIdx.x, cpl, n, it, z, p_old, part_rz, part_max, nwg_v, rz_hist, part_pAp, st, tol);
+    return;
+  }
   __shared__ double ldsd[4];
   __shared__ float ldsf[4];
   const int lane = threadIdx.x % kLanesPerRow;
-  const int g = xcd_logical_group(blockIdx.x, gridDim.x);
+  const int g = xcd_logical_group((int)blockIdx.x - cpl.nwg_couple, cpl.nwg_mesh);
   int row[R], ri[R], beg[R], end[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
@@ -443,19 +518,6 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int n, int it, const float*
       s1 += __shfl_xor(s1, off, kLanesPerRow);
       s2 += __shfl_xor(s2, off, kLanesPerRow);
     }
-    if (cptr) {
-      const int cb = row[r] < n ? 3 * cptr[ri[r]] : 0, ce = row[r] < n ? 3 * cptr[ri[r] + 1] : 0;
-      float c3[3];
-      couple_vlane(lane, cb, ce, ccol, cval, z, p_old, beta, it, c3);
-      for (int off = kLanesPerRow / 2; off > 0; off >>= 1) {
-        c3[0] += __shfl_xor(c3[0], off, kLanesPerRow);
-        c3[1] += __shfl_xor(c3[1], off, kLanesPerRow);
-        c3[2] += __shfl_xor(c3[2], off, kLanesPerRow);
-      }
-      s0 = s0 + c3[0];
-      s1 = s1 + c3[1];
-      s2 = s2 + c3[2];
-    }
     const int i = row[r];
     if (i < n && lane == 0) {
       float q0, q1, q2;
@@ -523,11 +585,14 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_sell(int n, int slices, int
                            const float* __restrict__ p_old, float* __restrict__ p_new, float* __restrict__ Ap,
                            const double* __restrict__ part_rz, const float* __restrict__ part_max, int nwg_v,
                            double* __restrict__ rz_hist, double* __restrict__ part_pAp, PcgStatus* __restrict__ st,
-                           float tol, const int32_t* __restrict__ cptr, const int32_t* __restrict__ ccol,
-                           const float* __restrict__ cval) {
+                           float tol, Coupling cpl) {
+  if ((int)blockIdx.x < cpl.nwg_couple) {
+    couple_rows_body((int)blockIdx.x, cpl, n, it, z, p_old, part_rz, part_max, nwg_v, rz_hist, part_pAp, st, tol);
+    return;
+  }
   __shared__ double ldsd[4];
   __shared__ float ldsf[4];
-  const int g = xcd_logical_group(blockIdx.x, gridDim.x);
+  const int g = xcd_logical_group((int)blockIdx.x - cpl.nwg_couple, cpl.nwg_mesh);
   const int lane = threadIdx.x & 63;
   // The slice index is wave-uniform; saying so makes its metadata scalar loads
   // and the slot tests scalar branches.
@@ -647,15 +712,6 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_sell(int n, int slices, int
   float sr[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) sr[c] = ((t[0][c] + t[4][c]) + (t[2][c] + t[6][c])) + ((t[1][c] + t[5][c]) + (t[3][c] + t[7][c]));
-  if (cptr) {
-    const int cb = live && row < n ? 3 * cptr[row] : 0, ce = live && row < n ? 3 * cptr[row + 1] : 0;
-    float cs[8][3];
-#pragma unroll
-    for (int v = 0; v < 8; ++v) couple_vlane(v, cb, ce, ccol, cval, z, p_old, beta, it, cs[v]);
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      sr[c] = sr[c] + (((cs[0][c] + cs[4][c]) + (cs[2][c] + cs[6][c])) + ((cs[1][c] + cs[5][c]) + (cs[3][c] + cs[7][c])));
-  }
   const float s0 = sr[0], s1 = sr[1], s2 = sr[2];
   double pap = 0.0;
   if (live && row < n) {
@@ -700,9 +756,7 @@ __device__ __forceinline__ void sell_split_body(int g, int n, int slices, int it
                            const float* __restrict__ z, const float* __restrict__ p_old, float* __restrict__ p_new,
                            float* __restrict__ Ap, const double* __restrict__ part_rz,
                            const float* __restrict__ part_max, int nwg_v, double* __restrict__ rz_hist,
-                           double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol,
-                           const int32_t* __restrict__ cptr, const int32_t* __restrict__ ccol,
-                           const float* __restrict__ cval) {
+                           double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol) {
   static_assert(L == 2 || L == 4, "two or four lanes per row");
   static_assert(S == 0 || S == 1 || S == 2 || S == 4, "a wave, or one, two or four slices per workgroup");
   constexpr int R = 64 / L;             // rows per wave
@@ -838,27 +892,7 @@ __device__ __forceinline__ void sell_split_body(int g, int n, int slices, int it
       sr[c] = h + __shfl_down(h, 16, 64);
     }
   }
-  if (cptr) {
-    // Lane group q holds the virtual lanes q, q + L, ...; the same tree as the
-    // mesh part above.
-    const int cb = live && row < n ? 3 * cptr[row] : 0, ce = live && row < n ? 3 * cptr[row + 1] : 0;
-    float cs[8 / L][3];
-#pragma unroll
-    for (int m = 0; m < 8 / L; ++m) couple_vlane(q + L * m, cb, ce, ccol, cval, z, p_old, beta, it, cs[m]);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      float h, tot;
-      if constexpr (L == 2) {
-        h = (cs[0][c] + cs[2][c]) + (cs[1][c] + cs[3][c]);
-        tot = h + __shfl_down(h, 32, 64);
-      } else {
-        h = cs[0][c] + cs[1][c];
-        h = h + __shfl_down(h, 32, 64);
-        tot = h + __shfl_down(h, 16, 64);
-      }
-      sr[c] = sr[c] + tot;
-    }
-  }
+
   double pap = 0.0;
   if (q == 0 && live && row < n) {
     float q0, q1, q2;
@@ -895,17 +929,19 @@ __global__ void __launch_bounds__(64 * (S == 0 ? 1 : S * L)) k_pcg_spmv_sell_spl
                            float* __restrict__ Ap, const double* __restrict__ part_rz,
                            const float* __restrict__ part_max, int nwg_v, double* __restrict__ rz_hist,
                            double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol,
-                           const int32_t* __restrict__ cptr, const int32_t* __restrict__ ccol,
-                           const float* __restrict__ cval) {
-  sell_split_body<L, S>(xcd_logical_group(blockIdx.x, gridDim.x), n, slices, it, vals, scol, slice_off, slice_w,
-                        uniform_w, z, p_old, p_new, Ap, part_rz, part_max, nwg_v, rz_hist, part_pAp, st, tol, cptr,
-                        ccol, cval);
+                           Coupling cpl) {
+  if ((int)blockIdx.x < cpl.nwg_couple) {
+    couple_rows_body((int)blockIdx.x, cpl, n, it, z, p_old, part_rz, part_max, nwg_v, rz_hist, part_pAp, st, tol);
+    return;
+  }
+  sell_split_body<L, S>(xcd_logical_group((int)blockIdx.x - cpl.nwg_couple, cpl.nwg_mesh), n, slices, it, vals, scol, slice_off, slice_w,
+                        uniform_w, z, p_old, p_new, Ap, part_rz, part_max, nwg_v, rz_hist, part_pAp, st, tol);
 }
 
 // alpha = r.z / p.Ap, x += alpha p, r -= alpha Ap, z = P^-1 r; one thread per
 // vertex.  The vertex's operands are loaded before the p.Ap reduction.
 __device__ __forceinline__ void update_body(int wg, int n, int it, const float* __restrict__ p,
-                             const float* __restrict__ Ap, const float* __restrict__ Pinv, int pe,
+                             const float* __restrict__ Ap, const float* __restrict__ Ap2, const float* __restrict__ Pinv, int pe,
                              int pi, float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
                              const double* __restrict__ part_pAp, int nwg_r, const double* __restrict__ rz_hist,
                              double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st) {
@@ -924,7 +960,7 @@ __device__ __forceinline__ void update_body(int wg, int n, int it, const float* 
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     pv[k] = p[3 * ii + k];
-    av[k] = Ap[3 * ii + k];
+    av[k] = Ap2 ? Ap[3 * ii + k] + Ap2[3 * ii + k] : Ap[3 * ii + k];
     xv[k] = x[3 * ii + k];
     rv0[k] = r[3 * ii + k];
   }
@@ -971,11 +1007,11 @@ __device__ __forceinline__ void update_body(int wg, int n, int it, const float* 
 }
 
 __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const float* __restrict__ p,
-                             const float* __restrict__ Ap, const float* __restrict__ Pinv, int pe,
+                             const float* __restrict__ Ap, const float* __restrict__ Ap2, const float* __restrict__ Pinv, int pe,
                              int pi, float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
                              const double* __restrict__ part_pAp, int nwg_r, const double* __restrict__ rz_hist,
                              double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st) {
-  update_body(blockIdx.x, n, it, p, Ap, Pinv, pe, pi, x, r, z, part_pAp, nwg_r, rz_hist, part_rz, part_max, st);
+  update_body(blockIdx.x, n, it, p, Ap, Ap2, Pinv, pe, pi, x, r, z, part_pAp, nwg_r, rz_hist, part_rz, part_max, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -1021,14 +1057,14 @@ __global__ void __launch_bounds__(128) k_pcg_spmv_sell_batch(const PcgSys* __res
   float* pcur = (it & 1) ? S.p1 : S.p0;
   sell_split_body<2, 1>((int)blockIdx.x - S.wg0_r, S.n, S.slices, it, S.vals, S.scol, S.slice_off, S.slice_w,
                         S.uniform_w, S.z, pold, pcur, S.Ap, S.part_rz, S.part_max, S.nwg_v, S.rz_hist, S.part_pAp,
-                        S.st, S.tol, nullptr, nullptr, nullptr);
+                        S.st, S.tol);
 }
 
 __global__ void __launch_bounds__(kBlock) k_pcg_update_batch(const PcgSys* __restrict__ sys,
                                                             const int32_t* __restrict__ wg_sys, int it) {
   const PcgSys& S = sys[wg_sys[blockIdx.x]];
   float* pcur = (it & 1) ? S.p1 : S.p0;
-  update_body((int)blockIdx.x - S.wg0_v, S.n, it, pcur, S.Ap, S.Pinv, S.pe, S.pi, S.x, S.r, S.z,
+  update_body((int)blockIdx.x - S.wg0_v, S.n, it, pcur, S.Ap, nullptr, S.Pinv, S.pe, S.pi, S.x, S.r, S.z,
               S.part_pAp, S.nwg_r, S.rz_hist, S.part_rz, S.part_max, S.st);
 }
 
