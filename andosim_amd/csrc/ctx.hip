@@ -1215,13 +1215,17 @@ void autotune_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const i
                   const float* rhs, int negate, float* xsol, int maxit, SpmvMeta* sm, const ando::SelfView* sv);
 
 // P_given / x0_given (device, both or neither): a preconditioner of 9 floats
-// per vertex and a starting guess instead of block-Jacobi from x0 = 0.
+// per vertex and a starting guess instead of block-Jacobi from x0 = 0
+// (x0_given may be xsol itself).  guard_x0: a starting guess whose residual is
+// no smaller than b's (relative residual >= 1, that of x0 = 0) is replaced by
+// x0 = 0 (one status read-back more).  With sv the starting residual
+// includes the coupling blocks' share of A x0.
 // sv (nullable): self-contact blocks that couple two vertices, multiplied
 // inside every SpMV (corrected modes).
 PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int32_t* cols, const int32_t* diag,
                const float* rhs, int negate, float* xsol, float tol, int maxit, SpmvMeta* sm,
                const float* P_given = nullptr, const float* x0_given = nullptr,
-               const ando::SelfView* sv = nullptr) {
+               const ando::SelfView* sv = nullptr, bool guard_x0 = false) {
   hipStream_t s = c->stream;
   const bool couple = sv != nullptr && sv->ptr != nullptr;
   const int R = c->spmv_rowsets;
@@ -1278,37 +1282,6 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
     c->stats.batched_solves++;
     return t_batch->solve(job);
   }
-  const float* Pc = c->Pinv.p;
-  int pe = n, pi = 1;  // block-Jacobi blocks as one plane per entry
-  if (sell)
-    hipLaunchKernelGGL(k_sell_fill, dim3(nblk(sm->slices, kBlock / 64)), dim3(kBlock), 0, s, sm->slices,
-                       (const int32_t*)sm->slice_off.p, (const int32_t*)sm->slice_w.p, (const int32_t*)sm->sell_perm.p,
-                       A, sm->sell_vals.p);
-  if (P_given) {
-    pe = 1;
-    pi = 9;
-    c->part_bmax.ensure((size_t)nwg_v + 1);
-    hipLaunchKernelGGL(k_pcg_init_given, dim3(nwg_v), dim3(vb), 0, s, n, A, rp, cols, P_given, rhs, negate,
-                       x0_given, xsol, c->r.p, c->z.p, c->pbuf0.p, c->part_rz.p, c->part_max.p, c->part_bmax.p);
-    Pc = P_given;
-  } else {
-    hipLaunchKernelGGL(k_pcg_init, dim3(nwg_v), dim3(vb), 0, s, n, A, diag, rhs, xsol, c->r.p, c->z.p,
-                       c->pbuf0.p, c->Pinv.p, pe, pi, c->part_rz.p, c->part_max.p, negate);
-  }
-  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, s, nwg_v, (const double*)c->part_rz.p,
-                     (const float*)c->part_max.p, (const float*)(P_given ? c->part_bmax.p : nullptr), c->rz_hist.p,
-                     c->pcg_status.p, tol);
-  HIPCHK(hipGetLastError());
-  PcgStatus st{};
-  int it = 0;
-  // The iterations go out in chunks of 8, 16, 32, 64, 128, then 256 iterations;
-  // after each chunk the status is copied to pinned host memory and read.  Launches
-  // queued after convergence return at the done flag (after issuing their
-  // first loads, which lets a live launch overlap the flag's latency).  Tried
-  // and slower on this path: queueing the next chunk before waiting, and
-  // sizing the first chunk from the previous step's solve.
-  int chunk = std::max(1, c->pcg_chunk);
-  const bool timing = c->kernel_timing != 0;
   // Coupling blocks of the self contacts: workgroups appended to the SpMV
   // grid (8 lanes per row with blocks) write them to Ap2; their partials
   // follow the mesh workgroups'.
@@ -1328,6 +1301,54 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
     cpl = ando::Coupling{c->ncrow, nwg_r, nwg_c, c->crow.p, sv->ptr, c->ccol.p, c->cval.p, c->Ap2.p};
     c->part_pAp.ensure((size_t)nwg_r + nwg_c + 1);
   }
+  const float* Pc = c->Pinv.p;
+  int pe = n, pi = 1;  // block-Jacobi blocks as one plane per entry
+  if (sell)
+    hipLaunchKernelGGL(k_sell_fill, dim3(nblk(sm->slices, kBlock / 64)), dim3(kBlock), 0, s, sm->slices,
+                       (const int32_t*)sm->slice_off.p, (const int32_t*)sm->slice_w.p, (const int32_t*)sm->sell_perm.p,
+                       A, sm->sell_vals.p);
+  if (P_given) {
+    pe = 1;
+    pi = 9;
+    c->part_bmax.ensure((size_t)nwg_v + 1);
+    // The coupling blocks' share of A x0 (in Ap2, zero off the coupling rows).
+    if (x0_given && cpl.ncrow > 0)
+      hipLaunchKernelGGL(ando::k_couple_mul, dim3(nblk(cpl.ncrow, 32)), dim3(256), 0, s, cpl, x0_given, cpl.Ap2);
+    hipLaunchKernelGGL(k_pcg_init_given, dim3(nwg_v), dim3(vb), 0, s, n, A, rp, cols, P_given, rhs, negate,
+                       x0_given, xsol, c->r.p, c->z.p, c->pbuf0.p, c->part_rz.p, c->part_max.p, c->part_bmax.p,
+                       (const float*)cpl.Ap2);
+    Pc = P_given;
+  } else {
+    hipLaunchKernelGGL(k_pcg_init, dim3(nwg_v), dim3(vb), 0, s, n, A, diag, rhs, xsol, c->r.p, c->z.p,
+                       c->pbuf0.p, c->Pinv.p, pe, pi, c->part_rz.p, c->part_max.p, negate);
+  }
+  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, s, nwg_v, (const double*)c->part_rz.p,
+                     (const float*)c->part_max.p, (const float*)(P_given ? c->part_bmax.p : nullptr), c->rz_hist.p,
+                     c->pcg_status.p, tol);
+  HIPCHK(hipGetLastError());
+  if (P_given && x0_given && guard_x0) {
+    PcgStatus s0{};
+    read_small(c, &s0, (const PcgStatus*)c->pcg_status.p, 1);
+    if (!(s0.rel < 1.0f)) {
+      hipLaunchKernelGGL(k_pcg_init_given, dim3(nwg_v), dim3(vb), 0, s, n, A, rp, cols, P_given, rhs, negate,
+                         (const float*)nullptr, xsol, c->r.p, c->z.p, c->pbuf0.p, c->part_rz.p, c->part_max.p,
+                         c->part_bmax.p, (const float*)nullptr);
+      hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, s, nwg_v, (const double*)c->part_rz.p,
+                         (const float*)c->part_max.p, (const float*)c->part_bmax.p, c->rz_hist.p, c->pcg_status.p,
+                         tol);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  PcgStatus st{};
+  int it = 0;
+  // The iterations go out in chunks of 8, 16, 32, 64, 128, then 256 iterations;
+  // after each chunk the status is copied to pinned host memory and read.  Launches
+  // queued after convergence return at the done flag (after issuing their
+  // first loads, which lets a live launch overlap the flag's latency).  Tried
+  // and slower on this path: queueing the next chunk before waiting, and
+  // sizing the first chunk from the previous step's solve.
+  int chunk = std::max(1, c->pcg_chunk);
+  const bool timing = c->kernel_timing != 0;
   size_t ev_used = 0;
   // Timed launches record their start and end with the dispatch itself
   // (hipExtLaunchKernelGGL), so the events add no packets between kernels.
@@ -1744,20 +1765,26 @@ float inner_newton(ando_ctx* c, const ando_params& p) {
     PcgOut po;
     const ando::SelfView sv = self_view(c);
     if (c->learned && c->nn.ready) {
-      // Learned preconditioner and warm start; plain block-Jacobi PCG from
-      // x0 = 0 is the fallback when it misses the tolerance.
+      // Learned preconditioner and warm start, within the plain solve's
+      // iteration cap: the warm start is dropped when its residual is no
+      // smaller than b's; the learned preconditioner gets half the cap, and a
+      // solve that misses the tolerance continues with block-Jacobi from its
+      // iterate for the rest.  Warm-start-only mode is block-Jacobi PCG from
+      // the warm start with the whole cap.
       nnh::forward(c->nn, c->nnw, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, false,
                    c->stream);
       nnh::warm_start(c->nnw, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->g.p, 1, c->stream);
+      const int cap = p.pcg_max_iters;
+      const bool precond = c->learned == 1;
+      const int first_cap = precond ? std::max(1, cap / 2) : cap;
       po = run_pcg(c, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol,
-                   p.pcg_max_iters, &c->spmv_meta, c->learned == 2 ? c->nnw.Dinv.p : c->nnw.P.p, c->nnw.y.p,
-                   &sv);
+                   first_cap, &c->spmv_meta, precond ? c->nnw.P.p : c->nnw.Dinv.p, c->nnw.y.p, &sv, true);
       c->stats.learned_solves++;
-      if (!po.converged) {
+      if (!po.converged && precond && cap - po.iters > 0) {
         c->stats.learned_fallbacks++;
         const int first = po.iters;
         po = run_pcg(c, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol,
-                     p.pcg_max_iters, &c->spmv_meta, nullptr, nullptr, &sv);
+                     cap - first, &c->spmv_meta, c->nnw.Dinv.p, c->d.p, &sv, true);
         c->stats.fallback_iterations += po.iters;
         po.iters += first;
       }

@@ -227,24 +227,27 @@ __global__ void k_pcg_init_given(int n, const float* __restrict__ A, const int32
                                  const float* __restrict__ g, int negate, const float* __restrict__ x0,
                                  float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
                                  float* __restrict__ p, double* __restrict__ part_rz, float* __restrict__ part_max,
-                                 float* __restrict__ part_bmax) {
+                                 float* __restrict__ part_bmax, const float* __restrict__ Cx0) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double rz = 0.0;
   float rm = 0.0f, bm = 0.0f;
   if (i < n) {
     float ax[3] = {0.0f, 0.0f, 0.0f};
-    for (int b = rp[i]; b < rp[i + 1]; ++b) {
-      const int j = cols[b];
-      const float* B = A + 9 * (size_t)b;
-      for (int k = 0; k < 3; ++k)
-        ax[k] = ax[k] + (B[3 * k] * x0[3 * j] + (B[3 * k + 1] * x0[3 * j + 1] + B[3 * k + 2] * x0[3 * j + 2]));
-    }
+    if (x0)
+      for (int b = rp[i]; b < rp[i + 1]; ++b) {
+        const int j = cols[b];
+        const float* B = A + 9 * (size_t)b;
+        for (int k = 0; k < 3; ++k)
+          ax[k] = ax[k] + (B[3 * k] * x0[3 * j] + (B[3 * k + 1] * x0[3 * j + 1] + B[3 * k + 2] * x0[3 * j + 2]));
+      }
+    if (x0 && Cx0)
+      for (int k = 0; k < 3; ++k) ax[k] = ax[k] + Cx0[3 * i + k];
     float rv[3];
     for (int k = 0; k < 3; ++k) {
       const float bk = negate ? -g[3 * i + k] : g[3 * i + k];
       rv[k] = bk - ax[k];
       r[3 * i + k] = rv[k];
-      x[3 * i + k] = x0[3 * i + k];
+      x[3 * i + k] = x0 ? x0[3 * i + k] : 0.0f;
       rm = fmaxf(rm, fabsf(rv[k]));
       bm = fmaxf(bm, fabsf(bk));
     }
@@ -404,6 +407,34 @@ __device__ __noinline__ void couple_rows_body(int w, const Coupling& C, int n, i
   }
   const double sp = block_reduce(pap, ldsd, [](double a, double b) { return a + b; });
   if (threadIdx.x == 0) part_pAp[C.nwg_mesh + w] = sp;
+}
+
+// out_i = (coupling blocks of row i) x for the coupling rows (out must be zero
+// elsewhere): the starting residual of a warm-started solve.  8 lanes per row,
+// 32 rows per 256-thread workgroup.
+__global__ void __launch_bounds__(256) k_couple_mul(Coupling C, const float* __restrict__ x, float* __restrict__ out) {
+  const int lane = threadIdx.x & 7;
+  const int k = blockIdx.x * 32 + (int)(threadIdx.x >> 3);
+  const int i = k < C.ncrow ? C.crow[k] : -1;
+  const int cb = i >= 0 ? 3 * C.cptr[i] : 0, ce = i >= 0 ? 3 * C.cptr[i + 1] : 0;
+  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+  for (int b = cb + lane; b < ce; b += 8) {
+    const int j = C.ccol[b];
+    const float* B = C.cval + 9 * (size_t)b;
+    s0 = s0 + (B[0] * x[3 * j] + (B[1] * x[3 * j + 1] + B[2] * x[3 * j + 2]));
+    s1 = s1 + (B[3] * x[3 * j] + (B[4] * x[3 * j + 1] + B[5] * x[3 * j + 2]));
+    s2 = s2 + (B[6] * x[3 * j] + (B[7] * x[3 * j + 1] + B[8] * x[3 * j + 2]));
+  }
+  for (int off = 4; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, 8);
+    s1 += __shfl_xor(s1, off, 8);
+    s2 += __shfl_xor(s2, off, 8);
+  }
+  if (i >= 0 && lane == 0) {
+    out[3 * i] = s0;
+    out[3 * i + 1] = s1;
+    out[3 * i + 2] = s2;
+  }
 }
 
 // One block row per group of kLanesPerRow lanes, one 3x3 block per lane per

@@ -1,8 +1,9 @@
 """Newton systems for training and evaluating the learned preconditioner.
 
 Each system is the first Newton linearisation of a substep, taken from a run
-of the plain GPU solver: H and b = -g at x with inertia target x_hat = x + dt v
-(Integrator::compute_gradient / assemble_system_matrix, integrator.cpp:246-544).
+of the plain GPU solver (corrected physics by default): H and b = -g at x with
+inertia target x_hat = x + dt v (Integrator::compute_gradient /
+assemble_system_matrix, integrator.cpp:246-544).
 Training data comes from the batched workload's sheets (side U{17..65}, 1-4
 pins, wind, ground wall).
 """
@@ -13,11 +14,15 @@ import numpy as np
 from . import native, scenes
 
 
-def collect(scene_list, steps: int, every: int = 1, start: int = 0, device: int = 0):
-    """Returns a list of (row_ptr, cols, H (nb,3,3), b (3n,)) float32/int32 systems."""
+def collect(scene_list, steps: int, every: int = 1, start: int = 0, device: int = 0, corrected_physics: bool = True):
+    """Returns a list of (row_ptr, cols, H (nb,3,3), b (3n,)) float32/int32
+    systems, from runs with the corrected physics unless told otherwise (the
+    matrix is the mesh part; self-contact coupling blocks are not in it)."""
     out = []
     for sc in scene_list:
         sim = native.GpuSim(sc, device=device)
+        if corrected_physics:
+            sim.set_mode(fp64_near=True, corrected_physics=True)
         rp, ci = sim.pattern()
         dt = np.float32(sc.params["dt"])
         for s in range(steps):

@@ -345,10 +345,11 @@ def learned_comparison(sim, sc, window, restore, replay, a):
       * the first Newton system of every window step (taken from the plain
         run), solved by plain PCG and by the learned preconditioner with its
         warm start (no fallback);
-      * the window replayed with ANDO_MODE_LEARNED_PRECOND (learned solves,
-        plain fallback when one misses the tolerance), PCG iterations per step
-        and steps/s, with the learned preconditioner and with the warm start
-        only (block-Jacobi preconditioner).
+      * the window replayed from its start state with plain block-Jacobi PCG
+        and with ANDO_MODE_LEARNED_PRECOND (the learned preconditioner for
+        half the iteration cap, then block-Jacobi from its iterate; or the
+        warm start only): PCG iterations, Newton iterations, failures per
+        step, steps/s and the validator at the end.
     """
     from andosim_amd import learned_precond, native
 
@@ -371,9 +372,13 @@ def learned_comparison(sim, sc, window, restore, replay, a):
     # Whole steps in each learned mode.
     sim.set_precond_weights(w)
 
-    def steps(warm_only):
-        sim.set_mode(fp64_near=True, kernel_timing=False, learned=True,
-                     warm_start_only=warm_only, corrected_physics=True)
+    def steps(mode):
+        """The window replayed from its start state with the given solver."""
+        if mode == "plain":
+            sim.set_mode(fp64_near=True, kernel_timing=False, corrected_physics=True)
+        else:
+            sim.set_mode(fp64_near=True, kernel_timing=False, learned=True, warm_start_only=(mode == "warm"),
+                         corrected_physics=True)
         restore()
         recs, nonfinite = [], None
         for k in window:
@@ -385,15 +390,23 @@ def learned_comparison(sim, sc, window, restore, replay, a):
                 break
         sim.set_mode(fp64_near=True, kernel_timing=False, learned=False, corrected_physics=True)
         el = sum(r[0] for r in recs)
-        return {"pcg_iters_per_step": round(sum(r[1]["pcg_iterations"] for r in recs) / len(recs), 2),
+        st = [r[1] for r in recs]
+        return {"pcg_iters_per_step": round(sum(t["pcg_iterations"] for t in st) / len(recs), 2),
                 "steps_per_s": round(len(recs) / el, 3), "nonfinite_at_step": nonfinite,
-                "solves": sum(r[1]["learned_solves"] for r in recs),
-                "fallbacks": sum(r[1]["learned_fallbacks"] for r in recs),
-                "fallback_iters_per_step": round(sum(r[1]["fallback_iterations"] for r in recs) / len(recs), 2)}
+                "newton_iters_per_step": round(sum(t["newton_iterations"] for t in st) / len(recs), 2),
+                "pcg_failures": sum(t["pcg_failures"] for t in st),
+                "line_search_failures": sum(t["line_search_failures"] for t in st),
+                "solves": sum(t["pcg_solves"] for t in st),
+                "learned_solves": sum(t["learned_solves"] for t in st),
+                "continued_in_block_jacobi": sum(t["learned_fallbacks"] for t in st),
+                "block_jacobi_iters_per_step": round(sum(t["fallback_iterations"] for t in st) / len(recs), 2),
+                "intersections_at_end": list(sim.count_intersections())}
 
     return {
-        "precond_and_warm_start": steps(False),
-        "warm_start_only": steps(True),
+        "window_substeps": [window[0], window[-1]],
+        "plain": steps("plain"),
+        "precond_and_warm_start": steps("learned"),
+        "warm_start_only": steps("warm"),
         "first_newton_system": {"plain_iters": [t[0] for t in table], "plain_converged": [t[1] for t in table],
                                 "learned_iters": [t[2] for t in table], "learned_converged": [t[3] for t in table],
                                 "warm_start_only_iters": [t[4] for t in table],
@@ -402,7 +415,7 @@ def learned_comparison(sim, sc, window, restore, replay, a):
                      "loss_first": round(float(np.mean(losses[:200])), 4),
                      "loss_last": round(float(np.mean(losses[-200:])), 4),
                      "data": "batched workload sheets (side U{17..65}, 1-4 pins, wind), substeps 20..100, "
-                             "and 4 sheets draped over a sphere"},
+                             "and 4 sheets draped over a sphere, corrected physics"},
     }
 
 

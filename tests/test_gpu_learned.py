@@ -110,3 +110,42 @@ def test_step_in_learned_mode():
         assert st["learned_solves"] == st["pcg_solves"]
         assert st["fallback_iterations"] <= st["pcg_iterations"]
     assert np.isfinite(g.get("x")).all()
+
+
+def test_learned_solves_stay_within_the_plain_cap():
+    """Never lose more than the cap: with weights far from trained, every
+    solve of a corrected-physics substep (self contacts, coupling blocks) uses
+    at most pcg_max_iters iterations in all -- half for the learned
+    preconditioner, the rest for block-Jacobi continuing from its iterate."""
+    sc = scenes.stack_resting(s=9, sheets=2, gap=8e-5)
+    sc.params["pcg_max_iters"] = 40
+    g = native.GpuSim(sc)
+    g.set_precond_weights(trained_like_weights(7))
+    g.set_mode(learned=True, corrected_physics=True)
+    for _ in range(5):
+        g.substep()
+        st = g.stats()
+        assert max(st["pcg_per_solve"]) <= 40, st["pcg_per_solve"]
+        assert st["learned_solves"] == st["pcg_solves"]
+    assert np.isfinite(g.get("x")).all()
+
+
+def test_warm_start_counts_the_coupling_blocks():
+    """A warm-started solve's starting residual includes the self-contact
+    coupling blocks' share of A x0: a substep in warm-start-only mode lands
+    where the plain solve does (to the PCG tolerance's reach), on the resting
+    stack whose coupling blocks are active."""
+    sc = scenes.stack_resting(s=9, sheets=2, gap=8e-5)
+    plain = native.GpuSim(sc)
+    plain.set_mode(corrected_physics=True)
+    ws = native.GpuSim(sc)
+    ws.set_precond_weights(trained_like_weights(4))
+    ws.set_mode(learned=True, warm_start_only=True, corrected_physics=True)
+    for k in range(3):
+        plain.set_state(ws.get("x"), ws.get("v"))
+        x0 = ws.get("x").astype(np.float64)
+        plain.substep()
+        ws.substep()
+        dp = plain.get("x").astype(np.float64) - x0
+        dw = ws.get("x").astype(np.float64) - x0
+        assert np.max(np.abs(dw - dp)) <= 1e-2 * np.max(np.abs(dp)), k
