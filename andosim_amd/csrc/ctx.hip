@@ -313,6 +313,7 @@ struct ando_ctx {
   DBuf<float> cval;
   DBuf<int32_t> crow, crow_flag, crow_pos;  // rows with coupling blocks (ncrow of them)
   int ncrow = 0;
+  int ncblk = 0;                            // coupling blocks (3 per vertex-view entry)
   DBuf<float> Ap2;                          // their coupling products during a solve
   DBuf<int32_t> pt_of;     // faithful one-triangle mesh: PT contact of each point vertex, or -1
   bool sv_ready = false;
@@ -972,6 +973,7 @@ void prepare_self(ando_ctx* c, const StepParams& P) {
   c->ccol.ensure(3 * (size_t)entries + 1);
   c->cval.ensure(27 * (size_t)entries + 1);
   c->ncrow = 0;
+  c->ncblk = 3 * entries;
   if (entries) {
     hipLaunchKernelGGL(k_self_blocks, dim3(nblk(3LL * entries)), dim3(kBlock), 0, s, entries, self_view(c),
                        c->barrier_tol, c->ccol.p, c->cval.p);
@@ -1433,6 +1435,9 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
       c->stats.timed_launches[1]++;
       c->stats.timed_ms[0] += a;
       c->stats.timed_ms[1] += b;
+      // The coupling workgroups' algorithmic bytes in that SpMV launch: 40 B
+      // per block; per row its index, range, Ap2 and z, p_old entries.
+      if (cpl.ncrow > 0) c->stats.timed_coupling_bytes += 40.0 * c->ncblk + 48.0 * cpl.ncrow;
     }
   }
   return PcgOut{st.iters, !st.failed};

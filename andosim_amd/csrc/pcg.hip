@@ -326,7 +326,7 @@ struct Coupling {
   float* Ap2;
 };
 
-__device__ __noinline__ void couple_rows_body(int w, const Coupling& C, int n, int it, const float* __restrict__ z,
+__device__ __forceinline__ void couple_rows_body(int w, const Coupling& C, int n, int it, const float* __restrict__ z,
                                               const float* __restrict__ p_old, const double* __restrict__ part_rz,
                                               const float* __restrict__ part_max, int nwg_v,
                                               const double* __restrict__ rz_hist, double* __restrict__ part_pAp,

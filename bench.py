@@ -485,21 +485,27 @@ def main(argv=None):
         sim.set_mode(fp64_near=True, kernel_timing=False, corrected_physics=True)
         t_launch = [sum(r[1]["timed_ms"][k] for r in ev_recs) for k in range(2)]
         n_launch = [sum(r[1]["timed_launches"][k] for r in ev_recs) for k in range(2)]
+        cpl_bytes = sum(r[1]["timed_coupling_bytes"] for r in ev_recs)
         ev_elapsed = sum(r[0] for r in ev_recs)
         if n_launch[0]:
             avg_s = t_launch[0] / n_launch[0] * 1e-3
-            by = spmv_bytes(n, nb)
+            # Mesh part plus the self-contact coupling workgroups of the launch.
+            cby = cpl_bytes / n_launch[0]
+            by = spmv_bytes(n, nb) + cby
             ach = by / avg_s / 1e9
             kname = {3: "ando::k_pcg_spmv_sell"}.get(kernel, "ando::k_pcg_spmv<3>")
             if kernel == 3 and lanes in (2, 4):
                 kname = f"ando::k_pcg_spmv_sell_split<{lanes}, {wg_slices}>"
             roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic(a.workload, n, nb, kname), "kernel": kname,
-                    "bytes_per_launch": by, "avg_launch_us": round(avg_s * 1e6, 2), "launches": n_launch[0]}
+                    "traffic": measured_traffic(a.workload + "_corrected_physics", n, nb, kname), "kernel": kname,
+                    "bytes_per_launch": int(round(by)), "coupling_bytes_per_launch": int(round(cby)),
+                    "avg_launch_us": round(avg_s * 1e6, 2), "launches": n_launch[0]}
             if n_launch[1]:
                 avg_u = t_launch[1] / n_launch[1] * 1e-3
-                roof["update_kernel_gbs"] = round(update_bytes(n) / avg_u / 1e9, 1)
+                # The update also reads Ap2 when the solves have coupling rows.
+                ub = update_bytes(n) + (12 * n if cpl_bytes > 0 else 0)
+                roof["update_kernel_gbs"] = round(ub / avg_u / 1e9, 1)
                 roof["update_avg_launch_us"] = round(avg_u * 1e6, 2)
                 roof["pcg_kernels_share_of_step"] = round((t_launch[0] + t_launch[1]) * 1e-3 / ev_elapsed, 3)
             roof["event_pass_ms_per_step"] = round(1e3 * ev_elapsed / len(ev_recs), 3)

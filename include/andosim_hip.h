@@ -164,6 +164,8 @@ typedef struct {
   int32_t max_candidates;           /* largest candidate list of one detection pass (PT, EE or rigid) */
   int32_t held_iterations;          /* corrected physics: Newton iterations that held the vertices of
                                        pairs within their separation that would not move apart */
+  double timed_coupling_bytes;      /* ANDO_MODE_KERNEL_TIMING: algorithmic bytes of the self-contact
+                                       coupling workgroups in the timed SpMV launches */
 } ando_stats;
 
 int32_t ando_abi_version(void);
