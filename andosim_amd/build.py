@@ -23,7 +23,8 @@ LIB = os.path.join(HERE, "libandosim_hip.so")
 EXT = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
 MODULE = os.path.join(HERE, "ando_barrier_core" + EXT)
 
-HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.environ.get("HIPCC", os.path.join(ROCM, "bin", "hipcc"))
 ARCH = os.environ.get("ANDO_OFFLOAD_ARCH", "gfx950")
 
 # Rounding semantics of the reference's x86-64 float build: no contraction,
@@ -44,7 +45,8 @@ def _run(cmd):
 def build_lib():
     src = os.path.join(CSRC, "ctx.hip")
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT, "include"), "-o", tmp, src])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT, "include"), "-o", tmp, src,
+          "-L", os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"])
     os.replace(tmp, LIB)
     return LIB
 

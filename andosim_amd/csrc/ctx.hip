@@ -4,6 +4,7 @@
 // runs as a sequence of kernel launches; the host only reads back the scalars
 // that steer control flow (gradient max-norm, line-search mask, PCG status,
 // contact totals).
+#include <rocsolver/rocsolver.h>
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -26,6 +27,7 @@
 #include "kernels.hip"
 #include "model.h"
 #include "pcg.hip"
+#include "tl.hip"
 #include "learned.hip"
 #include "lbvh.hip"
 #include "frame.hip"
@@ -197,6 +199,167 @@ void build_spmv_meta(SpmvMeta& sm, int n, const std::vector<int32_t>& row_ptr, c
   build_sell_meta(sm, n, row_ptr, cols, s);
 }
 
+// Two-level Schwarz PCG (tl.hip): the pattern's rows renumbered (Morton order
+// of the rest positions for a mesh), the sliced layout over the renumbered rows,
+// the coarse pattern (per patch pair, its fine blocks in ascending order), and
+// the solve's buffers.
+struct TlMeta {
+  bool ok = false;
+  // Coarse inverses, double-buffered: Acinv[cur] is used by the solves; the
+  // inverse of the latest system is made into the other buffer on the side
+  // stream (pending until the next solve takes it).
+  bool have = false, pending = false;
+  int cur = 0;
+  int n = 0, pr = 256, npatch = 0, nc = 0, ncb = 0;
+  long long nb = 0;
+  DBuf<int32_t> perm, iperm;
+  SpmvMeta sm;  // sell_col holds renumbered columns
+  DBuf<int32_t> cb_ij, cb_ptr, cb_blk;
+  DBuf<double> Ac[2];
+  DBuf<float> Acinv[2];
+  DBuf<int32_t> info[2];
+  DBuf<float> Dinv, lmax, x, r, z, p0, p1, Ap, qf;
+  DBuf<double> sa, sr0, sr1;  // per patch: sums of A p and of r (3 each)
+  DBuf<double> part_rz, part_pAp;
+  DBuf<float> part_max;
+};
+
+// Coarse unknowns stay dense: at most this many patches get a coarse level.
+constexpr int kTlMaxCoarsePatches = 1024;
+
+// Vertex order by 30-bit Morton code of the positions (10 bits per axis over
+// the bounding cube), ties by index.
+std::vector<int32_t> morton_order(const float* v, int n) {
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = std::min(lo[k], (double)v[3 * (size_t)i + k]);
+      hi[k] = std::max(hi[k], (double)v[3 * (size_t)i + k]);
+    }
+  const double ext = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), std::max(hi[2] - lo[2], 1e-30));
+  auto spread = [](uint32_t x) {
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+  };
+  std::vector<std::pair<uint32_t, int32_t>> key(n);
+  for (int i = 0; i < n; ++i) {
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+      const double t = ((double)v[3 * (size_t)i + k] - lo[k]) / ext * 1023.0;
+      q[k] = (uint32_t)std::min(1023.0, std::max(0.0, t));
+    }
+    key[i] = {spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2), i};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int32_t> perm(n);
+  for (int i = 0; i < n; ++i) perm[i] = key[i].second;
+  return perm;
+}
+
+// Builds T for the pattern (row_ptr, cols) with the row order perm (new ->
+// old): sliced layout of the renumbered rows (slot k of a row is its k-th BSR
+// block, columns renumbered), the coarse pattern and the solve's buffers.
+void build_tl_meta(TlMeta& T, int n, const std::vector<int32_t>& row_ptr, const std::vector<int32_t>& cols,
+                   const std::vector<int32_t>& perm, hipStream_t s) {
+  using namespace ando::tl;
+  T.ok = false;
+  std::vector<int32_t> iperm(n);
+  for (int i = 0; i < n; ++i) iperm[perm[i]] = i;
+  const int S = (n + 63) / 64;
+  std::vector<int32_t> off(S + 1, 0), w(S, 0);
+  for (int sl = 0; sl < S; ++sl) {
+    int W = 0;
+    for (int i = sl * 64; i < std::min(n, sl * 64 + 64); ++i) W = std::max(W, row_ptr[perm[i] + 1] - row_ptr[perm[i]]);
+    w[sl] = W;
+    const long long next = (long long)off[sl] + 64LL * W;
+    if (9 * next >= (1LL << 31)) return;
+    off[sl + 1] = (int32_t)next;
+  }
+  const size_t P = (size_t)off[S];
+  std::vector<int32_t> sperm(P, -1), scol(P, 0);
+  for (int sl = 0; sl < S; ++sl)
+    for (int l = 0; l < 64; ++l) {
+      const int i = sl * 64 + l;
+      for (int k = 0; k < w[sl]; ++k) {
+        const size_t pos = (size_t)off[sl] + 64 * (size_t)k + l;
+        if (i < n && row_ptr[perm[i]] + k < row_ptr[perm[i] + 1]) {
+          const int b = row_ptr[perm[i]] + k;
+          sperm[pos] = b;
+          scol[pos] = iperm[cols[b]];
+        } else {
+          scol[pos] = i < n ? i : 0;
+        }
+      }
+    }
+  SpmvMeta& sm = T.sm;
+  sm.slice_off.upload(off, s);
+  sm.slice_w.upload(w, s);
+  sm.sell_perm.upload(sperm, s);
+  sm.sell_col.upload(scol, s);
+  sm.sell_vals.ensure(9 * std::max<size_t>(P, 1));
+  sm.uniform_w = w[0];
+  for (int sl = 1; sl < S; ++sl)
+    if (w[sl] != w[0]) sm.uniform_w = 0;
+  sm.slices = S;
+  sm.slots = (long long)P;
+  sm.sell_ok = true;
+  T.n = n;
+  T.nb = row_ptr[n];
+  T.pr = patch_rows(n);
+  T.npatch = (n + T.pr - 1) / T.pr;
+  T.perm.upload(perm, s);
+  T.iperm.upload(iperm, s);
+  // Coarse pattern: the fine blocks of each patch pair (I, J), in block order.
+  T.nc = 0;
+  T.ncb = 0;
+  if (T.npatch <= kTlMaxCoarsePatches) {
+    const long long nb = row_ptr[n];
+    std::vector<std::pair<long long, int32_t>> kb((size_t)nb);
+    for (int o = 0; o < n; ++o)
+      for (int b = row_ptr[o]; b < row_ptr[o + 1]; ++b)
+        kb[(size_t)b] = {(long long)(iperm[o] / T.pr) * T.npatch + iperm[cols[b]] / T.pr, b};
+    std::sort(kb.begin(), kb.end());
+    std::vector<int32_t> ij, ptr, blk((size_t)nb);
+    for (size_t t = 0; t < kb.size(); ++t) {
+      if (t == 0 || kb[t].first != kb[t - 1].first) {
+        ij.push_back((int32_t)(kb[t].first / T.npatch));
+        ij.push_back((int32_t)(kb[t].first % T.npatch));
+        ptr.push_back((int32_t)t);
+      }
+      blk[t] = kb[t].second;
+    }
+    ptr.push_back((int32_t)nb);
+    T.ncb = (int)(ptr.size() - 1);
+    T.cb_ij.upload(ij, s);
+    T.cb_ptr.upload(ptr, s);
+    T.cb_blk.upload(blk, s);
+    T.nc = 3 * T.npatch;
+    for (int k = 0; k < 2; ++k) {
+      T.Ac[k].ensure((size_t)T.nc * T.nc);
+      T.Acinv[k].ensure((size_t)T.nc * T.nc);
+      T.info[k].ensure(2);
+    }
+  }
+  const size_t n3 = 3 * (size_t)n;
+  T.x.ensure(n3); T.r.ensure(n3); T.z.ensure(n3); T.p0.ensure(n3); T.p1.ensure(n3); T.Ap.ensure(n3);
+  T.qf.ensure(n3);
+  T.Dinv.ensure(9 * (size_t)n);
+  T.sa.ensure(3 * (size_t)T.npatch);
+  T.sr0.ensure(3 * (size_t)T.npatch);
+  T.sr1.ensure(3 * (size_t)T.npatch);
+  T.lmax.ensure((size_t)T.npatch);
+  T.part_rz.ensure((size_t)T.npatch + 1);
+  T.part_pAp.ensure((size_t)T.npatch + 1);
+  T.part_max.ensure((size_t)T.npatch + 1);
+  T.have = T.pending = false;
+  T.cur = 0;
+  T.ok = true;
+}
+
 #include "learned_host.hip"
 
 // One linear BVH (triangles or edges) and its build scratch.
@@ -333,7 +496,25 @@ struct ando_ctx {
   DBuf<float> fx_out;
   uint64_t det_gen = 0, listed_gen = ~0ull;
   std::vector<hipEvent_t> events;  // pool: start/stop pairs for timed launches
+  // Two-level Schwarz PCG (ANDO_MODE_TWO_LEVEL; tl.hip).
+  int two_level = 0;
+  int tl_cheb = 5;          // ANDO_TL_CHEB: Chebyshev degree of the patch solves (odd)
+  float tl_ratio = 30.0f;   // ANDO_TL_RATIO: lmax / lmin of the Chebyshev interval
+  int tl_coarse = 1;        // ANDO_TL_COARSE=0: no coarse level (patch polynomials only)
+  int tl_coarse_every = 0;  // ANDO_TL_COARSE_EVERY=1: every solve inverts its own coarse operator first
+  std::vector<int32_t> morton;  // the mesh's vertices in Morton order of the rest positions
+  TlMeta tl;                // for the mesh's block pattern
+  rocblas_handle rb = nullptr;
+  DBuf<uint8_t> rb_ws;      // its workspace, sized for coarse problems of rb_ws_nc unknowns
+  int rb_ws_nc = -1;
+  hipStream_t tl_stream = nullptr;  // coarse inversions for the next solve
+  hipEvent_t tl_ready = nullptr, tl_done = nullptr;
   ~ando_ctx() {
+    if (tl_stream) (void)hipStreamSynchronize(tl_stream);
+    if (rb) (void)rocblas_destroy_handle(rb);
+    if (tl_ready) (void)hipEventDestroy(tl_ready);
+    if (tl_done) (void)hipEventDestroy(tl_done);
+    if (tl_stream) (void)hipStreamDestroy(tl_stream);
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     if (status_pinned) (void)hipHostFree(status_pinned);
     if (sact_pinned) (void)hipHostFree(sact_pinned);
@@ -1443,6 +1624,205 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
   return PcgOut{st.iters, !st.failed};
 }
 
+// Two-level Schwarz PCG (tl.hip) on the matrix A with pattern T; same
+// contract as run_pcg with x0 = 0 (b = -g when negate), the result in xsol.
+PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, const float* rhs, int negate,
+                  float* xsol, float tol, int maxit, const ando::SelfView* sv) {
+  using namespace ando::tl;
+  hipStream_t s = c->stream;
+  const int n = T.n;
+  SpmvMeta& sm = T.sm;
+  c->stats.pcg_kernel = 4;
+  c->stats.spmv_lanes = kLanes;
+  c->stats.spmv_wg_slices = 0;
+  c->stats.pcg_vec_block = T.pr;
+  c->stats.pcg_autotuned = 0;
+  c->rz_hist.ensure((size_t)maxit + 2);
+  c->pcg_status.ensure(1);
+  hipLaunchKernelGGL(k_sell_fill, dim3(nblk(sm.slices, kBlock / 64)), dim3(kBlock), 0, s, sm.slices,
+                     (const int32_t*)sm.slice_off.p, (const int32_t*)sm.slice_w.p, (const int32_t*)sm.sell_perm.p, A,
+                     sm.sell_vals.p);
+  // Coarse operator Ac = P^T A P and its inverse (Cholesky-based, FP64).
+  // The first solve inverts its own Ac before it starts; later solves use the
+  // inverse of the previous solve's system (made on the side stream while that
+  // solve ran) and hand their own Ac to the side stream for the next solve.
+  // Any symmetric positive definite Ac gives a symmetric positive definite M,
+  // so an inverse one Newton iteration old changes only the iteration count.
+  const int nc = c->tl_coarse ? T.nc : 0;
+  if (nc > 0) {
+    const size_t nn = (size_t)nc * nc;
+    if (!c->rb) {
+      if (rocblas_create_handle(&c->rb) != rocblas_status_success) throw Fail(ANDO_EDEVICE, "rocblas_create_handle failed");
+    }
+    // A fixed workspace for the factorisation and inversion: without one the
+    // library allocates and frees device memory in every call, and each free
+    // synchronises the device.
+    if (c->rb_ws_nc != nc) {
+      size_t need_bytes = 0;
+      if (rocblas_start_device_memory_size_query(c->rb) != rocblas_status_success)
+        throw Fail(ANDO_EDEVICE, "rocblas workspace query failed");
+      (void)rocsolver_dpotrf(c->rb, rocblas_fill_lower, nc, nullptr, nc, nullptr);
+      (void)rocsolver_dpotri(c->rb, rocblas_fill_lower, nc, nullptr, nc, nullptr);
+      if (rocblas_stop_device_memory_size_query(c->rb, &need_bytes) != rocblas_status_success)
+        throw Fail(ANDO_EDEVICE, "rocblas workspace query failed");
+      if (c->tl_stream) HIPCHK(hipStreamSynchronize(c->tl_stream));
+      c->rb_ws.ensure(std::max<size_t>(need_bytes, 256));
+      if (rocblas_set_workspace(c->rb, c->rb_ws.p, c->rb_ws.n) != rocblas_status_success)
+        throw Fail(ANDO_EDEVICE, "rocblas_set_workspace failed");
+      c->rb_ws_nc = nc;
+    }
+    auto assemble = [&](int k) {
+      hipLaunchKernelGGL(k_tl_zero, dim3(nblk((long long)nn)), dim3(kBlock), 0, s, nn, T.Ac[k].p);
+      hipLaunchKernelGGL(k_tl_coarse, dim3(nblk(64LL * T.ncb)), dim3(kBlock), 0, s, T.ncb, nc,
+                         (const int32_t*)T.cb_ij.p, (const int32_t*)T.cb_ptr.p, (const int32_t*)T.cb_blk.p, A,
+                         T.Ac[k].p);
+      HIPCHK(hipGetLastError());
+    };
+    auto invert = [&](int k, hipStream_t on) {
+      if (rocblas_set_stream(c->rb, on) != rocblas_status_success) throw Fail(ANDO_EDEVICE, "rocblas_set_stream failed");
+      if (rocsolver_dpotrf(c->rb, rocblas_fill_lower, nc, T.Ac[k].p, nc, T.info[k].p) != rocblas_status_success)
+        throw Fail(ANDO_EDEVICE, "rocsolver_dpotrf failed");
+      if (rocsolver_dpotri(c->rb, rocblas_fill_lower, nc, T.Ac[k].p, nc, T.info[k].p + 1) != rocblas_status_success)
+        throw Fail(ANDO_EDEVICE, "rocsolver_dpotri failed");
+      hipLaunchKernelGGL(k_tl_cinv, dim3(nblk((long long)nn)), dim3(kBlock), 0, on, nc, (const double*)T.Ac[k].p,
+                         (const int*)T.info[k].p, T.Acinv[k].p);
+      HIPCHK(hipGetLastError());
+    };
+    if (!T.have || c->tl_coarse_every) {
+      if (T.pending) HIPCHK(hipStreamWaitEvent(s, c->tl_done, 0));
+      T.pending = false;
+      T.cur = 0;
+      assemble(0);
+      invert(0, s);
+      T.have = true;
+    } else {
+      if (!c->tl_stream) {
+        HIPCHK(hipStreamCreateWithFlags(&c->tl_stream, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c->tl_ready, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->tl_done, hipEventDisableTiming));
+      }
+      if (T.pending) {
+        HIPCHK(hipStreamWaitEvent(s, c->tl_done, 0));
+        T.cur = 1 - T.cur;
+        T.pending = false;
+      }
+      const int nxt = 1 - T.cur;
+      assemble(nxt);
+      HIPCHK(hipEventRecord(c->tl_ready, s));
+      HIPCHK(hipStreamWaitEvent(c->tl_stream, c->tl_ready, 0));
+      invert(nxt, c->tl_stream);
+      HIPCHK(hipEventRecord(c->tl_done, c->tl_stream));
+      T.pending = true;
+    }
+  }
+  const bool couple = sv != nullptr && sv->ptr != nullptr && c->ncblk > 0;
+  TlArgs a{};
+  a.n = n;
+  a.pr = T.pr;
+  a.npatch = T.npatch;
+  a.slices = sm.slices;
+  a.uniform_w = sm.uniform_w;
+  a.nc = nc;
+  a.cheb = c->tl_cheb;
+  a.lmin_ratio = c->tl_ratio;
+  a.perm = T.perm.p;
+  a.iperm = T.iperm.p;
+  a.vals = sm.sell_vals.p;
+  a.scol = sm.sell_col.p;
+  a.slice_off = sm.slice_off.p;
+  a.slice_w = sm.slice_w.p;
+  a.Dinv = T.Dinv.p;
+  a.lmax = T.lmax.p;
+  a.Acinv = T.Acinv[T.cur].p;
+  a.cptr = couple ? sv->ptr : nullptr;
+  a.ccol = couple ? c->ccol.p : nullptr;
+  a.cval = couple ? c->cval.p : nullptr;
+  hipLaunchKernelGGL(k_tl_init, dim3(nblk(n)), dim3(kBlock), 0, s, a, A, diag, rhs, negate, T.x.p, T.r.p, T.Dinv.p);
+  hipLaunchKernelGGL(k_tl_prep, dim3(T.npatch), dim3(patch_threads(T.pr)), 0, s, a, T.lmax.p, (const float*)T.r.p, T.qf.p,
+                     T.sr0.p);
+  hipLaunchKernelGGL(k_tl_start, dim3(T.npatch), dim3(patch_threads(T.pr)), 0, s, a, (const float*)T.r.p, (const float*)T.qf.p,
+                     (const double*)T.sr0.p, T.z.p, T.p0.p, T.part_rz.p, T.part_max.p);
+  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, s, T.npatch, (const double*)T.part_rz.p,
+                     (const float*)T.part_max.p, (const float*)nullptr, c->rz_hist.p, c->pcg_status.p, tol);
+  HIPCHK(hipGetLastError());
+  PcgStatus st{};
+  int it = 0;
+  int chunk = std::max(1, c->pcg_chunk);
+  const bool timing = c->kernel_timing != 0;
+  size_t ev_used = 0;
+  auto ev = [&](size_t k) -> hipEvent_t {
+    while (c->events.size() <= k) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      c->events.push_back(e);
+    }
+    return c->events[k];
+  };
+  auto launch_chunk = [&](int count) {
+    const int end = std::min(maxit, it + count);
+    for (; it < end; ++it) {
+      float* pold = (it == 0) ? T.p0.p : ((it - 1) & 1 ? T.p1.p : T.p0.p);
+      float* pcur = (it & 1) ? T.p1.p : T.p0.p;
+      hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+      if (timing) {
+        e0 = ev(ev_used); e1 = ev(ev_used + 1); e2 = ev(ev_used + 2); e3 = ev(ev_used + 3);
+        ev_used += 4;
+      }
+      double* sr_cur = (it & 1) ? T.sr1.p : T.sr0.p;
+      double* sr_next = (it & 1) ? T.sr0.p : T.sr1.p;
+      hipExtLaunchKernelGGL(k_tl_spmv, dim3(T.npatch), dim3(spmv_threads(T.pr)), 0, s, e0, e1, 0, a, it,
+                            (const float*)T.z.p, (const float*)pold, pcur, T.Ap.p, T.sa.p,
+                            (const double*)T.part_rz.p, (const float*)T.part_max.p, c->rz_hist.p, T.part_pAp.p,
+                            c->pcg_status.p, tol);
+      hipExtLaunchKernelGGL(k_tl_update, dim3(T.npatch), dim3(patch_threads(T.pr)), 0, s, e2, e3, 0, a, it, (const float*)pcur,
+                            (const float*)T.Ap.p, (const double*)sr_cur, (const double*)T.sa.p, sr_next, T.x.p,
+                            T.r.p, T.z.p, (const double*)T.part_pAp.p, (const double*)c->rz_hist.p, T.part_rz.p,
+                            T.part_max.p, c->pcg_status.p);
+    }
+    HIPCHK(hipGetLastError());
+  };
+  for (;;) {
+    launch_chunk(chunk);
+    HIPCHK(hipMemcpyAsync(c->status_pinned, c->pcg_status.p, sizeof(PcgStatus), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(c->status_ev, s));
+    HIPCHK(hipEventSynchronize(c->status_ev));
+    st = *c->status_pinned;
+    if (st.done) break;
+    if (it >= maxit) {
+      hipLaunchKernelGGL(k_pcg_final, dim3(1), dim3(kBlock), 0, s, it, T.part_max.p, T.npatch, c->pcg_status.p, tol);
+      HIPCHK(hipMemcpyAsync(c->status_pinned, c->pcg_status.p, sizeof(PcgStatus), hipMemcpyDeviceToHost, s));
+      sync(c);
+      st = *c->status_pinned;
+      break;
+    }
+    chunk = std::min(chunk * 2, c->pcg_chunk_max);
+  }
+  hipLaunchKernelGGL(k_tl_scatter, dim3(nblk(n)), dim3(kBlock), 0, s, n, (const int32_t*)T.perm.p,
+                     (const float*)T.x.p, xsol);
+  HIPCHK(hipGetLastError());
+  if (timing) {
+    sync(c);
+    for (size_t k = 0; 4 * k + 3 < ev_used && (int)k < st.iters; ++k) {
+      float ta = 0.0f, tb = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ta, c->events[4 * k], c->events[4 * k + 1]));
+      HIPCHK(hipEventElapsedTime(&tb, c->events[4 * k + 2], c->events[4 * k + 3]));
+      c->stats.timed_launches[0]++;
+      c->stats.timed_launches[1]++;
+      c->stats.timed_ms[0] += ta;
+      c->stats.timed_ms[1] += tb;
+      if (couple) c->stats.timed_coupling_bytes += 40.0 * c->ncblk;
+    }
+  }
+  return PcgOut{st.iters, !st.failed};
+}
+
+// The mesh pattern's two-level layout, built on first use.
+TlMeta& mesh_tl(ando_ctx* c) {
+  if (!c->tl.ok) build_tl_meta(c->tl, c->n, c->row_ptr, c->cols, c->morton, c->stream);
+  if (!c->tl.ok) throw Fail(ANDO_EDEVICE, "two-level PCG: sliced layout too large");
+  return c->tl;
+}
+
 // Launch-shape autotuner (ANDO_PCG_AUTOTUNE).  On the first sliced-ELL solve of
 // a system size, a short trial solve (32 iterations, tolerance 0) runs with each
 // candidate shape -- lanes per row, slices of 64 rows per SpMV workgroup, threads
@@ -1793,6 +2173,8 @@ float inner_newton(ando_ctx* c, const ando_params& p) {
         c->stats.fallback_iterations += po.iters;
         po.iters += first;
       }
+    } else if (c->two_level) {
+      po = run_pcg_tl(c, mesh_tl(c), c->H.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol, p.pcg_max_iters, &sv);
     } else {
       po = run_pcg(c, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol,
                    p.pcg_max_iters, &c->spmv_meta, nullptr, nullptr, &sv);
@@ -2016,6 +2398,13 @@ int ando_create(int32_t device, ando_ctx** out) {
       c->sell_lanes = (l == 1 || l == 4) ? l : 2;
     }
     if (const char* w = std::getenv("ANDO_PCG_AUTOTUNE")) c->pcg_autotune = std::min(2, std::max(0, std::atoi(w)));
+    if (const char* w = std::getenv("ANDO_TL_CHEB")) {
+      const int k = std::min(ando::tl::kMaxCheb, std::max(1, std::atoi(w)));
+      c->tl_cheb = (k % 2) ? k : k + 1;  // odd degrees keep the patch polynomial positive
+    }
+    if (const char* w = std::getenv("ANDO_TL_RATIO")) c->tl_ratio = std::max(1.5f, (float)std::atof(w));
+    if (const char* w = std::getenv("ANDO_TL_COARSE")) c->tl_coarse = std::atoi(w) != 0 ? 1 : 0;
+    if (const char* w = std::getenv("ANDO_TL_COARSE_EVERY")) c->tl_coarse_every = std::atoi(w) != 0 ? 1 : 0;
     if (const char* w = std::getenv("ANDO_PCG_ROWSETS")) c->spmv_rowsets = std::min(4, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_SPMV")) {
       c->spmv_env = std::strcmp(w, "sell") == 0 ? 2 : 0;
@@ -2055,6 +2444,10 @@ int ando_set_mesh(ando_ctx* c, const float* vertices, int32_t n, const int32_t* 
     build_topology(c, rs.edges);
     c->edges_h = rs.edges;
     c->d_edges.upload(c->edges_h, c->stream);
+    c->morton = morton_order(vertices, n);
+    if (c->tl_stream) HIPCHK(hipStreamSynchronize(c->tl_stream));  // a pending coarse inversion
+    c->tl.ok = false;
+    c->tl.pending = false;
     {
       double sum = 0.0;
       const size_t E = c->edges_h.size() / 2;
@@ -2251,7 +2644,7 @@ int ando_set_mode(ando_ctx* c, int32_t flags) {
     need((flags & ~(ANDO_MODE_FP64_NEAR_CONTACT | ANDO_MODE_KERNEL_TIMING |
                     ANDO_MODE_LEARNED_PRECOND | ANDO_MODE_LEARNED_WARM_START_ONLY |
                     ANDO_MODE_CORRECTED_CONTACT | ANDO_MODE_PCG_ROW_SPMV | ANDO_MODE_STATIC_RIGIDS |
-                    ANDO_MODE_ADDITIVE_CCD | ANDO_MODE_CORRECTED_PHYSICS)) == 0,
+                    ANDO_MODE_ADDITIVE_CCD | ANDO_MODE_CORRECTED_PHYSICS | ANDO_MODE_TWO_LEVEL)) == 0,
          ANDO_EINVAL, "unknown mode flag");
     need(!(flags & ANDO_MODE_LEARNED_PRECOND) || c->nn.ready, ANDO_ESTATE,
          "learned preconditioner mode needs ando_set_precond_weights first");
@@ -2268,6 +2661,7 @@ int ando_set_mode(ando_ctx* c, int32_t flags) {
     if (c->spmv_env < 0) c->spmv_sell = (flags & ANDO_MODE_PCG_ROW_SPMV) ? 0 : 1;
     c->static_rigids = (flags & ANDO_MODE_STATIC_RIGIDS) ? 1 : 0;
     c->additive_ccd = (flags & ANDO_MODE_ADDITIVE_CCD) ? 1 : 0;
+    c->two_level = (flags & ANDO_MODE_TWO_LEVEL) ? 1 : 0;
   });
 }
 
@@ -2515,9 +2909,26 @@ int ando_pcg(ando_ctx* c, int32_t num_rows, const int32_t* row_ptr, const int32_
       c->Pinv.ensure(9 * (size_t)n);
       c->part_rz.ensure(nblk(n) + 1); c->part_max.ensure(nblk(n) + 1); c->part_pAp.ensure(spmv_groups(n) + 1);
     }
-    SpmvMeta sm;
-    build_spmv_meta(sm, n, std::vector<int32_t>(row_ptr, row_ptr + n + 1), std::vector<int32_t>(cols, cols + nbk), s);
-    PcgOut po = run_pcg(c, n, dv.p, drp.p, dci.p, ddg.p, db.p, 0, dx.p, tol, max_iters, &sm);
+    PcgOut po{0, false};
+    if (c->two_level) {
+      // Rows in the Morton order of the context's mesh when the system has its
+      // size, otherwise in the given order.
+      std::vector<int32_t> order(n);
+      if (c->have_mesh && n == c->n) order = c->morton;
+      else
+        for (int i = 0; i < n; ++i) order[i] = i;
+      TlMeta T;
+      build_tl_meta(T, n, std::vector<int32_t>(row_ptr, row_ptr + n + 1), std::vector<int32_t>(cols, cols + nbk),
+                    order, s);
+      need(T.ok, ANDO_EINVAL, "two-level PCG: sliced layout too large");
+      po = run_pcg_tl(c, T, dv.p, ddg.p, db.p, 0, dx.p, tol, max_iters, nullptr);
+      sync(c);  // T's buffers are released on return
+    } else {
+      SpmvMeta sm;
+      build_spmv_meta(sm, n, std::vector<int32_t>(row_ptr, row_ptr + n + 1), std::vector<int32_t>(cols, cols + nbk),
+                      s);
+      po = run_pcg(c, n, dv.p, drp.p, dci.p, ddg.p, db.p, 0, dx.p, tol, max_iters, &sm);
+    }
     HIPCHK(hipMemcpyAsync(x, dx.p, 3 * sizeof(float) * (size_t)n, hipMemcpyDeviceToHost, s));
     sync(c);
     if (iterations) *iterations = po.iters;

@@ -338,11 +338,11 @@ class GpuSim:
     def set_mode(self, fp64_near: bool = True, kernel_timing: bool = False,
                  learned: bool = False, warm_start_only: bool = False, corrected: bool = False,
                  row_spmv: bool = False, static_rigids: bool = False, additive_ccd: bool = False,
-                 corrected_physics: bool = False):
+                 corrected_physics: bool = False, two_level: bool = False):
         flags = ((1 if fp64_near else 0) | (2 if kernel_timing else 0) |
                  (8 if learned else 0) | (16 if learned and warm_start_only else 0) | (32 if corrected else 0) |
                  (64 if row_spmv else 0) | (128 if static_rigids else 0) | (256 if additive_ccd else 0) |
-                 (512 if corrected_physics else 0))
+                 (512 if corrected_physics else 0) | (1024 if two_level else 0))
         check(self.L.ando_set_mode(self.h, flags))
 
     def apply_gravity(self, g=None, dt=None):

@@ -149,7 +149,8 @@ typedef struct {
   double timed_ms[2];               /* summed event durations of those launches */
   int32_t timed_iterations;         /* reserved (0) */
   int32_t pcg_kernel;               /* two kernels per iteration with 0: the BSR row SpMV,
-                                       3: the sliced-ELL SpMV (1 and 2 are not used) */
+                                       3: the sliced-ELL SpMV, 4: the two-level patch kernels
+                                       (ANDO_MODE_TWO_LEVEL; 1 and 2 are not used) */
   int32_t learned_solves;           /* solves that used the learned preconditioner */
   int32_t learned_fallbacks;        /* of those, solves that fell back to plain PCG */
   int32_t fallback_iterations;      /* plain PCG iterations spent in fallbacks (included in pcg_iterations) */
@@ -256,6 +257,17 @@ int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, co
  *     from the cloth.
  * Setting or clearing the flag resets the cached step constants. */
 #define ANDO_MODE_CORRECTED_PHYSICS 512
+/* ANDO_MODE_TWO_LEVEL (new): the step's Newton systems (and ando_pcg) are
+ * solved by CG with a two-level Schwarz preconditioner instead of block-Jacobi
+ * (pcg_solver.cpp:8-146): the block rows in Morton order of the rest positions,
+ * cut into patches of 256; per patch a fixed odd-degree Chebyshev polynomial in
+ * D^-1 A_pp (block-Jacobi scaling, the patch's own blocks; degree
+ * ANDO_TL_CHEB, default 9), plus a coarse correction with three unknowns per
+ * patch (A_c = P^T A P, inverted once per solve; for up to 1024 patches).  Same
+ * stopping test (max-norm relative residual, |p.Ap| < 1e-16 breakdown, the
+ * iteration cap), far fewer iterations on cloth systems.  ando_stats.pcg_kernel
+ * reads 4. */
+#define ANDO_MODE_TWO_LEVEL 1024
 int ando_set_mode(ando_ctx* ctx, int32_t flags);
 
 int ando_apply_gravity(ando_ctx* ctx, const float* gravity3, float dt);
