@@ -45,8 +45,7 @@ def _run(cmd):
 def build_lib():
     src = os.path.join(CSRC, "ctx.hip")
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT, "include"), "-o", tmp, src,
-          "-L", os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas", f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}"])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT, "include"), "-o", tmp, src])
     os.replace(tmp, LIB)
     return LIB
 

@@ -4,7 +4,6 @@
 // runs as a sequence of kernel launches; the host only reads back the scalars
 // that steer control flow (gradient max-norm, line-search mask, PCG status,
 // contact totals).
-#include <rocsolver/rocsolver.h>
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
@@ -205,19 +204,15 @@ void build_spmv_meta(SpmvMeta& sm, int n, const std::vector<int32_t>& row_ptr, c
 // the solve's buffers.
 struct TlMeta {
   bool ok = false;
-  // Coarse inverses, double-buffered: Acinv[cur] is used by the solves; the
-  // inverse of the latest system is made into the other buffer on the side
-  // stream (pending until the next solve takes it).
-  bool have = false, pending = false;
-  int cur = 0;
   int n = 0, pr = 256, npatch = 0, nc = 0, ncb = 0;
   long long nb = 0;
   DBuf<int32_t> perm, iperm;
   SpmvMeta sm;  // sell_col holds renumbered columns
   DBuf<int32_t> cb_ij, cb_ptr, cb_blk;
-  DBuf<double> Ac[2];
-  DBuf<float> Acinv[2];
-  DBuf<int32_t> info[2];
+  int ncp = 0;                // nc padded to a multiple of the inversion tile
+  DBuf<double> Ac, Acw, gjP;  // coarse operator, inversion work buffer, pivot block
+  DBuf<float> Acinv;
+  DBuf<int32_t> info;         // inversion failed
   DBuf<float> Dinv, lmax, x, r, z, p0, p1, Ap, qf;
   DBuf<double> sa, sr0, sr1;  // per patch: sums of A p and of r (3 each)
   DBuf<double> part_rz, part_pAp;
@@ -338,11 +333,12 @@ void build_tl_meta(TlMeta& T, int n, const std::vector<int32_t>& row_ptr, const 
     T.cb_ptr.upload(ptr, s);
     T.cb_blk.upload(blk, s);
     T.nc = 3 * T.npatch;
-    for (int k = 0; k < 2; ++k) {
-      T.Ac[k].ensure((size_t)T.nc * T.nc);
-      T.Acinv[k].ensure((size_t)T.nc * T.nc);
-      T.info[k].ensure(2);
-    }
+    T.ncp = ando::tl::kGjB * ((T.nc + ando::tl::kGjB - 1) / ando::tl::kGjB);
+    T.Ac.ensure((size_t)T.ncp * T.ncp);
+    T.Acw.ensure((size_t)T.ncp * T.ncp);
+    T.gjP.ensure((size_t)ando::tl::kGjB * ando::tl::kGjB);
+    T.Acinv.ensure((size_t)T.nc * T.nc);
+    T.info.ensure(1);
   }
   const size_t n3 = 3 * (size_t)n;
   T.x.ensure(n3); T.r.ensure(n3); T.z.ensure(n3); T.p0.ensure(n3); T.p1.ensure(n3); T.Ap.ensure(n3);
@@ -355,8 +351,6 @@ void build_tl_meta(TlMeta& T, int n, const std::vector<int32_t>& row_ptr, const 
   T.part_rz.ensure((size_t)T.npatch + 1);
   T.part_pAp.ensure((size_t)T.npatch + 1);
   T.part_max.ensure((size_t)T.npatch + 1);
-  T.have = T.pending = false;
-  T.cur = 0;
   T.ok = true;
 }
 
@@ -501,20 +495,9 @@ struct ando_ctx {
   int tl_cheb = 5;          // ANDO_TL_CHEB: Chebyshev degree of the patch solves (odd)
   float tl_ratio = 30.0f;   // ANDO_TL_RATIO: lmax / lmin of the Chebyshev interval
   int tl_coarse = 1;        // ANDO_TL_COARSE=0: no coarse level (patch polynomials only)
-  int tl_coarse_every = 0;  // ANDO_TL_COARSE_EVERY=1: every solve inverts its own coarse operator first
   std::vector<int32_t> morton;  // the mesh's vertices in Morton order of the rest positions
   TlMeta tl;                // for the mesh's block pattern
-  rocblas_handle rb = nullptr;
-  DBuf<uint8_t> rb_ws;      // its workspace, sized for coarse problems of rb_ws_nc unknowns
-  int rb_ws_nc = -1;
-  hipStream_t tl_stream = nullptr;  // coarse inversions for the next solve
-  hipEvent_t tl_ready = nullptr, tl_done = nullptr;
   ~ando_ctx() {
-    if (tl_stream) (void)hipStreamSynchronize(tl_stream);
-    if (rb) (void)rocblas_destroy_handle(rb);
-    if (tl_ready) (void)hipEventDestroy(tl_ready);
-    if (tl_done) (void)hipEventDestroy(tl_done);
-    if (tl_stream) (void)hipStreamDestroy(tl_stream);
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     if (status_pinned) (void)hipHostFree(status_pinned);
     if (sact_pinned) (void)hipHostFree(sact_pinned);
@@ -1642,78 +1625,30 @@ PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, c
   hipLaunchKernelGGL(k_sell_fill, dim3(nblk(sm.slices, kBlock / 64)), dim3(kBlock), 0, s, sm.slices,
                      (const int32_t*)sm.slice_off.p, (const int32_t*)sm.slice_w.p, (const int32_t*)sm.sell_perm.p, A,
                      sm.sell_vals.p);
-  // Coarse operator Ac = P^T A P and its inverse (Cholesky-based, FP64).
-  // The first solve inverts its own Ac before it starts; later solves use the
-  // inverse of the previous solve's system (made on the side stream while that
-  // solve ran) and hand their own Ac to the side stream for the next solve.
-  // Any symmetric positive definite Ac gives a symmetric positive definite M,
-  // so an inverse one Newton iteration old changes only the iteration count.
+  // Coarse operator Ac = P^T A P and its inverse (blocked Gauss-Jordan, FP64),
+  // made for every solve before its iterations start.
   const int nc = c->tl_coarse ? T.nc : 0;
   if (nc > 0) {
     const size_t nn = (size_t)nc * nc;
-    if (!c->rb) {
-      if (rocblas_create_handle(&c->rb) != rocblas_status_success) throw Fail(ANDO_EDEVICE, "rocblas_create_handle failed");
+    const int ncp = T.ncp, nbt = ncp / kGjB;
+    const size_t nnp = (size_t)ncp * ncp;
+    hipLaunchKernelGGL(k_tl_zero, dim3(nblk((long long)nnp)), dim3(kBlock), 0, s, nc, ncp, T.Ac.p, T.info.p);
+    hipLaunchKernelGGL(k_tl_coarse, dim3(nblk(64LL * T.ncb)), dim3(kBlock), 0, s, T.ncb, ncp,
+                       (const int32_t*)T.cb_ij.p, (const int32_t*)T.cb_ptr.p, (const int32_t*)T.cb_blk.p, A, T.Ac.p);
+    // Blocked Gauss-Jordan (tl.hip): nbt steps of a pivot-block inverse and a
+    // tile update, ping-ponging between Ac and Acw; then the FP32 copy.  The
+    // failure flag was cleared by k_tl_zero.
+    double* X = T.Ac.p;
+    double* Y = T.Acw.p;
+    for (int q = 0; q < nbt; ++q) {
+      hipLaunchKernelGGL(k_tl_gj_pivot, dim3(1), dim3(kBlock), 0, s, ncp, q, (const double*)X, T.gjP.p, T.info.p);
+      hipLaunchKernelGGL(k_tl_gj_update, dim3(nbt * nbt), dim3(kBlock), 0, s, nbt, ncp, q, (const double*)X,
+                         (const double*)T.gjP.p, Y, (const int*)T.info.p);
+      std::swap(X, Y);
     }
-    // A fixed workspace for the factorisation and inversion: without one the
-    // library allocates and frees device memory in every call, and each free
-    // synchronises the device.
-    if (c->rb_ws_nc != nc) {
-      size_t need_bytes = 0;
-      if (rocblas_start_device_memory_size_query(c->rb) != rocblas_status_success)
-        throw Fail(ANDO_EDEVICE, "rocblas workspace query failed");
-      (void)rocsolver_dpotrf(c->rb, rocblas_fill_lower, nc, nullptr, nc, nullptr);
-      (void)rocsolver_dpotri(c->rb, rocblas_fill_lower, nc, nullptr, nc, nullptr);
-      if (rocblas_stop_device_memory_size_query(c->rb, &need_bytes) != rocblas_status_success)
-        throw Fail(ANDO_EDEVICE, "rocblas workspace query failed");
-      if (c->tl_stream) HIPCHK(hipStreamSynchronize(c->tl_stream));
-      c->rb_ws.ensure(std::max<size_t>(need_bytes, 256));
-      if (rocblas_set_workspace(c->rb, c->rb_ws.p, c->rb_ws.n) != rocblas_status_success)
-        throw Fail(ANDO_EDEVICE, "rocblas_set_workspace failed");
-      c->rb_ws_nc = nc;
-    }
-    auto assemble = [&](int k) {
-      hipLaunchKernelGGL(k_tl_zero, dim3(nblk((long long)nn)), dim3(kBlock), 0, s, nn, T.Ac[k].p);
-      hipLaunchKernelGGL(k_tl_coarse, dim3(nblk(64LL * T.ncb)), dim3(kBlock), 0, s, T.ncb, nc,
-                         (const int32_t*)T.cb_ij.p, (const int32_t*)T.cb_ptr.p, (const int32_t*)T.cb_blk.p, A,
-                         T.Ac[k].p);
-      HIPCHK(hipGetLastError());
-    };
-    auto invert = [&](int k, hipStream_t on) {
-      if (rocblas_set_stream(c->rb, on) != rocblas_status_success) throw Fail(ANDO_EDEVICE, "rocblas_set_stream failed");
-      if (rocsolver_dpotrf(c->rb, rocblas_fill_lower, nc, T.Ac[k].p, nc, T.info[k].p) != rocblas_status_success)
-        throw Fail(ANDO_EDEVICE, "rocsolver_dpotrf failed");
-      if (rocsolver_dpotri(c->rb, rocblas_fill_lower, nc, T.Ac[k].p, nc, T.info[k].p + 1) != rocblas_status_success)
-        throw Fail(ANDO_EDEVICE, "rocsolver_dpotri failed");
-      hipLaunchKernelGGL(k_tl_cinv, dim3(nblk((long long)nn)), dim3(kBlock), 0, on, nc, (const double*)T.Ac[k].p,
-                         (const int*)T.info[k].p, T.Acinv[k].p);
-      HIPCHK(hipGetLastError());
-    };
-    if (!T.have || c->tl_coarse_every) {
-      if (T.pending) HIPCHK(hipStreamWaitEvent(s, c->tl_done, 0));
-      T.pending = false;
-      T.cur = 0;
-      assemble(0);
-      invert(0, s);
-      T.have = true;
-    } else {
-      if (!c->tl_stream) {
-        HIPCHK(hipStreamCreateWithFlags(&c->tl_stream, hipStreamNonBlocking));
-        HIPCHK(hipEventCreateWithFlags(&c->tl_ready, hipEventDisableTiming));
-        HIPCHK(hipEventCreateWithFlags(&c->tl_done, hipEventDisableTiming));
-      }
-      if (T.pending) {
-        HIPCHK(hipStreamWaitEvent(s, c->tl_done, 0));
-        T.cur = 1 - T.cur;
-        T.pending = false;
-      }
-      const int nxt = 1 - T.cur;
-      assemble(nxt);
-      HIPCHK(hipEventRecord(c->tl_ready, s));
-      HIPCHK(hipStreamWaitEvent(c->tl_stream, c->tl_ready, 0));
-      invert(nxt, c->tl_stream);
-      HIPCHK(hipEventRecord(c->tl_done, c->tl_stream));
-      T.pending = true;
-    }
+    hipLaunchKernelGGL(k_tl_cinv, dim3(nblk((long long)nn)), dim3(kBlock), 0, s, nc, ncp, (const double*)X,
+                       (const int*)T.info.p, T.Acinv.p);
+    HIPCHK(hipGetLastError());
   }
   const bool couple = sv != nullptr && sv->ptr != nullptr && c->ncblk > 0;
   TlArgs a{};
@@ -1733,7 +1668,7 @@ PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, c
   a.slice_w = sm.slice_w.p;
   a.Dinv = T.Dinv.p;
   a.lmax = T.lmax.p;
-  a.Acinv = T.Acinv[T.cur].p;
+  a.Acinv = T.Acinv.p;
   a.cptr = couple ? sv->ptr : nullptr;
   a.ccol = couple ? c->ccol.p : nullptr;
   a.cval = couple ? c->cval.p : nullptr;
@@ -2404,7 +2339,6 @@ int ando_create(int32_t device, ando_ctx** out) {
     }
     if (const char* w = std::getenv("ANDO_TL_RATIO")) c->tl_ratio = std::max(1.5f, (float)std::atof(w));
     if (const char* w = std::getenv("ANDO_TL_COARSE")) c->tl_coarse = std::atoi(w) != 0 ? 1 : 0;
-    if (const char* w = std::getenv("ANDO_TL_COARSE_EVERY")) c->tl_coarse_every = std::atoi(w) != 0 ? 1 : 0;
     if (const char* w = std::getenv("ANDO_PCG_ROWSETS")) c->spmv_rowsets = std::min(4, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_SPMV")) {
       c->spmv_env = std::strcmp(w, "sell") == 0 ? 2 : 0;
@@ -2445,9 +2379,7 @@ int ando_set_mesh(ando_ctx* c, const float* vertices, int32_t n, const int32_t* 
     c->edges_h = rs.edges;
     c->d_edges.upload(c->edges_h, c->stream);
     c->morton = morton_order(vertices, n);
-    if (c->tl_stream) HIPCHK(hipStreamSynchronize(c->tl_stream));  // a pending coarse inversion
     c->tl.ok = false;
-    c->tl.pending = false;
     {
       double sum = 0.0;
       const size_t E = c->edges_h.size() / 2;

@@ -217,6 +217,31 @@ __device__ __forceinline__ void wg_sum(double (&v)[N], double (*lds)[kMaxWaves])
   __syncthreads();
 }
 
+// wg_sum of N doubles together with a float maximum (one LDS round).
+template <int N>
+__device__ __forceinline__ void wg_sum_max(double (&v)[N], float& mx, double (*lds)[kMaxWaves],
+                                           float* ldsf) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+  for (int o = 32; o > 0; o >>= 1) {
+    for (int c = 0; c < N; ++c) v[c] += __shfl_down(v[c], o, 64);
+    mx = fmaxf(mx, __shfl_down(mx, o, 64));
+  }
+  if (lane == 0) {
+    for (int c = 0; c < N; ++c) lds[c][wid] = v[c];
+    ldsf[wid] = mx;
+  }
+  __syncthreads();
+  for (int c = 0; c < N; ++c) {
+    double s = lds[c][0];
+    for (int w = 1; w < nw; ++w) s += lds[c][w];
+    v[c] = s;
+  }
+  float m = ldsf[0];
+  for (int w = 1; w < nw; ++w) m = fmaxf(m, ldsf[w]);
+  mx = m;
+  __syncthreads();
+}
+
 // Rows 3 patch .. 3 patch + 2 of Ac^-1 times s (nc values): this thread's
 // share of the three dot products (reduced by the caller).
 __device__ __forceinline__ void coarse_rows(const TlArgs& a, int patch, const double* __restrict__ s,
@@ -266,7 +291,7 @@ __global__ void __launch_bounds__(kMaxPatchRows) k_tl_prep(TlArgs a, float* __re
                                                            double* __restrict__ sr) {
   __shared__ float lv[2 * 3 * kMaxPatchRows];
   __shared__ double ldd[3][kMaxWaves];
-  const int patch = blockIdx.x;
+  const int patch = xcd_logical_group((int)blockIdx.x, a.npatch);
   PatchRow P;
   load_patch_row(a, patch, P);
   const int ri = P.live ? P.row : a.n - 1;
@@ -320,7 +345,7 @@ __global__ void __launch_bounds__(kMaxPatchRows) k_tl_start(TlArgs a, const floa
   __shared__ double ldd[3][kMaxWaves];
   __shared__ double ldsd[kMaxWaves];
   __shared__ float ldsf[kMaxWaves];
-  const int patch = blockIdx.x;
+  const int patch = xcd_logical_group((int)blockIdx.x, a.npatch);
   const int i = patch * a.pr + threadIdx.x;
   double e[3];
   coarse_rows(a, patch, sr, e);
@@ -361,7 +386,9 @@ __global__ void __launch_bounds__(2 * kMaxPatchRows) k_tl_spmv(TlArgs a, int it,
   __shared__ double ldd[4][kMaxWaves];
   __shared__ double ldsd[kMaxWaves];
   __shared__ float ldsf[kMaxWaves];
-  const int patch = blockIdx.x;
+  // Patches numbered XCD-contiguously (the same numbering in every patch
+  // kernel), so neighbouring patches, which gather each other's rows, share an L2.
+  const int patch = xcd_logical_group((int)blockIdx.x, a.npatch);
   const int tid = threadIdx.x;
   const int wv = tid >> 6, lane = tid & 63;
   const int q = lane >> 5;
@@ -495,10 +522,9 @@ __global__ void __launch_bounds__(kMaxPatchRows) k_tl_update(TlArgs a, int it, c
                                                              float* __restrict__ part_max,
                                                              PcgStatus* __restrict__ st) {
   __shared__ float lv[2 * 3 * kMaxPatchRows];
-  __shared__ double ldd[6][kMaxWaves];
-  __shared__ double ldsd[kMaxWaves];
+  __shared__ double ldd[7][kMaxWaves];
   __shared__ float ldsf[kMaxWaves];
-  const int patch = blockIdx.x;
+  const int patch = xcd_logical_group((int)blockIdx.x, a.npatch);
   const int tid = threadIdx.x;
   const bool red = tid < kBlock;
   double pre[2 * kUnroll];
@@ -516,22 +542,23 @@ __global__ void __launch_bounds__(kMaxPatchRows) k_tl_update(TlArgs a, int it, c
   }
   for (int e = 0; e < 9; ++e) D[e] = a.Dinv[(size_t)e * a.n + ii];
   const float lmx = a.lmax[patch];
-  // Both coarse products do not depend on alpha.
-  double e12[6];
+  // Both coarse products do not depend on alpha; they are reduced with p.Ap.
+  // v = (p.Ap, (Ac^-1 sr) rows, (Ac^-1 sa) rows).
+  double v[7];
   {
     double e1[3], e2[3];
     coarse_rows(a, patch, sr, e1);
     coarse_rows(a, patch, sa, e2);
     for (int c = 0; c < 3; ++c) {
-      e12[c] = e1[c];
-      e12[3 + c] = e2[c];
+      v[1 + c] = e1[c];
+      v[4 + c] = e2[c];
     }
   }
   if (st->done) return;
-  const double pAp_t = red ? finish_sum_partials(part_pAp, a.npatch, pre, tid, kBlock) : 0.0;
+  v[0] = red ? finish_sum_partials(part_pAp, a.npatch, pre, tid, kBlock) : 0.0;
   const double rz_it = rz_hist[it];
-  const double pAp = block_reduce(pAp_t, ldsd, [](double u, double w) { return u + w; });
-  wg_sum(e12, ldd);
+  wg_sum(v, ldd);
+  const double pAp = v[0];
   if (fabs(pAp) < 1e-16) {
     if (tid == 0) {
       st->failed = 1;
@@ -550,24 +577,23 @@ __global__ void __launch_bounds__(kMaxPatchRows) k_tl_update(TlArgs a, int it, c
     }
   float y[3];
   patch_chebyshev(a, patch, P, lmx, D, rn, lv, y);
-  double rz = 0.0;
+  // w = (r.z, patch sum of the new r), with max|r|.
+  double w[4] = {0.0, (double)rn[0], (double)rn[1], (double)rn[2]};
   float rm = 0.0f;
-  double s[3] = {(double)rn[0], (double)rn[1], (double)rn[2]};
   if (P.live) {
     for (int c = 0; c < 3; ++c) {
-      const float zn = y[c] + (float)(e12[c] - (double)alpha * e12[3 + c]);
+      const float zn = y[c] + (float)(v[1 + c] - (double)alpha * v[4 + c]);
       z[3 * i + c] = zn;
-      rz += (double)rn[c] * (double)zn;
+      w[0] += (double)rn[c] * (double)zn;
       rm = fmaxf(rm, fabsf(rn[c]));
     }
     if (rm != rm) rm = __int_as_float(0x7f800000);
   }
-  wg_sum(s, ldd);
-  block_reduce_sum_max(rz, rm, ldsd, ldsf);
+  wg_sum_max(w, rm, ldd, ldsf);
   if (tid == 0) {
-    part_rz[patch] = rz;
+    part_rz[patch] = w[0];
     part_max[patch] = rm;
-    for (int c = 0; c < 3; ++c) sr_next[3 * patch + c] = s[c];
+    for (int c = 0; c < 3; ++c) sr_next[3 * patch + c] = w[1 + c];
   }
 }
 
@@ -579,16 +605,21 @@ __global__ void k_tl_scatter(int n, const int32_t* __restrict__ perm, const floa
   for (int c = 0; c < 3; ++c) xsol[3 * o + c] = xp[3 * i + c];
 }
 
-// ---- Coarse operator Ac = P^T A P (dense, FP64, column-major for the
-// factorisation).  One wave per coarse block (I, J): its fine blocks
-// blk[ptr[k] .. ptr[k + 1]) are summed in FP64 by 64 strided lanes and a
-// shuffle tree (fixed order).
-__global__ void k_tl_zero(size_t count, double* __restrict__ out) {
+// ---- Coarse operator Ac = P^T A P and its inverse.  Ac is dense in FP64,
+// padded to a multiple of kGjB rows with an identity block; one wave per
+// coarse block (I, J) sums its fine blocks blk[ptr[k] .. ptr[k + 1]) in FP64
+// (64 strided lanes and a shuffle tree, a fixed order).
+constexpr int kGjB = 32;  // tile of the blocked Gauss-Jordan inversion
+
+__global__ void k_tl_zero(int nc, int ncp, double* __restrict__ out, int* __restrict__ fail) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) out[i] = 0.0;
+  if (i == 0) *fail = 0;
+  if (i >= (size_t)ncp * ncp) return;
+  const int r = (int)(i / ncp), c = (int)(i % ncp);
+  out[i] = (r == c && r >= nc) ? 1.0 : 0.0;
 }
 
-__global__ void __launch_bounds__(kBlock) k_tl_coarse(int ncb, int nc, const int32_t* __restrict__ cb_ij,
+__global__ void __launch_bounds__(kBlock) k_tl_coarse(int ncb, int ncp, const int32_t* __restrict__ cb_ij,
                                                       const int32_t* __restrict__ cb_ptr,
                                                       const int32_t* __restrict__ cb_blk,
                                                       const float* __restrict__ A, double* __restrict__ Ac) {
@@ -606,21 +637,106 @@ __global__ void __launch_bounds__(kBlock) k_tl_coarse(int ncb, int nc, const int
   if (lane == 0) {
     const int I = cb_ij[2 * k], J = cb_ij[2 * k + 1];
     for (int u = 0; u < 3; ++u)
-      for (int w = 0; w < 3; ++w) Ac[(size_t)(3 * J + w) * nc + 3 * I + u] = s[3 * u + w];
+      for (int w = 0; w < 3; ++w) Ac[(size_t)(3 * I + u) * ncp + 3 * J + w] = s[3 * u + w];
   }
 }
 
-// The inverse's lower triangle (column-major, from the Cholesky-based
-// inversion) to a full row-major FP32 matrix; a failed factorisation (info
-// != 0) leaves no coarse correction (zeros), which keeps M positive definite.
-__global__ void k_tl_cinv(int nc, const double* __restrict__ L, const int* __restrict__ info,
+// Blocked Gauss-Jordan inversion of the symmetric positive definite Ac
+// (ncp = nb kGjB, row-major).  Step k: P = X_kk^-1 (k_tl_gj_pivot), then every
+// tile from X into Y (k_tl_gj_update):
+//   Y_kk = P,  Y_kj = P X_kj,  Y_ik = -X_ik P,  Y_ij = X_ij - X_ik P X_kj;
+// after nb steps the last buffer holds Ac^-1.  Positive definiteness makes
+// every pivot positive, so no pivoting is needed; a non-positive pivot sets
+// *fail, and the coarse correction is then left out.
+__global__ void __launch_bounds__(kBlock) k_tl_gj_pivot(int ncp, int k, const double* __restrict__ X,
+                                                        double* __restrict__ P, int* __restrict__ fail) {
+  __shared__ double T[kGjB][kGjB + 1];
+  const int t = threadIdx.x;
+  for (int e = t; e < kGjB * kGjB; e += blockDim.x) {
+    const int r = e / kGjB, c = e % kGjB;
+    T[r][c] = X[(size_t)(k * kGjB + r) * ncp + k * kGjB + c];
+  }
+  __syncthreads();
+  for (int q = 0; q < kGjB; ++q) {
+    const double piv = T[q][q];
+    if (!(piv > 0.0)) {
+      if (t == 0) *fail = 1;
+      return;  // uniform: every thread read the same pivot
+    }
+    const double ip = 1.0 / piv;
+    // The rest from the old row q and column q, then row q and column q.
+    __syncthreads();
+    for (int e = t; e < kGjB * kGjB; e += blockDim.x) {
+      const int r = e / kGjB, c = e % kGjB;
+      if (r != q && c != q) T[r][c] = T[r][c] - T[r][q] * ip * T[q][c];
+    }
+    __syncthreads();
+    for (int e = t; e < kGjB; e += blockDim.x) {
+      if (e != q) {
+        T[q][e] = T[q][e] * ip;
+        T[e][q] = -T[e][q] * ip;
+      }
+    }
+    if (t == 0) T[q][q] = ip;
+    __syncthreads();
+  }
+  for (int e = t; e < kGjB * kGjB; e += blockDim.x) P[e] = T[e / kGjB][e % kGjB];
+}
+
+// One tile of step k (one workgroup per tile, 256 threads, 4 entries each).
+__global__ void __launch_bounds__(kBlock) k_tl_gj_update(int nb, int ncp, int k, const double* __restrict__ X,
+                                                         const double* __restrict__ P, double* __restrict__ Y,
+                                                         const int* __restrict__ fail) {
+  __shared__ double Ps[kGjB][kGjB + 1], Xk[kGjB][kGjB + 1], Xi[kGjB][kGjB + 1], Tm[kGjB][kGjB + 1];
+  if (*fail) return;
+  const int ti = blockIdx.x / nb, tj = blockIdx.x % nb;
+  const int t = threadIdx.x;
+  auto tile = [&](int bi, int bj, int r, int c) { return X[(size_t)(bi * kGjB + r) * ncp + bj * kGjB + c]; };
+  for (int e = t; e < kGjB * kGjB; e += blockDim.x) {
+    const int r = e / kGjB, c = e % kGjB;
+    Ps[r][c] = P[e];
+    Xk[r][c] = tile(k, tj, r, c);  // X_kj
+    Xi[r][c] = tile(ti, k, r, c);  // X_ik
+  }
+  __syncthreads();
+  // Tm = P X_kj (the new row block k)
+  for (int e = t; e < kGjB * kGjB; e += blockDim.x) {
+    const int r = e / kGjB, c = e % kGjB;
+    double acc = 0.0;
+    for (int m = 0; m < kGjB; ++m) acc += Ps[r][m] * Xk[m][c];
+    Tm[r][c] = acc;
+  }
+  __syncthreads();
+  for (int e = t; e < kGjB * kGjB; e += blockDim.x) {
+    const int r = e / kGjB, c = e % kGjB;
+    double v;
+    if (ti == k && tj == k) {
+      v = Ps[r][c];
+    } else if (ti == k) {
+      v = Tm[r][c];
+    } else if (tj == k) {
+      double acc = 0.0;
+      for (int m = 0; m < kGjB; ++m) acc += Xi[r][m] * Ps[m][c];
+      v = -acc;
+    } else {
+      double acc = 0.0;
+      for (int m = 0; m < kGjB; ++m) acc += Xi[r][m] * Tm[m][c];
+      v = tile(ti, tj, r, c) - acc;
+    }
+    Y[(size_t)(ti * kGjB + r) * ncp + tj * kGjB + c] = v;
+  }
+}
+
+// The inverse (ncp-wide, row-major FP64) to a symmetric nc x nc row-major FP32
+// matrix; after a failed inversion, zeros (no coarse correction, M stays
+// positive definite).
+__global__ void k_tl_cinv(int nc, int ncp, const double* __restrict__ X, const int* __restrict__ fail,
                           float* __restrict__ out) {
   const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= (size_t)nc * nc) return;
   const int r = (int)(k / nc), c = (int)(k % nc);
-  const bool ok = info[0] == 0 && info[1] == 0;
-  const double v = r >= c ? L[(size_t)c * nc + r] : L[(size_t)r * nc + c];
-  out[k] = ok ? (float)v : 0.0f;
+  const double v = 0.5 * (X[(size_t)r * ncp + c] + X[(size_t)c * ncp + r]);
+  out[k] = *fail ? 0.0f : (float)v;
 }
 
 }  // namespace tl

@@ -12,14 +12,18 @@ HBM.  --warmup substeps from the initial state run untimed (first contact is
 at substep 9), then --steps contiguous substeps are timed; every timed state
 is checked finite and the intersection validator runs on the last one.
 
-The steps/s value comes from a pass without per-launch events.  A second
-pass from the same start state records HIP events around every PCG launch on
-the library's stream; the roofline entry for the dominant kernel (the PCG
-SpMV) comes from it.  The CPU baseline is the oracle (the same corrected
-physics, FP32) on the first timed substep from the same state, its
-element-wise loops on all host threads OpenMP is given; the FP64 oracle on
-that substep gives the accuracy figures.  The faithful mode (the reference's
-step as written) is reported beside it on the window where it stays finite.
+The steps/s value comes from a pass without per-launch events, with the
+reference's block-Jacobi PCG.  A second pass from the same start state records
+HIP events around every PCG launch on the library's stream; the roofline entry
+for the dominant kernel (the PCG SpMV) comes from it.  The same window is then
+stepped again with the two-level Schwarz preconditioner (ANDO_MODE_TWO_LEVEL)
+and with the learned preconditioner, for PCG iterations per step beside the
+plain solver's.  The CPU baseline is the oracle (the same corrected physics,
+FP32) on the substep half-way through the timed window, from the GPU's state
+there, its element-wise loops on all host threads OpenMP is given; the FP64
+oracle on that substep gives the accuracy figures.  The faithful mode (the
+reference's step as written) is reported beside it on the window where it
+stays finite.
 """
 from __future__ import annotations
 
@@ -215,6 +219,18 @@ def oracle_substep(sc, state, bits: int, physics: bool):
     return o.get("x").astype(np.float64), time.perf_counter() - t0, o.stats()
 
 
+def host_cpu_model() -> str:
+    """The host CPU's model name (from /proc/cpuinfo), or ''."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
 def rel_err(a, b) -> float:
     return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-30))
 
@@ -248,11 +264,14 @@ def restore_state(sim, state):
             st.rotation[k] = float(rs["rotation"].ravel()[k])
 
 
-def run_steps(sim, count: int, check_every: int = 0):
+def run_steps(sim, count: int, check_every: int = 0, snap_at=None, snaps=None):
     """`count` substeps; per-step (seconds, stats); stops at a non-finite
-    state.  Returns (records, finite)."""
+    state.  With snap_at, the state before substep snap_at (counted from 0)
+    is appended to snaps, outside the timed calls.  Returns (records, finite)."""
     recs = []
     for k in range(count):
+        if snap_at is not None and k == snap_at:
+            snaps.append(snapshot(sim))
         t0 = time.perf_counter()
         sim.substep()
         recs.append((time.perf_counter() - t0, sim.stats()))
@@ -465,7 +484,9 @@ def main(argv=None):
         print(json.dumps({"metric": METRIC, "error": "non-finite state during warm-up"}), flush=True)
         return 1
     saved = snapshot(sim)
-    recs, ok = run_steps(sim, a.steps, 1)
+    mid_snap = []
+    mid = a.steps // 2
+    recs, ok = run_steps(sim, a.steps, 1, snap_at=mid, snaps=mid_snap)
     if not ok:
         print(json.dumps({"metric": METRIC, "error": f"non-finite state at timed substep {len(recs)}"}), flush=True)
         return 1
@@ -512,6 +533,25 @@ def main(argv=None):
 
     frame = per_frame_leg(sim, sc, with_cpu=not a.no_cpu)
 
+    # The same window from the same start state with the two-level Schwarz
+    # preconditioner (ANDO_MODE_TWO_LEVEL) instead of block-Jacobi.
+    restore_state(sim, saved)
+    sim.set_mode(fp64_near=True, corrected_physics=True, two_level=True)
+    tl_recs, tl_ok = run_steps(sim, a.steps, 1)
+    tl_et, tl_ins = sim.count_intersections()
+    sim.set_mode(fp64_near=True, corrected_physics=True)
+    preconditioners = {
+        "window_substeps": [a.warmup, a.warmup + len(recs) - 1],
+        "block_jacobi": {**summarize(recs), "intersections_at_end": [et, ins]},
+        "two_level_schwarz": {**summarize(tl_recs), "finite": tl_ok, "intersections_at_end": [tl_et, tl_ins],
+                              "pcg_kernel": "two-level: Morton patches of %d rows, Chebyshev degree %s, coarse "
+                                            "3 unknowns per patch" % (tl_recs[-1][1].get("pcg_vec_block", 0),
+                                                                       os.environ.get("ANDO_TL_CHEB", "5"))},
+    }
+    preconditioners["two_level_iteration_ratio"] = round(
+        preconditioners["block_jacobi"]["pcg_iters_per_step"] /
+        max(preconditioners["two_level_schwarz"]["pcg_iters_per_step"], 1e-9), 2)
+
     learned = None
     if not a.no_learned:
         window = list(range(a.warmup, a.warmup + min(a.learned_steps, a.steps)))
@@ -530,15 +570,24 @@ def main(argv=None):
         from oracle import oracle as orc
         threads = orc.get_threads()
         orc.set_threads(threads)
-        x32, cs, st32 = oracle_substep(sc, saved, 32, True)
+        # One substep from the GPU's state half-way through the timed window
+        # (the cloth is on the cylinders there), as a bounded sample of the
+        # timed work.
+        state = mid_snap[0] if mid_snap else saved
+        at = a.warmup + (mid if mid_snap else 0)
+        x32, cs, st32 = oracle_substep(sc, state, 32, True)
+        gpu_mid = summarize(recs[mid:mid + 1]) if mid_snap else head
         cpu = {"value": round(1.0 / cs, 5), "unit": "steps/s", "cores": threads, "kind": "port",
-               "sample": f"substep {a.warmup} of {a.workload} (the first timed one) from the same state, "
+               "host_cpu": host_cpu_model(),
+               "sample": f"substep {at} of {a.workload} (half-way through the timed window) from the GPU's state, "
                          f"oracle FP32 with the corrected physics, OpenMP loops on {threads} threads (results "
-                         f"identical to 1 thread), {cs:.1f} s, {int(st32['pcg_iters'])} PCG iterations"}
-        restore_state(sim, saved)
+                         f"identical to 1 thread), {cs:.1f} s, {int(st32['pcg_iters'])} PCG iterations (the GPU: "
+                         f"{gpu_mid['ms_per_step']} ms, {gpu_mid['pcg_iters_per_step']} PCG iterations on that "
+                         f"substep)"}
+        restore_state(sim, state)
         sim.substep()
-        accuracy = accuracy_vs_fp64(sc, saved, sim.get("x").astype(np.float64), x32)
-        accuracy["at_substep"] = a.warmup
+        accuracy = accuracy_vs_fp64(sc, state, sim.get("x").astype(np.float64), x32)
+        accuracy["at_substep"] = at
 
     out = {
         "metric": METRIC, "value": head["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": len(recs),
@@ -559,7 +608,7 @@ def main(argv=None):
         "pcg_solves": head["pcg_solves"], "pcg_failures": head["pcg_failures"],
         "line_search_failures": head["line_search_failures"], "max_candidates": head["max_candidates"],
         "intersections_at_end": {"edge_triangle": et, "vertices_inside_bodies": ins},
-        "roofline": roof, "cpu_baseline": cpu, "accuracy_vs_fp64": accuracy,
+        "roofline": roof, "cpu_baseline": cpu, "accuracy_vs_fp64": accuracy, "preconditioners": preconditioners,
         "learned_precond": learned, "legs": legs, "per_frame_consumers": frame,
     }
     print(json.dumps(out), flush=True)
