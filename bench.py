@@ -13,12 +13,12 @@ at substep 9), then --steps contiguous substeps are timed; every timed state
 is checked finite and the intersection validator runs on the last one.
 
 The steps/s value comes from a pass without per-launch events, with the
-reference's block-Jacobi PCG.  A second pass from the same start state records
-HIP events around every PCG launch on the library's stream; the roofline entry
-for the dominant kernel (the PCG SpMV) comes from it.  The same window is then
-stepped again with the two-level Schwarz preconditioner (ANDO_MODE_TWO_LEVEL)
-and with the learned preconditioner, for PCG iterations per step beside the
-plain solver's.  The CPU baseline is the oracle (the same corrected physics,
+two-level Schwarz preconditioner (ANDO_MODE_TWO_LEVEL; --precond block-jacobi
+times the reference's block-Jacobi PCG instead).  A second pass from the same
+start state records HIP events around every PCG launch on the library's
+stream; the roofline entry for the dominant kernel comes from it.  The same
+window is then stepped again with the other preconditioner and with the
+learned preconditioner, for PCG iterations per step side by side.  The CPU baseline is the oracle (the same corrected physics,
 FP32) on the substep half-way through the timed window, from the GPU's state
 there, its element-wise loops on all host threads OpenMP is given; the FP64
 oracle on that substep gives the accuracy figures.  The faithful mode (the
@@ -52,6 +52,15 @@ def spmv_bytes(n: int, nb: int) -> int:
 def update_bytes(n: int) -> int:
     """Vector update: p, Ap, P^-1 (36 B), x, r read; x, r, z written."""
     return 120 * n
+
+
+def tl_update_bytes(n: int, nb: int, npatch: int) -> int:
+    """Two-level update (k_tl_update): the patch rows' blocks and columns
+    (40 B per block), p, Ap, x, r and the block-Jacobi block read, x, r, z
+    written (120 B per row), the coarse inverse (4 B per entry, 3 npatch
+    squared entries) and the patch sums of r and Ap (16 B per coarse unknown)."""
+    nc = 3 * npatch
+    return 40 * nb + 120 * n + 4 * nc * nc + 16 * nc
 
 
 def measured_traffic(workload: str, n: int, nb: int, kernel_prefix: str):
@@ -295,7 +304,7 @@ def summarize(recs):
             "max_candidates": max((t["max_candidates"] for t in st), default=0)}
 
 
-def physics_leg(name: str, steps: int, accuracy: bool):
+def physics_leg(name: str, steps: int, accuracy: bool, two_level: bool = False):
     """A workload run from its initial state with the corrected physics for
     `steps` substeps: throughput, solver counters, finiteness and the
     intersection validator at the end; optionally the FP64-oracle error of
@@ -303,7 +312,7 @@ def physics_leg(name: str, steps: int, accuracy: bool):
     from andosim_amd import native, scenes
     sc = scenes.WORKLOADS[name]()
     sim = native.GpuSim(sc, fp64_near=True)
-    sim.set_mode(fp64_near=True, corrected_physics=True)
+    sim.set_mode(fp64_near=True, corrected_physics=True, two_level=two_level)
     half = steps // 2
     recs, ok = run_steps(sim, half, 10)
     state = snapshot(sim)
@@ -315,7 +324,7 @@ def physics_leg(name: str, steps: int, accuracy: bool):
         more, ok = run_steps(sim, steps - half, 10)
         recs += more
     out = {"workload": name, "vertices": int(len(sc.verts)), "steps_run": len(recs), "finite": ok,
-           **summarize(recs)}
+           "precond": "two-level" if two_level else "block-Jacobi", **summarize(recs)}
     et, ins = sim.count_intersections()
     out["intersections_at_end"] = {"edge_triangle": et, "vertices_inside_bodies": ins}
     if accuracy and x_mid is not None:
@@ -461,6 +470,8 @@ def main(argv=None):
                     help="FP64-oracle error of a mid-run substep for every workload leg (minutes of host time)")
     ap.add_argument("--corrected-steps", type=int, default=20)
     ap.add_argument("--batched-steps", type=int, default=5)
+    ap.add_argument("--precond", choices=("two-level", "block-jacobi"), default="two-level",
+                    help="preconditioner of the timed run (the other one is run on the same window beside it)")
     ap.add_argument("--autotune", action="store_true",
                     help="every context picks its PCG launch shape by timing (ANDO_PCG_AUTOTUNE=1)")
     a = ap.parse_args(argv)
@@ -474,7 +485,12 @@ def main(argv=None):
 
     sc = scenes.WORKLOADS[a.workload]()
     sim = native.GpuSim(sc, fp64_near=True)
-    sim.set_mode(fp64_near=True, corrected_physics=True)
+    tl = a.precond == "two-level"
+
+    def mode(two_level, timing=False):
+        sim.set_mode(fp64_near=True, corrected_physics=True, two_level=two_level, kernel_timing=timing)
+
+    mode(False)
     n = len(sc.verts)
     rp, ci = sim.pattern()
     nb = len(ci)
@@ -486,6 +502,7 @@ def main(argv=None):
     saved = snapshot(sim)
     mid_snap = []
     mid = a.steps // 2
+    mode(tl)
     recs, ok = run_steps(sim, a.steps, 1, snap_at=mid, snaps=mid_snap)
     if not ok:
         print(json.dumps({"metric": METRIC, "error": f"non-finite state at timed substep {len(recs)}"}), flush=True)
@@ -501,56 +518,68 @@ def main(argv=None):
     roof = None
     if not a.no_timing:
         restore_state(sim, saved)
-        sim.set_mode(fp64_near=True, kernel_timing=True, corrected_physics=True)
+        mode(tl, timing=True)
         ev_recs, _ = run_steps(sim, min(a.event_steps, a.steps), 1)
-        sim.set_mode(fp64_near=True, kernel_timing=False, corrected_physics=True)
+        mode(tl)
         t_launch = [sum(r[1]["timed_ms"][k] for r in ev_recs) for k in range(2)]
         n_launch = [sum(r[1]["timed_launches"][k] for r in ev_recs) for k in range(2)]
         cpl_bytes = sum(r[1]["timed_coupling_bytes"] for r in ev_recs)
         ev_elapsed = sum(r[0] for r in ev_recs)
-        if n_launch[0]:
+        if n_launch[0] and n_launch[1]:
             avg_s = t_launch[0] / n_launch[0] * 1e-3
-            # Mesh part plus the self-contact coupling workgroups of the launch.
+            avg_u = t_launch[1] / n_launch[1] * 1e-3
+            # Self-contact coupling blocks multiplied in the SpMV launches.
             cby = cpl_bytes / n_launch[0]
-            by = spmv_bytes(n, nb) + cby
-            ach = by / avg_s / 1e9
-            kname = {3: "ando::k_pcg_spmv_sell"}.get(kernel, "ando::k_pcg_spmv<3>")
-            if kernel == 3 and lanes in (2, 4):
-                kname = f"ando::k_pcg_spmv_sell_split<{lanes}, {wg_slices}>"
+            sb = spmv_bytes(n, nb) + cby
+            if tl:
+                npatch = last.get("tl_patches", 0) or -(-n // vec_block)
+                ub = tl_update_bytes(n, nb, npatch)
+                kname, by, avg = "ando::tl::k_tl_update", ub, avg_u
+                if avg_s > avg_u:
+                    kname, by, avg = "ando::tl::k_tl_spmv", sb, avg_s
+            else:
+                ub = update_bytes(n) + (12 * n if cpl_bytes > 0 else 0)  # + Ap2 with coupling rows
+                kname = {3: "ando::k_pcg_spmv_sell"}.get(kernel, "ando::k_pcg_spmv<3>")
+                if kernel == 3 and lanes in (2, 4):
+                    kname = f"ando::k_pcg_spmv_sell_split<{lanes}, {wg_slices}>"
+                by, avg = sb, avg_s
+            ach = by / avg / 1e9
             roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(ach / HBM_PEAK_GBS, 4),
-                    "traffic": measured_traffic(a.workload + "_corrected_physics", n, nb, kname), "kernel": kname,
-                    "bytes_per_launch": int(round(by)), "coupling_bytes_per_launch": int(round(cby)),
-                    "avg_launch_us": round(avg_s * 1e6, 2), "launches": n_launch[0]}
-            if n_launch[1]:
-                avg_u = t_launch[1] / n_launch[1] * 1e-3
-                # The update also reads Ap2 when the solves have coupling rows.
-                ub = update_bytes(n) + (12 * n if cpl_bytes > 0 else 0)
-                roof["update_kernel_gbs"] = round(ub / avg_u / 1e9, 1)
-                roof["update_avg_launch_us"] = round(avg_u * 1e6, 2)
-                roof["pcg_kernels_share_of_step"] = round((t_launch[0] + t_launch[1]) * 1e-3 / ev_elapsed, 3)
-            roof["event_pass_ms_per_step"] = round(1e3 * ev_elapsed / len(ev_recs), 3)
+                    "traffic": measured_traffic(f"{a.workload}_{'two_level' if tl else 'corrected_physics'}", n, nb,
+                                                kname),
+                    "kernel": kname, "bytes_per_launch": int(round(by)), "avg_launch_us": round(avg * 1e6, 2),
+                    "launches": n_launch[0],
+                    "spmv_gbs": round(sb / avg_s / 1e9, 1), "spmv_avg_launch_us": round(avg_s * 1e6, 2),
+                    "spmv_bytes_per_launch": int(round(sb)), "coupling_bytes_per_launch": int(round(cby)),
+                    "update_gbs": round(ub / avg_u / 1e9, 1), "update_avg_launch_us": round(avg_u * 1e6, 2),
+                    "update_bytes_per_launch": int(ub),
+                    "pcg_kernels_share_of_step": round((t_launch[0] + t_launch[1]) * 1e-3 / ev_elapsed, 3),
+                    "event_pass_ms_per_step": round(1e3 * ev_elapsed / len(ev_recs), 3)}
 
     frame = per_frame_leg(sim, sc, with_cpu=not a.no_cpu)
 
-    # The same window from the same start state with the two-level Schwarz
-    # preconditioner (ANDO_MODE_TWO_LEVEL) instead of block-Jacobi.
+    # The same window from the same start state with the other preconditioner.
     restore_state(sim, saved)
-    sim.set_mode(fp64_near=True, corrected_physics=True, two_level=True)
-    tl_recs, tl_ok = run_steps(sim, a.steps, 1)
-    tl_et, tl_ins = sim.count_intersections()
-    sim.set_mode(fp64_near=True, corrected_physics=True)
+    mode(not tl)
+    o_recs, o_ok = run_steps(sim, a.steps, 1)
+    o_et, o_ins = sim.count_intersections()
+    mode(False)
+    tl_desc = "two-level: Morton patches of %d rows, Chebyshev degree %s, coarse 3 unknowns per patch" % (
+        (recs if tl else o_recs)[-1][1].get("pcg_vec_block", 0), os.environ.get("ANDO_TL_CHEB", "5"))
+    timed = {**summarize(recs), "intersections_at_end": [et, ins]}
+    other = {**summarize(o_recs), "finite": o_ok, "intersections_at_end": [o_et, o_ins]}
     preconditioners = {
         "window_substeps": [a.warmup, a.warmup + len(recs) - 1],
-        "block_jacobi": {**summarize(recs), "intersections_at_end": [et, ins]},
-        "two_level_schwarz": {**summarize(tl_recs), "finite": tl_ok, "intersections_at_end": [tl_et, tl_ins],
-                              "pcg_kernel": "two-level: Morton patches of %d rows, Chebyshev degree %s, coarse "
-                                            "3 unknowns per patch" % (tl_recs[-1][1].get("pcg_vec_block", 0),
-                                                                       os.environ.get("ANDO_TL_CHEB", "5"))},
+        "block_jacobi": other if tl else timed,
+        "two_level_schwarz": {**(timed if tl else other), "pcg_kernel": tl_desc},
     }
     preconditioners["two_level_iteration_ratio"] = round(
         preconditioners["block_jacobi"]["pcg_iters_per_step"] /
         max(preconditioners["two_level_schwarz"]["pcg_iters_per_step"], 1e-9), 2)
+    preconditioners["two_level_speedup"] = round(
+        preconditioners["two_level_schwarz"]["steps_per_s"] / max(preconditioners["block_jacobi"]["steps_per_s"],
+                                                                   1e-9), 3)
 
     learned = None
     if not a.no_learned:
@@ -559,8 +588,8 @@ def main(argv=None):
 
     legs = None
     if not a.no_legs:
-        legs = {"sphere129_corrected_physics": physics_leg("sphere129", 200, a.leg_accuracy),
-                "stack65x8_corrected_physics": physics_leg("stack65x8", 300, a.leg_accuracy),
+        legs = {"sphere129_corrected_physics": physics_leg("sphere129", 200, a.leg_accuracy, tl),
+                "stack65x8_corrected_physics": physics_leg("stack65x8", 300, a.leg_accuracy, tl),
                 "cyl257_faithful": faithful_leg(a.workload, 60, 20),
                 "batched": batched_leg(a.batched_steps),
                 "stack_resting_modes": corrected_contact_leg(a.corrected_steps)}
@@ -585,7 +614,9 @@ def main(argv=None):
                          f"{gpu_mid['ms_per_step']} ms, {gpu_mid['pcg_iters_per_step']} PCG iterations on that "
                          f"substep)"}
         restore_state(sim, state)
+        mode(tl)
         sim.substep()
+        mode(False)
         accuracy = accuracy_vs_fp64(sc, state, sim.get("x").astype(np.float64), x32)
         accuracy["at_substep"] = at
 
@@ -596,9 +627,12 @@ def main(argv=None):
         "config": {"workload": f"{a.workload}: 257x257 cloth (66,049 v, 131,072 tris) onto two crossed cylinders "
                                "r=0.1 m (4,800 rigid tris); dt=0.01, 5% strain limit, PCG tol 1e-6, "
                                "corrected physics with self contact",
-                   "vertices": n, "blocks": nb, "precond": "block-Jacobi (plain PCG)",
+                   "vertices": n, "blocks": nb,
+                   "precond": tl_desc if tl else "block-Jacobi (the reference's PCG)",
                    "timed_substeps": [a.warmup, a.warmup + len(recs) - 1],
                    "pcg_kernel": {0: "two-kernel, BSR row SpMV",
+                                  4: "two-kernel two-level: patch SpMV (two lanes per row), patch update with "
+                                     "Chebyshev steps in LDS and the coarse correction",
                                   3: f"two-kernel, sliced-ELL SpMV ({lanes} lane{'s' if lanes > 1 else ''} per row, "
                                      f"{wg_slices} slice{'s' if wg_slices != 1 else ''} per workgroup, "
                                      f"{vec_block}-thread vector kernels{', autotuned' if tuned else ''})"
