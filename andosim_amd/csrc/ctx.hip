@@ -1676,7 +1676,7 @@ PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, c
   hipLaunchKernelGGL(k_tl_init, dim3(nblk(n)), dim3(kBlock), 0, s, a, A, diag, rhs, negate, T.x.p, T.r.p, T.Dinv.p);
   hipLaunchKernelGGL(k_tl_prep, dim3(T.npatch), dim3(patch_threads(T.pr)), 0, s, a, T.lmax.p, (const float*)T.r.p, T.qf.p,
                      T.sr0.p);
-  hipLaunchKernelGGL(k_tl_start, dim3(T.npatch), dim3(patch_threads(T.pr)), 0, s, a, (const float*)T.r.p, (const float*)T.qf.p,
+  hipLaunchKernelGGL(k_tl_start, dim3(T.npatch), dim3(64 * ((T.pr + 63) / 64)), 0, s, a, (const float*)T.r.p, (const float*)T.qf.p,
                      (const double*)T.sr0.p, T.z.p, T.p0.p, T.part_rz.p, T.part_max.p);
   hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, s, T.npatch, (const double*)T.part_rz.p,
                      (const float*)T.part_max.p, (const float*)nullptr, c->rz_hist.p, c->pcg_status.p, tol);

@@ -36,7 +36,6 @@ namespace tl {
 constexpr int kMaxPatchRows = 512;
 constexpr int kLanes = 2;                        // lanes per row in the SpMV
 constexpr int kSpmvSlots = kSellChunk / kLanes;  // slots per lane held in registers by the SpMV
-constexpr int kRowSlots = kSellChunk;            // slots per row held in registers by the patch solves
 constexpr int kPowerIters = 10;                  // power iterations for the patch's largest eigenvalue
 constexpr float kLmaxSafety = 1.2f;              // applied to the power estimate
 constexpr int kMaxCheb = 15;
@@ -45,10 +44,8 @@ constexpr int kMaxWaves = 2 * kMaxPatchRows / 64;  // waves of the widest patch 
 // Rows per patch for n block rows: at least 256, at most 512, and no more than
 // 256 patches while that allows.
 inline int patch_rows(int n) { return std::min(kMaxPatchRows, std::max(256, (n + 255) / 256)); }
-// Workgroup sizes: the SpMV runs 32 rows (two lanes each) per wave, the patch
-// solves 64 rows (one lane each) per wave.
+// Workgroup size of the SpMV: 32 rows (two lanes each) per wave.
 inline int spmv_threads(int pr) { return 64 * ((pr + 31) / 32); }
-inline int patch_threads(int pr) { return 64 * ((pr + 63) / 64); }
 
 struct TlArgs {
   int n;                     // block rows
@@ -103,53 +100,68 @@ __device__ __forceinline__ void mul3(const float* D, const float* v, float* o) {
   o[2] = D[6] * v[0] + (D[7] * v[1] + D[8] * v[2]);
 }
 
-// The patch solves: one thread per row of the patch, the row's first
-// kRowSlots slots in registers (local column, or -1 when the column is outside
-// the patch or the slot is past the row's width).
+// The patch solves: kPatchLanes lanes per row of the patch (one: thread t
+// takes row t; two: as in the SpMV, wave w takes rows 32 w .. 32 w + 31 and
+// lane group q the slots of parity q); each lane keeps its first slots in
+// registers (local column, or -1 when the column is outside the patch or the
+// slot is past the row's width).
+constexpr int kPatchLanes = 2;
+constexpr int kPatchSlots = kSellChunk / kPatchLanes;
+inline int patch_threads(int pr) { return kPatchLanes == 2 ? spmv_threads(pr) : 64 * ((pr + 63) / 64); }
+
 struct PatchRow {
-  int lr, row;
+  int q, lr, row;
   bool live;
   SliceRow R;
-  int lc[kRowSlots];
-  float Ab[kRowSlots][9];
+  int lc[kPatchSlots];
+  float Ab[kPatchSlots][9];
 };
 
 __device__ __forceinline__ void load_patch_row(const TlArgs& a, int patch, PatchRow& P) {
   const int p0 = patch * a.pr;
-  P.lr = threadIdx.x;
+  const int tid = threadIdx.x;
+  if (kPatchLanes == 2) {
+    P.q = (tid >> 5) & 1;
+    P.lr = (tid >> 6) * 32 + (tid & 31);
+  } else {
+    P.q = 0;
+    P.lr = tid;
+  }
   P.row = p0 + P.lr;
   P.live = P.lr < a.pr && P.row < a.n;
   P.R = slice_row(a, P.live ? P.row : a.n - 1);
   if (!P.live) P.R.W = 0;
 #pragma unroll
-  for (int k = 0; k < kRowSlots; ++k) {
-    P.lc[k] = -1;
+  for (int m = 0; m < kPatchSlots; ++m) {
+    const int k = kPatchLanes * m + P.q;
+    P.lc[m] = -1;
     if (k < P.R.W) {
       const int lc = slot_col(a, P.R, k) - p0;
-      P.lc[k] = (lc >= 0 && lc < a.pr) ? lc : -1;
+      P.lc[m] = (lc >= 0 && lc < a.pr) ? lc : -1;
 #pragma unroll
-      for (int e = 0; e < 9; ++e) P.Ab[k][e] = slot_val(a, P.R, k, e);
+      for (int e = 0; e < 9; ++e) P.Ab[m][e] = slot_val(a, P.R, k, e);
     }
   }
 }
 
-// o = sum over the row's slots inside the patch of A_ij v_j (v in LDS, local numbering).
+// o = sum over the row's slots inside the patch of A_ij v_j (v in LDS, local
+// numbering); every lane of the row gets the sum.
 __device__ __forceinline__ void local_product(const TlArgs& a, int patch, const PatchRow& P,
                                               const float* __restrict__ lv, float (&o)[3]) {
   o[0] = o[1] = o[2] = 0.0f;
 #pragma unroll
-  for (int k = 0; k < kRowSlots; ++k) {
-    if (P.lc[k] >= 0) {
-      const int lc = P.lc[k];
+  for (int m = 0; m < kPatchSlots; ++m) {
+    if (P.lc[m] >= 0) {
+      const int lc = P.lc[m];
       const float v0 = lv[3 * lc], v1 = lv[3 * lc + 1], v2 = lv[3 * lc + 2];
-      o[0] = o[0] + (P.Ab[k][0] * v0 + (P.Ab[k][1] * v1 + P.Ab[k][2] * v2));
-      o[1] = o[1] + (P.Ab[k][3] * v0 + (P.Ab[k][4] * v1 + P.Ab[k][5] * v2));
-      o[2] = o[2] + (P.Ab[k][6] * v0 + (P.Ab[k][7] * v1 + P.Ab[k][8] * v2));
+      o[0] = o[0] + (P.Ab[m][0] * v0 + (P.Ab[m][1] * v1 + P.Ab[m][2] * v2));
+      o[1] = o[1] + (P.Ab[m][3] * v0 + (P.Ab[m][4] * v1 + P.Ab[m][5] * v2));
+      o[2] = o[2] + (P.Ab[m][6] * v0 + (P.Ab[m][7] * v1 + P.Ab[m][8] * v2));
     }
   }
   // Rows longer than the register chunk read their further slots again.
   const int p0 = patch * a.pr;
-  for (int k = kRowSlots; k < P.R.W; ++k) {
+  for (int k = kSellChunk + P.q; k < P.R.W; k += kPatchLanes) {
     const int lc = slot_col(a, P.R, k) - p0;
     if (lc >= 0 && lc < a.pr) {
       float B[9];
@@ -160,6 +172,8 @@ __device__ __forceinline__ void local_product(const TlArgs& a, int patch, const 
       o[2] = o[2] + (B[6] * v0 + (B[7] * v1 + B[8] * v2));
     }
   }
+  if (kPatchLanes == 2)
+    for (int c = 0; c < 3; ++c) o[c] = o[c] + __shfl_xor(o[c], 32, 64);
 }
 
 // y = C_p f: `a.cheb` steps of Chebyshev iteration for A_pp y = f from y = 0
@@ -180,7 +194,7 @@ __device__ __forceinline__ void patch_chebyshev(const TlArgs& a, int patch, cons
   }
   for (int j = 1; j < a.cheb; ++j) {
     float* buf = lv + (j & 1) * 3 * kMaxPatchRows;
-    if (P.lr < a.pr)
+    if (P.q == 0 && P.lr < a.pr)
       for (int c = 0; c < 3; ++c) buf[3 * P.lr + c] = P.live ? y[c] : 0.0f;
     __syncthreads();
     float ay[3];
@@ -286,7 +300,7 @@ __global__ void __launch_bounds__(kBlock) k_tl_init(TlArgs a, const float* __res
 
 // Per patch: the largest eigenvalue of D^-1 A_pp by power iteration (times
 // kLmaxSafety), then qf = C_p r and sr = P^T r (the patch's sum of r).
-__global__ void __launch_bounds__(kMaxPatchRows) k_tl_prep(TlArgs a, float* __restrict__ lmax_out,
+__global__ void __launch_bounds__(kPatchLanes * kMaxPatchRows) k_tl_prep(TlArgs a, float* __restrict__ lmax_out,
                                                            const float* __restrict__ r, float* __restrict__ qf,
                                                            double* __restrict__ sr) {
   __shared__ float lv[2 * 3 * kMaxPatchRows];
@@ -306,15 +320,18 @@ __global__ void __launch_bounds__(kMaxPatchRows) k_tl_prep(TlArgs a, float* __re
   float lam = 0.0f;
   for (int it = 0; it < kPowerIters; ++it) {
     float* buf = lv + (it & 1) * 3 * kMaxPatchRows;
-    if (P.lr < a.pr)
+    if (P.q == 0 && P.lr < a.pr)
       for (int c = 0; c < 3; ++c) buf[3 * P.lr + c] = v[c];
     __syncthreads();
     float av[3], w[3];
     local_product(a, patch, P, buf, av);
     mul3(D, av, w);
     if (!P.live) w[0] = w[1] = w[2] = 0.0f;
-    double nn[2] = {(double)w[0] * w[0] + (double)w[1] * w[1] + (double)w[2] * w[2],
-                    (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2]};
+    double nn[2] = {0.0, 0.0};
+    if (P.q == 0) {
+      nn[0] = (double)w[0] * w[0] + (double)w[1] * w[1] + (double)w[2] * w[2];
+      nn[1] = (double)v[0] * v[0] + (double)v[1] * v[1] + (double)v[2] * v[2];
+    }
     wg_sum(nn, ldd);
     const double nw = sqrt(nn[0]), nv = sqrt(nn[1]);
     lam = nv > 0.0 ? (float)(nw / nv) : 0.0f;
@@ -327,9 +344,11 @@ __global__ void __launch_bounds__(kMaxPatchRows) k_tl_prep(TlArgs a, float* __re
   for (int c = 0; c < 3; ++c) f[c] = P.live ? r[3 * ri + c] : 0.0f;
   float y[3];
   patch_chebyshev(a, patch, P, lmx, D, f, lv, y);
-  if (P.live)
+  if (P.q == 0 && P.live)
     for (int c = 0; c < 3; ++c) qf[3 * P.row + c] = y[c];
-  double s[3] = {f[0], f[1], f[2]};
+  double s[3] = {0.0, 0.0, 0.0};
+  if (P.q == 0)
+    for (int c = 0; c < 3; ++c) s[c] = f[c];
   wg_sum(s, ldd);
   if (threadIdx.x == 0)
     for (int c = 0; c < 3; ++c) sr[3 * patch + c] = s[c];
@@ -510,7 +529,7 @@ __global__ void __launch_bounds__(2 * kMaxPatchRows) k_tl_spmv(TlArgs a, int it,
 // alpha = r.z / p.Ap, x += alpha p, r -= alpha Ap, z = C_p r + (Ac^-1 P^T r)
 // with P^T r = sr - alpha sa, partial r.z and max|r|, and the patch sum of the
 // new r (sr_next); one workgroup per patch, one row per thread.
-__global__ void __launch_bounds__(kMaxPatchRows) k_tl_update(TlArgs a, int it, const float* __restrict__ p,
+__global__ void __launch_bounds__(kPatchLanes * kMaxPatchRows) k_tl_update(TlArgs a, int it, const float* __restrict__ p,
                                                              const float* __restrict__ Ap,
                                                              const double* __restrict__ sr,
                                                              const double* __restrict__ sa,
@@ -570,7 +589,7 @@ __global__ void __launch_bounds__(kMaxPatchRows) k_tl_update(TlArgs a, int it, c
   const float alpha = (float)(rz_it / pAp);
   float rn[3];
   for (int c = 0; c < 3; ++c) rn[c] = P.live ? rv0[c] - alpha * av[c] : 0.0f;
-  if (P.live)
+  if (P.q == 0 && P.live)
     for (int c = 0; c < 3; ++c) {
       x[3 * i + c] = xv[c] + alpha * pv[c];
       r[3 * i + c] = rn[c];
@@ -578,9 +597,10 @@ __global__ void __launch_bounds__(kMaxPatchRows) k_tl_update(TlArgs a, int it, c
   float y[3];
   patch_chebyshev(a, patch, P, lmx, D, rn, lv, y);
   // w = (r.z, patch sum of the new r), with max|r|.
-  double w[4] = {0.0, (double)rn[0], (double)rn[1], (double)rn[2]};
+  double w[4] = {0.0, 0.0, 0.0, 0.0};
   float rm = 0.0f;
-  if (P.live) {
+  if (P.q == 0 && P.live) {
+    for (int c = 0; c < 3; ++c) w[1 + c] = rn[c];
     for (int c = 0; c < 3; ++c) {
       const float zn = y[c] + (float)(v[1 + c] - (double)alpha * v[4 + c]);
       z[3 * i + c] = zn;
