@@ -1641,8 +1641,7 @@ PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, c
     double* X = T.Ac.p;
     double* Y = T.Acw.p;
     for (int q = 0; q < nbt; ++q) {
-      // One wave: its barriers cost next to nothing.
-      hipLaunchKernelGGL(k_tl_gj_pivot, dim3(1), dim3(64), 0, s, ncp, q, (const double*)X, T.gjP.p, T.info.p);
+      hipLaunchKernelGGL(k_tl_gj_pivot, dim3(1), dim3(kBlock), 0, s, ncp, q, (const double*)X, T.gjP.p, T.info.p);
       hipLaunchKernelGGL(k_tl_gj_update, dim3(nbt * nbt), dim3(kBlock), 0, s, nbt, ncp, q, (const double*)X,
                          (const double*)T.gjP.p, Y, (const int*)T.info.p);
       std::swap(X, Y);

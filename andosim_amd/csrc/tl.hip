@@ -668,7 +668,7 @@ __global__ void __launch_bounds__(kBlock) k_tl_coarse(int ncb, int ncp, const in
 // after nb steps the last buffer holds Ac^-1.  Positive definiteness makes
 // every pivot positive, so no pivoting is needed; a non-positive pivot sets
 // *fail, and the coarse correction is then left out.
-__global__ void __launch_bounds__(64) k_tl_gj_pivot(int ncp, int k, const double* __restrict__ X,
+__global__ void __launch_bounds__(kBlock) k_tl_gj_pivot(int ncp, int k, const double* __restrict__ X,
                                                         double* __restrict__ P, int* __restrict__ fail) {
   __shared__ double T[kGjB][kGjB + 1];
   const int t = threadIdx.x;
