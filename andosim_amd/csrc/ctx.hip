@@ -213,6 +213,8 @@ struct TlMeta {
   DBuf<double> Ac, Acw, gjP;  // coarse operator, inversion work buffer, pivot block
   DBuf<float> Acinv;
   DBuf<int32_t> info;         // inversion failed
+  bool coarse_ready = false;  // Acinv holds the inverse of an earlier solve's coarse operator
+  long long solves = 0;       // two-level solves on this layout
   DBuf<float> Dinv, lmax, x, r, z, p0, p1, Ap, qf;
   DBuf<double> sa, sr0, sr1;  // per patch: sums of A p and of r (3 each)
   DBuf<double> part_rz, part_pAp;
@@ -262,6 +264,8 @@ void build_tl_meta(TlMeta& T, int n, const std::vector<int32_t>& row_ptr, const 
                    const std::vector<int32_t>& perm, hipStream_t s) {
   using namespace ando::tl;
   T.ok = false;
+  T.coarse_ready = false;
+  T.solves = 0;
   std::vector<int32_t> iperm(n);
   for (int i = 0; i < n; ++i) iperm[perm[i]] = i;
   const int S = (n + 63) / 64;
@@ -415,6 +419,8 @@ struct ando_ctx {
   PcgStatus* status_pinned = nullptr;  // host copy of the PCG status (pinned)
   uint32_t* sact_pinned = nullptr;     // host copy of the strain-active count (pinned)
   PinnedStage hstage;                  // small read-backs (read_small)
+  PinnedStage bstage;                  // rigid body states after the coupled update, read at the step's end
+  int bodies_pending = 0;              // bodies in bstage still to be copied into the caller's records
   hipEvent_t status_ev = nullptr;
   int pcg_chunk = 8;
   int pcg_chunk_max = 256;  // ANDO_PCG_CHUNK_MAX: largest chunk (chunks double from pcg_chunk)
@@ -495,6 +501,7 @@ struct ando_ctx {
   int tl_cheb = 5;          // ANDO_TL_CHEB: Chebyshev degree of the patch solves (odd)
   float tl_ratio = 30.0f;   // ANDO_TL_RATIO: lmax / lmin of the Chebyshev interval
   int tl_coarse = 1;        // ANDO_TL_COARSE=0: no coarse level (patch polynomials only)
+  int tl_coarse_every = 1;  // ANDO_TL_COARSE_EVERY=k: the coarse inverse is rebuilt on every k-th solve
   std::vector<int32_t> morton;  // the mesh's vertices in Morton order of the rest positions
   TlMeta tl;                // for the mesh's block pattern
   ~ando_ctx() {
@@ -1626,9 +1633,16 @@ PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, c
                      (const int32_t*)sm.slice_off.p, (const int32_t*)sm.slice_w.p, (const int32_t*)sm.sell_perm.p, A,
                      sm.sell_vals.p);
   // Coarse operator Ac = P^T A P and its inverse (blocked Gauss-Jordan, FP64),
-  // made for every solve before its iterations start.
+  // made before the solve's iterations start: for every solve, or with
+  // tl_coarse_every = k on every k-th solve of this layout, the solves between
+  // keeping the last inverse (any fixed symmetric positive definite coarse
+  // term leaves the preconditioner symmetric positive definite and constant
+  // within a solve, which is all PCG needs).
   const int nc = c->tl_coarse ? T.nc : 0;
-  if (nc > 0) {
+  const bool rebuild =
+      nc > 0 && (!T.coarse_ready || c->tl_coarse_every <= 1 || T.solves % c->tl_coarse_every == 0);
+  ++T.solves;
+  if (rebuild) {
     const size_t nn = (size_t)nc * nc;
     const int ncp = T.ncp, nbt = ncp / kGjB;
     const size_t nnp = (size_t)ncp * ncp;
@@ -1649,6 +1663,7 @@ PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, c
     hipLaunchKernelGGL(k_tl_cinv, dim3(nblk((long long)nn)), dim3(kBlock), 0, s, nc, ncp, (const double*)X,
                        (const int*)T.info.p, T.Acinv.p);
     HIPCHK(hipGetLastError());
+    T.coarse_ready = true;
   }
   const bool couple = sv != nullptr && sv->ptr != nullptr && c->ncblk > 0;
   TlArgs a{};
@@ -2132,57 +2147,41 @@ float inner_newton(ando_ctx* c, const ando_params& p) {
   return 0.5f;
 }
 
-// RigidBody::integrate (rigid_body.cpp:151-176) with the accumulated force and
-// torque; rotation update by the axis-angle rotation matrix.
-void rigid_integrate(ando_rigid_body& B, const double* force, const double* torque, float dt) {
-  if (B.mass <= 0.0f) return;
-  float F[3] = {(float)force[0], (float)force[1], (float)force[2]};
-  float T[3] = {(float)torque[0], (float)torque[1], (float)torque[2]};
-  auto mul33 = [](const float* A, const float* Bm, float* O) {
-    for (int r = 0; r < 3; ++r)
-      for (int cc = 0; cc < 3; ++cc)
-        O[3 * r + cc] = A[3 * r] * Bm[cc] + (A[3 * r + 1] * Bm[3 + cc] + A[3 * r + 2] * Bm[6 + cc]);
-  };
-  float acc[3];
-  for (int k = 0; k < 3; ++k) acc[k] = F[k] / B.mass;
-  for (int k = 0; k < 3; ++k) B.linear_velocity[k] = B.linear_velocity[k] + dt * acc[k];
-  float RI[9], Rt[9], Iw[9];
-  mul33(B.rotation, B.inertia_body_inv, RI);
-  for (int r = 0; r < 3; ++r)
-    for (int cc = 0; cc < 3; ++cc) Rt[3 * r + cc] = B.rotation[3 * cc + r];
-  mul33(RI, Rt, Iw);
-  for (int k = 0; k < 3; ++k) {
-    float aa = Iw[3 * k] * T[0] + (Iw[3 * k + 1] * T[1] + Iw[3 * k + 2] * T[2]);
-    B.angular_velocity[k] = B.angular_velocity[k] + dt * aa;
+// Body records of the caller <-> device body states (rigid_body.h:72-85).
+void pack_bodies(const ando_rigid_body* bodies, int nbod, std::vector<BodyState>& bs) {
+  bs.resize(nbod);
+  for (int b = 0; b < nbod; ++b) {
+    const ando_rigid_body& B = bodies[b];
+    std::memcpy(bs[b].pos, B.position, sizeof(bs[b].pos));
+    std::memcpy(bs[b].rot, B.rotation, sizeof(bs[b].rot));
+    std::memcpy(bs[b].lin, B.linear_velocity, sizeof(bs[b].lin));
+    std::memcpy(bs[b].ang, B.angular_velocity, sizeof(bs[b].ang));
+    std::memcpy(bs[b].Ibinv, B.inertia_body_inv, sizeof(bs[b].Ibinv));
+    bs[b].mass = B.mass;
   }
-  for (int k = 0; k < 3; ++k) B.position[k] = B.position[k] + dt * B.linear_velocity[k];
-  const float* w = B.angular_velocity;
-  float om = std::sqrt(w[0] * w[0] + (w[1] * w[1] + w[2] * w[2]));
-  if ((double)om > 1e-8) {
-    float th = om * dt;
-    float ax[3] = {w[0] / om, w[1] / om, w[2] / om};
-    float sn = std::sin(th), cs = std::cos(th);
-    float sa[3], ca[3];
-    for (int k = 0; k < 3; ++k) { sa[k] = sn * ax[k]; ca[k] = (1.0f - cs) * ax[k]; }
-    float Rm[9];
-    float tmp = ca[0] * ax[1];
-    Rm[1] = tmp - sa[2]; Rm[3] = tmp + sa[2];
-    tmp = ca[0] * ax[2];
-    Rm[2] = tmp + sa[1]; Rm[6] = tmp - sa[1];
-    tmp = ca[1] * ax[2];
-    Rm[5] = tmp - sa[0]; Rm[7] = tmp + sa[0];
-    for (int k = 0; k < 3; ++k) Rm[4 * k] = ca[k] * ax[k] + cs;
-    float Rn[9];
-    mul33(Rm, B.rotation, Rn);
-    std::memcpy(B.rotation, Rn, sizeof(Rn));
+}
+
+// After the step's final synchronisation: the coupled bodies' new pose and
+// velocities (k_rigid_integrate) into the caller's records.
+void finish_bodies(ando_ctx* c, ando_rigid_body* bodies) {
+  const int nbod = c->bodies_pending;
+  c->bodies_pending = 0;
+  const BodyState* bs = static_cast<const BodyState*>(c->bstage.p);
+  for (int b = 0; b < nbod; ++b) {
+    std::memcpy(bodies[b].position, bs[b].pos, sizeof(bs[b].pos));
+    std::memcpy(bodies[b].rotation, bs[b].rot, sizeof(bs[b].rot));
+    std::memcpy(bodies[b].linear_velocity, bs[b].lin, sizeof(bs[b].lin));
+    std::memcpy(bodies[b].angular_velocity, bs[b].ang, sizeof(bs[b].ang));
   }
 }
 
 // Integrator::apply_rigid_coupling (integrator.cpp:676-754): contacts at the
 // final positions push the bodies with -dV/dg n at the witness point.
-// The per-body sums run on the device (k_rigid_forces); only 6 doubles per
-// body come back, and the body update itself (a few dozen flops per body, on
-// the caller's ando_rigid_body) stays on the host.
+// The per-body sums (k_rigid_forces) and RigidBody::integrate
+// (k_rigid_integrate) run on the device; the new body states are copied to
+// page-locked memory with the step's last synchronisation and written into
+// the caller's records by finish_bodies, so the coupling adds no host round
+// trip.
 // Corrected physics: the contacts within gap_max of the final positions (the
 // point boxes grown by gap_max) with the corrected stiffness, and the force
 // V' n, which pushes the body away from the cloth.
@@ -2219,9 +2218,15 @@ void rigid_coupling(ando_ctx* c, const ando_params& p, ando_rigid_body* bodies, 
   hipLaunchKernelGGL(k_rigid_forces_final, dim3(1), dim3(kBlock), 0, s, G, nbod, (const double*)c->rw_part.p,
                      c->rw_out.p);
   HIPCHK(hipGetLastError());
-  std::vector<double> w(6 * (size_t)nbod);
-  read_small(c, w.data(), (const double*)c->rw_out.p, w.size());
-  for (int b = 0; b < nbod; ++b) rigid_integrate(bodies[b], &w[6 * b], &w[6 * b + 3], p.dt);
+  std::vector<BodyState> bs;
+  pack_bodies(bodies, nbod, bs);
+  c->body_state.upload(bs, s);
+  hipLaunchKernelGGL(k_rigid_integrate, dim3(1), dim3(64), 0, s, nbod, (const double*)c->rw_out.p, c->body_state.p,
+                     p.dt);
+  HIPCHK(hipGetLastError());
+  BodyState* h = static_cast<BodyState*>(c->bstage.ensure((size_t)nbod * sizeof(BodyState)));
+  HIPCHK(hipMemcpyAsync(h, c->body_state.p, (size_t)nbod * sizeof(BodyState), hipMemcpyDeviceToHost, s));
+  c->bodies_pending = nbod;
 }
 
 // Integrator::apply_contact_restitution (integrator.cpp:565-674).
@@ -2256,16 +2261,8 @@ void restitution(ando_ctx* c, const ando_params& p, ando_rigid_body* bodies, int
   if (nbod > 0) {
     upload_bodies(c, bodies, nbod);
     detect_rigid(c, bodies, nbod, p.dt);
-    std::vector<BodyState> bs(nbod);
-    for (int b = 0; b < nbod; ++b) {
-      const ando_rigid_body& B = bodies[b];
-      std::memcpy(bs[b].pos, B.position, sizeof(bs[b].pos));
-      std::memcpy(bs[b].rot, B.rotation, sizeof(bs[b].rot));
-      std::memcpy(bs[b].lin, B.linear_velocity, sizeof(bs[b].lin));
-      std::memcpy(bs[b].ang, B.angular_velocity, sizeof(bs[b].ang));
-      std::memcpy(bs[b].Ibinv, B.inertia_body_inv, sizeof(bs[b].Ibinv));
-      bs[b].mass = B.mass;
-    }
+    std::vector<BodyState> bs;
+    pack_bodies(bodies, nbod, bs);
     c->body_state.upload(bs, s);
     if (c->rc_total)
       hipLaunchKernelGGL(k_restitution_rigid, dim3(1), dim3(64), 0, s, c->rc_total, c->rc.p, c->v.p, c->masses.p,
@@ -2339,6 +2336,7 @@ int ando_create(int32_t device, ando_ctx** out) {
     }
     if (const char* w = std::getenv("ANDO_TL_RATIO")) c->tl_ratio = std::max(1.5f, (float)std::atof(w));
     if (const char* w = std::getenv("ANDO_TL_COARSE")) c->tl_coarse = std::atoi(w) != 0 ? 1 : 0;
+    if (const char* w = std::getenv("ANDO_TL_COARSE_EVERY")) c->tl_coarse_every = std::max(1, std::atoi(w));
     if (const char* w = std::getenv("ANDO_PCG_ROWSETS")) c->spmv_rowsets = std::min(4, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_SPMV")) {
       c->spmv_env = std::strcmp(w, "sell") == 0 ? 2 : 0;
@@ -2668,8 +2666,10 @@ int ando_step(ando_ctx* c, const ando_params* params, ando_rigid_body* bodies, i
     }
     // Read with the step's last synchronisation.
     HIPCHK(hipMemcpyAsync(c->sact_pinned, c->scalars.p + 2, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    c->bodies_pending = 0;
     if (num_bodies > 0 && !c->static_rigids) rigid_coupling(c, p, bodies, num_bodies);
     sync(c);
+    finish_bodies(c, bodies);
     c->stats.strain_active = (int32_t)*c->sact_pinned;
     c->stats.step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   });

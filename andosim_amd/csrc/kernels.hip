@@ -424,6 +424,69 @@ __global__ void k_rigid_forces_final(int G, int nbod, const double* __restrict__
   out[t] = s;
 }
 
+// RigidBody::integrate (rigid_body.cpp:151-176), one thread per body, with the
+// force and torque of k_rigid_forces_final (w[6 b .. 6 b + 5], rounded to
+// float as the reference's Vec3 accumulators): semi-implicit Euler on the
+// velocities, then the position, then the rotation premultiplied by the
+// axis-angle matrix of omega dt (AngleAxis::toRotationMatrix's formula).  The
+// float expressions and their association are the host restatement's; sin and
+// cos are evaluated in double and rounded once.  Bodies with mass <= 0 do not
+// move.
+__global__ void k_rigid_integrate(int nbod, const double* __restrict__ w, BodyState* __restrict__ bs, float dt) {
+  const int b = threadIdx.x;
+  if (b >= nbod) return;
+  BodyState B = bs[b];
+  if (B.mass <= 0.0f) return;
+  auto mul33 = [](const float* A, const float* Bm, float* O) {
+    for (int r = 0; r < 3; ++r)
+      for (int cc = 0; cc < 3; ++cc)
+        O[3 * r + cc] = A[3 * r] * Bm[cc] + (A[3 * r + 1] * Bm[3 + cc] + A[3 * r + 2] * Bm[6 + cc]);
+  };
+  float F[3], T[3];
+  for (int k = 0; k < 3; ++k) {
+    F[k] = (float)w[6 * b + k];
+    T[k] = (float)w[6 * b + 3 + k];
+  }
+  for (int k = 0; k < 3; ++k) B.lin[k] = B.lin[k] + dt * (F[k] / B.mass);
+  float RI[9], Rt[9], Iw[9];
+  mul33(B.rot, B.Ibinv, RI);
+  for (int r = 0; r < 3; ++r)
+    for (int cc = 0; cc < 3; ++cc) Rt[3 * r + cc] = B.rot[3 * cc + r];
+  mul33(RI, Rt, Iw);
+  for (int k = 0; k < 3; ++k) {
+    const float aa = Iw[3 * k] * T[0] + (Iw[3 * k + 1] * T[1] + Iw[3 * k + 2] * T[2]);
+    B.ang[k] = B.ang[k] + dt * aa;
+  }
+  for (int k = 0; k < 3; ++k) B.pos[k] = B.pos[k] + dt * B.lin[k];
+  const float* om3 = B.ang;
+  const float om = sqrtf(om3[0] * om3[0] + (om3[1] * om3[1] + om3[2] * om3[2]));
+  if ((double)om > 1e-8) {
+    const float th = om * dt;
+    const float ax[3] = {om3[0] / om, om3[1] / om, om3[2] / om};
+    const float sn = (float)sin((double)th), cs = (float)cos((double)th);
+    float sa[3], ca[3];
+    for (int k = 0; k < 3; ++k) {
+      sa[k] = sn * ax[k];
+      ca[k] = (1.0f - cs) * ax[k];
+    }
+    float Rm[9];
+    float tmp = ca[0] * ax[1];
+    Rm[1] = tmp - sa[2];
+    Rm[3] = tmp + sa[2];
+    tmp = ca[0] * ax[2];
+    Rm[2] = tmp + sa[1];
+    Rm[6] = tmp - sa[1];
+    tmp = ca[1] * ax[2];
+    Rm[5] = tmp - sa[0];
+    Rm[7] = tmp + sa[0];
+    for (int k = 0; k < 3; ++k) Rm[4 * k] = ca[k] * ax[k] + cs;
+    float Rn[9];
+    mul33(Rm, B.rot, Rn);
+    for (int e = 0; e < 9; ++e) B.rot[e] = Rn[e];
+  }
+  bs[b] = B;
+}
+
 // ---------------------------------------------------------------------------
 // Exclusive scan of int32 counts (three phases, 1024 items per workgroup).
 // ---------------------------------------------------------------------------

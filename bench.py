@@ -38,6 +38,15 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+_T0 = time.perf_counter()
+
+
+def progress(msg: str) -> None:
+    """One line on stderr per finished stage (and every few substeps of a long
+    run), with the seconds since start, so a long run is never silent."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 METRIC = "sim steps/s + PCG iters/step, 66k-vert cloth: learned-precond vs plain PCG, 1e-6 tol"
 HBM_PEAK_GBS = 8000.0
 
@@ -95,6 +104,7 @@ def corrected_contact_leg(steps: int):
     out = {"workload": "stack_resting: 8 sheets 65x65 (33,800 v, 65,536 tris, 98,304 edges) 0.08 mm apart, "
                        "jitter 0.03 mm, ground wall", "steps": steps}
     for mode in ("faithful", "corrected", "corrected_additive_ccd", "corrected_physics"):
+        progress(f"stack_resting leg: {mode}")
         sc = scenes.stack_resting()
         sim = native.GpuSim(sc)
         sim.set_mode(fp64_near=True, corrected=(mode != "faithful"), additive_ccd=(mode == "corrected_additive_ccd"),
@@ -185,6 +195,7 @@ def batched_leg(steps: int, count: int = 256):
 
     out = {"workload": f"batch{count}: {count} independent cloths, sides U{{17..65}}", "substeps": steps}
     for mode in ("sequential", "batched"):
+        progress(f"batched leg: {mode}")
         sims = [native.GpuSim(sc) for sc in scenes.batch256(count=count)]
         for g in sims:
             g.set_mode(fp64_near=True, corrected_physics=True)
@@ -284,6 +295,8 @@ def run_steps(sim, count: int, check_every: int = 0, snap_at=None, snaps=None):
         t0 = time.perf_counter()
         sim.substep()
         recs.append((time.perf_counter() - t0, sim.stats()))
+        if k % 25 == 24:
+            progress(f"  substep {k + 1} of {count}")
         if (check_every and k % check_every == check_every - 1) or k == count - 1:
             if not finite_state(sim):
                 return recs, False
@@ -310,6 +323,7 @@ def physics_leg(name: str, steps: int, accuracy: bool, two_level: bool = False):
     intersection validator at the end; optionally the FP64-oracle error of
     the GPU's substep from the state half-way through."""
     from andosim_amd import native, scenes
+    progress(f"{name} leg: {steps} substeps, corrected physics")
     sc = scenes.WORKLOADS[name]()
     sim = native.GpuSim(sc, fp64_near=True)
     sim.set_mode(fp64_near=True, corrected_physics=True, two_level=two_level)
@@ -339,6 +353,7 @@ def faithful_leg(name: str, max_explore: int, steps: int):
     m/gap^2 stiffness stalls PCG at its cap; the FP32 oracle does the same), so
     it is timed on the substeps from first contact to the last finite one."""
     from andosim_amd import native, scenes
+    progress(f"{name} leg: faithful mode")
     sc = scenes.WORKLOADS[name]()
     sim = native.GpuSim(sc, fp64_near=True)
     first, saved, last_ok = None, None, -1
@@ -381,10 +396,12 @@ def learned_comparison(sim, sc, window, restore, replay, a):
     """
     from andosim_amd import learned_precond, native
 
+    progress("learned preconditioner: training")
     t0 = time.perf_counter()
     systems = learned_precond.training_systems()
     w, losses = learned_precond.train(systems, steps=a.train_steps)
     t_train = time.perf_counter() - t0
+    progress(f"learned preconditioner: trained in {t_train:.1f} s")
     # First Newton systems of the window steps, from the plain run.
     restore()
     dt = np.float32(sc.params["dt"])
@@ -402,6 +419,7 @@ def learned_comparison(sim, sc, window, restore, replay, a):
 
     def steps(mode):
         """The window replayed from its start state with the given solver."""
+        progress(f"learned comparison: {mode} solves")
         if mode == "plain":
             sim.set_mode(fp64_near=True, kernel_timing=False, corrected_physics=True)
         else:
@@ -495,6 +513,7 @@ def main(argv=None):
     rp, ci = sim.pattern()
     nb = len(ci)
 
+    progress(f"{a.workload}: {n} vertices, {nb} blocks; {a.warmup} warm-up substeps")
     warm, ok = run_steps(sim, a.warmup, 1)
     if not ok:
         print(json.dumps({"metric": METRIC, "error": "non-finite state during warm-up"}), flush=True)
@@ -503,11 +522,13 @@ def main(argv=None):
     mid_snap = []
     mid = a.steps // 2
     mode(tl)
+    progress(f"timed run: {a.steps} substeps ({a.precond})")
     recs, ok = run_steps(sim, a.steps, 1, snap_at=mid, snaps=mid_snap)
     if not ok:
         print(json.dumps({"metric": METRIC, "error": f"non-finite state at timed substep {len(recs)}"}), flush=True)
         return 1
     head = summarize(recs)
+    progress(f"timed run: {head['steps_per_s']} steps/s, {head['pcg_iters_per_step']} PCG iterations per substep")
     et, ins = sim.count_intersections()
     last = recs[-1][1]
     kernel, lanes = last["pcg_kernel"], last.get("spmv_lanes", 0)
@@ -517,6 +538,7 @@ def main(argv=None):
     # Event pass: from the same start state, every PCG launch bracketed by events.
     roof = None
     if not a.no_timing:
+        progress("event pass: PCG launches bracketed by HIP events")
         restore_state(sim, saved)
         mode(tl, timing=True)
         ev_recs, _ = run_steps(sim, min(a.event_steps, a.steps), 1)
@@ -557,9 +579,11 @@ def main(argv=None):
                     "pcg_kernels_share_of_step": round((t_launch[0] + t_launch[1]) * 1e-3 / ev_elapsed, 3),
                     "event_pass_ms_per_step": round(1e3 * ev_elapsed / len(ev_recs), 3)}
 
+    progress("per-frame consumers")
     frame = per_frame_leg(sim, sc, with_cpu=not a.no_cpu)
 
     # The same window from the same start state with the other preconditioner.
+    progress(f"same window with {'block-Jacobi' if tl else 'the two-level preconditioner'}")
     restore_state(sim, saved)
     mode(not tl)
     o_recs, o_ok = run_steps(sim, a.steps, 1)
@@ -597,6 +621,7 @@ def main(argv=None):
     cpu = accuracy = None
     if not a.no_cpu:
         from oracle import oracle as orc
+        progress("CPU baseline: one oracle substep (FP32), then the FP64 oracle for the accuracy")
         threads = orc.get_threads()
         orc.set_threads(threads)
         # One substep from the GPU's state half-way through the timed window

@@ -269,6 +269,45 @@ def test_rigid_coupling_moves_a_light_body_like_the_oracle():
     assert moved
 
 
+@pytest.mark.parametrize("corrected", [False, True])
+def test_rigid_integrate_on_the_device_spins_like_the_oracle(corrected):
+    """RigidBody::integrate (rigid_body.cpp:151-176) runs on the device after
+    the coupling sums: a light sphere given a linear and an angular velocity
+    under the falling sheet translates and rotates (axis-angle update) like
+    the FP32 oracle's body.  Each step starts from the oracle's state; the
+    FP64 oracle from the same state measures the float rounding of the
+    nearly cancelling contact sums."""
+    sc = light_sphere_scene()
+    o = OracleSim(sc, 32)
+    o.set_physics(corrected)
+    rs = o.rigid_state(0)
+    rs["linear_velocity"] = np.array([0.05, -0.02, 0.1])
+    rs["angular_velocity"] = np.array([0.7, -2.1, 3.4])
+    o.set_rigid_pose(0, rs)
+    g = native.GpuSim(sc)
+    g.set_mode(corrected_physics=corrected)
+    contacts = 0
+    for k in range(6):
+        sync_state(o, g, 1)
+        o64 = OracleSim(sc, 64)
+        o64.set_physics(corrected)
+        o64.set("x", o.get("x"))
+        o64.set("v", o.get("v"))
+        o64.set_rigid_pose(0, o.rigid_state(0))
+        o.substep()
+        o64.substep()
+        g.substep()
+        contacts += g.stats()["rigid_contacts"]
+        rg, ro, r64 = g.rigid_state(0), o.rigid_state(0), o64.rigid_state(0)
+        for key in ("position", "rotation", "linear_velocity", "angular_velocity"):
+            scale = max(float(np.max(np.abs(ro[key]))), 1e-6)
+            noise = float(np.max(np.abs(ro[key] - r64[key])))
+            assert np.max(np.abs(rg[key] - ro[key])) <= max(2e-6 * scale, 2.0 * noise), (k, key, rg[key], ro[key])
+        assert rel_err(g.get("x"), o.get("x")) < 1e-6
+    assert float(np.max(np.abs(rg["rotation"] - np.eye(3)))) > 1e-2
+    assert contacts > 0
+
+
 def test_static_rigids_leave_bodies_unchanged():
     sc = scenes.small_sheet(9, pins=0, wall=False, rigid=True)
     a, b = native.GpuSim(sc), native.GpuSim(sc)
