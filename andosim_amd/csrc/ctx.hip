@@ -501,7 +501,7 @@ struct ando_ctx {
   int tl_cheb = 5;          // ANDO_TL_CHEB: Chebyshev degree of the patch solves (odd)
   float tl_ratio = 30.0f;   // ANDO_TL_RATIO: lmax / lmin of the Chebyshev interval
   int tl_coarse = 1;        // ANDO_TL_COARSE=0: no coarse level (patch polynomials only)
-  int tl_coarse_every = 1;  // ANDO_TL_COARSE_EVERY=k: the coarse inverse is rebuilt on every k-th solve
+  int tl_coarse_every = 2;  // ANDO_TL_COARSE_EVERY=k: the coarse inverse is rebuilt on every k-th solve
   std::vector<int32_t> morton;  // the mesh's vertices in Morton order of the rest positions
   TlMeta tl;                // for the mesh's block pattern
   ~ando_ctx() {
@@ -1421,12 +1421,14 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
   c->pcg_status.ensure(1);
   c->part_pAp.ensure((size_t)nwg_r + 1);
   // Batched solves (ando_step_batch) are set up by the batch: the Newton
-  // system (b = -g) on the default sliced-ELL grid, without coupling blocks,
-  // learned preconditioner or kernel timing.  A shape chosen through the
-  // environment keeps the solve out of the batch; one chosen by the autotuner
-  // gives way to the batch's grid.
+  // system (b = -g) on the default sliced-ELL grid, without coupling blocks
+  // (a self-contact view with no coupling rows adds nothing to the solve, so
+  // it may join), learned preconditioner or kernel timing.  A shape chosen
+  // through the environment keeps the solve out of the batch; one chosen by
+  // the autotuner gives way to the batch's grid.
   const bool default_shape = c->sell_lanes == 2 && sell_slices == 1 && vb == kBlock;
-  if (t_batch && t_batch->device == c->device && sell && !couple && P_given == nullptr && x0_given == nullptr &&
+  const bool coupling_rows = couple && c->ncrow > 0;
+  if (t_batch && t_batch->device == c->device && sell && !coupling_rows && P_given == nullptr && x0_given == nullptr &&
       negate == 1 && c->kernel_timing == 0 && (default_shape || c->autotuned)) {
     c->stats.spmv_lanes = 2;
     c->stats.spmv_wg_slices = 1;
@@ -1655,7 +1657,7 @@ PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, c
     double* X = T.Ac.p;
     double* Y = T.Acw.p;
     for (int q = 0; q < nbt; ++q) {
-      hipLaunchKernelGGL(k_tl_gj_pivot, dim3(1), dim3(kBlock), 0, s, ncp, q, (const double*)X, T.gjP.p, T.info.p);
+      hipLaunchKernelGGL(k_tl_gj_pivot, dim3(1), dim3(64), 0, s, ncp, q, (const double*)X, T.gjP.p, T.info.p);
       hipLaunchKernelGGL(k_tl_gj_update, dim3(nbt * nbt), dim3(kBlock), 0, s, nbt, ncp, q, (const double*)X,
                          (const double*)T.gjP.p, Y, (const int*)T.info.p);
       std::swap(X, Y);

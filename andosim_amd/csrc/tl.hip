@@ -668,39 +668,51 @@ __global__ void __launch_bounds__(kBlock) k_tl_coarse(int ncb, int ncp, const in
 // after nb steps the last buffer holds Ac^-1.  Positive definiteness makes
 // every pivot positive, so no pivoting is needed; a non-positive pivot sets
 // *fail, and the coarse correction is then left out.
-__global__ void __launch_bounds__(kBlock) k_tl_gj_pivot(int ncp, int k, const double* __restrict__ X,
-                                                        double* __restrict__ P, int* __restrict__ fail) {
-  __shared__ double T[kGjB][kGjB + 1];
-  const int t = threadIdx.x;
-  for (int e = t; e < kGjB * kGjB; e += blockDim.x) {
-    const int r = e / kGjB, c = e % kGjB;
-    T[r][c] = X[(size_t)(k * kGjB + r) * ncp + k * kGjB + c];
-  }
-  __syncthreads();
+//
+// The pivot block is inverted by one wave with the block in registers: lane l
+// holds column l mod 32 of rows 16 (l / 32) .. 16 (l / 32) + 15.  Each of the
+// 32 pivot steps exchanges only the pivot row and column through LDS, so a step
+// costs two LDS round trips instead of a pass over the whole block; every
+// entry goes through the same operations in the same order as an in-LDS
+// elimination (new T_rc = T_rc - (T_rq / T_qq) T_qc with the reciprocal formed
+// once, row q scaled by it, column q by its negative, the pivot replaced by it).
+static_assert(kGjB == 32, "the register layout of k_tl_gj_pivot assumes 32 x 32 pivot blocks");
+__global__ void __launch_bounds__(64) k_tl_gj_pivot(int ncp, int k, const double* __restrict__ X,
+                                                    double* __restrict__ P, int* __restrict__ fail) {
+  __shared__ double colq[kGjB], rowq[kGjB];
+  const int lane = threadIdx.x;
+  const int c = lane & 31, h = lane >> 5;
+  double A[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) A[i] = X[(size_t)(k * kGjB + 16 * h + i) * ncp + k * kGjB + c];
+#pragma unroll
   for (int q = 0; q < kGjB; ++q) {
-    const double piv = T[q][q];
+    if (c == q) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) colq[16 * h + i] = A[i];
+    }
+    if (h == (q >> 4)) rowq[c] = A[q & 15];
+    __syncthreads();
+    const double piv = colq[q];
     if (!(piv > 0.0)) {
-      if (t == 0) *fail = 1;
-      return;  // uniform: every thread read the same pivot
+      if (lane == 0) *fail = 1;
+      return;  // uniform: every lane read the same pivot
     }
     const double ip = 1.0 / piv;
-    // The rest from the old row q and column q, then row q and column q.
-    __syncthreads();
-    for (int e = t; e < kGjB * kGjB; e += blockDim.x) {
-      const int r = e / kGjB, c = e % kGjB;
-      if (r != q && c != q) T[r][c] = T[r][c] - T[r][q] * ip * T[q][c];
+    const double rq = rowq[c];  // T_qc before the step
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int r = 16 * h + i;
+      const double cq = colq[r];  // T_rq before the step
+      if (r != q && c != q) A[i] = A[i] - cq * ip * rq;
+      else if (r == q && c != q) A[i] = rq * ip;
+      else if (r != q) A[i] = -cq * ip;
+      else A[i] = ip;
     }
-    __syncthreads();
-    for (int e = t; e < kGjB; e += blockDim.x) {
-      if (e != q) {
-        T[q][e] = T[q][e] * ip;
-        T[e][q] = -T[e][q] * ip;
-      }
-    }
-    if (t == 0) T[q][q] = ip;
-    __syncthreads();
+    __syncthreads();  // the next step overwrites colq and rowq
   }
-  for (int e = t; e < kGjB * kGjB; e += blockDim.x) P[e] = T[e / kGjB][e % kGjB];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) P[(16 * h + i) * kGjB + c] = A[i];
 }
 
 // One tile of step k (one workgroup per tile, 256 threads, 4 entries each).

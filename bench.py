@@ -589,8 +589,10 @@ def main(argv=None):
     o_recs, o_ok = run_steps(sim, a.steps, 1)
     o_et, o_ins = sim.count_intersections()
     mode(False)
-    tl_desc = "two-level: Morton patches of %d rows, Chebyshev degree %s, coarse 3 unknowns per patch" % (
-        (recs if tl else o_recs)[-1][1].get("pcg_vec_block", 0), os.environ.get("ANDO_TL_CHEB", "5"))
+    tl_desc = ("two-level: Morton patches of %d rows, Chebyshev degree %s, coarse 3 unknowns per patch, coarse "
+               "inverse rebuilt every %s solves" % ((recs if tl else o_recs)[-1][1].get("pcg_vec_block", 0),
+                                                    os.environ.get("ANDO_TL_CHEB", "5"),
+                                                    os.environ.get("ANDO_TL_COARSE_EVERY", "2")))
     timed = {**summarize(recs), "intersections_at_end": [et, ins]}
     other = {**summarize(o_recs), "finite": o_ok, "intersections_at_end": [o_et, o_ins]}
     preconditioners = {

@@ -260,10 +260,14 @@ int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, co
 /* ANDO_MODE_TWO_LEVEL (new): the step's Newton systems (and ando_pcg) are
  * solved by CG with a two-level Schwarz preconditioner instead of block-Jacobi
  * (pcg_solver.cpp:8-146): the block rows in Morton order of the rest positions,
- * cut into patches of 256; per patch a fixed odd-degree Chebyshev polynomial in
- * D^-1 A_pp (block-Jacobi scaling, the patch's own blocks; degree
- * ANDO_TL_CHEB, default 9), plus a coarse correction with three unknowns per
- * patch (A_c = P^T A P, inverted once per solve; for up to 1024 patches).  Same
+ * cut into patches of 256 to 512 rows; per patch a fixed odd-degree Chebyshev
+ * polynomial in D^-1 A_pp (block-Jacobi scaling, the patch's own blocks;
+ * degree ANDO_TL_CHEB, default 5), plus a coarse correction with three
+ * unknowns per patch (A_c = P^T A P, inverted in FP64 before the solve's
+ * iterations; for up to 1024 patches).  In ando_step the inverse is rebuilt
+ * on every ANDO_TL_COARSE_EVERY-th solve (default 2) and the solves between
+ * keep the last one, which leaves the preconditioner symmetric positive
+ * definite and fixed within each solve; ando_pcg always rebuilds it.  Same
  * stopping test (max-norm relative residual, |p.Ap| < 1e-16 breakdown, the
  * iteration cap), far fewer iterations on cloth systems.  ando_stats.pcg_kernel
  * reads 4. */

@@ -83,7 +83,36 @@ def test_batch_mixed_modes_one_thread_per_cloth():
         assert np.array_equal(a.get("x"), b.get("x"))
         assert np.array_equal(a.get("v"), b.get("v"))
     assert bat[0].stats()["batched_solves"] > 0
-    assert bat[-1].stats()["batched_solves"] == 0
+    # The stacked sheets touch: solves with coupling rows keep their own launches.
+    sl = bat[-1].stats()
+    assert sl["batched_solves"] < sl["pcg_solves"]
+
+
+def test_batch_corrected_physics_equals_sequential():
+    """Corrected-physics cloths (self-contact view, per-iteration detection,
+    CCD line search): solves without coupling rows join the batched launches,
+    and every result equals stepping the cloths one by one, bit for bit."""
+    print("batch of 12 corrected-physics cloths", flush=True)
+    scs = scenes.batch256(count=12)
+    seq = [native.GpuSim(sc) for sc in scs]
+    bat = [native.GpuSim(sc) for sc in scs]
+    for g in seq + bat:
+        g.set_mode(fp64_near=True, corrected_physics=True)
+    batched = solves = 0
+    for _ in range(4):
+        for g in seq:
+            g.substep()
+        for g in bat:
+            g.apply_gravity()
+        native.step_batch(bat, threads=6)
+        batched += sum(g.stats()["batched_solves"] for g in bat)
+        solves += sum(g.stats()["pcg_solves"] for g in bat)
+    for a, b in zip(seq, bat):
+        assert np.array_equal(a.get("x"), b.get("x"))
+        assert np.array_equal(a.get("v"), b.get("v"))
+        sa, sb = a.stats(), b.stats()
+        assert sa["pcg_iterations"] == sb["pcg_iterations"] and sa["newton_iterations"] == sb["newton_iterations"]
+    assert batched > 0 and solves > 0
 
 
 def test_batch_with_rigid_bodies_and_errors():
