@@ -27,10 +27,20 @@
 // and AngleAxis are restated with different algorithms (see svd32 and
 // spd_general below) and agree with Eigen only to rounding; they are pinned by
 // identities (U S V^T = F, ordering, derivatives by finite differences), not
-// bit for bit.  Everything else is pinned by the checks in tests/test_oracle.py
-// (the reference's known answers in tests/golden/, closed forms, brute force,
-// finite differences, invariants); the learned-preconditioner oracle
-// (oracle/learned.py) by tests/test_learned_oracle.py.
+// bit for bit.  Everything else is pinned by checks whose expected values come
+// from the mathematics or the reference's own tests, not from this file's
+// formulas: tests/test_oracle.py (the reference's known answers in
+// tests/golden/, closed forms, brute force, finite differences, invariants);
+// tests/test_oracle_pins.py (strain-limit gradient and Hessian against the
+// singular-value derivatives, friction against the derivative of its
+// potential, contact and wall stiffness closed forms, RigidBody::integrate
+// against an exact spin and Euler's and Newton's laws, the reference's
+// rigid-coupled known answer); tests/test_oracle_physics.py (the corrected
+// physics mode: membrane gradient and Hessian closed forms, attracting pins,
+// CCD line search, free fall).  The learned-preconditioner oracle
+// (oracle/learned.py) is pinned by tests/test_learned_oracle.py, the
+// two-level model (oracle/two_level.py) by dense solves in
+// tests/test_two_level_model.py.
 
 #include <algorithm>
 #include <array>
