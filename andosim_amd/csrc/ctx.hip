@@ -1387,6 +1387,38 @@ struct BatchPcg {
 void autotune_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int32_t* cols, const int32_t* diag,
                   const float* rhs, int negate, float* xsol, int maxit, SpmvMeta* sm, const ando::SelfView* sv);
 
+// Iterations to queue in a solve's next chunk.  After each chunk the host
+// reads the status, which carries the max-norm relative residual of the
+// latest convergence test (rel at iteration rel_it).  From the rate at which
+// its logarithm fell since the previous read (from 1 at iteration 0 on the
+// first read) the iterations still needed are estimated and queued with a 15%
+// margin plus two, at least 8 and at most the largest chunk; without a usable
+// rate (a residual that rose, a tolerance of 0) the chunk doubles.  Launches
+// queued past convergence still issue their first loads and cost a few
+// microseconds each, while a status read costs one host round trip, so a
+// close estimate pays.  The chunking changes no arithmetic.
+struct ChunkPlan {
+  float last_rel = 1.0f;
+  int last_it = 0;
+  int next(int chunk, int issued, const PcgStatus& st, float tol, int chunk_max) {
+    int out = std::min(chunk * 2, chunk_max);
+    if (tol > 0.0f && st.rel_it > last_it && st.rel > 0.0f && st.rel < last_rel) {
+      const double rate =
+          (std::log((double)st.rel) - std::log((double)last_rel)) / (double)(st.rel_it - last_it);
+      const double need = std::log((double)tol / (double)st.rel) / rate;  // iterations past rel_it
+      if (std::isfinite(need) && need > 0.0) {
+        const double ahead = 1.15 * need + 2.0 - (double)(issued - st.rel_it);
+        out = (int)std::min<double>((double)chunk_max, std::max(8.0, std::ceil(ahead)));
+      }
+    }
+    if (st.rel_it > last_it) {
+      last_rel = st.rel;
+      last_it = st.rel_it;
+    }
+    return out;
+  }
+};
+
 // P_given / x0_given (device, both or neither): a preconditioner of 9 floats
 // per vertex and a starting guess instead of block-Jacobi from x0 = 0
 // (x0_given may be xsol itself).  guard_x0: a starting guess whose residual is
@@ -1516,13 +1548,15 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
   }
   PcgStatus st{};
   int it = 0;
-  // The iterations go out in chunks of 8, 16, 32, 64, 128, then 256 iterations;
-  // after each chunk the status is copied to pinned host memory and read.  Launches
-  // queued after convergence return at the done flag (after issuing their
-  // first loads, which lets a live launch overlap the flag's latency).  Tried
-  // and slower on this path: queueing the next chunk before waiting, and
+  // The iterations go out in chunks: 8 first, then as many as the residual's
+  // convergence rate predicts (ChunkPlan; doubling up to 256 without a rate);
+  // after each chunk the status is copied to pinned host memory and read.
+  // Launches queued after convergence return at the done flag (after issuing
+  // their first loads, which lets a live launch overlap the flag's latency).
+  // Tried and slower on this path: queueing the next chunk before waiting, and
   // sizing the first chunk from the previous step's solve.
   int chunk = std::max(1, c->pcg_chunk);
+  ChunkPlan plan;
   const bool timing = c->kernel_timing != 0;
   size_t ev_used = 0;
   // Timed launches record their start and end with the dispatch itself
@@ -1595,7 +1629,7 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
       st = *c->status_pinned;
       break;
     }
-    chunk = std::min(chunk * 2, c->pcg_chunk_max);
+    chunk = plan.next(chunk, it, st, tol, c->pcg_chunk_max);
   }
   if (timing) sync(c);
   if (timing) {
@@ -1700,6 +1734,7 @@ PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, c
   PcgStatus st{};
   int it = 0;
   int chunk = std::max(1, c->pcg_chunk);
+  ChunkPlan plan;
   const bool timing = c->kernel_timing != 0;
   size_t ev_used = 0;
   auto ev = [&](size_t k) -> hipEvent_t {
@@ -1747,7 +1782,7 @@ PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, c
       st = *c->status_pinned;
       break;
     }
-    chunk = std::min(chunk * 2, c->pcg_chunk_max);
+    chunk = plan.next(chunk, it, st, tol, c->pcg_chunk_max);
   }
   hipLaunchKernelGGL(k_tl_scatter, dim3(nblk(n)), dim3(kBlock), 0, s, n, (const int32_t*)T.perm.p,
                      (const float*)T.x.p, xsol);

@@ -25,9 +25,9 @@ struct PcgStatus {
   int32_t done;
   int32_t iters;
   int32_t failed;
-  int32_t pad;
+  int32_t rel_it;  // the iteration whose convergence test last wrote rel (0: none yet)
   float bmax;
-  float rel;
+  float rel;       // max-norm relative residual: the verdict once done, else the latest test's
 };
 
 // Reductions of per-workgroup partials.  Each thread issues its kUnroll loads
@@ -285,6 +285,7 @@ __device__ __forceinline__ void pcg_start_body(int nwg, const double* __restrict
     float rel = ((double)bmax < 1e-16) ? rmax : rmax / bmax;
     st->bmax = bmax;
     st->rel = rel;
+    st->rel_it = 0;
     st->failed = 0;
     st->iters = 0;
     st->done = rel < tol ? 1 : 0;
@@ -507,7 +508,11 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int n, int it, const float*
       return;
     }
     beta = (float)(rz / rz_prev);
-    if (g == 0 && threadIdx.x == 0) rz_hist[it] = rz;
+    if (g == 0 && threadIdx.x == 0) {
+      rz_hist[it] = rz;
+      st->rel = rel;  // progress for the host's chunk sizing
+      st->rel_it = it;
+    }
   }
   double pap = 0.0;
 #pragma unroll
@@ -692,7 +697,11 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_sell(int n, int slices, int
       return;
     }
     beta = (float)(rz / rz_prev);
-    if (g == 0 && threadIdx.x == 0) rz_hist[it] = rz;
+    if (g == 0 && threadIdx.x == 0) {
+      rz_hist[it] = rz;
+      st->rel = rel;  // progress for the host's chunk sizing
+      st->rel_it = it;
+    }
   }
   // Slot k's product goes to partial t[k % 8]; the row sum combines the eight
   // partials as ((t0 + t4) + (t2 + t6)) + ((t1 + t5) + (t3 + t7)), the order in
@@ -864,7 +873,11 @@ __device__ __forceinline__ void sell_split_body(int g, int n, int slices, int it
       return;
     }
     beta = (float)(rz / rz_prev);
-    if (g == 0 && tid == 0) rz_hist[it] = rz;
+    if (g == 0 && tid == 0) {
+      rz_hist[it] = rz;
+      st->rel = rel;  // progress for the host's chunk sizing
+      st->rel_it = it;
+    }
   }
   // T[m] is partial t[L m + q] of the one-lane kernel.
   float T[KC][3];

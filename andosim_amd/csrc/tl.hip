@@ -469,7 +469,11 @@ __global__ void __launch_bounds__(2 * kMaxPatchRows) k_tl_spmv(TlArgs a, int it,
       return;
     }
     beta = (float)(rz / rz_prev);
-    if (patch == 0 && tid == 0) rz_hist[it] = rz;
+    if (patch == 0 && tid == 0) {
+      rz_hist[it] = rz;
+      st->rel = rel;  // progress for the host's chunk sizing
+      st->rel_it = it;
+    }
   }
   float T[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
@@ -492,18 +496,39 @@ __global__ void __launch_bounds__(2 * kMaxPatchRows) k_tl_spmv(TlArgs a, int it,
     T[1] = T[1] + (B[3] * pv[0] + (B[4] * pv[1] + B[5] * pv[2]));
     T[2] = T[2] + (B[6] * pv[0] + (B[7] * pv[1] + B[8] * pv[2]));
   }
-  // Self-contact coupling blocks of the row (original numbering).
+  // Self-contact coupling blocks of the row (original numbering).  A row in
+  // a contact cluster can carry dozens of them, so each lane requests four
+  // blocks (their columns, values and p, z entries) before using any; the
+  // sums keep the block order.
   if (a.cptr && live) {
     const int o = a.perm[row];
     const int cb = 3 * a.cptr[o], ce = 3 * a.cptr[o + 1];
-    for (int b = cb + q; b < ce; b += kLanes) {
-      const int j = a.iperm[a.ccol[b]];
-      const float* B = a.cval + 9 * (size_t)b;
-      float pv[3];
-      for (int c = 0; c < 3; ++c) pv[c] = it > 0 ? z[3 * j + c] + beta * p_old[3 * j + c] : p_old[3 * j + c];
-      T[0] = T[0] + (B[0] * pv[0] + (B[1] * pv[1] + B[2] * pv[2]));
-      T[1] = T[1] + (B[3] * pv[0] + (B[4] * pv[1] + B[5] * pv[2]));
-      T[2] = T[2] + (B[6] * pv[0] + (B[7] * pv[1] + B[8] * pv[2]));
+    constexpr int kU = 4;
+    for (int b0 = cb + q; b0 < ce; b0 += kLanes * kU) {
+      int j[kU];
+      float B[kU][9], pv[kU][3];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int b = b0 + kLanes * u;
+        if (b < ce) {
+          j[u] = a.iperm[a.ccol[b]];
+#pragma unroll
+          for (int e = 0; e < 9; ++e) B[u][e] = a.cval[9 * (size_t)b + e];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (b0 + kLanes * u < ce)
+          for (int c = 0; c < 3; ++c)
+            pv[u][c] = it > 0 ? z[3 * j[u] + c] + beta * p_old[3 * j[u] + c] : p_old[3 * j[u] + c];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        if (b0 + kLanes * u < ce) {
+          T[0] = T[0] + (B[u][0] * pv[u][0] + (B[u][1] * pv[u][1] + B[u][2] * pv[u][2]));
+          T[1] = T[1] + (B[u][3] * pv[u][0] + (B[u][4] * pv[u][1] + B[u][5] * pv[u][2]));
+          T[2] = T[2] + (B[u][6] * pv[u][0] + (B[u][7] * pv[u][1] + B[u][8] * pv[u][2]));
+        }
+      }
     }
   }
   float ap[3];
