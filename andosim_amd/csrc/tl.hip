@@ -496,43 +496,68 @@ __global__ void __launch_bounds__(2 * kMaxPatchRows) k_tl_spmv(TlArgs a, int it,
     T[1] = T[1] + (B[3] * pv[0] + (B[4] * pv[1] + B[5] * pv[2]));
     T[2] = T[2] + (B[6] * pv[0] + (B[7] * pv[1] + B[8] * pv[2]));
   }
-  // Self-contact coupling blocks of the row (original numbering).  A row in
-  // a contact cluster can carry dozens of them, so each lane requests four
-  // blocks (their columns, values and p, z entries) before using any; the
-  // sums keep the block order.
-  if (a.cptr && live) {
-    const int o = a.perm[row];
-    const int cb = 3 * a.cptr[o], ce = 3 * a.cptr[o + 1];
-    constexpr int kU = 4;
-    for (int b0 = cb + q; b0 < ce; b0 += kLanes * kU) {
-      int j[kU];
-      float B[kU][9], pv[kU][3];
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        const int b = b0 + kLanes * u;
-        if (b < ce) {
-          j[u] = a.iperm[a.ccol[b]];
-#pragma unroll
-          for (int e = 0; e < 9; ++e) B[u][e] = a.cval[9 * (size_t)b + e];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kU; ++u)
-        if (b0 + kLanes * u < ce)
-          for (int c = 0; c < 3; ++c)
-            pv[u][c] = it > 0 ? z[3 * j[u] + c] + beta * p_old[3 * j[u] + c] : p_old[3 * j[u] + c];
-#pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        if (b0 + kLanes * u < ce) {
-          T[0] = T[0] + (B[u][0] * pv[u][0] + (B[u][1] * pv[u][1] + B[u][2] * pv[u][2]));
-          T[1] = T[1] + (B[u][3] * pv[u][0] + (B[u][4] * pv[u][1] + B[u][5] * pv[u][2]));
-          T[2] = T[2] + (B[u][6] * pv[u][0] + (B[u][7] * pv[u][1] + B[u][8] * pv[u][2]));
-        }
-      }
-    }
-  }
   float ap[3];
   for (int c = 0; c < 3; ++c) ap[c] = T[c] + __shfl_xor(T[c], 32, 64);
+  // Self-contact coupling blocks of the patch rows (original numbering),
+  // multiplied by the whole workgroup: 8 lanes per row, rows in rounds of
+  // blockDim / 8, each lane requesting four blocks (columns, values, p and z
+  // entries) before using any, in block order.  A row in a contact cluster can
+  // carry dozens of blocks; spreading them over 8 lanes keeps such a row from
+  // holding the workgroup.  The row sums reach the row's lanes through LDS.
+  if (a.cptr) {
+    __shared__ float lcp[3 * kMaxPatchRows];
+    const int cl = tid & 7, nslot = (int)(blockDim.x >> 3);
+    constexpr int kU = 4;
+    for (int r0 = 0; r0 < a.pr; r0 += nslot) {
+      const int clr = r0 + (tid >> 3), crow = patch * a.pr + clr;
+      int cb = 0, ce = 0;
+      if (clr < a.pr && crow < a.n) {
+        const int o = a.perm[crow];
+        cb = 3 * a.cptr[o];
+        ce = 3 * a.cptr[o + 1];
+      }
+      float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+      for (int b0 = cb + cl; b0 < ce; b0 += 8 * kU) {
+        int j[kU];
+        float B[kU][9], pv[kU][3];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int b = b0 + 8 * u;
+          if (b < ce) {
+            j[u] = a.iperm[a.ccol[b]];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) B[u][e] = a.cval[9 * (size_t)b + e];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+          if (b0 + 8 * u < ce)
+            for (int c = 0; c < 3; ++c)
+              pv[u][c] = it > 0 ? z[3 * j[u] + c] + beta * p_old[3 * j[u] + c] : p_old[3 * j[u] + c];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          if (b0 + 8 * u < ce) {
+            s0 = s0 + (B[u][0] * pv[u][0] + (B[u][1] * pv[u][1] + B[u][2] * pv[u][2]));
+            s1 = s1 + (B[u][3] * pv[u][0] + (B[u][4] * pv[u][1] + B[u][5] * pv[u][2]));
+            s2 = s2 + (B[u][6] * pv[u][0] + (B[u][7] * pv[u][1] + B[u][8] * pv[u][2]));
+          }
+        }
+      }
+      for (int off = 4; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off, 8);
+        s1 += __shfl_xor(s1, off, 8);
+        s2 += __shfl_xor(s2, off, 8);
+      }
+      if (cl == 0 && clr < a.pr) {
+        lcp[3 * clr] = s0;
+        lcp[3 * clr + 1] = s1;
+        lcp[3 * clr + 2] = s2;
+      }
+    }
+    __syncthreads();
+    if (live)
+      for (int c = 0; c < 3; ++c) ap[c] = ap[c] + lcp[3 * lr + c];
+  }
   // p.Ap partial, and the patch sum of A p for the coarse part.
   double sums[4] = {0.0, 0.0, 0.0, 0.0};
   if (q == 0 && live) {
