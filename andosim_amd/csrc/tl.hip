@@ -721,15 +721,21 @@ __global__ void __launch_bounds__(kBlock) k_tl_coarse(int ncb, int ncp, const in
 //
 // The pivot block is inverted by one wave with the block in registers: lane l
 // holds column l mod 32 of rows 16 (l / 32) .. 16 (l / 32) + 15.  Each of the
-// 32 pivot steps exchanges only the pivot row and column through LDS, so a step
-// costs two LDS round trips instead of a pass over the whole block; every
-// entry goes through the same operations in the same order as an in-LDS
-// elimination (new T_rc = T_rc - (T_rq / T_qq) T_qc with the reciprocal formed
-// once, row q scaled by it, column q by its negative, the pivot replaced by it).
+// 32 pivot steps broadcasts the pivot row and column through cross-lane
+// operations (the pivot by readlane; row q from this lane or from lane l xor 32;
+// column q from lane q or q + 32), with no LDS and no barrier; every entry goes
+// through the same operations in the same order as an in-LDS elimination (new
+// T_rc = T_rc - (T_rq / T_qq) T_qc with the reciprocal formed once, row q
+// scaled by it, column q by its negative, the pivot replaced by it).
 static_assert(kGjB == 32, "the register layout of k_tl_gj_pivot assumes 32 x 32 pivot blocks");
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(u & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(u >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 __global__ void __launch_bounds__(64) k_tl_gj_pivot(int ncp, int k, const double* __restrict__ X,
                                                     double* __restrict__ P, int* __restrict__ fail) {
-  __shared__ double colq[kGjB], rowq[kGjB];
   const int lane = threadIdx.x;
   const int c = lane & 31, h = lane >> 5;
   double A[16];
@@ -737,29 +743,27 @@ __global__ void __launch_bounds__(64) k_tl_gj_pivot(int ncp, int k, const double
   for (int i = 0; i < 16; ++i) A[i] = X[(size_t)(k * kGjB + 16 * h + i) * ncp + k * kGjB + c];
 #pragma unroll
   for (int q = 0; q < kGjB; ++q) {
-    if (c == q) {
-#pragma unroll
-      for (int i = 0; i < 16; ++i) colq[16 * h + i] = A[i];
-    }
-    if (h == (q >> 4)) rowq[c] = A[q & 15];
-    __syncthreads();
-    const double piv = colq[q];
+    const int hq = q >> 4, iq = q & 15;
+    const double piv = readlane_f64(A[iq], q + 32 * hq);  // T_qq
     if (!(piv > 0.0)) {
       if (lane == 0) *fail = 1;
       return;  // uniform: every lane read the same pivot
     }
     const double ip = 1.0 / piv;
-    const double rq = rowq[c];  // T_qc before the step
+    const double other = __shfl_xor(A[iq], 32, 64);
+    const double rq = h == hq ? A[iq] : other;  // T_qc before the step
+    double cqv[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) cqv[i] = __shfl(A[i], q + 32 * h, 64);  // T_rq before the step
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       const int r = 16 * h + i;
-      const double cq = colq[r];  // T_rq before the step
+      const double cq = cqv[i];
       if (r != q && c != q) A[i] = A[i] - cq * ip * rq;
       else if (r == q && c != q) A[i] = rq * ip;
       else if (r != q) A[i] = -cq * ip;
       else A[i] = ip;
     }
-    __syncthreads();  // the next step overwrites colq and rowq
   }
 #pragma unroll
   for (int i = 0; i < 16; ++i) P[(16 * h + i) * kGjB + c] = A[i];

@@ -185,7 +185,7 @@ def per_frame_leg(sim, sc, reps: int = 20, with_cpu: bool = True):
     return out
 
 
-def batched_leg(steps: int, count: int = 256):
+def batched_leg(steps: int, count: int = 256, accuracy: bool = False):
     """The batched workload (`count` cloths of side U{17..65}, 1-4 pins, wind,
     ground wall; about 481k vertices for 256), corrected physics: cloth-steps/s stepping the
     contexts one after another (the reference's only way) and through
@@ -194,15 +194,20 @@ def batched_leg(steps: int, count: int = 256):
     from andosim_amd import native, scenes
 
     out = {"workload": f"batch{count}: {count} independent cloths, sides U{{17..65}}", "substeps": steps}
+    scs = scenes.batch256(count=count)
+    big = int(np.argmax([len(sc.verts) for sc in scs]))
     for mode in ("sequential", "batched"):
         progress(f"batched leg: {mode}")
-        sims = [native.GpuSim(sc) for sc in scenes.batch256(count=count)]
+        sims = [native.GpuSim(sc) for sc in scs]
         for g in sims:
             g.set_mode(fp64_near=True, corrected_physics=True)
         for g in sims:  # warm-up substep (constants, allocations)
             g.substep()
         t = 0.0
-        for _ in range(steps):
+        before = None
+        for k in range(steps):
+            if mode == "batched" and k == steps - 1:
+                before = snapshot(sims[big])
             t0 = time.perf_counter()
             if mode == "sequential":
                 for g in sims:
@@ -218,6 +223,9 @@ def batched_leg(steps: int, count: int = 256):
                      "finite": all(finite_state(g) for g in sims)}
         if mode == "batched":
             out[mode]["host_threads"] = 64
+            if accuracy and before is not None:
+                out[mode]["accuracy_vs_fp64"] = accuracy_vs_fp64(scs[big], before, sims[big].get("x").astype(np.float64))
+                out[mode]["accuracy_vs_fp64"]["cloth"] = f"the largest ({len(scs[big].verts)} vertices), last substep"
             out[mode]["batched_solve_share"] = round(
                 sum(g.stats()["batched_solves"] for g in sims) / max(1, sum(g.stats()["pcg_solves"] for g in sims)), 3)
     return out
@@ -255,16 +263,20 @@ def rel_err(a, b) -> float:
     return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-30))
 
 
-def accuracy_vs_fp64(sc, state, x_gpu, x_fp32=None, physics: bool = True):
+def accuracy_vs_fp64(sc, state, x_gpu, x_fp32=None, physics: bool = True, extra=None):
     """max |x - x64| / max |x64| after one substep from `state` for the GPU and
-    for the FP32 oracle, against the FP64 oracle (host threads)."""
+    for the FP32 oracle, against the FP64 oracle (host threads); `extra` maps
+    names to further GPU results from the same state."""
     x64, t64, _ = oracle_substep(sc, state, 64, physics)
     if x_fp32 is None:
         x_fp32, _, _ = oracle_substep(sc, state, 32, physics)
-    return {"reference": "oracle in FP64 on the host, same start state, one substep",
-            "gpu_max_rel_err": float(f"{rel_err(x_gpu, x64):.3e}"),
-            "fp32_oracle_max_rel_err": float(f"{rel_err(x_fp32, x64):.3e}"),
-            "fp64_oracle_seconds": round(t64, 1)}
+    out = {"reference": "oracle in FP64 on the host, same start state, one substep",
+           "gpu_max_rel_err": float(f"{rel_err(x_gpu, x64):.3e}"),
+           "fp32_oracle_max_rel_err": float(f"{rel_err(x_fp32, x64):.3e}"),
+           "fp64_oracle_seconds": round(t64, 1)}
+    for name, x in (extra or {}).items():
+        out[name + "_max_rel_err"] = float(f"{rel_err(x, x64):.3e}")
+    return out
 
 
 def snapshot(sim):
@@ -484,8 +496,8 @@ def main(argv=None):
     ap.add_argument("--learned-steps", type=int, default=30)
     ap.add_argument("--train-steps", type=int, default=8000)
     ap.add_argument("--no-legs", action="store_true", help="skip the other workloads")
-    ap.add_argument("--leg-accuracy", action="store_true",
-                    help="FP64-oracle error of a mid-run substep for every workload leg (minutes of host time)")
+    ap.add_argument("--no-leg-accuracy", action="store_true",
+                    help="skip the FP64-oracle error of a mid-run substep of every workload leg (about 20 s of host time)")
     ap.add_argument("--corrected-steps", type=int, default=20)
     ap.add_argument("--batched-steps", type=int, default=5)
     ap.add_argument("--precond", choices=("two-level", "block-jacobi"), default="two-level",
@@ -614,10 +626,11 @@ def main(argv=None):
 
     legs = None
     if not a.no_legs:
-        legs = {"sphere129_corrected_physics": physics_leg("sphere129", 200, a.leg_accuracy, tl),
-                "stack65x8_corrected_physics": physics_leg("stack65x8", 300, a.leg_accuracy, tl),
+        acc = not (a.no_leg_accuracy or a.no_cpu)
+        legs = {"sphere129_corrected_physics": physics_leg("sphere129", 200, acc, tl),
+                "stack65x8_corrected_physics": physics_leg("stack65x8", 300, acc, tl),
                 "cyl257_faithful": faithful_leg(a.workload, 60, 20),
-                "batched": batched_leg(a.batched_steps),
+                "batched": batched_leg(a.batched_steps, accuracy=acc),
                 "stack_resting_modes": corrected_contact_leg(a.corrected_steps)}
 
     cpu = accuracy = None
@@ -643,8 +656,15 @@ def main(argv=None):
         restore_state(sim, state)
         mode(tl)
         sim.substep()
+        x_gpu = sim.get("x").astype(np.float64)
+        # The same substep with the other preconditioner on the GPU.
+        restore_state(sim, state)
+        mode(not tl)
+        sim.substep()
+        x_other = sim.get("x").astype(np.float64)
         mode(False)
-        accuracy = accuracy_vs_fp64(sc, state, sim.get("x").astype(np.float64), x32)
+        accuracy = accuracy_vs_fp64(sc, state, x_gpu, x32,
+                                    extra={"gpu_block_jacobi" if tl else "gpu_two_level": x_other})
         accuracy["at_substep"] = at
 
     out = {
