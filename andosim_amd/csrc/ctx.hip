@@ -1400,6 +1400,10 @@ void autotune_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const i
 struct ChunkPlan {
   float last_rel = 1.0f;
   int last_it = 0;
+  // Iterations from here on lie past the predicted convergence: their launches
+  // read the done flag first (spec), so those that find the solve over cost
+  // little; the others pay one flag read before their loads.
+  int spec_from = 1 << 30;
   int next(int chunk, int issued, const PcgStatus& st, float tol, int chunk_max) {
     int out = std::min(chunk * 2, chunk_max);
     if (tol > 0.0f && st.rel_it > last_it && st.rel > 0.0f && st.rel < last_rel) {
@@ -1409,6 +1413,7 @@ struct ChunkPlan {
       if (std::isfinite(need) && need > 0.0) {
         const double ahead = 1.15 * need + 2.0 - (double)(issued - st.rel_it);
         out = (int)std::min<double>((double)chunk_max, std::max(8.0, std::ceil(ahead)));
+        spec_from = st.rel_it + (int)std::min(std::ceil(need), 1e9);
       }
     }
     if (st.rel_it > last_it) {
@@ -1575,6 +1580,7 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
       float* pold = (it == 0) ? c->pbuf0.p : ((it - 1) & 1 ? c->pbuf1.p : c->pbuf0.p);
       float* pcur = (it & 1) ? c->pbuf1.p : c->pbuf0.p;
       hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+      const int spec = it >= plan.spec_from ? 1 : 0;
       if (timing) {
         e0 = ev(ev_used); e1 = ev(ev_used + 1); e2 = ev(ev_used + 2); e3 = ev(ev_used + 3);
         ev_used += 4;
@@ -1583,7 +1589,7 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
         hipExtLaunchKernelGGL(kernel, dim3(nwg_r + nwg_c), dim3(kBlock), 0, s, e0, e1, 0, n, it, A, rp, cols,
                               (const float*)c->z.p, (const float*)pold, pcur, c->Ap.p, (const double*)c->part_rz.p,
                               (const float*)c->part_max.p, nwg_v, c->rz_hist.p, c->part_pAp.p, c->pcg_status.p, tol,
-                              cpl);
+                              cpl, spec);
       };
       if (sell) {
         auto sell_launch = [&](auto kernel, int threads) {
@@ -1592,7 +1598,7 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
                                 (const int32_t*)sm->slice_off.p, (const int32_t*)sm->slice_w.p, sm->uniform_w,
                                 (const float*)c->z.p, (const float*)pold, pcur, c->Ap.p,
                                 (const double*)c->part_rz.p, (const float*)c->part_max.p, nwg_v, c->rz_hist.p,
-                                c->part_pAp.p, c->pcg_status.p, tol, cpl);
+                                c->part_pAp.p, c->pcg_status.p, tol, cpl, spec);
         };
         if (c->sell_lanes == 2 && sell_slices == 0) sell_launch(k_pcg_spmv_sell_split<2, 0>, 64);
         else if (c->sell_lanes == 2 && sell_slices == 1) sell_launch(k_pcg_spmv_sell_split<2, 1>, 128);
@@ -1611,7 +1617,7 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
       hipExtLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(vb), 0, s, e2, e3, 0, n, it, (const float*)pcur,
                             (const float*)c->Ap.p, (const float*)cpl.Ap2, Pc, pe, pi, xsol, c->r.p, c->z.p,
                             (const double*)c->part_pAp.p, nwg_r + nwg_c, (const double*)c->rz_hist.p, c->part_rz.p,
-                            c->part_max.p, c->pcg_status.p);
+                            c->part_max.p, c->pcg_status.p, spec);
     }
     HIPCHK(hipGetLastError());
   };
@@ -1751,6 +1757,7 @@ PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, c
       float* pold = (it == 0) ? T.p0.p : ((it - 1) & 1 ? T.p1.p : T.p0.p);
       float* pcur = (it & 1) ? T.p1.p : T.p0.p;
       hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+      const int spec = it >= plan.spec_from ? 1 : 0;
       if (timing) {
         e0 = ev(ev_used); e1 = ev(ev_used + 1); e2 = ev(ev_used + 2); e3 = ev(ev_used + 3);
         ev_used += 4;
@@ -1760,11 +1767,11 @@ PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, c
       hipExtLaunchKernelGGL(k_tl_spmv, dim3(T.npatch), dim3(spmv_threads(T.pr)), 0, s, e0, e1, 0, a, it,
                             (const float*)T.z.p, (const float*)pold, pcur, T.Ap.p, T.sa.p,
                             (const double*)T.part_rz.p, (const float*)T.part_max.p, c->rz_hist.p, T.part_pAp.p,
-                            c->pcg_status.p, tol);
+                            c->pcg_status.p, tol, spec);
       hipExtLaunchKernelGGL(k_tl_update, dim3(T.npatch), dim3(patch_threads(T.pr)), 0, s, e2, e3, 0, a, it, (const float*)pcur,
                             (const float*)T.Ap.p, (const double*)sr_cur, (const double*)T.sa.p, sr_next, T.x.p,
                             T.r.p, T.z.p, (const double*)T.part_pAp.p, (const double*)c->rz_hist.p, T.part_rz.p,
-                            T.part_max.p, c->pcg_status.p);
+                            T.part_max.p, c->pcg_status.p, spec);
     }
     HIPCHK(hipGetLastError());
   };
@@ -3054,10 +3061,10 @@ int ando_pcg_noalloc(int32_t device, int32_t num_rows, int32_t num_blocks, const
         hipLaunchKernelGGL(k_pcg_spmv<R>, dim3(nwg_r), dim3(kBlock), 0, s, n, it, values, row_ptr, cols,
                            (const float*)z, (const float*)pold, pcur, Ap, (const double*)part_rz,
                            (const float*)part_max, nwg_v, rz_hist, part_pAp, st, tol,
-                           ando::Coupling{0, nwg_r, 0, nullptr, nullptr, nullptr, nullptr, nullptr});
+                           ando::Coupling{0, nwg_r, 0, nullptr, nullptr, nullptr, nullptr, nullptr}, 0);
         hipLaunchKernelGGL(k_pcg_update, dim3(nwg_v), dim3(kBlock), 0, s, n, it, (const float*)pcur,
                            (const float*)Ap, (const float*)nullptr, (const float*)Pinv, n, 1, x, r, z, (const double*)part_pAp, nwg_r,
-                           (const double*)rz_hist, part_rz, part_max, st);
+                           (const double*)rz_hist, part_rz, part_max, st, 0);
       }
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));

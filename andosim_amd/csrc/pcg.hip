@@ -455,7 +455,10 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv(int n, int it, const float*
                            float* __restrict__ Ap, const double* __restrict__ part_rz,
                            const float* __restrict__ part_max, int nwg_v, double* __restrict__ rz_hist,
                            double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol,
-                           Coupling cpl) {
+                           Coupling cpl, int spec) {
+  // A launch the host queued past the predicted convergence (spec) reads the
+  // done flag before anything else and returns at once once the solve is over.
+  if (spec && st->done) return;
   constexpr int kRowsPerGroup = kRowsPerPass * R;
   if ((int)blockIdx.x < cpl.nwg_couple) {
     couple_rows_body((int)blockIdx.x, cpl, n, it, z, p_old, part_rz, part_max, nwg_v, rz_hist, part_pAp, st, tol);
@@ -621,7 +624,10 @@ __global__ void __launch_bounds__(kBlock) k_pcg_spmv_sell(int n, int slices, int
                            const float* __restrict__ p_old, float* __restrict__ p_new, float* __restrict__ Ap,
                            const double* __restrict__ part_rz, const float* __restrict__ part_max, int nwg_v,
                            double* __restrict__ rz_hist, double* __restrict__ part_pAp, PcgStatus* __restrict__ st,
-                           float tol, Coupling cpl) {
+                           float tol, Coupling cpl, int spec) {
+  // A launch the host queued past the predicted convergence (spec) reads the
+  // done flag before anything else and returns at once once the solve is over.
+  if (spec && st->done) return;
   if ((int)blockIdx.x < cpl.nwg_couple) {
     couple_rows_body((int)blockIdx.x, cpl, n, it, z, p_old, part_rz, part_max, nwg_v, rz_hist, part_pAp, st, tol);
     return;
@@ -973,7 +979,10 @@ __global__ void __launch_bounds__(64 * (S == 0 ? 1 : S * L)) k_pcg_spmv_sell_spl
                            float* __restrict__ Ap, const double* __restrict__ part_rz,
                            const float* __restrict__ part_max, int nwg_v, double* __restrict__ rz_hist,
                            double* __restrict__ part_pAp, PcgStatus* __restrict__ st, float tol,
-                           Coupling cpl) {
+                           Coupling cpl, int spec) {
+  // A launch the host queued past the predicted convergence (spec) reads the
+  // done flag before anything else and returns at once once the solve is over.
+  if (spec && st->done) return;
   if ((int)blockIdx.x < cpl.nwg_couple) {
     couple_rows_body((int)blockIdx.x, cpl, n, it, z, p_old, part_rz, part_max, nwg_v, rz_hist, part_pAp, st, tol);
     return;
@@ -1054,7 +1063,9 @@ __global__ void __launch_bounds__(kBlock) k_pcg_update(int n, int it, const floa
                              const float* __restrict__ Ap, const float* __restrict__ Ap2, const float* __restrict__ Pinv, int pe,
                              int pi, float* __restrict__ x, float* __restrict__ r, float* __restrict__ z,
                              const double* __restrict__ part_pAp, int nwg_r, const double* __restrict__ rz_hist,
-                             double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st) {
+                             double* __restrict__ part_rz, float* __restrict__ part_max, PcgStatus* __restrict__ st,
+                             int spec) {
+  if (spec && st->done) return;  // queued past the predicted convergence: flag first
   update_body(blockIdx.x, n, it, p, Ap, Ap2, Pinv, pe, pi, x, r, z, part_pAp, nwg_r, rz_hist, part_rz, part_max, st);
 }
 

@@ -401,7 +401,8 @@ __global__ void __launch_bounds__(2 * kMaxPatchRows) k_tl_spmv(TlArgs a, int it,
                                                                const float* __restrict__ part_max,
                                                                double* __restrict__ rz_hist,
                                                                double* __restrict__ part_pAp,
-                                                               PcgStatus* __restrict__ st, float tol) {
+                                                               PcgStatus* __restrict__ st, float tol, int spec) {
+  if (spec && st->done) return;  // queued past the predicted convergence: flag first
   __shared__ double ldd[4][kMaxWaves];
   __shared__ double ldsd[kMaxWaves];
   __shared__ float ldsf[kMaxWaves];
@@ -589,7 +590,8 @@ __global__ void __launch_bounds__(kPatchLanes * kMaxPatchRows) k_tl_update(TlArg
                                                              const double* __restrict__ rz_hist,
                                                              double* __restrict__ part_rz,
                                                              float* __restrict__ part_max,
-                                                             PcgStatus* __restrict__ st) {
+                                                             PcgStatus* __restrict__ st, int spec) {
+  if (spec && st->done) return;  // queued past the predicted convergence: flag first
   __shared__ float lv[2 * 3 * kMaxPatchRows];
   __shared__ double ldd[7][kMaxWaves];
   __shared__ float ldsf[kMaxWaves];

@@ -263,10 +263,13 @@ def rel_err(a, b) -> float:
     return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-30))
 
 
-def accuracy_vs_fp64(sc, state, x_gpu, x_fp32=None, physics: bool = True, extra=None):
+def accuracy_vs_fp64(sc, state, x_gpu, x_fp32=None, physics: bool = True, extra=None, envelope: bool = False):
     """max |x - x64| / max |x64| after one substep from `state` for the GPU and
     for the FP32 oracle, against the FP64 oracle (host threads); `extra` maps
-    names to further GPU results from the same state."""
+    names to further GPU results from the same state.  With `envelope`, also
+    the FP32 oracle from the state with one coordinate moved by one ulp: how
+    far rounding alone moves an FP32 substep (contact-heavy substeps amplify
+    it), the scale against which the GPU's distance is read."""
     x64, t64, _ = oracle_substep(sc, state, 64, physics)
     if x_fp32 is None:
         x_fp32, _, _ = oracle_substep(sc, state, 32, physics)
@@ -276,6 +279,16 @@ def accuracy_vs_fp64(sc, state, x_gpu, x_fp32=None, physics: bool = True, extra=
            "fp64_oracle_seconds": round(t64, 1)}
     for name, x in (extra or {}).items():
         out[name + "_max_rel_err"] = float(f"{rel_err(x, x64):.3e}")
+    if envelope:
+        x0, v0, rig = state
+        xp = x0.copy()
+        i = int(np.argmax(np.abs(x0[:, 2])))
+        xp[i, 2] = np.nextafter(xp[i, 2], np.float32(np.inf), dtype=np.float32)
+        x32p, _, _ = oracle_substep(sc, (xp, v0, rig), 32, physics)
+        out["fp32_oracle_one_ulp_max_rel_err"] = float(f"{rel_err(x32p, x64):.3e}")
+        out["fp32_oracle_one_ulp_vs_fp32_oracle"] = float(f"{rel_err(x32p, x_fp32):.3e}")
+        out["gpu_vs_fp32_oracle"] = float(f"{rel_err(x_gpu, x_fp32):.3e}")
+        out["one_ulp"] = f"z of vertex {i} moved up by one FP32 ulp"
     return out
 
 
@@ -664,7 +677,7 @@ def main(argv=None):
         x_other = sim.get("x").astype(np.float64)
         mode(False)
         accuracy = accuracy_vs_fp64(sc, state, x_gpu, x32,
-                                    extra={"gpu_block_jacobi" if tl else "gpu_two_level": x_other})
+                                    extra={"gpu_block_jacobi" if tl else "gpu_two_level": x_other}, envelope=True)
         accuracy["at_substep"] = at
 
     out = {
