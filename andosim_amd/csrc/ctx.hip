@@ -1391,12 +1391,12 @@ void autotune_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const i
 // reads the status, which carries the max-norm relative residual of the
 // latest convergence test (rel at iteration rel_it).  From the rate at which
 // its logarithm fell since the previous read (from 1 at iteration 0 on the
-// first read) the iterations still needed are estimated and queued with a 15%
-// margin plus two, at least 8 and at most the largest chunk; without a usable
-// rate (a residual that rose, a tolerance of 0) the chunk doubles.  Launches
-// queued past convergence still issue their first loads and cost a few
-// microseconds each, while a status read costs one host round trip, so a
-// close estimate pays.  The chunking changes no arithmetic.
+// first read) the iterations still needed are estimated and queued (plus one;
+// at least 8, at most the largest chunk); without a usable rate (a residual
+// that rose, a tolerance of 0) the chunk doubles.  Launches queued past
+// convergence cost a few microseconds each when they issue their loads first
+// and about one when they read the done flag first (spec), while a status
+// read costs one host round trip.  The chunking changes no arithmetic.
 struct ChunkPlan {
   float last_rel = 1.0f;
   int last_it = 0;
@@ -1411,9 +1411,11 @@ struct ChunkPlan {
           (std::log((double)st.rel) - std::log((double)last_rel)) / (double)(st.rel_it - last_it);
       const double need = std::log((double)tol / (double)st.rel) / rate;  // iterations past rel_it
       if (std::isfinite(need) && need > 0.0) {
-        const double ahead = 1.15 * need + 2.0 - (double)(issued - st.rel_it);
+        const double ahead = need + 1.0 - (double)(issued - st.rel_it);
         out = (int)std::min<double>((double)chunk_max, std::max(8.0, std::ceil(ahead)));
-        spec_from = st.rel_it + (int)std::min(std::ceil(need), 1e9);
+        // CG usually converges faster than the rate so far predicts, so the
+        // last quarter of the prediction is already queued as speculative.
+        spec_from = st.rel_it + (int)std::min(std::floor(0.75 * need), 1e9);
       }
     }
     if (st.rel_it > last_it) {
