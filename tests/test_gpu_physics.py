@@ -232,3 +232,41 @@ def test_physics_drape_stays_intersection_free(name, make, steps):
             assert np.all(np.isfinite(g.get("x"))), k
             assert g.count_intersections() == (0, 0), (name, k)
     print(f"{name}: pcg failures {fails}", flush=True)
+
+
+def test_physics_linearize_bit_exact_on_cyl257_in_contact():
+    """The headline workload at full size (257 x 257 cloth on two cylinders,
+    corrected physics): the state after 15 substeps on the GPU (the cloth lies
+    on the cylinders, thousands of self and rigid candidates), copied into the
+    FP32 oracle; the Newton system there -- contact candidates from the swept
+    boxes, barriers, membrane, strain limiting, pins, inertia -- is
+    bit-identical (FP64 near-contact island off)."""
+    print("physics linearize cyl257 after 15 substeps", flush=True)
+    sc = scenes.cyl257()
+    g = gpu_sim(sc, fp64_near=True)
+    for _ in range(15):
+        g.substep()
+    st = g.stats()
+    assert st["rigid_contacts"] > 0 and st["contacts"] > 1000, st["contacts"]
+    x, v = g.get("x").copy(), g.get("v").copy()
+    o = oracle_sim(sc)
+    o.set("x", x)
+    o.set("v", v)
+    for b in range(len(sc.rigids)):
+        o.set_rigid_pose(b, g.rigid_state(b))
+    t = gpu_sim(sc, fp64_near=False)
+    t.set_state(x, v)
+    for b in range(len(sc.rigids)):
+        rs, dst = g.rigid_state(b), t.bodies[b]
+        for k in range(3):
+            dst.position[k] = float(np.float32(rs["position"][k]))
+            dst.linear_velocity[k] = float(np.float32(rs["linear_velocity"][k]))
+            dst.angular_velocity[k] = float(np.float32(rs["angular_velocity"][k]))
+        for k in range(9):
+            dst.rotation[k] = float(np.float32(rs["rotation"].ravel()[k]))
+    xhat = (x + np.float32(sc.params["dt"]) * v).astype(np.float32)
+    go, Ho = o.linearize(xhat)
+    gg, Hg = t.linearize(xhat)
+    print(f"  contacts of the last substep {st['contacts']} (rigid {st['rigid_contacts']})", flush=True)
+    assert np.array_equal(gg.reshape(-1), go.astype(np.float32).reshape(-1)), rel_err(gg, go.reshape(-1))
+    assert np.array_equal(Hg, Ho.astype(np.float32)), rel_err(Hg, Ho)
