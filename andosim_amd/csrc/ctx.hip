@@ -218,7 +218,7 @@ struct TlMeta {
   DBuf<float> Dinv, lmax, x, r, z, p0, p1, Ap, qf;
   DBuf<double> sa, sr0, sr1;  // per patch: sums of A p and of r (3 each)
   DBuf<double> part_rz, part_pAp;
-  DBuf<float> part_max;
+  DBuf<float> part_max, part_bmax;
 };
 
 // Coarse unknowns stay dense: at most this many patches get a coarse level.
@@ -502,6 +502,7 @@ struct ando_ctx {
   float tl_ratio = 30.0f;   // ANDO_TL_RATIO: lmax / lmin of the Chebyshev interval
   int tl_coarse = 1;        // ANDO_TL_COARSE=0: no coarse level (patch polynomials only)
   int tl_coarse_every = 2;  // ANDO_TL_COARSE_EVERY=k: the coarse inverse is rebuilt on every k-th solve
+  int tl_warm = 0;          // ANDO_TL_WARM=1: Newton iterations after the first start from the scaled last direction
   std::vector<int32_t> morton;  // the mesh's vertices in Morton order of the rest positions
   TlMeta tl;                // for the mesh's block pattern
   ~ando_ctx() {
@@ -1661,7 +1662,8 @@ PcgOut run_pcg(ando_ctx* c, int n, const float* A, const int32_t* rp, const int3
 // Two-level Schwarz PCG (tl.hip) on the matrix A with pattern T; same
 // contract as run_pcg with x0 = 0 (b = -g when negate), the result in xsol.
 PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, const float* rhs, int negate,
-                  float* xsol, float tol, int maxit, const ando::SelfView* sv) {
+                  float* xsol, float tol, int maxit, const ando::SelfView* sv, const float* x0 = nullptr,
+                  float x0_scale = 0.0f) {
   using namespace ando::tl;
   hipStream_t s = c->stream;
   const int n = T.n;
@@ -1731,14 +1733,35 @@ PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, c
   a.cptr = couple ? sv->ptr : nullptr;
   a.ccol = couple ? c->ccol.p : nullptr;
   a.cval = couple ? c->cval.p : nullptr;
-  hipLaunchKernelGGL(k_tl_init, dim3(nblk(n)), dim3(kBlock), 0, s, a, A, diag, rhs, negate, T.x.p, T.r.p, T.Dinv.p);
-  hipLaunchKernelGGL(k_tl_prep, dim3(T.npatch), dim3(patch_threads(T.pr)), 0, s, a, T.lmax.p, (const float*)T.r.p, T.qf.p,
-                     T.sr0.p);
-  hipLaunchKernelGGL(k_tl_start, dim3(T.npatch), dim3(64 * ((T.pr + 63) / 64)), 0, s, a, (const float*)T.r.p, (const float*)T.qf.p,
-                     (const double*)T.sr0.p, T.z.p, T.p0.p, T.part_rz.p, T.part_max.p);
-  hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, s, T.npatch, (const double*)T.part_rz.p,
-                     (const float*)T.part_max.p, (const float*)nullptr, c->rz_hist.p, c->pcg_status.p, tol);
-  HIPCHK(hipGetLastError());
+  // Start: x0 = 0, or a warm start x0 = x0_scale * x0 (r0 = b - A x0, the
+  // residual still measured against max|b|), dropped for x0 = 0 when its
+  // residual is no smaller than b's.
+  auto start = [&](bool warm) {
+    if (warm) {
+      hipLaunchKernelGGL(k_tl_gather_x0, dim3(nblk(n)), dim3(kBlock), 0, s, n, (const int32_t*)T.perm.p, x0, x0_scale,
+                         T.x.p);
+      hipLaunchKernelGGL(k_tl_ax0, dim3(nblk(n)), dim3(kBlock), 0, s, a, (const float*)T.x.p, T.Ap.p);
+      T.part_bmax.ensure((size_t)nblk(n) + 1);
+    }
+    hipLaunchKernelGGL(k_tl_init, dim3(nblk(n)), dim3(kBlock), 0, s, a, A, diag, rhs, negate, T.x.p, T.r.p, T.Dinv.p,
+                       warm ? (const float*)T.Ap.p : nullptr, warm ? T.part_bmax.p : nullptr);
+    hipLaunchKernelGGL(k_tl_prep, dim3(T.npatch), dim3(patch_threads(T.pr)), 0, s, a, T.lmax.p, (const float*)T.r.p,
+                       T.qf.p, T.sr0.p);
+    hipLaunchKernelGGL(k_tl_start, dim3(T.npatch), dim3(64 * ((T.pr + 63) / 64)), 0, s, a, (const float*)T.r.p,
+                       (const float*)T.qf.p, (const double*)T.sr0.p, T.z.p, T.p0.p, T.part_rz.p, T.part_max.p);
+    hipLaunchKernelGGL(k_pcg_start, dim3(1), dim3(kBlock), 0, s, T.npatch, (const double*)T.part_rz.p,
+                       (const float*)T.part_max.p, (const float*)nullptr, c->rz_hist.p, c->pcg_status.p, tol);
+    if (warm)
+      hipLaunchKernelGGL(k_tl_fix_bmax, dim3(1), dim3(kBlock), 0, s, nblk(n), (const float*)T.part_bmax.p,
+                         (const float*)T.part_max.p, T.npatch, c->pcg_status.p, tol);
+    HIPCHK(hipGetLastError());
+  };
+  start(x0 != nullptr);
+  if (x0 != nullptr) {
+    PcgStatus s0{};
+    read_small(c, &s0, (const PcgStatus*)c->pcg_status.p, 1);
+    if (!(s0.rel < 1.0f)) start(false);
+  }
   PcgStatus st{};
   int it = 0;
   int chunk = std::max(1, c->pcg_chunk);
@@ -2136,6 +2159,10 @@ float inner_newton(ando_ctx* c, const ando_params& p) {
   int maxit = p.max_newton_steps;
   if (p.enable_friction) maxit = std::max(maxit, p.friction_min_newton_steps);
   hipStream_t s = c->stream;
+  // The last iteration's accepted step: x moved by 1.25 alpha d, so the next
+  // Newton direction is close to (1 - 1.25 alpha) d where the system changed
+  // little (the two-level solver's optional warm start).
+  float prev_alpha = -1.0f;
   for (int it = 0; it < maxit; ++it) {
     c->stats.newton_iterations++;
     if (c->cp) refresh_cp(c, p);
@@ -2170,7 +2197,10 @@ float inner_newton(ando_ctx* c, const ando_params& p) {
         po.iters += first;
       }
     } else if (c->two_level) {
-      po = run_pcg_tl(c, mesh_tl(c), c->H.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol, p.pcg_max_iters, &sv);
+      const float ws = 1.0f - 1.25f * prev_alpha;
+      const bool warm = c->tl_warm && prev_alpha >= 0.0f && ws > 0.0f;
+      po = run_pcg_tl(c, mesh_tl(c), c->H.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol, p.pcg_max_iters, &sv,
+                      warm ? (const float*)c->d.p : nullptr, ws);
     } else {
       po = run_pcg(c, c->n, c->H.p, c->d_row_ptr.p, c->d_cols.p, c->d_diag.p, c->g.p, 1, c->d.p, p.pcg_tol,
                    p.pcg_max_iters, &c->spmv_meta, nullptr, nullptr, &sv);
@@ -2189,6 +2219,7 @@ float inner_newton(ando_ctx* c, const ando_params& p) {
     hipLaunchKernelGGL(k_axpy, dim3(nblk(3LL * c->n)), dim3(kBlock), 0, s, 3 * c->n, sc, c->d.p, c->x.p);
     HIPCHK(hipGetLastError());
     if ((double)alpha > 0.99) return 1.0f;
+    prev_alpha = alpha;
   }
   return 0.5f;
 }
@@ -2383,6 +2414,7 @@ int ando_create(int32_t device, ando_ctx** out) {
     if (const char* w = std::getenv("ANDO_TL_RATIO")) c->tl_ratio = std::max(1.5f, (float)std::atof(w));
     if (const char* w = std::getenv("ANDO_TL_COARSE")) c->tl_coarse = std::atoi(w) != 0 ? 1 : 0;
     if (const char* w = std::getenv("ANDO_TL_COARSE_EVERY")) c->tl_coarse_every = std::max(1, std::atoi(w));
+    if (const char* w = std::getenv("ANDO_TL_WARM")) c->tl_warm = std::atoi(w) != 0 ? 1 : 0;
     if (const char* w = std::getenv("ANDO_PCG_ROWSETS")) c->spmv_rowsets = std::min(4, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_SPMV")) {
       c->spmv_env = std::strcmp(w, "sell") == 0 ? 2 : 0;

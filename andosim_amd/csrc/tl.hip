@@ -269,13 +269,77 @@ __device__ __forceinline__ void coarse_rows(const TlArgs& a, int patch, const do
   }
 }
 
-// Solve set-up, per permuted row: r = +-g[perm], x = 0, the block-Jacobi
-// inverse of the row's diagonal block (pcg_solver.cpp:77-135).
+// Warm start: x0 (permuted) = s * x0_orig[perm].
+__global__ void k_tl_gather_x0(int n, const int32_t* __restrict__ perm, const float* __restrict__ x0, float s,
+                               float* __restrict__ xp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = perm[i];
+  for (int c = 0; c < 3; ++c) xp[3 * i + c] = s * x0[3 * o + c];
+}
+
+// out = A x (permuted rows and columns; the mesh blocks from the sliced
+// layout in slot order, then the row's self-contact coupling blocks), one
+// thread per row: the starting residual of a warm-started solve.
+__global__ void __launch_bounds__(kBlock) k_tl_ax0(TlArgs a, const float* __restrict__ xp, float* __restrict__ out) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= a.n) return;
+  const SliceRow R = slice_row(a, row);
+  float T[3] = {0.0f, 0.0f, 0.0f};
+  for (int k = 0; k < R.W; ++k) {
+    const int j = slot_col(a, R, k);
+    float B[9];
+    for (int e = 0; e < 9; ++e) B[e] = slot_val(a, R, k, e);
+    T[0] = T[0] + (B[0] * xp[3 * j] + (B[1] * xp[3 * j + 1] + B[2] * xp[3 * j + 2]));
+    T[1] = T[1] + (B[3] * xp[3 * j] + (B[4] * xp[3 * j + 1] + B[5] * xp[3 * j + 2]));
+    T[2] = T[2] + (B[6] * xp[3 * j] + (B[7] * xp[3 * j + 1] + B[8] * xp[3 * j + 2]));
+  }
+  if (a.cptr) {
+    const int o = a.perm[row];
+    for (int b = 3 * a.cptr[o]; b < 3 * a.cptr[o + 1]; ++b) {
+      const int j = a.iperm[a.ccol[b]];
+      const float* B = a.cval + 9 * (size_t)b;
+      T[0] = T[0] + (B[0] * xp[3 * j] + (B[1] * xp[3 * j + 1] + B[2] * xp[3 * j + 2]));
+      T[1] = T[1] + (B[3] * xp[3 * j] + (B[4] * xp[3 * j + 1] + B[5] * xp[3 * j + 2]));
+      T[2] = T[2] + (B[6] * xp[3 * j] + (B[7] * xp[3 * j + 1] + B[8] * xp[3 * j + 2]));
+    }
+  }
+  for (int c = 0; c < 3; ++c) out[3 * row + c] = T[c];
+}
+
+// With a warm start the relative residual is measured against max|b|, not
+// max|r0|: the status written by the start gets that denominator.
+__global__ void k_tl_fix_bmax(int nwg, const float* __restrict__ part_bmax, const float* __restrict__ part_max,
+                              int nmax, PcgStatus* __restrict__ st, float tol) {
+  __shared__ float ldsf[4];
+  const float bmax = reduce_max_partials(part_bmax, nwg, ldsf);
+  const float rmax = reduce_max_partials(part_max, nmax, ldsf);
+  if (threadIdx.x == 0) {
+    const float rel = ((double)bmax < 1e-16) ? rmax : rmax / bmax;
+    st->bmax = bmax;
+    st->rel = rel;
+    st->done = rel < tol ? 1 : 0;
+  }
+}
+
+// Solve set-up, per permuted row: r = +-g[perm] (minus A x0 when ax0 is
+// given, keeping x = x0; else x = 0), the block-Jacobi inverse of the row's
+// diagonal block (pcg_solver.cpp:77-135), and with part_bmax the
+// workgroup's max|b|.
 __global__ void __launch_bounds__(kBlock) k_tl_init(TlArgs a, const float* __restrict__ A,
                                                     const int32_t* __restrict__ diag, const float* __restrict__ g,
                                                     int negate, float* __restrict__ x, float* __restrict__ r,
-                                                    float* __restrict__ Dinv) {
+                                                    float* __restrict__ Dinv, const float* __restrict__ ax0,
+                                                    float* __restrict__ part_bmax) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (part_bmax) {
+    __shared__ float ldsf[4];
+    float bm = 0.0f;
+    if (i < a.n)
+      for (int c = 0; c < 3; ++c) bm = fmaxf(bm, fabsf(g[3 * a.perm[i] + c]));
+    bm = block_reduce(bm, ldsf, [](float u, float w) { return fmaxf(u, w); });
+    if (threadIdx.x == 0) part_bmax[blockIdx.x] = bm;
+  }
   if (i >= a.n) return;
   const int o = a.perm[i];
   const float* B = A + 9 * (size_t)diag[o];
@@ -293,8 +357,13 @@ __global__ void __launch_bounds__(kBlock) k_tl_init(TlArgs a, const float* __res
   for (int e = 0; e < 9; ++e) Dinv[(size_t)e * a.n + i] = P[e];
   for (int c = 0; c < 3; ++c) {
     const float gv = g[3 * o + c];
-    r[3 * i + c] = negate ? -gv : gv;
-    x[3 * i + c] = 0.0f;
+    const float bv = negate ? -gv : gv;
+    if (ax0) {
+      r[3 * i + c] = bv - ax0[3 * i + c];
+    } else {
+      r[3 * i + c] = bv;
+      x[3 * i + c] = 0.0f;
+    }
   }
 }
 

@@ -49,6 +49,45 @@ def test_two_level_random_patterns(n, seed):
     print(f"  iterations two-level {it} (model {it_m}), block-Jacobi {it_p}", flush=True)
 
 
+@pytest.mark.parametrize("env,degree,coarse", [({"ANDO_TL_CHEB": "3"}, 3, True), ({"ANDO_TL_CHEB": "7"}, 7, True),
+                                              ({"ANDO_TL_COARSE": "0"}, 5, False)])
+def test_two_level_options_match_model(monkeypatch, env, degree, coarse):
+    """The solver's options (read when the context is created): another
+    Chebyshev degree, or no coarse level; iteration counts next to the CPU
+    model's with the same settings, residual within reach."""
+    print(f"two-level options {env}", flush=True)
+    n = 1000
+    rp, ci, vals, b = random_spd_bsr(n, 7, extra_rows=(0, n // 2, n - 1))
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    sim = native.GpuSim()
+    sim.set_mode(two_level=True)
+    x, it, ok = sim.pcg(rp, ci, vals, b, tol=1e-6, maxit=2000)
+    assert ok
+    assert dense_residual(rp, ci, vals, x, b) <= 1e-5 * np.max(np.abs(b))
+    A = tlm.bsr_to_csr(rp, ci, vals, n)
+    _, it_m, ok_m = tlm.solve(A, b, np.arange(n), maxit=2000, degree=degree, coarse=coarse)
+    print(f"  iterations {it} (model {it_m})", flush=True)
+    assert ok_m
+    assert abs(it - it_m) <= max(3, 0.15 * it_m), (it, it_m)
+
+
+def test_two_level_coarse_every_solve_drape(monkeypatch):
+    """ANDO_TL_COARSE_EVERY=1 (the coarse inverse rebuilt for every solve of
+    the step, not every second one): a 65 x 65 cylinder drape stays finite
+    and intersection-free over 30 substeps, with the two-level solver."""
+    print("two-level drape, coarse inverse every solve", flush=True)
+    monkeypatch.setenv("ANDO_TL_COARSE_EVERY", "1")
+    sc = scenes.cyl257(s=65)
+    g = native.GpuSim(sc)
+    g.set_mode(fp64_near=True, corrected_physics=True, two_level=True)
+    for k in range(30):
+        g.substep()
+        assert g.stats()["pcg_kernel"] == 4
+    assert np.all(np.isfinite(g.get("x")))
+    assert g.count_intersections() == (0, 0)
+
+
 def cloth_system(sc, substeps):
     """The first Newton system after `substeps` corrected-physics substeps."""
     sim = native.GpuSim(sc)
