@@ -1321,6 +1321,7 @@ PYBIND11_MODULE(ando_barrier_core, m) {
       .def_readonly("long_hit_sorts", &ando_stats::long_hit_sorts)
       .def_readonly("max_candidates", &ando_stats::max_candidates)
       .def_readonly("held_iterations", &ando_stats::held_iterations)
+      .def_readonly("capped_trials", &ando_stats::capped_trials)
       .def_readonly("timed_coupling_bytes", &ando_stats::timed_coupling_bytes)
       .def_readonly("learned_solves", &ando_stats::learned_solves)
       .def_readonly("learned_fallbacks", &ando_stats::learned_fallbacks)

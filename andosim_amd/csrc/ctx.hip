@@ -40,6 +40,10 @@ struct Fail : std::runtime_error {
   int code;
   Fail(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
+// A detection whose candidates would exceed kMaxCandidates.
+struct CandidateOverflow : Fail {
+  using Fail::Fail;
+};
 
 #define HIPCHK(expr)                                                                        \
   do {                                                                                      \
@@ -502,7 +506,7 @@ struct ando_ctx {
   float tl_ratio = 30.0f;   // ANDO_TL_RATIO: lmax / lmin of the Chebyshev interval
   int tl_coarse = 1;        // ANDO_TL_COARSE=0: no coarse level (patch polynomials only)
   int tl_coarse_every = 2;  // ANDO_TL_COARSE_EVERY=k: the coarse inverse is rebuilt on every k-th solve
-  int tl_warm = 0;          // ANDO_TL_WARM=1: Newton iterations after the first start from the scaled last direction
+  int tl_warm = 1;          // ANDO_TL_WARM=0: every Newton solve starts from x0 = 0 (else from the scaled last direction)
   std::vector<int32_t> morton;  // the mesh's vertices in Morton order of the rest positions
   TlMeta tl;                // for the mesh's block pattern
   ~ando_ctx() {
@@ -840,8 +844,8 @@ int32_t candidate_scan(ando_ctx* c, int n, const int32_t* in, int32_t* out, DBuf
   unsigned long long t = 0;
   read_small(c, &t, (const unsigned long long*)c->cand64.p, 1);
   if (t > kMaxCandidates)
-    throw Fail(ANDO_EDEVICE, "contact candidates of one detection exceed 2^27 (" + std::to_string(t) +
-                                 "): a trial motion spans too much of the mesh");
+    throw CandidateOverflow(ANDO_EDEVICE, "contact candidates of one detection exceed 2^27 (" + std::to_string(t) +
+                                              "): a trial motion spans too much of the mesh");
   c->stats.max_candidates = std::max(c->stats.max_candidates, (int32_t)t);
   return (int32_t)t;
 }
@@ -2083,11 +2087,25 @@ float line_search_cp(ando_ctx* c, const ando_params& p) {
   double md2, mh2;
   std::memcpy(&md2, &mb[0], sizeof(double));
   std::memcpy(&mh2, &mb[1], sizeof(double));
-  const int t0 = cp_first_trial(c->mean_edge, md2, mh2);
-  hipLaunchKernelGGL(ando::cp::k_scale, dim3(nblk(n3)), dim3(kBlock), 0, s, n3, std::ldexp(1.25f, -t0),
-                     (const float*)c->d.p, c->ed.p);
-  HIPCHK(hipGetLastError());
-  detect_cp(c, c->cpD, p.contact_gap_max, c->ed.p);
+  int t0 = cp_first_trial(c->mean_edge, md2, mh2);
+  // A trial motion whose swept boxes would give a detection more than
+  // kMaxCandidates candidates (some vertex moving centimetres makes the bound
+  // above large, and a garbage direction moves every vertex that far) starts
+  // the trials one halving later instead; the oracle, which has no such
+  // capacity limit, takes the first trial as it is.
+  for (;;) {
+    hipLaunchKernelGGL(ando::cp::k_scale, dim3(nblk(n3)), dim3(kBlock), 0, s, n3, std::ldexp(1.25f, -t0),
+                       (const float*)c->d.p, c->ed.p);
+    HIPCHK(hipGetLastError());
+    try {
+      detect_cp(c, c->cpD, p.contact_gap_max, c->ed.p);
+      break;
+    } catch (const CandidateOverflow&) {
+      if (t0 >= 19) throw;
+      ++t0;
+      ++c->stats.capped_trials;
+    }
+  }
   // Pairs within their separation that would not move apart hold their cloth
   // vertices: up to four passes (a hold changes its neighbours' motion), then
   // every such pair's vertices.
@@ -2553,6 +2571,11 @@ int ando_set_state(ando_ctx* c, const float* positions, const float* velocities,
       HIPCHK(hipMemcpyAsync(c->xprev.p, c->x.p, 3 * sizeof(float) * (size_t)n, hipMemcpyDeviceToDevice, s));
       c->have_prev = true;
     }
+    // A new state starts the coarse-inverse schedule afresh: the first solve
+    // from it rebuilds the coarse inverse, so the steps that follow depend on
+    // the state alone and not on the solves made before it was set.
+    c->tl.coarse_ready = false;
+    c->tl.solves = 0;
     sync(c);
     if (positions && velocities && masses) c->have_state = true;
   });

@@ -92,7 +92,8 @@ class Stats(C.Structure):
                 ("fallback_iterations", C.c_int32), ("spmv_lanes", C.c_int32),
                 ("spmv_wg_slices", C.c_int32), ("batched_solves", C.c_int32), ("pcg_vec_block", C.c_int32),
                 ("pcg_autotuned", C.c_int32), ("long_hit_sorts", C.c_int32), ("max_candidates", C.c_int32),
-                ("held_iterations", C.c_int32), ("timed_coupling_bytes", C.c_double)]
+                ("held_iterations", C.c_int32), ("capped_trials", C.c_int32),
+                ("timed_coupling_bytes", C.c_double)]
 
 
 _lib = None

@@ -47,6 +47,17 @@ def progress(msg: str) -> None:
     print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
+def leg(name: str, fn, *args, **kw):
+    """Runs one of the legs reported beside the headline; a solver error in it
+    (RuntimeError from the library) is recorded as {"error": ...} instead of
+    ending the run, so the headline line is always printed."""
+    try:
+        return fn(*args, **kw)
+    except RuntimeError as e:
+        progress(f"{name}: error: {e}")
+        return {"error": str(e)}
+
+
 METRIC = "sim steps/s + PCG iters/step, 66k-vert cloth: learned-precond vs plain PCG, 1e-6 tol"
 HBM_PEAK_GBS = 8000.0
 
@@ -344,7 +355,8 @@ def summarize(recs):
             "pcg_failures": sum(t["pcg_failures"] for t in st),
             "line_search_failures": sum(t["line_search_failures"] for t in st),
             "newton_iters_per_step": round(sum(t["newton_iterations"] for t in st) / n, 2),
-            "max_candidates": max((t["max_candidates"] for t in st), default=0)}
+            "max_candidates": max((t["max_candidates"] for t in st), default=0),
+            "capped_trials": sum(t["capped_trials"] for t in st)}
 
 
 def physics_leg(name: str, steps: int, accuracy: bool, two_level: bool = False):
@@ -571,13 +583,24 @@ def main(argv=None):
         progress("event pass: PCG launches bracketed by HIP events")
         restore_state(sim, saved)
         mode(tl, timing=True)
-        ev_recs, _ = run_steps(sim, min(a.event_steps, a.steps), 1)
+        try:
+            ev_recs, _ = run_steps(sim, min(a.event_steps, a.steps), 1)
+            ev_err = None
+        except RuntimeError as e:
+            progress(f"event pass: error: {e}")
+            ev_recs, ev_err = [], str(e)
         mode(tl)
+        # The pass replays the timed substeps from the same state: the same
+        # iteration counts show that the timed kernels are the ones measured.
+        replay = [r[1]["pcg_iterations"] for r in ev_recs] == [r[1]["pcg_iterations"] for r in recs[:len(ev_recs)]]
+        if ev_err:
+            roof = {"error": ev_err}
+            ev_recs = []
         t_launch = [sum(r[1]["timed_ms"][k] for r in ev_recs) for k in range(2)]
         n_launch = [sum(r[1]["timed_launches"][k] for r in ev_recs) for k in range(2)]
         cpl_bytes = sum(r[1]["timed_coupling_bytes"] for r in ev_recs)
         ev_elapsed = sum(r[0] for r in ev_recs)
-        if n_launch[0] and n_launch[1]:
+        if ev_recs and n_launch[0] and n_launch[1]:
             avg_s = t_launch[0] / n_launch[0] * 1e-3
             avg_u = t_launch[1] / n_launch[1] * 1e-3
             # Self-contact coupling blocks multiplied in the SpMV launches.
@@ -607,49 +630,59 @@ def main(argv=None):
                     "update_gbs": round(ub / avg_u / 1e9, 1), "update_avg_launch_us": round(avg_u * 1e6, 2),
                     "update_bytes_per_launch": int(ub),
                     "pcg_kernels_share_of_step": round((t_launch[0] + t_launch[1]) * 1e-3 / ev_elapsed, 3),
-                    "event_pass_ms_per_step": round(1e3 * ev_elapsed / len(ev_recs), 3)}
+                    "event_pass_ms_per_step": round(1e3 * ev_elapsed / len(ev_recs), 3),
+                    "event_pass_replays_timed_iterations": replay}
 
     progress("per-frame consumers")
-    frame = per_frame_leg(sim, sc, with_cpu=not a.no_cpu)
+    frame = leg("per-frame consumers", per_frame_leg, sim, sc, with_cpu=not a.no_cpu)
 
     # The same window from the same start state with the other preconditioner.
     progress(f"same window with {'block-Jacobi' if tl else 'the two-level preconditioner'}")
     restore_state(sim, saved)
     mode(not tl)
-    o_recs, o_ok = run_steps(sim, a.steps, 1)
+    try:
+        o_recs, o_ok = run_steps(sim, a.steps, 1)
+        o_err = None
+    except RuntimeError as e:
+        progress(f"other preconditioner: error: {e}")
+        o_recs, o_ok, o_err = [], False, str(e)
     o_et, o_ins = sim.count_intersections()
     mode(False)
+    tl_recs = recs if tl else o_recs
     tl_desc = ("two-level: Morton patches of %d rows, Chebyshev degree %s, coarse 3 unknowns per patch, coarse "
-               "inverse rebuilt every %s solves" % ((recs if tl else o_recs)[-1][1].get("pcg_vec_block", 0),
+               "inverse rebuilt every %s solves" % (tl_recs[-1][1].get("pcg_vec_block", 0) if tl_recs else 0,
                                                     os.environ.get("ANDO_TL_CHEB", "5"),
                                                     os.environ.get("ANDO_TL_COARSE_EVERY", "2")))
     timed = {**summarize(recs), "intersections_at_end": [et, ins]}
-    other = {**summarize(o_recs), "finite": o_ok, "intersections_at_end": [o_et, o_ins]}
+    other = ({"error": o_err} if o_err else
+             {**summarize(o_recs), "finite": o_ok, "intersections_at_end": [o_et, o_ins]})
     preconditioners = {
         "window_substeps": [a.warmup, a.warmup + len(recs) - 1],
         "block_jacobi": other if tl else timed,
         "two_level_schwarz": {**(timed if tl else other), "pcg_kernel": tl_desc},
     }
-    preconditioners["two_level_iteration_ratio"] = round(
-        preconditioners["block_jacobi"]["pcg_iters_per_step"] /
-        max(preconditioners["two_level_schwarz"]["pcg_iters_per_step"], 1e-9), 2)
-    preconditioners["two_level_speedup"] = round(
-        preconditioners["two_level_schwarz"]["steps_per_s"] / max(preconditioners["block_jacobi"]["steps_per_s"],
-                                                                   1e-9), 3)
+    if not o_err:
+        preconditioners["two_level_iteration_ratio"] = round(
+            preconditioners["block_jacobi"]["pcg_iters_per_step"] /
+            max(preconditioners["two_level_schwarz"]["pcg_iters_per_step"], 1e-9), 2)
+        preconditioners["two_level_speedup"] = round(
+            preconditioners["two_level_schwarz"]["steps_per_s"] /
+            max(preconditioners["block_jacobi"]["steps_per_s"], 1e-9), 3)
 
     learned = None
     if not a.no_learned:
         window = list(range(a.warmup, a.warmup + min(a.learned_steps, a.steps)))
-        learned = learned_comparison(sim, sc, window, lambda: restore_state(sim, saved), None, a)
+        learned = leg("learned preconditioner", learned_comparison, sim, sc, window,
+                      lambda: restore_state(sim, saved), None, a)
 
     legs = None
     if not a.no_legs:
         acc = not (a.no_leg_accuracy or a.no_cpu)
-        legs = {"sphere129_corrected_physics": physics_leg("sphere129", 200, acc, tl),
-                "stack65x8_corrected_physics": physics_leg("stack65x8", 300, acc, tl),
-                "cyl257_faithful": faithful_leg(a.workload, 60, 20),
-                "batched": batched_leg(a.batched_steps, accuracy=acc),
-                "stack_resting_modes": corrected_contact_leg(a.corrected_steps)}
+        legs = {"sphere129_corrected_physics": leg("sphere129", physics_leg, "sphere129", 200, acc, tl),
+                "stack65x8_corrected_physics": leg("stack65x8", physics_leg, "stack65x8", 300, acc, tl),
+                "cyl257_faithful": leg("faithful", faithful_leg, a.workload, 60, 20),
+                "batched": leg("batched", batched_leg, a.batched_steps, accuracy=acc),
+                "stack_resting_modes": leg("resting modes", corrected_contact_leg, a.corrected_steps)}
 
     cpu = accuracy = None
     if not a.no_cpu:
@@ -671,19 +704,23 @@ def main(argv=None):
                          f"identical to 1 thread), {cs:.1f} s, {int(st32['pcg_iters'])} PCG iterations (the GPU: "
                          f"{gpu_mid['ms_per_step']} ms, {gpu_mid['pcg_iters_per_step']} PCG iterations on that "
                          f"substep)"}
-        restore_state(sim, state)
-        mode(tl)
-        sim.substep()
-        x_gpu = sim.get("x").astype(np.float64)
-        # The same substep with the other preconditioner on the GPU.
-        restore_state(sim, state)
-        mode(not tl)
-        sim.substep()
-        x_other = sim.get("x").astype(np.float64)
-        mode(False)
-        accuracy = accuracy_vs_fp64(sc, state, x_gpu, x32,
-                                    extra={"gpu_block_jacobi" if tl else "gpu_two_level": x_other}, envelope=True)
-        accuracy["at_substep"] = at
+        def gpu_substep_accuracy():
+            restore_state(sim, state)
+            mode(tl)
+            sim.substep()
+            x_gpu = sim.get("x").astype(np.float64)
+            # The same substep with the other preconditioner on the GPU.
+            restore_state(sim, state)
+            mode(not tl)
+            sim.substep()
+            x_other = sim.get("x").astype(np.float64)
+            mode(False)
+            acc = accuracy_vs_fp64(sc, state, x_gpu, x32,
+                                   extra={"gpu_block_jacobi" if tl else "gpu_two_level": x_other}, envelope=True)
+            acc["at_substep"] = at
+            return acc
+
+        accuracy = leg("accuracy", gpu_substep_accuracy)
 
     out = {
         "metric": METRIC, "value": head["steps_per_s"], "unit": "steps/s", "n_gpus": 1, "steps": len(recs),
@@ -706,6 +743,7 @@ def main(argv=None):
         "newton_iters_per_step": head["newton_iters_per_step"],
         "pcg_solves": head["pcg_solves"], "pcg_failures": head["pcg_failures"],
         "line_search_failures": head["line_search_failures"], "max_candidates": head["max_candidates"],
+        "capped_trials": head["capped_trials"],
         "intersections_at_end": {"edge_triangle": et, "vertices_inside_bodies": ins},
         "roofline": roof, "cpu_baseline": cpu, "accuracy_vs_fp64": accuracy, "preconditioners": preconditioners,
         "learned_precond": learned, "legs": legs, "per_frame_consumers": frame,

@@ -165,6 +165,9 @@ typedef struct {
   int32_t max_candidates;           /* largest candidate list of one detection pass (PT, EE or rigid) */
   int32_t held_iterations;          /* corrected physics: Newton iterations that held the vertices of
                                        pairs within their separation that would not move apart */
+  int32_t capped_trials;            /* corrected physics: line searches whose first trial motion was
+                                       halved (once per count) so that its detection holds at most 2^27
+                                       candidates */
   double timed_coupling_bytes;      /* ANDO_MODE_KERNEL_TIMING: algorithmic bytes of the self-contact
                                        coupling workgroups in the timed SpMV launches */
 } ando_stats;
@@ -267,7 +270,13 @@ int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, co
  * iterations; for up to 1024 patches).  In ando_step the inverse is rebuilt
  * on every ANDO_TL_COARSE_EVERY-th solve (default 2) and the solves between
  * keep the last one, which leaves the preconditioner symmetric positive
- * definite and fixed within each solve; ando_pcg always rebuilds it.  Same
+ * definite and fixed within each solve (the count restarts at each
+ * ando_set_state, so a step depends only on the state it starts from);
+ * ando_pcg always rebuilds it.  In
+ * ando_step a Newton solve after the first of an inner Newton loop starts
+ * from (1 - 1.25 alpha) times the last direction (alpha: the last accepted
+ * line-search step), falling back to x0 = 0 when that start's residual is no
+ * smaller than b's (ANDO_TL_WARM=0: always x0 = 0).  Same
  * stopping test (max-norm relative residual, |p.Ap| < 1e-16 breakdown, the
  * iteration cap), far fewer iterations on cloth systems.  ando_stats.pcg_kernel
  * reads 4. */

@@ -88,6 +88,28 @@ def test_two_level_coarse_every_solve_drape(monkeypatch):
     assert g.count_intersections() == (0, 0)
 
 
+def test_two_level_cold_start_drape(monkeypatch):
+    """ANDO_TL_WARM=0 (every Newton solve from x0 = 0, as the reference's
+    PCG) and the warm start: the 65 x 65 cylinder drape stays finite and
+    intersection-free over 30 substeps either way (the iteration counts of
+    the two are printed; which is lower depends on the workload)."""
+    print("two-level drape, cold and warm starts", flush=True)
+    sc = scenes.cyl257(s=65)
+    iters = {}
+    for warm in ("0", "1"):
+        monkeypatch.setenv("ANDO_TL_WARM", warm)
+        g = native.GpuSim(sc)
+        g.set_mode(fp64_near=True, corrected_physics=True, two_level=True)
+        total = 0
+        for _ in range(30):
+            g.substep()
+            total += g.stats()["pcg_iterations"]
+        assert np.all(np.isfinite(g.get("x")))
+        assert g.count_intersections() == (0, 0)
+        iters[warm] = total
+    print(f"  PCG iterations over 30 substeps: cold {iters['0']}, warm {iters['1']}", flush=True)
+
+
 def cloth_system(sc, substeps):
     """The first Newton system after `substeps` corrected-physics substeps."""
     sim = native.GpuSim(sc)
@@ -168,6 +190,49 @@ def test_two_level_substeps_match_oracle():
         assert rel_err(xt, x64) <= 3.0 * ex + 1e-6, (k, rel_err(xt, x64), ex)
         assert float(np.max(np.abs(vt - v64))) <= 3.0 * ev + 1e-4
         g.substep()
+
+
+def test_two_level_replay_from_a_saved_state_is_exact():
+    """A substep depends only on the state it starts from: after 14 substeps
+    of the 65 x 65 cylinder drape (in contact from substep 9), the state is
+    saved and set again, 6 more substeps run, the state is set back and the
+    same 6 run again.  The coarse-inverse schedule (rebuilt every second
+    solve) restarts at ando_set_state, so the replay repeats every PCG
+    iteration count and every position bit for bit, whatever the solve count
+    of the run before."""
+    print("two-level replay from a saved state", flush=True)
+    sc = scenes.cyl257(s=65)
+    g = native.GpuSim(sc)
+    g.set_mode(fp64_near=True, corrected_physics=True, two_level=True)
+    for _ in range(14):
+        g.substep()
+    x0, v0 = g.get("x").copy(), g.get("v").copy()
+    rig0 = [g.rigid_state(b) for b in range(g.nbodies)]
+
+    def restore():
+        g.set_state(x0, v0)
+        for b, rs in enumerate(rig0):
+            st = g.bodies[b]
+            for k in range(3):
+                st.position[k] = float(rs["position"][k])
+                st.linear_velocity[k] = float(rs["linear_velocity"][k])
+                st.angular_velocity[k] = float(rs["angular_velocity"][k])
+            for k in range(9):
+                st.rotation[k] = float(rs["rotation"].ravel()[k])
+
+    runs = []
+    for rep in range(2):
+        restore()
+        iters, solves = [], 0
+        for _ in range(6):
+            g.substep()
+            st = g.stats()
+            iters.append(st["pcg_iterations"])
+            solves += st["pcg_solves"]
+        runs.append((iters, g.get("x").copy()))
+        print(f"  run {rep}: PCG iterations {iters} ({solves} solves)", flush=True)
+    assert runs[0][0] == runs[1][0]
+    assert np.array_equal(runs[0][1], runs[1][1])
 
 
 def test_two_level_drape_stays_intersection_free():
