@@ -367,6 +367,10 @@ void build_tl_meta(TlMeta& T, int n, const std::vector<int32_t>& row_ptr, const 
 // One linear BVH (triangles or edges) and its build scratch.
 struct LbvhTree {
   int P = 0;
+  // The primitives (index array, vertices per primitive) whose sorted order
+  // and hierarchy the tree holds, and the refits made on them since.
+  const int32_t* topo_prims = nullptr;
+  int topo_kind = 0, refits = 0;
   DBuf<float> boxes, table, nodes, cbox, bpart, bounds;
   DBuf<uint32_t> keys, keys2;
   DBuf<int32_t> vals, vals2, counts, offsets, left, right, first, last, scan_sums, scan_total;
@@ -507,6 +511,7 @@ struct ando_ctx {
   int tl_coarse = 1;        // ANDO_TL_COARSE=0: no coarse level (patch polynomials only)
   int tl_coarse_every = 2;  // ANDO_TL_COARSE_EVERY=k: the coarse inverse is rebuilt on every k-th solve
   int tl_warm = 1;          // ANDO_TL_WARM=0: every Newton solve starts from x0 = 0 (else from the scaled last direction)
+  int lbvh_refit = 1;       // ANDO_LBVH_REFIT=0: every detection builds its trees in full
   std::vector<int32_t> morton;  // the mesh's vertices in Morton order of the rest positions
   TlMeta tl;                // for the mesh's block pattern
   ~ando_ctx() {
@@ -894,38 +899,55 @@ void finish_pass(ando_ctx* c, int base, int count, int nq) {
 // boxes, Morton codes, stable radix sort, Karras hierarchy, node boxes.
 // vbox (optional): per-vertex boxes whose unions are the primitive boxes;
 // otherwise the primitives' own boxes at the current positions.
-void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims, const float* vbox = nullptr) {
+// refit: when the tree already holds a hierarchy over the same primitives,
+// keep its sorted order and hierarchy and recompute only the boxes (leaf,
+// sparse table, node and child boxes).  A query finds every primitive whose
+// box overlaps its own whatever the hierarchy, and sorts its hits, so the
+// candidate lists are the same as after a full build; only the traversal
+// work depends on how well the old order fits the new boxes.  At most
+// kMaxRefits refits follow one build.
+constexpr int kMaxRefits = 64;
+void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims, const float* vbox = nullptr,
+                bool refit = false) {
   using namespace ando::lbvh;
   hipStream_t s = c->stream;
+  refit = refit && c->lbvh_refit && P >= 2 && T.P == P && T.topo_prims == prims && T.topo_kind == kind &&
+          T.refits < kMaxRefits;
   T.P = P;
-  if (P == 0) return;
+  if (P == 0) {
+    T.topo_prims = nullptr;
+    return;
+  }
   const int G = nblk(P);
   T.boxes.ensure(6 * (size_t)P);
   if (vbox)
     hipLaunchKernelGGL(k_prim_boxes_union, dim3(G), dim3(256), 0, s, P, kind, prims, vbox, T.boxes.p);
   else
     hipLaunchKernelGGL(k_prim_boxes, dim3(G), dim3(256), 0, s, P, kind, prims, (const float*)c->x.p, T.boxes.p);
-  T.bpart.ensure(6 * (size_t)G);
-  T.bounds.ensure(6);
-  hipLaunchKernelGGL(k_centroid_bounds, dim3(G), dim3(256), 0, s, P, (const float*)T.boxes.p, T.bpart.p);
-  hipLaunchKernelGGL(k_centroid_bounds_final, dim3(1), dim3(256), 0, s, G, (const float*)T.bpart.p, T.bounds.p);
-  T.keys.ensure(P); T.keys2.ensure(P); T.vals.ensure(P); T.vals2.ensure(P);
-  hipLaunchKernelGGL(k_morton, dim3(G), dim3(256), 0, s, P, (const float*)T.boxes.p, (const float*)T.bounds.p,
-                     T.keys.p, T.vals.p);
-  radix_sort(c, P, T.keys, T.vals, T.keys2, T.vals2, T.counts, T.offsets, T.scan_sums, T.scan_total);
+  if (!refit) {
+    T.bpart.ensure(6 * (size_t)G);
+    T.bounds.ensure(6);
+    hipLaunchKernelGGL(k_centroid_bounds, dim3(G), dim3(256), 0, s, P, (const float*)T.boxes.p, T.bpart.p);
+    hipLaunchKernelGGL(k_centroid_bounds_final, dim3(1), dim3(256), 0, s, G, (const float*)T.bpart.p, T.bounds.p);
+    T.keys.ensure(P); T.keys2.ensure(P); T.vals.ensure(P); T.vals2.ensure(P);
+    hipLaunchKernelGGL(k_morton, dim3(G), dim3(256), 0, s, P, (const float*)T.boxes.p, (const float*)T.bounds.p,
+                       T.keys.p, T.vals.p);
+    radix_sort(c, P, T.keys, T.vals, T.keys2, T.vals2, T.counts, T.offsets, T.scan_sums, T.scan_total);
+  }
   int levels = 1;
   while ((1 << levels) <= P) ++levels;
   T.table.ensure(6 * (size_t)P * levels);
   hipLaunchKernelGGL(k_table_level0, dim3(G), dim3(256), 0, s, P, (const float*)T.boxes.p,
                      (const int32_t*)T.vals.p, T.table.p);
-  for (int k = 1; k < levels; ++k)
-    hipLaunchKernelGGL(k_table_level, dim3(G), dim3(256), 0, s, P, 1 << (k - 1),
-                       (const float*)(T.table.p + (size_t)(k - 1) * 6 * P), T.table.p + (size_t)k * 6 * P);
+  for (int k0 = 0; k0 + 1 < levels; k0 += 3)
+    hipLaunchKernelGGL(k_table_levels, dim3(nblk(P - (2 << k0) + 1)), dim3(256), 0, s, P, k0,
+                       std::min(3, levels - 1 - k0), T.table.p);
   if (P >= 2) {
     T.left.ensure(P - 1); T.right.ensure(P - 1); T.first.ensure(P - 1); T.last.ensure(P - 1);
     T.nodes.ensure(6 * (size_t)(P - 1));
-    hipLaunchKernelGGL(k_karras, dim3(nblk(P - 1)), dim3(256), 0, s, P, (const uint32_t*)T.keys.p, T.left.p,
-                       T.right.p, T.first.p, T.last.p);
+    if (!refit)
+      hipLaunchKernelGGL(k_karras, dim3(nblk(P - 1)), dim3(256), 0, s, P, (const uint32_t*)T.keys.p, T.left.p,
+                         T.right.p, T.first.p, T.last.p);
     hipLaunchKernelGGL(k_node_boxes, dim3(nblk(P - 1)), dim3(256), 0, s, P, (const int32_t*)T.first.p,
                        (const int32_t*)T.last.p, (const float*)T.table.p, T.nodes.p);
     T.cbox.ensure(12 * (size_t)(P - 1));
@@ -933,6 +955,13 @@ void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims,
                        (const int32_t*)T.right.p, (const float*)T.nodes.p, (const float*)T.table.p, T.cbox.p);
   }
   HIPCHK(hipGetLastError());
+  if (refit) {
+    ++T.refits;
+  } else {
+    T.topo_prims = prims;
+    T.topo_kind = kind;
+    T.refits = 0;
+  }
 }
 
 // Self contacts exist in corrected mode, and in faithful mode on a mesh of
@@ -1031,8 +1060,10 @@ void detect_cp(ando_ctx* c, const ando::DetectBodies& D, float gm, const float* 
   hipLaunchKernelGGL(ando::cp::k_vertex_boxes, dim3(nblk(n)), dim3(kBlock), 0, s, n, (const float*)c->x.p, ed,
                      c->vbox.p);
   HIPCHK(hipGetLastError());
-  lbvh_build(c, c->tri_tree, m, 3, c->d_tris.p, c->vbox.p);
-  lbvh_build(c, c->edge_tree, E, 2, c->d_edges.p, c->vbox.p);
+  // The swept boxes of a step's line searches refit the trees the step's
+  // first detection (ed null: the points x) built.
+  lbvh_build(c, c->tri_tree, m, 3, c->d_tris.p, c->vbox.p, ed != nullptr);
+  lbvh_build(c, c->edge_tree, E, 2, c->d_edges.p, c->vbox.p, ed != nullptr);
   c->sc_err.ensure(1);
   HIPCHK(hipMemsetAsync(c->sc_err.p, 0, sizeof(uint32_t), s));
   const int q = std::max(n, E);
@@ -2433,6 +2464,7 @@ int ando_create(int32_t device, ando_ctx** out) {
     if (const char* w = std::getenv("ANDO_TL_COARSE")) c->tl_coarse = std::atoi(w) != 0 ? 1 : 0;
     if (const char* w = std::getenv("ANDO_TL_COARSE_EVERY")) c->tl_coarse_every = std::max(1, std::atoi(w));
     if (const char* w = std::getenv("ANDO_TL_WARM")) c->tl_warm = std::atoi(w) != 0 ? 1 : 0;
+    if (const char* w = std::getenv("ANDO_LBVH_REFIT")) c->lbvh_refit = std::atoi(w) != 0 ? 1 : 0;
     if (const char* w = std::getenv("ANDO_PCG_ROWSETS")) c->spmv_rowsets = std::min(4, std::max(1, std::atoi(w)));
     if (const char* w = std::getenv("ANDO_PCG_SPMV")) {
       c->spmv_env = std::strcmp(w, "sell") == 0 ? 2 : 0;

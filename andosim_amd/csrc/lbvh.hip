@@ -269,10 +269,26 @@ __global__ void k_table_level0(int P, const float* __restrict__ boxes, const int
   if (s >= P) return;
   store_box(T0, s, load_box(boxes, order[s]));
 }
-__global__ void k_table_level(int P, int half, const float* __restrict__ prev, float* __restrict__ next) {
+// Levels k0 + 1 .. k0 + L (L <= 3) in one pass, from level k0 (h = 2^k0
+// boxes wide): level k0 + j at s is formed from the 2^j level-k0 boxes
+// s + t h with the pairing of the level-by-level rule (level k at s = level
+// k - 1 at s united with level k - 1 at s + 2^(k-1)), so every entry equals
+// the one that rule gives.  Level k at s exists for s + 2^k <= P.
+__global__ void k_table_levels(int P, int k0, int L, float* __restrict__ table) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s + 2 * half > P) return;
-  store_box(next, s, unite(load_box(prev, s), load_box(prev, s + half)));
+  const int h = 1 << k0;
+  if (s + 2 * h > P) return;
+  const float* base = table + (size_t)k0 * 6 * P;
+  Box cur = unite(load_box(base, s), load_box(base, s + h));
+  store_box(table + (size_t)(k0 + 1) * 6 * P, s, cur);
+  if (L < 2 || s + 4 * h > P) return;
+  cur = unite(cur, unite(load_box(base, s + 2 * h), load_box(base, s + 3 * h)));
+  store_box(table + (size_t)(k0 + 2) * 6 * P, s, cur);
+  if (L < 3 || s + 8 * h > P) return;
+  const Box o1 = unite(load_box(base, s + 4 * h), load_box(base, s + 5 * h));
+  const Box o2 = unite(load_box(base, s + 6 * h), load_box(base, s + 7 * h));
+  cur = unite(cur, unite(o1, o2));
+  store_box(table + (size_t)(k0 + 3) * 6 * P, s, cur);
 }
 // Node box = union over its sorted range [first, last] (two overlapping
 // power-of-two windows of the table).

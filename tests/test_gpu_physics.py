@@ -234,6 +234,36 @@ def test_physics_drape_stays_intersection_free(name, make, steps):
     print(f"{name}: pcg failures {fails}", flush=True)
 
 
+@pytest.mark.parametrize("name,make,steps", [
+    ("cyl65", lambda: scenes.cyl257(s=65), 16),
+    ("stack_resting", scenes.stack_resting, 6),
+])
+def test_physics_refit_equals_full_builds(monkeypatch, name, make, steps):
+    """The line searches' swept-box detections refit the trees of the step's
+    first detection (ANDO_LBVH_REFIT, default on) instead of building them
+    again: every substep bit-identical to full builds at every detection --
+    positions, velocities, candidate and contact counts, PCG iterations --
+    through first contact on the cylinders (cyl65, contact from substep 9)
+    and in the dense self contact of eight stacked sheets."""
+    print(f"refit against full builds, {name}", flush=True)
+    runs = {}
+    for refit in ("1", "0"):
+        monkeypatch.setenv("ANDO_LBVH_REFIT", refit)
+        g = gpu_sim(make())
+        seq = []
+        for _ in range(steps):
+            g.substep()
+            st = g.stats()
+            seq.append((g.get("x").copy(), g.get("v").copy(),
+                        (st["max_candidates"], st["contacts"], st["rigid_contacts"], st["pcg_iterations"],
+                         st["newton_iterations"])))
+        runs[refit] = seq
+    for k, (a, b) in enumerate(zip(runs["1"], runs["0"])):
+        assert a[2] == b[2], (k, a[2], b[2])
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), k
+    print(f"  last substep: {runs['1'][-1][2]}", flush=True)
+
+
 def test_physics_linearize_bit_exact_on_cyl257_in_contact():
     """The headline workload at full size (257 x 257 cloth on two cylinders,
     corrected physics): the state after 15 substeps on the GPU (the cloth lies
