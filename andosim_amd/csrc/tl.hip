@@ -94,10 +94,13 @@ __device__ __forceinline__ float slot_val(const TlArgs& a, const SliceRow& R, in
   return a.vals[9 * (size_t)R.soff + (size_t)(9 * k + e) * 64 + R.rs];
 }
 
+// The preconditioner's own arithmetic (block-Jacobi blocks, patch products)
+// uses fused multiply-adds: it has no counterpart in the reference to match,
+// and the Chebyshev steps are bound by these instructions.
 __device__ __forceinline__ void mul3(const float* D, const float* v, float* o) {
-  o[0] = D[0] * v[0] + (D[1] * v[1] + D[2] * v[2]);
-  o[1] = D[3] * v[0] + (D[4] * v[1] + D[5] * v[2]);
-  o[2] = D[6] * v[0] + (D[7] * v[1] + D[8] * v[2]);
+  o[0] = fmaf(D[0], v[0], fmaf(D[1], v[1], D[2] * v[2]));
+  o[1] = fmaf(D[3], v[0], fmaf(D[4], v[1], D[5] * v[2]));
+  o[2] = fmaf(D[6], v[0], fmaf(D[7], v[1], D[8] * v[2]));
 }
 
 // The patch solves: kPatchLanes lanes per row of the patch (one: thread t
@@ -154,9 +157,9 @@ __device__ __forceinline__ void local_product(const TlArgs& a, int patch, const 
     if (P.lc[m] >= 0) {
       const int lc = P.lc[m];
       const float v0 = lv[3 * lc], v1 = lv[3 * lc + 1], v2 = lv[3 * lc + 2];
-      o[0] = o[0] + (P.Ab[m][0] * v0 + (P.Ab[m][1] * v1 + P.Ab[m][2] * v2));
-      o[1] = o[1] + (P.Ab[m][3] * v0 + (P.Ab[m][4] * v1 + P.Ab[m][5] * v2));
-      o[2] = o[2] + (P.Ab[m][6] * v0 + (P.Ab[m][7] * v1 + P.Ab[m][8] * v2));
+      o[0] = fmaf(P.Ab[m][0], v0, fmaf(P.Ab[m][1], v1, fmaf(P.Ab[m][2], v2, o[0])));
+      o[1] = fmaf(P.Ab[m][3], v0, fmaf(P.Ab[m][4], v1, fmaf(P.Ab[m][5], v2, o[1])));
+      o[2] = fmaf(P.Ab[m][6], v0, fmaf(P.Ab[m][7], v1, fmaf(P.Ab[m][8], v2, o[2])));
     }
   }
   // Rows longer than the register chunk read their further slots again.
@@ -167,9 +170,9 @@ __device__ __forceinline__ void local_product(const TlArgs& a, int patch, const 
       float B[9];
       for (int e = 0; e < 9; ++e) B[e] = slot_val(a, P.R, k, e);
       const float v0 = lv[3 * lc], v1 = lv[3 * lc + 1], v2 = lv[3 * lc + 2];
-      o[0] = o[0] + (B[0] * v0 + (B[1] * v1 + B[2] * v2));
-      o[1] = o[1] + (B[3] * v0 + (B[4] * v1 + B[5] * v2));
-      o[2] = o[2] + (B[6] * v0 + (B[7] * v1 + B[8] * v2));
+      o[0] = fmaf(B[0], v0, fmaf(B[1], v1, fmaf(B[2], v2, o[0])));
+      o[1] = fmaf(B[3], v0, fmaf(B[4], v1, fmaf(B[5], v2, o[1])));
+      o[2] = fmaf(B[6], v0, fmaf(B[7], v1, fmaf(B[8], v2, o[2])));
     }
   }
   if (kPatchLanes == 2)
@@ -205,7 +208,7 @@ __device__ __forceinline__ void patch_chebyshev(const TlArgs& a, int patch, cons
     float dr[3];
     mul3(D, res, dr);
     for (int c = 0; c < 3; ++c) {
-      d[c] = c1 * d[c] + c2 * dr[c];
+      d[c] = fmaf(c1, d[c], c2 * dr[c]);
       y[c] = y[c] + d[c];
     }
     rho = rho_n;
@@ -254,6 +257,37 @@ __device__ __forceinline__ void wg_sum_max(double (&v)[N], float& mx, double (*l
   for (int w = 1; w < nw; ++w) m = fmaxf(m, ldsf[w]);
   mx = m;
   __syncthreads();
+}
+
+// Workgroup sums of N doubles (and a float maximum) that only the first
+// threads need: thread c < N returns sum c, thread N gets the maximum in mx.
+// Each wave sums down to its lane 0, one LDS round, then thread c adds the
+// waves' values in wave order (the order of wg_sum).  kHalf: only lanes 0-31
+// of every wave hold values, the others zeros.
+template <int N, bool kHalf>
+__device__ __forceinline__ double wg_sum_max_first(double (&v)[N], float& mx, double (*lds)[kMaxWaves],
+                                                   float* ldsf) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+  for (int o = kHalf ? 16 : 32; o > 0; o >>= 1) {
+    for (int c = 0; c < N; ++c) v[c] += __shfl_down(v[c], o, 64);
+    mx = fmaxf(mx, __shfl_down(mx, o, 64));
+  }
+  if (lane == 0) {
+    for (int c = 0; c < N; ++c) lds[c][wid] = v[c];
+    ldsf[wid] = mx;
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  double s = 0.0;
+  if (t < N) {
+    s = lds[t][0];
+    for (int w = 1; w < nw; ++w) s += lds[t][w];
+  } else if (t == N) {
+    float m = ldsf[0];
+    for (int w = 1; w < nw; ++w) m = fmaxf(m, ldsf[w]);
+    mx = m;
+  }
+  return s;
 }
 
 // Rows 3 patch .. 3 patch + 2 of Ac^-1 times s (nc values): this thread's
@@ -639,11 +673,13 @@ __global__ void __launch_bounds__(2 * kMaxPatchRows) k_tl_spmv(TlArgs a, int it,
     sums[0] = (double)pn[0] * ap[0] + (double)pn[1] * ap[1] + (double)pn[2] * ap[2];
     for (int c = 0; c < 3; ++c) sums[1 + c] = ap[c];
   }
-  wg_sum(sums, ldd);
-  if (tid == 0) {
-    part_pAp[patch] = sums[0];
-    for (int c = 0; c < 3; ++c) sa[3 * patch + c] = sums[1 + c];
-  }
+  __shared__ float ldsf_end[kMaxWaves];
+  float no_max = 0.0f;
+  const double tot = wg_sum_max_first<4, true>(sums, no_max, ldd, ldsf_end);
+  if (tid == 0)
+    part_pAp[patch] = tot;
+  else if (tid < 4)
+    sa[3 * patch + tid - 1] = tot;
 }
 
 // alpha = r.z / p.Ap, x += alpha p, r -= alpha Ap, z = C_p r + (Ac^-1 P^T r)
@@ -662,13 +698,41 @@ __global__ void __launch_bounds__(kPatchLanes * kMaxPatchRows) k_tl_update(TlArg
                                                              PcgStatus* __restrict__ st, int spec) {
   if (spec && st->done) return;  // queued past the predicted convergence: flag first
   __shared__ float lv[2 * 3 * kMaxPatchRows];
-  __shared__ double ldd[7][kMaxWaves];
+  __shared__ double ldd[2][kMaxWaves];
+  __shared__ double ldf[4][kMaxWaves];
   __shared__ float ldsf[kMaxWaves];
   const int patch = xcd_logical_group((int)blockIdx.x, a.npatch);
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6, nw = (int)(blockDim.x >> 6);
+  // Scalars first: they head the longest chains.
+  const int done = st->done;
+  const double rz_it = rz_hist[it];
   const bool red = tid < kBlock;
   double pre[2 * kUnroll];
   if (red) prefetch_partials(part_pAp, a.npatch, pre, tid, kBlock);
+  // Both coarse products do not depend on alpha: entry j of (rows 3 patch ..
+  // 3 patch + 2 of Ac^-1 sr, the same rows of Ac^-1 sa) is summed by wave
+  // nw - 6 + j (a workgroup has at least 8 waves), every lane requesting its
+  // kCoarseU columns before adding.
+  constexpr int kCoarseU = 12;
+  const int cj = wid - (nw - 6);
+  double cv = 0.0;
+  if (cj >= 0 && a.nc > 0) {
+    const double* sv = cj < 3 ? sr : sa;
+    const float* crow = a.Acinv + (size_t)(3 * patch + (cj < 3 ? cj : cj - 3)) * a.nc;
+    for (int base = 0; base < a.nc; base += 64 * kCoarseU) {
+      float cm[kCoarseU];
+      double cs[kCoarseU];
+#pragma unroll
+      for (int u = 0; u < kCoarseU; ++u) {
+        const int col = base + 64 * u + lane;
+        const bool in = col < a.nc;
+        cm[u] = in ? crow[col] : 0.0f;
+        cs[u] = in ? sv[col] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kCoarseU; ++u) cv += (double)cm[u] * cs[u];
+    }
+  }
   PatchRow P;
   load_patch_row(a, patch, P);
   const int i = P.row;
@@ -682,23 +746,24 @@ __global__ void __launch_bounds__(kPatchLanes * kMaxPatchRows) k_tl_update(TlArg
   }
   for (int e = 0; e < 9; ++e) D[e] = a.Dinv[(size_t)e * a.n + ii];
   const float lmx = a.lmax[patch];
-  // Both coarse products do not depend on alpha; they are reduced with p.Ap.
-  // v = (p.Ap, (Ac^-1 sr) rows, (Ac^-1 sa) rows).
-  double v[7];
-  {
-    double e1[3], e2[3];
-    coarse_rows(a, patch, sr, e1);
-    coarse_rows(a, patch, sa, e2);
-    for (int c = 0; c < 3; ++c) {
-      v[1 + c] = e1[c];
-      v[4 + c] = e2[c];
-    }
+  if (done) return;
+  // One LDS round: the p.Ap partial sums of the first four waves and the six
+  // coarse entries; every thread then reads the ten values.
+  double pp = red ? finish_sum_partials(part_pAp, a.npatch, pre, tid, kBlock) : 0.0;
+  for (int o = 32; o > 0; o >>= 1) {
+    pp += __shfl_down(pp, o, 64);
+    cv += __shfl_down(cv, o, 64);
   }
-  if (st->done) return;
-  v[0] = red ? finish_sum_partials(part_pAp, a.npatch, pre, tid, kBlock) : 0.0;
-  const double rz_it = rz_hist[it];
-  wg_sum(v, ldd);
-  const double pAp = v[0];
+  if (lane == 0) {
+    if (wid < kBlock / 64) ldd[0][wid] = pp;
+    if (cj >= 0) ldd[1][cj] = cv;
+  }
+  __syncthreads();
+  double pAp = ldd[0][0];
+  for (int w = 1; w < kBlock / 64; ++w) pAp += ldd[0][w];
+  double v[7];
+  v[0] = pAp;
+  for (int j = 0; j < 6; ++j) v[1 + j] = ldd[1][j];
   if (fabs(pAp) < 1e-16) {
     if (tid == 0) {
       st->failed = 1;
@@ -717,7 +782,8 @@ __global__ void __launch_bounds__(kPatchLanes * kMaxPatchRows) k_tl_update(TlArg
     }
   float y[3];
   patch_chebyshev(a, patch, P, lmx, D, rn, lv, y);
-  // w = (r.z, patch sum of the new r), with max|r|.
+  // w = (r.z, patch sum of the new r), with max|r|: only thread c < 5 needs
+  // them, to write the partials.
   double w[4] = {0.0, 0.0, 0.0, 0.0};
   float rm = 0.0f;
   if (P.q == 0 && P.live) {
@@ -730,12 +796,13 @@ __global__ void __launch_bounds__(kPatchLanes * kMaxPatchRows) k_tl_update(TlArg
     }
     if (rm != rm) rm = __int_as_float(0x7f800000);
   }
-  wg_sum_max(w, rm, ldd, ldsf);
-  if (tid == 0) {
-    part_rz[patch] = w[0];
+  const double tot = wg_sum_max_first<4, true>(w, rm, ldf, ldsf);
+  if (tid == 0)
+    part_rz[patch] = tot;
+  else if (tid < 4)
+    sr_next[3 * patch + tid - 1] = tot;
+  else if (tid == 4)
     part_max[patch] = rm;
-    for (int c = 0; c < 3; ++c) sr_next[3 * patch + c] = w[1 + c];
-  }
 }
 
 __global__ void k_tl_scatter(int n, const int32_t* __restrict__ perm, const float* __restrict__ xp,
