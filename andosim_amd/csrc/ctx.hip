@@ -223,6 +223,7 @@ struct TlMeta {
   DBuf<double> sa, sr0, sr1;  // per patch: sums of A p and of r (3 each)
   DBuf<double> part_rz, part_pAp;
   DBuf<float> part_max, part_bmax;
+  DBuf<int32_t> cpr, ccolp;   // coupling blocks in the permuted numbering (TlArgs::cpr, ccolp)
 };
 
 // Coarse unknowns stay dense: at most this many patches get a coarse level.
@@ -1768,6 +1769,16 @@ PcgOut run_pcg_tl(ando_ctx* c, TlMeta& T, const float* A, const int32_t* diag, c
   a.cptr = couple ? sv->ptr : nullptr;
   a.ccol = couple ? c->ccol.p : nullptr;
   a.cval = couple ? c->cval.p : nullptr;
+  if (couple) {
+    T.cpr.ensure(2 * (size_t)n);
+    T.ccolp.ensure((size_t)c->ncblk);
+    hipLaunchKernelGGL(k_tl_couple_perm, dim3(nblk(std::max(n, c->ncblk))), dim3(kBlock), 0, s, n, c->ncblk,
+                       (const int32_t*)T.perm.p, (const int32_t*)T.iperm.p, sv->ptr, (const int32_t*)c->ccol.p,
+                       T.cpr.p, T.ccolp.p);
+    HIPCHK(hipGetLastError());
+    a.cpr = T.cpr.p;
+    a.ccolp = T.ccolp.p;
+  }
   // Start: x0 = 0, or a warm start x0 = x0_scale * x0 (r0 = b - A x0, the
   // residual still measured against max|b|), dropped for x0 = 0 when its
   // residual is no smaller than b's.

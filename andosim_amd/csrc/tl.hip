@@ -40,6 +40,9 @@ constexpr int kPowerIters = 10;                  // power iterations for the pat
 constexpr float kLmaxSafety = 1.2f;              // applied to the power estimate
 constexpr int kMaxCheb = 15;
 constexpr int kMaxWaves = 2 * kMaxPatchRows / 64;  // waves of the widest patch workgroup (the SpMV's)
+// Rounds of the SpMV's coupling pass: blockDim / 8 rows per round, and
+// blockDim = spmv_threads(pr) >= 2 pr, so four rounds cover the patch.
+constexpr int kCplRounds = 4;
 
 // Rows per patch for n block rows: at least 256, at most 512, and no more than
 // 256 patches while that allows.
@@ -70,7 +73,26 @@ struct TlArgs {
   const int32_t* cptr;
   const int32_t* ccol;
   const float* cval;
+  // The same blocks for the SpMV's coupling pass in the permuted numbering:
+  // cpr[2 i], cpr[2 i + 1] = first and end block of permuted row i, ccolp =
+  // permuted column per block (built by k_tl_couple_perm for each solve).
+  const int32_t* cpr;
+  const int32_t* ccolp;
 };
+
+// Coupling blocks in the permuted numbering (see TlArgs::cpr, ccolp).
+__global__ void k_tl_couple_perm(int n, int nblk3, const int32_t* __restrict__ perm,
+                                 const int32_t* __restrict__ iperm, const int32_t* __restrict__ cptr,
+                                 const int32_t* __restrict__ ccol, int32_t* __restrict__ cpr,
+                                 int32_t* __restrict__ ccolp) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    const int o = perm[t];
+    cpr[2 * t] = 3 * cptr[o];
+    cpr[2 * t + 1] = 3 * cptr[o + 1];
+  }
+  if (t < nblk3) ccolp[t] = iperm[ccol[t]];
+}
 
 // A row's place in the sliced layout: the slice's first slot position, the
 // row within the slice and the slice width (0 past the last slice).
@@ -529,6 +551,18 @@ __global__ void __launch_bounds__(2 * kMaxPatchRows) k_tl_spmv(TlArgs a, int it,
     prefetch_partials(part_rz, a.npatch, pre_rz, tid, kBlock);
     prefetch_partials(part_max, a.npatch, pre_max, tid, kBlock);
   }
+  // The coupling pass's row ranges, requested with the first loads.
+  int cbr[kCplRounds], cer[kCplRounds];
+  if (a.cptr) {
+    const int nslot = (int)(blockDim.x >> 3);
+#pragma unroll
+    for (int k = 0; k < kCplRounds; ++k) {
+      const int clr = k * nslot + (tid >> 3), crow = patch * a.pr + clr;
+      const bool ok = clr < a.pr && crow < a.n;
+      cbr[k] = ok ? a.cpr[2 * crow] : 0;
+      cer[k] = ok ? a.cpr[2 * crow + 1] : 0;
+    }
+  }
   int jc[kSpmvSlots];
   float Ab[kSpmvSlots][9];
 #pragma unroll
@@ -612,14 +646,9 @@ __global__ void __launch_bounds__(2 * kMaxPatchRows) k_tl_spmv(TlArgs a, int it,
     __shared__ float lcp[3 * kMaxPatchRows];
     const int cl = tid & 7, nslot = (int)(blockDim.x >> 3);
     constexpr int kU = 4;
-    for (int r0 = 0; r0 < a.pr; r0 += nslot) {
-      const int clr = r0 + (tid >> 3), crow = patch * a.pr + clr;
-      int cb = 0, ce = 0;
-      if (clr < a.pr && crow < a.n) {
-        const int o = a.perm[crow];
-        cb = 3 * a.cptr[o];
-        ce = 3 * a.cptr[o + 1];
-      }
+    for (int k = 0; k < kCplRounds; ++k) {
+      const int clr = k * nslot + (tid >> 3);
+      const int cb = cbr[k], ce = cer[k];
       float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
       for (int b0 = cb + cl; b0 < ce; b0 += 8 * kU) {
         int j[kU];
@@ -628,7 +657,7 @@ __global__ void __launch_bounds__(2 * kMaxPatchRows) k_tl_spmv(TlArgs a, int it,
         for (int u = 0; u < kU; ++u) {
           const int b = b0 + 8 * u;
           if (b < ce) {
-            j[u] = a.iperm[a.ccol[b]];
+            j[u] = a.ccolp[b];
 #pragma unroll
             for (int e = 0; e < 9; ++e) B[u][e] = a.cval[9 * (size_t)b + e];
           }
