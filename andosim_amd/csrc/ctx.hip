@@ -372,7 +372,7 @@ struct LbvhTree {
   // and hierarchy the tree holds, and the refits made on them since.
   const int32_t* topo_prims = nullptr;
   int topo_kind = 0, refits = 0;
-  DBuf<float> boxes, table, nodes, cbox, bpart, bounds;
+  DBuf<float> boxes, table, cbox, bpart, bounds;
   DBuf<uint32_t> keys, keys2;
   DBuf<int32_t> vals, vals2, counts, offsets, left, right, first, last, scan_sums, scan_total;
 };
@@ -921,7 +921,14 @@ void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims,
   }
   const int G = nblk(P);
   T.boxes.ensure(6 * (size_t)P);
-  if (vbox)
+  int levels = 1;
+  while ((1 << levels) <= P) ++levels;
+  T.table.ensure(6 * (size_t)P * levels);
+  const bool fused_leaves = refit && vbox;  // boxes and table level 0 in one pass
+  if (fused_leaves)
+    hipLaunchKernelGGL(k_refit_leaves, dim3(G), dim3(256), 0, s, P, kind, prims, (const int32_t*)T.vals.p, vbox,
+                       T.boxes.p, T.table.p);
+  else if (vbox)
     hipLaunchKernelGGL(k_prim_boxes_union, dim3(G), dim3(256), 0, s, P, kind, prims, vbox, T.boxes.p);
   else
     hipLaunchKernelGGL(k_prim_boxes, dim3(G), dim3(256), 0, s, P, kind, prims, (const float*)c->x.p, T.boxes.p);
@@ -935,25 +942,21 @@ void lbvh_build(ando_ctx* c, LbvhTree& T, int P, int kind, const int32_t* prims,
                        T.keys.p, T.vals.p);
     radix_sort(c, P, T.keys, T.vals, T.keys2, T.vals2, T.counts, T.offsets, T.scan_sums, T.scan_total);
   }
-  int levels = 1;
-  while ((1 << levels) <= P) ++levels;
-  T.table.ensure(6 * (size_t)P * levels);
-  hipLaunchKernelGGL(k_table_level0, dim3(G), dim3(256), 0, s, P, (const float*)T.boxes.p,
-                     (const int32_t*)T.vals.p, T.table.p);
+  if (!fused_leaves)
+    hipLaunchKernelGGL(k_table_level0, dim3(G), dim3(256), 0, s, P, (const float*)T.boxes.p,
+                       (const int32_t*)T.vals.p, T.table.p);
   for (int k0 = 0; k0 + 1 < levels; k0 += 3)
     hipLaunchKernelGGL(k_table_levels, dim3(nblk(P - (2 << k0) + 1)), dim3(256), 0, s, P, k0,
                        std::min(3, levels - 1 - k0), T.table.p);
   if (P >= 2) {
     T.left.ensure(P - 1); T.right.ensure(P - 1); T.first.ensure(P - 1); T.last.ensure(P - 1);
-    T.nodes.ensure(6 * (size_t)(P - 1));
     if (!refit)
       hipLaunchKernelGGL(k_karras, dim3(nblk(P - 1)), dim3(256), 0, s, P, (const uint32_t*)T.keys.p, T.left.p,
                          T.right.p, T.first.p, T.last.p);
-    hipLaunchKernelGGL(k_node_boxes, dim3(nblk(P - 1)), dim3(256), 0, s, P, (const int32_t*)T.first.p,
-                       (const int32_t*)T.last.p, (const float*)T.table.p, T.nodes.p);
     T.cbox.ensure(12 * (size_t)(P - 1));
     hipLaunchKernelGGL(k_child_boxes, dim3(nblk(P - 1)), dim3(256), 0, s, P, (const int32_t*)T.left.p,
-                       (const int32_t*)T.right.p, (const float*)T.nodes.p, (const float*)T.table.p, T.cbox.p);
+                       (const int32_t*)T.right.p, (const int32_t*)T.first.p, (const int32_t*)T.last.p,
+                       (const float*)T.table.p, T.cbox.p);
   }
   HIPCHK(hipGetLastError());
   if (refit) {

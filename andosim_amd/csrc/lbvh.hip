@@ -1,5 +1,6 @@
 // Corrected-mode self contact: a linear BVH over triangles and over edges,
-// rebuilt every detection, and the point-triangle / edge-edge queries.
+// built at a step's first detection (refitted for the swept boxes of its line
+// searches), and the point-triangle / edge-edge queries.
 //
 // The reference's self broad phase (collision.cpp:63-276) never reports a
 // pair for meshes with two or more primitives (its BVH only stores a
@@ -11,10 +12,11 @@
 //
 // Build: primitive boxes -> centroid bounds -> 30-bit Morton codes -> stable
 // LSD radix sort (8-bit digits, per-workgroup LDS histograms and wave-ballot
-// ranking) -> Karras hierarchy (one thread per internal node) -> node boxes
+// ranking) -> Karras hierarchy (one thread per internal node) -> child boxes
 // from a sparse table of box unions over the sorted primitives (every node
 // covers a contiguous sorted range, so its box is the union of two
-// power-of-two windows; no atomics, deterministic).  Query: one thread per
+// power-of-two windows; no atomics, deterministic).  A refit keeps the sorted
+// order and the hierarchy and recomputes the boxes.  Query: one thread per
 // vertex (or edge), stack traversal, a count pass and a fill pass into an
 // exclusive-scanned list, each query's hits sorted by primitive index so the
 // list order is the brute-force order (vertex, triangle) / (e1, e2).
@@ -290,30 +292,47 @@ __global__ void k_table_levels(int P, int k0, int L, float* __restrict__ table) 
   cur = unite(cur, unite(o1, o2));
   store_box(table + (size_t)(k0 + 3) * 6 * P, s, cur);
 }
-// Node box = union over its sorted range [first, last] (two overlapping
-// power-of-two windows of the table).
-__global__ void k_node_boxes(int P, const int32_t* __restrict__ first, const int32_t* __restrict__ last,
-                             const float* __restrict__ table, float* __restrict__ nodes) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P - 1) return;
-  const int a = first[i], b = last[i];
+// Box of the sorted range [a, b] = the union of two overlapping power-of-two
+// windows of the table.
+__device__ inline Box range_box(const float* __restrict__ table, int P, int a, int b) {
   const int len = b - a + 1;
   const int k = 31 - __clz((uint32_t)len);
   const float* Tk = table + (size_t)k * 6 * P;
-  store_box(nodes, i, unite(load_box(Tk, a), load_box(Tk, b - (1 << k) + 1)));
+  return unite(load_box(Tk, a), load_box(Tk, b - (1 << k) + 1));
 }
 
 // Boxes of both children of every internal node side by side (12 floats per
-// node: left, right; a child that is a leaf takes its primitive's box), so a
-// traversal tests both children of a node with one contiguous read.
+// node: left, right), so a traversal tests both children of a node with one
+// contiguous read: a leaf child takes its primitive's box (table level 0), an
+// internal child the box of its sorted range.
 __global__ void k_child_boxes(int P, const int32_t* __restrict__ left, const int32_t* __restrict__ right,
-                              const float* __restrict__ nodes, const float* __restrict__ leaves,
-                              float* __restrict__ cbox) {
+                              const int32_t* __restrict__ first, const int32_t* __restrict__ last,
+                              const float* __restrict__ table, float* __restrict__ cbox) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P - 1) return;
   const int l = left[i], r = right[i];
-  store_box(cbox, 2 * i, l < 0 ? load_box(leaves, ~l) : load_box(nodes, l));
-  store_box(cbox, 2 * i + 1, r < 0 ? load_box(leaves, ~r) : load_box(nodes, r));
+  store_box(cbox, 2 * i, l < 0 ? load_box(table, ~l) : range_box(table, P, first[l], last[l]));
+  store_box(cbox, 2 * i + 1, r < 0 ? load_box(table, ~r) : range_box(table, P, first[r], last[r]));
+}
+
+// Refit: the primitive boxes from the per-vertex boxes (as k_prim_boxes_union)
+// stored both by primitive and, through the kept sorted order, as table
+// level 0.
+__global__ void k_refit_leaves(int P, int kind, const int32_t* __restrict__ prims, const int32_t* __restrict__ order,
+                               const float* __restrict__ vbox, float* __restrict__ boxes, float* __restrict__ T0) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P) return;
+  const int i = order[s];
+  Box b = load_box(vbox, prims[kind * i]);
+  for (int c = 1; c < kind; ++c) {
+    const Box v = load_box(vbox, prims[kind * i + c]);
+    for (int k = 0; k < 3; ++k) {
+      b.lo[k] = fminf(b.lo[k], v.lo[k]);
+      b.hi[k] = fmaxf(b.hi[k], v.hi[k]);
+    }
+  }
+  store_box(boxes, i, b);
+  store_box(T0, s, b);
 }
 
 // Triangle barycentrics of p (collision.cpp:11-43): clamped, renormalised.
