@@ -507,8 +507,8 @@ struct ando_ctx {
   std::vector<hipEvent_t> events;  // pool: start/stop pairs for timed launches
   // Two-level Schwarz PCG (ANDO_MODE_TWO_LEVEL; tl.hip).
   int two_level = 0;
-  int tl_cheb = 5;          // ANDO_TL_CHEB: Chebyshev degree of the patch solves (odd)
-  float tl_ratio = 30.0f;   // ANDO_TL_RATIO: lmax / lmin of the Chebyshev interval
+  int tl_cheb = 9;          // ANDO_TL_CHEB: Chebyshev degree of the patch solves (odd)
+  float tl_ratio = 5.0f;    // ANDO_TL_RATIO: lmax / lmin of the Chebyshev interval
   int tl_coarse = 1;        // ANDO_TL_COARSE=0: no coarse level (patch polynomials only)
   int tl_coarse_every = 2;  // ANDO_TL_COARSE_EVERY=k: the coarse inverse is rebuilt on every k-th solve
   int tl_warm = 1;          // ANDO_TL_WARM=0: every Newton solve starts from x0 = 0 (else from the scaled last direction)

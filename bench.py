@@ -651,7 +651,7 @@ def main(argv=None):
     tl_recs = recs if tl else o_recs
     tl_desc = ("two-level: Morton patches of %d rows, Chebyshev degree %s, coarse 3 unknowns per patch, coarse "
                "inverse rebuilt every %s solves" % (tl_recs[-1][1].get("pcg_vec_block", 0) if tl_recs else 0,
-                                                    os.environ.get("ANDO_TL_CHEB", "5"),
+                                                    os.environ.get("ANDO_TL_CHEB", "9"),
                                                     os.environ.get("ANDO_TL_COARSE_EVERY", "2")))
     timed = {**summarize(recs), "intersections_at_end": [et, ins]}
     other = ({"error": o_err} if o_err else

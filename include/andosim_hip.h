@@ -265,7 +265,8 @@ int ando_set_walls(ando_ctx* ctx, const float* normals, const float* offsets, co
  * (pcg_solver.cpp:8-146): the block rows in Morton order of the rest positions,
  * cut into patches of 256 to 512 rows; per patch a fixed odd-degree Chebyshev
  * polynomial in D^-1 A_pp (block-Jacobi scaling, the patch's own blocks;
- * degree ANDO_TL_CHEB, default 5), plus a coarse correction with three
+ * degree ANDO_TL_CHEB, default 9, over [lmax / ANDO_TL_RATIO, lmax],
+ * ratio default 5), plus a coarse correction with three
  * unknowns per patch (A_c = P^T A P, inverted in FP64 before the solve's
  * iterations; for up to 1024 patches).  In ando_step the inverse is rebuilt
  * on every ANDO_TL_COARSE_EVERY-th solve (default 2) and the solves between

@@ -62,7 +62,7 @@ def bsr_to_csr(rp, ci, vals, n):
 class TwoLevel:
     """The preconditioner for one system (A: 3n x 3n CSR, float64)."""
 
-    def __init__(self, A, order, degree: int = 5, ratio: float = 30.0, coarse: bool = True):
+    def __init__(self, A, order, degree: int = 9, ratio: float = 5.0, coarse: bool = True):
         n = A.shape[0] // 3
         self.n, self.degree, self.ratio = n, degree, ratio
         self.order = np.asarray(order)
@@ -134,7 +134,7 @@ class TwoLevel:
         return z
 
 
-def solve(A, b, order, tol=1e-6, maxit=1000, degree=5, ratio=30.0, coarse=True, precision=64):
+def solve(A, b, order, tol=1e-6, maxit=1000, degree=9, ratio=5.0, coarse=True, precision=64):
     """Two-level PCG on A x = b (z = M r every iteration, as the GPU forms it).
     precision=32 rounds the CG vectors and the step sizes to float32 after
     every operation, as the GPU stores them (dot products and M stay float64).

@@ -50,7 +50,7 @@ def test_two_level_random_patterns(n, seed):
 
 
 @pytest.mark.parametrize("env,degree,coarse", [({"ANDO_TL_CHEB": "3"}, 3, True), ({"ANDO_TL_CHEB": "7"}, 7, True),
-                                              ({"ANDO_TL_COARSE": "0"}, 5, False)])
+                                              ({"ANDO_TL_COARSE": "0"}, 9, False)])
 def test_two_level_options_match_model(monkeypatch, env, degree, coarse):
     """The solver's options (read when the context is created): another
     Chebyshev degree, or no coarse level; iteration counts next to the CPU
