@@ -314,6 +314,10 @@ void build_tl_meta(TlMeta& T, int n, const std::vector<int32_t>& row_ptr, const 
   T.n = n;
   T.nb = row_ptr[n];
   T.pr = patch_rows(n);
+  // ANDO_TL_PATCH_ROWS: rows per patch (256 to 512; the kernels need at least
+  // eight waves per workgroup and at most kMaxPatchRows rows).
+  if (const char* w = std::getenv("ANDO_TL_PATCH_ROWS"))
+    T.pr = std::min(ando::tl::kMaxPatchRows, std::max(256, std::atoi(w)));
   T.npatch = (n + T.pr - 1) / T.pr;
   T.perm.upload(perm, s);
   T.iperm.upload(iperm, s);
